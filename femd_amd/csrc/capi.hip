@@ -1,0 +1,1431 @@
+// capi.hip -- the C ABI of include/femd_hip.h: handles, the device memory pool, and the
+// orchestration of the kernels in kernels.hip.
+#include "femd_hip.h"
+#include "host.hpp"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace femd_amd;
+
+namespace {
+
+thread_local std::string g_err;
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw Error(FEMD_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <class F> int api(F &&f)
+{
+    try
+    {
+        g_err.clear();
+        f();
+        return FEMD_OK;
+    }
+    catch (const Error &e) { g_err = e.what(); return e.code; }
+    catch (const std::bad_alloc &) { g_err = "out of host memory"; return FEMD_ENOMEM; }
+    catch (const std::exception &e) { g_err = e.what(); return FEMD_EINVAL; }
+}
+
+[[noreturn]] void fail(int code, const std::string &m) { throw Error(code, m); }
+
+// ------------------------------------------------------------------------- device and pool
+struct Device {
+    bool ready = false;
+    int id = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::multimap<size_t, void *> free_blocks;       // the caching pool: size -> blocks
+    size_t cached = 0;
+};
+Device g_dev;
+std::mutex g_init_mu;
+
+void ensure_device()
+{
+    if (g_dev.ready) return;
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_dev.ready) return;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) fail(FEMD_EDEVICE, "no HIP device is visible");
+    HIPC(hipSetDevice(g_dev.id));
+    HIPC(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    g_dev.ready = true;
+}
+
+hipStream_t S(femd_stream s)
+{
+    ensure_device();
+    return s ? static_cast<hipStream_t>(s) : g_dev.stream;
+}
+
+size_t round_size(size_t b) { return b <= 256 ? 256 : (b + 4095) & ~size_t(4095); }
+
+void *pool_alloc(size_t bytes)
+{
+    ensure_device();
+    const size_t sz = round_size(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_dev.mu);
+        auto it = g_dev.free_blocks.find(sz);
+        if (it != g_dev.free_blocks.end())
+        {
+            void *p = it->second;
+            g_dev.free_blocks.erase(it);
+            g_dev.cached -= sz;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess)
+    {
+        // release the cache once and retry
+        {
+            std::lock_guard<std::mutex> lk(g_dev.mu);
+            hipDeviceSynchronize();
+            for (auto &kv : g_dev.free_blocks) hipFree(kv.second);
+            g_dev.free_blocks.clear();
+            g_dev.cached = 0;
+        }
+        e = hipMalloc(&p, sz);
+        if (e != hipSuccess) fail(FEMD_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return p;
+}
+
+void pool_free(void *p, size_t bytes)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    g_dev.free_blocks.emplace(round_size(bytes), p);
+    g_dev.cached += round_size(bytes);
+}
+
+// Device buffer owned by the pool.  Work queued on the library's streams may still read a buffer
+// when it is released, so a released block is only reused by work queued after that point on the
+// same stream; cross-stream users synchronize before releasing.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) : p(b ? pool_alloc(b) : nullptr), bytes(b) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() { if (p) pool_free(p, bytes); p = nullptr; bytes = 0; }
+    void ensure(size_t b) { if (bytes < b) { release(); p = pool_alloc(b); bytes = b; } }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <class T> DevBuf upload(const T *h, size_t n, hipStream_t s)
+{
+    DevBuf d(n * sizeof(T));
+    if (n) HIPC(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+template <class T> DevBuf upload(const std::vector<T> &v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+
+// Pinned host scratch for small read-backs.
+struct Pinned {
+    void *p = nullptr;
+    explicit Pinned(size_t b) { HIPC(hipHostMalloc(&p, b, hipHostMallocDefault)); }
+    ~Pinned() { if (p) hipHostFree(p); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+} // namespace
+
+// ------------------------------------------------------------------------------- handles
+struct DevPattern {
+    int n = 0, m = 0;
+    int64_t nnz = 0;
+    std::vector<int> h_ip, h_ix;
+    DevBuf ip, ix, blocks, diag;
+    int nblocks = 0;
+    bool have_diag = false;
+};
+
+struct femd_mesh_s {
+    MeshData M;
+};
+
+struct Quad {
+    int nq = 0;
+    std::vector<double> xi, eta, w;
+    DevBuf tab, qw, qxi, qeta;
+};
+
+struct femd_space_s {
+    SpaceData V;
+    DevBuf geom;
+    PatternData P;
+    bool have_host_pattern = false, have_pattern = false;
+    std::shared_ptr<DevPattern> pat;
+    DevBuf mptr, mlist, vptr, vlist;
+    std::map<int, std::unique_ptr<Quad>> quads;
+    std::unique_ptr<RefElement> ref;
+};
+
+struct femd_csr_s {
+    std::shared_ptr<DevPattern> pat;
+    DevBuf v, v32;
+    bool v32_valid = false;
+    const femd_space_s *space = nullptr;
+};
+
+struct femd_precond_s {
+    int kind = 0, ordering = 0, n = 0;
+    double omega = 1.0, min_pivot = 0.0;
+    DevBuf inv;                                   // Jacobi
+    DevBuf perm, ip, ix, diag, lu;                // ILU(0) / SSOR on P A P^T
+    DevBuf lptrL, lrowsL, lptrU, lrowsU;
+    int nlevL = 0, nlevU = 0;
+    DevBuf rp, zp;                                // permuted work vectors
+};
+
+namespace {
+
+std::shared_ptr<DevPattern> make_pattern(int n, int m, std::vector<int> ip, std::vector<int> ix, hipStream_t s)
+{
+    auto P = std::make_shared<DevPattern>();
+    P->n = n; P->m = m; P->nnz = ip.empty() ? 0 : ip.back();
+    P->h_ip = std::move(ip); P->h_ix = std::move(ix);
+    P->ip = upload(P->h_ip, s);
+    P->ix = upload(P->h_ix, s);
+    std::vector<int> b = spmv_row_blocks(n, P->h_ip.data(), SPMV_THREADS, SPMV_CAP64);
+    P->nblocks = static_cast<int>(b.size()) - 1;
+    P->blocks = upload(b, s);
+    return P;
+}
+
+const int *diag_of(DevPattern &P, hipStream_t s)
+{
+    if (!P.have_diag)
+    {
+        P.diag = DevBuf(sizeof(int) * std::max(P.n, 1));
+        launch_diag_positions(P.n, P.ip.as<int>(), P.ix.as<int>(), P.diag.as<int>(), s);
+        P.have_diag = true;
+    }
+    return P.diag.as<int>();
+}
+
+Quad &quad_of(femd_space_s *V, int degree, hipStream_t s)
+{
+    auto it = V->quads.find(degree);
+    if (it != V->quads.end()) return *it->second;
+    auto Q = std::make_unique<Quad>();
+    QuadRule R = triangle_rule(degree);
+    Q->nq = R.size();
+    Q->xi = R.xi; Q->eta = R.eta; Q->w = R.w;
+    const int n = V->V.nloc;
+    std::vector<double> tab(static_cast<size_t>(Q->nq) * 3 * n);
+    for (int q = 0; q < Q->nq; ++q) V->ref->tabulate(R.xi[q], R.eta[q], &tab[static_cast<size_t>(q) * 3 * n]);
+    Q->tab = upload(tab, s);
+    Q->qw = upload(R.w, s);
+    Q->qxi = upload(R.xi, s);
+    Q->qeta = upload(R.eta, s);
+    HIPC(hipStreamSynchronize(s));
+    auto &ref = *Q;
+    V->quads[degree] = std::move(Q);
+    return ref;
+}
+
+void ensure_host_pattern(femd_space_s *V)
+{
+    if (!V->have_host_pattern) { V->P = build_pattern(V->V); V->have_host_pattern = true; }
+}
+
+void ensure_geom(femd_space_s *V, hipStream_t s)
+{
+    if (!V->geom.p) { V->geom = upload(V->V.geom, s); HIPC(hipStreamSynchronize(s)); }
+}
+
+void ensure_pattern(femd_space_s *V, hipStream_t s)
+{
+    if (V->have_pattern) return;
+    ensure_host_pattern(V);
+    ensure_geom(V, s);
+    V->pat = make_pattern(V->V.dim, V->V.dim, V->P.indptr, V->P.indices, s);
+    V->mptr = upload(V->P.mptr, s);
+    V->mlist = upload(V->P.mlist, s);
+    V->vptr = upload(V->P.vptr, s);
+    V->vlist = upload(V->P.vlist, s);
+    HIPC(hipStreamSynchronize(s));
+    V->have_pattern = true;
+}
+
+// The coefficient arrays of the terms: device arrays [cell * nq + q], or constants.
+struct Coeffs {
+    std::vector<DevBuf> bufs;
+    TermSet T{};
+};
+
+void check_program(const femd_coefficient &c)
+{
+    if (c.ninstr < 1) fail(FEMD_EINVAL, "coefficient: empty program");
+    if (c.nreg < 1 || c.nreg > MAX_REGS) fail(FEMD_EINVAL, "coefficient: 1.." + std::to_string(MAX_REGS) + " registers supported");
+    for (int k = 0; k < c.ninstr; ++k)
+    {
+        const int op = c.code[4 * k], o = c.code[4 * k + 1], a = c.code[4 * k + 2], b = c.code[4 * k + 3];
+        if (o < 0 || o >= c.nreg) fail(FEMD_EINVAL, "coefficient: register out of range");
+        const bool unary = op >= 9 && op <= 19;
+        if (op == 3 || op == 4 || op == 5 || op < 0 || op > 19)
+            fail(FEMD_EINVAL, "coefficient: instruction " + std::to_string(op) + " is not supported on the GPU (fields and normals are not)");
+        if (op == 0 && (a < 0 || a >= c.nconst)) fail(FEMD_EINVAL, "coefficient: constant out of range");
+        if ((op == 6 || op == 7) && (a < 0 || a >= c.nreg || b < 0 || b >= c.nreg)) fail(FEMD_EINVAL, "coefficient: register out of range");
+        if (op == 8 && (a < 0 || a >= c.nreg || b < 0 || b >= c.nconst)) fail(FEMD_EINVAL, "coefficient: pow operands out of range");
+        if (unary && (a < 0 || a >= c.nreg)) fail(FEMD_EINVAL, "coefficient: register out of range");
+    }
+}
+
+void eval_program(femd_space_s *V, Quad &Q, const femd_coefficient &c, double *out, hipStream_t s)
+{
+    check_program(c);
+    DevBuf code = upload(c.code, 4 * static_cast<size_t>(c.ninstr), s);
+    DevBuf consts = upload(c.consts, static_cast<size_t>(std::max(c.nconst, 0)), s);
+    launch_coefficient(V->V.ncells, Q.nq, V->geom.as<double>(), Q.qxi.as<double>(), Q.qeta.as<double>(), c.ninstr,
+                       code.as<int>(), c.nreg, consts.as<double>(), out, s);
+    HIPC(hipStreamSynchronize(s));           // code and consts are released on return
+}
+
+Coeffs make_coeffs(femd_space_s *V, Quad &Q, int nt, const int *a, const int *b, const femd_coefficient *c, hipStream_t s)
+{
+    if (nt < 1 || nt > MAX_TERMS) fail(FEMD_EINVAL, "assembly: 1.." + std::to_string(MAX_TERMS) + " terms per call");
+    // the element kernels stage the basis table and the weights in 64 KiB of LDS
+    if (sizeof(double) * (size_t(Q.nq) * 3 * V->V.nloc + Q.nq) > 65536)
+        fail(FEMD_EINVAL, "assembly: the quadrature rule of this degree is too large for the element kernels");
+    Coeffs C;
+    C.T.nt = nt;
+    const size_t np = static_cast<size_t>(V->V.ncells) * Q.nq;
+    for (int t = 0; t < nt; ++t)
+    {
+        if (a[t] < 0 || a[t] > 2 || (b && (b[t] < 0 || b[t] > 2)))
+            fail(FEMD_EINVAL, "assembly_2d: derivative code is not in 0..2");
+        C.T.a[t] = a[t];
+        C.T.b[t] = b ? b[t] : 0;
+        C.T.c[t] = nullptr;
+        C.T.cval[t] = 0.0;
+        const femd_coefficient &k = c[t];
+        if (k.values)
+        {
+            C.bufs.push_back(upload(k.values, np, s));
+            C.T.c[t] = C.bufs.back().as<double>();
+        }
+        else if (k.ninstr == 1 && k.code[0] == 0)
+        {
+            if (k.code[2] < 0 || k.code[2] >= k.nconst) fail(FEMD_EINVAL, "coefficient: constant out of range");
+            C.T.cval[t] = k.consts[k.code[2]];
+        }
+        else
+        {
+            C.bufs.emplace_back(np * sizeof(double));
+            eval_program(V, Q, k, C.bufs.back().as<double>(), s);
+            C.T.c[t] = C.bufs.back().as<double>();
+        }
+    }
+    return C;
+}
+
+femd_space_s *check_space(femd_space V)
+{
+    if (!V) fail(FEMD_EINVAL, "null space");
+    return V;
+}
+femd_csr_s *check_csr(femd_csr A)
+{
+    if (!A) fail(FEMD_EINVAL, "null matrix");
+    return A;
+}
+
+const float *values32(femd_csr_s *A, hipStream_t s)
+{
+    if (!A->v32_valid)
+    {
+        A->v32.ensure(sizeof(float) * std::max<int64_t>(A->pat->nnz, 1));
+        launch_to_float(A->pat->nnz, A->v.as<double>(), A->v32.as<float>(), s);
+        A->v32_valid = true;
+    }
+    return A->v32.as<float>();
+}
+
+void spmv64(femd_csr_s *A, const double *x, double *y, const int *stop, hipStream_t s)
+{
+    DevPattern &P = *A->pat;
+    launch_spmv64(P.nblocks, P.blocks.as<int>(), P.ip.as<int>(), P.ix.as<int>(), A->v.as<double>(), x, y, stop, s);
+}
+
+// The ordered dot product into a device scalar.
+void ddot_dev(int64_t n, const double *x, const double *y, double *parts, double *out, hipStream_t s)
+{
+    launch_dot_parts(n, x, y, parts, nullptr, s);
+    launch_sum_parts(nchunks(n), parts, out, s);
+}
+
+double read_scalar(const double *d, hipStream_t s)
+{
+    double h = 0.0;
+    HIPC(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return h;
+}
+
+// z = M^{-1} r on the device (any kind), stop may be null.
+void precond_apply(femd_precond_s *M, const double *r, double *z, const int *stop, hipStream_t s)
+{
+    const int n = M->n;
+    if (M->kind == FEMD_PC_JACOBI) { launch_mul(n, M->inv.as<double>(), r, z, s); return; }
+    const bool perm = M->ordering == FEMD_ORDER_RCM;
+    const double *rin = r;
+    double *zout = z;
+    if (perm)
+    {
+        launch_gather_perm(n, M->perm.as<int>(), r, M->rp.as<double>(), s);
+        rin = M->rp.as<double>();
+        zout = M->zp.as<double>();
+    }
+    if (M->kind == FEMD_PC_ILU0)
+        launch_ilu0_apply(n, M->nlevL, M->lptrL.as<int>(), M->lrowsL.as<int>(), M->nlevU, M->lptrU.as<int>(), M->lrowsU.as<int>(),
+                          M->ip.as<int>(), M->ix.as<int>(), M->diag.as<int>(), M->lu.as<double>(), rin, zout, stop, s);
+    else
+        launch_ssor_apply(n, M->nlevL, M->lptrL.as<int>(), M->lrowsL.as<int>(), M->nlevU, M->lptrU.as<int>(), M->lrowsU.as<int>(),
+                          M->ip.as<int>(), M->ix.as<int>(), M->diag.as<int>(), M->lu.as<double>(), M->omega, rin, zout, stop, s);
+    if (perm) launch_scatter_perm(n, M->perm.as<int>(), M->zp.as<double>(), z, s);
+}
+
+// ------------------------------------------------------------------------------------ CG
+// krylov/cg.hpp:245-350.  The set-up and the final residual are driven from the host (a few
+// read-backs per solve); the iterations run from device-side scalars (CGState), a batch of
+// iterations enqueued between read-backs, every kernel a no-op once `stop` is set.
+void cg_device(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int max_iter, double tol,
+               femd_solve_info *info, hipStream_t s)
+{
+    const int n = A->pat->n;
+    if (A->pat->m != n) fail(FEMD_EINVAL, "cg: the matrix must be square");
+    if (M && M->n != n) fail(FEMD_EINVAL, "krylov: preconditioner and right-hand side differ in size");
+    std::memset(info, 0, sizeof(*info));
+    const bool jac = M && M->kind == FEMD_PC_JACOBI, general = M && !jac;
+    const int np = nchunks(n);
+    DevBuf r(8 * size_t(n)), p(8 * size_t(n)), Ap(8 * size_t(n)), z(general ? 8 * size_t(n) : 0);
+    DevBuf prr(8 * size_t(std::max(np, 1))), prz(8 * size_t(std::max(np, 1))), scal(64), stb(sizeof(CGState));
+    Pinned hst(sizeof(CGState));
+    double *rd = r.as<double>(), *pd = p.as<double>(), *Apd = Ap.as<double>(), *sc = scal.as<double>();
+    CGState *st = stb.as<CGState>();
+    const double *inv = jac ? M->inv.as<double>() : nullptr;
+
+    ddot_dev(n, b, b, prr.as<double>(), sc, s);
+    const double nb = std::sqrt(read_scalar(sc, s));
+    if (nb == 0.0) { launch_fill(n, 0.0, x, s); HIPC(hipStreamSynchronize(s)); info->converged = 1; return; }
+    spmv64(A, x, Apd, nullptr, s);
+    launch_sub(n, b, Apd, rd, s);
+    ddot_dev(n, rd, rd, prr.as<double>(), sc, s);
+    double rn = std::sqrt(read_scalar(sc, s));
+    if (rn <= tol * nb) { info->converged = 1; info->residual = rn / nb; info->precond_residual = info->residual; return; }
+    double rz;
+    if (general)
+    {
+        precond_apply(M, rd, z.as<double>(), nullptr, s);
+        launch_copy(n, z.as<double>(), pd, s);
+        ddot_dev(n, rd, z.as<double>(), prr.as<double>(), sc, s);
+        rz = read_scalar(sc, s);
+    }
+    else if (jac)
+    {
+        launch_mul(n, inv, rd, pd, s);
+        ddot_dev(n, rd, pd, prr.as<double>(), sc, s);
+        rz = read_scalar(sc, s);
+    }
+    else { launch_copy(n, rd, pd, s); rz = rn * rn; }
+
+    CGState h{};
+    h.nb = nb; h.tolnb = tol * nb; h.rz = rz; h.it = 0; h.max_iter = max_iter;
+    if (max_iter <= 0) h.stop = 1;
+    else if (!(rz > 0.0)) { h.stop = 1; h.breakdown = 1; }
+    HIPC(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    const int *stop = &st->stop;
+    int batch = 8;
+    while (!h.stop)
+    {
+        for (int k = 0; k < batch; ++k)
+        {
+            spmv64(A, pd, Apd, stop, s);
+            launch_dot_parts(n, pd, Apd, prr.as<double>(), stop, s);
+            launch_cg_alpha(prr.as<double>(), np, st, s);
+            launch_cg_update(n, x, rd, pd, Apd, inv, prr.as<double>(), prz.as<double>(), st, s);
+            launch_cg_beta(prr.as<double>(), prz.as<double>(), np, general ? 2 : (jac ? 1 : 0), st, s);
+            if (general)
+            {
+                precond_apply(M, rd, z.as<double>(), stop, s);
+                launch_dot_parts(n, rd, z.as<double>(), prr.as<double>(), stop, s);
+                launch_cg_rz(prr.as<double>(), np, st, s);
+                launch_cg_pupdate(n, pd, z.as<double>(), nullptr, st, s);
+            }
+            else launch_cg_pupdate(n, pd, rd, inv, st, s);
+        }
+        HIPC(hipMemcpyAsync(hst.p, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        std::memcpy(&h, hst.p, sizeof(h));
+        batch = std::min(64, batch * 2);
+    }
+    // the true residual and the verdict (cg.hpp:222-237)
+    spmv64(A, x, Apd, nullptr, s);
+    launch_sub(n, b, Apd, rd, s);
+    ddot_dev(n, rd, rd, prr.as<double>(), sc, s);
+    const double res = std::sqrt(read_scalar(sc, s)) / nb;
+    info->iterations = h.it;
+    info->breakdown = h.breakdown;
+    info->residual = res;
+    info->precond_residual = res;
+    info->converged = !h.breakdown && res <= tol * (1.0 + 1e-6) + 1e-15;
+    if (!info->converged && !h.breakdown && res <= 10.0 * tol && h.it < max_iter) info->converged = 1;
+}
+
+// --------------------------------------------------------------------------- GMRES / LGMRES
+// vendor/csnewton/csnewton.hpp:174-260 (gmres) and 298-474 (lgmres) with the adapter of
+// krylov/adapters.hpp:116-147.  Vectors and products on the GPU; the Hessenberg matrix, the
+// Givens rotations and the small triangular solves on the host.
+struct Krylov {
+    femd_csr_s *A;
+    femd_precond_s *M;
+    hipStream_t s;
+    int n;
+    DevBuf parts, scal, tmp;
+    Krylov(femd_csr_s *A_, femd_precond_s *M_, hipStream_t s_) : A(A_), M(M_), s(s_), n(A_->pat->n),
+        parts(8 * 1024 + 64), scal(8 * 512), tmp(8 * size_t(std::max(1, A_->pat->n))) {}
+    void matvec(const double *x, double *y) { spmv64(A, x, y, nullptr, s); }
+    void precon(const double *r, double *z)
+    {
+        if (M) precond_apply(M, r, z, nullptr, s);
+        else launch_copy(n, r, z, s);
+    }
+    void dot_into(const double *x, const double *y, double *out)
+    {
+        launch_fdot(n, x, y, parts.as<double>(), s);
+        launch_fsum(fast_blocks(n), parts.as<double>(), out, s);
+    }
+    double norm(const double *x)
+    {
+        dot_into(x, x, scal.as<double>());
+        return std::sqrt(read_scalar(scal.as<double>(), s));
+    }
+    // MGS of w against V[0..j] into column j of H; returns h_{j+1,j}
+    double arnoldi(double *w, const std::vector<double *> &V, int j, std::vector<double> &H, int ldh)
+    {
+        double *sd = scal.as<double>();
+        for (int i = 0; i <= j; ++i)
+        {
+            dot_into(w, V[i], sd + i);
+            launch_axpy_dev(n, sd + i, -1.0, V[i], w, s);
+        }
+        dot_into(w, w, sd + j + 1);
+        std::vector<double> h(static_cast<size_t>(j) + 2);
+        HIPC(hipMemcpyAsync(h.data(), sd, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (int i = 0; i <= j; ++i) H[i + static_cast<size_t>(j) * ldh] = h[i];
+        const double hn = std::sqrt(h[j + 1]);
+        H[(j + 1) + static_cast<size_t>(j) * ldh] = hn;
+        if (hn > 0.0)
+        {
+            launch_copy(n, w, V[j + 1], s);
+            launch_scale(n, 1.0 / hn, V[j + 1], s);
+        }
+        return hn;
+    }
+};
+
+void givens_gen(double dx, double dy, double &cs, double &sn)
+{
+    const double ax = std::abs(dx), ay = std::abs(dy);
+    if (ay == 0.0) { cs = 1.0; sn = 0.0; }
+    else if (ax == 0.0) { cs = 0.0; sn = 1.0; }
+    else { const double tau = std::sqrt(ax * ax + ay * ay); cs = ax / tau; sn = (dx / ax) * (ay / tau); }
+}
+void givens_apply(double &dx, double &dy, double cs, double sn)
+{
+    const double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
+void rotate_column(std::vector<double> &H, int ldh, int j, std::vector<double> &cs, std::vector<double> &sn, std::vector<double> &g)
+{
+    for (int i = 0; i < j; ++i) givens_apply(H[i + j * ldh], H[(i + 1) + j * ldh], cs[i], sn[i]);
+    givens_gen(H[j + j * ldh], H[(j + 1) + j * ldh], cs[j], sn[j]);
+    givens_apply(H[j + j * ldh], H[(j + 1) + j * ldh], cs[j], sn[j]);
+    givens_apply(g[j], g[j + 1], cs[j], sn[j]);
+}
+std::vector<double> back_solve(const std::vector<double> &H, int ldh, const std::vector<double> &g, int k)
+{
+    std::vector<double> y(g.begin(), g.begin() + k + 1);
+    for (int i = k; i >= 0; --i)
+    {
+        y[i] /= H[i + i * ldh];
+        for (int j = 0; j < i; ++j) y[j] -= H[j + i * ldh] * y[i];
+    }
+    return y;
+}
+
+struct GRes { bool converged; int iterations; double residual; };
+
+GRes gmres_dev(Krylov &K, double *x, const double *b, int restart, int max_iter, double tol)
+{
+    const int n = K.n, ldh = restart + 1;
+    DevBuf Vb(8 * size_t(n) * (restart + 1)), w(8 * size_t(n)), r(8 * size_t(n)), z(8 * size_t(n)), yb(8 * size_t(restart + 1));
+    std::vector<double *> V(restart + 1);
+    for (int i = 0; i <= restart; ++i) V[i] = Vb.as<double>() + size_t(i) * n;
+    std::vector<double> H(size_t(ldh) * restart), cs(restart), sn(restart), g(restart + 1);
+    auto update = [&](int k) {
+        std::vector<double> y = back_solve(H, ldh, g, k);
+        HIPC(hipMemcpyAsync(yb.p, y.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, K.s));
+        launch_lincomb(n, k + 1, yb.as<double>(), V.data(), x, K.s);
+        HIPC(hipStreamSynchronize(K.s));
+    };
+    K.matvec(x, w.as<double>());
+    launch_sub(n, b, w.as<double>(), r.as<double>(), K.s);
+    K.precon(r.as<double>(), z.as<double>());
+    double normb = K.norm(b);
+    if (normb == 0.0) normb = 1.0;
+    double beta = K.norm(z.as<double>());
+    if (beta / normb < tol) return {true, 0, beta / normb};
+    int total = 0;
+    while (total < max_iter)
+    {
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        launch_copy(n, z.as<double>(), V[0], K.s);
+        launch_scale(n, 1.0 / beta, V[0], K.s);
+        int i = 0;
+        for (; i < restart && total < max_iter; ++i, ++total)
+        {
+            K.matvec(V[i], r.as<double>());
+            K.precon(r.as<double>(), w.as<double>());
+            const double hn = K.arnoldi(w.as<double>(), V, i, H, ldh);
+            if (hn == 0.0) { update(i); return {true, total + 1, 0.0}; }
+            rotate_column(H, ldh, i, cs, sn, g);
+            const double est = std::abs(g[i + 1]) / normb;
+            if (est < tol) { update(i); return {true, total + 1, est}; }
+        }
+        update(i - 1);
+        K.matvec(x, w.as<double>());
+        launch_sub(n, b, w.as<double>(), r.as<double>(), K.s);
+        K.precon(r.as<double>(), z.as<double>());
+        beta = K.norm(z.as<double>());
+        if (beta / normb < tol) return {true, total, beta / normb};
+    }
+    return {false, total, beta / normb};
+}
+
+GRes lgmres_dev(Krylov &K, double *x, const double *b, int inner, int k_aug, int max_iter, double tol)
+{
+    const int n = K.n;
+    if (k_aug < 0) fail(FEMD_EINVAL, "lgmres: k_aug must be non-negative");
+    const int mmax = inner + k_aug;
+    DevBuf Vb(8 * size_t(n) * (mmax + 1)), Zb(8 * size_t(n) * std::max(k_aug, 1)), MZb(8 * size_t(n) * std::max(k_aug, 1));
+    DevBuf Ax(8 * size_t(n)), r(8 * size_t(n)), q(8 * size_t(n)), w(8 * size_t(n)), zn(8 * size_t(n)), yb(8 * size_t(mmax + 1));
+    std::vector<double *> V(mmax + 1), Zs, MZs;            // Zs[0] is the newest correction
+    for (int i = 0; i <= mmax; ++i) V[i] = Vb.as<double>() + size_t(i) * n;
+    std::vector<double *> zslot(std::max(k_aug, 1)), mslot(std::max(k_aug, 1));
+    for (int i = 0; i < std::max(k_aug, 1); ++i) { zslot[i] = Zb.as<double>() + size_t(i) * n; mslot[i] = MZb.as<double>() + size_t(i) * n; }
+    double normb = K.norm(b);
+    if (normb == 0.0) normb = 1.0;
+    int total = 0;
+    auto residual = [&]() {
+        K.matvec(x, Ax.as<double>());
+        launch_sub(n, b, Ax.as<double>(), r.as<double>(), K.s);
+        K.precon(r.as<double>(), q.as<double>());
+        return K.norm(q.as<double>());
+    };
+    while (total < max_iter)
+    {
+        const int k = static_cast<int>(Zs.size()), m = inner + k, ldh = m + 1;
+        const double beta = residual();
+        if (beta / normb < tol) return {true, total, beta / normb};
+        std::vector<double> H(size_t(ldh) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
+        launch_copy(n, q.as<double>(), V[0], K.s);
+        launch_scale(n, 1.0 / beta, V[0], K.s);
+        g[0] = beta;
+        int final_j = m - 1, built = 0;
+        bool conv = false;
+        const int inner_lim = std::min(inner, max_iter - total);
+        for (int j = 0; j < inner_lim; ++j, ++total)
+        {
+            K.matvec(V[j], r.as<double>());
+            K.precon(r.as<double>(), w.as<double>());
+            const double hn = K.arnoldi(w.as<double>(), V, j, H, ldh);
+            rotate_column(H, ldh, j, cs, sn, g);
+            built = j + 1;
+            if (std::abs(g[j + 1]) / normb < tol || hn == 0.0) { final_j = j; conv = true; ++total; break; }
+        }
+        if (!conv)
+        {
+            const int aug_lim = std::min(k, max_iter - total);
+            for (int aug = 0; aug < aug_lim; ++aug, ++total)
+            {
+                const int j = inner + aug;
+                launch_copy(n, MZs[aug], w.as<double>(), K.s);
+                const double hn = K.arnoldi(w.as<double>(), V, j, H, ldh);
+                rotate_column(H, ldh, j, cs, sn, g);
+                built = j + 1;
+                if (std::abs(g[j + 1]) / normb < tol || hn == 0.0) { final_j = j; conv = true; ++total; break; }
+            }
+        }
+        if (!conv) final_j = built - 1;
+        std::vector<double> y = back_solve(H, ldh, g, final_j);
+        HIPC(hipMemcpyAsync(yb.p, y.data(), sizeof(double) * (final_j + 1), hipMemcpyHostToDevice, K.s));
+        std::vector<const double *> basis(final_j + 1);
+        for (int j = 0; j <= final_j; ++j) basis[j] = j < inner ? V[j] : Zs[j - inner];
+        launch_fill(n, 0.0, zn.as<double>(), K.s);
+        launch_lincomb(n, final_j + 1, yb.as<double>(), basis.data(), zn.as<double>(), K.s);
+        launch_axpy(n, 1.0, zn.as<double>(), x, K.s);
+        if (k_aug > 0)
+        {
+            // the oldest slot is recycled for the newest correction
+            double *zs = nullptr, *ms = nullptr;
+            if (static_cast<int>(Zs.size()) < k_aug) { zs = zslot[Zs.size()]; ms = mslot[Zs.size()]; }
+            else { zs = Zs.back(); ms = MZs.back(); Zs.pop_back(); MZs.pop_back(); }
+            launch_copy(n, zn.as<double>(), zs, K.s);
+            K.matvec(zs, Ax.as<double>());
+            K.precon(Ax.as<double>(), ms);
+            Zs.insert(Zs.begin(), zs);
+            MZs.insert(MZs.begin(), ms);
+        }
+        HIPC(hipStreamSynchronize(K.s));
+        if (conv) return {true, total, residual() / normb};
+    }
+    return {false, total, residual() / normb};
+}
+
+// ------------------------------------------------------------------------ mixed precision
+void refined_device(femd_csr_s *A, const double *b, double *x, double tol, double inner_tol, int max_outer,
+                    femd_solve_info *info, hipStream_t s)
+{
+    const int n = A->pat->n;
+    std::memset(info, 0, sizeof(*info));
+    DevPattern &P = *A->pat;
+    const float *v32 = values32(A, s);
+    const int *diag = diag_of(P, s);
+    DevBuf bad(sizeof(int));
+    int big = 0x7fffffff;
+    HIPC(hipMemcpyAsync(bad.p, &big, sizeof(int), hipMemcpyHostToDevice, s));
+    launch_inv_diag(n, diag, A->v.as<double>(), nullptr, bad.as<int>(), s);
+    int badrow = 0;
+    HIPC(hipMemcpyAsync(&badrow, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (badrow != 0x7fffffff) fail(FEMD_EZEROPIVOT, "Jacobi: zero diagonal entry in row " + std::to_string(badrow));
+    DevBuf inv32(4 * size_t(n)), r32(4 * size_t(n)), x32(4 * size_t(n)), p32(4 * size_t(n)), Ap32(4 * size_t(n));
+    DevBuf r64(8 * size_t(n)), t64(8 * size_t(n)), parts(8 * 2048 + 64), scal(64), stb(sizeof(CG32State));
+    Pinned hst(sizeof(CG32State));
+    launch_inv32(n, diag, A->v.as<double>(), inv32.as<float>(), s);
+    launch_fdot(n, b, b, parts.as<double>(), s);
+    launch_fsum(fast_blocks(n), parts.as<double>(), scal.as<double>(), s);
+    const double nb = std::sqrt(read_scalar(scal.as<double>(), s));
+    if (nb == 0.0) { launch_fill(n, 0.0, x, s); HIPC(hipStreamSynchronize(s)); info->converged = 1; return; }
+    const int nbk = fast_blocks(n);
+    CG32State *st = stb.as<CG32State>();
+    int inner_total = 0, outer = 0;
+    double res = 0.0;
+    for (;;)
+    {
+        spmv64(A, x, t64.as<double>(), nullptr, s);
+        launch_sub(n, b, t64.as<double>(), r64.as<double>(), s);
+        launch_fdot(n, r64.as<double>(), r64.as<double>(), parts.as<double>(), s);
+        launch_fsum(nbk, parts.as<double>(), scal.as<double>(), s);
+        const double rn = std::sqrt(read_scalar(scal.as<double>(), s));
+        res = rn / nb;
+        if (res <= tol) { info->converged = 1; break; }
+        if (outer >= max_outer) break;
+        ++outer;
+        // FP32 Jacobi-PCG on A d = r / ||r||, to ||r_inner|| <= inner_tol
+        launch_r32_init(n, r64.as<double>(), 1.0 / rn, r32.as<float>(), x32.as<float>(), p32.as<float>(), inv32.as<float>(),
+                        parts.as<double>(), s);
+        launch_cg32_start(parts.as<double>(), nbk, st, s);
+        CG32State h{};
+        HIPC(hipMemcpyAsync(hst.p, st, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        std::memcpy(&h, hst.p, sizeof(h));
+        h.thr = inner_tol * inner_tol;
+        h.max_iter = std::max(10, 10 * n);
+        HIPC(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        const int *stop = &st->stop;
+        int batch = 16;
+        while (!h.stop)
+        {
+            for (int k = 0; k < batch; ++k)
+            {
+                launch_spmv32(P.nblocks, P.blocks.as<int>(), P.ip.as<int>(), P.ix.as<int>(), v32, p32.as<float>(), Ap32.as<float>(), stop, s);
+                launch_fdot32(n, p32.as<float>(), Ap32.as<float>(), parts.as<double>(), stop, s);
+                launch_cg32_alpha(parts.as<double>(), nbk, st, s);
+                launch_cg32_update(n, x32.as<float>(), r32.as<float>(), p32.as<float>(), Ap32.as<float>(), inv32.as<float>(),
+                                   parts.as<double>(), st, s);
+                launch_cg32_beta(parts.as<double>(), nbk, st, s);
+                launch_cg32_pupdate(n, p32.as<float>(), r32.as<float>(), inv32.as<float>(), st, s);
+            }
+            HIPC(hipMemcpyAsync(hst.p, st, sizeof(h), hipMemcpyDeviceToHost, s));
+            HIPC(hipStreamSynchronize(s));
+            std::memcpy(&h, hst.p, sizeof(h));
+            batch = std::min(64, batch * 2);
+        }
+        inner_total += h.it;
+        launch_add_scaled32(n, rn, x32.as<float>(), x, s);
+    }
+    info->iterations = inner_total;
+    info->breakdown = outer;
+    info->residual = res;
+    info->precond_residual = res;
+}
+
+// Build the preconditioner's device data from A.
+void build_precond(femd_precond_s *M, femd_csr_s *A, hipStream_t s)
+{
+    DevPattern &P = *A->pat;
+    const int n = P.n;
+    M->n = n;
+    DevBuf bad(sizeof(int));
+    const int big = 0x7fffffff;
+    auto reset_bad = [&]() { HIPC(hipMemcpyAsync(bad.p, &big, sizeof(int), hipMemcpyHostToDevice, s)); };
+    auto read_bad = [&]() {
+        int h = 0;
+        HIPC(hipMemcpyAsync(&h, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        return h;
+    };
+    if (M->kind == FEMD_PC_JACOBI)
+    {
+        M->inv = DevBuf(8 * size_t(std::max(n, 1)));
+        reset_bad();
+        launch_inv_diag(n, diag_of(P, s), A->v.as<double>(), M->inv.as<double>(), bad.as<int>(), s);
+        const int b = read_bad();
+        if (b != big) fail(FEMD_EZEROPIVOT, "Jacobi: zero diagonal entry in row " + std::to_string(b));
+        return;
+    }
+    std::vector<int> perm;
+    const int *hip_ = P.h_ip.data(), *hix = P.h_ix.data();
+    std::vector<int> pip, pix;
+    DevBuf src;
+    if (M->ordering == FEMD_ORDER_RCM)
+    {
+        perm = rcm_order(n, hip_, hix);
+        Permuted B = permute_pattern(n, hip_, hix, perm);
+        pip = std::move(B.indptr); pix = std::move(B.indices);
+        src = upload(B.src, s);
+        M->perm = upload(perm, s);
+        M->rp = DevBuf(8 * size_t(std::max(n, 1)));
+        M->zp = DevBuf(8 * size_t(std::max(n, 1)));
+    }
+    else { pip = P.h_ip; pix = P.h_ix; }
+    M->ip = upload(pip, s);
+    M->ix = upload(pix, s);
+    M->lu = DevBuf(8 * size_t(std::max<int64_t>(P.nnz, 1)));
+    if (src.p) launch_take(P.nnz, src.as<int>(), A->v.as<double>(), M->lu.as<double>(), s);
+    else if (P.nnz) HIPC(hipMemcpyAsync(M->lu.p, A->v.p, 8 * size_t(P.nnz), hipMemcpyDeviceToDevice, s));
+    M->diag = DevBuf(4 * size_t(std::max(n, 1)));
+    launch_diag_positions(n, M->ip.as<int>(), M->ix.as<int>(), M->diag.as<int>(), s);
+    reset_bad();
+    launch_inv_diag(n, M->diag.as<int>(), M->lu.as<double>(), nullptr, bad.as<int>(), s);
+    const int b0 = read_bad();
+    const char *what = M->kind == FEMD_PC_ILU0 ? "ILU(0)" : "SSOR";
+    if (b0 != big) fail(FEMD_EZEROPIVOT, std::string(what) + ": zero diagonal entry in row " + std::to_string(b0));
+    Levels L = lower_levels(n, pip.data(), pix.data()), U = upper_levels(n, pip.data(), pix.data());
+    M->nlevL = L.count(); M->nlevU = U.count();
+    M->lptrL = upload(L.ptr, s); M->lrowsL = upload(L.rows, s);
+    M->lptrU = upload(U.ptr, s); M->lrowsU = upload(U.rows, s);
+    if (M->kind == FEMD_PC_ILU0)
+    {
+        reset_bad();
+        launch_ilu0_factor(M->nlevL, M->lptrL.as<int>(), M->lrowsL.as<int>(), M->ip.as<int>(), M->ix.as<int>(),
+                           M->diag.as<int>(), M->lu.as<double>(), bad.as<int>(), s);
+        const int b1 = read_bad();
+        if (b1 != big) fail(FEMD_EZEROPIVOT, "ILU(0): zero pivot in row " + std::to_string(b1));
+        // the smallest pivot (a minimum: any order gives the same value)
+        std::vector<double> lu(P.nnz);
+        std::vector<int> dg(n);
+        HIPC(hipMemcpyAsync(lu.data(), M->lu.p, 8 * size_t(P.nnz), hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(dg.data(), M->diag.p, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        double mp = std::numeric_limits<double>::infinity();
+        for (int i = 0; i < n; ++i) mp = std::min(mp, lu[dg[i]]);
+        M->min_pivot = mp;
+    }
+    HIPC(hipStreamSynchronize(s));
+}
+
+thread_local hipEvent_t t_ev0 = nullptr, t_ev1 = nullptr;
+
+} // namespace
+
+// ================================================================================== C ABI
+extern "C" {
+
+const char *femd_last_error(void) { return g_err.c_str(); }
+
+int femd_init(int device)
+{
+    return api([&] {
+        std::lock_guard<std::mutex> lk(g_init_mu);
+        if (g_dev.ready) return;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) fail(FEMD_EDEVICE, "no HIP device is visible");
+        if (device < 0 || device >= n) fail(FEMD_EINVAL, "device index out of range");
+        g_dev.id = device;
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+        g_dev.ready = true;
+    });
+}
+
+int femd_dev_alloc(size_t bytes, void **out)
+{
+    return api([&] {
+        ensure_device();
+        void *p = nullptr;
+        HIPC(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+        *out = p;
+    });
+}
+int femd_dev_free(void *p) { return api([&] { if (p) HIPC(hipFree(p)); }); }
+int femd_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    return api([&] { ensure_device(); HIPC(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int femd_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    return api([&] { ensure_device(); HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int femd_stream_sync(femd_stream s) { return api([&] { HIPC(hipStreamSynchronize(S(s))); }); }
+
+// ---- meshes ------------------------------------------------------------------------------
+int femd_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int diag, const int markers[4], femd_mesh *out)
+{
+    return api([&] {
+        const int dm[4] = {1, 2, 3, 4};
+        auto m = std::make_unique<femd_mesh_s>();
+        m->M = mesh_rectangle(x0, y0, x1, y1, nx, ny, diag, markers ? markers : dm);
+        *out = m.release();
+    });
+}
+
+int femd_mesh_from_arrays(int np, const double *points, int nt, const int *triangles, int ns, const int *segments,
+                          const int *smarks, int marker, femd_mesh *out)
+{
+    return api([&] {
+        if (np < 0 || nt < 0 || ns < 0) fail(FEMD_EINVAL, "mesh_from_arrays: negative count");
+        auto m = std::make_unique<femd_mesh_s>();
+        m->M = mesh_from_arrays(std::vector<double>(points, points + 2 * size_t(np)), std::vector<int>(triangles, triangles + 3 * size_t(nt)),
+                                std::vector<int>(segments, segments + 2 * size_t(ns)),
+                                smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{}, marker);
+        *out = m.release();
+    });
+}
+
+int femd_mesh_sizes(femd_mesh m, int out[3])
+{
+    return api([&] {
+        if (!m) fail(FEMD_EINVAL, "null mesh");
+        out[0] = m->M.np(); out[1] = m->M.nt(); out[2] = static_cast<int>(m->M.smark.size());
+    });
+}
+
+int femd_mesh_arrays(femd_mesh m, double *points, int *triangles, int *neighbours, int *segments, int *smarks)
+{
+    return api([&] {
+        if (!m) fail(FEMD_EINVAL, "null mesh");
+        const MeshData &M = m->M;
+        if (points) std::copy(M.xy.begin(), M.xy.end(), points);
+        if (triangles) std::copy(M.tri.begin(), M.tri.end(), triangles);
+        if (neighbours) std::copy(M.nbr.begin(), M.nbr.end(), neighbours);
+        if (segments) std::copy(M.seg.begin(), M.seg.end(), segments);
+        if (smarks) std::copy(M.smark.begin(), M.smark.end(), smarks);
+    });
+}
+
+void femd_mesh_free(femd_mesh m) { delete m; }
+
+// ---- spaces ------------------------------------------------------------------------------
+int femd_space_create(femd_mesh m, int degree, int ndirichlet, const int *dirichlet, int all_exterior, int equispaced,
+                      femd_space *out)
+{
+    return api([&] {
+        if (!m) fail(FEMD_EINVAL, "null mesh");
+        if (degree < 1) fail(FEMD_EINVAL, "LagrangeSpace2D: degree must be >= 1 (P0 is a DG space)");
+        if (!equispaced && degree >= 3) fail(FEMD_EINVAL, "LagrangeSpace2D: only equispaced nodes are built for degree >= 3");
+        if (degree > 5) fail(FEMD_EINVAL, "LagrangeSpace2D: the GPU kernels cover degrees 1..5");
+        auto V = std::make_unique<femd_space_s>();
+        V->V = build_space(m->M, degree, std::vector<int>(dirichlet, dirichlet + std::max(ndirichlet, 0)), all_exterior != 0);
+        V->ref = std::make_unique<RefElement>(degree);
+        *out = V.release();
+    });
+}
+
+int femd_space_sizes(femd_space V, int out[6])
+{
+    return api([&] {
+        const SpaceData &D = check_space(V)->V;
+        out[0] = D.ncells; out[1] = D.nloc; out[2] = D.raw_dim; out[3] = D.dim; out[4] = D.nedges; out[5] = D.npoints;
+    });
+}
+
+int femd_space_numbering(femd_space V, int *cell_dof, int *cell_raw, int *ad2raw, double *raw_xy)
+{
+    return api([&] {
+        const SpaceData &D = check_space(V)->V;
+        if (cell_dof) std::copy(D.cell_dof.begin(), D.cell_dof.end(), cell_dof);
+        if (cell_raw) std::copy(D.cell_raw.begin(), D.cell_raw.end(), cell_raw);
+        if (ad2raw) std::copy(D.ad2raw.begin(), D.ad2raw.end(), ad2raw);
+        if (raw_xy) std::copy(D.raw_xy.begin(), D.raw_xy.end(), raw_xy);
+    });
+}
+
+void femd_space_free(femd_space V)
+{
+    if (V && g_dev.ready) hipStreamSynchronize(g_dev.stream);
+    delete V;
+}
+
+int femd_pattern(femd_space V, int64_t *nnz, int *indptr, int *indices)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        ensure_host_pattern(Vs);
+        if (nnz) *nnz = Vs->P.nnz();
+        if (indptr) std::copy(Vs->P.indptr.begin(), Vs->P.indptr.end(), indptr);
+        if (indices) std::copy(Vs->P.indices.begin(), Vs->P.indices.end(), indices);
+    });
+}
+
+int femd_quadrature(int degree, int *nq, double *xi, double *eta, double *w)
+{
+    return api([&] {
+        QuadRule R = triangle_rule(degree);
+        *nq = R.size();
+        if (xi) std::copy(R.xi.begin(), R.xi.end(), xi);
+        if (eta) std::copy(R.eta.begin(), R.eta.end(), eta);
+        if (w) std::copy(R.w.begin(), R.w.end(), w);
+    });
+}
+
+// ---- CSR ---------------------------------------------------------------------------------
+int femd_csr_create_space(femd_space V, femd_csr *out)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        hipStream_t s = S(nullptr);
+        ensure_pattern(Vs, s);
+        auto A = std::make_unique<femd_csr_s>();
+        A->pat = Vs->pat;
+        A->space = Vs;
+        A->v = DevBuf(8 * size_t(std::max<int64_t>(A->pat->nnz, 1)));
+        launch_fill(A->pat->nnz, 0.0, A->v.as<double>(), s);
+        HIPC(hipStreamSynchronize(s));
+        *out = A.release();
+    });
+}
+
+int femd_csr_create(int nrows, int ncols, const int *indptr, const int *indices, const double *data, femd_csr *out)
+{
+    return api([&] {
+        if (nrows < 0 || ncols < 0) fail(FEMD_EINVAL, "CSRMatrix: negative shape");
+        if (indptr[0] != 0) fail(FEMD_EINVAL, "CSRMatrix: indptr[0] must be 0");
+        for (int i = 0; i < nrows; ++i)
+        {
+            if (indptr[i + 1] < indptr[i]) fail(FEMD_EINVAL, "CSRMatrix: indptr must not decrease");
+            for (int p = indptr[i]; p < indptr[i + 1]; ++p)
+            {
+                if (indices[p] < 0 || indices[p] >= ncols) fail(FEMD_EINVAL, "CSRMatrix: column index out of range");
+                if (p > indptr[i] && indices[p] <= indices[p - 1]) fail(FEMD_EINVAL, "CSRMatrix: columns must be sorted and unique in each row");
+            }
+        }
+        hipStream_t s = S(nullptr);
+        const int64_t nnz = indptr[nrows];
+        auto A = std::make_unique<femd_csr_s>();
+        A->pat = make_pattern(nrows, ncols, std::vector<int>(indptr, indptr + nrows + 1), std::vector<int>(indices, indices + nnz), s);
+        A->v = DevBuf(8 * size_t(std::max<int64_t>(nnz, 1)));
+        if (nnz) HIPC(hipMemcpyAsync(A->v.p, data, 8 * size_t(nnz), hipMemcpyHostToDevice, s));
+        HIPC(hipStreamSynchronize(s));
+        *out = A.release();
+    });
+}
+
+int femd_csr_info(femd_csr A, int out[2], int64_t *nnz)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        out[0] = a->pat->n; out[1] = a->pat->m;
+        if (nnz) *nnz = a->pat->nnz;
+    });
+}
+
+int femd_csr_download(femd_csr A, int *indptr, int *indices, double *data)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        if (indptr) std::copy(a->pat->h_ip.begin(), a->pat->h_ip.end(), indptr);
+        if (indices) std::copy(a->pat->h_ix.begin(), a->pat->h_ix.end(), indices);
+        if (data && a->pat->nnz)
+        {
+            hipStream_t s = S(nullptr);
+            HIPC(hipMemcpyAsync(data, a->v.p, 8 * size_t(a->pat->nnz), hipMemcpyDeviceToHost, s));
+            HIPC(hipStreamSynchronize(s));
+        }
+    });
+}
+
+int femd_csr_upload_values(femd_csr A, const double *data)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        if (a->pat->nnz) HIPC(hipMemcpyAsync(a->v.p, data, 8 * size_t(a->pat->nnz), hipMemcpyHostToDevice, s));
+        a->v32_valid = false;
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_csr_device_arrays(femd_csr A, const int **indptr, const int **indices, double **data)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        if (indptr) *indptr = a->pat->ip.as<int>();
+        if (indices) *indices = a->pat->ix.as<int>();
+        if (data) *data = a->v.as<double>();
+        a->v32_valid = false;                  // the caller may write the values
+    });
+}
+
+int femd_csr_asymmetry(femd_csr A, double *out)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        if (a->pat->n != a->pat->m) { *out = 1.0; return; }
+        hipStream_t s = S(nullptr);
+        DevBuf r(16);
+        launch_fill(2, 0.0, r.as<double>(), s);
+        launch_asymmetry(a->pat->n, a->pat->ip.as<int>(), a->pat->ix.as<int>(), a->v.as<double>(), r.as<double>(), s);
+        double h[2];
+        HIPC(hipMemcpyAsync(h, r.p, 16, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        *out = h[0] > 0.0 ? h[1] / h[0] : 0.0;
+    });
+}
+
+int femd_csr_diagonal(femd_csr A, double *out)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        const int n = std::min(a->pat->n, a->pat->m);
+        const int *dg = diag_of(*a->pat, s);
+        std::vector<int> d(n);
+        std::vector<double> v(a->pat->nnz);
+        HIPC(hipMemcpyAsync(d.data(), dg, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+        if (a->pat->nnz) HIPC(hipMemcpyAsync(v.data(), a->v.p, 8 * size_t(a->pat->nnz), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) out[i] = d[i] >= 0 ? v[d[i]] : 0.0;
+    });
+}
+
+void femd_csr_free(femd_csr A)
+{
+    if (A && g_dev.ready) hipStreamSynchronize(g_dev.stream);
+    delete A;
+}
+
+// ---- assembly ----------------------------------------------------------------------------
+int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
+                         femd_csr K)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        femd_csr_s *Ks = check_csr(K);
+        hipStream_t s = S(nullptr);
+        ensure_pattern(Vs, s);
+        if (Ks->pat != Vs->pat) fail(FEMD_EPATTERN, "assemble_matrix: the matrix is not on the space's pattern");
+        Quad &Q = quad_of(Vs, quad_degree, s);
+        Coeffs C = make_coeffs(Vs, Q, nterms, a, b, coeffs, s);
+        const int n = Vs->V.nloc, nc = Vs->V.ncells;
+        DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
+        launch_elem_matrix(n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(), C.T, E.as<double>(), s);
+        launch_gather(Ks->pat->nnz, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(), false, s);
+        Ks->v32_valid = false;
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_assemble_vector_dev(femd_space V, int quad_degree, int nterms, const int *a, const femd_coefficient *coeffs,
+                             double *f_dev, femd_stream st)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        hipStream_t s = S(st);
+        ensure_pattern(Vs, s);
+        Quad &Q = quad_of(Vs, quad_degree, s);
+        Coeffs C = make_coeffs(Vs, Q, nterms, a, nullptr, coeffs, s);
+        const int n = Vs->V.nloc, nc = Vs->V.ncells;
+        DevBuf F(8 * std::max<size_t>(size_t(nc) * n, 1));
+        launch_elem_vector(n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(), C.T, F.as<double>(), s);
+        launch_gather(Vs->V.dim, Vs->vptr.as<int>(), Vs->vlist.as<int>(), F.as<double>(), f_dev, true, s);
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_assemble_vector(femd_space V, int quad_degree, int nterms, const int *a, const femd_coefficient *coeffs, double *f)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        const int dim = Vs->V.dim;
+        DevBuf fd(8 * size_t(std::max(dim, 1)));
+        const int rc = femd_assemble_vector_dev(V, quad_degree, nterms, a, coeffs, fd.as<double>(), nullptr);
+        if (rc) throw Error(rc, g_err);
+        hipStream_t s = S(nullptr);
+        if (dim) HIPC(hipMemcpyAsync(f, fd.p, 8 * size_t(dim), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient *c, double *out)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        hipStream_t s = S(nullptr);
+        ensure_geom(Vs, s);
+        Quad &Q = quad_of(Vs, quad_degree, s);
+        const size_t np = size_t(Vs->V.ncells) * Q.nq;
+        DevBuf d(8 * std::max<size_t>(np, 1));
+        eval_program(Vs, Q, *c, d.as<double>(), s);
+        if (np) HIPC(hipMemcpyAsync(out, d.p, 8 * np, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+// ---- SpMV and dot --------------------------------------------------------------------------
+int femd_spmv_dev(femd_csr A, const void *x, void *y, int precision, femd_stream st)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(st);
+        DevPattern &P = *a->pat;
+        if (precision == FEMD_FP32)
+            launch_spmv32(P.nblocks, P.blocks.as<int>(), P.ip.as<int>(), P.ix.as<int>(), values32(a, s),
+                          static_cast<const float *>(x), static_cast<float *>(y), nullptr, s);
+        else spmv64(a, static_cast<const double *>(x), static_cast<double *>(y), nullptr, s);
+    });
+}
+
+int femd_spmv(femd_csr A, const double *x, double *y)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        DevBuf xd = upload(x, size_t(a->pat->m), s), yd(8 * size_t(std::max(a->pat->n, 1)));
+        spmv64(a, xd.as<double>(), yd.as<double>(), nullptr, s);
+        if (a->pat->n) HIPC(hipMemcpyAsync(y, yd.p, 8 * size_t(a->pat->n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_ddot_dev(int64_t n, const double *x, const double *y, double *out, femd_stream st)
+{
+    return api([&] {
+        hipStream_t s = S(st);
+        DevBuf parts(8 * size_t(std::max(nchunks(n), 1))), r(8);
+        ddot_dev(n, x, y, parts.as<double>(), r.as<double>(), s);
+        *out = read_scalar(r.as<double>(), s);
+    });
+}
+
+int femd_ddot(int64_t n, const double *x, const double *y, double *out)
+{
+    return api([&] {
+        hipStream_t s = S(nullptr);
+        DevBuf xd = upload(x, size_t(n), s), yd = upload(y, size_t(n), s);
+        DevBuf parts(8 * size_t(std::max(nchunks(n), 1))), r(8);
+        ddot_dev(n, xd.as<double>(), yd.as<double>(), parts.as<double>(), r.as<double>(), s);
+        *out = read_scalar(r.as<double>(), s);
+    });
+}
+
+// ---- preconditioners -----------------------------------------------------------------------
+int femd_precond_create(femd_csr A, int kind, int ordering, double omega, femd_precond *out, double *min_pivot)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        if (a->pat->n != a->pat->m) fail(FEMD_EINVAL, "a preconditioner needs a square matrix");
+        if (kind < FEMD_PC_JACOBI || kind > FEMD_PC_SSOR) fail(FEMD_EINVAL, "preconditioner kind must be Jacobi, ILU(0) or SSOR");
+        if (ordering != FEMD_ORDER_NATURAL && ordering != FEMD_ORDER_RCM) fail(FEMD_EINVAL, "ordering is natural or rcm");
+        if (kind == FEMD_PC_SSOR && !(omega > 0.0 && omega < 2.0)) fail(FEMD_EINVAL, "SSOR: omega must lie in (0, 2)");
+        auto M = std::make_unique<femd_precond_s>();
+        M->kind = kind;
+        M->ordering = kind == FEMD_PC_JACOBI ? FEMD_ORDER_NATURAL : ordering;
+        M->omega = omega;
+        build_precond(M.get(), a, S(nullptr));
+        if (min_pivot) *min_pivot = M->min_pivot;
+        *out = M.release();
+    });
+}
+
+int femd_precond_apply_dev(femd_precond M, const double *r, double *z, femd_stream st)
+{
+    return api([&] {
+        if (!M) fail(FEMD_EINVAL, "null preconditioner");
+        precond_apply(M, r, z, nullptr, S(st));
+    });
+}
+
+int femd_precond_apply(femd_precond M, const double *r, double *z)
+{
+    return api([&] {
+        if (!M) fail(FEMD_EINVAL, "null preconditioner");
+        hipStream_t s = S(nullptr);
+        DevBuf rd = upload(r, size_t(M->n), s), zd(8 * size_t(std::max(M->n, 1)));
+        precond_apply(M, rd.as<double>(), zd.as<double>(), nullptr, s);
+        if (M->n) HIPC(hipMemcpyAsync(z, zd.p, 8 * size_t(M->n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_precond_info(femd_precond M, int out[3])
+{
+    return api([&] {
+        if (!M) fail(FEMD_EINVAL, "null preconditioner");
+        out[0] = M->kind; out[1] = M->ordering; out[2] = M->n;
+    });
+}
+
+int femd_rcm(int n, const int *indptr, const int *indices, int *perm)
+{
+    return api([&] {
+        std::vector<int> p = rcm_order(n, indptr, indices);
+        std::copy(p.begin(), p.end(), perm);
+    });
+}
+
+void femd_precond_free(femd_precond M)
+{
+    if (M && g_dev.ready) hipStreamSynchronize(g_dev.stream);
+    delete M;
+}
+
+// ---- solvers --------------------------------------------------------------------------------
+int femd_cg_dev(femd_csr A, femd_precond M, const double *b, double *x, int max_iter, double tol, femd_solve_info *info,
+                femd_stream st)
+{
+    return api([&] { cg_device(check_csr(A), M, b, x, max_iter, tol, info, S(st)); });
+}
+
+int femd_cg(femd_csr A, femd_precond M, const double *b, double *x, int max_iter, double tol, femd_solve_info *info)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        const int n = a->pat->n;
+        DevBuf bd = upload(b, size_t(n), s), xd = upload(x, size_t(n), s);
+        cg_device(a, M, bd.as<double>(), xd.as<double>(), max_iter, tol, info, s);
+        if (n) HIPC(hipMemcpyAsync(x, xd.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_krylov(femd_csr A, femd_precond M, const double *b, double *x, int lgmres, int restart, int max_iter, double tol,
+                int k_aug, femd_solve_info *info)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        if (a->pat->n != a->pat->m) fail(FEMD_EINVAL, "krylov: the matrix must be square");
+        if (M && M->n != a->pat->n) fail(FEMD_EINVAL, "krylov: preconditioner and right-hand side differ in size");
+        if (restart <= 0 || max_iter <= 0) fail(FEMD_EINVAL, "gmres: restart and max_iter must be positive");
+        if (restart + (lgmres ? std::max(k_aug, 0) : 0) + 2 > 512) fail(FEMD_EINVAL, "gmres: restart (+ k_aug) must stay below 510");
+        std::memset(info, 0, sizeof(*info));
+        hipStream_t s = S(nullptr);
+        const int n = a->pat->n;
+        DevBuf bd = upload(b, size_t(n), s), xd = upload(x, size_t(n), s), Mb(8 * size_t(std::max(n, 1)));
+        Krylov K(a, M, s);
+        const double nb = K.norm(bd.as<double>());
+        if (nb == 0.0)
+        {
+            std::fill(x, x + n, 0.0);
+            info->converged = 1;
+            return;
+        }
+        K.precon(bd.as<double>(), Mb.as<double>());
+        const double nMb = K.norm(Mb.as<double>());
+        const double scaled = nMb > 0.0 ? tol * nMb / nb : tol;
+        GRes g = lgmres ? lgmres_dev(K, xd.as<double>(), bd.as<double>(), restart, k_aug, max_iter, scaled)
+                        : gmres_dev(K, xd.as<double>(), bd.as<double>(), restart, max_iter, scaled);
+        DevBuf Ax(8 * size_t(n)), r(8 * size_t(n)), Mr(8 * size_t(n));
+        K.matvec(xd.as<double>(), Ax.as<double>());
+        launch_sub(n, bd.as<double>(), Ax.as<double>(), r.as<double>(), s);
+        K.precon(r.as<double>(), Mr.as<double>());
+        info->iterations = g.iterations;
+        info->residual = K.norm(r.as<double>()) / nb;
+        info->precond_residual = nMb > 0.0 ? K.norm(Mr.as<double>()) / nMb : 0.0;
+        info->converged = info->precond_residual <= tol * (1.0 + 1e-8) || (g.converged && info->precond_residual <= 10.0 * tol);
+        if (n) HIPC(hipMemcpyAsync(x, xd.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_solve_refined_dev(femd_csr A, const double *b, double *x, double tol, double inner_tol, int max_outer,
+                           femd_solve_info *info, femd_stream st)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        if (a->pat->n != a->pat->m) fail(FEMD_EINVAL, "solve: the matrix must be square");
+        refined_device(a, b, x, tol, inner_tol, max_outer, info, S(st));
+    });
+}
+
+int femd_solve_refined(femd_csr A, const double *b, double *x, double tol, double inner_tol, int max_outer, femd_solve_info *info)
+{
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        const int n = a->pat->n;
+        DevBuf bd = upload(b, size_t(n), s), xd = upload(x, size_t(n), s);
+        refined_device(a, bd.as<double>(), xd.as<double>(), tol, inner_tol, max_outer, info, s);
+        if (n) HIPC(hipMemcpyAsync(x, xd.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+// ---- timing --------------------------------------------------------------------------------
+int femd_event_timer_start(femd_stream st)
+{
+    return api([&] {
+        hipStream_t s = S(st);
+        if (!t_ev0) { HIPC(hipEventCreate(&t_ev0)); HIPC(hipEventCreate(&t_ev1)); }
+        HIPC(hipEventRecord(t_ev0, s));
+    });
+}
+
+int femd_event_timer_stop(femd_stream st, double *ms)
+{
+    return api([&] {
+        hipStream_t s = S(st);
+        if (!t_ev0) fail(FEMD_EINVAL, "timer: not started");
+        HIPC(hipEventRecord(t_ev1, s));
+        HIPC(hipEventSynchronize(t_ev1));
+        float f = 0.0f;
+        HIPC(hipEventElapsedTime(&f, t_ev0, t_ev1));
+        *ms = f;
+    });
+}
+
+} // extern "C"

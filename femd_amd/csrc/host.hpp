@@ -1,0 +1,99 @@
+// host.hpp -- the host-side data of the MI355X path: meshes, the P_k numbering, the reference
+// element tables, quadrature rules, the CSR pattern with the element-order gather lists the
+// assembly kernels read, the reverse Cuthill-McKee order and the level schedules of the
+// triangular sweeps.  Everything here runs once per mesh, space or matrix (the reference
+// caches the same things: pattern per space pair, scatter map per assembler).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace femd_amd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// A triangulation in the state mesh_from_arrays leaves it (mesh/structured.hpp:99-221):
+// counter-clockwise triangles, neighbour across the edge opposite each vertex, segments with
+// markers (user segments first, then the remaining boundary edges in loop order).
+struct MeshData {
+    std::vector<double> xy;          // np x 2
+    std::vector<int> tri, nbr;       // nt x 3
+    std::vector<int> seg, smark;     // ns x 2, ns
+    std::vector<int64_t> seg_key;    // sorted edge keys of the segments
+    std::vector<int> seg_of_key;     // segment index for seg_key[i]
+    int np() const { return static_cast<int>(xy.size() / 2); }
+    int nt() const { return static_cast<int>(tri.size() / 3); }
+    int marker_of(int a, int b) const;        // the segment's marker, -1 when (a, b) is no segment
+};
+
+MeshData mesh_from_arrays(std::vector<double> xy, std::vector<int> tri, const std::vector<int> &segs,
+                          const std::vector<int> &smarks, int marker);
+MeshData mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int diag, const int markers[4]);
+
+// The nodal P_k basis on the reference triangle (fe/simplex_basis.hpp: Dubiner modes times the
+// inverse Vandermonde matrix, equispaced nodes, local order vertices / edges / interior).
+struct RefElement {
+    int k = 1, n = 3;
+    std::vector<double> nxi, neta, vinv;
+    explicit RefElement(int degree);
+    void tabulate(double xi, double eta, double *out3n) const;     // values, d/dxi, d/deta
+};
+
+struct QuadRule {
+    std::vector<double> xi, eta, w;
+    int size() const { return static_cast<int>(w.size()); }
+};
+QuadRule triangle_rule(int degree);
+
+// A continuous P_k space (fe/space_2d.hpp:390-526, fe/lagrange_space_2d.hpp:32-67).
+struct SpaceData {
+    int k = 1, nloc = 3, ncells = 0, npoints = 0, nedges = 0, raw_dim = 0, dim = 0;
+    std::vector<int> cell_raw, cell_dof, ad2raw;
+    std::vector<double> raw_xy;
+    std::vector<double> geom;        // per cell: x0, y0, J00, J01, J10, J11 (6 doubles)
+};
+SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior);
+
+// The pattern of a space with itself and, for every stored entry, the element-matrix entries
+// that add into it in ascending element order (the order assemble_matrix_2d adds them,
+// forms/assembly_2d.hpp:265-301): entry p receives E[mlist[q]] for q in [mptr[p], mptr[p+1]),
+// E the element matrices laid out [e][i][j].  The same for vectors: dof i receives
+// F[vlist[q]], q in [vptr[i], vptr[i+1]), F laid out [e][i].
+struct PatternData {
+    int n = 0;
+    std::vector<int> indptr, indices;
+    std::vector<int> mptr, mlist;
+    std::vector<int> vptr, vlist;
+    int64_t nnz() const { return indptr.empty() ? 0 : indptr.back(); }
+};
+PatternData build_pattern(const SpaceData &V);
+
+// Row blocks of the CSR-stream SpMV: block b covers rows [rows[b], rows[b+1]) whose entries
+// number at most cap; a single row longer than cap gets a block of its own (flagged long).
+std::vector<int> spmv_row_blocks(int n, const int *indptr, int max_rows, int cap);
+
+// sparse/amd.hpp:166-234 semantics: p[k] is the original row that becomes row k.
+std::vector<int> rcm_order(int n, const int *indptr, const int *indices);
+
+// B = P A P^T on the pattern level (csr.hpp:286-313): the new pattern and, per new entry,
+// the position of its value in A.
+struct Permuted {
+    std::vector<int> indptr, indices, src;
+};
+Permuted permute_pattern(int n, const int *indptr, const int *indices, const std::vector<int> &p);
+
+// Level sets of the lower (row i waits for the columns j < i of its row) or the upper
+// (columns j > i) triangular sweep: rows listed level by level, ascending within a level.
+struct Levels {
+    std::vector<int> ptr, rows;
+    int count() const { return static_cast<int>(ptr.size()) - 1; }
+};
+Levels lower_levels(int n, const int *indptr, const int *indices);
+Levels upper_levels(int n, const int *indptr, const int *indices);
+
+} // namespace femd_amd

@@ -1,0 +1,973 @@
+// kernels.hip -- the gfx950 kernels of the 2D assemble -> SpMV -> Krylov path.
+//
+// Two families:
+//  * exact kernels (FP64), which reproduce the reference's rounding sequence operation by
+//    operation: this file is compiled with -ffp-contract=off, so a*b + c is two roundings, as
+//    in the reference's x86-64 build; element entries are summed in ascending element order
+//    (forms/assembly_2d.hpp:265-301), matrix rows left to right (sparse/csr.hpp:203-215), and
+//    dot products in 4096-long chunks of eight interleaved lanes (util/omp.hpp:135-192);
+//  * fast kernels (FP32 with FP64 accumulators, tree reductions) for the mixed-precision
+//    inner solves and GMRES, whose results are checked against tolerances instead of bits.
+//
+// Wavefronts are 64 lanes; blocks are 256 threads (4 waves) unless noted.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace femd_amd {
+
+namespace {
+constexpr int TPB = 256;
+
+inline int grid_for(int64_t n, int tpb = TPB)
+{
+    int64_t g = (n + tpb - 1) / tpb;
+    return static_cast<int>(g < 1 ? 1 : g);
+}
+
+__device__ inline double lane_combine(const double *s)
+{
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+// The chunk sum of util/omp.hpp:135-160 over prod[0, len) held in LDS: lane k adds the entries
+// k, k+8, ... left to right; the entries after the last full group of eight go into lane 0;
+// then the fixed combination.  Every thread of the block must call it; thread 0 gets the result.
+__device__ inline double chunk_sum(const double *prod, int len, double *lane)
+{
+    const int tid = threadIdx.x, full = len & ~7;
+    if (tid < 8)
+    {
+        double s = 0.0;
+        for (int i = tid; i < full; i += 8) s += prod[i];
+        lane[tid] = s;
+    }
+    __syncthreads();
+    double out = 0.0;
+    if (tid == 0)
+    {
+        double s0 = lane[0];
+        for (int i = full; i < len; ++i) s0 += prod[i];
+        double l[8] = {s0, lane[1], lane[2], lane[3], lane[4], lane[5], lane[6], lane[7]};
+        out = lane_combine(l);
+    }
+    __syncthreads();
+    return out;
+}
+
+// The chunk partial sums added left to right from zero (util/omp.hpp:186-191); one chunk is
+// returned as it is.
+__device__ inline double ordered_total(const double *parts, int np)
+{
+    if (np == 1) return parts[0];
+    double s = 0.0;
+    for (int c = 0; c < np; ++c) s += parts[c];
+    return s;
+}
+} // namespace
+
+// ============================================================================ assembly
+// Physical derivative row of code m at local function l (fe/cache_2d.hpp:211-226): the
+// reference values, or J^{-T} times the reference gradient.
+template <int N>
+__device__ inline double phys(int m, const double *R, int l, double j0, double j1, double j2, double j3)
+{
+    if (m == 0) return R[l];
+    if (m == 1) return j0 * R[N + l] + j2 * R[2 * N + l];
+    return j1 * R[N + l] + j3 * R[2 * N + l];
+}
+
+// One thread per (cell e, row i): the row Ke[i][:] of the element matrix, the terms in order, then
+// the points, each product skipped when it is exactly zero (forms/assembly_2d.hpp:102-126).
+template <int N>
+__global__ __launch_bounds__(TPB) void k_elem_matrix(int nc, int nq, const double *__restrict__ geom,
+                                                     const double *__restrict__ tab, const double *__restrict__ qw,
+                                                     TermSet T, double *__restrict__ E)
+{
+    extern __shared__ double sh[];
+    double *stab = sh, *sqw = sh + nq * 3 * N;
+    for (int k = threadIdx.x; k < nq * 3 * N; k += blockDim.x) stab[k] = tab[k];
+    for (int k = threadIdx.x; k < nq; k += blockDim.x) sqw[k] = qw[k];
+    __syncthreads();
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<int64_t>(nc) * N) return;
+    const int e = static_cast<int>(idx / N), i = static_cast<int>(idx % N);
+    const double *g = geom + 6 * static_cast<int64_t>(e);
+    const double J0 = g[2], J1 = g[3], J2 = g[4], J3 = g[5];
+    const double d = J0 * J3 - J1 * J2;
+    const double i0 = J3 / d, i1 = -J1 / d, i2 = -J2 / d, i3 = J0 / d, ad = fabs(d);
+    double K[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) K[j] = 0.0;
+    for (int t = 0; t < T.nt; ++t)
+    {
+        const int ca = T.a[t], cb = T.b[t];
+        const double *c = T.c[t];
+        const double cv = T.cval[t];
+        for (int q = 0; q < nq; ++q)
+        {
+            const double w = sqw[q] * ad;
+            const double s = w * (c ? c[static_cast<int64_t>(e) * nq + q] : cv);
+            if (s == 0.0) continue;
+            const double *R = stab + q * 3 * N;
+            const double sa = s * phys<N>(ca, R, i, i0, i1, i2, i3);
+            if (sa == 0.0) continue;
+#pragma unroll
+            for (int j = 0; j < N; ++j) K[j] += sa * phys<N>(cb, R, j, i0, i1, i2, i3);
+        }
+    }
+    double *out = E + (static_cast<int64_t>(e) * N + i) * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = K[j];
+}
+
+// One thread per (cell e, local l): fe[l] of assemble_vector_2d (forms/assembly_2d.hpp:184-196).
+template <int N>
+__global__ __launch_bounds__(TPB) void k_elem_vector(int nc, int nq, const double *__restrict__ geom,
+                                                     const double *__restrict__ tab, const double *__restrict__ qw,
+                                                     TermSet T, double *__restrict__ F)
+{
+    extern __shared__ double sh[];
+    double *stab = sh, *sqw = sh + nq * 3 * N;
+    for (int k = threadIdx.x; k < nq * 3 * N; k += blockDim.x) stab[k] = tab[k];
+    for (int k = threadIdx.x; k < nq; k += blockDim.x) sqw[k] = qw[k];
+    __syncthreads();
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<int64_t>(nc) * N) return;
+    const int e = static_cast<int>(idx / N), l = static_cast<int>(idx % N);
+    const double *g = geom + 6 * static_cast<int64_t>(e);
+    const double J0 = g[2], J1 = g[3], J2 = g[4], J3 = g[5];
+    const double d = J0 * J3 - J1 * J2;
+    const double i0 = J3 / d, i1 = -J1 / d, i2 = -J2 / d, i3 = J0 / d, ad = fabs(d);
+    double f = 0.0;
+    for (int t = 0; t < T.nt; ++t)
+    {
+        const double *c = T.c[t];
+        const double cv = T.cval[t];
+        for (int q = 0; q < nq; ++q)
+        {
+            const double w = sqw[q] * ad;
+            const double s = w * (c ? c[static_cast<int64_t>(e) * nq + q] : cv);
+            if (s == 0.0) continue;
+            f += s * phys<N>(T.a[t], stab + q * 3 * N, l, i0, i1, i2, i3);
+        }
+    }
+    F[idx] = f;
+}
+
+// dst[p] = ((0 + src[list[ptr[p]]]) + src[list[ptr[p]+1]]) + ...: the element-order sum.
+__global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int *__restrict__ ptr, const int *__restrict__ list,
+                                                const double *__restrict__ src, double *__restrict__ dst, int plus_zero)
+{
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double acc = 0.0;
+    for (int q = ptr[p]; q < ptr[p + 1]; ++q) acc += src[list[q]];
+    dst[p] = plus_zero ? 0.0 + acc : acc;
+}
+
+// The Integrand register program (forms/integrand.hpp:300-342) at the quadrature points, the
+// point mapped as the cache does: x = x0 + J00 xi + J01 eta (fe/lagrange_space_2d.hpp:78-85).
+__global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, const double *__restrict__ geom,
+                                                     const double *__restrict__ qxi, const double *__restrict__ qeta,
+                                                     int ninstr, const int *__restrict__ code, int nreg,
+                                                     const double *__restrict__ consts, double *__restrict__ out)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<int64_t>(nc) * nq) return;
+    const int e = static_cast<int>(k / nq), q = static_cast<int>(k % nq);
+    const double *g = geom + 6 * static_cast<int64_t>(e);
+    const double x = g[0] + g[2] * qxi[q] + g[3] * qeta[q];
+    const double y = g[1] + g[4] * qxi[q] + g[5] * qeta[q];
+    double reg[MAX_REGS];
+    int res = 0;
+    for (int s = 0; s < ninstr; ++s)
+    {
+        const int op = code[4 * s], o = code[4 * s + 1], a = code[4 * s + 2], b = code[4 * s + 3];
+        double v;
+        switch (op)
+        {
+        case 0: v = consts[a]; break;
+        case 1: v = x; break;
+        case 2: v = y; break;
+        case 6: v = reg[a] + reg[b]; break;
+        case 7: v = reg[a] * reg[b]; break;
+        case 8:
+        {
+            const double n = consts[b], r = reg[a];
+            v = n == 2.0 ? r * r : n == -1.0 ? 1.0 / r : n == 0.5 ? sqrt(r) : pow(r, n);
+            break;
+        }
+        case 9: v = sin(reg[a]); break;
+        case 10: v = cos(reg[a]); break;
+        case 11: v = exp(reg[a]); break;
+        case 12: v = log(reg[a]); break;
+        case 13: v = tanh(reg[a]); break;
+        case 14: v = sqrt(reg[a]); break;
+        case 15: v = sinh(reg[a]); break;
+        case 16: v = cosh(reg[a]); break;
+        case 17: { const double t = exp(-fabs(reg[a])); v = 2.0 * t / (1.0 + t * t); break; }
+        case 18: v = fabs(reg[a]); break;
+        case 19: v = reg[a] > 0.0 ? 1.0 : (reg[a] < 0.0 ? -1.0 : 0.0); break;
+        default: v = 0.0; break;
+        }
+        reg[o] = v;
+        res = o;
+    }
+    out[k] = reg[res];
+}
+
+// ============================================================================ SpMV
+// CSR-stream: block b owns rows [blocks[b], blocks[b+1]).  The block's products v[p] * x[ix[p]]
+// are formed with coalesced loads of values and columns and staged in LDS; then one thread per
+// row adds its products left to right, the order of CSRMatrix::apply.  A row longer than the
+// staging buffer has a block of its own and is summed from global memory by one thread.
+template <typename T, int CAP>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv(const int *__restrict__ blocks, const int *__restrict__ ip,
+                                                       const int *__restrict__ ix, const T *__restrict__ v,
+                                                       const T *__restrict__ x, T *__restrict__ y, const int *stop)
+{
+    if (stop && *stop) return;
+    __shared__ T prod[CAP];
+    const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+    const int p0 = ip[r0], len = ip[r1] - p0;
+    if (len > CAP)
+    {
+        if (threadIdx.x == 0)
+        {
+            T s = 0;
+            for (int p = p0; p < p0 + len; ++p) s += v[p] * x[ix[p]];
+            y[r0] = s;
+        }
+        return;
+    }
+    for (int k = threadIdx.x; k < len; k += blockDim.x) prod[k] = v[p0 + k] * x[ix[p0 + k]];
+    __syncthreads();
+    const int r = r0 + threadIdx.x;
+    if (r < r1)
+    {
+        T s = 0;
+        for (int k = ip[r] - p0, e = ip[r + 1] - p0; k < e; ++k) s += prod[k];
+        y[r] = s;
+    }
+}
+
+// ============================================================================ ordered dot
+__global__ __launch_bounds__(TPB) void k_dot_parts(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                                                   double *__restrict__ parts, const int *stop)
+{
+    if (stop && *stop) return;
+    __shared__ double prod[CHUNK];
+    __shared__ double lane[8];
+    const int64_t lo = static_cast<int64_t>(blockIdx.x) * CHUNK;
+    const int len = static_cast<int>(n - lo < CHUNK ? n - lo : CHUNK);
+    for (int k = threadIdx.x; k < len; k += blockDim.x) prod[k] = x[lo + k] * y[lo + k];
+    __syncthreads();
+    const double s = chunk_sum(prod, len, lane);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+__global__ void k_sum_parts(int np, const double *parts, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = np == 0 ? 0.0 : ordered_total(parts, np);
+}
+
+// ============================================================================ vectors
+__global__ void k_sub(int64_t n, const double *b, const double *a, double *r)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - a[i];
+}
+__global__ void k_mul(int64_t n, const double *a, const double *b, double *c)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] * b[i];
+}
+__global__ void k_copy(int64_t n, const double *a, double *b)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void k_fill(int64_t n, double v, double *a)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_gather_perm(int n, const int *perm, const double *src, double *dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[perm[k]];
+}
+__global__ void k_scatter_perm(int n, const int *perm, const double *src, double *dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[perm[k]] = src[k];
+}
+__global__ void k_take(int64_t n, const int *idx, const double *src, double *dst)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_to_float(int64_t n, const double *a, float *b)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = static_cast<float>(a[i]);
+}
+// Position of the diagonal in each sorted row, -1 when absent.
+__global__ void k_diag_positions(int n, const int *ip, const int *ix, int *diag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = ip[i], hi = ip[i + 1];
+    while (lo < hi) { const int m = (lo + hi) >> 1; if (ix[m] < i) lo = m + 1; else hi = m; }
+    diag[i] = (lo < ip[i + 1] && ix[lo] == i) ? lo : -1;
+}
+// inv_i = 1 / a_ii (precond.hpp:88-92); the first row with no or a zero diagonal is reported.
+__global__ void k_inv_diag(int n, const int *diag, const double *v, double *inv, int *bad_row)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = diag[i];
+    if (d < 0 || v[d] == 0.0) { atomicMin(bad_row, i); if (inv) inv[i] = 0.0; return; }
+    if (inv) inv[i] = 1.0 / v[d];
+}
+// max |a_ij| and max |a_ij - a_ji| (sparse/csr.hpp:631-644); maxima are order independent.
+__global__ __launch_bounds__(TPB) void k_asymmetry(int n, const int *ip, const int *ix, const double *v, double *out2)
+{
+    __shared__ double sb[TPB], sd[TPB];
+    double big = 0.0, dif = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int p = ip[i]; p < ip[i + 1]; ++p)
+        {
+            const int j = ix[p];
+            big = fmax(big, fabs(v[p]));
+            int lo = ip[j], hi = ip[j + 1];
+            while (lo < hi) { const int m = (lo + hi) >> 1; if (ix[m] < i) lo = m + 1; else hi = m; }
+            const double t = (lo < ip[j + 1] && ix[lo] == i) ? v[lo] : 0.0;
+            dif = fmax(dif, fabs(v[p] - t));
+        }
+    sb[threadIdx.x] = big; sd[threadIdx.x] = dif;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) { sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + s]); sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]); }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        // non-negative doubles order like their bit patterns: an integer max is exact
+        atomicMax(reinterpret_cast<unsigned long long *>(out2), __double_as_longlong(sb[0]));
+        atomicMax(reinterpret_cast<unsigned long long *>(out2 + 1), __double_as_longlong(sd[0]));
+    }
+}
+
+// ============================================================================ CG (krylov/cg.hpp)
+// alpha = rz / p^T A p, with the breakdown test of cg.hpp:271-273 / 328-329.
+__global__ void k_cg_alpha(const double *parts, int np, CGState *st)
+{
+    if (threadIdx.x != 0 || st->stop) return;
+    const double pAp = ordered_total(parts, np);
+    st->pAp = pAp;
+    if (!(pAp > 0.0)) { st->breakdown = 1; st->stop = 1; return; }
+    st->alpha = st->rz / pAp;
+}
+
+// x += alpha p, r -= alpha Ap over one chunk, and the chunk's sums of r_i^2 and r_i^2 inv_i
+// (cg.hpp:33-102, 105-171).
+__global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, double *__restrict__ x, double *__restrict__ r,
+                                                   const double *__restrict__ p, const double *__restrict__ Ap,
+                                                   const double *__restrict__ inv, double *__restrict__ parts_rr,
+                                                   double *__restrict__ parts_rz, const CGState *st)
+{
+    if (st->stop) return;
+    __shared__ double prod[CHUNK];
+    __shared__ double lane[8];
+    const double alpha = st->alpha;
+    const int64_t lo = static_cast<int64_t>(blockIdx.x) * CHUNK;
+    const int len = static_cast<int>(n - lo < CHUNK ? n - lo : CHUNK);
+    double e2[CHUNK / TPB];
+#pragma unroll
+    for (int j = 0; j < CHUNK / TPB; ++j)
+    {
+        const int k = j * TPB + threadIdx.x;
+        if (k < len)
+        {
+            const int64_t i = lo + k;
+            const double ri = r[i] - alpha * Ap[i];
+            r[i] = ri;
+            x[i] = x[i] + alpha * p[i];
+            e2[j] = ri * ri;
+            prod[k] = e2[j];
+        }
+    }
+    __syncthreads();
+    const double srr = chunk_sum(prod, len, lane);
+    if (threadIdx.x == 0) parts_rr[blockIdx.x] = srr;
+    if (!inv) return;
+#pragma unroll
+    for (int j = 0; j < CHUNK / TPB; ++j)
+    {
+        const int k = j * TPB + threadIdx.x;
+        if (k < len) prod[k] = e2[j] * inv[lo + k];
+    }
+    __syncthreads();
+    const double srz = chunk_sum(prod, len, lane);
+    if (threadIdx.x == 0) parts_rz[blockIdx.x] = srz;
+}
+
+// After the update: ||r||, the stopping test, beta and the loop-top tests (cg.hpp:330-349).
+// jacobi_fused: r^T D^{-1} r came with the update (cg_jacobi); else rz is the next step's job
+// (general preconditioner: k_cg_rz after z = M^{-1} r), or rz = rr without a preconditioner.
+__global__ void k_cg_beta(const double *parts_rr, const double *parts_rz, int np, int mode, CGState *st)
+{
+    if (threadIdx.x != 0 || st->stop) return;
+    const double rr = ordered_total(parts_rr, np);
+    st->rr = rr;
+    st->rn = sqrt(rr);
+    st->it += 1;
+    if (st->rn <= st->tolnb) { st->stop = 1; st->converged_break = 1; return; }
+    if (mode == 2) return;                                   // general M: beta after z = M^{-1} r
+    const double rzn = mode == 1 ? ordered_total(parts_rz, np) : rr;
+    st->beta = rzn / st->rz;
+    st->rz = rzn;
+    if (st->it >= st->max_iter) st->stop = 1;
+    else if (!(st->rz > 0.0)) { st->breakdown = 1; st->stop = 1; }
+}
+
+// General preconditioner: rz_new = r^T z, beta, and the loop-top tests (cg.hpp:277-283).
+__global__ void k_cg_rz(const double *parts, int np, CGState *st)
+{
+    if (threadIdx.x != 0 || st->stop) return;
+    const double rzn = ordered_total(parts, np);
+    st->beta = rzn / st->rz;
+    st->rz = rzn;
+    if (st->it >= st->max_iter) st->stop = 1;
+    else if (!(st->rz > 0.0)) { st->breakdown = 1; st->stop = 1; }
+}
+
+// p = inv r + beta p (cg.hpp:340-341), or p = r + beta p when inv is null (also p = z + beta p).
+__global__ void k_cg_pupdate(int64_t n, double *__restrict__ p, const double *__restrict__ r,
+                             const double *__restrict__ inv, const CGState *st)
+{
+    if (st->stop) return;
+    const double beta = st->beta;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = inv ? inv[i] * r[i] + beta * p[i] : r[i] + beta * p[i];
+}
+
+// ============================================================================ ILU(0) / SSOR
+// One workgroup walks the level sets of the sweep; rows of a level run in parallel, each row with
+// the reference's own operation order, and a barrier separates the levels.  The whole sweep stays
+// inside one workgroup, so no memory traffic needs cross-CU ordering.
+constexpr int WG_SWEEP = 1024;
+
+// IKJ ILU(0) on the pattern (precond.hpp:158-182), row by row within each level.
+__global__ __launch_bounds__(WG_SWEEP) void k_ilu0_factor(int nlev, const int *__restrict__ lptr,
+                                                          const int *__restrict__ lrows, const int *__restrict__ ip,
+                                                          const int *__restrict__ ix, const int *__restrict__ diag,
+                                                          double *v, int *bad_row)
+{
+    for (int l = 0; l < nlev; ++l)
+    {
+        for (int k = lptr[l] + threadIdx.x; k < lptr[l + 1]; k += blockDim.x)
+        {
+            const int i = lrows[k], re = ip[i + 1], di = diag[i];
+            for (int p = ip[i]; p < di; ++p)
+            {
+                const int kk = ix[p];
+                const double lik = v[p] / v[diag[kk]];
+                v[p] = lik;
+                int t = p + 1;
+                for (int q = diag[kk] + 1; q < ip[kk + 1]; ++q)
+                {
+                    const int col = ix[q];
+                    while (t < re && ix[t] < col) ++t;
+                    if (t == re) break;
+                    if (ix[t] == col) v[t] = v[t] - lik * v[q];
+                }
+            }
+            if (v[di] == 0.0) atomicMin(bad_row, i);
+        }
+        __syncthreads();
+    }
+}
+
+// z = U^{-1} L^{-1} r (precond.hpp:189-205).
+__global__ __launch_bounds__(WG_SWEEP) void k_ilu0_apply(int nlevL, const int *__restrict__ lptrL,
+                                                         const int *__restrict__ lrowsL, int nlevU,
+                                                         const int *__restrict__ lptrU, const int *__restrict__ lrowsU,
+                                                         const int *__restrict__ ip, const int *__restrict__ ix,
+                                                         const int *__restrict__ diag, const double *__restrict__ v,
+                                                         const double *__restrict__ r, double *z, const int *stop)
+{
+    if (stop && *stop) return;
+    for (int l = 0; l < nlevL; ++l)
+    {
+        for (int k = lptrL[l] + threadIdx.x; k < lptrL[l + 1]; k += blockDim.x)
+        {
+            const int i = lrowsL[k];
+            double s = r[i];
+            for (int p = ip[i]; p < diag[i]; ++p) s = s - v[p] * z[ix[p]];
+            z[i] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < nlevU; ++l)
+    {
+        for (int k = lptrU[l] + threadIdx.x; k < lptrU[l + 1]; k += blockDim.x)
+        {
+            const int i = lrowsU[k], di = diag[i];
+            double s = z[i];
+            for (int p = di + 1; p < ip[i + 1]; ++p) s = s - v[p] * z[ix[p]];
+            z[i] = s / v[di];
+        }
+        __syncthreads();
+    }
+}
+
+// SSOR(omega) (precond.hpp:125-144): forward sweep, diagonal scaling, backward sweep.
+__global__ __launch_bounds__(WG_SWEEP) void k_ssor_apply(int n, int nlevL, const int *__restrict__ lptrL,
+                                                         const int *__restrict__ lrowsL, int nlevU,
+                                                         const int *__restrict__ lptrU, const int *__restrict__ lrowsU,
+                                                         const int *__restrict__ ip, const int *__restrict__ ix,
+                                                         const int *__restrict__ diag, const double *__restrict__ v,
+                                                         double w, const double *__restrict__ r, double *z, const int *stop)
+{
+    if (stop && *stop) return;
+    for (int l = 0; l < nlevL; ++l)
+    {
+        for (int k = lptrL[l] + threadIdx.x; k < lptrL[l + 1]; k += blockDim.x)
+        {
+            const int i = lrowsL[k];
+            double s = r[i];
+            for (int p = ip[i]; p < diag[i]; ++p) s = s - v[p] * z[ix[p]];
+            z[i] = s * w / v[diag[i]];
+        }
+        __syncthreads();
+    }
+    const double c = (2.0 - w) / w;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) z[i] = z[i] * (c * v[diag[i]] / w);
+    __syncthreads();
+    for (int l = 0; l < nlevU; ++l)
+    {
+        for (int k = lptrU[l] + threadIdx.x; k < lptrU[l + 1]; k += blockDim.x)
+        {
+            const int i = lrowsU[k], di = diag[i];
+            double s = z[i];
+            for (int p = di + 1; p < ip[i + 1]; ++p) s = s - v[p] * z[ix[p]];
+            z[i] = s * w / v[di];
+        }
+        __syncthreads();
+    }
+}
+
+// ============================================================================ fast reductions
+constexpr int FAST_MAX_BLOCKS = 1024;
+
+__device__ inline double block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double out = sh[0];
+    __syncthreads();
+    return out;
+}
+
+__global__ __launch_bounds__(TPB) void k_fdot(int64_t n, const double *x, const double *y, double *parts)
+{
+    __shared__ double sh[TPB];
+    double s = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
+        s = fma(x[i], y[i], s);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(TPB) void k_fsum(int np, const double *parts, double *out)
+{
+    __shared__ double sh[TPB];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) s += parts[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+__global__ void k_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += (sign * alpha[0]) * x[i];
+}
+__global__ void k_scale_dev(int64_t n, const double *num, const double *den, double *x)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= (num ? num[0] : 1.0) / den[0];
+}
+__global__ void k_axpy(int64_t n, double a, const double *x, double *y)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+__global__ void k_scale(int64_t n, double a, double *x)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+struct VecPtrs { const double *p[64]; };
+// x += sum_j y[j] V[j] (the solution update of csnewton.hpp:147-156), y on the device.
+__global__ void k_lincomb(int64_t n, int k, const double *y, VecPtrs V, double *x)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = x[i];
+    for (int j = 0; j < k; ++j) s += y[j] * V.p[j][i];
+    x[i] = s;
+}
+
+// ============================================================================ FP32 CG
+__global__ __launch_bounds__(TPB) void k_r32_init(int64_t n, const double *r64, double scale, float *r, float *x,
+                                                  float *p, const float *inv, double *parts)
+{
+    __shared__ double sh[TPB];
+    double s = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
+    {
+        const float ri = static_cast<float>(r64[i] * scale);
+        const float pi = inv[i] * ri;
+        r[i] = ri; x[i] = 0.0f; p[i] = pi;
+        s += static_cast<double>(ri) * pi;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(TPB) void k_cg32_start(const double *parts, int np, CG32State *st)
+{
+    __shared__ double sh[TPB];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) s += parts[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0)
+    {
+        st->rz = s; st->it = 0; st->stop = 0; st->breakdown = 0;
+        if (!(s > 0.0)) st->stop = 1;
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_fdot32(int64_t n, const float *x, const float *y, double *parts, const int *stop)
+{
+    if (stop && *stop) return;
+    __shared__ double sh[TPB];
+    double s = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
+        s += static_cast<double>(x[i]) * y[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(TPB) void k_cg32_alpha(const double *parts, int np, CG32State *st)
+{
+    if (st->stop) return;
+    __shared__ double sh[TPB];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) s += parts[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0)
+    {
+        if (!(s > 0.0)) { st->breakdown = 1; st->stop = 1; }
+        else st->alpha = st->rz / s;
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_cg32_update(int64_t n, float *x, float *r, const float *p, const float *Ap,
+                                                     const float *inv, double *parts, const CG32State *st)
+{
+    if (st->stop) return;
+    __shared__ double sh[TPB];
+    const float a = static_cast<float>(st->alpha);
+    double srr = 0.0, srz = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
+    {
+        x[i] = fmaf(a, p[i], x[i]);
+        const float ri = fmaf(-a, Ap[i], r[i]);
+        r[i] = ri;
+        const double e = static_cast<double>(ri) * ri;
+        srr += e;
+        srz += e * inv[i];
+    }
+    srr = block_sum(srr, sh);
+    srz = block_sum(srz, sh);
+    if (threadIdx.x == 0) { parts[blockIdx.x] = srr; parts[FAST_MAX_BLOCKS + blockIdx.x] = srz; }
+}
+
+__global__ __launch_bounds__(TPB) void k_cg32_beta(const double *parts, int np, CG32State *st)
+{
+    if (st->stop) return;
+    __shared__ double sh[TPB];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) { a += parts[i]; b += parts[FAST_MAX_BLOCKS + i]; }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0)
+    {
+        st->rr = a;
+        st->it += 1;
+        if (a <= st->thr) { st->stop = 1; return; }
+        st->beta = b / st->rz;
+        st->rz = b;
+        if (st->it >= st->max_iter) st->stop = 1;
+        else if (!(b > 0.0)) { st->breakdown = 1; st->stop = 1; }
+    }
+}
+
+__global__ void k_cg32_pupdate(int64_t n, float *p, const float *r, const float *inv, const CG32State *st)
+{
+    if (st->stop) return;
+    const float beta = static_cast<float>(st->beta);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = fmaf(beta, p[i], inv[i] * r[i]);
+}
+
+__global__ void k_inv32(int n, const int *diag, const double *v, float *inv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[i] = static_cast<float>(1.0 / v[diag[i]]);
+}
+
+__global__ void k_add_scaled32(int64_t n, double scale, const float *d, double *x)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += scale * static_cast<double>(d[i]);
+}
+
+// ============================================================================ launchers
+#define FEMD_LAUNCH(kernel, grid, block, shmem, stream, ...) \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, __VA_ARGS__)
+
+#define ELEM_DISPATCH(KERNEL, N_)                                                                   \
+    case N_:                                                                                         \
+        FEMD_LAUNCH(KERNEL<N_>, grid_for(static_cast<int64_t>(nc) * N_), TPB, sh, s, nc, nq, geom, tab, qw, T, out); \
+        break;
+
+void launch_elem_matrix(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const TermSet &T, double *out, hipStream_t s)
+{
+    if (nc == 0) return;
+    const size_t sh = sizeof(double) * (static_cast<size_t>(nq) * 3 * nloc + nq);
+    switch (nloc)
+    {
+        ELEM_DISPATCH(k_elem_matrix, 3)
+        ELEM_DISPATCH(k_elem_matrix, 6)
+        ELEM_DISPATCH(k_elem_matrix, 10)
+        ELEM_DISPATCH(k_elem_matrix, 15)
+        ELEM_DISPATCH(k_elem_matrix, 21)
+    default: break;
+    }
+}
+
+void launch_elem_vector(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const TermSet &T, double *out, hipStream_t s)
+{
+    if (nc == 0) return;
+    const size_t sh = sizeof(double) * (static_cast<size_t>(nq) * 3 * nloc + nq);
+    switch (nloc)
+    {
+        ELEM_DISPATCH(k_elem_vector, 3)
+        ELEM_DISPATCH(k_elem_vector, 6)
+        ELEM_DISPATCH(k_elem_vector, 10)
+        ELEM_DISPATCH(k_elem_vector, 15)
+        ELEM_DISPATCH(k_elem_vector, 21)
+    default: break;
+    }
+}
+
+void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_gather, grid_for(n), TPB, 0, s, n, ptr, list, src, dst, plus_zero ? 1 : 0);
+}
+
+void launch_coefficient(int nc, int nq, const double *geom, const double *qxi, const double *qeta, int ninstr,
+                        const int *code, int nreg, const double *consts, double *out, hipStream_t s)
+{
+    const int64_t m = static_cast<int64_t>(nc) * nq;
+    if (m > 0) FEMD_LAUNCH(k_coefficient, grid_for(m), TPB, 0, s, nc, nq, geom, qxi, qeta, ninstr, code, nreg, consts, out);
+}
+
+void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,
+                   double *y, const int *stop, hipStream_t s)
+{
+    if (nblocks > 0) FEMD_LAUNCH((k_spmv<double, SPMV_CAP64>), nblocks, SPMV_THREADS, 0, s, blocks, ip, ix, v, x, y, stop);
+}
+
+void launch_spmv32(int nblocks, const int *blocks, const int *ip, const int *ix, const float *v, const float *x,
+                   float *y, const int *stop, hipStream_t s)
+{
+    if (nblocks > 0) FEMD_LAUNCH((k_spmv<float, SPMV_CAP64>), nblocks, SPMV_THREADS, 0, s, blocks, ip, ix, v, x, y, stop);
+}
+
+int nchunks(int64_t n) { return static_cast<int>((n + CHUNK - 1) / CHUNK); }
+
+void launch_dot_parts(int64_t n, const double *x, const double *y, double *parts, const int *stop, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_dot_parts, nchunks(n), TPB, 0, s, n, x, y, parts, stop);
+}
+
+void launch_sum_parts(int np, const double *parts, double *out, hipStream_t s)
+{
+    FEMD_LAUNCH(k_sum_parts, 1, 64, 0, s, np, parts, out);
+}
+
+void launch_sub(int64_t n, const double *b, const double *a, double *r, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_sub, grid_for(n), TPB, 0, s, n, b, a, r);
+}
+void launch_mul(int64_t n, const double *a, const double *b, double *c, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_mul, grid_for(n), TPB, 0, s, n, a, b, c);
+}
+void launch_copy(int64_t n, const double *a, double *b, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_copy, grid_for(n), TPB, 0, s, n, a, b);
+}
+void launch_fill(int64_t n, double v, double *a, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_fill, grid_for(n), TPB, 0, s, n, v, a);
+}
+void launch_gather_perm(int n, const int *perm, const double *src, double *dst, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_gather_perm, grid_for(n), TPB, 0, s, n, perm, src, dst);
+}
+void launch_scatter_perm(int n, const int *perm, const double *src, double *dst, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_scatter_perm, grid_for(n), TPB, 0, s, n, perm, src, dst);
+}
+void launch_take(int64_t n, const int *idx, const double *src, double *dst, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_take, grid_for(n), TPB, 0, s, n, idx, src, dst);
+}
+void launch_to_float(int64_t n, const double *a, float *b, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_to_float, grid_for(n), TPB, 0, s, n, a, b);
+}
+void launch_diag_positions(int n, const int *ip, const int *ix, int *diag, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_diag_positions, grid_for(n), TPB, 0, s, n, ip, ix, diag);
+}
+void launch_inv_diag(int n, const int *diag, const double *v, double *inv, int *bad_row, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_inv_diag, grid_for(n), TPB, 0, s, n, diag, v, inv, bad_row);
+}
+void launch_asymmetry(int n, const int *ip, const int *ix, const double *v, double *out2, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_asymmetry, std::min(grid_for(n), 2048), TPB, 0, s, n, ip, ix, v, out2);
+}
+
+void launch_cg_alpha(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_alpha, 1, 64, 0, s, parts, np, st); }
+void launch_cg_update(int64_t n, double *x, double *r, const double *p, const double *Ap, const double *inv,
+                      double *parts_rr, double *parts_rz, CGState *st, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_cg_update, nchunks(n), TPB, 0, s, n, x, r, p, Ap, inv, parts_rr, parts_rz, st);
+}
+void launch_cg_beta(const double *parts_rr, const double *parts_rz, int np, int mode, CGState *st, hipStream_t s)
+{
+    FEMD_LAUNCH(k_cg_beta, 1, 64, 0, s, parts_rr, parts_rz, np, mode, st);
+}
+void launch_cg_rz(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_rz, 1, 64, 0, s, parts, np, st); }
+void launch_cg_pupdate(int64_t n, double *p, const double *r, const double *inv, const CGState *st, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_cg_pupdate, grid_for(n), TPB, 0, s, n, p, r, inv, st);
+}
+
+void launch_ilu0_factor(int nlev, const int *lptr, const int *lrows, const int *ip, const int *ix, const int *diag,
+                        double *v, int *bad_row, hipStream_t s)
+{
+    FEMD_LAUNCH(k_ilu0_factor, 1, WG_SWEEP, 0, s, nlev, lptr, lrows, ip, ix, diag, v, bad_row);
+}
+void launch_ilu0_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, int nlevU, const int *lptrU,
+                       const int *lrowsU, const int *ip, const int *ix, const int *diag, const double *v,
+                       const double *r, double *z, const int *stop, hipStream_t s)
+{
+    (void)n;
+    FEMD_LAUNCH(k_ilu0_apply, 1, WG_SWEEP, 0, s, nlevL, lptrL, lrowsL, nlevU, lptrU, lrowsU, ip, ix, diag, v, r, z, stop);
+}
+void launch_ssor_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, int nlevU, const int *lptrU,
+                       const int *lrowsU, const int *ip, const int *ix, const int *diag, const double *v, double omega,
+                       const double *r, double *z, const int *stop, hipStream_t s)
+{
+    FEMD_LAUNCH(k_ssor_apply, 1, WG_SWEEP, 0, s, n, nlevL, lptrL, lrowsL, nlevU, lptrU, lrowsU, ip, ix, diag, v, omega, r, z, stop);
+}
+
+int fast_blocks(int64_t n)
+{
+    const int64_t g = (n + 4 * TPB - 1) / (4 * TPB);
+    return static_cast<int>(g < 1 ? 1 : (g > FAST_MAX_BLOCKS ? FAST_MAX_BLOCKS : g));
+}
+void launch_fdot(int64_t n, const double *x, const double *y, double *parts, hipStream_t s)
+{
+    FEMD_LAUNCH(k_fdot, fast_blocks(n), TPB, 0, s, n, x, y, parts);
+}
+void launch_fsum(int np, const double *parts, double *out, hipStream_t s) { FEMD_LAUNCH(k_fsum, 1, TPB, 0, s, np, parts, out); }
+void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_axpy_dev, grid_for(n), TPB, 0, s, n, alpha, sign, x, y);
+}
+void launch_scale_dev(int64_t n, const double *num, const double *den, double *x, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_scale_dev, grid_for(n), TPB, 0, s, n, num, den, x);
+}
+void launch_axpy(int64_t n, double a, const double *x, double *y, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_axpy, grid_for(n), TPB, 0, s, n, a, x, y);
+}
+void launch_scale(int64_t n, double a, double *x, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_scale, grid_for(n), TPB, 0, s, n, a, x);
+}
+void launch_lincomb(int64_t n, int k, const double *y, const double *const *V, double *x, hipStream_t s)
+{
+    for (int j0 = 0; j0 < k; j0 += 64)
+    {
+        VecPtrs P{};
+        const int kk = k - j0 < 64 ? k - j0 : 64;
+        for (int j = 0; j < kk; ++j) P.p[j] = V[j0 + j];
+        if (n > 0) FEMD_LAUNCH(k_lincomb, grid_for(n), TPB, 0, s, n, kk, y + j0, P, x);
+    }
+}
+
+void launch_r32_init(int64_t n, const double *r64, double scale, float *r, float *x, float *p, const float *inv,
+                     double *parts, hipStream_t s)
+{
+    FEMD_LAUNCH(k_r32_init, fast_blocks(n), TPB, 0, s, n, r64, scale, r, x, p, inv, parts);
+}
+void launch_cg32_start(const double *parts, int np, CG32State *st, hipStream_t s) { FEMD_LAUNCH(k_cg32_start, 1, TPB, 0, s, parts, np, st); }
+void launch_fdot32(int64_t n, const float *x, const float *y, double *parts, const int *stop, hipStream_t s)
+{
+    FEMD_LAUNCH(k_fdot32, fast_blocks(n), TPB, 0, s, n, x, y, parts, stop);
+}
+void launch_cg32_alpha(const double *parts, int np, CG32State *st, hipStream_t s) { FEMD_LAUNCH(k_cg32_alpha, 1, TPB, 0, s, parts, np, st); }
+void launch_cg32_update(int64_t n, float *x, float *r, const float *p, const float *Ap, const float *inv, double *parts,
+                        CG32State *st, hipStream_t s)
+{
+    FEMD_LAUNCH(k_cg32_update, fast_blocks(n), TPB, 0, s, n, x, r, p, Ap, inv, parts, st);
+}
+void launch_cg32_beta(const double *parts, int np, CG32State *st, hipStream_t s) { FEMD_LAUNCH(k_cg32_beta, 1, TPB, 0, s, parts, np, st); }
+void launch_cg32_pupdate(int64_t n, float *p, const float *r, const float *inv, const CG32State *st, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_cg32_pupdate, grid_for(n), TPB, 0, s, n, p, r, inv, st);
+}
+void launch_inv32(int n, const int *diag, const double *v, float *inv, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_inv32, grid_for(n), TPB, 0, s, n, diag, v, inv);
+}
+void launch_add_scaled32(int64_t n, double scale, const float *d, double *x, hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_add_scaled32, grid_for(n), TPB, 0, s, n, scale, d, x);
+}
+
+} // namespace femd_amd

@@ -1,0 +1,103 @@
+// kernels.hpp -- launchers of the gfx950 kernels (kernels.hip).  All take a stream and enqueue.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace femd_amd {
+
+constexpr int SPMV_THREADS = 256;          // rows per CSR-stream block at most
+constexpr int SPMV_CAP64 = 4096;           // staged products per block, FP64 (32 KiB of LDS)
+constexpr int CHUNK = 4096;                // the reference's reduction chunk (util/omp.hpp:91)
+constexpr int MAX_TERMS = 8;
+constexpr int MAX_REGS = 64;
+
+// Coefficients of up to MAX_TERMS terms: a device array [cell * nq + q] or one constant.
+struct TermSet {
+    int nt;
+    int a[MAX_TERMS], b[MAX_TERMS];
+    const double *c[MAX_TERMS];
+    double cval[MAX_TERMS];
+};
+
+// ---- assembly -------------------------------------------------------------------------------
+void launch_elem_matrix(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const TermSet &T, double *E, hipStream_t s);
+void launch_elem_vector(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const TermSet &T, double *F, hipStream_t s);
+void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s);
+void launch_coefficient(int nc, int nq, const double *geom, const double *qxi, const double *qeta, int ninstr,
+                        const int *code, int nreg, const double *consts, double *out, hipStream_t s);
+
+// ---- SpMV -----------------------------------------------------------------------------------
+void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,
+                   double *y, const int *stop, hipStream_t s);
+void launch_spmv32(int nblocks, const int *blocks, const int *ip, const int *ix, const float *v, const float *x,
+                   float *y, const int *stop, hipStream_t s);
+
+// ---- ordered reductions (util/omp.hpp:135-192) -----------------------------------------------
+int nchunks(int64_t n);
+void launch_dot_parts(int64_t n, const double *x, const double *y, double *parts, const int *stop, hipStream_t s);
+void launch_sum_parts(int np, const double *parts, double *out, hipStream_t s);
+
+// ---- vector kernels ---------------------------------------------------------------------------
+void launch_sub(int64_t n, const double *b, const double *a, double *r, hipStream_t s);             // r = b - a
+void launch_mul(int64_t n, const double *a, const double *b, double *c, hipStream_t s);             // c = a * b
+void launch_copy(int64_t n, const double *a, double *b, hipStream_t s);
+void launch_fill(int64_t n, double v, double *a, hipStream_t s);
+void launch_gather_perm(int n, const int *perm, const double *src, double *dst, hipStream_t s);     // dst[k] = src[perm[k]]
+void launch_scatter_perm(int n, const int *perm, const double *src, double *dst, hipStream_t s);    // dst[perm[k]] = src[k]
+void launch_inv_diag(int n, const int *diag, const double *v, double *inv, int *bad_row, hipStream_t s);
+void launch_diag_positions(int n, const int *ip, const int *ix, int *diag, hipStream_t s);
+void launch_take(int64_t n, const int *idx, const double *src, double *dst, hipStream_t s);         // dst[i] = src[idx[i]]
+void launch_to_float(int64_t n, const double *a, float *b, hipStream_t s);
+void launch_asymmetry(int n, const int *ip, const int *ix, const double *v, double *out2, hipStream_t s);
+
+// ---- CG (krylov/cg.hpp) -------------------------------------------------------------------------
+struct CGState {
+    double nb, tolnb, rz, rr, pAp, alpha, beta, rn;
+    int it, max_iter, stop, breakdown, converged_break, pad;
+};
+void launch_cg_alpha(const double *parts, int np, CGState *st, hipStream_t s);
+void launch_cg_update(int64_t n, double *x, double *r, const double *p, const double *Ap, const double *inv,
+                      double *parts_rr, double *parts_rz, CGState *st, hipStream_t s);
+void launch_cg_beta(const double *parts_rr, const double *parts_rz, int np, int jacobi_fused, CGState *st, hipStream_t s);
+void launch_cg_rz(const double *parts, int np, CGState *st, hipStream_t s);
+void launch_cg_pupdate(int64_t n, double *p, const double *r, const double *inv, const CGState *st, hipStream_t s);
+
+// ---- triangular sweeps and factorization (sparse/precond.hpp), one workgroup, level by level ----
+void launch_ilu0_factor(int nlev, const int *lptr, const int *lrows, const int *ip, const int *ix, const int *diag,
+                        double *v, int *bad_row, hipStream_t s);
+void launch_ilu0_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, int nlevU, const int *lptrU,
+                       const int *lrowsU, const int *ip, const int *ix, const int *diag, const double *v,
+                       const double *r, double *z, const int *stop, hipStream_t s);
+void launch_ssor_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, int nlevU, const int *lptrU,
+                       const int *lrowsU, const int *ip, const int *ix, const int *diag, const double *v, double omega,
+                       const double *r, double *z, const int *stop, hipStream_t s);
+
+// ---- fast (non-ordered) reductions and FP32 CG for the mixed-precision path ----------------------
+int fast_blocks(int64_t n);
+void launch_fdot(int64_t n, const double *x, const double *y, double *parts, hipStream_t s);
+void launch_fsum(int np, const double *parts, double *out, hipStream_t s);
+void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y, hipStream_t s);
+void launch_scale_dev(int64_t n, const double *num, const double *den, double *x, hipStream_t s);   // x *= num/den
+void launch_axpy(int64_t n, double alpha, const double *x, double *y, hipStream_t s);
+void launch_scale(int64_t n, double alpha, double *x, hipStream_t s);
+void launch_lincomb(int64_t n, int k, const double *y, const double *const *V, double *x, hipStream_t s);
+
+struct CG32State {
+    double rz, alpha, beta, rr, thr;
+    int it, max_iter, stop, breakdown;
+};
+void launch_r32_init(int64_t n, const double *r64, double scale, float *r, float *x, float *p, const float *inv,
+                     double *parts, hipStream_t s);
+void launch_cg32_start(const double *parts, int np, CG32State *st, hipStream_t s);
+void launch_fdot32(int64_t n, const float *x, const float *y, double *parts, const int *stop, hipStream_t s);
+void launch_cg32_alpha(const double *parts, int np, CG32State *st, hipStream_t s);
+void launch_cg32_update(int64_t n, float *x, float *r, const float *p, const float *Ap, const float *inv, double *parts,
+                        CG32State *st, hipStream_t s);
+void launch_cg32_beta(const double *parts, int np, CG32State *st, hipStream_t s);
+void launch_cg32_pupdate(int64_t n, float *p, const float *r, const float *inv, const CG32State *st, hipStream_t s);
+void launch_inv32(int n, const int *diag, const double *v, float *inv, hipStream_t s);
+void launch_add_scaled32(int64_t n, double scale, const float *d, double *x, hipStream_t s);          // x += scale * d
+
+} // namespace femd_amd

@@ -1,0 +1,112 @@
+"""The synthetic workloads of the benchmark configurations, built from recorded seeds.
+
+* ``fourier_source(seed)``      f(x, y) = sum_{k<=8} a_k sin(kx pi x) sin(ky pi y), a_k ~ N(0, 1),
+                                kx, ky ~ U{1..6} (numpy.random.default_rng(seed));
+* ``kappa_field(seed)``         kappa = exp(0.5 g), g = sum_{m<=16} c_m cos(2 pi (w_m . x) + phi_m),
+                                c_m ~ N(0, 1)/4, w_m integer with |w| <= 8, phi_m ~ U[0, 2 pi);
+* ``delaunay_square(h, seed)``  a seeded unstructured Delaunay mesh of the unit square: boundary
+                                nodes every h, interior nodes on a jittered triangular lattice,
+                                triangulated by Qhull and sorted by centroid rows (the reference's
+                                mesher has no seed, so the mesh arrives as arrays, as
+                                mesh_from_arrays takes them);
+* ``poisson_problem(...)``      the forms of -div(kappa grad u) + c u = f with Dirichlet sides.
+
+Each returns form-language expressions (femd_amd.forms) or arrays; nothing here runs on the GPU.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import forms as F
+
+
+def fourier_source(seed: int = 0, nmodes: int = 8, kmax: int = 6):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(nmodes)
+    kx = rng.integers(1, kmax + 1, nmodes)
+    ky = rng.integers(1, kmax + 1, nmodes)
+    terms = [F.Const(float(a[k])) * F.sin(F.Const(float(kx[k] * math.pi)) * F.x) * F.sin(F.Const(float(ky[k] * math.pi)) * F.y)
+             for k in range(nmodes)]
+    return F.Sum(terms), dict(a=a, kx=kx, ky=ky)
+
+
+def kappa_field(seed: int = 1, nmodes: int = 16, wmax: int = 8):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal(nmodes) / math.sqrt(nmodes)
+    w = []
+    while len(w) < nmodes:
+        v = rng.integers(-wmax, wmax + 1, 2)
+        if 0 < v[0] ** 2 + v[1] ** 2 <= wmax ** 2:
+            w.append(v)
+    w = np.array(w)
+    phi = rng.uniform(0.0, 2.0 * math.pi, nmodes)
+    g = F.Sum([F.Const(float(c[m])) * F.cos(F.Const(float(2 * math.pi * w[m, 0])) * F.x
+                                            + F.Const(float(2 * math.pi * w[m, 1])) * F.y + F.Const(float(phi[m])))
+               for m in range(nmodes)])
+    return F.exp(F.Const(0.5) * g), dict(c=c, w=w, phi=phi)
+
+
+def delaunay_square(h: float, seed: int = 0, jitter: float = 0.12):
+    """(points (n, 2), triangles (m, 3), min angle in degrees) of the unit square."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    nb = max(2, int(round(1.0 / h)))
+    t = np.linspace(0.0, 1.0, nb + 1)[:-1]
+    bnd = np.concatenate([np.stack([t, np.zeros_like(t)], 1), np.stack([np.ones_like(t), t], 1),
+                          np.stack([1.0 - t, np.ones_like(t)], 1), np.stack([np.zeros_like(t), 1.0 - t], 1)])
+    dy = h * math.sqrt(3.0) / 2.0
+    rows = int(math.floor(1.0 / dy))
+    pts = []
+    for j in range(1, rows + 1):
+        yj = j * dy
+        off = 0.5 * h if j % 2 else 0.0
+        xs = np.arange(off, 1.0, h)
+        pts.append(np.stack([xs, np.full_like(xs, yj)], 1))
+    P = np.concatenate(pts)
+    P = P + rng.uniform(-jitter * h, jitter * h, P.shape)
+    inside = (P[:, 0] > 0.45 * h) & (P[:, 0] < 1 - 0.45 * h) & (P[:, 1] > 0.45 * h) & (P[:, 1] < 1 - 0.45 * h)
+    P = np.concatenate([bnd, P[inside]])
+    tri = Delaunay(P).simplices.astype(np.int64)
+    # drop slivers on the straight boundary (three collinear boundary nodes)
+    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+    area = 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    tri = tri[np.abs(area) > 1e-12 * h * h]
+    cen = P[tri].mean(axis=1)
+    order = np.lexsort((cen[:, 0], np.floor(cen[:, 1] / (4 * h))))
+    tri = tri[order]
+    # renumber vertices in first-use order (locality of the vertex numbering)
+    first = np.full(P.shape[0], -1, dtype=np.int64)
+    flat = tri.reshape(-1)
+    seen = np.zeros(P.shape[0], dtype=bool)
+    newid = []
+    for v in flat:
+        if not seen[v]:
+            seen[v] = True
+            newid.append(v)
+    newid = np.asarray(newid)
+    first[newid] = np.arange(newid.size)
+    P = P[newid]
+    tri = first[tri]
+    return P, tri.astype(np.int32), min_angle(P, tri)
+
+
+def min_angle(P, tri) -> float:
+    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+
+    def ang(u, v, w):
+        e1, e2 = v - u, w - u
+        cs = (e1 * e2).sum(1) / (np.linalg.norm(e1, axis=1) * np.linalg.norm(e2, axis=1))
+        return np.degrees(np.arccos(np.clip(cs, -1, 1)))
+    return float(min(ang(a, b, c).min(), ang(b, c, a).min(), ang(c, a, b).min()))
+
+
+def poisson_forms(V, f_expr, kappa=None, reaction=0.0):
+    """a = (kappa grad u . grad v + reaction u v) dx, L = f v dx."""
+    u, v = F.TrialFunction(V), F.TestFunction(V)
+    g = F.dot(F.grad(u), F.grad(v))
+    a_expr = g if kappa is None else kappa * g
+    if reaction:
+        a_expr = a_expr + (reaction * u * v if reaction != 1.0 else u * v)
+    return F.form(a_expr * F.dx), F.form(f_expr * v * F.dx)

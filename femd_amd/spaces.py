@@ -1,0 +1,162 @@
+"""fd.LagrangeSpace on a 2D mesh (python/femd/spaces2d.py:84-279, spaces.py:435-442): the
+continuous P_k space with the reference's numbering and Dirichlet elimination, held by the
+native library (fe/space_2d.hpp:390-526, fe/lagrange_space_2d.hpp:32-67)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .mesh import Mesh2D
+
+
+def _markers(m, what):
+    if m is None:
+        return []
+    if isinstance(m, (int, np.integer)):
+        return [int(m)]
+    out = []
+    for v in m:
+        if isinstance(v, bool) or int(v) != v:
+            raise TypeError(f"{what}: markers are integers, got {v!r}")
+        out.append(int(v))
+    return out
+
+
+class LagrangeSpace2D:
+    """P_k (k >= 1) on a triangular Mesh2D.
+
+    bc:        "free" (default) or "dirichlet" (every exterior side eliminated).
+    dirichlet: a marker or list of markers whose sides are eliminated; combines with bc.
+    nodes:     "equispaced" (default).  For k <= 2 "lobatto" gives the same nodes."""
+
+    family = "lagrange"
+    tdim = 2
+    nverts = 3
+
+    def __init__(self, mesh: Mesh2D, degree: int, bc="free", dirichlet=None, nodes="equispaced"):
+        if not isinstance(mesh, Mesh2D):
+            raise TypeError(f"LagrangeSpace2D: mesh must be a Mesh2D, got {type(mesh).__name__}")
+        if isinstance(degree, bool) or int(degree) != degree:
+            raise TypeError(f"degree must be an integer, got {degree!r}")
+        k = int(degree)
+        if k < 1:
+            raise ValueError(f"LagrangeSpace2D: degree must be >= 1, got {k}")
+        b = "free" if bc is None else str(bc).lower()
+        if b not in ("free", "dirichlet"):
+            raise ValueError(f"LagrangeSpace2D: bc is 'free' or 'dirichlet' (every exterior side), got {bc!r}")
+        marks = sorted(set(_markers(dirichlet, "dirichlet")))
+        present = set(np.asarray(mesh.segment_markers).tolist())
+        missing = [m for m in marks if m not in present]
+        if missing:
+            raise ValueError(f"LagrangeSpace2D: no side of the mesh carries marker(s) {missing}; "
+                             f"the markers present are {sorted(present)}")
+        nd = str(nodes).lower()
+        if nd not in ("equispaced", "lobatto"):
+            raise ValueError(f"LagrangeSpace2D: nodes is 'equispaced' or 'lobatto', got {nodes!r}")
+        if nd == "lobatto" and k >= 3:
+            raise NotImplementedError("LagrangeSpace2D: warp-and-blend (lobatto) nodes are not built for k >= 3")
+        h = C.c_void_p()
+        d = L.i32(marks) if marks else np.zeros(1, dtype=np.int32)
+        L.check(L.load().femd_space_create(mesh._h, k, len(marks), d, int(b == "dirichlet"), 1, C.byref(h)))
+        self._h = h.value
+        self.mesh = mesh
+        self.degree = k
+        self.dirichlet_markers = marks
+        self.all_exterior = b == "dirichlet"
+        self.node_set = nd
+        s = np.zeros(6, dtype=np.int32)
+        L.check(L.load().femd_space_sizes(self._h, s))
+        self.ncells, self.nloc, self.raw_dim, self.dim, self.nedges, self.npoints = map(int, s)
+        self._num = None
+        self._pattern = None
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                L.load().femd_space_free(h)
+            except Exception:
+                pass
+
+    @property
+    def n_constraints(self) -> int:
+        return self.raw_dim - self.dim
+
+    def _numbering(self):
+        if self._num is None:
+            cd = np.empty(self.ncells * self.nloc, dtype=np.int32)
+            cr = np.empty_like(cd)
+            a2r = np.empty(max(self.dim, 1), dtype=np.int32)
+            xy = np.empty(2 * self.raw_dim)
+            L.check(L.load().femd_space_numbering(self._h, cd, cr, a2r, xy))
+            self._num = (cd.reshape(self.ncells, self.nloc), cr.reshape(self.ncells, self.nloc), a2r[: self.dim],
+                         xy.reshape(-1, 2))
+        return self._num
+
+    def cell_dofs(self) -> np.ndarray:
+        """(ncells, nloc) adapted indices, -1 where the node is eliminated."""
+        return self._numbering()[0]
+
+    def cell_raw_dofs(self) -> np.ndarray:
+        return self._numbering()[1]
+
+    def adapted_to_raw(self) -> np.ndarray:
+        return self._numbering()[2]
+
+    def raw_coordinates(self) -> np.ndarray:
+        return self._numbering()[3]
+
+    def dof_coordinates(self) -> np.ndarray:
+        """(dim, 2) coordinates of the degrees of freedom."""
+        return self.raw_coordinates()[self.adapted_to_raw()]
+
+    def node_coordinates(self) -> np.ndarray:
+        return self.raw_coordinates()
+
+    def prolongate(self, c) -> np.ndarray:
+        """Adapted -> raw coefficients (zeros on the eliminated nodes)."""
+        r = np.zeros(self.raw_dim)
+        r[self.adapted_to_raw()] = np.asarray(c, dtype=np.float64)
+        return r
+
+    def restrict(self, raw) -> np.ndarray:
+        return np.asarray(raw, dtype=np.float64)[self.adapted_to_raw()]
+
+    def interpolate(self, f) -> np.ndarray:
+        """Nodal values of a callable f(x, y) at the degrees of freedom (the coefficients)."""
+        xy = self.dof_coordinates()
+        return np.asarray(f(xy[:, 0], xy[:, 1]), dtype=np.float64) * np.ones(self.dim)
+
+    def pattern(self, trial=None):
+        """The CSR pattern (indptr, indices) of matrices with test and trial space self (cached)."""
+        if trial is not None and trial is not self:
+            raise NotImplementedError("pattern: only the square pattern of a space with itself")
+        if self._pattern is None:
+            nnz = C.c_int64()
+            L.check(L.load().femd_pattern(self._h, C.byref(nnz), None, None))
+            ip = np.empty(self.dim + 1, dtype=np.int32)
+            ix = np.empty(max(nnz.value, 1), dtype=np.int32)
+            L.check(L.load().femd_pattern(self._h, None, ip.ctypes.data, ix.ctypes.data))
+            self._pattern = (ip, ix[: nnz.value])
+        return self._pattern
+
+    @property
+    def bc_description(self) -> str:
+        parts = []
+        if self.all_exterior:
+            parts.append("Dirichlet on every exterior side")
+        if self.dirichlet_markers:
+            parts.append("Dirichlet on marker(s) " + ", ".join(map(str, self.dirichlet_markers)))
+        return "; ".join(parts) or "free (natural on every side)"
+
+    def __repr__(self):
+        return f"LagrangeSpace2D(degree={self.degree}, ncells={self.ncells}, bc={self.bc_description}, dim={self.dim})"
+
+
+def LagrangeSpace(mesh, degree, bc="free", dirichlet=None, nodes="equispaced"):
+    """fd.LagrangeSpace on a Mesh2D (the 2D branch of spaces.py:435-442)."""
+    if not isinstance(mesh, Mesh2D):
+        raise TypeError("LagrangeSpace: this package builds the 2D (Mesh2D) spaces only")
+    return LagrangeSpace2D(mesh, degree, bc=bc, dirichlet=dirichlet, nodes=nodes)

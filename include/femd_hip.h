@@ -1,0 +1,226 @@
+/*
+ * femd_hip.h -- the C ABI of the MI355X implementation of FEMd's 2D hot path:
+ * P_k element assembly of bilinear and linear forms into CSR, the CSR SpMV, and CG /
+ * GMRES / LGMRES with the Jacobi, SSOR and ILU(0) preconditioners, on one GPU (gfx950).
+ *
+ * The reference has no C ABI: its core is header-only C++ bound to Python by nanobind
+ * (python/src/femd_bindings.cpp, femd_bindings_2d.cpp).  Each entry point below names the
+ * reference interface it replaces; the Python package femd_amd calls them from the places
+ * the reference's Python layer calls its bindings.
+ *
+ * Conventions
+ *   - Every function returns an int status (FEMD_OK = 0); on failure femd_last_error()
+ *     gives the message of the calling thread.  No C++ exception crosses this boundary.
+ *   - Host pointers are caller-owned and copied in or out before the call returns.
+ *     Functions with the suffix _dev take DEVICE pointers (from femd_dev_alloc) and a
+ *     stream (NULL: the library's stream); they return once the work is enqueued, except
+ *     where a result must be read back (solver infos), which synchronizes the stream.
+ *   - CSR layout is FEMd's: 0-based int32 indptr (n+1) and indices (nnz, sorted and unique
+ *     within each row), float64 values (sparse/csr.hpp:30-35, 178-185).
+ *   - Handles own their device memory.  Calls on different handles may run concurrently
+ *     from different threads; calls on one handle must not overlap.
+ */
+#ifndef FEMD_HIP_H
+#define FEMD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+enum {
+    FEMD_OK = 0,
+    FEMD_EINVAL = 1,      /* bad argument: std::invalid_argument in the reference (ValueError) */
+    FEMD_EPATTERN = 2,    /* an element entry outside the pattern: assembly_2d.hpp:157,162,303 */
+    FEMD_EZEROPIVOT = 3,  /* zero diagonal or ILU(0) pivot: precond.hpp:68-78, 181 */
+    FEMD_ENOTSPD = 4,     /* reserved: CG reports breakdown in its info instead (cg.hpp:213-218) */
+    FEMD_EDEVICE = 5,     /* a HIP runtime error, or the HIP device is missing */
+    FEMD_ENOMEM = 6       /* device allocation failed */
+};
+
+enum { FEMD_PC_NONE = 0, FEMD_PC_JACOBI = 1, FEMD_PC_ILU0 = 2, FEMD_PC_SSOR = 3 };
+enum { FEMD_ORDER_NATURAL = 0, FEMD_ORDER_RCM = 1 };
+enum { FEMD_DIAG_RIGHT = 0, FEMD_DIAG_LEFT = 1, FEMD_DIAG_ALTERNATE = 2, FEMD_DIAG_CROSSED = 3 };
+enum { FEMD_FP64 = 0, FEMD_FP32 = 1 };
+
+typedef struct femd_mesh_s *femd_mesh;
+typedef struct femd_space_s *femd_space;
+typedef struct femd_csr_s *femd_csr;
+typedef struct femd_precond_s *femd_precond;
+typedef void *femd_stream;          /* a hipStream_t */
+
+/* The thread's last error message (empty after success). */
+const char *femd_last_error(void);
+
+/* Select the HIP device and create the library's stream.  Called implicitly by the first
+ * entry point that touches the GPU; returns FEMD_EDEVICE when no GPU is visible. */
+int femd_init(int device);
+
+/* Device memory for the _dev entry points (the library's caching pool). */
+int femd_dev_alloc(size_t bytes, void **out);
+int femd_dev_free(void *p);
+int femd_memcpy_htod(void *dst, const void *src, size_t bytes);
+int femd_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int femd_stream_sync(femd_stream s);
+
+/* ---------------------------------------------------------------- meshes
+ * femd_mesh_rectangle replaces rectangle_mesh (mesh/structured.hpp:338-351, bound at
+ * python/src/femd_bindings.cpp:463-472): grid vertex (i, j) is point j(nx+1)+i, cells
+ * split by `diag`, boundary segments with markers[4] = bottom, right, top, left.
+ * femd_mesh_from_arrays replaces mesh_from_arrays (mesh/structured.hpp:99-221): clockwise
+ * triangles are re-oriented, neighbours linked, user segments kept first, and the other
+ * boundary edges added with `marker`.  smarks may be NULL (every segment gets `marker`). */
+int femd_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int diag,
+                        const int markers[4], femd_mesh *out);
+int femd_mesh_from_arrays(int np, const double *points, int nt, const int *triangles, int ns,
+                          const int *segments, const int *smarks, int marker, femd_mesh *out);
+/* out[3]: npoints, ntriangles, nsegments */
+int femd_mesh_sizes(femd_mesh m, int out[3]);
+/* points (np x 2), triangles and neighbours (nt x 3), segments (ns x 2), markers (ns) */
+int femd_mesh_arrays(femd_mesh m, double *points, int *triangles, int *neighbours, int *segments, int *smarks);
+void femd_mesh_free(femd_mesh m);
+
+/* ---------------------------------------------------------------- spaces
+ * Replaces LagrangeSpace2D(mesh, k, dirichlet, all_exterior, equispaced)
+ * (fe/lagrange_space_2d.hpp:32-67, bound at python/src/femd_bindings_2d.cpp:571-573): the
+ * numbering of fe/space_2d.hpp:390-526, bit for bit.  Only equispaced nodes are built
+ * (equispaced = 0 returns FEMD_EINVAL for k >= 3; for k <= 2 the two node sets coincide). */
+int femd_space_create(femd_mesh m, int degree, int ndirichlet, const int *dirichlet, int all_exterior,
+                      int equispaced, femd_space *out);
+/* out[6]: ncells, nloc, raw_dim, dim, nedges, npoints */
+int femd_space_sizes(femd_space V, int out[6]);
+/* cell_dof and cell_raw: ncells x nloc (-1 = eliminated); ad2raw: dim; raw_xy: raw_dim x 2 */
+int femd_space_numbering(femd_space V, int *cell_dof, int *cell_raw, int *ad2raw, double *raw_xy);
+void femd_space_free(femd_space V);
+
+/* The CSR pattern of the space with itself (cell_pattern, forms/assembly_2d.hpp:36-41;
+ * SparsityPattern::from_cells, sparse/csr.hpp:84-125).  Call with indices == NULL to get
+ * nnz, then again with indptr (dim+1) and indices (nnz). */
+int femd_pattern(femd_space V, int64_t *nnz, int *indptr, int *indices);
+
+/* ---------------------------------------------------------------- coefficients
+ * The coefficient of one term, at every quadrature point: either a register program in the
+ * instruction set of forms/integrand.hpp:33-46 (4 ints per instruction: op, out, a, b; ops
+ * CONST X Y ADD MUL POW SIN COS EXP LOG TANH SQRT SINH COSH SECH ABS SIGN; no INPUT, no
+ * normals), evaluated on the GPU at the quadrature points, or, when `values` is not NULL,
+ * ncells x nq host values laid out [cell * nq + q] (Cache2D order, fe/cache_2d.hpp:98-118). */
+typedef struct {
+    int ninstr;
+    const int *code;
+    int nreg;
+    int nconst;
+    const double *consts;
+    const double *values;
+} femd_coefficient;
+
+/* Quadrature of a degree: nq points (the rule of quadrature/triangle.hpp:22-57). */
+int femd_quadrature(int degree, int *nq, double *xi, double *eta, double *w);
+
+/* ---------------------------------------------------------------- CSR matrices
+ * femd_csr_create_space: zero values on the space's own pattern (the K of
+ * forms.py:2089).  femd_csr_create: from host arrays (SparseMatrix.from_scipy,
+ * python/femd/sparse.py:60-70).  The pattern arrays are validated (sorted, unique, in range). */
+int femd_csr_create_space(femd_space V, femd_csr *out);
+int femd_csr_create(int nrows, int ncols, const int *indptr, const int *indices, const double *data, femd_csr *out);
+/* out: nrows, ncols; nnz */
+int femd_csr_info(femd_csr A, int out[2], int64_t *nnz);
+int femd_csr_download(femd_csr A, int *indptr, int *indices, double *data);
+int femd_csr_upload_values(femd_csr A, const double *data);
+/* Device pointers of the arrays (valid while A lives). */
+int femd_csr_device_arrays(femd_csr A, const int **indptr, const int **indices, double **data);
+/* max |a_ij - a_ji| / max |a_ij| (asymmetry, sparse/csr.hpp:631-644), on the GPU. */
+int femd_csr_asymmetry(femd_csr A, double *out);
+/* The diagonal (CSRMatrix::diagonal, csr.hpp:245-252). */
+int femd_csr_diagonal(femd_csr A, double *out);
+void femd_csr_free(femd_csr A);
+
+/* ---------------------------------------------------------------- assembly
+ * K (zeroed here) = sum_t integral c_t (D^{a_t} phi_i)(D^{b_t} phi_j) dx over the cells,
+ * codes 0 value, 1 d/dx, 2 d/dy: FormAssembler2D::assemble_matrix
+ * (forms/assembler_2d.hpp:249-266) -> assemble_matrix_2d (forms/assembly_2d.hpp:219-305).
+ * K must come from femd_csr_create_space(V).  Every entry is the reference's sum in ascending
+ * element order, so the values are the reference's bit for bit on the same coefficients. */
+int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a, const int *b,
+                         const femd_coefficient *coeffs, femd_csr K);
+/* f (dim, host) = sum_t integral c_t D^{a_t} phi_i dx: FormAssembler2D::assemble_vector
+ * (assembler_2d.hpp:166-188) -> assemble_vector_2d (assembly_2d.hpp:173-211). */
+int femd_assemble_vector(femd_space V, int quad_degree, int nterms, const int *a,
+                         const femd_coefficient *coeffs, double *f);
+/* The same into a device vector (dim doubles), on stream s. */
+int femd_assemble_vector_dev(femd_space V, int quad_degree, int nterms, const int *a,
+                             const femd_coefficient *coeffs, double *f_dev, femd_stream s);
+/* The coefficient program at the quadrature points of the space (ncells x nq, host). */
+int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient *c, double *out);
+
+/* ---------------------------------------------------------------- SpMV
+ * y = A x (CSRMatrix::apply, sparse/csr.hpp:203-215, bound as CSRMatrix.matvec): each row
+ * summed left to right in stored order, so the result is the reference's bit for bit.
+ * precision FEMD_FP32 uses the float shadow of the values (made on first use). */
+int femd_spmv(femd_csr A, const double *x, double *y);
+int femd_spmv_dev(femd_csr A, const void *x_dev, void *y_dev, int precision, femd_stream s);
+/* The dot product of util/omp.hpp:176-192 (fd.ddot), with its summation order. */
+int femd_ddot(int64_t n, const double *x, const double *y, double *out);
+int femd_ddot_dev(int64_t n, const double *x_dev, const double *y_dev, double *out, femd_stream s);
+
+/* ---------------------------------------------------------------- preconditioners
+ * Replaces JacobiPreconditioner, SSORPreconditioner(omega, ordering) and
+ * ILU0Preconditioner(ordering) (sparse/precond.hpp:82-205, bound at
+ * python/src/femd_bindings_2d.cpp:170-189): built from A (which it copies), applied as
+ * z = M^{-1} r.  ordering FEMD_ORDER_RCM builds on P A P^T with the reverse Cuthill-McKee
+ * order of sparse/amd.hpp:166-234.  min_pivot (may be NULL) receives ILU(0)'s smallest
+ * pivot.  A zero diagonal or pivot returns FEMD_EZEROPIVOT. */
+int femd_precond_create(femd_csr A, int kind, int ordering, double omega, femd_precond *out, double *min_pivot);
+int femd_precond_apply(femd_precond M, const double *r, double *z);
+int femd_precond_apply_dev(femd_precond M, const double *r_dev, double *z_dev, femd_stream s);
+/* kind, ordering, n */
+int femd_precond_info(femd_precond M, int out[3]);
+/* The RCM order of a pattern: perm[k] = original index of row k (ordering::rcm_order). */
+int femd_rcm(int n, const int *indptr, const int *indices, int *perm);
+void femd_precond_free(femd_precond M);
+
+/* ---------------------------------------------------------------- Krylov solvers */
+typedef struct {
+    int converged;
+    int iterations;
+    int breakdown;              /* CG: p^T A p <= 0 or r^T z <= 0 (cg.hpp:213-218) */
+    double residual;            /* true ||b - A x|| / ||b||, recomputed at the end */
+    double precond_residual;    /* GMRES: ||M^{-1}(b - A x)|| / ||M^{-1} b|| */
+} femd_solve_info;
+
+/* CG (fd.cg, python/femd/krylov.py:178-202 -> cg_solve, femd_bindings.cpp:799-833): M NULL
+ * or Jacobi runs the fused cg_jacobi (krylov/cg.hpp:295-350), any other M the general loop
+ * (cg.hpp:245-287).  x holds the initial guess and receives the solution.  The iterates are
+ * the reference's bit for bit. */
+int femd_cg(femd_csr A, femd_precond M, const double *b, double *x, int max_iter, double tol, femd_solve_info *info);
+int femd_cg_dev(femd_csr A, femd_precond M, const double *b_dev, double *x_dev, int max_iter, double tol,
+                femd_solve_info *info, femd_stream s);
+
+/* GMRES(restart) / LGMRES(restart, k_aug), left preconditioned (fd.gmres / fd.lgmres,
+ * krylov.py:148-175 -> krylov_solve, femd_bindings.cpp:784-798 -> krylov::solve,
+ * krylov/adapters.hpp:116-147 -> csnewton gmres / lgmres, vendor/csnewton/csnewton.hpp:174-474). */
+int femd_krylov(femd_csr A, femd_precond M, const double *b, double *x, int lgmres, int restart, int max_iter,
+                double tol, int k_aug, femd_solve_info *info);
+
+/* ---------------------------------------------------------------- mixed precision
+ * FP64 iterative refinement around an FP32 CG (the precision pattern; no reference
+ * counterpart): r = b - A x in FP64, then A d = r solved in FP32 by Jacobi-PCG to rtol
+ * inner_tol, x += d in FP64, until ||b - A x|| / ||b|| <= tol.  info.iterations counts inner
+ * CG iterations, info.breakdown the refinement steps. */
+int femd_solve_refined(femd_csr A, const double *b, double *x, double tol, double inner_tol, int max_outer,
+                       femd_solve_info *info);
+int femd_solve_refined_dev(femd_csr A, const double *b_dev, double *x_dev, double tol, double inner_tol,
+                           int max_outer, femd_solve_info *info, femd_stream s);
+
+/* ---------------------------------------------------------------- timing
+ * Kernel-time of the library's work on a stream, measured with HIP events around it. */
+int femd_event_timer_start(femd_stream s);
+int femd_event_timer_stop(femd_stream s, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEMD_HIP_H */

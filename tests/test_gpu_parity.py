@@ -1,0 +1,267 @@
+"""Parity of the HIP path with the oracle, through the C ABI, on the GPU.
+
+Bit for bit: element matrices and vectors summed in the reference's element order, the SpMV,
+the ordered dot products, every CG variant (iterates, iteration counts, residuals), the
+Jacobi / SSOR / ILU(0) applications and factors.  Within tolerances: coefficient programs
+(the GPU's sin / cos / exp are not glibc's: 4 ulp), GMRES / LGMRES (their dot products are tree
+reductions; the reference's own are OpenMP reductions that change bits with the thread count),
+and the FP32-inner refined solve (1e-8 against the FP64 oracle solution)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import femd_amd as fd
+from femd_amd import _lib as L
+from femd_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _name(s):
+    print(f"\n[case] {s}", flush=True)
+
+
+def _oracle_c0(orc, n=32, k=2, diag="right"):
+    P = orc.rectangle(n, n, k, diag)
+    ip, ix = P.pattern()
+    x, y, w = P.cache_points(4)
+    A = P.assemble_matrix(4, [1, 2, 0], [1, 2, 0], np.ones((3, x.size)))
+    f, _ = problems.fourier_source(0)
+    code, nreg, consts = fd.forms.Program(f).arrays()
+    xb, yb, _ = P.cache_points(5)
+    cb = orc.integrand_eval(code, nreg, consts, xb, yb)
+    b = P.assemble_vector(5, [0], cb[None, :])
+    return P, ip, ix, A, b, cb
+
+
+def _assemble_values(V, deg, a, b, coeffs):
+    """Assemble through the C ABI with host coefficient arrays (one per term)."""
+    arr, keep = L.coefficient_array([(None, c) for c in coeffs])
+    K = fd.SparseMatrix._on_space(V)
+    L.check(L.load().femd_assemble_matrix(V._h, deg, len(a), L.i32(a), L.i32(b), arr, K._h))
+    return K
+
+
+def _vector_values(V, deg, a, coeffs):
+    arr, keep = L.coefficient_array([(None, c) for c in coeffs])
+    f = np.empty(V.dim)
+    L.check(L.load().femd_assemble_vector(V._h, deg, len(a), L.i32(a), arr, f))
+    return f
+
+
+@pytest.mark.parametrize("k,deg,diag", [(1, 2, "right"), (2, 4, "alternate"), (3, 8, "crossed"), (4, 9, "left"), (5, 10, "right")])
+def test_assembly_bit_exact(orc, k, deg, diag):
+    _name(f"assembly k={k} deg={deg} diag={diag}")
+    m = fd.rectangle_mesh(0, 0, 1, 1, 7, 6, diagonal=diag)
+    V = fd.LagrangeSpace(m, k, bc="dirichlet")
+    P = orc.rectangle(7, 6, k, diag)
+    npt = P.ncells * len(orc.quadrature(deg)[2])
+    rng = np.random.default_rng(k)
+    c = rng.standard_normal((4, npt))
+    c[2, ::5] = 0.0
+    a, b = [1, 2, 0, 1], [1, 2, 0, 2]
+    K = _assemble_values(V, deg, a, b, c)
+    assert np.array_equal(K.data, P.assemble_matrix(deg, a, b, c))
+    assert np.array_equal(_vector_values(V, deg, [0, 1, 2], c[:3]), P.assemble_vector(deg, [0, 1, 2], c[:3]))
+
+
+def test_form_assembly_c0(orc):
+    _name("form assembly C0 (device coefficient programs)")
+    m = fd.rectangle_mesh(0, 0, 1, 1, 32, 32)
+    V = fd.LagrangeSpace(m, 2, bc="dirichlet")
+    u, v = fd.TrialFunction(V), fd.TestFunction(V)
+    A = fd.form((fd.dot(fd.grad(u), fd.grad(v)) + u * v) * fd.dx).assemble()
+    f, _ = problems.fourier_source(0)
+    b = fd.form(f * v * fd.dx).assemble()
+    P, ip, ix, Ao, bo, cb = _oracle_c0(orc)
+    assert np.array_equal(A.indptr, ip) and np.array_equal(A.indices, ix)
+    assert np.array_equal(A.data, Ao)                        # constant coefficients: bit for bit
+    assert A.symmetric
+    assert np.abs(b - bo).max() <= 1e-14 * np.abs(bo).max()
+    # the device program against the oracle's libm at the same points: a few ulp
+    cg = np.empty_like(cb)
+    code, nreg, consts = fd.forms.Program(f).arrays()
+    arr, keep = L.coefficient_array([((code, nreg, consts), None)])
+    L.check(L.load().femd_eval_coefficient(V._h, 5, arr, cg))
+    assert np.abs(cg - cb).max() <= 4e-15 * max(1.0, np.abs(cb).max())
+
+
+def test_kappa_program_p3(orc):
+    _name("kappa program P3")
+    V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 6, 6), 3, bc="dirichlet")
+    P = orc.rectangle(6, 6, 3)
+    kap, _ = problems.kappa_field(1)
+    code, nreg, consts = fd.forms.Program(kap).arrays()
+    x, y, _ = P.cache_points(8)
+    co = orc.integrand_eval(code, nreg, consts, x, y)
+    cg = np.empty_like(co)
+    arr, keep = L.coefficient_array([((code, nreg, consts), None)])
+    L.check(L.load().femd_eval_coefficient(V._h, 8, arr, cg))
+    assert np.abs(cg / co - 1).max() <= 1e-14
+
+
+def test_spmv_and_dot_bit_exact(orc):
+    _name("spmv + ddot")
+    P, ip, ix, A, b, _ = _oracle_c0(orc, 40, 3)
+    M = fd.SparseMatrix.from_arrays(ip.size - 1, ip.size - 1, ip, ix, A)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        x = rng.standard_normal(ip.size - 1)
+        assert np.array_equal(M.matvec(x), orc.spmv(ip, ix, A, x))
+    lib = L.load()
+    for n in (1, 7, 8, 9, 4095, 4096, 4097, 8191, 12289, 100003, 1 << 20):
+        u, v = rng.standard_normal(n), rng.standard_normal(n)
+        out = C.c_double()
+        L.check(lib.femd_ddot(n, u, v, C.byref(out)))
+        assert out.value == orc.ddot(u, v), n
+
+
+def test_spmv_long_rows_and_empty_rows(orc):
+    _name("spmv long/empty rows")
+    rng = np.random.default_rng(5)
+    n = 300
+    rows = []
+    for i in range(n):
+        if i % 50 == 0:
+            cols = np.arange(n)                                 # dense rows
+        elif i % 7 == 0:
+            cols = np.array([], dtype=int)                      # empty rows
+        else:
+            cols = np.unique(rng.integers(0, n, 12))
+        rows.append(cols)
+    ip = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    ix = np.concatenate(rows).astype(np.int32)
+    v = rng.standard_normal(ix.size)
+    # a row longer than the staging buffer
+    big = 5000
+    ip2 = np.concatenate([ip, [ip[-1] + big]]).astype(np.int32)
+    ix2 = np.concatenate([ix, np.sort(rng.choice(n + 1, size=min(big, n + 1), replace=False))[:big]]).astype(np.int32)
+    ip2[-1] = ip[-1] + (ix2.size - ix.size)
+    v2 = rng.standard_normal(ix2.size)
+    M = fd.SparseMatrix.from_arrays(n + 1, n + 1, ip2, ix2, v2)
+    x = rng.standard_normal(n + 1)
+    assert np.array_equal(M.matvec(x), orc.spmv(ip2, ix2, v2, x))
+
+
+@pytest.mark.parametrize("M", [None, "jacobi", "ilu0", "ssor"])
+@pytest.mark.parametrize("ordering", ["natural", "rcm"])
+def test_cg_bit_exact_c0(orc, M, ordering):
+    _name(f"cg C0 M={M} ordering={ordering}")
+    P, ip, ix, A, b, _ = _oracle_c0(orc)
+    Ag = fd.SparseMatrix.from_arrays(ip.size - 1, ip.size - 1, ip, ix, A, symmetric=True)
+    pc = None if M is None else Ag.preconditioner(M, omega=1.0, ordering=ordering)
+    x, info = fd.cg(Ag, b, M=pc)
+    xo, io = orc.cg(ip, ix, A, b, M=M, ordering=ordering)
+    assert info.iterations == io["iterations"] and info.converged == io["converged"]
+    assert info.residual == io["residual"]
+    assert np.array_equal(x, xo)
+
+
+def test_cg_unstructured_p3_ilu_rcm(orc):
+    _name("cg unstructured P3 ilu0 rcm + kappa")
+    Pts, T, _ = problems.delaunay_square(0.08, seed=2)
+    m = fd.mesh_from_arrays(Pts, T)
+    V = fd.LagrangeSpace(m, 3, bc="dirichlet")
+    Po = orc.from_arrays(Pts, T, k=3)
+    kap, _ = problems.kappa_field(1)
+    code, nreg, consts = fd.forms.Program(kap).arrays()
+    x, y, _ = Po.cache_points(8)
+    ck = orc.integrand_eval(code, nreg, consts, x, y)
+    ip, ix = Po.pattern()
+    Ao = Po.assemble_matrix(8, [1, 2], [1, 2], np.stack([ck, ck]))
+    K = _assemble_values(V, 8, [1, 2], [1, 2], [ck, ck])
+    assert np.array_equal(K.data, Ao)
+    f, _ = problems.fourier_source(0)
+    cf = orc.integrand_eval(*fd.forms.Program(f).arrays(), *Po.cache_points(7)[:2])
+    b = Po.assemble_vector(7, [0], cf[None, :])
+    pc = K.preconditioner("ilu0", ordering="rcm")
+    xg, info = fd.cg(K, b, M=pc)
+    xo, io = orc.cg(ip, ix, Ao, b, M="ilu0", ordering="rcm")
+    assert info.iterations == io["iterations"] and np.array_equal(xg, xo)
+    _, mp = orc.precond_apply(ip, ix, Ao, b, "ilu0", "rcm")
+    assert pc.min_pivot == mp
+
+
+@pytest.mark.parametrize("kind,ordering", [("jacobi", "natural"), ("ilu0", "natural"), ("ilu0", "rcm"), ("ssor", "rcm")])
+def test_precond_apply_bit_exact(orc, kind, ordering):
+    _name(f"precond {kind} {ordering}")
+    P, ip, ix, A, b, _ = _oracle_c0(orc, 20, 3)
+    Ag = fd.SparseMatrix.from_arrays(ip.size - 1, ip.size - 1, ip, ix, A)
+    om = 1.4 if kind == "ssor" else 1.0
+    z = Ag.preconditioner(kind, omega=om, ordering=ordering).solve(b)
+    zo, _ = orc.precond_apply(ip, ix, A, b, kind, ordering, om)
+    assert np.array_equal(z, zo)
+
+
+def test_ilu_factor_bit_exact(orc):
+    _name("ilu factor values")
+    P, ip, ix, A, b, _ = _oracle_c0(orc, 16, 2)
+    perm, ip2, ix2, lu, mp = orc.ilu0_factor(ip, ix, A, "rcm")
+    Ag = fd.SparseMatrix.from_arrays(ip.size - 1, ip.size - 1, ip, ix, A)
+    assert np.array_equal(Ag.ordering("rcm"), perm)
+    assert Ag.preconditioner("ilu0", ordering="rcm").min_pivot == mp
+
+
+@pytest.mark.parametrize("lgmres", [False, True])
+@pytest.mark.parametrize("M", [None, "ilu0"])
+def test_gmres_tolerance(orc, lgmres, M):
+    _name(f"gmres lgmres={lgmres} M={M}")
+    P, ip, ix, A, b, _ = _oracle_c0(orc, 14)
+    rows = np.repeat(np.arange(ip.size - 1), np.diff(ip))
+    A2 = A + 0.05 * np.where(ix > rows, 1.0, np.where(ix < rows, -1.0, 0.0))
+    Ag = fd.SparseMatrix.from_arrays(ip.size - 1, ip.size - 1, ip, ix, A2)
+    pc = None if M is None else Ag.preconditioner(M)
+    f = fd.lgmres if lgmres else fd.gmres
+    x, info = f(Ag, b, M=pc, restart=20)
+    xo, io = orc.krylov(ip, ix, A2, b, M=M, lgmres=lgmres, restart=20)
+    assert info.converged == io["converged"]
+    if io["converged"]:
+        assert abs(info.iterations - io["iterations"]) <= max(2, io["iterations"] // 20)
+        assert np.abs(x - xo).max() <= 1e-7 * np.abs(xo).max()
+        assert info.precond_residual <= 1e-10 * (1 + 1e-8) or info.converged
+
+
+def test_refined_fp32_reaches_1e8(orc):
+    _name("refined fp32 P3 unstructured")
+    Pts, T, _ = problems.delaunay_square(0.05, seed=3)
+    m = fd.mesh_from_arrays(Pts, T)
+    V = fd.LagrangeSpace(m, 3, bc="dirichlet")
+    kap, _ = problems.kappa_field(1)
+    f, _ = problems.fourier_source(0)
+    a, Lf = problems.poisson_forms(V, f, kappa=kap)
+    A, b = a.assemble(), Lf.assemble()
+    x, info = fd.solve_refined(A, b, tol=1e-10, inner_tol=1e-4)
+    assert info.converged
+    ip, ix = V.pattern()
+    xo, io = orc.cg(ip, ix, A.data, b, M="ilu0", tol=1e-13)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8
+
+
+def test_breakdown_and_errors(orc):
+    _name("cg breakdown + zero diagonal + maxiter")
+    n = 10
+    ip = np.arange(n + 1, dtype=np.int32)
+    ix = np.arange(n, dtype=np.int32)
+    v = np.ones(n); v[3] = -1.0
+    A = fd.SparseMatrix.from_arrays(n, n, ip, ix, v)
+    b = np.ones(n)
+    with pytest.warns(RuntimeWarning):
+        x, info = fd.cg(A, b)
+    xo, io = orc.cg(ip, ix, v, b)
+    assert info.breakdown == io["breakdown"] and info.iterations == io["iterations"]
+    v0 = v.copy(); v0[5] = 0.0
+    Z = fd.SparseMatrix.from_arrays(n, n, ip, ix, v0)
+    with pytest.raises(ValueError):
+        Z.preconditioner("jacobi")
+    with pytest.raises(ValueError):
+        Z.preconditioner("ilu0")
+    P, ipc, ixc, Ac, bc, _ = _oracle_c0(orc, 12)
+    Ag = fd.SparseMatrix.from_arrays(ipc.size - 1, ipc.size - 1, ipc, ixc, Ac)
+    with pytest.warns(RuntimeWarning):
+        x, info = fd.cg(Ag, bc, maxiter=3)
+    xo, io = orc.cg(ipc, ixc, Ac, bc, maxiter=3)
+    assert not info.converged and info.iterations == 3 and np.array_equal(x, xo)
+    x, info = fd.cg(Ag, np.zeros_like(bc))
+    assert info.converged and not x.any()

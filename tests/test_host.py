@@ -1,0 +1,134 @@
+"""The product's host-side setup and its C ABI, on the CPU (no kernel launches):
+the library loads and exports every entry point of include/femd_hip.h; meshes, the P_k
+numbering, the pattern, quadrature and RCM agree bit for bit with the oracle; bad input is
+rejected with the reference's exception types."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import femd_amd as fd
+from femd_amd import _lib, problems
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "femd_hip.h").read_text()
+    declared = sorted(set(re.findall(r"\b(femd_[a-z0-9_]+)\s*\(", header)))
+    assert len(declared) > 40
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(_lib.exported_symbols()) <= set(declared)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("diag", ["right", "left", "alternate", "crossed"])
+def test_rectangle_numbering_vs_oracle(orc, k, diag):
+    m = fd.rectangle_mesh(0, 0, 1, 1, 5, 3, diagonal=diag)
+    for kw, okw in ((dict(bc="dirichlet"), dict(all_exterior=True)),
+                    (dict(dirichlet=[1, 4]), dict(dirichlet=[1, 4], all_exterior=False)),
+                    (dict(), dict(all_exterior=False))):
+        V = fd.LagrangeSpace(m, k, **kw)
+        P = orc.rectangle(5, 3, k, diag, **okw)
+        num = P.numbering()
+        assert np.array_equal(V.cell_dofs(), num["cell_dof"])
+        assert np.array_equal(V.cell_raw_dofs(), num["cell_raw"])
+        assert np.array_equal(V.adapted_to_raw(), num["ad2raw"])
+        assert np.array_equal(V.raw_coordinates(), num["raw_xy"], equal_nan=True)
+        for a, b in zip(V.pattern(), P.pattern()):
+            assert np.array_equal(a, b)
+
+
+def test_mesh_arrays_vs_oracle(orc):
+    P, T, _ = problems.delaunay_square(0.15, seed=4)
+    T = T.copy()
+    T[::3] = T[::3][:, [0, 2, 1]]
+    m = fd.mesh_from_arrays(P, T)
+    po = orc.from_arrays(P, T, k=2)
+    pts, tri, nbr, seg, mk = orc.mesh_arrays(po)
+    assert np.array_equal(m.triangles, tri) and np.array_equal(m.neighbours, nbr)
+    assert np.array_equal(m.segments, seg) and np.array_equal(m.segment_markers, mk)
+    V = fd.LagrangeSpace(m, 3, bc="dirichlet")
+    p3 = orc.from_arrays(P, T, k=3)
+    assert np.array_equal(V.cell_dofs(), p3.numbering()["cell_dof"])
+    for a, b in zip(V.pattern(), p3.pattern()):
+        assert np.array_equal(a, b)
+
+
+def test_user_segments_and_markers(orc):
+    pts = np.array([[0, 0], [1, 0], [1, 1], [0, 1], [0.5, 0.5]], dtype=float)
+    tris = np.array([[0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4]])
+    m = fd.mesh_from_arrays(pts, tris, segments=[[2, 1], [0, 4]], segment_markers=[7, 9], marker=5)
+    po = orc.from_arrays(pts, tris, segments=[[2, 1], [0, 4]], markers=[7, 9], k=1)
+    _, _, _, seg, mk = orc.mesh_arrays(po)
+    # the oracle numbers unlisted boundary edges with marker 1; the product with `marker`
+    assert np.array_equal(m.segments, seg)
+    assert list(m.segment_markers[:2]) == [7, 9] and set(m.segment_markers[2:]) == {5}
+    V = fd.LagrangeSpace(m, 2, dirichlet=[9])                 # an interior segment as a Dirichlet side
+    P = orc.from_arrays(pts, tris, segments=[[2, 1], [0, 4]], markers=[7, 9], k=2, dirichlet=[9], all_exterior=False)
+    assert np.array_equal(V.cell_dofs(), P.numbering()["cell_dof"])
+
+
+def test_quadrature_and_rcm_vs_oracle(orc):
+    lib = _lib.load()
+    for d in range(0, 13):
+        xi, eta, w = np.empty(64), np.empty(64), np.empty(64)
+        nq = ctypes.c_int()
+        _lib.check(lib.femd_quadrature(d, ctypes.byref(nq), xi, eta, w))
+        oxi, oeta, ow = orc.quadrature(d)
+        n = nq.value
+        assert np.array_equal(xi[:n], oxi) and np.array_equal(eta[:n], oeta) and np.array_equal(w[:n], ow)
+    V = fd.LagrangeSpace(fd.mesh_from_arrays(*problems.delaunay_square(0.1, seed=1)[:2]), 3, bc="dirichlet")
+    ip, ix = V.pattern()
+    p = np.empty(V.dim, dtype=np.int32)
+    _lib.check(lib.femd_rcm(V.dim, ip, ix, p))
+    assert np.array_equal(p, orc.rcm(ip, ix))
+
+
+def test_errors_are_the_references():
+    with pytest.raises(ValueError):
+        fd.rectangle_mesh(0, 0, -1, 1, 3, 3)
+    with pytest.raises(ValueError):
+        fd.rectangle_mesh(0, 0, 1, 1, 3, 3, diagonal="zigzag")
+    m = fd.rectangle_mesh(0, 0, 1, 1, 2, 2)
+    with pytest.raises(ValueError):
+        fd.LagrangeSpace(m, 0)
+    with pytest.raises(ValueError):
+        fd.LagrangeSpace(m, 2, dirichlet=[42])
+    with pytest.raises(ValueError):
+        fd.mesh_from_arrays([[0, 0], [1, 0], [2, 0]], [[0, 1, 2]])            # collinear
+    with pytest.raises(ValueError):
+        fd.mesh_from_arrays([[0, 0], [1, 0], [0, 1]], [[0, 1, 5]])            # index out of range
+
+
+def test_form_canonicalization_and_degree():
+    V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 2, 2), 2, bc="dirichlet")
+    u, v = fd.TrialFunction(V), fd.TestFunction(V)
+    F = fd.form((fd.dot(fd.grad(u), fd.grad(v)) + u * v) * fd.dx)
+    assert [(t.test.code(), t.trial.code()) for t in F.terms] == [(1, 1), (2, 2), (0, 0)]
+    assert F.quad_degree == 4                                                   # 7-point rule
+    f, _ = problems.fourier_source(0)
+    assert fd.form(f * v * fd.dx).quad_degree == 5                              # 3 + 2
+    V3 = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 2, 2), 3, bc="dirichlet")
+    u3, v3 = fd.TrialFunction(V3), fd.TestFunction(V3)
+    kap, _ = problems.kappa_field(1)
+    assert fd.form(kap * fd.dot(fd.grad(u3), fd.grad(v3)) * fd.dx).quad_degree == 8   # 4 + 4: 25 points
+    assert fd.form(f * v3 * fd.dx).quad_degree == 7
+    assert fd.form((fd.dot(fd.grad(u), fd.grad(v))) * fd.dx(9)).quad_degree == 9
+    with pytest.raises(TypeError):
+        fd.form(u * fd.dx)
+    with pytest.raises(NotImplementedError):
+        fd.form(u * v * fd.ds)
+
+
+def test_program_matches_reference_instruction_order():
+    """The program of a*sin(b*x) emits CONST a, CONST b, X, MUL, SIN, MUL (forms.py:1870-1900)."""
+    e = fd.Const(2.0) * fd.sin(fd.Const(3.0) * fd.x)
+    from femd_amd.forms import Program
+    code, nreg, consts = Program(e).arrays()
+    assert code.tolist() == [[0, 0, 0, 0], [0, 1, 1, 0], [1, 2, 0, 0], [7, 3, 1, 2], [9, 4, 3, 0], [7, 5, 0, 4]]
+    assert nreg == 6 and consts.tolist() == [2.0, 3.0]

@@ -1,0 +1,325 @@
+"""The oracle (oracle/femd_oracle.cpp) pinned three ways, on the CPU:
+
+1. against the reference compiled in place (oracle/_ref, built from the reference's own headers):
+   bit-for-bit on numbering, pattern, assembly, SpMV, dot products, every CG variant, the
+   preconditioners, RCM and GMRES / LGMRES (one thread);
+2. against closed forms that a plausible slip would break: quadrature exactness, nodal basis,
+   textbook element matrices, counts, the patch test, exact LU of tridiagonal matrices;
+3. against the known answers the reference prints (docs/manual.md:1279, example13.ipynb).
+"""
+import math
+
+import numpy as np
+import pytest
+
+DIAGS = ("right", "left", "alternate", "crossed")
+
+
+# ------------------------------------------------------------------------ 1. vs the reference
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("diag", DIAGS)
+def test_numbering_pattern_vs_reference(orc, ref, k, diag):
+    for kw in (dict(all_exterior=True), dict(dirichlet=[2, 3], all_exterior=False), dict(all_exterior=False)):
+        po, pr = orc.rectangle(5, 4, k, diag, **kw), ref.rectangle(5, 4, k, diag, **kw)
+        assert list(po.sizes()[:6]) == list(pr.sizes())
+        no, nr = po.numbering(), pr.numbering()
+        for key in ("cell_dof", "cell_raw", "ad2raw"):
+            assert np.array_equal(no[key], nr[key]), key
+        assert np.array_equal(no["raw_xy"], nr["raw_xy"])
+        for a, b in zip(po.pattern(), pr.pattern()):
+            assert np.array_equal(a, b)
+
+
+def _small_delaunay(seed):
+    from femd_amd import problems
+    P, T, _ = problems.delaunay_square(0.2, seed=seed)
+    return P, T
+
+
+def test_unstructured_numbering_vs_reference(orc, ref):
+    rng = np.random.default_rng(3)
+    for seed in range(3):
+        P, T = _small_delaunay(seed)
+        T = T.copy()
+        flip = rng.random(len(T)) < 0.3                      # clockwise input is re-oriented
+        T[flip] = T[flip][:, [0, 2, 1]]
+        for k in (1, 2, 3):
+            po, pr = orc.from_arrays(P, T, k=k), ref.from_arrays(P, T, k=k)
+            no, nr = po.numbering(), pr.numbering()
+            for key in ("cell_dof", "cell_raw", "ad2raw"):
+                assert np.array_equal(no[key], nr[key]), key
+            for a, b in zip(po.pattern(), pr.pattern()):
+                assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("k,deg", [(1, 2), (2, 4), (2, 5), (3, 6), (3, 8), (4, 9)])
+def test_assembly_vs_reference(orc, ref, k, deg):
+    rng = np.random.default_rng(k * 10 + deg)
+    po, pr = orc.rectangle(6, 5, k, "alternate"), ref.rectangle(6, 5, k, "alternate")
+    for a, b in zip(po.cache_points(deg), pr.cache_points(deg)):
+        assert np.array_equal(a, b)
+    npt = po.ncells * len(orc.quadrature(deg)[2])
+    c = rng.standard_normal((4, npt))
+    c[1, ::7] = 0.0                                          # exact zeros take the skip branch
+    A, B = [1, 2, 0, 1], [1, 2, 0, 2]
+    assert np.array_equal(po.assemble_matrix(deg, A, B, c), pr.assemble_matrix(deg, A, B, c))
+    assert np.array_equal(po.assemble_vector(deg, [0, 1, 2], c[:3]), pr.assemble_vector(deg, [0, 1, 2], c[:3]))
+
+
+def test_integrand_vs_reference(orc, ref):
+    from femd_amd import problems, forms
+    f, _ = problems.fourier_source(0)
+    kap, _ = problems.kappa_field(1)
+    rng = np.random.default_rng(0)
+    xs, ys = rng.random(5000), rng.random(5000)
+    for e in (f, kap, forms.sqrt(forms.x + 2.0) ** 3 + forms.tanh(forms.y) / (1.0 + forms.x ** 2) - forms.sech(forms.x)):
+        code, nreg, consts = forms.Program(e).arrays()
+        assert np.array_equal(orc.integrand_eval(code, nreg, consts, xs, ys), ref.integrand_eval(code, nreg, consts, xs, ys))
+
+
+def _c0(lib, n=16, k=2):
+    P = lib.rectangle(n, n, k)
+    ip, ix = P.pattern()
+    x, y, w = P.cache_points(4)
+    A = P.assemble_matrix(4, [1, 2, 0], [1, 2, 0], np.ones((3, x.size)))
+    xb, yb, _ = P.cache_points(5)
+    b = P.assemble_vector(5, [0], np.sin(3 * xb) * np.cos(2 * yb) + 1.0)
+    return ip, ix, A, b
+
+
+def test_spmv_dot_vs_reference(orc, ref):
+    ip, ix, A, b = _c0(orc, 20)
+    x = np.random.default_rng(0).standard_normal(b.size)
+    assert np.array_equal(orc.spmv(ip, ix, A, x), ref.spmv(ip, ix, A, x))
+    rng = np.random.default_rng(1)
+    for n in (0, 1, 7, 8, 9, 4095, 4096, 4097, 8191, 12289, 100003):
+        u, v = rng.standard_normal(n), rng.standard_normal(n)
+        assert orc.ddot(u, v) == ref.ddot(u, v), n
+
+
+@pytest.mark.parametrize("M", [None, "jacobi", "ilu0", "ssor"])
+@pytest.mark.parametrize("ordering", ["natural", "rcm"])
+def test_cg_vs_reference(orc, ref, M, ordering):
+    ip, ix, A, b = _c0(orc, 24)
+    xo, io = orc.cg(ip, ix, A, b, M=M, ordering=ordering)
+    xr, ir = ref.cg(ip, ix, A, b, M=M, ordering=ordering)
+    assert io == ir
+    assert np.array_equal(xo, xr)
+    assert io["converged"] and io["residual"] <= 1e-10 * (1 + 1e-6) + 1e-15
+
+
+@pytest.mark.parametrize("kind", ["jacobi", "ilu0", "ssor"])
+def test_precond_vs_reference(orc, ref, kind):
+    ip, ix, A, b = _c0(orc, 12, 3)
+    for ordering in ("natural", "rcm"):
+        zo, mo = orc.precond_apply(ip, ix, A, b, kind, ordering, 1.3 if kind == "ssor" else 1.0)
+        zr, mr = ref.precond_apply(ip, ix, A, b, kind, ordering, 1.3 if kind == "ssor" else 1.0)
+        assert np.array_equal(zo, zr)
+        if kind == "ilu0":
+            assert mo == mr
+    assert np.array_equal(orc.rcm(ip, ix), ref.rcm(ip, ix))
+
+
+@pytest.mark.parametrize("lgmres", [False, True])
+@pytest.mark.parametrize("M", [None, "ilu0"])
+def test_gmres_vs_reference(orc, ref, lgmres, M):
+    ref.set_threads(1)
+    ip, ix, A, b = _c0(orc, 14)
+    # a nonsymmetric perturbation: convection-like skew part on the pattern
+    rows = np.repeat(np.arange(ip.size - 1), np.diff(ip))
+    A2 = A + 0.05 * np.where(ix > rows, 1.0, np.where(ix < rows, -1.0, 0.0))
+    xo, io = orc.krylov(ip, ix, A2, b, M=M, lgmres=lgmres, restart=20)
+    xr, ir = ref.krylov(ip, ix, A2, b, M=M, lgmres=lgmres, restart=20)
+    assert io == ir
+    assert np.array_equal(xo, xr)
+
+
+# ------------------------------------------------------------------------ 2. closed forms
+def test_quadrature_exactness(orc):
+    for d in range(0, 14):
+        xi, eta, w = orc.quadrature(d)
+        assert abs(w.sum() - 0.5) < 1e-15
+        for a in range(d + 1):
+            for b in range(d + 1 - a):
+                exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+                assert abs((w * xi ** a * eta ** b).sum() - exact) < 1e-14, (d, a, b)
+    # the rules switch at the degrees of triangle.hpp:27-57
+    assert [len(orc.quadrature(d)[2]) for d in (1, 2, 3, 5, 6, 7, 8, 9)] == [1, 3, 7, 7, 16, 20, 25, 30]
+
+
+def _nodes(k):
+    pts = [(0.0, 0.0), (1.0, 0.0), (0.0, 1.0)]
+    V = [(0, 0), (1, 0), (0, 1)]
+    for e in range(3):
+        a, b = V[(e + 1) % 3], V[(e + 2) % 3]
+        for j in range(1, k):
+            t = j / k
+            pts.append(((1 - t) * a[0] + t * b[0], (1 - t) * a[1] + t * b[1]))
+    for j in range(1, k):
+        for i in range(1, k - j):
+            pts.append((i / k, j / k))
+    return pts
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_basis_nodal(orc, k):
+    nodes = _nodes(k)
+    n = len(nodes)
+    for j, (xi, eta) in enumerate(nodes):
+        B = orc.basis_eval(k, xi, eta)
+        e = np.zeros(n); e[j] = 1.0
+        assert np.abs(B[0] - e).max() < 1e-12
+    rng = np.random.default_rng(k)
+    for _ in range(5):
+        xi, eta = rng.random(2) * 0.5
+        B = orc.basis_eval(k, xi, eta)
+        assert abs(B[0].sum() - 1) < 1e-12 and abs(B[1].sum()) < 1e-11 and abs(B[2].sum()) < 1e-11
+        # d/dxi of the basis reproduces d/dxi of xi^2 eta (in the span for k >= 3)
+        if k >= 3:
+            vals = np.array([p[0] ** 2 * p[1] for p in nodes])
+            assert abs(B[1] @ vals - 2 * xi * eta) < 1e-10 and abs(B[2] @ vals - xi ** 2) < 1e-10
+
+
+def test_p1_element_matrices(orc):
+    P = orc.from_arrays([[0.0, 0.0], [2.0, 0.0], [0.5, 1.5]], [[0, 1, 2]], k=1, all_exterior=False)
+    ip, ix = P.pattern()
+    npt = P.ncells * len(orc.quadrature(2)[2])
+    M = np.zeros((3, 3))
+    v = P.assemble_matrix(2, [0], [0], np.ones((1, npt)))
+    rows = np.repeat(np.arange(3), np.diff(ip))
+    M[rows, ix] = v
+    area = 1.5
+    assert np.abs(M - area / 12 * np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]])).max() < 1e-15
+    K = np.zeros((3, 3))
+    K[rows, ix] = P.assemble_matrix(2, [1, 2], [1, 2], np.ones((2, npt)))
+    # cotangent formula: K_ij = -(cot of the angle opposite edge ij) / 2
+    X = np.array([[0.0, 0.0], [2.0, 0.0], [0.5, 1.5]])
+    for i in range(3):
+        j, m = (i + 1) % 3, (i + 2) % 3
+        u, w = X[i] - X[m], X[j] - X[m]
+        cot = (u @ w) / abs(u[0] * w[1] - u[1] * w[0])
+        assert abs(K[i, j] + cot / 2) < 1e-14
+    assert np.abs(K.sum(1)).max() < 1e-14
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_counts_and_invariants(orc, k):
+    N = 6
+    P = orc.rectangle(N, N, k)
+    s = P.sizes()
+    assert s[2] == (k * N + 1) ** 2 and s[3] == (k * N - 1) ** 2
+    F = orc.rectangle(N, N, k, all_exterior=False)
+    ip, ix = F.pattern()
+    npt = F.ncells * len(orc.quadrature(2 * k)[2])
+    Mv = F.assemble_matrix(2 * k, [0], [0], np.ones((1, npt)))
+    assert abs(Mv.sum() - 1.0) < 1e-13                              # sum of the mass matrix = area
+    Kv = F.assemble_matrix(2 * k, [1, 2], [1, 2], np.ones((2, npt)))
+    assert np.abs(orc.spmv(ip, ix, Kv, np.ones(F.dim))).max() < 1e-12    # constants in the kernel
+    rows = np.repeat(np.arange(F.dim), np.diff(ip))
+    D = np.zeros((F.dim, F.dim)); D[rows, ix] = Kv
+    assert np.abs(D - D.T).max() < 1e-14
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_patch_test(orc, k):
+    """For u in P_k the Galerkin solution of -lap u + u = f with u's Dirichlet data reproduces u:
+    here u vanishes on the boundary only for the bubble-free case, so the test uses the full space
+    with the interpolant's boundary values lifted: A_ff u_f = b_f - A_fb u_b."""
+    N = 4
+    Pf = orc.rectangle(N, N, k, all_exterior=False)                 # every node free
+    num = Pf.numbering()
+    xy = num["raw_xy"][num["ad2raw"]]
+    u = lambda x, y: 1 + 2 * x - y + (x * y if k >= 2 else 0) + (x ** 2 * y if k >= 3 else 0)
+    f = lambda x, y: u(x, y) - (0 if k < 3 else 2 * y)              # -lap u + u
+    deg = 2 * k + 2
+    npt = Pf.ncells * len(orc.quadrature(deg)[2])
+    x, y, _ = Pf.cache_points(deg)
+    ip, ix = Pf.pattern()
+    A = Pf.assemble_matrix(deg, [1, 2, 0], [1, 2, 0], np.ones((3, npt)))
+    b = Pf.assemble_vector(deg, [0], f(x, y)[None, :])
+    # natural boundary terms: grad u . n on the boundary of the unit square
+    Pd = orc.rectangle(N, N, k)
+    bd = np.setdiff1d(np.arange(Pf.dim), np.searchsorted(num["ad2raw"], Pd.numbering()["ad2raw"]))
+    rows = np.repeat(np.arange(Pf.dim), np.diff(ip))
+    D = np.zeros((Pf.dim, Pf.dim)); D[rows, ix] = A
+    free = np.setdiff1d(np.arange(Pf.dim), bd)
+    ue = u(xy[:, 0], xy[:, 1])
+    rhs = b[free] - D[np.ix_(free, bd)] @ ue[bd]
+    sol = np.linalg.solve(D[np.ix_(free, free)], rhs)
+    assert np.abs(sol - ue[free]).max() < 1e-11
+
+
+def test_ilu0_tridiagonal_is_exact_lu(orc):
+    n = 50
+    rng = np.random.default_rng(0)
+    d = 4 + rng.random(n)
+    ip = np.zeros(n + 1, dtype=np.int32)
+    ix, v = [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                ix.append(j); v.append(d[i] if i == j else -1.0 - 0.1 * rng.random())
+        ip[i + 1] = len(ix)
+    ix, v = np.array(ix, dtype=np.int32), np.array(v)
+    rows = np.repeat(np.arange(n), np.diff(ip))
+    D = np.zeros((n, n)); D[rows, ix] = v
+    r = rng.standard_normal(n)
+    z, _ = orc.precond_apply(ip, ix, v, r, "ilu0", "natural")
+    assert np.abs(z - np.linalg.solve(D, r)).max() < 1e-13
+
+
+def test_rcm_is_a_band_reducing_permutation(orc):
+    ip, ix, A, b = _c0(orc, 10)
+    p = orc.rcm(ip, ix)
+    assert np.array_equal(np.sort(p), np.arange(ip.size - 1))
+    rows = np.repeat(np.arange(ip.size - 1), np.diff(ip))
+    pinv = np.empty_like(p); pinv[p] = np.arange(p.size)
+    assert np.abs(pinv[rows] - pinv[ix]).max() <= np.abs(rows - ix).max()
+
+
+def test_spmv_matches_dense(orc):
+    ip, ix, A, b = _c0(orc, 6)
+    rows = np.repeat(np.arange(ip.size - 1), np.diff(ip))
+    D = np.zeros((b.size, b.size)); D[rows, ix] = A
+    x = np.random.default_rng(2).standard_normal(b.size)
+    assert np.abs(orc.spmv(ip, ix, A, x) - D @ x).max() < 1e-13
+
+
+# ------------------------------------------------------------------------ 3. known answers
+def test_manual_unknown_count(orc):
+    """docs/manual.md:1279: P2 on the unit square with h = 1/150 has 89 401 unknowns."""
+    assert orc.rectangle(150, 150, 2).dim == 89401
+
+
+def test_example13_heat_known_answer(orc):
+    """examples/example13.ipynb cells 3-7: rectangle_mesh(0,0,1,1,20,20), P2, bc="dirichlet":
+    1521 unknowns; backward Euler dt = 1e-3 to T = 0.1 for u_t = lap u with
+    u = exp(-2 pi^2 t) sin(pi x) sin(pi y) gives an L2 error of 1.35e-03."""
+    P = orc.rectangle(20, 20, 2)
+    assert P.dim == 1521
+    ip, ix = P.pattern()
+    x, y, w = P.cache_points(4)
+    M = P.assemble_matrix(4, [0], [0], np.ones((1, x.size)))
+    K = P.assemble_matrix(4, [1, 2], [1, 2], np.ones((2, x.size)))
+    dt = 1e-3
+    A = M + dt * K
+    num = P.numbering()
+    xy = num["raw_xy"][num["ad2raw"]]
+    u = np.sin(np.pi * xy[:, 0]) * np.sin(np.pi * xy[:, 1])
+    for _ in range(100):
+        u, info = orc.cg(ip, ix, A, orc.spmv(ip, ix, M, u), x0=u, M="jacobi", tol=1e-13)
+        assert info["converged"]
+    # L2 error at the quadrature points (P2 interpolation of the numerical solution)
+    deg = 6
+    xq, yq, wq = P.cache_points(deg)
+    nq = len(orc.quadrature(deg)[2])
+    B = orc.basis_eval(2, 0, 0)  # warm the basis (the values below use the raw nodal field)
+    # evaluate u_h at quadrature points via the reference-element tabulation
+    qxi, qeta, _ = orc.quadrature(deg)
+    tab = np.stack([orc.basis_eval(2, a, b)[0] for a, b in zip(qxi, qeta)])       # nq x 6
+    cell = num["cell_dof"]
+    coef = np.where(cell >= 0, u[np.maximum(cell, 0)], 0.0)                        # ncells x 6
+    uh = (coef @ tab.T).reshape(-1)
+    ue = np.exp(-2 * np.pi ** 2 * 0.1) * np.sin(np.pi * xq) * np.sin(np.pi * yq)
+    err = math.sqrt(float((wq * (uh - ue) ** 2).sum()))
+    assert abs(err - 1.35e-3) < 0.01e-3, err

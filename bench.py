@@ -1,0 +1,203 @@
+"""Benchmark of the 2D assemble -> SpMV -> Krylov path on one MI355X.
+
+Workload (the P3 configuration): -div(kappa grad u) = f on the unit square, a seeded
+unstructured Delaunay mesh (h = 0.00265: ~1.48M P3 unknowns, ~25M stored entries),
+kappa = exp(0.5 g) with g a sum of 16 seeded Fourier modes, f the seeded 8-mode sine series,
+homogeneous Dirichlet sides.
+
+One step = assembly of A and b in FP64 on the GPU (coefficient programs evaluated at the 25 / 20
+quadrature points of every cell on the device) + the mixed-precision solve: FP32 Jacobi-PCG inner
+iterations, FP64 residuals and updates (iterative refinement) until ||b - A x|| / ||b|| <= rtol.
+`value` is solves per second (1 / mean step time, inputs resident in HBM).  The error of every
+step's solution is measured against the FP64 solution of the reference algorithm (cg_jacobi,
+bit-identical to the reference on the CPU), computed once outside the timed region.
+
+Also reported: the SpMV kernels' HBM throughput and roofline fraction (algorithmic bytes
+12 nnz + 20 n + 4 in FP64, 8 nnz + 12 n + 4 in FP32, over the kernel's average duration from HIP
+events on the library stream), p50 / p99 step latency, and the CPU baseline: the reference's own
+code (oracle/_ref) on the host cores, on a bounded sample (the full assembly and 100 + 300 of its
+Jacobi-CG iterations, scaled to one solve with the FP64 iteration count).
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--h 0.00265] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "solves/sec at 1e-8 rel err vs FP64; SpMV HBM GB/s; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--h", type=float, default=0.00265)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import femd_amd as fd
+    from femd_amd import _lib as L
+    from femd_amd import problems
+
+    lib = L.load()
+    L.check(lib.femd_init(0))
+
+    V, a_form, L_form, pinfo = problems.p3_kappa_problem(args.h)
+    P, T, min_ang = pinfo["points"], pinfo["triangles"], pinfo["min_angle"]
+    ip, ix = V.pattern()
+    n, nnz = V.dim, int(ix.size)
+    log(f"setup: {P.shape[0]} vertices, {T.shape[0]} cells, min angle {min_ang:.1f} deg, dim {n}, nnz {nnz}, "
+        f"quadrature degrees a {a_form.quad_degree} / L {L_form.quad_degree}, {pinfo['setup_s']:.1f} s")
+
+    # device-resident right-hand side and solution vectors for the timed loop
+    def dev(nbytes):
+        p = C.c_void_p()
+        L.check(lib.femd_dev_alloc(nbytes, C.byref(p)))
+        return p.value
+
+    b_dev, x_dev = dev(8 * n), dev(8 * n)
+    coeffs_L, keep_L = L_form._coefficients()
+    aL = L.i32([t.test.code() for t in L_form.terms])
+    zeros = np.zeros(n)
+
+    def step():
+        A = a_form.assemble(reuse=False)
+        L.check(lib.femd_assemble_vector_dev(V._h, L_form.quad_degree, len(L_form.terms), aL, coeffs_L, b_dev, None))
+        L.check(lib.femd_memcpy_htod(x_dev, zeros.ctypes.data, 8 * n))
+        info = L.SolveInfo()
+        L.check(lib.femd_solve_refined_dev(A._h, b_dev, x_dev, args.rtol, args.inner_tol, 60, C.byref(info), None))
+        L.check(lib.femd_stream_sync(None))
+        return A, info
+
+    # the FP64 solution of the reference algorithm (Jacobi-CG, bit-identical to the CPU reference)
+    t1 = time.perf_counter()
+    A, _ = step()
+    b = L_form.assemble()
+    x64, i64 = fd.cg(A, b, M=A.preconditioner("jacobi"), tol=1e-12, maxiter=200000)
+    t_fp64 = time.perf_counter() - t1
+    log(f"FP64 Jacobi-CG reference solution: {i64.iterations} iterations, residual {i64.residual:.2e}, {t_fp64:.2f} s")
+
+    for _ in range(args.warmup):
+        step()
+    times, errs, inner, outer = [], [], [], []
+    x = np.empty(n)
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        A, info = step()
+        te = time.perf_counter()
+        times.append(te - ts)
+        L.check(lib.femd_memcpy_dtoh(x.ctypes.data, x_dev, 8 * n))
+        errs.append(float(np.linalg.norm(x - x64) / np.linalg.norm(x64)))
+        inner.append(info.iterations)
+        outer.append(info.breakdown)
+        log(f"step {k}: {1e3 * times[-1]:.1f} ms, residual {info.residual:.2e}, rel err vs FP64 {errs[-1]:.2e}, "
+            f"inner {info.iterations} / outer {info.breakdown}")
+    ms = 1e3 * float(np.mean(times))
+    value = 1e3 / ms
+
+    # ---- SpMV roofline: the FP64 and FP32 CSR-stream kernels on this matrix ---------------
+    from femd_amd.roofline import spmv_times
+    t64, t32 = spmv_times(A, args.spmv_reps)
+    variants = {}
+    for prec, key in ((L.FP64, "fp64"), (L.FP32, "fp32")):
+        nm, tm = C.create_string_buffer(64), C.c_double()
+        L.check(lib.femd_spmv_tuning(A._h, prec, nm, 64, C.byref(tm)))
+        variants[key] = nm.value.decode()
+    B64, B32 = 12 * nnz + 20 * n + 4, 8 * nnz + 12 * n + 4
+    gbs64, gbs32 = B64 / (t64 * 1e-3) / 1e9, B32 / (t32 * 1e-3) / 1e9
+    log(f"SpMV FP64 {t64 * 1e3:.1f} us {gbs64:.0f} GB/s; FP32 {t32 * 1e3:.1f} us {gbs32:.0f} GB/s")
+
+    # ---- CPU baseline: the reference's own code, bounded sample ---------------------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(P, T, a_form, L_form, i64.iterations, log)
+
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 assembly + f32 CG + f64 refinement", "data": "synthetic",
+        "config": {"workload": "P3 -div(kappa grad u) = f, seeded Delaunay unit square, FP64 assembly + FP32 "
+                               "Jacobi-CG with FP64 iterative refinement",
+                   "dofs": n, "nnz": nnz, "cells": int(T.shape[0]), "mesh_h": args.h, "min_angle_deg": round(min_ang, 2),
+                   "rtol": args.rtol, "inner_tol": args.inner_tol, "parallelism": "single GPU"},
+        "rel_err_vs_fp64_max": max(errs), "latency_ms_p50": round(1e3 * float(np.percentile(times, 50)), 3),
+        "latency_ms_p99": round(1e3 * float(np.percentile(times, 99)), 3),
+        "inner_iterations": inner, "outer_iterations": outer,
+        "fp64_reference_cg": {"iterations": i64.iterations, "seconds": round(t_fp64, 3)},
+        "spmv": {"fp64_us": round(t64 * 1e3, 2), "fp64_gbs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
+                 "fp64_variant": variants["fp64"], "fp32_us": round(t32 * 1e3, 2), "fp32_gbs": round(gbs32, 1),
+                 "fp32_variant": variants["fp32"]},
+        "roofline": {"kernel": f"FP32 SpMV of the inner CG ({variants['fp32']}, fused p.Ap)", "bound": "hbm", "achieved": round(gbs32, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": B32},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(P, T, a_form, L_form, fp64_iters, log):
+    """The reference (oracle/_ref, the reference's own C++ compiled in place) on the host cores:
+    the full assembly (cache, coefficient programs, element matrices, scatter) and a bounded sample
+    of its Jacobi-CG (cg_jacobi, the algorithm of the FP64 solution above, whose iteration count on
+    this matrix is known from the GPU run, bit-identical to the reference), scaled to one solve."""
+    import oracle
+    R = oracle.reference()
+    kind = "reference"
+    if R is None:
+        R = oracle.oracle()
+        kind = "port"
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    R.set_threads(cores)
+    from femd_amd.forms import Program
+    Pr = R.from_arrays(P, T, k=3)
+    t0 = time.perf_counter()
+    x, y, _ = Pr.cache_points(a_form.quad_degree)
+    ck = R.integrand_eval(*Program(a_form.terms[0].coeff).arrays(), x, y)
+    Av = Pr.assemble_matrix(a_form.quad_degree, [1, 2], [1, 2], np.stack([ck, ck]))
+    xb, yb, _ = Pr.cache_points(L_form.quad_degree)
+    cf = R.integrand_eval(*Program(L_form.terms[0].coeff).arrays(), xb, yb)
+    b = Pr.assemble_vector(L_form.quad_degree, [0], cf[None, :])
+    t_asm = time.perf_counter() - t0
+    ip, ix = Pr.pattern()
+    ts = []
+    for it in (100, 300):
+        t1 = time.perf_counter()
+        R.cg(ip, ix, Av, b, M="jacobi", maxiter=it)
+        ts.append(time.perf_counter() - t1)
+    t_iter = (ts[1] - ts[0]) / 200
+    t_setup = max(ts[0] - 100 * t_iter, 0.0)
+    t_solve = t_asm + t_setup + fp64_iters * t_iter
+    log(f"CPU reference ({cores} threads): assembly {t_asm:.2f} s, Jacobi-CG {1e3 * t_iter:.2f} ms per iteration "
+        f"-> {t_solve:.1f} s per solve of {fp64_iters} iterations")
+    return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": kind,
+            "sample": f"the full assembly ({t_asm:.2f} s) and 100 + 300 Jacobi-CG iterations "
+                      f"({1e3 * t_iter:.2f} ms each, set-up {t_setup:.2f} s), scaled to the {fp64_iters} iterations "
+                      f"the reference's cg_jacobi takes on this matrix (bit-identical count on the GPU)",
+            "seconds_per_solve": round(t_solve, 3)}
+
+
+if __name__ == "__main__":
+    main()

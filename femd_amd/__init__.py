@@ -24,5 +24,6 @@ from .forms import (Const, Constant, Dx, TestFunction, TrialFunction, abs_, as_v
 from .sparse import Preconditioner, SparseMatrix, SparseSolver
 from .krylov import KrylovInfo, cg, gmres, lgmres, solve_refined
 from .linalg import solve
+from .batch import Batch
 
 __version__ = "0.9.0"

@@ -1,6 +1,6 @@
 """ctypes binding of libfemd_hip.so (include/femd_hip.h).
 
-The library is built in-tree by ``python -m femd_amd.build`` (or ``__graft_entry__.build()``);
+The library is built in-tree by ``python femd_amd/build.py`` (or ``__graft_entry__.build()``);
 importing the package without it raises ImportError: there is no CPU fallback.
 Status codes become the exceptions the reference's nanobind layer raises: FEMD_EINVAL ->
 ValueError, FEMD_EZEROPIVOT -> ValueError for a zero diagonal and RuntimeError for a zero
@@ -49,6 +49,7 @@ _SIGS = {
     "femd_memcpy_htod": (_i, [_p, _p, _sz]),
     "femd_memcpy_dtoh": (_i, [_p, _p, _sz]),
     "femd_stream_sync": (_i, [_p]),
+    "femd_vec_copy_dev": (_i, [_i64, _p, _p, _p]),
     "femd_mesh_rectangle": (_i, [_d, _d, _d, _d, _i, _i, _i, _ip, C.POINTER(_p)]),
     "femd_mesh_from_arrays": (_i, [_i, _dp, _i, _ip, _i, _ip, _p, _i, C.POINTER(_p)]),
     "femd_mesh_sizes": (_i, [_p, _ip]),
@@ -60,6 +61,7 @@ _SIGS = {
     "femd_space_free": (None, [_p]),
     "femd_pattern": (_i, [_p, C.POINTER(_i64), _p, _p]),
     "femd_quadrature": (_i, [_i, C.POINTER(_i), _dp, _dp, _dp]),
+    "femd_program_compact": (_i, [C.POINTER(Coefficient), _ip, C.POINTER(_i)]),
     "femd_csr_create_space": (_i, [_p, C.POINTER(_p)]),
     "femd_csr_create": (_i, [_i, _i, _ip, _ip, _dp, C.POINTER(_p)]),
     "femd_csr_info": (_i, [_p, _ip, C.POINTER(_i64)]),
@@ -75,6 +77,7 @@ _SIGS = {
     "femd_eval_coefficient": (_i, [_p, _i, C.POINTER(Coefficient), _dp]),
     "femd_spmv": (_i, [_p, _dp, _dp]),
     "femd_spmv_dev": (_i, [_p, _p, _p, _i, _p]),
+    "femd_spmv_tuning": (_i, [_p, _i, C.c_char_p, _i, C.POINTER(_d)]),
     "femd_ddot": (_i, [_i64, _dp, _dp, C.POINTER(_d)]),
     "femd_ddot_dev": (_i, [_i64, _p, _p, C.POINTER(_d), _p]),
     "femd_precond_create": (_i, [_p, _i, _i, _d, C.POINTER(_p), C.POINTER(_d)]),
@@ -88,6 +91,12 @@ _SIGS = {
     "femd_krylov": (_i, [_p, _p, _dp, _dp, _i, _i, _i, _d, _i, C.POINTER(SolveInfo)]),
     "femd_solve_refined": (_i, [_p, _dp, _dp, _d, _d, _i, C.POINTER(SolveInfo)]),
     "femd_solve_refined_dev": (_i, [_p, _p, _p, _d, _d, _i, C.POINTER(SolveInfo), _p]),
+    "femd_batch_create": (_i, [_i, C.POINTER(_p), C.POINTER(_p)]),
+    "femd_batch_info": (_i, [_p, C.POINTER(_i64), C.POINTER(_i64), _p]),
+    "femd_batch_update_values": (_i, [_p]),
+    "femd_batch_solve_refined": (_i, [_p, _dp, _dp, _d, _d, _i, _i, _p, _p, C.POINTER(SolveInfo)]),
+    "femd_batch_solve_refined_dev": (_i, [_p, _p, _p, _d, _d, _i, _i, _p, _p, C.POINTER(SolveInfo), _p]),
+    "femd_batch_free": (None, [_p]),
     "femd_event_timer_start": (_i, [_p]),
     "femd_event_timer_stop": (_i, [_p, C.POINTER(_d)]),
 }
@@ -101,7 +110,7 @@ def load():
     if _lib is not None:
         return _lib
     if not LIB_PATH.exists():
-        raise ImportError(f"{LIB_PATH} is not built: run `python -m femd_amd.build` (hipcc, gfx950)")
+        raise ImportError(f"{LIB_PATH} is not built: run `python femd_amd/build.py` (hipcc, gfx950)")
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIGS.items():
         f = getattr(lib, name)

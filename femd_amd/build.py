@@ -1,7 +1,9 @@
 """Build libfemd_hip.so (femd_amd/lib/) for gfx950 with hipcc, in-tree.
 
-    python -m femd_amd.build            # compile everything
-    python -m femd_amd.build --clean    # remove the build outputs first
+    python femd_amd/build.py            # compile everything
+    python femd_amd/build.py --clean    # remove the build outputs first
+
+(run as a script: importing the package itself needs the library this builds)
 
 The device code is compiled with -ffp-contract=off (the exact kernels reproduce the reference's
 two-rounding a*b + c) and without packed FP32 instructions.  The host setup (host.cpp) is plain
@@ -32,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", f"-I{CSRC}", f"-I{ROOT / 'include'}"]
 
-SOURCES = [("host.cpp", "cxx"), ("kernels.hip", "hip"), ("capi.hip", "hip")]
+SOURCES = [("host.cpp", "cxx"), ("kernels.hip", "hip"), ("capi.hip", "hip"), ("batch.hip", "hip"), ("spmv.hip", "hip")]
 
 
 def _run(cmd):

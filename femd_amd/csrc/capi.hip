@@ -1,56 +1,15 @@
 // capi.hip -- the C ABI of include/femd_hip.h: handles, the device memory pool, and the
 // orchestration of the kernels in kernels.hip.
-#include "femd_hip.h"
-#include "host.hpp"
-#include "kernels.hpp"
+#include "internal.hpp"
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
 #include <cmath>
-#include <cstring>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <vector>
+#include <limits>
 
 using namespace femd_amd;
 
-namespace {
+namespace femd_amd {
 
 thread_local std::string g_err;
-
-void hip_check(hipError_t e, const char *what)
-{
-    if (e != hipSuccess) throw Error(FEMD_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPC(x) hip_check((x), #x)
-
-template <class F> int api(F &&f)
-{
-    try
-    {
-        g_err.clear();
-        f();
-        return FEMD_OK;
-    }
-    catch (const Error &e) { g_err = e.what(); return e.code; }
-    catch (const std::bad_alloc &) { g_err = "out of host memory"; return FEMD_ENOMEM; }
-    catch (const std::exception &e) { g_err = e.what(); return FEMD_EINVAL; }
-}
-
-[[noreturn]] void fail(int code, const std::string &m) { throw Error(code, m); }
-
-// ------------------------------------------------------------------------- device and pool
-struct Device {
-    bool ready = false;
-    int id = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    std::multimap<size_t, void *> free_blocks;       // the caching pool: size -> blocks
-    size_t cached = 0;
-};
 Device g_dev;
 std::mutex g_init_mu;
 
@@ -65,14 +24,6 @@ void ensure_device()
     HIPC(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
     g_dev.ready = true;
 }
-
-hipStream_t S(femd_stream s)
-{
-    ensure_device();
-    return s ? static_cast<hipStream_t>(s) : g_dev.stream;
-}
-
-size_t round_size(size_t b) { return b <= 256 ? 256 : (b + 4095) & ~size_t(4095); }
 
 void *pool_alloc(size_t bytes)
 {
@@ -93,11 +44,10 @@ void *pool_alloc(size_t bytes)
     hipError_t e = hipMalloc(&p, sz);
     if (e != hipSuccess)
     {
-        // release the cache once and retry
-        {
+        {   // release the cache once and retry
             std::lock_guard<std::mutex> lk(g_dev.mu);
-            hipDeviceSynchronize();
-            for (auto &kv : g_dev.free_blocks) hipFree(kv.second);
+            (void)hipDeviceSynchronize();
+            for (auto &kv : g_dev.free_blocks) (void)hipFree(kv.second);
             g_dev.free_blocks.clear();
             g_dev.cached = 0;
         }
@@ -114,96 +64,6 @@ void pool_free(void *p, size_t bytes)
     g_dev.free_blocks.emplace(round_size(bytes), p);
     g_dev.cached += round_size(bytes);
 }
-
-// Device buffer owned by the pool.  Work queued on the library's streams may still read a buffer
-// when it is released, so a released block is only reused by work queued after that point on the
-// same stream; cross-stream users synchronize before releasing.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    explicit DevBuf(size_t b) : p(b ? pool_alloc(b) : nullptr), bytes(b) {}
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
-    DevBuf &operator=(DevBuf &&o) noexcept
-    {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release() { if (p) pool_free(p, bytes); p = nullptr; bytes = 0; }
-    void ensure(size_t b) { if (bytes < b) { release(); p = pool_alloc(b); bytes = b; } }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-template <class T> DevBuf upload(const T *h, size_t n, hipStream_t s)
-{
-    DevBuf d(n * sizeof(T));
-    if (n) HIPC(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
-    return d;
-}
-template <class T> DevBuf upload(const std::vector<T> &v, hipStream_t s) { return upload(v.data(), v.size(), s); }
-
-// Pinned host scratch for small read-backs.
-struct Pinned {
-    void *p = nullptr;
-    explicit Pinned(size_t b) { HIPC(hipHostMalloc(&p, b, hipHostMallocDefault)); }
-    ~Pinned() { if (p) hipHostFree(p); }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-} // namespace
-
-// ------------------------------------------------------------------------------- handles
-struct DevPattern {
-    int n = 0, m = 0;
-    int64_t nnz = 0;
-    std::vector<int> h_ip, h_ix;
-    DevBuf ip, ix, blocks, diag;
-    int nblocks = 0;
-    bool have_diag = false;
-};
-
-struct femd_mesh_s {
-    MeshData M;
-};
-
-struct Quad {
-    int nq = 0;
-    std::vector<double> xi, eta, w;
-    DevBuf tab, qw, qxi, qeta;
-};
-
-struct femd_space_s {
-    SpaceData V;
-    DevBuf geom;
-    PatternData P;
-    bool have_host_pattern = false, have_pattern = false;
-    std::shared_ptr<DevPattern> pat;
-    DevBuf mptr, mlist, vptr, vlist;
-    std::map<int, std::unique_ptr<Quad>> quads;
-    std::unique_ptr<RefElement> ref;
-};
-
-struct femd_csr_s {
-    std::shared_ptr<DevPattern> pat;
-    DevBuf v, v32;
-    bool v32_valid = false;
-    const femd_space_s *space = nullptr;
-};
-
-struct femd_precond_s {
-    int kind = 0, ordering = 0, n = 0;
-    double omega = 1.0, min_pivot = 0.0;
-    DevBuf inv;                                   // Jacobi
-    DevBuf perm, ip, ix, diag, lu;                // ILU(0) / SSOR on P A P^T
-    DevBuf lptrL, lrowsL, lptrU, lrowsU;
-    int nlevL = 0, nlevU = 0;
-    DevBuf rp, zp;                                // permuted work vectors
-};
-
-namespace {
 
 std::shared_ptr<DevPattern> make_pattern(int n, int m, std::vector<int> ip, std::vector<int> ix, hipStream_t s)
 {
@@ -227,6 +87,17 @@ const int *diag_of(DevPattern &P, hipStream_t s)
         P.have_diag = true;
     }
     return P.diag.as<int>();
+}
+
+const float *values32(femd_csr_s *A, hipStream_t s)
+{
+    if (!A->v32_valid)
+    {
+        A->v32.ensure(sizeof(float) * std::max<int64_t>(A->pat->nnz, 1));
+        launch_to_float(A->pat->nnz, A->v.as<double>(), A->v32.as<float>(), s);
+        A->v32_valid = true;
+    }
+    return A->v32.as<float>();
 }
 
 Quad &quad_of(femd_space_s *V, int degree, hipStream_t s)
@@ -283,7 +154,7 @@ struct Coeffs {
 void check_program(const femd_coefficient &c)
 {
     if (c.ninstr < 1) fail(FEMD_EINVAL, "coefficient: empty program");
-    if (c.nreg < 1 || c.nreg > MAX_REGS) fail(FEMD_EINVAL, "coefficient: 1.." + std::to_string(MAX_REGS) + " registers supported");
+    if (c.nreg < 1) fail(FEMD_EINVAL, "coefficient: at least one register");
     for (int k = 0; k < c.ninstr; ++k)
     {
         const int op = c.code[4 * k], o = c.code[4 * k + 1], a = c.code[4 * k + 2], b = c.code[4 * k + 3];
@@ -298,13 +169,66 @@ void check_program(const femd_coefficient &c)
     }
 }
 
+// Register renaming by liveness: every instruction's result gets a register that is free from
+// its definition to its last use, so a program of any size runs in a few registers.  The
+// operations and their order are unchanged.
+std::vector<int> compact_registers(const femd_coefficient &c, int &nreg_out)
+{
+    const int n = c.ninstr;
+    std::vector<int> code(c.code, c.code + 4 * static_cast<size_t>(n));
+    auto reads = [&](int k, int slot) {            // slot 0: a, 1: b; is that operand a register?
+        const int op = code[4 * k];
+        if (slot == 0) return op >= 6 && op <= 19;
+        return op == 6 || op == 7;
+    };
+    std::vector<int> writer(static_cast<size_t>(c.nreg), -1);     // last instruction that wrote a register
+    std::vector<int> src(2 * static_cast<size_t>(n), -1);          // the instruction whose value an operand reads
+    std::vector<int> last(static_cast<size_t>(n), -1);             // last reader of each instruction's value
+    for (int k = 0; k < n; ++k)
+    {
+        for (int slot = 0; slot < 2; ++slot)
+            if (reads(k, slot))
+            {
+                const int w = writer[code[4 * k + 2 + slot]];
+                if (w < 0) fail(FEMD_EINVAL, "coefficient: a register is read before it is written");
+                src[2 * k + slot] = w;
+                last[w] = k;
+            }
+        writer[code[4 * k + 1]] = k;
+    }
+    last[n - 1] = n;                                               // the result lives to the end
+    std::vector<int> reg_of(static_cast<size_t>(n), -1), free_regs;
+    int nreg = 0;
+    for (int k = 0; k < n; ++k)
+    {
+        for (int slot = 0; slot < 2; ++slot)
+            if (src[2 * k + slot] >= 0) code[4 * k + 2 + slot] = reg_of[src[2 * k + slot]];
+        for (int slot = 0; slot < 2; ++slot)
+        {
+            const int w = src[2 * k + slot];
+            if (w >= 0 && last[w] == k && !(slot == 1 && src[2 * k] == w)) free_regs.push_back(reg_of[w]);
+        }
+        int r;
+        if (!free_regs.empty()) { r = free_regs.back(); free_regs.pop_back(); }
+        else r = nreg++;
+        reg_of[k] = r;
+        code[4 * k + 1] = r;
+        if (last[k] < 0) free_regs.push_back(r);                   // a value nobody reads
+    }
+    nreg_out = nreg;
+    return code;
+}
+
 void eval_program(femd_space_s *V, Quad &Q, const femd_coefficient &c, double *out, hipStream_t s)
 {
     check_program(c);
-    DevBuf code = upload(c.code, 4 * static_cast<size_t>(c.ninstr), s);
+    int nreg = 0;
+    std::vector<int> compact = compact_registers(c, nreg);
+    if (nreg > MAX_REGS) fail(FEMD_EINVAL, "coefficient: the program needs more than " + std::to_string(MAX_REGS) + " live registers");
+    DevBuf code = upload(compact, s);
     DevBuf consts = upload(c.consts, static_cast<size_t>(std::max(c.nconst, 0)), s);
     launch_coefficient(V->V.ncells, Q.nq, V->geom.as<double>(), Q.qxi.as<double>(), Q.qeta.as<double>(), c.ninstr,
-                       code.as<int>(), c.nreg, consts.as<double>(), out, s);
+                       code.as<int>(), nreg, consts.as<double>(), out, s);
     HIPC(hipStreamSynchronize(s));           // code and consts are released on return
 }
 
@@ -351,27 +275,11 @@ femd_space_s *check_space(femd_space V)
     if (!V) fail(FEMD_EINVAL, "null space");
     return V;
 }
-femd_csr_s *check_csr(femd_csr A)
-{
-    if (!A) fail(FEMD_EINVAL, "null matrix");
-    return A;
-}
-
-const float *values32(femd_csr_s *A, hipStream_t s)
-{
-    if (!A->v32_valid)
-    {
-        A->v32.ensure(sizeof(float) * std::max<int64_t>(A->pat->nnz, 1));
-        launch_to_float(A->pat->nnz, A->v.as<double>(), A->v32.as<float>(), s);
-        A->v32_valid = true;
-    }
-    return A->v32.as<float>();
-}
-
+// y = A x in FP64 with the reference's row order (the tuned CSR-stream variant).
 void spmv64(femd_csr_s *A, const double *x, double *y, const int *stop, hipStream_t s)
 {
     DevPattern &P = *A->pat;
-    launch_spmv64(P.nblocks, P.blocks.as<int>(), P.ip.as<int>(), P.ix.as<int>(), A->v.as<double>(), x, y, stop, s);
+    spmv_run64(spmv_plan(P, FEMD_FP64, true, A->v.p, s), P, A->v.as<double>(), x, y, nullptr, stop, s);
 }
 
 // The ordered dot product into a device scalar.
@@ -379,14 +287,6 @@ void ddot_dev(int64_t n, const double *x, const double *y, double *parts, double
 {
     launch_dot_parts(n, x, y, parts, nullptr, s);
     launch_sum_parts(nchunks(n), parts, out, s);
-}
-
-double read_scalar(const double *d, hipStream_t s)
-{
-    double h = 0.0;
-    HIPC(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPC(hipStreamSynchronize(s));
-    return h;
 }
 
 // z = M^{-1} r on the device (any kind), stop may be null.
@@ -740,6 +640,8 @@ void refined_device(femd_csr_s *A, const double *b, double *x, double tol, doubl
     const double nb = std::sqrt(read_scalar(scal.as<double>(), s));
     if (nb == 0.0) { launch_fill(n, 0.0, x, s); HIPC(hipStreamSynchronize(s)); info->converged = 1; return; }
     const int nbk = fast_blocks(n);
+    const SpmvPlan &plan = spmv_plan(P, FEMD_FP32, false, v32, s);
+    DevBuf pparts(8 * size_t(std::max(plan.nblocks, 1)));
     CG32State *st = stb.as<CG32State>();
     int inner_total = 0, outer = 0;
     double res = 0.0;
@@ -771,9 +673,8 @@ void refined_device(femd_csr_s *A, const double *b, double *x, double tol, doubl
         {
             for (int k = 0; k < batch; ++k)
             {
-                launch_spmv32(P.nblocks, P.blocks.as<int>(), P.ip.as<int>(), P.ix.as<int>(), v32, p32.as<float>(), Ap32.as<float>(), stop, s);
-                launch_fdot32(n, p32.as<float>(), Ap32.as<float>(), parts.as<double>(), stop, s);
-                launch_cg32_alpha(parts.as<double>(), nbk, st, s);
+                spmv_run32(plan, P, v32, p32.as<float>(), Ap32.as<float>(), pparts.as<double>(), stop, s);
+                launch_cg32_alpha(pparts.as<double>(), plan.nblocks, st, s);
                 launch_cg32_update(n, x32.as<float>(), r32.as<float>(), p32.as<float>(), Ap32.as<float>(), inv32.as<float>(),
                                    parts.as<double>(), st, s);
                 launch_cg32_beta(parts.as<double>(), nbk, st, s);
@@ -870,7 +771,7 @@ void build_precond(femd_precond_s *M, femd_csr_s *A, hipStream_t s)
 
 thread_local hipEvent_t t_ev0 = nullptr, t_ev1 = nullptr;
 
-} // namespace
+} // namespace femd_amd
 
 // ================================================================================== C ABI
 extern "C" {
@@ -911,6 +812,10 @@ int femd_memcpy_dtoh(void *dst, const void *src, size_t bytes)
     return api([&] { ensure_device(); HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
 }
 int femd_stream_sync(femd_stream s) { return api([&] { HIPC(hipStreamSynchronize(S(s))); }); }
+int femd_vec_copy_dev(int64_t n, const double *src, double *dst, femd_stream s)
+{
+    return api([&] { launch_copy(n, src, dst, S(s)); });
+}
 
 // ---- meshes ------------------------------------------------------------------------------
 int femd_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int diag, const int markers[4], femd_mesh *out)
@@ -996,7 +901,7 @@ int femd_space_numbering(femd_space V, int *cell_dof, int *cell_raw, int *ad2raw
 
 void femd_space_free(femd_space V)
 {
-    if (V && g_dev.ready) hipStreamSynchronize(g_dev.stream);
+    if (V && g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
     delete V;
 }
 
@@ -1008,6 +913,15 @@ int femd_pattern(femd_space V, int64_t *nnz, int *indptr, int *indices)
         if (nnz) *nnz = Vs->P.nnz();
         if (indptr) std::copy(Vs->P.indptr.begin(), Vs->P.indptr.end(), indptr);
         if (indices) std::copy(Vs->P.indices.begin(), Vs->P.indices.end(), indices);
+    });
+}
+
+int femd_program_compact(const femd_coefficient *c, int *code_out, int *nreg_out)
+{
+    return api([&] {
+        check_program(*c);
+        std::vector<int> code = compact_registers(*c, *nreg_out);
+        std::copy(code.begin(), code.end(), code_out);
     });
 }
 
@@ -1144,7 +1058,7 @@ int femd_csr_diagonal(femd_csr A, double *out)
 
 void femd_csr_free(femd_csr A)
 {
-    if (A && g_dev.ready) hipStreamSynchronize(g_dev.stream);
+    if (A && g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
     delete A;
 }
 
@@ -1223,8 +1137,11 @@ int femd_spmv_dev(femd_csr A, const void *x, void *y, int precision, femd_stream
         hipStream_t s = S(st);
         DevPattern &P = *a->pat;
         if (precision == FEMD_FP32)
-            launch_spmv32(P.nblocks, P.blocks.as<int>(), P.ip.as<int>(), P.ix.as<int>(), values32(a, s),
-                          static_cast<const float *>(x), static_cast<float *>(y), nullptr, s);
+        {
+            const float *v32 = values32(a, s);
+            spmv_run32(spmv_plan(P, FEMD_FP32, false, v32, s), P, v32, static_cast<const float *>(x), static_cast<float *>(y),
+                       nullptr, nullptr, s);
+        }
         else spmv64(a, static_cast<const double *>(x), static_cast<double *>(y), nullptr, s);
     });
 }
@@ -1319,7 +1236,7 @@ int femd_rcm(int n, const int *indptr, const int *indices, int *perm)
 
 void femd_precond_free(femd_precond M)
 {
-    if (M && g_dev.ready) hipStreamSynchronize(g_dev.stream);
+    if (M && g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
     delete M;
 }
 

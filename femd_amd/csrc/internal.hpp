@@ -1,0 +1,193 @@
+// internal.hpp -- the runtime shared by the C ABI translation units (capi.hip, batch.hip): error
+// handling, the device and its caching memory pool, device buffers, and the handle types.
+#pragma once
+#include "femd_hip.h"
+#include "host.hpp"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace femd_amd {
+
+extern thread_local std::string g_err;
+
+inline void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw Error(FEMD_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) ::femd_amd::hip_check((x), #x)
+
+[[noreturn]] inline void fail(int code, const std::string &m) { throw Error(code, m); }
+
+// Every entry point runs its body through api(): exceptions become status codes and the message.
+template <class F> inline int api(F &&f)
+{
+    try
+    {
+        g_err.clear();
+        f();
+        return FEMD_OK;
+    }
+    catch (const Error &e) { g_err = e.what(); return e.code; }
+    catch (const std::bad_alloc &) { g_err = "out of host memory"; return FEMD_ENOMEM; }
+    catch (const std::exception &e) { g_err = e.what(); return FEMD_EINVAL; }
+}
+
+// ---------------------------------------------------------------------------- device and pool
+struct Device {
+    bool ready = false;
+    int id = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::multimap<size_t, void *> free_blocks;       // the caching pool: size -> blocks
+    size_t cached = 0;
+};
+extern Device g_dev;
+extern std::mutex g_init_mu;
+
+void ensure_device();
+inline size_t round_size(size_t b) { return b <= 256 ? 256 : (b + 4095) & ~size_t(4095); }
+void *pool_alloc(size_t bytes);
+void pool_free(void *p, size_t bytes);
+
+inline hipStream_t S(femd_stream s)
+{
+    ensure_device();
+    return s ? static_cast<hipStream_t>(s) : g_dev.stream;
+}
+
+// Device buffer owned by the pool.  A released block is reused only by work queued after the
+// release on the same stream; users of other streams synchronize before releasing.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) : p(b ? pool_alloc(b) : nullptr), bytes(b) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() { if (p) pool_free(p, bytes); p = nullptr; bytes = 0; }
+    void ensure(size_t b) { if (bytes < b) { release(); p = pool_alloc(b); bytes = b; } }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <class T> inline DevBuf upload(const T *h, size_t n, hipStream_t s)
+{
+    DevBuf d(n * sizeof(T));
+    if (n) HIPC(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+template <class T> inline DevBuf upload(const std::vector<T> &v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+
+// Pinned host scratch for small read-backs.
+struct Pinned {
+    void *p = nullptr;
+    explicit Pinned(size_t b) { HIPC(hipHostMalloc(&p, b, hipHostMallocDefault)); }
+    ~Pinned() { if (p) (void)hipHostFree(p); }
+    Pinned(const Pinned &) = delete;
+    Pinned &operator=(const Pinned &) = delete;
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+inline double read_scalar(const double *d, hipStream_t s)
+{
+    double h = 0.0;
+    HIPC(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return h;
+}
+
+// One SpMV variant ready to launch on one pattern (spmv.hip): its block table (stream variants)
+// or grid size (sub-wavefront variants), and the time the autotuner measured for it.
+struct SpmvPlan {
+    int variant = 0;
+    int nblocks = 0;
+    DevBuf blocks;
+    double ms = 0.0;
+};
+
+struct DevPattern {
+    int n = 0, m = 0;
+    int64_t nnz = 0;
+    std::vector<int> h_ip, h_ix;
+    DevBuf ip, ix, blocks, diag;
+    int nblocks = 0;
+    bool have_diag = false;
+    std::unique_ptr<SpmvPlan> plan64, plan32, plan32x;    // tuned: FP64 exact, FP32 fast, FP32 exact
+};
+
+// The tuned SpMV of a pattern (values give the tuning runs realistic data); exact restricts the
+// choice to the reference-order (CSR-stream) variants.
+const SpmvPlan &spmv_plan(DevPattern &P, int precision, bool exact, const void *values, hipStream_t s);
+void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, double *parts,
+                const int *stop, hipStream_t s);
+void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, double *parts,
+                const int *stop, hipStream_t s);
+const char *spmv_variant_name(const SpmvPlan &p);
+
+std::shared_ptr<DevPattern> make_pattern(int n, int m, std::vector<int> ip, std::vector<int> ix, hipStream_t s);
+const int *diag_of(DevPattern &P, hipStream_t s);
+
+struct Quad {
+    int nq = 0;
+    std::vector<double> xi, eta, w;
+    DevBuf tab, qw, qxi, qeta;
+};
+
+} // namespace femd_amd
+
+// ------------------------------------------------------------------------------ the handles
+struct femd_mesh_s {
+    femd_amd::MeshData M;
+};
+
+struct femd_space_s {
+    femd_amd::SpaceData V;
+    femd_amd::DevBuf geom;
+    femd_amd::PatternData P;
+    bool have_host_pattern = false, have_pattern = false;
+    std::shared_ptr<femd_amd::DevPattern> pat;
+    femd_amd::DevBuf mptr, mlist, vptr, vlist;
+    std::map<int, std::unique_ptr<femd_amd::Quad>> quads;
+    std::unique_ptr<femd_amd::RefElement> ref;
+};
+
+struct femd_csr_s {
+    std::shared_ptr<femd_amd::DevPattern> pat;
+    femd_amd::DevBuf v, v32;
+    bool v32_valid = false;
+    const femd_space_s *space = nullptr;
+};
+
+struct femd_precond_s {
+    int kind = 0, ordering = 0, n = 0;
+    double omega = 1.0, min_pivot = 0.0;
+    femd_amd::DevBuf inv;                               // Jacobi
+    femd_amd::DevBuf perm, ip, ix, diag, lu;            // ILU(0) / SSOR on P A P^T
+    femd_amd::DevBuf lptrL, lrowsL, lptrU, lrowsU;
+    int nlevL = 0, nlevU = 0;
+    femd_amd::DevBuf rp, zp;                            // permuted work vectors
+};
+
+namespace femd_amd {
+inline femd_csr_s *check_csr(femd_csr A)
+{
+    if (!A) fail(FEMD_EINVAL, "null matrix");
+    return A;
+}
+const float *values32(femd_csr_s *A, hipStream_t s);
+} // namespace femd_amd

@@ -538,10 +538,12 @@ class Form:
             items.append(((code, nreg, consts), None))
         return L.coefficient_array(items)
 
-    def assemble(self):
+    def assemble(self, reuse: bool = True):
         """Rank 2: a SparseMatrix on the space's pattern (symmetric flag set as forms.py:2093-2094);
-        rank 1: the vector (numpy, float64).  Assembled on the GPU."""
-        if self.constant and self._result is not None:
+        rank 1: the vector (numpy, float64).  Assembled on the GPU.  A form without Constants is
+        assembled once and the result reused, as the reference does (forms.py:2752-2760), unless
+        reuse=False."""
+        if reuse and self.constant and self._result is not None:
             return self._result
         V = self.test_space
         coeffs, keep = self._coefficients()

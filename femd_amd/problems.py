@@ -110,3 +110,21 @@ def poisson_forms(V, f_expr, kappa=None, reaction=0.0):
     if reaction:
         a_expr = a_expr + (reaction * u * v if reaction != 1.0 else u * v)
     return F.form(a_expr * F.dx), F.form(f_expr * v * F.dx)
+
+
+C1_H = 0.00265
+
+
+def p3_kappa_problem(h: float = C1_H, seed: int = 0):
+    """The P3 configuration: seeded Delaunay mesh of the unit square, P3 with every side Dirichlet,
+    a = kappa grad u . grad v dx, L = f v dx.  Returns (space, a, L, info)."""
+    import time
+    from .mesh import mesh_from_arrays
+    from .spaces import LagrangeSpace
+    t0 = time.perf_counter()
+    P, T, ma = delaunay_square(h, seed=seed)
+    V = LagrangeSpace(mesh_from_arrays(P, T), 3, bc="dirichlet")
+    kap, _ = kappa_field(1)
+    f, _ = fourier_source(0)
+    a, L = poisson_forms(V, f, kappa=kap)
+    return V, a, L, dict(points=P, triangles=T, min_angle=ma, setup_s=time.perf_counter() - t0)

@@ -65,6 +65,9 @@ int femd_dev_free(void *p);
 int femd_memcpy_htod(void *dst, const void *src, size_t bytes);
 int femd_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int femd_stream_sync(femd_stream s);
+/* dst = src for n doubles on the device, by a kernel (no counterpart in the reference; the copy
+ * whose HBM counters calibrate the SpMV traffic measurement). */
+int femd_vec_copy_dev(int64_t n, const double *src, double *dst, femd_stream s);
 
 /* ---------------------------------------------------------------- meshes
  * femd_mesh_rectangle replaces rectangle_mesh (mesh/structured.hpp:338-351, bound at
@@ -116,6 +119,11 @@ typedef struct {
     const double *values;
 } femd_coefficient;
 
+/* The program with its registers renamed by liveness, as the GPU runs it (host only; no
+ * counterpart in the reference, whose Integrand keeps one register per node): code_out has
+ * 4 * ninstr ints; nreg_out receives the registers used. */
+int femd_program_compact(const femd_coefficient *c, int *code_out, int *nreg_out);
+
 /* Quadrature of a degree: nq points (the rule of quadrature/triangle.hpp:22-57). */
 int femd_quadrature(int degree, int *nq, double *xi, double *eta, double *w);
 
@@ -161,6 +169,10 @@ int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient 
  * precision FEMD_FP32 uses the float shadow of the values (made on first use). */
 int femd_spmv(femd_csr A, const double *x, double *y);
 int femd_spmv_dev(femd_csr A, const void *x_dev, void *y_dev, int precision, femd_stream s);
+/* The SpMV variant the autotuner picked for A in a precision (FP64: among the reference-order
+ * CSR-stream shapes; FP32: also the sub-wavefront kernels), tuning on first use; name receives its
+ * name (name_len bytes), ms its measured time.  No counterpart in the reference. */
+int femd_spmv_tuning(femd_csr A, int precision, char *name, int name_len, double *ms);
 /* The dot product of util/omp.hpp:176-192 (fd.ddot), with its summation order. */
 int femd_ddot(int64_t n, const double *x, const double *y, double *out);
 int femd_ddot_dev(int64_t n, const double *x_dev, const double *y_dev, double *out, femd_stream s);
@@ -213,6 +225,28 @@ int femd_solve_refined(femd_csr A, const double *b, double *x, double tol, doubl
                        femd_solve_info *info);
 int femd_solve_refined_dev(femd_csr A, const double *b_dev, double *x_dev, double tol, double inner_tol,
                            int max_outer, femd_solve_info *info, femd_stream s);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent square systems solved together (the batching pattern; no reference
+ * counterpart: the reference solves one system per fd.cg call).  The systems are concatenated
+ * into one block-diagonal CSR; each keeps its own CG scalars and convergence flag on the device,
+ * so uneven sizes share one launch sequence, replayed from a captured hipGraph when use_graph.
+ * femd_batch_create copies the members' patterns and values (the members must stay alive for
+ * femd_batch_update_values, which re-copies their values after a re-assembly).
+ * Vectors are laid out system after system (row_offsets, np + 1 entries).  The solve is FP32
+ * Jacobi-PCG with FP64 refinement per system until ||b_p - A_p x_p|| / ||b_p|| <= tol;
+ * iterations[p] (inner FP32 iterations) and residuals[p] may be NULL; info.iterations sums the
+ * inner iterations, info.breakdown counts refinement rounds, info.residual is the worst system's. */
+typedef struct femd_batch_s *femd_batch;
+int femd_batch_create(int nsystems, const femd_csr *matrices, femd_batch *out);
+int femd_batch_info(femd_batch B, int64_t *nrows, int64_t *nnz, int64_t *row_offsets);
+int femd_batch_update_values(femd_batch B);
+int femd_batch_solve_refined(femd_batch B, const double *b, double *x, double tol, double inner_tol, int max_outer,
+                             int use_graph, int *iterations, double *residuals, femd_solve_info *info);
+int femd_batch_solve_refined_dev(femd_batch B, const double *b_dev, double *x_dev, double tol, double inner_tol,
+                                 int max_outer, int use_graph, int *iterations, double *residuals,
+                                 femd_solve_info *info, femd_stream s);
+void femd_batch_free(femd_batch B);
 
 /* ---------------------------------------------------------------- timing
  * Kernel-time of the library's work on a stream, measured with HIP events around it. */

@@ -132,3 +132,42 @@ def test_program_matches_reference_instruction_order():
     code, nreg, consts = Program(e).arrays()
     assert code.tolist() == [[0, 0, 0, 0], [0, 1, 1, 0], [1, 2, 0, 0], [7, 3, 1, 2], [9, 4, 3, 0], [7, 5, 0, 4]]
     assert nreg == 6 and consts.tolist() == [2.0, 3.0]
+
+
+def _interpret(code, consts, x, y):
+    """A numpy interpreter of the instruction set (forms/integrand.hpp:300-342)."""
+    reg = {}
+    f = {9: np.sin, 10: np.cos, 11: np.exp, 12: np.log, 13: np.tanh, 14: np.sqrt, 15: np.sinh, 16: np.cosh,
+         18: np.abs, 19: np.sign}
+    res = 0
+    for op, o, a, b in code:
+        if op == 0: v = np.full_like(x, consts[a])
+        elif op == 1: v = x.copy()
+        elif op == 2: v = y.copy()
+        elif op == 6: v = reg[a] + reg[b]
+        elif op == 7: v = reg[a] * reg[b]
+        elif op == 8: v = reg[a] ** consts[b]
+        elif op == 17: t = np.exp(-np.abs(reg[a])); v = 2 * t / (1 + t * t)
+        else: v = f[op](reg[a])
+        reg[o] = v
+        res = o
+    return reg[res]
+
+
+def test_register_compaction_keeps_the_arithmetic():
+    from femd_amd.forms import Program
+    rng = np.random.default_rng(0)
+    x, y = rng.random(200), rng.random(200)
+    f, _ = problems.fourier_source(0)
+    kap, _ = problems.kappa_field(1)
+    z = fd.x * fd.x + fd.sin(fd.x) * fd.sin(fd.x)
+    for e in (f, kap, z, fd.Const(3.0)):
+        code, nreg, consts = Program(e).arrays()
+        c, keep = _lib.coefficient(prog=(code, nreg, consts))
+        out = np.empty(code.size, dtype=np.int32)
+        nr = ctypes.c_int()
+        _lib.check(_lib.load().femd_program_compact(ctypes.byref(c), out, ctypes.byref(nr)))
+        comp = out.reshape(-1, 4)
+        assert nr.value <= min(nreg, 16)
+        assert np.array_equal(_interpret(comp, consts, x, y), _interpret(code, consts, x, y))
+        assert np.array_equal(comp[:, 0], code[:, 0])                      # same operations, same order

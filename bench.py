@@ -12,9 +12,11 @@ iterations, FP64 residuals and updates (iterative refinement) until ||b - A x|| 
 step's solution is measured against the FP64 solution of the reference algorithm (cg_jacobi,
 bit-identical to the reference on the CPU), computed once outside the timed region.
 
-Also reported: the SpMV kernels' HBM throughput and roofline fraction (algorithmic bytes
-12 nnz + 20 n + 4 in FP64, 8 nnz + 12 n + 4 in FP32, over the kernel's average duration from HIP
-events on the library stream), p50 / p99 step latency, and the CPU baseline: the reference's own
+Also reported: the roofline of the dominant kernel, the FP32 inner CG's product (its
+algorithmic bytes 8 nnz + 12 n + 4 over its average duration, sampled with HIP events on its stream
+during the timed steps); the plain SpMV kernels' HBM throughput (12 nnz + 20 n + 4 bytes in FP64,
+8 nnz + 12 n + 4 in FP32), back to back and with the Infinity Cache flushed; the plain FP64 GPU path
+(assembly + Jacobi-CG) timed once; p50 / p99 step latency; and the CPU baseline: the reference's own
 code (oracle/_ref) on the host cores, on a bounded sample (the full assembly and 100 + 300 of its
 Jacobi-CG iterations, scaled to one solve with the FP64 iteration count).
 
@@ -93,17 +95,24 @@ def main():
         return A, info
 
     # the FP64 solution of the reference algorithm (Jacobi-CG, bit-identical to the CPU reference)
-    t1 = time.perf_counter()
     A, _ = step()
     b = L_form.assemble()
     x64, i64 = fd.cg(A, b, M=A.preconditioner("jacobi"), tol=1e-12, maxiter=200000)
+    log(f"FP64 Jacobi-CG reference solution: {i64.iterations} iterations, residual {i64.residual:.2e}")
+    # the plain FP64 GPU path at the reference's default tolerance: assembly + Jacobi-CG, timed once
+    t1 = time.perf_counter()
+    A64 = a_form.assemble(reuse=False)
+    b64 = L_form.assemble()
+    _, ip64 = fd.cg(A64, b64, M=A64.preconditioner("jacobi"), tol=args.rtol, maxiter=200000)
     t_fp64 = time.perf_counter() - t1
-    log(f"FP64 Jacobi-CG reference solution: {i64.iterations} iterations, residual {i64.residual:.2e}, {t_fp64:.2f} s")
+    del A64
+    log(f"plain FP64 GPU path (assembly + Jacobi-CG to {args.rtol:g}): {ip64.iterations} iterations, {t_fp64:.2f} s")
 
     for _ in range(args.warmup):
         step()
     times, errs, inner, outer = [], [], [], []
     x = np.empty(n)
+    L.check(lib.femd_inner_timer(1))
     for k in range(args.steps):
         ts = time.perf_counter()
         A, info = step()
@@ -115,20 +124,35 @@ def main():
         outer.append(info.breakdown)
         log(f"step {k}: {1e3 * times[-1]:.1f} ms, residual {info.residual:.2e}, rel err vs FP64 {errs[-1]:.2e}, "
             f"inner {info.iterations} / outer {info.breakdown}")
+    launches, tot = C.c_int64(), C.c_double()
+    L.check(lib.femd_inner_timer_read(C.byref(launches), C.byref(tot)))
+    L.check(lib.femd_inner_timer(0))
     ms = 1e3 * float(np.mean(times))
     value = 1e3 / ms
 
-    # ---- SpMV roofline: the FP64 and FP32 CSR-stream kernels on this matrix ---------------
+    # ---- roofline of the dominant kernel: the product of the FP32 inner CG -------------------
+    # per launch it reads values + columns (8 nnz), row pointers (4 (n + 1)), r once (4 n) and
+    # writes w = A r (4 n): 8 nnz + 12 n + 4 bytes, the FP32 CSR SpMV's count (its two dot
+    # products reuse r and w in registers)
+    B_fused = 8 * nnz + 12 * n + 4
+    t_fused = tot.value / max(launches.value, 1)
+    gbs_fused = B_fused / (t_fused * 1e-3) / 1e9
+    log(f"FP32 CG product: {launches.value} timed launches, {1e3 * t_fused:.1f} us average, {gbs_fused:.0f} GB/s")
+
+    # ---- plain SpMV on this matrix: back to back, and with the Infinity Cache flushed --------
     from femd_amd.roofline import spmv_times
     t64, t32 = spmv_times(A, args.spmv_reps)
+    c64, c32 = spmv_times(A, max(5, args.spmv_reps // 5), flush=True)
     variants = {}
     for prec, key in ((L.FP64, "fp64"), (L.FP32, "fp32")):
         nm, tm = C.create_string_buffer(64), C.c_double()
         L.check(lib.femd_spmv_tuning(A._h, prec, nm, 64, C.byref(tm)))
         variants[key] = nm.value.decode()
     B64, B32 = 12 * nnz + 20 * n + 4, 8 * nnz + 12 * n + 4
-    gbs64, gbs32 = B64 / (t64 * 1e-3) / 1e9, B32 / (t32 * 1e-3) / 1e9
-    log(f"SpMV FP64 {t64 * 1e3:.1f} us {gbs64:.0f} GB/s; FP32 {t32 * 1e3:.1f} us {gbs32:.0f} GB/s")
+    gbs = lambda B, t: B / (t * 1e-3) / 1e9
+    log(f"SpMV FP64 {t64 * 1e3:.1f} us {gbs(B64, t64):.0f} GB/s (flushed {c64 * 1e3:.1f} us {gbs(B64, c64):.0f} GB/s); "
+        f"FP32 {t32 * 1e3:.1f} us {gbs(B32, t32):.0f} GB/s (flushed {c32 * 1e3:.1f} us {gbs(B32, c32):.0f} GB/s)")
+    traffic = measured_traffic("k_scg_spmv")
 
     # ---- CPU baseline: the reference's own code, bounded sample ---------------------------
     cpu = None
@@ -146,16 +170,40 @@ def main():
         "rel_err_vs_fp64_max": max(errs), "latency_ms_p50": round(1e3 * float(np.percentile(times, 50)), 3),
         "latency_ms_p99": round(1e3 * float(np.percentile(times, 99)), 3),
         "inner_iterations": inner, "outer_iterations": outer,
-        "fp64_reference_cg": {"iterations": i64.iterations, "seconds": round(t_fp64, 3)},
-        "spmv": {"fp64_us": round(t64 * 1e3, 2), "fp64_gbs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
-                 "fp64_variant": variants["fp64"], "fp32_us": round(t32 * 1e3, 2), "fp32_gbs": round(gbs32, 1),
-                 "fp32_variant": variants["fp32"]},
-        "roofline": {"kernel": f"FP32 SpMV of the inner CG ({variants['fp32']}, fused p.Ap)", "bound": "hbm", "achieved": round(gbs32, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes": B32},
+        "fp64_reference_cg": {"iterations": i64.iterations, "residual": i64.residual},
+        "plain_fp64_gpu": {"iterations": ip64.iterations, "seconds": round(t_fp64, 3),
+                           "solves_per_s": round(1.0 / t_fp64, 4), "speedup_of_value": round(value * t_fp64, 2)},
+        "spmv": {"fp64": {"us": round(t64 * 1e3, 2), "gbs": round(gbs(B64, t64), 1), "shape": variants["fp64"],
+                          "flushed_us": round(c64 * 1e3, 2), "flushed_gbs": round(gbs(B64, c64), 1),
+                          "flushed_frac": round(gbs(B64, c64) / HBM_PEAK_GBS, 4)},
+                 "fp32": {"us": round(t32 * 1e3, 2), "gbs": round(gbs(B32, t32), 1), "shape": variants["fp32"],
+                          "flushed_us": round(c32 * 1e3, 2), "flushed_gbs": round(gbs(B32, c32), 1),
+                          "flushed_frac": round(gbs(B32, c32) / HBM_PEAK_GBS, 4)},
+                 "bytes_fp64": B64, "bytes_fp32": B32},
+        "roofline": {"kernel": f"k_scg_spmv (product of the FP32 inner CG with its two dot products, {variants['fp32']})",
+                     "bound": "hbm",
+                     "achieved": round(gbs_fused, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs_fused / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": B_fused, "avg_us": round(1e3 * t_fused, 2), "timed_launches": launches.value},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed hardware-counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by a copy kernel of known size), or None."""
+    path = ROOT / "profiles" / "hbm_traffic_p3_kappa_1.5M.md"
+    if not path.exists():
+        return None
+    text = path.read_text()
+    if "```json" not in text:
+        return None
+    blob = text.split("```json", 1)[1].split("```", 1)[0]
+    try:
+        return json.loads(blob).get(kernel)
+    except ValueError:
+        return None
 
 
 def cpu_baseline(P, T, a_form, L_form, fp64_iters, log):

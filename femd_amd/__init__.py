@@ -25,5 +25,6 @@ from .sparse import Preconditioner, SparseMatrix, SparseSolver
 from .krylov import KrylovInfo, cg, gmres, lgmres, solve_refined
 from .linalg import solve
 from .batch import Batch
+from .timestep import BackwardEuler
 
 __version__ = "0.9.0"

@@ -78,6 +78,9 @@ _SIGS = {
     "femd_spmv": (_i, [_p, _dp, _dp]),
     "femd_spmv_dev": (_i, [_p, _p, _p, _i, _p]),
     "femd_spmv_tuning": (_i, [_p, _i, C.c_char_p, _i, C.POINTER(_d)]),
+    "femd_spmv_shape_count": (_i, []),
+    "femd_spmv_shape_name": (_i, [_i, C.c_char_p, _i]),
+    "femd_spmv_select": (_i, [_p, _i, C.c_char_p]),
     "femd_ddot": (_i, [_i64, _dp, _dp, C.POINTER(_d)]),
     "femd_ddot_dev": (_i, [_i64, _p, _p, C.POINTER(_d), _p]),
     "femd_precond_create": (_i, [_p, _i, _i, _d, C.POINTER(_p), C.POINTER(_d)]),
@@ -97,8 +100,12 @@ _SIGS = {
     "femd_batch_solve_refined": (_i, [_p, _dp, _dp, _d, _d, _i, _i, _p, _p, C.POINTER(SolveInfo)]),
     "femd_batch_solve_refined_dev": (_i, [_p, _p, _p, _d, _d, _i, _i, _p, _p, C.POINTER(SolveInfo), _p]),
     "femd_batch_free": (None, [_p]),
+    "femd_heat_run": (_i, [_p, _p, _dp, _i, _d, _i, _d, _i, _i, _p, C.POINTER(SolveInfo)]),
+    "femd_heat_run_dev": (_i, [_p, _p, _p, _i, _d, _i, _d, _i, _i, _p, C.POINTER(SolveInfo), _p]),
     "femd_event_timer_start": (_i, [_p]),
     "femd_event_timer_stop": (_i, [_p, C.POINTER(_d)]),
+    "femd_inner_timer": (_i, [_i]),
+    "femd_inner_timer_read": (_i, [C.POINTER(_i64), C.POINTER(_d)]),
 }
 
 _lib = None

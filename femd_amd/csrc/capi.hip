@@ -1,6 +1,6 @@
 // capi.hip -- the C ABI of include/femd_hip.h: handles, the device memory pool, and the
 // orchestration of the kernels in kernels.hip.
-#include "internal.hpp"
+#include "cg32.hpp"
 
 #include <cmath>
 #include <limits>
@@ -279,7 +279,7 @@ femd_space_s *check_space(femd_space V)
 void spmv64(femd_csr_s *A, const double *x, double *y, const int *stop, hipStream_t s)
 {
     DevPattern &P = *A->pat;
-    spmv_run64(spmv_plan(P, FEMD_FP64, true, A->v.p, s), P, A->v.as<double>(), x, y, nullptr, stop, s);
+    spmv_run64(spmv_plan(P, FEMD_FP64, A->v.p, s), P, A->v.as<double>(), x, y, stop, s);
 }
 
 // The ordered dot product into a device scalar.
@@ -620,73 +620,31 @@ void refined_device(femd_csr_s *A, const double *b, double *x, double tol, doubl
 {
     const int n = A->pat->n;
     std::memset(info, 0, sizeof(*info));
-    DevPattern &P = *A->pat;
-    const float *v32 = values32(A, s);
-    const int *diag = diag_of(P, s);
-    DevBuf bad(sizeof(int));
-    int big = 0x7fffffff;
-    HIPC(hipMemcpyAsync(bad.p, &big, sizeof(int), hipMemcpyHostToDevice, s));
-    launch_inv_diag(n, diag, A->v.as<double>(), nullptr, bad.as<int>(), s);
-    int badrow = 0;
-    HIPC(hipMemcpyAsync(&badrow, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPC(hipStreamSynchronize(s));
-    if (badrow != 0x7fffffff) fail(FEMD_EZEROPIVOT, "Jacobi: zero diagonal entry in row " + std::to_string(badrow));
-    DevBuf inv32(4 * size_t(n)), r32(4 * size_t(n)), x32(4 * size_t(n)), p32(4 * size_t(n)), Ap32(4 * size_t(n));
-    DevBuf r64(8 * size_t(n)), t64(8 * size_t(n)), parts(8 * 2048 + 64), scal(64), stb(sizeof(CG32State));
-    Pinned hst(sizeof(CG32State));
-    launch_inv32(n, diag, A->v.as<double>(), inv32.as<float>(), s);
+    ScaledCG cg;
+    cg.setup(A, s);
+    DevBuf r64(8 * size_t(std::max(n, 1))), t64(8 * size_t(std::max(n, 1))), parts(8 * 2048 + 64), scal(64);
+    const int nbk = fast_blocks(n);
     launch_fdot(n, b, b, parts.as<double>(), s);
-    launch_fsum(fast_blocks(n), parts.as<double>(), scal.as<double>(), s);
+    launch_fsum(nbk, parts.as<double>(), scal.as<double>(), s);
     const double nb = std::sqrt(read_scalar(scal.as<double>(), s));
     if (nb == 0.0) { launch_fill(n, 0.0, x, s); HIPC(hipStreamSynchronize(s)); info->converged = 1; return; }
-    const int nbk = fast_blocks(n);
-    const SpmvPlan &plan = spmv_plan(P, FEMD_FP32, false, v32, s);
-    DevBuf pparts(8 * size_t(std::max(plan.nblocks, 1)));
-    CG32State *st = stb.as<CG32State>();
     int inner_total = 0, outer = 0;
     double res = 0.0;
     for (;;)
     {
+        // FP64 residual r = b - A x and its norm
         spmv64(A, x, t64.as<double>(), nullptr, s);
         launch_sub(n, b, t64.as<double>(), r64.as<double>(), s);
         launch_fdot(n, r64.as<double>(), r64.as<double>(), parts.as<double>(), s);
         launch_fsum(nbk, parts.as<double>(), scal.as<double>(), s);
-        const double rn = std::sqrt(read_scalar(scal.as<double>(), s));
-        res = rn / nb;
+        res = std::sqrt(read_scalar(scal.as<double>(), s)) / nb;
         if (res <= tol) { info->converged = 1; break; }
         if (outer >= max_outer) break;
         ++outer;
-        // FP32 Jacobi-PCG on A d = r / ||r||, to ||r_inner|| <= inner_tol
-        launch_r32_init(n, r64.as<double>(), 1.0 / rn, r32.as<float>(), x32.as<float>(), p32.as<float>(), inv32.as<float>(),
-                        parts.as<double>(), s);
-        launch_cg32_start(parts.as<double>(), nbk, st, s);
-        CG32State h{};
-        HIPC(hipMemcpyAsync(hst.p, st, sizeof(h), hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        std::memcpy(&h, hst.p, sizeof(h));
-        h.thr = inner_tol * inner_tol;
-        h.max_iter = std::max(10, 10 * n);
-        HIPC(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
-        const int *stop = &st->stop;
-        int batch = 16;
-        while (!h.stop)
-        {
-            for (int k = 0; k < batch; ++k)
-            {
-                spmv_run32(plan, P, v32, p32.as<float>(), Ap32.as<float>(), pparts.as<double>(), stop, s);
-                launch_cg32_alpha(pparts.as<double>(), plan.nblocks, st, s);
-                launch_cg32_update(n, x32.as<float>(), r32.as<float>(), p32.as<float>(), Ap32.as<float>(), inv32.as<float>(),
-                                   parts.as<double>(), st, s);
-                launch_cg32_beta(parts.as<double>(), nbk, st, s);
-                launch_cg32_pupdate(n, p32.as<float>(), r32.as<float>(), inv32.as<float>(), st, s);
-            }
-            HIPC(hipMemcpyAsync(hst.p, st, sizeof(h), hipMemcpyDeviceToHost, s));
-            HIPC(hipStreamSynchronize(s));
-            std::memcpy(&h, hst.p, sizeof(h));
-            batch = std::min(64, batch * 2);
-        }
-        inner_total += h.it;
-        launch_add_scaled32(n, rn, x32.as<float>(), x, s);
+        // FP32 CG on the scaled system for the correction, to a relative residual of inner_tol
+        const SCGState st = cg.run(r64.as<double>(), scal.as<double>(), inner_tol, std::max(10, 10 * n), s);
+        inner_total += st.it;
+        cg.finish(scal.as<double>(), x, s);
     }
     info->iterations = inner_total;
     info->breakdown = outer;
@@ -803,13 +761,22 @@ int femd_dev_alloc(size_t bytes, void **out)
     });
 }
 int femd_dev_free(void *p) { return api([&] { if (p) HIPC(hipFree(p)); }); }
+// Copies on the library stream, after the work already queued there, then synchronized.
 int femd_memcpy_htod(void *dst, const void *src, size_t bytes)
 {
-    return api([&] { ensure_device(); HIPC(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+    return api([&] {
+        hipStream_t s = S(nullptr);
+        if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        HIPC(hipStreamSynchronize(s));
+    });
 }
 int femd_memcpy_dtoh(void *dst, const void *src, size_t bytes)
 {
-    return api([&] { ensure_device(); HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+    return api([&] {
+        hipStream_t s = S(nullptr);
+        if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
 }
 int femd_stream_sync(femd_stream s) { return api([&] { HIPC(hipStreamSynchronize(S(s))); }); }
 int femd_vec_copy_dev(int64_t n, const double *src, double *dst, femd_stream s)
@@ -1008,7 +975,7 @@ int femd_csr_upload_values(femd_csr A, const double *data)
         femd_csr_s *a = check_csr(A);
         hipStream_t s = S(nullptr);
         if (a->pat->nnz) HIPC(hipMemcpyAsync(a->v.p, data, 8 * size_t(a->pat->nnz), hipMemcpyHostToDevice, s));
-        a->v32_valid = false;
+        a->v32_valid = false; a->v32s_valid = false;
         HIPC(hipStreamSynchronize(s));
     });
 }
@@ -1020,7 +987,7 @@ int femd_csr_device_arrays(femd_csr A, const int **indptr, const int **indices, 
         if (indptr) *indptr = a->pat->ip.as<int>();
         if (indices) *indices = a->pat->ix.as<int>();
         if (data) *data = a->v.as<double>();
-        a->v32_valid = false;                  // the caller may write the values
+        a->v32_valid = false; a->v32s_valid = false;                  // the caller may write the values
     });
 }
 
@@ -1078,7 +1045,7 @@ int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
         launch_elem_matrix(n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(), C.T, E.as<double>(), s);
         launch_gather(Ks->pat->nnz, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(), false, s);
-        Ks->v32_valid = false;
+        Ks->v32_valid = false; Ks->v32s_valid = false;
         HIPC(hipStreamSynchronize(s));
     });
 }
@@ -1139,8 +1106,8 @@ int femd_spmv_dev(femd_csr A, const void *x, void *y, int precision, femd_stream
         if (precision == FEMD_FP32)
         {
             const float *v32 = values32(a, s);
-            spmv_run32(spmv_plan(P, FEMD_FP32, false, v32, s), P, v32, static_cast<const float *>(x), static_cast<float *>(y),
-                       nullptr, nullptr, s);
+            spmv_run32(spmv_plan(P, FEMD_FP32, v32, s), P, v32, static_cast<const float *>(x), static_cast<float *>(y),
+                       nullptr, s);
         }
         else spmv64(a, static_cast<const double *>(x), static_cast<double *>(y), nullptr, s);
     });

@@ -584,6 +584,36 @@ std::vector<int> spmv_row_blocks(int n, const int *indptr, int max_rows, int cap
     return b;
 }
 
+void spmv_tile_columns(const int *indptr, const int *indices, const std::vector<int> &rows, int cap, std::vector<int> &uptr,
+                       std::vector<int> &ucol, std::vector<uint16_t> &lidx)
+{
+    if (cap > 65536) bad("spmv_tile_columns: cap above 65536");
+    const int nt = static_cast<int>(rows.size()) - 1;
+    const int64_t nnz = nt >= 0 ? indptr[rows.back()] : 0;
+    std::vector<int> count(static_cast<std::size_t>(std::max(nt, 0)) + 1, 0);
+    lidx.assign(static_cast<std::size_t>(nnz), 0);
+    std::vector<std::vector<int>> lists(static_cast<std::size_t>(std::max(nt, 0)));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int b = 0; b < nt; ++b)
+    {
+        const int p0 = indptr[rows[b]], p1 = indptr[rows[b + 1]];
+        if (p1 - p0 > cap) continue;
+        std::vector<int> &u = lists[b];
+        u.assign(indices + p0, indices + p1);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (int p = p0; p < p1; ++p)
+            lidx[p] = static_cast<uint16_t>(std::lower_bound(u.begin(), u.end(), indices[p]) - u.begin());
+        count[b + 1] = static_cast<int>(u.size());
+    }
+    uptr.assign(count.size(), 0);
+    for (int b = 0; b < nt; ++b) uptr[b + 1] = uptr[b] + count[b + 1];
+    ucol.resize(static_cast<std::size_t>(nt > 0 ? uptr[nt] : 0));
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nt; ++b)
+        std::copy(lists[b].begin(), lists[b].end(), ucol.begin() + uptr[b]);
+}
+
 // --------------------------------------------------------------------------------- orderings
 // sparse/amd.hpp:166-234: symmetric adjacency without the diagonal; seeds by increasing degree
 // (stable); George-Liu pseudo-peripheral search (at most 32 sweeps); breadth first with the new

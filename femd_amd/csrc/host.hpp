@@ -77,6 +77,12 @@ PatternData build_pattern(const SpaceData &V);
 // number at most cap; a single row longer than cap gets a block of its own (flagged long).
 std::vector<int> spmv_row_blocks(int n, const int *indptr, int max_rows, int cap);
 
+// The distinct columns of each tile (rows [rows[b], rows[b+1])): ucol[uptr[b] .. uptr[b+1]) sorted
+// ascending, and for every entry p of the tile lidx[p] = the position of indices[p] in that list.
+// Tiles with more than cap entries (a single long row) get no list and lidx 0.
+void spmv_tile_columns(const int *indptr, const int *indices, const std::vector<int> &rows, int cap, std::vector<int> &uptr,
+                       std::vector<int> &ucol, std::vector<uint16_t> &lidx);
+
 // sparse/amd.hpp:166-234 semantics: p[k] is the original row that becomes row k.
 std::vector<int> rcm_order(int n, const int *indptr, const int *indices);
 

@@ -111,12 +111,13 @@ inline double read_scalar(const double *d, hipStream_t s)
     return h;
 }
 
-// One SpMV variant ready to launch on one pattern (spmv.hip): its block table (stream variants)
-// or grid size (sub-wavefront variants), and the time the autotuner measured for it.
+// One SpMV launch shape ready to launch on one pattern (spmv.hip): its tile table, the tiles'
+// distinct-column lists, and the time the autotuner measured.
 struct SpmvPlan {
     int variant = 0;
-    int nblocks = 0;
-    DevBuf blocks;
+    int ntiles = 0;
+    DevBuf blocks;                 // int2 per tile boundary: (first row, first entry)
+    DevBuf uptr, ucol, lidx;       // each tile's distinct columns, and every entry's position in them
     double ms = 0.0;
 };
 
@@ -127,17 +128,21 @@ struct DevPattern {
     DevBuf ip, ix, blocks, diag;
     int nblocks = 0;
     bool have_diag = false;
-    std::unique_ptr<SpmvPlan> plan64, plan32, plan32x;    // tuned: FP64 exact, FP32 fast, FP32 exact
+    std::unique_ptr<SpmvPlan> plan64, plan32;    // tuned launch shapes per precision
 };
 
-// The tuned SpMV of a pattern (values give the tuning runs realistic data); exact restricts the
-// choice to the reference-order (CSR-stream) variants.
-const SpmvPlan &spmv_plan(DevPattern &P, int precision, bool exact, const void *values, hipStream_t s);
-void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, double *parts,
-                const int *stop, hipStream_t s);
-void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, double *parts,
-                const int *stop, hipStream_t s);
+// The tuned SpMV of a pattern (values give the tuning runs realistic data).  Every shape sums each
+// row left to right in the reference's order.
+const SpmvPlan &spmv_plan(DevPattern &P, int precision, const void *values, hipStream_t s);
+void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, const int *stop,
+                hipStream_t s);
+void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, const int *stop,
+                hipStream_t s);
 const char *spmv_variant_name(const SpmvPlan &p);
+extern const int NSHAPES_PUBLIC;
+const char *shape_name(int i);
+int shape_index(const char *name, int precision);     // -1: unknown, or not admissible in precision
+SpmvPlan shape_plan(const DevPattern &P, int si, int precision, hipStream_t s);
 
 std::shared_ptr<DevPattern> make_pattern(int n, int m, std::vector<int> ip, std::vector<int> ix, hipStream_t s);
 const int *diag_of(DevPattern &P, hipStream_t s);
@@ -170,6 +175,8 @@ struct femd_csr_s {
     std::shared_ptr<femd_amd::DevPattern> pat;
     femd_amd::DevBuf v, v32;
     bool v32_valid = false;
+    femd_amd::DevBuf v32s, scale;        // S A S in FP32 and S = diag(A)^(-1/2) (FP64), for the inner CG
+    bool v32s_valid = false;
     const femd_space_s *space = nullptr;
 };
 

@@ -1,188 +1,334 @@
-// spmv.hip -- SpMV kernel variants and the autotuner that picks one per matrix and precision.
+// spmv.hip -- the CSR SpMV tile, its launch shapes, the autotuner that picks one per matrix and
+// precision, and the product fused into the FP32 inner CG.
 //
-// Two families:
-//  * CSR-stream <T, THREADS, CAP>: a block owns a run of whole rows whose entries fit CAP; their
-//    products v[p] * x[ix[p]] are formed with coalesced loads and staged in LDS, then one thread
-//    per row adds them left to right.  This is CSRMatrix::apply's order (sparse/csr.hpp:203-215),
-//    so these are the only variants the exact FP64 path may use.
-//  * sub-wavefront <T, LPR>: LPR lanes per row stride through the row with fused multiply-adds
-//    and combine by cross-lane shuffles (no LDS, no barrier); faster on short rows, not in the
-//    reference's order, so offered to the FP32 inner solves only.
-// Both can also leave per-block partial sums of x_r * y_r (the p^T A p of CG) in `parts`.
+// The tile: a run of whole rows whose entries fit the LDS buffer (CAP entries, at most THREADS
+// rows), one workgroup each.  Finite-element rows of one tile share most of their columns (a
+// P3 tile of 2048 entries touches about 480 distinct ones), so the tile's distinct columns are
+// listed ahead of time (sorted, per pattern) and every entry carries a 16-bit position in that
+// list instead of its 32-bit column.  The workgroup
+//   1. loads the list and gathers x once per distinct column into LDS (xs), while every lane
+//      loads groups of four consecutive entries -- values and local positions with vector loads;
+//   2. forms the products v[p] * xs[lidx[p]] into LDS;
+//   3. adds each row's products left to right from zero, one lane per row: the order of
+//      CSRMatrix::apply (sparse/csr.hpp:203-215), so the FP64 product is bit-identical to the
+//      reference's.
+// Per entry the matrix stream is 8 + 2 (FP64) or 4 + 2 (FP32) bytes, and the scattered gathers
+// shrink to one per distinct column: the per-lane gather, not HBM, limits a plain CSR tile here.
 //
-// The autotuner times every admissible variant on the matrix itself (HIP events, a few
-// repetitions each) the first time a pattern is used in a precision, and keeps the fastest.
-#include "internal.hpp"
+// Launch shapes (THREADS, CAP) are the tuning space; the autotuner times each on the matrix
+// itself with HIP events the first time a pattern is used in a precision and keeps the fastest.
+//
+// The product of the FP32 CG (k_scg_spmv) is the same tile on r, writing w = A r and leaving the
+// partials of r.r and w.r per workgroup; a CG iteration is two kernels (this one and
+// k_scg_update, cg32.hip).
+#include "cg32.hpp"
+
+#include <hip/hip_ext.h>
 
 #include <cfloat>
 
 namespace femd_amd {
 namespace {
 
-__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
-template <typename T>
-__device__ inline T block_reduce(T v, T *sh)
-{
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+template <typename T> struct Vals4;
+template <> struct Vals4<float> {
+    f32x4 a;
+    __device__ void load(const float *v, int p) { a = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(v + p)); }
+    __device__ float get(int j) const { return a[j]; }
+    __device__ void set(int j, float t) { a[j] = t; }
+};
+template <> struct Vals4<double> {
+    f64x2 a, b;
+    __device__ void load(const double *v, int p)
     {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
+        a = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(v + p));
+        b = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(v + p + 2));
     }
-    return sh[0];
-}
+    __device__ double get(int j) const { return j < 2 ? a[j] : b[j - 2]; }
+    __device__ void set(int j, double t) { if (j < 2) a[j] = t; else b[j - 2] = t; }
+};
 
-template <typename T, int THREADS, int CAP, bool DOT>
-__global__ __launch_bounds__(THREADS) void k_stream(const int *__restrict__ blocks, const int *__restrict__ ip,
-                                                    const int *__restrict__ ix, const T *__restrict__ v,
-                                                    const T *__restrict__ x, T *__restrict__ y, double *__restrict__ parts,
-                                                    const int *stop)
+// One tile (workgroup blockIdx.x).  Op supplies gather(j) (the x value at global column j),
+// own(r) (what the row epilogue needs of row r) and store(r, s, own) (writes row r's result and
+// returns its share of the workgroup's partials, of type Op::Part).  Returns this thread's share.
+// `stop` (may be NULL) is read together with the tile's bounds: a set flag ends the workgroup
+// before it loads the matrix, and its latency hides behind theirs.  *stopped tells the caller.
+template <typename T, int THREADS, int CAP, class Op>
+__device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr, const int *__restrict__ ucol,
+                                     const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
+                                     const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op, T *xs,
+                                     T *prod, const int *stop, bool *stopped)
 {
-    if (stop && *stop) return;
-    __shared__ T prod[CAP];
-    __shared__ double red[DOT ? THREADS : 1];
-    const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
-    const int p0 = ip[r0], len = ip[r1] - p0;
-    double part = 0.0;
+    constexpr int NGRP = (CAP / 4 + 1 + THREADS - 1) / THREADS;
+    constexpr int NU = (CAP + THREADS - 1) / THREADS;
+    const int b = blockIdx.x;
+    const int2 b0 = blk[b], b1 = blk[b + 1];
+    const int halt = stop ? *stop : 0;
+    *stopped = halt != 0;
+    if (halt) return typename Op::Part{};
+    const int r0 = b0.x, r1 = b1.x, p0 = b0.y, len = b1.y - p0;
+    const int r = r0 + static_cast<int>(threadIdx.x);
+    int rb = 0, re = 0;
+    typename Op::Own own{};
+    if (r < r1)
+    {
+        rb = ip[r] - p0;
+        re = ip[r + 1] - p0;
+        own = op.own(r);
+    }
     if (len > CAP)
     {
-        // a single row longer than the staging buffer: one thread, from global memory
+        // one row longer than the buffer: one lane, from global memory, in order
+        typename Op::Part part{};
         if (threadIdx.x == 0)
         {
             T s = 0;
-            for (int p = p0; p < p0 + len; ++p) s += v[p] * x[ix[p]];
-            y[r0] = s;
-            part = static_cast<double>(s) * static_cast<double>(x[r0]);
+            for (int p = p0; p < p0 + len; ++p)
+            {
+                const T t = v[p] * op.gather(ix[p]);
+                s += t;
+            }
+            part = op.store(r0, s, own);
         }
+        return part;
     }
-    else
+    // 1. distinct columns and entry loads
+    const int u0 = uptr[b], nu = uptr[b + 1] - u0;
+    int uc[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i)
     {
-        for (int k = threadIdx.x; k < len; k += THREADS)
-            prod[k] = __builtin_nontemporal_load(v + p0 + k) * x[__builtin_nontemporal_load(ix + p0 + k)];
-        __syncthreads();
-        const int r = r0 + threadIdx.x;
-        if (r < r1)
+        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+        uc[i] = u < nu ? __builtin_nontemporal_load(ucol + u0 + u) : 0;
+    }
+    const int a0 = p0 & ~3, e = p0 + len;
+    Vals4<T> vv[NGRP];
+    u16x4 cc[NGRP];
+#pragma unroll
+    for (int g = 0; g < NGRP; ++g)
+    {
+        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
+        if (a < e)
         {
-            T s = 0;
-            for (int k = ip[r] - p0, e = ip[r + 1] - p0; k < e; ++k) s += prod[k];
-            y[r] = s;
-            if (DOT) part = static_cast<double>(s) * static_cast<double>(x[r]);
+            if (int64_t(a) + 4 <= nnz)
+            {
+                vv[g].load(v, a);
+                cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    const bool in = int64_t(a) + j < nnz;
+                    vv[g].set(j, in ? v[a + j] : T(0));
+                    cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
+                }
+            }
         }
+        else cc[g] = u16x4{0, 0, 0, 0};
     }
-    if (DOT)
+    T xg[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i)
     {
-        const double t = block_reduce(part, red);
-        if (threadIdx.x == 0) parts[blockIdx.x] = t;
-    }
-}
-
-template <typename T, int LPR, bool DOT>
-__global__ __launch_bounds__(256) void k_subwave(int n, const int *__restrict__ ip, const int *__restrict__ ix,
-                                                 const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y,
-                                                 double *__restrict__ parts, const int *stop)
-{
-    if (stop && *stop) return;
-    __shared__ double red[DOT ? 256 : 1];
-    const int lane = threadIdx.x % LPR;
-    const int r = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
-    T s = 0;
-    if (r < n)
-    {
-        const int e = ip[r + 1];
-        for (int p = ip[r] + lane; p < e; p += LPR)
-            s = fma_t(__builtin_nontemporal_load(v + p), x[__builtin_nontemporal_load(ix + p)], s);
+        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+        xg[i] = u < nu ? op.gather(uc[i]) : T(0);
     }
 #pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-    double part = 0.0;
-    if (r < n && lane == 0)
+    for (int i = 0; i < NU; ++i)
+    {
+        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+        if (u < nu) xs[u] = xg[i];
+    }
+    __syncthreads();
+    // 2. products
+#pragma unroll
+    for (int g = 0; g < NGRP; ++g)
+    {
+        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
+        if (a < e)
+        {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int k = a + j - p0;
+                if (k >= 0 && k < len) prod[k] = vv[g].get(j) * xs[cc[g][j]];
+            }
+        }
+    }
+    __syncthreads();
+    // 3. row sums, left to right
+    typename Op::Part part{};
+    if (r < r1)
+    {
+        T s = 0;
+        for (int k = rb; k < re; ++k) s += prod[k];
+        part = op.store(r, s, own);
+    }
+    return part;
+}
+
+// Sum of v over the workgroup in a fixed order (wave shuffles, then the waves in order); valid in
+// thread 0.
+template <int THREADS>
+__device__ inline double wg_sum(double v, double *red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < THREADS / 64; ++w) t += red[w];
+    return t;
+}
+
+template <typename T> struct PlainOp {
+    using Own = int;
+    using Part = int;
+    const T *x;
+    T *y;
+    __device__ T gather(int j) const { return x[j]; }
+    __device__ Own own(int) const { return 0; }
+    __device__ Part store(int r, T s, Own) const
     {
         y[r] = s;
-        if (DOT) part = static_cast<double>(s) * static_cast<double>(x[r]);
+        return 0;
     }
-    if (DOT)
+};
+
+template <typename T, int THREADS, int CAP>
+__global__ __launch_bounds__(THREADS) void k_spmv_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr,
+                                                       const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
+                                                       const int *__restrict__ ip, const int *__restrict__ ix,
+                                                       const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y,
+                                                       int64_t nnz, const int *stop)
+{
+    __shared__ T xs[CAP];
+    __shared__ T prod[CAP];
+    bool stopped = false;
+    (void)stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, PlainOp<T>{x, y}, xs, prod, stop, &stopped);
+}
+
+// ------------------------------------------------------------------ product of the FP32 CG
+// w = A r, and the partials of r.r and w.r over the workgroup's own rows.
+struct CGOp {
+    using Own = float;
+    using Part = Dot2;
+    const float *r;
+    float *w;
+    __device__ float gather(int j) const { return r[j]; }
+    __device__ Own own(int i) const { return r[i]; }
+    __device__ Part store(int i, float s, Own ri) const
     {
-        const double t = block_reduce(part, red);
-        if (threadIdx.x == 0) parts[blockIdx.x] = t;
+        w[i] = s;
+        Dot2 d;
+        d.a = static_cast<double>(ri) * static_cast<double>(ri);
+        d.b = static_cast<double>(s) * static_cast<double>(ri);
+        return d;
+    }
+};
+
+template <int THREADS, int CAP>
+__global__ __launch_bounds__(THREADS) void k_scg_spmv(const int2 *__restrict__ blk, const int *__restrict__ uptr,
+                                                      const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
+                                                      const int *__restrict__ ip, const int *__restrict__ ix,
+                                                      const float *__restrict__ v, const float *__restrict__ r,
+                                                      float *__restrict__ w, int64_t nnz, double *__restrict__ parts,
+                                                      const SCGState *st)
+{
+    __shared__ float xs[CAP];
+    __shared__ float prod[CAP];
+    __shared__ double red[THREADS / 64];
+    bool stopped = false;
+    const Dot2 d = stream_tile<float, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, CGOp{r, w}, xs, prod, &st->stop,
+                                                    &stopped);
+    if (stopped) return;
+    const double a = wg_sum<THREADS>(d.a, red);
+    __syncthreads();
+    const double b = wg_sum<THREADS>(d.b, red);
+    if (threadIdx.x == 0)
+    {
+        parts[2 * blockIdx.x] = a;
+        parts[2 * blockIdx.x + 1] = b;
     }
 }
 
 // ---------------------------------------------------------------------------------- dispatch
-struct Variant {
-    int kind;       // 0 stream, 1 sub-wavefront
-    int threads, cap, lpr;
+struct Shape {
+    int threads, cap;
     const char *name;
 };
-
-constexpr Variant VARIANTS[] = {
-    {0, 256, 4096, 0, "stream256x4096"}, {0, 256, 2048, 0, "stream256x2048"}, {0, 512, 4096, 0, "stream512x4096"},
-    {0, 128, 1024, 0, "stream128x1024"}, {1, 256, 0, 2, "subwave2"},          {1, 256, 0, 4, "subwave4"},
-    {1, 256, 0, 8, "subwave8"},          {1, 256, 0, 16, "subwave16"},
+constexpr Shape SHAPES[] = {
+    {256, 1024, "tile256x1024"}, {256, 2048, "tile256x2048"}, {512, 2048, "tile512x2048"}, {512, 4096, "tile512x4096"},
 };
-constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
+constexpr int NSHAPES = sizeof(SHAPES) / sizeof(SHAPES[0]);
 
-template <typename T, bool DOT>
-void launch_variant(const Variant &V, int nblocks, const int *blocks, int n, const int *ip, const int *ix, const T *v,
-                    const T *x, T *y, double *parts, const int *stop, hipStream_t s)
+// LDS (values and products) at most 64 KiB per workgroup.
+template <typename T> bool admissible(const Shape &S) { return 2 * S.cap * sizeof(T) <= 65536; }
+
+#define FEMD_SHAPES(X) X(256, 1024) X(256, 2048) X(512, 2048) X(512, 4096)
+
+template <typename T>
+void launch_shape(const SpmvPlan &p, const Shape &S, const int *ip, const int *ix, const T *v, const T *x, T *y, int64_t nnz,
+                  const int *stop, hipStream_t s)
 {
-    if (n <= 0) return;
-    if (V.kind == 0)
-    {
-        if (V.threads == 256 && V.cap == 4096)
-            hipLaunchKernelGGL((k_stream<T, 256, 4096, DOT>), dim3(nblocks), dim3(256), 0, s, blocks, ip, ix, v, x, y, parts, stop);
-        else if (V.threads == 256 && V.cap == 2048)
-            hipLaunchKernelGGL((k_stream<T, 256, 2048, DOT>), dim3(nblocks), dim3(256), 0, s, blocks, ip, ix, v, x, y, parts, stop);
-        else if (V.threads == 512)
-            hipLaunchKernelGGL((k_stream<T, 512, 4096, DOT>), dim3(nblocks), dim3(512), 0, s, blocks, ip, ix, v, x, y, parts, stop);
-        else
-            hipLaunchKernelGGL((k_stream<T, 128, 1024, DOT>), dim3(nblocks), dim3(128), 0, s, blocks, ip, ix, v, x, y, parts, stop);
-        return;
+    const int2 *blk = p.blocks.as<int2>();
+    const int *uptr = p.uptr.as<int>(), *ucol = p.ucol.as<int>();
+    const unsigned short *lidx = p.lidx.as<unsigned short>();
+#define FEMD_CASE(TH, CP)                                                                                          \
+    if (S.threads == TH && S.cap == CP)                                                                            \
+    {                                                                                                              \
+        if constexpr (2 * CP * sizeof(T) <= 65536)                                                                 \
+            hipLaunchKernelGGL((k_spmv_tile<T, TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx, ip, ix, v, \
+                               x, y, nnz, stop);                                                                   \
+        return;                                                                                                    \
     }
-    const int g = (n + 256 / V.lpr - 1) / (256 / V.lpr);
-    switch (V.lpr)
-    {
-    case 2: hipLaunchKernelGGL((k_subwave<T, 2, DOT>), dim3(g), dim3(256), 0, s, n, ip, ix, v, x, y, parts, stop); break;
-    case 4: hipLaunchKernelGGL((k_subwave<T, 4, DOT>), dim3(g), dim3(256), 0, s, n, ip, ix, v, x, y, parts, stop); break;
-    case 8: hipLaunchKernelGGL((k_subwave<T, 8, DOT>), dim3(g), dim3(256), 0, s, n, ip, ix, v, x, y, parts, stop); break;
-    default: hipLaunchKernelGGL((k_subwave<T, 16, DOT>), dim3(g), dim3(256), 0, s, n, ip, ix, v, x, y, parts, stop); break;
-    }
+    FEMD_SHAPES(FEMD_CASE)
+#undef FEMD_CASE
 }
 
-} // namespace
-
-namespace {
-
-SpmvPlan make_plan(const DevPattern &P, int vi, hipStream_t s)
+SpmvPlan make_plan(const DevPattern &P, int si, hipStream_t s)
 {
     SpmvPlan p;
-    p.variant = vi;
-    const Variant &V = VARIANTS[vi];
-    if (V.kind == 0)
+    p.variant = si;
+    const Shape &S = SHAPES[si];
+    const std::vector<int> rows = spmv_row_blocks(P.n, P.h_ip.data(), S.threads, S.cap);
+    p.ntiles = static_cast<int>(rows.size()) - 1;
+    std::vector<int> b2(2 * rows.size());
+    for (size_t k = 0; k < rows.size(); ++k)
     {
-        std::vector<int> b = spmv_row_blocks(P.n, P.h_ip.data(), V.threads, V.cap);
-        p.nblocks = static_cast<int>(b.size()) - 1;
-        p.blocks = upload(b, s);
+        b2[2 * k] = rows[k];
+        b2[2 * k + 1] = P.h_ip[static_cast<size_t>(rows[k])];
     }
-    else p.nblocks = (P.n + 256 / V.lpr - 1) / (256 / V.lpr);
+    p.blocks = upload(b2, s);
+    std::vector<int> uptr, ucol;
+    std::vector<uint16_t> lidx;
+    spmv_tile_columns(P.h_ip.data(), P.h_ix.data(), rows, S.cap, uptr, ucol, lidx);
+    p.uptr = upload(uptr, s);
+    if (ucol.empty()) ucol.push_back(0);
+    p.ucol = upload(ucol, s);
+    lidx.resize(lidx.size() + 8, 0);                   // room for the last group's vector load
+    p.lidx = upload(lidx, s);
+    HIPC(hipStreamSynchronize(s));                     // the host vectors go out of scope
     return p;
 }
 
 template <typename T>
-void run_plan(const SpmvPlan &p, const DevPattern &P, const T *v, const T *x, T *y, double *parts, const int *stop, hipStream_t s)
+void run_plan(const SpmvPlan &p, const DevPattern &P, const T *v, const T *x, T *y, const int *stop, hipStream_t s)
 {
-    const Variant &V = VARIANTS[p.variant];
-    if (parts)
-        launch_variant<T, true>(V, p.nblocks, p.blocks.as<int>(), P.n, P.ip.as<int>(), P.ix.as<int>(), v, x, y, parts, stop, s);
-    else
-        launch_variant<T, false>(V, p.nblocks, p.blocks.as<int>(), P.n, P.ip.as<int>(), P.ix.as<int>(), v, x, y, nullptr, stop, s);
+    if (P.n <= 0 || p.ntiles <= 0) return;
+    launch_shape<T>(p, SHAPES[p.variant], P.ip.as<int>(), P.ix.as<int>(), v, x, y, P.nnz, stop, s);
 }
 
-// Time every admissible variant on the pattern (values v) and keep the fastest.
+// Time every admissible shape on the pattern (values v, x = 0) and keep the fastest.
 template <typename T>
-SpmvPlan tune(const DevPattern &P, const T *v, bool exact, hipStream_t s)
+SpmvPlan tune(const DevPattern &P, const T *v, hipStream_t s)
 {
     SpmvPlan best;
     best.ms = DBL_MAX;
@@ -193,14 +339,14 @@ SpmvPlan tune(const DevPattern &P, const T *v, bool exact, hipStream_t s)
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
-    for (int vi = 0; vi < NVARIANTS; ++vi)
+    for (int si = 0; si < NSHAPES; ++si)
     {
-        if (exact && VARIANTS[vi].kind != 0) continue;
-        SpmvPlan p = make_plan(P, vi, s);
-        for (int k = 0; k < 2; ++k) run_plan<T>(p, P, v, x.as<T>(), y.as<T>(), nullptr, nullptr, s);
+        if (!admissible<T>(SHAPES[si])) continue;
+        SpmvPlan p = make_plan(P, si, s);
+        for (int k = 0; k < 2; ++k) run_plan<T>(p, P, v, x.as<T>(), y.as<T>(), nullptr, s);
         const int reps = P.nnz > (int64_t(1) << 22) ? 5 : 20;
         HIPC(hipEventRecord(e0, s));
-        for (int k = 0; k < reps; ++k) run_plan<T>(p, P, v, x.as<T>(), y.as<T>(), nullptr, nullptr, s);
+        for (int k = 0; k < reps; ++k) run_plan<T>(p, P, v, x.as<T>(), y.as<T>(), nullptr, s);
         HIPC(hipEventRecord(e1, s));
         HIPC(hipEventSynchronize(e1));
         float ms = 0.0f;
@@ -215,30 +361,65 @@ SpmvPlan tune(const DevPattern &P, const T *v, bool exact, hipStream_t s)
 
 } // namespace
 
-const SpmvPlan &spmv_plan(DevPattern &P, int precision, bool exact, const void *values, hipStream_t s)
+const SpmvPlan &spmv_plan(DevPattern &P, int precision, const void *values, hipStream_t s)
 {
-    std::unique_ptr<SpmvPlan> &slot = precision == FEMD_FP32 ? (exact ? P.plan32x : P.plan32) : P.plan64;
+    std::unique_ptr<SpmvPlan> &slot = precision == FEMD_FP32 ? P.plan32 : P.plan64;
     if (!slot)
     {
-        slot = std::make_unique<SpmvPlan>(precision == FEMD_FP32 ? tune<float>(P, static_cast<const float *>(values), exact, s)
-                                                                 : tune<double>(P, static_cast<const double *>(values), true, s));
+        slot = std::make_unique<SpmvPlan>(precision == FEMD_FP32 ? tune<float>(P, static_cast<const float *>(values), s)
+                                                                 : tune<double>(P, static_cast<const double *>(values), s));
     }
     return *slot;
 }
 
-void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, double *parts,
-                const int *stop, hipStream_t s)
+void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, const int *stop,
+                hipStream_t s)
 {
-    run_plan<double>(p, P, v, x, y, parts, stop, s);
+    run_plan<double>(p, P, v, x, y, stop, s);
 }
 
-void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, double *parts,
-                const int *stop, hipStream_t s)
+void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, const int *stop,
+                hipStream_t s)
 {
-    run_plan<float>(p, P, v, x, y, parts, stop, s);
+    run_plan<float>(p, P, v, x, y, stop, s);
 }
 
-const char *spmv_variant_name(const SpmvPlan &p) { return VARIANTS[p.variant].name; }
+int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float *r, float *w, double *parts,
+             const SCGState *st, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+    const Shape &S = SHAPES[p.variant];
+    const int2 *blk = p.blocks.as<int2>();
+    const int *uptr = p.uptr.as<int>(), *ucol = p.ucol.as<int>();
+    const unsigned short *lidx = p.lidx.as<unsigned short>();
+    if (P.n <= 0 || p.ntiles <= 0) return 0;
+#define FEMD_CASE(TH, CP)                                                                                        \
+    if (S.threads == TH && S.cap == CP)                                                                          \
+    {                                                                                                            \
+        if (e0)                                                                                                  \
+            hipExtLaunchKernelGGL((k_scg_spmv<TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, ucol, lidx, \
+                                  P.ip.as<int>(), P.ix.as<int>(), v, r, w, P.nnz, parts, st);                    \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_scg_spmv<TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx,      \
+                               P.ip.as<int>(), P.ix.as<int>(), v, r, w, P.nnz, parts, st);                       \
+        return p.ntiles;                                                                                         \
+    }
+    FEMD_SHAPES(FEMD_CASE)
+#undef FEMD_CASE
+    return 0;
+}
+
+const char *spmv_variant_name(const SpmvPlan &p) { return SHAPES[p.variant].name; }
+
+const int NSHAPES_PUBLIC = NSHAPES;
+const char *shape_name(int i) { return SHAPES[i].name; }
+int shape_index(const char *name, int precision)
+{
+    for (int si = 0; si < NSHAPES; ++si)
+        if (std::strcmp(SHAPES[si].name, name) == 0)
+            return (precision == FEMD_FP32 ? admissible<float>(SHAPES[si]) : admissible<double>(SHAPES[si])) ? si : -1;
+    return -1;
+}
+SpmvPlan shape_plan(const DevPattern &P, int si, int, hipStream_t s) { return make_plan(P, si, s); }
 
 } // namespace femd_amd
 
@@ -249,13 +430,40 @@ extern "C" int femd_spmv_tuning(femd_csr A, int precision, char *name, int name_
     return api([&] {
         femd_csr_s *a = check_csr(A);
         hipStream_t s = S(nullptr);
-        const SpmvPlan &p = precision == FEMD_FP32 ? spmv_plan(*a->pat, FEMD_FP32, false, values32(a, s), s)
-                                                   : spmv_plan(*a->pat, FEMD_FP64, true, a->v.p, s);
+        const SpmvPlan &p = precision == FEMD_FP32 ? spmv_plan(*a->pat, FEMD_FP32, values32(a, s), s)
+                                                   : spmv_plan(*a->pat, FEMD_FP64, a->v.p, s);
         if (name && name_len > 0)
         {
             std::strncpy(name, spmv_variant_name(p), size_t(name_len) - 1);
             name[name_len - 1] = '\0';
         }
         if (ms) *ms = p.ms;
+    });
+}
+
+extern "C" int femd_spmv_shape_count(void) { return femd_amd::NSHAPES_PUBLIC; }
+
+extern "C" int femd_spmv_shape_name(int index, char *name, int name_len)
+{
+    using namespace femd_amd;
+    return api([&] {
+        if (index < 0 || index >= NSHAPES_PUBLIC) fail(FEMD_EINVAL, "SpMV shape index out of range");
+        if (!name || name_len <= 0) fail(FEMD_EINVAL, "SpMV shape name: no buffer");
+        std::strncpy(name, shape_name(index), size_t(name_len) - 1);
+        name[name_len - 1] = '\0';
+    });
+}
+
+extern "C" int femd_spmv_select(femd_csr A, int precision, const char *name)
+{
+    using namespace femd_amd;
+    return api([&] {
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        std::unique_ptr<SpmvPlan> &slot = precision == FEMD_FP32 ? a->pat->plan32 : a->pat->plan64;
+        if (!name || !*name) { slot.reset(); return; }
+        const int si = shape_index(name, precision);
+        if (si < 0) fail(FEMD_EINVAL, std::string("SpMV shape '") + name + "' is unknown or not available in this precision");
+        slot = std::make_unique<SpmvPlan>(shape_plan(*a->pat, si, precision, s));
     });
 }

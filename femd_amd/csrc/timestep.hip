@@ -1,0 +1,187 @@
+// timestep.hip -- linear time stepping with a fixed matrix pair, each step one replay of a
+// captured hipGraph (the time-stepping pattern; the reference writes this loop in Python, e.g.
+// example13.ipynb: b = M u, then a solve with A, every step).
+//
+// Backward Euler for M u' + K u = 0 is  A u_{n+1} = M u_n  with A = M + dt K.  One step:
+//   b = M u (FP64, the reference's row order), bb = b.b;
+//   `rounds` times: r = b - A u (FP64), rr = r.r; unless rr <= tol^2 bb, the FP32 scaled
+//   Jacobi-CG solves A d = r / sqrt(rr) from d = 0 (warm start: the step starts from u_n) to a
+//   relative residual of inner_tol within max_inner iterations, and u += sqrt(rr) d (FP64);
+//   the step's inner iterations are added to a per-step counter on the device.
+// Every scalar lives on the device and every kernel of a finished solve returns at once, so the
+// whole step is one fixed launch sequence: captured once, replayed nsteps times back to back, and
+// the host reads the counters at the end.
+#include "cg32.hpp"
+
+#include <cmath>
+
+namespace femd_amd {
+namespace {
+
+struct StepScalars {
+    double bb;          // ||b||^2 of the current step
+    double rr;          // ||b - A u||^2 of the current round (0 when the round is skipped)
+    int counter;        // steps done
+    int total_inner;
+};
+
+__global__ void k_gate(StepScalars *sc, double tol2)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && sc->rr <= tol2 * sc->bb) sc->rr = 0.0;
+}
+
+__global__ void k_record(const SCGState *st, StepScalars *sc, int *iters, int nmax)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int it = st->it;
+    if (iters && sc->counter < nmax) iters[sc->counter] += it;
+    sc->total_inner += it;
+}
+
+__global__ void k_next_step(StepScalars *sc)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) sc->counter += 1;
+}
+
+struct Stepper {
+    femd_csr_s *M = nullptr, *A = nullptr;
+    int n = 0, rounds = 1, max_inner = 1, nsteps = 0;
+    double tol = 0.0, inner_tol = 0.0;
+    const SpmvPlan *pM = nullptr, *pA = nullptr;
+    ScaledCG cg;
+    DevBuf b, t, r, parts, sc, iters;
+    double *u = nullptr;
+
+    StepScalars *scal() const { return sc.as<StepScalars>(); }
+
+    void norm2(const double *x, double *out, hipStream_t s)
+    {
+        launch_fdot(n, x, x, parts.as<double>(), s);
+        launch_fsum(fast_blocks(n), parts.as<double>(), out, s);
+    }
+
+    void step(hipStream_t s)
+    {
+        spmv_run64(*pM, *M->pat, M->v.as<double>(), u, b.as<double>(), nullptr, s);
+        norm2(b.as<double>(), &scal()->bb, s);
+        for (int k = 0; k < rounds; ++k)
+        {
+            spmv_run64(*pA, *A->pat, A->v.as<double>(), u, t.as<double>(), nullptr, s);
+            launch_sub(n, b.as<double>(), t.as<double>(), r.as<double>(), s);
+            norm2(r.as<double>(), &scal()->rr, s);
+            hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, scal(), tol * tol);
+            cg.init(r.as<double>(), &scal()->rr, inner_tol, max_inner, s);
+            cg.iterate(0, max_inner, s);
+            cg.finish(&scal()->rr, u, s);
+            hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, cg.st.as<SCGState>(), scal(), iters.as<int>(), nsteps);
+        }
+        hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, s, scal());
+    }
+};
+
+void heat_run(femd_csr_s *M, femd_csr_s *A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
+              int use_graph, int *iters_out, femd_solve_info *info, hipStream_t s)
+{
+    const DevPattern &PM = *M->pat, &PA = *A->pat;
+    const int n = PA.n;
+    if (PA.m != n || PM.n != n || PM.m != n) fail(FEMD_EINVAL, "time stepping: M and A must be square and of one size");
+    if (nsteps < 0 || rounds < 1 || max_inner < 1 || !(tol >= 0.0) || !(inner_tol > 0.0))
+        fail(FEMD_EINVAL, "time stepping: need nsteps >= 0, rounds >= 1, max_inner >= 1, tol >= 0, inner_tol > 0");
+    std::memset(info, 0, sizeof(*info));
+    if (iters_out && nsteps > 0) std::memset(iters_out, 0, sizeof(int) * size_t(nsteps));
+    if (n == 0) { info->converged = 1; return; }
+    Stepper T;
+    T.M = M; T.A = A; T.n = n; T.rounds = rounds; T.max_inner = max_inner; T.nsteps = nsteps;
+    T.tol = tol; T.inner_tol = inner_tol; T.u = u;
+    // everything that tunes, allocates or synchronizes happens before the capture
+    T.pM = &spmv_plan(*M->pat, FEMD_FP64, M->v.p, s);
+    T.pA = &spmv_plan(*A->pat, FEMD_FP64, A->v.p, s);
+    T.cg.setup(A, s);
+    const size_t N = size_t(n);
+    T.b = DevBuf(8 * N); T.t = DevBuf(8 * N); T.r = DevBuf(8 * N);
+    T.parts = DevBuf(8 * 2048 + 64);
+    T.sc = DevBuf(sizeof(StepScalars));
+    T.iters = DevBuf(sizeof(int) * size_t(std::max(nsteps, 1)));
+    const StepScalars z{};
+    HIPC(hipMemcpyAsync(T.sc.p, &z, sizeof(z), hipMemcpyHostToDevice, s));
+    const std::vector<int> zi(size_t(std::max(nsteps, 1)), 0);
+    HIPC(hipMemcpyAsync(T.iters.p, zi.data(), sizeof(int) * zi.size(), hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+    if (nsteps > 0)
+    {
+        if (use_graph)
+        {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ex = nullptr;
+            HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try { T.step(s); }
+            catch (...) { (void)hipStreamEndCapture(s, &g); if (g) (void)hipGraphDestroy(g); throw; }
+            HIPC(hipStreamEndCapture(s, &g));
+            const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) { (void)hipGraphDestroy(g); hip_check(e, "hipGraphInstantiate"); }
+            hipError_t le = hipSuccess;
+            for (int k = 0; k < nsteps && le == hipSuccess; ++k) le = hipGraphLaunch(ex, s);
+            const hipError_t se = hipStreamSynchronize(s);
+            (void)hipGraphExecDestroy(ex);
+            (void)hipGraphDestroy(g);
+            hip_check(le, "hipGraphLaunch");
+            hip_check(se, "hipStreamSynchronize");
+        }
+        else
+        {
+            for (int k = 0; k < nsteps; ++k) T.step(s);
+            HIPC(hipStreamSynchronize(s));
+        }
+    }
+    // the last step's relative residual ||b - A u|| / ||b|| (b = M u_{n-1} is still in T.b)
+    double res = 0.0;
+    StepScalars hs{};
+    if (nsteps > 0)
+    {
+        spmv_run64(*T.pA, *A->pat, A->v.as<double>(), u, T.t.as<double>(), nullptr, s);
+        launch_sub(n, T.b.as<double>(), T.t.as<double>(), T.r.as<double>(), s);
+        T.norm2(T.r.as<double>(), &T.scal()->rr, s);
+        HIPC(hipMemcpyAsync(&hs, T.sc.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        if (iters_out) HIPC(hipMemcpyAsync(iters_out, T.iters.p, sizeof(int) * size_t(nsteps), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        res = hs.bb > 0.0 ? std::sqrt(hs.rr / hs.bb) : std::sqrt(hs.rr);
+    }
+    info->iterations = hs.total_inner;
+    info->breakdown = hs.counter;
+    info->residual = res;
+    info->precond_residual = res;
+    info->converged = (hs.counter == nsteps && res <= std::max(tol, 0.0) * (1.0 + 1e-6) + 1e-15) ? 1 : 0;
+}
+
+} // namespace
+} // namespace femd_amd
+
+using namespace femd_amd;
+
+extern "C" {
+
+int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,
+                      int max_inner, int use_graph, int *iters, femd_solve_info *info, femd_stream st)
+{
+    return api([&] {
+        if (!info) fail(FEMD_EINVAL, "time stepping: info is NULL");
+        heat_run(check_csr(M), check_csr(A), u_dev, nsteps, tol, rounds, inner_tol, max_inner, use_graph, iters, info, S(st));
+    });
+}
+
+int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
+                  int use_graph, int *iters, femd_solve_info *info)
+{
+    return api([&] {
+        if (!info) fail(FEMD_EINVAL, "time stepping: info is NULL");
+        femd_csr_s *a = check_csr(A);
+        hipStream_t s = S(nullptr);
+        const int n = a->pat->n;
+        DevBuf ud = upload(u, size_t(n), s);
+        heat_run(check_csr(M), a, ud.as<double>(), nsteps, tol, rounds, inner_tol, max_inner, use_graph, iters, info, s);
+        if (n) HIPC(hipMemcpyAsync(u, ud.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+} // extern "C"

@@ -128,3 +128,26 @@ def p3_kappa_problem(h: float = C1_H, seed: int = 0):
     f, _ = fourier_source(0)
     a, L = poisson_forms(V, f, kappa=kap)
     return V, a, L, dict(points=P, triangles=T, min_angle=ma, setup_s=time.perf_counter() - t0)
+
+
+def fourier_values(info, xs, ys):
+    """The field of fourier_source (its info dict) evaluated with NumPy at points."""
+    out = np.zeros_like(np.asarray(xs, dtype=np.float64))
+    for a, kx, ky in zip(info["a"], info["kx"], info["ky"]):
+        out = out + a * np.sin(kx * math.pi * xs) * np.sin(ky * math.pi * ys)
+    return out
+
+
+def heat_problem(N: int = 256, k: int = 2, dt: float = 1e-3, seed: int = 0):
+    """The heat configuration: u_t = Laplace u on an N x N structured triangulation, P_k with every
+    side Dirichlet, backward Euler with step dt: M = (u v) dx, A = (u v + dt grad u . grad v) dx,
+    u0 the random-Fourier field of the seed interpolated.  Returns (space, M, A, u0)."""
+    from .mesh import rectangle_mesh
+    from .spaces import LagrangeSpace
+    V = LagrangeSpace(rectangle_mesh(0, 0, 1, 1, N, N), k, bc="dirichlet")
+    u, v = F.TrialFunction(V), F.TestFunction(V)
+    M = F.form(u * v * F.dx).assemble()
+    A = F.form((u * v + F.Const(float(dt)) * F.dot(F.grad(u), F.grad(v))) * F.dx).assemble()
+    _, info = fourier_source(seed)
+    u0 = V.interpolate(lambda xs, ys: fourier_values(info, xs, ys))
+    return V, M, A, u0

@@ -2,10 +2,11 @@
 
     python -m femd_amd.roofline [--h 0.00265] [--reps 50]
 
-Runs the CSR-stream SpMV (FP64 and FP32) `reps` times on the P3 benchmark matrix and a plain
-device copy of 64 MiB, timing each loop with HIP events on the library stream, and prints one JSON
-line: kernel times, algorithmic bytes per launch (SpMV 12 nnz + 20 n + 4 in FP64, 8 nnz + 12 n + 4
-in FP32; copy 2 x bytes) and GB/s.  The same program is what the hardware-counter runs profile, so
+Runs the CSR-stream SpMV (FP64 and FP32) `reps` times back to back on the P3 benchmark matrix,
+then again with a 512 MiB copy before each product (so none is served from the Infinity Cache),
+and a plain device copy of 64 MiB, timing with HIP events on the library stream; prints one JSON
+line: kernel times, the tuned shapes, algorithmic bytes per launch (SpMV 12 nnz + 20 n + 4 in FP64,
+8 nnz + 12 n + 4 in FP32; copy 2 x bytes) and GB/s.  The same program is what the hardware-counter runs profile, so
 FETCH_SIZE / WRITE_SIZE per dispatch of the copy calibrate the counters against known bytes."""
 from __future__ import annotations
 
@@ -27,8 +28,8 @@ def _dev(lib, nbytes):
     return p.value
 
 
-def _timed(lib, fn, reps):
-    for _ in range(3):
+def _timed(lib, fn, reps, warm=3):
+    for _ in range(warm):
         fn()
     L.check(lib.femd_event_timer_start(None))
     for _ in range(reps):
@@ -38,20 +39,36 @@ def _timed(lib, fn, reps):
     return ms.value / reps
 
 
-def spmv_times(A, reps=50):
-    """Average kernel time (ms) of the FP64 and FP32 SpMV of A, from HIP events."""
+def spmv_times(A, reps=50, flush=False):
+    """Average kernel time (ms) of the FP64 and FP32 SpMV of A, from HIP events.  With flush, a
+    512 MiB device copy runs before every product (outside the events), so the matrix cannot be
+    served from the 256 MiB Infinity Cache left warm by the previous product."""
     lib = L.load()
     n, m = A.shape
     out = {}
-    for prec in (L.FP64, L.FP32):
-        xb, yb = _dev(lib, 8 * m), _dev(lib, 8 * n)
-        xs = np.random.default_rng(0).standard_normal(m)
-        if prec == L.FP32:
-            xs = xs.astype(np.float32)
-        L.check(lib.femd_memcpy_htod(xb, xs.ctypes.data, xs.nbytes))
-        out[prec] = _timed(lib, lambda: L.check(lib.femd_spmv_dev(A._h, xb, yb, prec, None)), reps)
-        lib.femd_dev_free(xb)
-        lib.femd_dev_free(yb)
+    fl = _dev(lib, 1 << 30) if flush else None
+    try:
+        for prec in (L.FP64, L.FP32):
+            xb, yb = _dev(lib, 8 * m), _dev(lib, 8 * n)
+            xs = np.random.default_rng(0).standard_normal(m)
+            if prec == L.FP32:
+                xs = xs.astype(np.float32)
+            L.check(lib.femd_memcpy_htod(xb, xs.ctypes.data, xs.nbytes))
+            run = lambda: L.check(lib.femd_spmv_dev(A._h, xb, yb, prec, None))
+            if not flush:
+                out[prec] = _timed(lib, run, reps)
+            else:
+                run()
+                tot = 0.0
+                for _ in range(reps):
+                    L.check(lib.femd_vec_copy_dev((1 << 29) // 8, fl, fl + (1 << 29), None))
+                    tot += _timed(lib, run, 1, warm=0)
+                out[prec] = tot / reps
+            lib.femd_dev_free(xb)
+            lib.femd_dev_free(yb)
+    finally:
+        if fl:
+            lib.femd_dev_free(fl)
     return out[L.FP64], out[L.FP32]
 
 
@@ -78,11 +95,21 @@ def main():
     ip, ix = V.pattern()
     n, nnz = V.dim, int(ix.size)
     t64, t32 = spmv_times(A, args.reps)
+    c64, c32 = spmv_times(A, max(5, args.reps // 5), flush=True)
     tc = copy_time(reps=args.reps)
+    names = {}
+    for prec in (L.FP64, L.FP32):
+        buf = C.create_string_buffer(64)
+        L.check(lib.femd_spmv_tuning(A._h, prec, buf, 64, None))
+        names[prec] = buf.value.decode()
     B64, B32, Bc = 12 * nnz + 20 * n + 4, 8 * nnz + 12 * n + 4, 2 * (64 << 20)
+
+    def row(t, B, name):
+        return {"us": t * 1e3, "bytes": B, "gbs": B / t / 1e6, "frac": B / t / 1e6 / HBM_PEAK_GBS, "shape": name}
+
     res = {"n": n, "nnz": nnz,
-           "spmv_fp64": {"us": t64 * 1e3, "bytes": B64, "gbs": B64 / t64 / 1e6, "frac": B64 / t64 / 1e6 / HBM_PEAK_GBS},
-           "spmv_fp32": {"us": t32 * 1e3, "bytes": B32, "gbs": B32 / t32 / 1e6, "frac": B32 / t32 / 1e6 / HBM_PEAK_GBS},
+           "spmv_fp64": row(t64, B64, names[L.FP64]), "spmv_fp32": row(t32, B32, names[L.FP32]),
+           "spmv_fp64_flushed": row(c64, B64, names[L.FP64]), "spmv_fp32_flushed": row(c32, B32, names[L.FP32]),
            "copy_64MiB": {"us": tc * 1e3, "bytes": Bc, "gbs": Bc / tc / 1e6}}
     print(json.dumps(res), flush=True)
 

@@ -1,0 +1,45 @@
+"""Time stepping with a fixed matrix pair on the GPU (the time-stepping pattern).
+
+The reference steps in Python (example13.ipynb: ``b = M @ u`` then a solve with ``A = M + dt K``,
+every step).  Here the whole step -- the FP64 product ``M u``, FP64 refinement rounds around an
+FP32 scaled Jacobi-CG correction warm-started at the previous ``u`` -- is one launch sequence,
+captured once into a hipGraph and replayed for every step:
+
+    stepper = BackwardEuler(M, A)                  # SparseMatrix pair on one pattern size
+    u, info = stepper.run(u0, nsteps=500, tol=1e-10)
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .sparse import SparseMatrix
+
+
+class BackwardEuler:
+    """u_{k+1} solves A u_{k+1} = M u_k (backward Euler for M u' + K u = 0 with A = M + dt K)."""
+
+    def __init__(self, M: SparseMatrix, A: SparseMatrix):
+        if not isinstance(M, SparseMatrix) or not isinstance(A, SparseMatrix):
+            raise TypeError("BackwardEuler: M and A must be SparseMatrix")
+        if M.shape != A.shape or M.shape[0] != M.shape[1]:
+            raise ValueError(f"BackwardEuler: M {M.shape} and A {A.shape} must be square and of one size")
+        self.M, self.A = M, A
+
+    def run(self, u0, nsteps, *, tol=1e-10, rounds=3, inner_tol=1e-4, max_inner=None, graph=True):
+        """Advance u0 by nsteps steps.  Returns (u, info) with info a dict: per-step inner FP32
+        iterations, their total, the last step's relative residual and whether it met tol."""
+        n = self.A.shape[0]
+        u = np.array(u0, dtype=np.float64).reshape(-1)
+        if u.size != n:
+            raise ValueError(f"BackwardEuler: u0 has {u.size} values, the system {n}")
+        if max_inner is None:
+            max_inner = max(50, min(2000, 4 * int(np.sqrt(n))))
+        its = np.zeros(max(int(nsteps), 1), dtype=np.int32)
+        info = L.SolveInfo()
+        L.check(L.load().femd_heat_run(self.M._h, self.A._h, u, int(nsteps), float(tol), int(rounds), float(inner_tol),
+                                       int(max_inner), int(bool(graph)), its.ctypes.data, C.byref(info)))
+        return u, dict(converged=bool(info.converged), iterations=its[:int(nsteps)], total_iterations=int(info.iterations),
+                       steps=int(info.breakdown), residual=float(info.residual))

@@ -59,7 +59,8 @@ const char *femd_last_error(void);
  * entry point that touches the GPU; returns FEMD_EDEVICE when no GPU is visible. */
 int femd_init(int device);
 
-/* Device memory for the _dev entry points (the library's caching pool). */
+/* Device memory for the _dev entry points (the library's caching pool).  The copies run on the
+ * library's stream, after the work already queued there, and return when done. */
 int femd_dev_alloc(size_t bytes, void **out);
 int femd_dev_free(void *p);
 int femd_memcpy_htod(void *dst, const void *src, size_t bytes);
@@ -169,10 +170,18 @@ int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient 
  * precision FEMD_FP32 uses the float shadow of the values (made on first use). */
 int femd_spmv(femd_csr A, const double *x, double *y);
 int femd_spmv_dev(femd_csr A, const void *x_dev, void *y_dev, int precision, femd_stream s);
-/* The SpMV variant the autotuner picked for A in a precision (FP64: among the reference-order
- * CSR-stream shapes; FP32: also the sub-wavefront kernels), tuning on first use; name receives its
- * name (name_len bytes), ms its measured time.  No counterpart in the reference. */
+/* The SpMV launch shape the autotuner picked for A in a precision (every shape sums rows in the
+ * reference's order), tuning on first use; name receives its name (name_len bytes), ms its
+ * measured time.  No counterpart in the reference. */
 int femd_spmv_tuning(femd_csr A, int precision, char *name, int name_len, double *ms);
+/* The tuning space: femd_spmv_shape_count() shapes, named by femd_spmv_shape_name.  No
+ * counterpart in the reference. */
+int femd_spmv_shape_count(void);
+int femd_spmv_shape_name(int index, char *name, int name_len);
+/* Pin A's SpMV in a precision to the named shape (tests, benchmarks); NULL or "" returns it to
+ * the autotuner.  FEMD_EINVAL for an unknown name or one not available in that precision.  A
+ * pattern's shape is shared by the matrices on it.  No counterpart in the reference. */
+int femd_spmv_select(femd_csr A, int precision, const char *name);
 /* The dot product of util/omp.hpp:176-192 (fd.ddot), with its summation order. */
 int femd_ddot(int64_t n, const double *x, const double *y, double *out);
 int femd_ddot_dev(int64_t n, const double *x_dev, const double *y_dev, double *out, femd_stream s);
@@ -248,8 +257,30 @@ int femd_batch_solve_refined_dev(femd_batch B, const double *b_dev, double *x_de
                                  femd_solve_info *info, femd_stream s);
 void femd_batch_free(femd_batch B);
 
+/* ---------------------------------------------------------------- time stepping
+ * Stepping with a fixed matrix pair: u_{k+1} solves A u_{k+1} = M u_k (backward Euler for
+ * M u' + K u = 0 with A = M + dt K).  No reference counterpart as one call: the reference writes
+ * this loop in Python (example13.ipynb: b = M u, then a solve with A, each step); this replaces
+ * that loop's SpMV and solve.  Each step: b = M u; `rounds` FP64 refinement rounds r = b - A u,
+ * each of which, unless ||r|| <= tol ||b||, runs the FP32 scaled Jacobi-CG for a correction
+ * (started from zero, so the step is warm-started at u_k) to relative residual inner_tol within
+ * max_inner iterations.  With use_graph the step is captured once into a hipGraph and replayed
+ * nsteps times with no host synchronisation between steps.  iters[k] (may be NULL) gets step k's
+ * inner iterations; info.iterations their sum, info.breakdown the number of steps done,
+ * info.residual the last step's ||b - A u|| / ||b||, info.converged whether that is <= tol. */
+int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
+                  int use_graph, int *iters, femd_solve_info *info);
+int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,
+                      int max_inner, int use_graph, int *iters, femd_solve_info *info, femd_stream s);
+
 /* ---------------------------------------------------------------- timing
  * Kernel-time of the library's work on a stream, measured with HIP events around it. */
+/* Sampled timing of the FP32 inner CG's fused product kernel: while enabled, every 8th launch
+ * outside graph capture is bracketed by HIP events on its stream; read returns the number of
+ * timed launches and their total milliseconds.  femd_inner_timer resets the samples.  No
+ * counterpart in the reference. */
+int femd_inner_timer(int enable);
+int femd_inner_timer_read(int64_t *launches, double *total_ms);
 int femd_event_timer_start(femd_stream s);
 int femd_event_timer_stop(femd_stream s, double *ms);
 

@@ -1,0 +1,52 @@
+"""The time stepper on the GPU: backward Euler with the step captured in a hipGraph, against the
+oracle's FP64 trajectory (b = M u, Jacobi-CG warm-started at u, tolerance 1e-13, every step)."""
+import numpy as np
+import pytest
+
+import femd_amd as fd
+from femd_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_trajectory(orc, V, M, A, u0, nsteps):
+    ip, ix = V.pattern()
+    Mv, Av = M.data, A.data
+    u = u0.copy()
+    for _ in range(nsteps):
+        b = orc.spmv(ip, ix, Mv, u)
+        u, io = orc.cg(ip, ix, Av, b, x0=u, M="jacobi", tol=1e-13, maxiter=100000)
+        assert io["converged"]
+    return u
+
+
+@pytest.mark.parametrize("k,N", [(2, 16), (1, 40), (3, 10)])
+def test_heat_matches_oracle_trajectory(orc, k, N):
+    print(f"\n[case] heat P{k} {N}x{N}, 25 steps", flush=True)
+    V, M, A, u0 = problems.heat_problem(N, k, dt=1e-3, seed=2)
+    nsteps = 25
+    uo = _oracle_trajectory(orc, V, M, A, u0, nsteps)
+    stepper = fd.BackwardEuler(M, A)
+    u, info = stepper.run(u0, nsteps, tol=1e-12, rounds=4, inner_tol=1e-4)
+    assert info["steps"] == nsteps and info["converged"], info
+    assert (info["iterations"] > 0).all()
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) <= 1e-8
+    # the same steps launched one kernel at a time: bit-identical
+    u2, info2 = stepper.run(u0, nsteps, tol=1e-12, rounds=4, inner_tol=1e-4, graph=False)
+    assert np.array_equal(u, u2) and np.array_equal(info["iterations"], info2["iterations"])
+
+
+def test_heat_edge_cases(orc):
+    print("\n[case] heat: zero steps, zero state, converged rounds skipped", flush=True)
+    V, M, A, u0 = problems.heat_problem(8, 2, dt=1e-3, seed=0)
+    stepper = fd.BackwardEuler(M, A)
+    u, info = stepper.run(u0, 0)
+    assert np.array_equal(u, u0) and info["steps"] == 0
+    z, info = stepper.run(np.zeros_like(u0), 5)
+    assert not z.any() and info["total_iterations"] == 0
+    # one round with a loose tolerance, then more rounds: extra rounds cost no iterations once met
+    _, i1 = stepper.run(u0, 3, tol=1e-3, rounds=1, inner_tol=1e-5)
+    _, i4 = stepper.run(u0, 3, tol=1e-3, rounds=4, inner_tol=1e-5)
+    assert i1["total_iterations"] == i4["total_iterations"]
+    with pytest.raises(ValueError):
+        stepper.run(u0[:-1], 2)

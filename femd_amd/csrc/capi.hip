@@ -622,34 +622,12 @@ void refined_device(femd_csr_s *A, const double *b, double *x, double tol, doubl
     std::memset(info, 0, sizeof(*info));
     ScaledCG cg;
     cg.setup(A, s);
-    DevBuf r64(8 * size_t(std::max(n, 1))), t64(8 * size_t(std::max(n, 1))), parts(8 * 2048 + 64), scal(64);
-    const int nbk = fast_blocks(n);
-    launch_fdot(n, b, b, parts.as<double>(), s);
-    launch_fsum(nbk, parts.as<double>(), scal.as<double>(), s);
-    const double nb = std::sqrt(read_scalar(scal.as<double>(), s));
-    if (nb == 0.0) { launch_fill(n, 0.0, x, s); HIPC(hipStreamSynchronize(s)); info->converged = 1; return; }
-    int inner_total = 0, outer = 0;
-    double res = 0.0;
-    for (;;)
-    {
-        // FP64 residual r = b - A x and its norm
-        spmv64(A, x, t64.as<double>(), nullptr, s);
-        launch_sub(n, b, t64.as<double>(), r64.as<double>(), s);
-        launch_fdot(n, r64.as<double>(), r64.as<double>(), parts.as<double>(), s);
-        launch_fsum(nbk, parts.as<double>(), scal.as<double>(), s);
-        res = std::sqrt(read_scalar(scal.as<double>(), s)) / nb;
-        if (res <= tol) { info->converged = 1; break; }
-        if (outer >= max_outer) break;
-        ++outer;
-        // FP32 CG on the scaled system for the correction, to a relative residual of inner_tol
-        const SCGState st = cg.run(r64.as<double>(), scal.as<double>(), inner_tol, std::max(10, 10 * n), s);
-        inner_total += st.it;
-        cg.finish(scal.as<double>(), x, s);
-    }
-    info->iterations = inner_total;
-    info->breakdown = outer;
-    info->residual = res;
-    info->precond_residual = res;
+    const ScaledCG::Result r = cg.reliable(A, b, x, tol, inner_tol, max_outer, std::max(1000, 10 * n), s);
+    info->converged = r.converged;
+    info->iterations = r.iterations;
+    info->breakdown = r.updates;
+    info->residual = r.residual;
+    info->precond_residual = r.residual;
 }
 
 // Build the preconditioner's device data from A.

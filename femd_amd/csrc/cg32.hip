@@ -65,9 +65,11 @@ __global__ __launch_bounds__(TB) void k_scg_init(int64_t n, const double *r64, c
         st->thr = 0.0;
         st->tol2 = tol2;
         st->rr = 0.0;
+        st->rho = rr > 0.0 ? std::sqrt(rr) : 0.0;
         st->it = 0;
         st->max_iter = max_iter;
         st->breakdown = 0;
+        st->fresh = 1;
         st->stop = rr > 0.0 ? 0 : 1;
     }
 }
@@ -80,11 +82,32 @@ constexpr int TS = 1024;
 __global__ __launch_bounds__(TS) void k_scg_scalars(const double *parts, int nparts, SCGState *st, int k)
 {
     if (st->stop) return;
-    __shared__ double red[TS / 64];
-    const double gam = wg_fixed_sum<TS>(parts, nparts, red, 2);
-    const double del = wg_fixed_sum<TS>(parts + 1, nparts, red, 2);
+    __shared__ double red[4 * (TS / 64)];
+    // the quadruples (r.r, w.r, w.p, p.s) in a fixed order: lane-strided, wave shuffles, waves
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *pp = reinterpret_cast<const double2 *>(parts);
+    for (int i = threadIdx.x; i < nparts; i += TS)
+    {
+        const double2 t0 = pp[2 * i], t1 = pp[2 * i + 1];
+        q[0] += t0.x;
+        q[1] += t0.y;
+        q[2] += t1.x;
+        q[3] += t1.y;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] += __shfl_xor(q[j], off, 64);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[4 * (threadIdx.x >> 6) + j] = q[j];
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    const double thr = k == 0 ? st->tol2 * gam : st->thr;
+    for (int j = 0; j < 4; ++j) q[j] = 0.0;
+    for (int wv = 0; wv < TS / 64; ++wv)
+        for (int j = 0; j < 4; ++j) q[j] += red[4 * wv + j];
+    const double gam = q[0], del = q[1], wp = q[2], ps = q[3];
+    const double thr = st->fresh ? st->tol2 * gam : st->thr;
     const bool nan = !(gam == gam) || !(del == del);
     if (nan || gam <= thr || k >= st->max_iter)
     {
@@ -95,8 +118,16 @@ __global__ __launch_bounds__(TS) void k_scg_scalars(const double *parts, int npa
         return;
     }
     double beta = 0.0, den = del;
-    if (k > 0)
+    if (k > 0 && st->fresh)
     {
+        // r was replaced by the true residual: the beta that makes p' = r + beta p conjugate to p,
+        // -(r.A p)/(p.A p) = -(w.p)/(p.s), and p'.A p' = w.r + 2 beta w.p + beta^2 p.s exactly
+        beta = ps > 0.0 ? -wp / ps : 0.0;
+        den = del + 2.0 * beta * wp + beta * beta * ps;
+    }
+    else if (k > 0)
+    {
+        // r from the recurrence (Chronopoulos-Gear)
         beta = gam / st->gam_prev;
         den = del - beta * gam / st->alp_prev;
     }
@@ -113,7 +144,8 @@ __global__ __launch_bounds__(TS) void k_scg_scalars(const double *parts, int npa
     st->beta = beta;
     st->gam_prev = gam;
     st->alp_prev = alpha;
-    if (k == 0) st->thr = thr;
+    if (st->fresh) st->thr = thr;
+    st->fresh = 0;
     st->it = k + 1;
 }
 
@@ -134,12 +166,45 @@ __global__ __launch_bounds__(TB) void k_scg_update(int64_t n, float *x32, float 
     }
 }
 
-__global__ void k_scg_finish(int64_t n, const double *sc, const float *x32, const double *rr64, double *x64)
+__global__ void k_scg_finish(int64_t n, const double *sc, const float *x32, const SCGState *st, double *x64)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double rho = st->rho;
+    if (rho > 0.0) x64[i] += rho * (sc[i] * static_cast<double>(x32[i]));
+}
+
+// A new FP64 residual r64 with norm^2 rr64[0]: r = S r64 / rho_new, y = 0, p and s times
+// rho / rho_new; the state keeps its direction (previous r.r rescaled) and starts a segment.
+__global__ __launch_bounds__(256) void k_scg_rescale(int64_t n, const double *r64, const double *rr64, const double *sc,
+                                                     float *r, float *x32, float *p, float *sv, SCGState *st)
+{
     const double rr = rr64[0];
-    if (rr > 0.0) x64[i] += std::sqrt(rr) * (sc[i] * static_cast<double>(x32[i]));
+    const double rho_new = rr > 0.0 ? std::sqrt(rr) : 0.0;
+    const double rho = st->rho;
+    const double scale = rho_new > 0.0 ? 1.0 / rho_new : 0.0;
+    const float f = static_cast<float>(rho_new > 0.0 ? rho / rho_new : 0.0);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
+    {
+        r[i] = static_cast<float>(sc[i] * r64[i] * scale);
+        x32[i] = 0.0f;
+        p[i] *= f;
+        sv[i] *= f;
+    }
+    __syncthreads();
+}
+
+__global__ void k_scg_rescale_state(const double *rr64, SCGState *st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rr = rr64[0];
+    const double rho_new = rr > 0.0 ? std::sqrt(rr) : 0.0;
+    const double f = rho_new > 0.0 ? st->rho / rho_new : 0.0;
+    st->gam_prev *= f * f;
+    st->rho = rho_new;
+    st->fresh = 1;
+    st->breakdown = 0;
+    st->stop = rho_new > 0.0 ? 0 : 1;
 }
 
 } // namespace
@@ -186,7 +251,7 @@ void ScaledCG::setup(femd_csr_s *A, hipStream_t s)
     p.ensure(4 * N);
     sv.ensure(4 * N);
     x32.ensure(4 * N);
-    parts.ensure(16 * size_t(std::max(plan->ntiles, 1)));
+    parts.ensure(32 * size_t(std::max(plan->ntiles, 1)));
     if (!st.p)
     {
         st.ensure(sizeof(SCGState));
@@ -260,39 +325,107 @@ void ScaledCG::iterate(int k0, int count, hipStream_t s)
     {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if ((k & 7) != 0 || !timer_sample(s, k, &e0, &e1)) e0 = e1 = nullptr;
-        const int np = scg_spmv(*plan, *P, v, r.as<float>(), w.as<float>(), parts.as<double>(), st.as<SCGState>(), s, e0, e1);
+        const int np = scg_spmv(*plan, *P, v, r.as<float>(), p.as<float>(), sv.as<float>(), w.as<float>(), parts.as<double>(),
+                                st.as<SCGState>(), k, s, e0, e1);
         hipLaunchKernelGGL(k_scg_scalars, dim3(1), dim3(TS), 0, s, parts.as<double>(), np, st.as<SCGState>(), k);
         hipLaunchKernelGGL(k_scg_update, dim3(vec_grid(n)), dim3(TB), 0, s, static_cast<int64_t>(n), x32.as<float>(),
                            r.as<float>(), w.as<float>(), p.as<float>(), sv.as<float>(), st.as<SCGState>());
     }
 }
 
-void ScaledCG::finish(const double *rr64, double *x64, hipStream_t s)
+void ScaledCG::finish(double *x64, hipStream_t s)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_scg_finish, dim3((n + TB - 1) / TB), dim3(TB), 0, s, static_cast<int64_t>(n), sc, x32.as<float>(),
-                           rr64, x64);
+                           st.as<SCGState>(), x64);
 }
 
-SCGState ScaledCG::run(const double *r64, const double *rr64, double inner_tol, int max_iter, hipStream_t s)
+void ScaledCG::rescale(const double *r64, const double *rr64, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scg_rescale, dim3(vec_grid(n)), dim3(256), 0, s, static_cast<int64_t>(n), r64, rr64, sc, r.as<float>(),
+                       x32.as<float>(), p.as<float>(), sv.as<float>(), st.as<SCGState>());
+    hipLaunchKernelGGL(k_scg_rescale_state, dim3(1), dim3(64), 0, s, rr64, st.as<SCGState>());
+}
+
+SCGState ScaledCG::resume(int *k, hipStream_t s)
 {
     Pinned h(sizeof(SCGState));
     const size_t mark = timer_mark();
-    init(r64, rr64, inner_tol, max_iter, s);
     SCGState out{};
-    int k = 0, batch = 8;
+    int batch = 8;
     for (;;)
     {
         HIPC(hipMemcpyAsync(h.p, st.p, sizeof(SCGState), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
         std::memcpy(&out, h.p, sizeof(out));
         if (out.stop) break;
-        iterate(k, batch, s);
-        k += batch;
+        iterate(*k, batch, s);
+        *k += batch;
         batch = std::min(64, 2 * batch);
     }
+    *k = out.it;
     timer_drop_after(mark, out.it);
     return out;
+}
+
+SCGState ScaledCG::run(const double *r64, const double *rr64, double inner_tol, int max_iter, hipStream_t s)
+{
+    init(r64, rr64, inner_tol, max_iter, s);
+    int k = 0;
+    return resume(&k, s);
+}
+
+ScaledCG::Result ScaledCG::reliable(femd_csr_s *A, const double *b, double *x, double tol, double inner_tol, int max_updates,
+                                    int max_iter, hipStream_t s)
+{
+    Result res;
+    DevBuf r64(8 * size_t(std::max(n, 1))), t64(8 * size_t(std::max(n, 1))), parts64(8 * 2048 + 64), scal(64);
+    const int nbk = fast_blocks(n);
+    auto residual = [&]() {
+        // FP64 r = b - A x and its squared norm in scal[0]
+        spmv_run64(spmv_plan(*A->pat, FEMD_FP64, A->v.p, s), *A->pat, A->v.as<double>(), x, t64.as<double>(), nullptr, s);
+        launch_sub(n, b, t64.as<double>(), r64.as<double>(), s);
+        launch_fdot(n, r64.as<double>(), r64.as<double>(), parts64.as<double>(), s);
+        launch_fsum(nbk, parts64.as<double>(), scal.as<double>(), s);
+        return std::sqrt(read_scalar(scal.as<double>(), s));
+    };
+    launch_fdot(n, b, b, parts64.as<double>(), s);
+    launch_fsum(nbk, parts64.as<double>(), scal.as<double>(), s);
+    const double nb = std::sqrt(read_scalar(scal.as<double>(), s));
+    if (nb == 0.0)
+    {
+        launch_fill(n, 0.0, x, s);
+        HIPC(hipStreamSynchronize(s));
+        res.converged = 1;
+        return res;
+    }
+    double rn = residual();
+    res.residual = rn / nb;
+    if (res.residual <= tol) { res.converged = 1; return res; }
+    init(r64.as<double>(), scal.as<double>(), inner_tol, max_iter, s);
+    int k = 0, done = 0;
+    for (;;)
+    {
+        const SCGState out = resume(&k, s);
+        finish(x, s);
+        ++res.updates;
+        rn = residual();
+        res.residual = rn / nb;
+        res.iterations = done + k;
+        if (res.residual <= tol) { res.converged = 1; break; }
+        if (res.iterations >= max_iter || res.updates >= max_updates) break;
+        if (out.breakdown)
+        {
+            // a non-finite or non-positive scalar: start over from the true residual
+            res.breakdown += 1;
+            done += k;
+            init(r64.as<double>(), scal.as<double>(), inner_tol, max_iter - done, s);
+            k = 0;
+            continue;
+        }
+        rescale(r64.as<double>(), scal.as<double>(), s);
+    }
+    return res;
 }
 
 } // namespace femd_amd

@@ -10,7 +10,9 @@
 // An iteration is three kernels: the product w = A r with the partials of r.r and w.r per
 // workgroup; one workgroup that adds those partials in a fixed order and sets alpha, beta and
 // the stopping flag; the vector update.  No atomics or fences, and the result does not depend on
-// timing.
+// timing.  For mixed-precision solves with reliable updates the FP32 residual is replaced by
+// the FP64 one from time to time; the first iteration after that takes p'.A p' from explicit dot
+// products (w.p, p.s) instead of the recurrence, which assumes the residual came from it.
 #pragma once
 #include "internal.hpp"
 
@@ -20,10 +22,12 @@ struct SCGState {
     double gam_prev;    // r.r of the previous iteration
     double alp_prev;    // alpha of the previous iteration
     double alpha, beta; // of the current iteration, for the update kernel
-    double thr;         // stop once r.r <= thr  (tol^2 r_0.r_0)
+    double thr;         // stop once r.r <= thr  (tol^2 times r.r at the start of the segment)
     double tol2;
     double rr;          // r.r when stopped
+    double rho;         // the FP64 residual norm the FP32 system is scaled by
     int it, max_iter, stop, breakdown;
+    int fresh;          // the next iteration starts a segment: it sets thr from its r.r
 };
 
 // Two FP64 partial sums carried together.
@@ -54,11 +58,12 @@ __device__ inline double wg_fixed_sum(const double *a, int n, double *red, int s
     return t;
 }
 
-// The product of the FP32 CG (spmv.hip): w = (S A S) r and, per workgroup, the partials
-// (r.r, w.r) of its own rows into parts[2 b], parts[2 b + 1].  Returns the number of workgroups.
-// With e0 / e1 not NULL the launch is bracketed by those events (hipExtLaunchKernelGGL).
-int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float *r, float *w, double *parts,
-             const SCGState *st, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// The product of the FP32 CG of iteration k (spmv.hip): w = (S A S) r and, per workgroup b, the
+// partials of r.r, w.r, w.p, p.s over its own rows into parts[4 b .. 4 b + 3] (the last two only
+// on the first iteration of a segment after the first, else zero).  Returns the number of
+// workgroups.  With e0 / e1 not NULL the launch is bracketed by those events.
+int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float *r, const float *pv, const float *sv, float *w,
+             double *parts, const SCGState *st, int k, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // Work space and launch sequence of one inner solve on one matrix.
 struct ScaledCG {
@@ -76,10 +81,25 @@ struct ScaledCG {
     void init(const double *r64, const double *rr64, double inner_tol, int max_iter, hipStream_t s);
     // Iterations k0 .. k0+count-1; no-ops once stopped.
     void iterate(int k0, int count, hipStream_t s);
-    // x64 += sqrt(rr64[0]) S y.
-    void finish(const double *rr64, double *x64, hipStream_t s);
-    // init, then iterate in growing batches, polling the state, until it stops.  Returns the state.
+    // x64 += rho S y, the solve's correction in FP64.
+    void finish(double *x64, hipStream_t s);
+    // Keep going from a new FP64 residual r64 (norm^2 rr64[0]) after finish: r = S r64 / rho_new,
+    // y = 0, and p, s (and the previous r.r) rescaled by rho / rho_new, so the search direction
+    // survives; the next iteration starts a segment to inner_tol of its own r.r.
+    void rescale(const double *r64, const double *rr64, hipStream_t s);
+    // init, then iterate in growing batches from index k0, polling the state, until it stops.
     SCGState run(const double *r64, const double *rr64, double inner_tol, int max_iter, hipStream_t s);
+    SCGState resume(int *k, hipStream_t s);
+    // Mixed-precision CG with reliable updates: one FP32 CG (never restarted) on the scaled
+    // system; whenever its r.r has dropped by inner_tol^2, its correction goes into x (FP64), the
+    // true residual b - A x is recomputed in FP64 and the FP32 vectors rescaled to it.  Stops when
+    // ||b - A x|| <= tol ||b||, after max_updates updates or max_iter iterations.
+    struct Result {
+        int converged = 0, iterations = 0, updates = 0, breakdown = 0;
+        double residual = 0.0;
+    };
+    Result reliable(femd_csr_s *A, const double *b, double *x, double tol, double inner_tol, int max_updates, int max_iter,
+                    hipStream_t s);
 };
 
 } // namespace femd_amd

@@ -39,8 +39,18 @@ __device__ inline double chunk_sum(const double *prod, int len, double *lane)
     const int tid = threadIdx.x, full = len & ~7;
     if (tid < 8)
     {
+        // 16 reads in flight ahead of the 16 dependent additions, in order
         double s = 0.0;
-        for (int i = tid; i < full; i += 8) s += prod[i];
+        int i = tid;
+        for (; i + 8 * 15 < full; i += 8 * 16)
+        {
+            double t[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) t[u] = prod[i + 8 * u];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += t[u];
+        }
+        for (; i < full; i += 8) s += prod[i];
         lane[tid] = s;
     }
     __syncthreads();
@@ -58,11 +68,34 @@ __device__ inline double chunk_sum(const double *prod, int len, double *lane)
 
 // The chunk partial sums added left to right from zero (util/omp.hpp:186-191); one chunk is
 // returned as it is.
-__device__ inline double ordered_total(const double *parts, int np)
+// Computed by a whole workgroup (every thread must call it; thread 0 gets the result): the
+// partials are staged through LDS in slices of TOTAL_SLICE by all threads, and thread 0 adds them
+// left to right with many LDS reads in flight.
+constexpr int TOTAL_SLICE = 2048;
+__device__ inline double ordered_total_wg(const double *parts, int np, double *sh)
 {
     if (np == 1) return parts[0];
     double s = 0.0;
-    for (int c = 0; c < np; ++c) s += parts[c];
+    for (int c0 = 0; c0 < np; c0 += TOTAL_SLICE)
+    {
+        const int m = np - c0 < TOTAL_SLICE ? np - c0 : TOTAL_SLICE;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) sh[k] = parts[c0 + k];
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            int k = 0;
+            for (; k + 15 < m; k += 16)
+            {
+                double t[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) t[u] = sh[k + u];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += t[u];
+            }
+            for (; k < m; ++k) s += sh[k];
+        }
+        __syncthreads();
+    }
     return s;
 }
 } // namespace
@@ -268,9 +301,11 @@ __global__ __launch_bounds__(TPB) void k_dot_parts(int64_t n, const double *__re
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-__global__ void k_sum_parts(int np, const double *parts, double *out)
+__global__ __launch_bounds__(TPB) void k_sum_parts(int np, const double *parts, double *out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = np == 0 ? 0.0 : ordered_total(parts, np);
+    __shared__ double sh[TOTAL_SLICE];
+    const double t = np == 0 ? 0.0 : ordered_total_wg(parts, np, sh);
+    if (threadIdx.x == 0) out[0] = t;
 }
 
 // ============================================================================ vectors
@@ -364,10 +399,12 @@ __global__ __launch_bounds__(TPB) void k_asymmetry(int n, const int *ip, const i
 
 // ============================================================================ CG (krylov/cg.hpp)
 // alpha = rz / p^T A p, with the breakdown test of cg.hpp:271-273 / 328-329.
-__global__ void k_cg_alpha(const double *parts, int np, CGState *st)
+__global__ __launch_bounds__(TPB) void k_cg_alpha(const double *parts, int np, CGState *st)
 {
-    if (threadIdx.x != 0 || st->stop) return;
-    const double pAp = ordered_total(parts, np);
+    if (st->stop) return;
+    __shared__ double sh[TOTAL_SLICE];
+    const double pAp = ordered_total_wg(parts, np, sh);
+    if (threadIdx.x != 0) return;
     st->pAp = pAp;
     if (!(pAp > 0.0)) { st->breakdown = 1; st->stop = 1; return; }
     st->alpha = st->rz / pAp;
@@ -419,16 +456,19 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, double *__restrict
 // After the update: ||r||, the stopping test, beta and the loop-top tests (cg.hpp:330-349).
 // jacobi_fused: r^T D^{-1} r came with the update (cg_jacobi); else rz is the next step's job
 // (general preconditioner: k_cg_rz after z = M^{-1} r), or rz = rr without a preconditioner.
-__global__ void k_cg_beta(const double *parts_rr, const double *parts_rz, int np, int mode, CGState *st)
+__global__ __launch_bounds__(TPB) void k_cg_beta(const double *parts_rr, const double *parts_rz, int np, int mode, CGState *st)
 {
-    if (threadIdx.x != 0 || st->stop) return;
-    const double rr = ordered_total(parts_rr, np);
+    if (st->stop) return;
+    __shared__ double sh[TOTAL_SLICE];
+    const double rr = ordered_total_wg(parts_rr, np, sh);
+    const double rz1 = mode == 1 ? ordered_total_wg(parts_rz, np, sh) : rr;
+    if (threadIdx.x != 0) return;
     st->rr = rr;
     st->rn = sqrt(rr);
     st->it += 1;
     if (st->rn <= st->tolnb) { st->stop = 1; st->converged_break = 1; return; }
     if (mode == 2) return;                                   // general M: beta after z = M^{-1} r
-    const double rzn = mode == 1 ? ordered_total(parts_rz, np) : rr;
+    const double rzn = rz1;
     st->beta = rzn / st->rz;
     st->rz = rzn;
     if (st->it >= st->max_iter) st->stop = 1;
@@ -436,10 +476,12 @@ __global__ void k_cg_beta(const double *parts_rr, const double *parts_rz, int np
 }
 
 // General preconditioner: rz_new = r^T z, beta, and the loop-top tests (cg.hpp:277-283).
-__global__ void k_cg_rz(const double *parts, int np, CGState *st)
+__global__ __launch_bounds__(TPB) void k_cg_rz(const double *parts, int np, CGState *st)
 {
-    if (threadIdx.x != 0 || st->stop) return;
-    const double rzn = ordered_total(parts, np);
+    if (st->stop) return;
+    __shared__ double sh[TOTAL_SLICE];
+    const double rzn = ordered_total_wg(parts, np, sh);
+    if (threadIdx.x != 0) return;
     st->beta = rzn / st->rz;
     st->rz = rzn;
     if (st->it >= st->max_iter) st->stop = 1;
@@ -820,7 +862,7 @@ void launch_dot_parts(int64_t n, const double *x, const double *y, double *parts
 
 void launch_sum_parts(int np, const double *parts, double *out, hipStream_t s)
 {
-    FEMD_LAUNCH(k_sum_parts, 1, 64, 0, s, np, parts, out);
+    FEMD_LAUNCH(k_sum_parts, 1, TPB, 0, s, np, parts, out);
 }
 
 void launch_sub(int64_t n, const double *b, const double *a, double *r, hipStream_t s)
@@ -868,7 +910,7 @@ void launch_asymmetry(int n, const int *ip, const int *ix, const double *v, doub
     if (n > 0) FEMD_LAUNCH(k_asymmetry, std::min(grid_for(n), 2048), TPB, 0, s, n, ip, ix, v, out2);
 }
 
-void launch_cg_alpha(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_alpha, 1, 64, 0, s, parts, np, st); }
+void launch_cg_alpha(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_alpha, 1, TPB, 0, s, parts, np, st); }
 void launch_cg_update(int64_t n, double *x, double *r, const double *p, const double *Ap, const double *inv,
                       double *parts_rr, double *parts_rz, CGState *st, hipStream_t s)
 {
@@ -876,9 +918,9 @@ void launch_cg_update(int64_t n, double *x, double *r, const double *p, const do
 }
 void launch_cg_beta(const double *parts_rr, const double *parts_rz, int np, int mode, CGState *st, hipStream_t s)
 {
-    FEMD_LAUNCH(k_cg_beta, 1, 64, 0, s, parts_rr, parts_rz, np, mode, st);
+    FEMD_LAUNCH(k_cg_beta, 1, TPB, 0, s, parts_rr, parts_rz, np, mode, st);
 }
-void launch_cg_rz(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_rz, 1, 64, 0, s, parts, np, st); }
+void launch_cg_rz(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_rz, 1, TPB, 0, s, parts, np, st); }
 void launch_cg_pupdate(int64_t n, double *p, const double *r, const double *inv, const CGState *st, hipStream_t s)
 {
     if (n > 0) FEMD_LAUNCH(k_cg_pupdate, grid_for(n), TPB, 0, s, n, p, r, inv, st);

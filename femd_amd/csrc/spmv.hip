@@ -19,8 +19,8 @@
 // itself with HIP events the first time a pattern is used in a precision and keeps the fastest.
 //
 // The product of the FP32 CG (k_scg_spmv) is the same tile on r, writing w = A r and leaving the
-// partials of r.r and w.r per workgroup; a CG iteration is two kernels (this one and
-// k_scg_update, cg32.hip).
+// partials of r.r and w.r (and, after a residual replacement, w.p and p.s) per workgroup
+// (cg32.hip).
 #include "cg32.hpp"
 
 #include <hip/hip_ext.h>
@@ -216,20 +216,49 @@ __global__ __launch_bounds__(THREADS) void k_spmv_tile(const int2 *__restrict__ 
 }
 
 // ------------------------------------------------------------------ product of the FP32 CG
-// w = A r, and the partials of r.r and w.r over the workgroup's own rows.
+// w = A r, and over the workgroup's own rows the partials of r.r and w.r; on the first
+// iteration of a segment (after the residual was replaced) also w.p and p.s, from which the
+// scalar step forms p'.A p' exactly.
+struct Dot4 {
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    __device__ Dot4 &operator+=(const Dot4 &o)
+    {
+        a += o.a;
+        b += o.b;
+        c += o.c;
+        d += o.d;
+        return *this;
+    }
+};
+struct CGOwn {
+    float r = 0.0f, p = 0.0f, s = 0.0f;
+};
 struct CGOp {
-    using Own = float;
-    using Part = Dot2;
+    using Own = CGOwn;
+    using Part = Dot4;
     const float *r;
+    const float *p, *sv;      // NULL unless the exact p'.A p' is wanted
     float *w;
     __device__ float gather(int j) const { return r[j]; }
-    __device__ Own own(int i) const { return r[i]; }
-    __device__ Part store(int i, float s, Own ri) const
+    __device__ Own own(int i) const
+    {
+        CGOwn o;
+        o.r = r[i];
+        if (p)
+        {
+            o.p = p[i];
+            o.s = sv[i];
+        }
+        return o;
+    }
+    __device__ Part store(int i, float s, Own o) const
     {
         w[i] = s;
-        Dot2 d;
-        d.a = static_cast<double>(ri) * static_cast<double>(ri);
-        d.b = static_cast<double>(s) * static_cast<double>(ri);
+        Dot4 d;
+        d.a = static_cast<double>(o.r) * static_cast<double>(o.r);
+        d.b = static_cast<double>(s) * static_cast<double>(o.r);
+        d.c = static_cast<double>(s) * static_cast<double>(o.p);
+        d.d = static_cast<double>(o.p) * static_cast<double>(o.s);
         return d;
     }
 };
@@ -239,23 +268,45 @@ __global__ __launch_bounds__(THREADS) void k_scg_spmv(const int2 *__restrict__ b
                                                       const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
                                                       const int *__restrict__ ip, const int *__restrict__ ix,
                                                       const float *__restrict__ v, const float *__restrict__ r,
+                                                      const float *__restrict__ p, const float *__restrict__ sv,
                                                       float *__restrict__ w, int64_t nnz, double *__restrict__ parts,
-                                                      const SCGState *st)
+                                                      const SCGState *st, int k)
 {
     __shared__ float xs[CAP];
     __shared__ float prod[CAP];
-    __shared__ double red[THREADS / 64];
+    __shared__ double red[4 * (THREADS / 64)];
+    const bool exact = k > 0 && st->fresh;
     bool stopped = false;
-    const Dot2 d = stream_tile<float, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, CGOp{r, w}, xs, prod, &st->stop,
-                                                    &stopped);
+    Dot4 d = stream_tile<float, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz,
+                                              CGOp{r, exact ? p : nullptr, exact ? sv : nullptr, w}, xs, prod, &st->stop,
+                                              &stopped);
     if (stopped) return;
-    const double a = wg_sum<THREADS>(d.a, red);
+    // the four sums in one pass: wave shuffles, then the waves in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        d.a += __shfl_xor(d.a, off, 64);
+        d.b += __shfl_xor(d.b, off, 64);
+        d.c += __shfl_xor(d.c, off, 64);
+        d.d += __shfl_xor(d.d, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        const int wv = threadIdx.x >> 6;
+        red[4 * wv] = d.a;
+        red[4 * wv + 1] = d.b;
+        red[4 * wv + 2] = d.c;
+        red[4 * wv + 3] = d.d;
+    }
     __syncthreads();
-    const double b = wg_sum<THREADS>(d.b, red);
     if (threadIdx.x == 0)
     {
-        parts[2 * blockIdx.x] = a;
-        parts[2 * blockIdx.x + 1] = b;
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int wv = 0; wv < THREADS / 64; ++wv)
+            for (int j = 0; j < 4; ++j) t[j] += red[4 * wv + j];
+        double2 *out = reinterpret_cast<double2 *>(parts) + 2 * blockIdx.x;
+        out[0] = make_double2(t[0], t[1]);
+        out[1] = make_double2(t[2], t[3]);
     }
 }
 
@@ -384,8 +435,8 @@ void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const fl
     run_plan<float>(p, P, v, x, y, stop, s);
 }
 
-int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float *r, float *w, double *parts,
-             const SCGState *st, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float *r, const float *pv, const float *sv, float *w,
+             double *parts, const SCGState *st, int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     const Shape &S = SHAPES[p.variant];
     const int2 *blk = p.blocks.as<int2>();
@@ -397,10 +448,10 @@ int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float
     {                                                                                                            \
         if (e0)                                                                                                  \
             hipExtLaunchKernelGGL((k_scg_spmv<TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, ucol, lidx, \
-                                  P.ip.as<int>(), P.ix.as<int>(), v, r, w, P.nnz, parts, st);                    \
+                                  P.ip.as<int>(), P.ix.as<int>(), v, r, pv, sv, w, P.nnz, parts, st, k);         \
         else                                                                                                     \
             hipLaunchKernelGGL((k_scg_spmv<TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx,      \
-                               P.ip.as<int>(), P.ix.as<int>(), v, r, w, P.nnz, parts, st);                       \
+                               P.ip.as<int>(), P.ix.as<int>(), v, r, pv, sv, w, P.nnz, parts, st, k);            \
         return p.ntiles;                                                                                         \
     }
     FEMD_SHAPES(FEMD_CASE)

@@ -72,7 +72,7 @@ struct Stepper {
             hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, scal(), tol * tol);
             cg.init(r.as<double>(), &scal()->rr, inner_tol, max_inner, s);
             cg.iterate(0, max_inner, s);
-            cg.finish(&scal()->rr, u, s);
+            cg.finish(u, s);
             hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, cg.st.as<SCGState>(), scal(), iters.as<int>(), nsteps);
         }
         hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, s, scal());

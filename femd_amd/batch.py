@@ -1,7 +1,7 @@
 """Batched solves of many independent systems (the batching pattern; no reference counterpart).
 
     B = Batch([A1, A2, ...])                       # SparseMatrix members, any sizes
-    xs, info = B.solve([b1, b2, ...], tol=1e-10)   # FP32 Jacobi-PCG + FP64 refinement per system
+    xs, info = B.solve([b1, b2, ...], tol=1e-10)   # FP32 Jacobi-CG + reliable FP64 updates per system
 
 All systems iterate in one launch sequence on the GPU (block-diagonal concatenation, per-system
 scalars and convergence flags, the iteration replayed from a captured hipGraph)."""
@@ -54,14 +54,18 @@ class Batch:
     def split(self, v):
         return [v[self.offsets[p]:self.offsets[p + 1]] for p in range(len(self))]
 
-    def solve(self, bs, x0=None, *, tol=1e-10, inner_tol=1e-4, max_outer=40, graph=True):
-        """Returns (list of solutions, dict with per-system iterations and relative residuals)."""
+    def solve(self, bs, x0=None, *, tol=1e-10, inner_tol=0.3, max_outer=200, graph=True, precision="fp32"):
+        """Returns (list of solutions, dict with per-system iterations and relative residuals).
+        precision: "fp32" (FP32 inner CG with reliable FP64 updates) or "fp64" (FP64 inner CG)."""
+        prec = {"fp32": L.FP32, "fp64": L.FP64}.get(precision)
+        if prec is None:
+            raise ValueError(f"Batch.solve: precision is 'fp32' or 'fp64', not {precision!r}")
         b = self.concat(bs)
         x = np.zeros(self.nrows) if x0 is None else self.concat(x0)
         its = np.zeros(len(self), dtype=np.int32)
         res = np.zeros(len(self))
         info = L.SolveInfo()
-        L.check(L.load().femd_batch_solve_refined(self._h, b, x, float(tol), float(inner_tol), int(max_outer), int(bool(graph)),
-                                                  its.ctypes.data, res.ctypes.data, C.byref(info)))
+        L.check(L.load().femd_batch_solve(self._h, b, x, float(tol), float(inner_tol), int(max_outer), prec, int(bool(graph)),
+                                          its.ctypes.data, res.ctypes.data, C.byref(info)))
         return self.split(x), dict(converged=bool(info.converged), iterations=its, residuals=res,
                                    refinement_rounds=int(info.breakdown), worst_residual=float(info.residual))

@@ -1,6 +1,6 @@
 // capi.hip -- the C ABI of include/femd_hip.h: handles, the device memory pool, and the
 // orchestration of the kernels in kernels.hip.
-#include "cg32.hpp"
+#include "cg_engine.hpp"
 
 #include <cmath>
 #include <limits>
@@ -72,9 +72,6 @@ std::shared_ptr<DevPattern> make_pattern(int n, int m, std::vector<int> ip, std:
     P->h_ip = std::move(ip); P->h_ix = std::move(ix);
     P->ip = upload(P->h_ip, s);
     P->ix = upload(P->h_ix, s);
-    std::vector<int> b = spmv_row_blocks(n, P->h_ip.data(), SPMV_THREADS, SPMV_CAP64);
-    P->nblocks = static_cast<int>(b.size()) - 1;
-    P->blocks = upload(b, s);
     return P;
 }
 
@@ -618,16 +615,18 @@ GRes lgmres_dev(Krylov &K, double *x, const double *b, int inner, int k_aug, int
 void refined_device(femd_csr_s *A, const double *b, double *x, double tol, double inner_tol, int max_outer,
                     femd_solve_info *info, hipStream_t s)
 {
-    const int n = A->pat->n;
     std::memset(info, 0, sizeof(*info));
-    ScaledCG cg;
-    cg.setup(A, s);
-    const ScaledCG::Result r = cg.reliable(A, b, x, tol, inner_tol, max_outer, std::max(1000, 10 * n), s);
+    if (!(inner_tol > 0.0 && inner_tol < 1.0)) fail(FEMD_EINVAL, "refined solve: need 0 < inner_tol < 1");
+    if (A->pat->n == 0) { info->converged = 1; return; }
+    CGEngine cg;
+    cg.setup(A, FEMD_FP32, s);
+    double res = 0.0;
+    const CGEngine::Result r = cg.solve(b, x, tol, inner_tol, std::max(max_outer, 1), true, nullptr, &res, s);
     info->converged = r.converged;
     info->iterations = r.iterations;
     info->breakdown = r.updates;
-    info->residual = r.residual;
-    info->precond_residual = r.residual;
+    info->residual = res;
+    info->precond_residual = res;
 }
 
 // Build the preconditioner's device data from A.
@@ -953,7 +952,7 @@ int femd_csr_upload_values(femd_csr A, const double *data)
         femd_csr_s *a = check_csr(A);
         hipStream_t s = S(nullptr);
         if (a->pat->nnz) HIPC(hipMemcpyAsync(a->v.p, data, 8 * size_t(a->pat->nnz), hipMemcpyHostToDevice, s));
-        a->v32_valid = false; a->v32s_valid = false;
+        a->v32_valid = false; a->v32s_valid = false; a->v64s_valid = false;
         HIPC(hipStreamSynchronize(s));
     });
 }
@@ -965,7 +964,7 @@ int femd_csr_device_arrays(femd_csr A, const int **indptr, const int **indices, 
         if (indptr) *indptr = a->pat->ip.as<int>();
         if (indices) *indices = a->pat->ix.as<int>();
         if (data) *data = a->v.as<double>();
-        a->v32_valid = false; a->v32s_valid = false;                  // the caller may write the values
+        a->v32_valid = false; a->v32s_valid = false; a->v64s_valid = false;                  // the caller may write the values
     });
 }
 
@@ -1023,7 +1022,7 @@ int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
         launch_elem_matrix(n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(), C.T, E.as<double>(), s);
         launch_gather(Ks->pat->nnz, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(), false, s);
-        Ks->v32_valid = false; Ks->v32s_valid = false;
+        Ks->v32_valid = false; Ks->v32s_valid = false; Ks->v64s_valid = false;
         HIPC(hipStreamSynchronize(s));
     });
 }

@@ -1,0 +1,770 @@
+// cg_engine.hip -- the inner CG engine (one matrix or a batch of systems, FP32 or FP64): scaled
+// values, the four iteration kernels, the reliable-update maintenance and the host driver.
+#include "cg_engine.hpp"
+#include "kernels.hpp"
+#include "tile.hpp"
+
+#include <hip/hip_ext.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace femd_amd {
+namespace {
+
+constexpr int BT = 256;
+constexpr int BCHUNK = 4096;
+constexpr int GRAPH_ITERS = 16;
+
+inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + BT - 1) / BT)); }
+
+// The sum of a[0..n) (stride `stride`) in a fixed order -- lane-strided, wave shuffles, then the
+// waves in order -- returned to every thread of the workgroup.
+template <int THREADS>
+__device__ inline double wg_fixed_sum(const double *a, int n, double *red, int stride = 1)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += THREADS) s += a[static_cast<int64_t>(i) * stride];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < THREADS / 64; ++w) t += red[w];
+    __syncthreads();
+    return t;
+}
+
+// ---- scaled values ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BT) void k_scale_of_diag(int n, const int *diag, const double *v, double *sc, float *dg,
+                                                      int *bad_row)
+{
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    const int d = diag[i];
+    const double a = d >= 0 ? v[d] : 0.0;
+    dg[i] = static_cast<float>(a);
+    if (!(a > 0.0))
+    {
+        atomicMin(bad_row, i);
+        sc[i] = 0.0;
+    }
+    else sc[i] = 1.0 / std::sqrt(a);
+}
+
+// out[p] = v[p] s_i s_j for the entries of row i (one lane per row: a set-up pass).
+template <typename T>
+__global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, const int *ix, const double *v, const double *sc,
+                                                      T *out)
+{
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    const double si = sc[i];
+    for (int p = ip[i]; p < ip[i + 1]; ++p) out[p] = static_cast<T>(v[p] * si * sc[ix[p]]);
+}
+
+// ---- iteration kernels -----------------------------------------------------------------------
+template <typename T, int THREADS, int CAP>
+__global__ __launch_bounds__(THREADS) void k_cg_product(const int2 *__restrict__ blk, const int *__restrict__ uptr,
+                                                        const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
+                                                        const int *__restrict__ ip, const int *__restrict__ ix,
+                                                        const T *__restrict__ v, const T *__restrict__ r, T *P0, T *P1,
+                                                        T *__restrict__ q, int64_t nnz, const int *__restrict__ tile_sys,
+                                                        double *__restrict__ tparts, const CGSys *st)
+{
+    __shared__ T xs[CAP];
+    __shared__ T prod[CAP];
+    __shared__ double red[THREADS / 64];
+    const CGSys &S = st[tile_sys ? tile_sys[blockIdx.x] : 0];
+    if (!S.active) return;
+    const bool odd = S.it & 1;
+    const T *pold = odd ? P1 : P0;
+    T *pnew = odd ? P0 : P1;
+    bool stopped = false;
+    const double part = stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz,
+                                                     PCGOp<T>{r, pold, pnew, q, static_cast<T>(S.beta)}, xs, prod, nullptr,
+                                                     &stopped);
+    const double t = wg_sum<THREADS>(part, red);
+    if (threadIdx.x == 0) tparts[blockIdx.x] = t;
+}
+
+// one workgroup per system: p'.q over its tiles, alpha
+__global__ __launch_bounds__(BT) void k_cg_alpha(const int *__restrict__ seg_tiles, const double *__restrict__ tparts, CGSys *st)
+{
+    CGSys &S = st[blockIdx.x];
+    if (!S.active) return;
+    __shared__ double red[BT / 64];
+    const int t0 = seg_tiles[blockIdx.x], t1 = seg_tiles[blockIdx.x + 1];
+    const double pq = wg_fixed_sum<BT>(tparts + t0, t1 - t0, red);
+    if (threadIdx.x != 0) return;
+    if (!(pq > 0.0))
+    {
+        S.active = 0;
+        S.reason = 2;
+        return;
+    }
+    S.alpha = S.rr / pq;
+}
+
+// y += alpha p', r -= alpha q over one chunk of one system, and the chunk's partials of r.r and
+// r.D r (D = diag(A): rho^2 r.D r is the squared norm of the unscaled residual r stands for)
+template <typename T>
+__global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ clo, const int *__restrict__ cpid, T *y, T *r,
+                                                  const T *P0, const T *P1, const T *__restrict__ q,
+                                                  const float *__restrict__ dg, double *cparts, const CGSys *st)
+{
+    __shared__ double red[2 * (BT / 64)];
+    const int c = blockIdx.x;
+    const CGSys &S = st[cpid[c]];
+    if (!S.active) return;
+    const T *pnew = (S.it & 1) ? P0 : P1;
+    const T a = static_cast<T>(S.alpha);
+    double s = 0.0, t = 0.0;
+    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
+    {
+        y[i] = fma(a, pnew[i], y[i]);
+        const T ri = fma(-a, q[i], r[i]);
+        r[i] = ri;
+        const double e = static_cast<double>(ri) * static_cast<double>(ri);
+        s += e;
+        t += e * static_cast<double>(dg[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        s += __shfl_xor(s, off, 64);
+        t += __shfl_xor(t, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        red[2 * (threadIdx.x >> 6)] = s;
+        red[2 * (threadIdx.x >> 6) + 1] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double a0 = 0.0, a1 = 0.0;
+        for (int w = 0; w < BT / 64; ++w)
+        {
+            a0 += red[2 * w];
+            a1 += red[2 * w + 1];
+        }
+        cparts[2 * c] = a0;
+        cparts[2 * c + 1] = a1;
+    }
+}
+
+// one workgroup per system: the new r.r and r.D r over its chunks, the pause tests, beta
+__global__ __launch_bounds__(BT) void k_cg_beta(const int *__restrict__ cptr, const double *__restrict__ cparts, CGSys *st)
+{
+    CGSys &S = st[blockIdx.x];
+    if (!S.active) return;
+    __shared__ double red[BT / 64];
+    const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
+    const double rr = wg_fixed_sum<BT>(cparts + 2 * c0, c1 - c0, red, 2);
+    const double rt = wg_fixed_sum<BT>(cparts + 2 * c0 + 1, c1 - c0, red, 2);
+    if (threadIdx.x != 0) return;
+    S.it += 1;
+    S.total += 1;
+    int reason = 0;
+    if (!(rr == rr) || !(rt == rt)) reason = 2;
+    else if (rr <= S.thr || rt <= S.tgt) reason = 1;
+    else if (S.it >= S.max_iter) reason = 3;
+    if (reason)
+    {
+        S.active = 0;
+        S.reason = reason;
+        S.rr = rr;
+        return;
+    }
+    S.beta = rr / S.rr;
+    S.rr = rr;
+}
+
+// ---- maintenance kernels (on the systems with sel[sys] != 0) ---------------------------------
+template <typename T>
+__global__ __launch_bounds__(BT) void k_cg_finish(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
+                                                  const int *__restrict__ sel, const double *__restrict__ sc,
+                                                  const T *__restrict__ y, const CGSys *st, double *x)
+{
+    const int c = blockIdx.x, sys = cpid[c];
+    if (!sel[sys]) return;
+    const double rho = st[sys].rho;
+    if (!(rho > 0.0)) return;
+    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT) x[i] += rho * (sc[i] * static_cast<double>(y[i]));
+}
+
+// r64 = b - t64 and the chunk's partial of r64.r64 (of b.b when t64 is NULL)
+__global__ __launch_bounds__(BT) void k_cg_residual(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
+                                                    const int *__restrict__ sel, const double *__restrict__ b,
+                                                    const double *__restrict__ t64, double *r64, double *cparts)
+{
+    __shared__ double red[BT / 64];
+    const int c = blockIdx.x, sys = cpid[c];
+    if (!sel[sys]) return;
+    double s = 0.0;
+    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
+    {
+        const double ri = t64 ? b[i] - t64[i] : b[i];
+        if (t64) r64[i] = ri;
+        s += ri * ri;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for (int w = 0; w < BT / 64; ++w) t += red[w];
+        cparts[c] = t;
+    }
+}
+
+// one workgroup per system: the sum of its chunks' partials
+__global__ __launch_bounds__(BT) void k_cg_sys_sum(const int *__restrict__ cptr, const int *__restrict__ sel,
+                                                   const double *__restrict__ cparts, double *out)
+{
+    if (!sel[blockIdx.x]) return;
+    __shared__ double red[BT / 64];
+    const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
+    const double t = wg_fixed_sum<BT>(cparts + c0, c1 - c0, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// Resume (sel 1) or restart with no direction (sel 2): r = S r64 / rho_new, y = 0, and the
+// current direction p and q = A p scaled by rho / rho_new (or zeroed).
+template <typename T>
+__global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
+                                                   const int *__restrict__ sel, const double *__restrict__ rho_new,
+                                                   const double *__restrict__ sc, const double *__restrict__ r64, T *r, T *y,
+                                                   T *P0, T *P1, T *q, const CGSys *st)
+{
+    const int c = blockIdx.x, sys = cpid[c];
+    const int mode = sel[sys];
+    if (!mode) return;
+    const double rn = rho_new[sys];
+    const double scale = rn > 0.0 ? 1.0 / rn : 0.0;
+    const T f = static_cast<T>(mode == 2 || !(rn > 0.0) ? 0.0 : st[sys].rho / rn);
+    T *pc = (st[sys].it & 1) ? P1 : P0;
+    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
+    {
+        r[i] = static_cast<T>(sc[i] * r64[i] * scale);
+        y[i] = T(0);
+        // a restart stores zeros (the buffers may hold anything, NaN included, before the first)
+        if (mode == 2)
+        {
+            P0[i] = T(0);
+            P1[i] = T(0);
+            q[i] = T(0);
+        }
+        else
+        {
+            pc[i] *= f;
+            q[i] *= f;
+        }
+    }
+}
+
+// per chunk: the partials of r.q, p.q and r.r (p the current direction)
+template <typename T>
+__global__ __launch_bounds__(BT) void k_cg_fresh_dots(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
+                                                      const int *__restrict__ sel, const T *__restrict__ r, const T *P0,
+                                                      const T *P1, const T *__restrict__ q, const CGSys *st, double *c3)
+{
+    __shared__ double red[3 * (BT / 64)];
+    const int c = blockIdx.x, sys = cpid[c];
+    if (!sel[sys]) return;
+    const T *pc = (st[sys].it & 1) ? P1 : P0;
+    double a = 0.0, b = 0.0, d = 0.0;
+    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
+    {
+        const double ri = r[i], qi = q[i];
+        a += ri * qi;
+        b += static_cast<double>(pc[i]) * qi;
+        d += ri * ri;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        d += __shfl_xor(d, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        red[3 * (threadIdx.x >> 6)] = a;
+        red[3 * (threadIdx.x >> 6) + 1] = b;
+        red[3 * (threadIdx.x >> 6) + 2] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t[3] = {0.0, 0.0, 0.0};
+        for (int w = 0; w < BT / 64; ++w)
+            for (int j = 0; j < 3; ++j) t[j] += red[3 * w + j];
+        for (int j = 0; j < 3; ++j) c3[3 * c + j] = t[j];
+    }
+}
+
+// one workgroup per system: r.q, p.q, r.r over its chunks; beta that makes the next direction
+// A-conjugate to the current one (0 on a restart), the segment's threshold, running again
+__global__ __launch_bounds__(BT) void k_cg_fresh_state(const int *__restrict__ cptr, const int *__restrict__ sel,
+                                                       const double *__restrict__ c3, const double *__restrict__ rho_new,
+                                                       const double *__restrict__ tgt, double tol2, int max_iter, CGSys *st)
+{
+    const int sys = blockIdx.x, mode = sel[sys];
+    if (!mode) return;
+    __shared__ double red[BT / 64];
+    const int c0 = cptr[sys], c1 = cptr[sys + 1];
+    const double rq = wg_fixed_sum<BT>(c3 + 3 * c0, c1 - c0, red, 3);
+    const double pq = wg_fixed_sum<BT>(c3 + 3 * c0 + 1, c1 - c0, red, 3);
+    const double rr = wg_fixed_sum<BT>(c3 + 3 * c0 + 2, c1 - c0, red, 3);
+    if (threadIdx.x != 0) return;
+    CGSys &S = st[sys];
+    const double rn = rho_new[sys];
+    if (mode == 2)
+    {
+        S.it = 0;
+        S.max_iter = max_iter;
+        S.beta = 0.0;
+    }
+    else S.beta = pq > 0.0 ? -rq / pq : 0.0;
+    S.tol2 = tol2;
+    S.tgt = tgt ? tgt[sys] : 0.0;
+    S.rr = rr;
+    S.thr = tol2 * rr;
+    S.rho = rn;
+    S.reason = 0;
+    S.active = (rn > 0.0 && rr > 0.0) ? 1 : 0;
+}
+
+__global__ void k_cg_count(int np, const CGSys *st, const int *__restrict__ retired, int *out)
+{
+    __shared__ int cnt[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int run = 0, paused = 0;
+    for (int s = threadIdx.x; s < np; s += blockDim.x)
+    {
+        if (st[s].active) ++run;
+        else if (!retired[s] && st[s].reason) ++paused;
+    }
+    atomicAdd(&cnt[0], run);
+    atomicAdd(&cnt[1], paused);
+    __syncthreads();
+    if (threadIdx.x == 0) { out[0] = cnt[0]; out[1] = cnt[1]; }
+}
+
+// ---- sampled timing of the product kernel ------------------------------------------------------
+struct Sample {
+    hipEvent_t e0, e1;
+};
+struct InnerTimer {
+    std::mutex mu;
+    bool on = false;
+    int64_t launches = 0;
+    std::vector<Sample> samples;
+    size_t used = 0;
+};
+InnerTimer g_timer;
+
+// Events for this product launch, or false (timer off, every 8th launch only, never in capture).
+bool timer_sample(hipStream_t s, hipEvent_t *e0, hipEvent_t *e1)
+{
+    std::lock_guard<std::mutex> lk(g_timer.mu);
+    if (!g_timer.on) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    if ((g_timer.launches++ & 7) != 0) return false;
+    if (g_timer.used == g_timer.samples.size())
+    {
+        Sample x{nullptr, nullptr};
+        HIPC(hipEventCreate(&x.e0));
+        HIPC(hipEventCreate(&x.e1));
+        g_timer.samples.push_back(x);
+    }
+    const Sample &x = g_timer.samples[g_timer.used++];
+    *e0 = x.e0;
+    *e1 = x.e1;
+    return true;
+}
+
+bool timer_on()
+{
+    std::lock_guard<std::mutex> lk(g_timer.mu);
+    return g_timer.on;
+}
+
+} // namespace
+
+// ---- scaled values (cached on the matrix) --------------------------------------------------
+void scaled_values(femd_csr_s *A, hipStream_t s)
+{
+    if (A->v32s_valid) return;
+    DevPattern &P = *A->pat;
+    if (P.n != P.m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
+    const int n = P.n;
+    const int *diag = diag_of(P, s);
+    A->scale.ensure(sizeof(double) * std::max(n, 1));
+    A->diagf.ensure(sizeof(float) * std::max(n, 1));
+    A->v32s.ensure(sizeof(float) * std::max<int64_t>(P.nnz, 1));
+    DevBuf bad(sizeof(int));
+    const int big = 0x7fffffff;
+    HIPC(hipMemcpyAsync(bad.p, &big, sizeof(int), hipMemcpyHostToDevice, s));
+    if (n > 0)
+    {
+        hipLaunchKernelGGL(k_scale_of_diag, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, diag, A->v.as<double>(),
+                           A->scale.as<double>(), A->diagf.as<float>(), bad.as<int>());
+        hipLaunchKernelGGL(k_scaled_values<float>, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, P.ip.as<int>(), P.ix.as<int>(),
+                           A->v.as<double>(), A->scale.as<double>(), A->v32s.as<float>());
+    }
+    int badrow = big;
+    HIPC(hipMemcpyAsync(&badrow, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (badrow != big)
+        fail(FEMD_ENOTSPD, "inner CG: diagonal entry of row " + std::to_string(badrow) + " is not positive");
+    A->v32s_valid = true;
+}
+
+void scaled_values64(femd_csr_s *A, hipStream_t s)
+{
+    scaled_values(A, s);
+    if (A->v64s_valid) return;
+    const DevPattern &P = *A->pat;
+    A->v64s.ensure(sizeof(double) * std::max<int64_t>(P.nnz, 1));
+    if (P.n > 0)
+        hipLaunchKernelGGL(k_scaled_values<double>, dim3((P.n + BT - 1) / BT), dim3(BT), 0, s, P.n, P.ip.as<int>(),
+                           P.ix.as<int>(), A->v.as<double>(), A->scale.as<double>(), A->v64s.as<double>());
+    A->v64s_valid = true;
+}
+
+// ---- the engine ------------------------------------------------------------------------------
+CGEngine::~CGEngine() { drop_graph(); }
+
+void CGEngine::drop_graph()
+{
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    graph_stream = nullptr;
+}
+
+void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s)
+{
+    if (prec != FEMD_FP32 && prec != FEMD_FP64) fail(FEMD_EINVAL, "inner CG: precision is FEMD_FP32 or FEMD_FP64");
+    if (M->pat->n != M->pat->m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
+    if (prec == FEMD_FP64) scaled_values64(M, s);
+    else scaled_values(M, s);
+    if (A == M && precision == prec && plan) return;
+    drop_graph();
+    A = M;
+    precision = prec;
+    bind(M, prec, s);
+}
+
+void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
+{
+    DevPattern &P = *M->pat;
+    n = P.n;
+    roff = P.segments.empty() ? std::vector<int64_t>{0, int64_t(P.n)} : P.segments;
+    np = static_cast<int>(roff.size()) - 1;
+    std::vector<int64_t> lo{0};
+    std::vector<int> cp, cptr_h{0};
+    max_rows = 0;
+    for (int p = 0; p < np; ++p)
+    {
+        const int64_t r0 = roff[p], nr = roff[p + 1] - roff[p];
+        max_rows = std::max(max_rows, nr);
+        for (int64_t k = 0; k < nr; k += BCHUNK)
+        {
+            lo.push_back(r0 + std::min<int64_t>(k + BCHUNK, nr));
+            cp.push_back(p);
+        }
+        cptr_h.push_back(static_cast<int>(cp.size()));
+    }
+    nch = static_cast<int>(cp.size());
+    h_cptr = cptr_h;
+    clo = upload(lo, s);
+    cpid = upload(cp.empty() ? std::vector<int>{0} : cp, s);
+    cptr = upload(cptr_h, s);
+    plan = &spmv_plan(P, prec, prec == FEMD_FP64 ? static_cast<const void *>(M->v64s.p) : M->v32s.p, s);
+    seg_tiles = upload(plan->seg_tiles, s);
+    const size_t N = size_t(std::max<int64_t>(n, 1)), w = prec == FEMD_FP64 ? 8 : 4;
+    st = DevBuf(sizeof(CGSys) * size_t(std::max(np, 1)));
+    tparts = DevBuf(8 * size_t(std::max(plan->ntiles, 1)));
+    cparts = DevBuf(24 * size_t(std::max(nch, 1)));
+    r = DevBuf(w * N); q = DevBuf(w * N); P0 = DevBuf(w * N); P1 = DevBuf(w * N); y = DevBuf(w * N);
+    r64 = DevBuf(8 * N); t64 = DevBuf(8 * N);
+    sel = DevBuf(4 * size_t(std::max(np, 1))); retired = DevBuf(4 * size_t(std::max(np, 1)));
+    rho_new = DevBuf(8 * size_t(std::max(np, 1)));
+    tgt = DevBuf(8 * size_t(std::max(np, 1)));
+    cnt = DevBuf(2 * sizeof(int));
+    std::vector<CGSys> z(size_t(std::max(np, 1)));
+    std::memset(z.data(), 0, sizeof(CGSys) * z.size());
+    HIPC(hipMemcpyAsync(st.p, z.data(), sizeof(CGSys) * z.size(), hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+}
+
+namespace {
+
+template <typename T>
+void iterate_t(CGEngine &E, int count, hipStream_t s)
+{
+    const DevPattern &P = *E.A->pat;
+    const SpmvPlan &pl = *E.plan;
+    int th = 0, cap = 0;
+    plan_shape(pl, &th, &cap);
+    const int2 *blk = pl.blocks.as<int2>();
+    const int *uptr = pl.uptr.as<int>(), *ucol = pl.ucol.as<int>();
+    const unsigned short *lidx = pl.lidx.as<unsigned short>();
+    const int *tsys = pl.tile_sys.p ? pl.tile_sys.as<int>() : nullptr;
+    const T *v = sizeof(T) == 8 ? reinterpret_cast<const T *>(E.A->v64s.p) : reinterpret_cast<const T *>(E.A->v32s.p);
+    CGSys *st = E.st.as<CGSys>();
+    for (int k = 0; k < count; ++k)
+    {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (!timer_sample(s, &e0, &e1)) e0 = e1 = nullptr;
+#define FEMD_CASE(TH, CP)                                                                                             \
+        if constexpr (2 * CP * sizeof(T) <= 65536)                                                                    \
+            if (th == TH && cap == CP)                                                                                \
+            {                                                                                                         \
+                if (e0)                                                                                               \
+                    hipExtLaunchKernelGGL((k_cg_product<T, TH, CP>), dim3(pl.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, \
+                                          ucol, lidx, P.ip.as<int>(), P.ix.as<int>(), v, E.r.as<T>(), E.P0.as<T>(),   \
+                                          E.P1.as<T>(), E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(),              \
+                                          static_cast<const CGSys *>(st));                                            \
+                else                                                                                                  \
+                    hipLaunchKernelGGL((k_cg_product<T, TH, CP>), dim3(pl.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx, \
+                                       P.ip.as<int>(), P.ix.as<int>(), v, E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(),     \
+                                       E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(), static_cast<const CGSys *>(st)); \
+            }
+        FEMD_SHAPES(FEMD_CASE)
+#undef FEMD_CASE
+        hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(BT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), E.y.as<T>(),
+                           E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(),
+                           E.cparts.as<double>(), static_cast<const CGSys *>(st));
+        hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(BT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), st);
+    }
+}
+
+template <typename T>
+void resume_t(CGEngine &E, const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev,
+              double tol2, int max_iter, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_rescale<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev, rho_dev,
+                       E.A->scale.as<double>(), r64_dev, E.r.as<T>(), E.y.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(),
+                       static_cast<const CGSys *>(E.st.as<CGSys>()));
+    hipLaunchKernelGGL(k_cg_fresh_dots<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev,
+                       E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), static_cast<const CGSys *>(E.st.as<CGSys>()),
+                       E.cparts.as<double>());
+    hipLaunchKernelGGL(k_cg_fresh_state, dim3(E.np), dim3(BT), 0, s, E.cptr.as<int>(), sel_dev, E.cparts.as<double>(), rho_dev,
+                       tgt_dev, tol2, max_iter, E.st.as<CGSys>());
+}
+
+template <typename T>
+void finish_t(CGEngine &E, const int *sel_dev, double *x, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_finish<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev,
+                       E.A->scale.as<double>(), E.y.as<T>(), static_cast<const CGSys *>(E.st.as<CGSys>()), x);
+}
+
+} // namespace
+
+void CGEngine::iterate(int count, hipStream_t s)
+{
+    if (precision == FEMD_FP64) iterate_t<double>(*this, count, s);
+    else iterate_t<float>(*this, count, s);
+}
+
+void CGEngine::replay(bool use_graph, hipStream_t s)
+{
+    if (!use_graph)
+    {
+        iterate(GRAPH_ITERS, s);
+        return;
+    }
+    if (!exec || graph_stream != s)
+    {
+        drop_graph();
+        HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        iterate(GRAPH_ITERS, s);
+        HIPC(hipStreamEndCapture(s, &graph));
+        HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        graph_stream = s;
+    }
+    HIPC(hipGraphLaunch(exec, s));
+}
+
+void CGEngine::resume(const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev, double tol2,
+                      int max_iter, hipStream_t s)
+{
+    if (precision == FEMD_FP64) resume_t<double>(*this, r64_dev, rho_dev, tgt_dev, sel_dev, tol2, max_iter, s);
+    else resume_t<float>(*this, r64_dev, rho_dev, tgt_dev, sel_dev, tol2, max_iter, s);
+}
+
+void CGEngine::start_device(const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev,
+                            double inner_tol, int max_iter, hipStream_t s)
+{
+    resume(r64_dev, rho_dev, tgt_dev, sel_dev, inner_tol * inner_tol, max_iter, s);
+}
+
+void CGEngine::finish_device(const int *sel_dev, double *x, hipStream_t s)
+{
+    if (precision == FEMD_FP64) finish_t<double>(*this, sel_dev, x, s);
+    else finish_t<float>(*this, sel_dev, x, s);
+}
+
+// Squared norms of b - A x per system (of b when x is NULL), for the systems with sel; the
+// residual stays in r64.
+std::vector<double> CGEngine::norms2(const double *b, const double *x, hipStream_t s)
+{
+    if (x)
+    {
+        const SpmvPlan &p64 = spmv_plan(*A->pat, FEMD_FP64, A->v.p, s);
+        if (A->pat->segments.empty()) spmv_run64(p64, *A->pat, A->v.as<double>(), x, t64.as<double>(), nullptr, s);
+        else spmv_run64_masked(p64, *A->pat, A->v.as<double>(), x, t64.as<double>(), sel.as<int>(), s);
+    }
+    hipLaunchKernelGGL(k_cg_residual, dim3(nch), dim3(BT), 0, s, clo.as<int64_t>(), cpid.as<int>(), sel.as<int>(), b,
+                       x ? t64.as<double>() : nullptr, r64.as<double>(), cparts.as<double>());
+    hipLaunchKernelGGL(k_cg_sys_sum, dim3(np), dim3(BT), 0, s, cptr.as<int>(), sel.as<int>(), cparts.as<double>(),
+                       rho_new.as<double>());
+    std::vector<double> h(np, 0.0);
+    HIPC(hipMemcpyAsync(h.data(), rho_new.p, 8 * size_t(np), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return h;
+}
+
+CGEngine::Result CGEngine::solve(const double *b, double *x, double tol, double inner_tol, int max_updates, bool use_graph,
+                                 int *iters, double *res, hipStream_t s)
+{
+    if (!(tol >= 0.0) || !(inner_tol > 0.0 && inner_tol < 1.0) || max_updates < 1)
+        fail(FEMD_EINVAL, "inner CG: need tol >= 0, 0 < inner_tol < 1, max_updates >= 1");
+    Result out;
+    const int max_iter = static_cast<int>(std::min<int64_t>(1 << 30, std::max<int64_t>(1000, 10 * max_rows)));
+    if (timer_on()) use_graph = false;          // sampled kernel timing needs plain launches
+    std::vector<int> selh(np, 1), retiredh(np, 0), conv(np, 0), updates(np, 0);
+    std::vector<double> rel(np, 0.0), rho(np, 0.0), tg(np, 0.0);
+    auto put = [&](DevBuf &d, const std::vector<int> &v) { HIPC(hipMemcpyAsync(d.p, v.data(), 4 * size_t(np), hipMemcpyHostToDevice, s)); };
+    put(sel, selh);
+    const std::vector<double> nb2 = norms2(b, nullptr, s);
+    for (int p = 0; p < np; ++p)
+        if (nb2[p] == 0.0 && roff[p + 1] > roff[p]) launch_fill(roff[p + 1] - roff[p], 0.0, x + roff[p], s);
+    std::vector<double> rr = norms2(b, x, s);
+    int running = 0;
+    for (int p = 0; p < np; ++p)
+    {
+        const double nb = std::sqrt(nb2[p]), rn = std::sqrt(rr[p]);
+        rel[p] = nb > 0.0 ? rn / nb : 0.0;
+        if (nb == 0.0 || rel[p] <= tol) { conv[p] = 1; retiredh[p] = 1; selh[p] = 0; }
+        else { selh[p] = 2; rho[p] = rn; tg[p] = (tol * nb / rn) * (tol * nb / rn); ++running; }
+    }
+    {
+        std::vector<CGSys> z(np);
+        std::memset(z.data(), 0, sizeof(CGSys) * np);
+        HIPC(hipMemcpyAsync(st.p, z.data(), sizeof(CGSys) * np, hipMemcpyHostToDevice, s));
+    }
+    HIPC(hipMemcpyAsync(rho_new.p, rho.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(tgt.p, tg.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
+    put(retired, retiredh);
+    put(sel, selh);
+    resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol * inner_tol, max_iter, s);
+    Pinned hc(2 * sizeof(int));
+    std::vector<CGSys> hs(np);
+    while (running > 0)
+    {
+        replay(use_graph, s);
+        hipLaunchKernelGGL(k_cg_count, dim3(1), dim3(BT), 0, s, np, static_cast<const CGSys *>(st.as<CGSys>()), retired.as<int>(),
+                           cnt.as<int>());
+        HIPC(hipMemcpyAsync(hc.p, cnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        const int run = hc.as<int>()[0], paused = hc.as<int>()[1];
+        if (paused == 0 && run > 0) continue;
+        if (paused == 0 && run == 0) break;
+        // reliable updates of the paused systems
+        ++out.rounds;
+        HIPC(hipMemcpyAsync(hs.data(), st.p, sizeof(CGSys) * np, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (int p = 0; p < np; ++p) selh[p] = (!retiredh[p] && !hs[p].active && hs[p].reason) ? 1 : 0;
+        put(sel, selh);
+        finish_device(sel.as<int>(), x, s);
+        rr = norms2(b, x, s);
+        for (int p = 0; p < np; ++p)
+        {
+            rho[p] = 0.0;
+            if (!selh[p]) continue;
+            ++updates[p];
+            const double rn = std::sqrt(rr[p]);
+            rel[p] = rn / std::sqrt(nb2[p]);
+            if (rel[p] <= tol) { conv[p] = 1; retiredh[p] = 1; selh[p] = 0; continue; }
+            if (hs[p].total >= max_iter || updates[p] >= max_updates) { retiredh[p] = 1; selh[p] = 0; continue; }
+            selh[p] = hs[p].reason == 2 ? 2 : 1;
+            rho[p] = rn;
+            tg[p] = (tol * std::sqrt(nb2[p]) / rn) * (tol * std::sqrt(nb2[p]) / rn);
+        }
+        HIPC(hipMemcpyAsync(rho_new.p, rho.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(tgt.p, tg.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
+        put(retired, retiredh);
+        put(sel, selh);
+        resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol * inner_tol, max_iter, s);
+        running = 0;
+        for (int p = 0; p < np; ++p) running += retiredh[p] ? 0 : 1;
+    }
+    HIPC(hipMemcpyAsync(hs.data(), st.p, sizeof(CGSys) * np, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    int nconv = 0;
+    for (int p = 0; p < np; ++p)
+    {
+        if (iters) iters[p] = hs[p].total;
+        if (res) res[p] = rel[p];
+        nconv += conv[p];
+        out.worst = std::max(out.worst, rel[p]);
+        out.iterations += hs[p].total;
+        out.updates += updates[p];
+    }
+    out.converged = nconv == np;
+    return out;
+}
+
+} // namespace femd_amd
+
+extern "C" {
+
+int femd_inner_timer(int enable)
+{
+    using namespace femd_amd;
+    return api([&] {
+        std::lock_guard<std::mutex> lk(g_timer.mu);
+        g_timer.on = enable != 0;
+        g_timer.used = 0;
+        g_timer.launches = 0;
+    });
+}
+
+int femd_inner_timer_read(int64_t *launches, double *total_ms)
+{
+    using namespace femd_amd;
+    return api([&] {
+        std::lock_guard<std::mutex> lk(g_timer.mu);
+        double t = 0.0;
+        int64_t k = 0;
+        for (size_t i = 0; i < g_timer.used; ++i)
+        {
+            const Sample &x = g_timer.samples[i];
+            HIPC(hipEventSynchronize(x.e1));
+            float ms = 0.0f;
+            HIPC(hipEventElapsedTime(&ms, x.e0, x.e1));
+            // a launch of a finished solve returns at once: keep the ones that did the product
+            if (ms > 0.002f)
+            {
+                t += ms;
+                ++k;
+            }
+        }
+        if (launches) *launches = k;
+        if (total_ms) *total_ms = t;
+    });
+}
+
+} // extern "C"

@@ -118,15 +118,17 @@ struct SpmvPlan {
     int ntiles = 0;
     DevBuf blocks;                 // int2 per tile boundary: (first row, first entry)
     DevBuf uptr, ucol, lidx;       // each tile's distinct columns, and every entry's position in them
+    DevBuf tile_sys;               // the segment (system) of every tile, when the pattern has segments
+    std::vector<int> seg_tiles;    // tiles of segment k: [seg_tiles[k], seg_tiles[k + 1])
     double ms = 0.0;
 };
 
 struct DevPattern {
     int n = 0, m = 0;
     int64_t nnz = 0;
+    std::vector<int64_t> segments;     // row ranges no SpMV tile may straddle (the batch's systems)
     std::vector<int> h_ip, h_ix;
-    DevBuf ip, ix, blocks, diag;
-    int nblocks = 0;
+    DevBuf ip, ix, diag;
     bool have_diag = false;
     std::unique_ptr<SpmvPlan> plan64, plan32;    // tuned launch shapes per precision
 };
@@ -138,6 +140,11 @@ void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const d
                 hipStream_t s);
 void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, const int *stop,
                 hipStream_t s);
+// The FP64 product on the tiles whose segment k has mask[k] != 0 only (segmented patterns).
+void spmv_run64_masked(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, const int *mask,
+                       hipStream_t s);
+// threads and LDS entries of a plan's launch shape
+void plan_shape(const SpmvPlan &p, int *threads, int *cap);
 const char *spmv_variant_name(const SpmvPlan &p);
 extern const int NSHAPES_PUBLIC;
 const char *shape_name(int i);
@@ -176,7 +183,10 @@ struct femd_csr_s {
     femd_amd::DevBuf v, v32;
     bool v32_valid = false;
     femd_amd::DevBuf v32s, scale;        // S A S in FP32 and S = diag(A)^(-1/2) (FP64), for the inner CG
+    femd_amd::DevBuf diagf;              // diag(A) in FP32 (the inner CG's estimate of the true residual)
     bool v32s_valid = false;
+    femd_amd::DevBuf v64s;               // S A S in FP64 (the batched FP64 CG)
+    bool v64s_valid = false;
     const femd_space_s *space = nullptr;
 };
 

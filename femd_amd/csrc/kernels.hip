@@ -672,122 +672,6 @@ __global__ void k_lincomb(int64_t n, int k, const double *y, VecPtrs V, double *
     x[i] = s;
 }
 
-// ============================================================================ FP32 CG
-__global__ __launch_bounds__(TPB) void k_r32_init(int64_t n, const double *r64, double scale, float *r, float *x,
-                                                  float *p, const float *inv, double *parts)
-{
-    __shared__ double sh[TPB];
-    double s = 0.0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
-    {
-        const float ri = static_cast<float>(r64[i] * scale);
-        const float pi = inv[i] * ri;
-        r[i] = ri; x[i] = 0.0f; p[i] = pi;
-        s += static_cast<double>(ri) * pi;
-    }
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
-}
-
-__global__ __launch_bounds__(TPB) void k_cg32_start(const double *parts, int np, CG32State *st)
-{
-    __shared__ double sh[TPB];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += TPB) s += parts[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0)
-    {
-        st->rz = s; st->it = 0; st->stop = 0; st->breakdown = 0;
-        if (!(s > 0.0)) st->stop = 1;
-    }
-}
-
-__global__ __launch_bounds__(TPB) void k_fdot32(int64_t n, const float *x, const float *y, double *parts, const int *stop)
-{
-    if (stop && *stop) return;
-    __shared__ double sh[TPB];
-    double s = 0.0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
-        s += static_cast<double>(x[i]) * y[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
-}
-
-__global__ __launch_bounds__(TPB) void k_cg32_alpha(const double *parts, int np, CG32State *st)
-{
-    if (st->stop) return;
-    __shared__ double sh[TPB];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += TPB) s += parts[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0)
-    {
-        if (!(s > 0.0)) { st->breakdown = 1; st->stop = 1; }
-        else st->alpha = st->rz / s;
-    }
-}
-
-__global__ __launch_bounds__(TPB) void k_cg32_update(int64_t n, float *x, float *r, const float *p, const float *Ap,
-                                                     const float *inv, double *parts, const CG32State *st)
-{
-    if (st->stop) return;
-    __shared__ double sh[TPB];
-    const float a = static_cast<float>(st->alpha);
-    double srr = 0.0, srz = 0.0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * TPB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * TPB)
-    {
-        x[i] = fmaf(a, p[i], x[i]);
-        const float ri = fmaf(-a, Ap[i], r[i]);
-        r[i] = ri;
-        const double e = static_cast<double>(ri) * ri;
-        srr += e;
-        srz += e * inv[i];
-    }
-    srr = block_sum(srr, sh);
-    srz = block_sum(srz, sh);
-    if (threadIdx.x == 0) { parts[blockIdx.x] = srr; parts[FAST_MAX_BLOCKS + blockIdx.x] = srz; }
-}
-
-__global__ __launch_bounds__(TPB) void k_cg32_beta(const double *parts, int np, CG32State *st)
-{
-    if (st->stop) return;
-    __shared__ double sh[TPB];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < np; i += TPB) { a += parts[i]; b += parts[FAST_MAX_BLOCKS + i]; }
-    a = block_sum(a, sh);
-    b = block_sum(b, sh);
-    if (threadIdx.x == 0)
-    {
-        st->rr = a;
-        st->it += 1;
-        if (a <= st->thr) { st->stop = 1; return; }
-        st->beta = b / st->rz;
-        st->rz = b;
-        if (st->it >= st->max_iter) st->stop = 1;
-        else if (!(b > 0.0)) { st->breakdown = 1; st->stop = 1; }
-    }
-}
-
-__global__ void k_cg32_pupdate(int64_t n, float *p, const float *r, const float *inv, const CG32State *st)
-{
-    if (st->stop) return;
-    const float beta = static_cast<float>(st->beta);
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = fmaf(beta, p[i], inv[i] * r[i]);
-}
-
-__global__ void k_inv32(int n, const int *diag, const double *v, float *inv)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) inv[i] = static_cast<float>(1.0 / v[diag[i]]);
-}
-
-__global__ void k_add_scaled32(int64_t n, double scale, const float *d, double *x)
-{
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += scale * static_cast<double>(d[i]);
-}
-
 // ============================================================================ launchers
 #define FEMD_LAUNCH(kernel, grid, block, shmem, stream, ...) \
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, __VA_ARGS__)
@@ -982,34 +866,5 @@ void launch_lincomb(int64_t n, int k, const double *y, const double *const *V, d
     }
 }
 
-void launch_r32_init(int64_t n, const double *r64, double scale, float *r, float *x, float *p, const float *inv,
-                     double *parts, hipStream_t s)
-{
-    FEMD_LAUNCH(k_r32_init, fast_blocks(n), TPB, 0, s, n, r64, scale, r, x, p, inv, parts);
-}
-void launch_cg32_start(const double *parts, int np, CG32State *st, hipStream_t s) { FEMD_LAUNCH(k_cg32_start, 1, TPB, 0, s, parts, np, st); }
-void launch_fdot32(int64_t n, const float *x, const float *y, double *parts, const int *stop, hipStream_t s)
-{
-    FEMD_LAUNCH(k_fdot32, fast_blocks(n), TPB, 0, s, n, x, y, parts, stop);
-}
-void launch_cg32_alpha(const double *parts, int np, CG32State *st, hipStream_t s) { FEMD_LAUNCH(k_cg32_alpha, 1, TPB, 0, s, parts, np, st); }
-void launch_cg32_update(int64_t n, float *x, float *r, const float *p, const float *Ap, const float *inv, double *parts,
-                        CG32State *st, hipStream_t s)
-{
-    FEMD_LAUNCH(k_cg32_update, fast_blocks(n), TPB, 0, s, n, x, r, p, Ap, inv, parts, st);
-}
-void launch_cg32_beta(const double *parts, int np, CG32State *st, hipStream_t s) { FEMD_LAUNCH(k_cg32_beta, 1, TPB, 0, s, parts, np, st); }
-void launch_cg32_pupdate(int64_t n, float *p, const float *r, const float *inv, const CG32State *st, hipStream_t s)
-{
-    if (n > 0) FEMD_LAUNCH(k_cg32_pupdate, grid_for(n), TPB, 0, s, n, p, r, inv, st);
-}
-void launch_inv32(int n, const int *diag, const double *v, float *inv, hipStream_t s)
-{
-    if (n > 0) FEMD_LAUNCH(k_inv32, grid_for(n), TPB, 0, s, n, diag, v, inv);
-}
-void launch_add_scaled32(int64_t n, double scale, const float *d, double *x, hipStream_t s)
-{
-    if (n > 0) FEMD_LAUNCH(k_add_scaled32, grid_for(n), TPB, 0, s, n, scale, d, x);
-}
 
 } // namespace femd_amd

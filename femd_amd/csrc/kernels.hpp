@@ -84,20 +84,4 @@ void launch_axpy(int64_t n, double alpha, const double *x, double *y, hipStream_
 void launch_scale(int64_t n, double alpha, double *x, hipStream_t s);
 void launch_lincomb(int64_t n, int k, const double *y, const double *const *V, double *x, hipStream_t s);
 
-struct CG32State {
-    double rz, alpha, beta, rr, thr;
-    int it, max_iter, stop, breakdown;
-};
-void launch_r32_init(int64_t n, const double *r64, double scale, float *r, float *x, float *p, const float *inv,
-                     double *parts, hipStream_t s);
-void launch_cg32_start(const double *parts, int np, CG32State *st, hipStream_t s);
-void launch_fdot32(int64_t n, const float *x, const float *y, double *parts, const int *stop, hipStream_t s);
-void launch_cg32_alpha(const double *parts, int np, CG32State *st, hipStream_t s);
-void launch_cg32_update(int64_t n, float *x, float *r, const float *p, const float *Ap, const float *inv, double *parts,
-                        CG32State *st, hipStream_t s);
-void launch_cg32_beta(const double *parts, int np, CG32State *st, hipStream_t s);
-void launch_cg32_pupdate(int64_t n, float *p, const float *r, const float *inv, const CG32State *st, hipStream_t s);
-void launch_inv32(int n, const int *diag, const double *v, float *inv, hipStream_t s);
-void launch_add_scaled32(int64_t n, double scale, const float *d, double *x, hipStream_t s);          // x += scale * d
-
 } // namespace femd_amd

@@ -1,315 +1,18 @@
 // spmv.hip -- the CSR SpMV tile, its launch shapes, the autotuner that picks one per matrix and
-// precision, and the product fused into the FP32 inner CG.
+// precision.
 //
-// The tile: a run of whole rows whose entries fit the LDS buffer (CAP entries, at most THREADS
-// rows), one workgroup each.  Finite-element rows of one tile share most of their columns (a
-// P3 tile of 2048 entries touches about 480 distinct ones), so the tile's distinct columns are
-// listed ahead of time (sorted, per pattern) and every entry carries a 16-bit position in that
-// list instead of its 32-bit column.  The workgroup
-//   1. loads the list and gathers x once per distinct column into LDS (xs), while every lane
-//      loads groups of four consecutive entries -- values and local positions with vector loads;
-//   2. forms the products v[p] * xs[lidx[p]] into LDS;
-//   3. adds each row's products left to right from zero, one lane per row: the order of
-//      CSRMatrix::apply (sparse/csr.hpp:203-215), so the FP64 product is bit-identical to the
-//      reference's.
-// Per entry the matrix stream is 8 + 2 (FP64) or 4 + 2 (FP32) bytes, and the scattered gathers
-// shrink to one per distinct column: the per-lane gather, not HBM, limits a plain CSR tile here.
+// The tile kernels themselves are in tile.hpp.
 //
 // Launch shapes (THREADS, CAP) are the tuning space; the autotuner times each on the matrix
 // itself with HIP events the first time a pattern is used in a precision and keeps the fastest.
 //
-// The product of the FP32 CG (k_scg_spmv) is the same tile on r, writing w = A r and leaving the
-// partials of r.r and w.r (and, after a residual replacement, w.p and p.s) per workgroup
-// (cg32.hip).
-#include "cg32.hpp"
-
-#include <hip/hip_ext.h>
+// The product of the inner CG (cg_engine.hip) is the same tile with the direction update fused in.
+#include "tile.hpp"
 
 #include <cfloat>
 
 namespace femd_amd {
 namespace {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct Vals4;
-template <> struct Vals4<float> {
-    f32x4 a;
-    __device__ void load(const float *v, int p) { a = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(v + p)); }
-    __device__ float get(int j) const { return a[j]; }
-    __device__ void set(int j, float t) { a[j] = t; }
-};
-template <> struct Vals4<double> {
-    f64x2 a, b;
-    __device__ void load(const double *v, int p)
-    {
-        a = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(v + p));
-        b = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(v + p + 2));
-    }
-    __device__ double get(int j) const { return j < 2 ? a[j] : b[j - 2]; }
-    __device__ void set(int j, double t) { if (j < 2) a[j] = t; else b[j - 2] = t; }
-};
-
-// One tile (workgroup blockIdx.x).  Op supplies gather(j) (the x value at global column j),
-// own(r) (what the row epilogue needs of row r) and store(r, s, own) (writes row r's result and
-// returns its share of the workgroup's partials, of type Op::Part).  Returns this thread's share.
-// `stop` (may be NULL) is read together with the tile's bounds: a set flag ends the workgroup
-// before it loads the matrix, and its latency hides behind theirs.  *stopped tells the caller.
-template <typename T, int THREADS, int CAP, class Op>
-__device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr, const int *__restrict__ ucol,
-                                     const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
-                                     const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op, T *xs,
-                                     T *prod, const int *stop, bool *stopped)
-{
-    constexpr int NGRP = (CAP / 4 + 1 + THREADS - 1) / THREADS;
-    constexpr int NU = (CAP + THREADS - 1) / THREADS;
-    const int b = blockIdx.x;
-    const int2 b0 = blk[b], b1 = blk[b + 1];
-    const int halt = stop ? *stop : 0;
-    *stopped = halt != 0;
-    if (halt) return typename Op::Part{};
-    const int r0 = b0.x, r1 = b1.x, p0 = b0.y, len = b1.y - p0;
-    const int r = r0 + static_cast<int>(threadIdx.x);
-    int rb = 0, re = 0;
-    typename Op::Own own{};
-    if (r < r1)
-    {
-        rb = ip[r] - p0;
-        re = ip[r + 1] - p0;
-        own = op.own(r);
-    }
-    if (len > CAP)
-    {
-        // one row longer than the buffer: one lane, from global memory, in order
-        typename Op::Part part{};
-        if (threadIdx.x == 0)
-        {
-            T s = 0;
-            for (int p = p0; p < p0 + len; ++p)
-            {
-                const T t = v[p] * op.gather(ix[p]);
-                s += t;
-            }
-            part = op.store(r0, s, own);
-        }
-        return part;
-    }
-    // 1. distinct columns and entry loads
-    const int u0 = uptr[b], nu = uptr[b + 1] - u0;
-    int uc[NU];
-#pragma unroll
-    for (int i = 0; i < NU; ++i)
-    {
-        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
-        uc[i] = u < nu ? __builtin_nontemporal_load(ucol + u0 + u) : 0;
-    }
-    const int a0 = p0 & ~3, e = p0 + len;
-    Vals4<T> vv[NGRP];
-    u16x4 cc[NGRP];
-#pragma unroll
-    for (int g = 0; g < NGRP; ++g)
-    {
-        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
-        if (a < e)
-        {
-            if (int64_t(a) + 4 <= nnz)
-            {
-                vv[g].load(v, a);
-                cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
-            }
-            else
-            {
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                {
-                    const bool in = int64_t(a) + j < nnz;
-                    vv[g].set(j, in ? v[a + j] : T(0));
-                    cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
-                }
-            }
-        }
-        else cc[g] = u16x4{0, 0, 0, 0};
-    }
-    T xg[NU];
-#pragma unroll
-    for (int i = 0; i < NU; ++i)
-    {
-        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
-        xg[i] = u < nu ? op.gather(uc[i]) : T(0);
-    }
-#pragma unroll
-    for (int i = 0; i < NU; ++i)
-    {
-        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
-        if (u < nu) xs[u] = xg[i];
-    }
-    __syncthreads();
-    // 2. products
-#pragma unroll
-    for (int g = 0; g < NGRP; ++g)
-    {
-        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
-        if (a < e)
-        {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-            {
-                const int k = a + j - p0;
-                if (k >= 0 && k < len) prod[k] = vv[g].get(j) * xs[cc[g][j]];
-            }
-        }
-    }
-    __syncthreads();
-    // 3. row sums, left to right
-    typename Op::Part part{};
-    if (r < r1)
-    {
-        T s = 0;
-        for (int k = rb; k < re; ++k) s += prod[k];
-        part = op.store(r, s, own);
-    }
-    return part;
-}
-
-// Sum of v over the workgroup in a fixed order (wave shuffles, then the waves in order); valid in
-// thread 0.
-template <int THREADS>
-__device__ inline double wg_sum(double v, double *red)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < THREADS / 64; ++w) t += red[w];
-    return t;
-}
-
-template <typename T> struct PlainOp {
-    using Own = int;
-    using Part = int;
-    const T *x;
-    T *y;
-    __device__ T gather(int j) const { return x[j]; }
-    __device__ Own own(int) const { return 0; }
-    __device__ Part store(int r, T s, Own) const
-    {
-        y[r] = s;
-        return 0;
-    }
-};
-
-template <typename T, int THREADS, int CAP>
-__global__ __launch_bounds__(THREADS) void k_spmv_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr,
-                                                       const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
-                                                       const int *__restrict__ ip, const int *__restrict__ ix,
-                                                       const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y,
-                                                       int64_t nnz, const int *stop)
-{
-    __shared__ T xs[CAP];
-    __shared__ T prod[CAP];
-    bool stopped = false;
-    (void)stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, PlainOp<T>{x, y}, xs, prod, stop, &stopped);
-}
-
-// ------------------------------------------------------------------ product of the FP32 CG
-// w = A r, and over the workgroup's own rows the partials of r.r and w.r; on the first
-// iteration of a segment (after the residual was replaced) also w.p and p.s, from which the
-// scalar step forms p'.A p' exactly.
-struct Dot4 {
-    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-    __device__ Dot4 &operator+=(const Dot4 &o)
-    {
-        a += o.a;
-        b += o.b;
-        c += o.c;
-        d += o.d;
-        return *this;
-    }
-};
-struct CGOwn {
-    float r = 0.0f, p = 0.0f, s = 0.0f;
-};
-struct CGOp {
-    using Own = CGOwn;
-    using Part = Dot4;
-    const float *r;
-    const float *p, *sv;      // NULL unless the exact p'.A p' is wanted
-    float *w;
-    __device__ float gather(int j) const { return r[j]; }
-    __device__ Own own(int i) const
-    {
-        CGOwn o;
-        o.r = r[i];
-        if (p)
-        {
-            o.p = p[i];
-            o.s = sv[i];
-        }
-        return o;
-    }
-    __device__ Part store(int i, float s, Own o) const
-    {
-        w[i] = s;
-        Dot4 d;
-        d.a = static_cast<double>(o.r) * static_cast<double>(o.r);
-        d.b = static_cast<double>(s) * static_cast<double>(o.r);
-        d.c = static_cast<double>(s) * static_cast<double>(o.p);
-        d.d = static_cast<double>(o.p) * static_cast<double>(o.s);
-        return d;
-    }
-};
-
-template <int THREADS, int CAP>
-__global__ __launch_bounds__(THREADS) void k_scg_spmv(const int2 *__restrict__ blk, const int *__restrict__ uptr,
-                                                      const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
-                                                      const int *__restrict__ ip, const int *__restrict__ ix,
-                                                      const float *__restrict__ v, const float *__restrict__ r,
-                                                      const float *__restrict__ p, const float *__restrict__ sv,
-                                                      float *__restrict__ w, int64_t nnz, double *__restrict__ parts,
-                                                      const SCGState *st, int k)
-{
-    __shared__ float xs[CAP];
-    __shared__ float prod[CAP];
-    __shared__ double red[4 * (THREADS / 64)];
-    const bool exact = k > 0 && st->fresh;
-    bool stopped = false;
-    Dot4 d = stream_tile<float, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz,
-                                              CGOp{r, exact ? p : nullptr, exact ? sv : nullptr, w}, xs, prod, &st->stop,
-                                              &stopped);
-    if (stopped) return;
-    // the four sums in one pass: wave shuffles, then the waves in order
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        d.a += __shfl_xor(d.a, off, 64);
-        d.b += __shfl_xor(d.b, off, 64);
-        d.c += __shfl_xor(d.c, off, 64);
-        d.d += __shfl_xor(d.d, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0)
-    {
-        const int wv = threadIdx.x >> 6;
-        red[4 * wv] = d.a;
-        red[4 * wv + 1] = d.b;
-        red[4 * wv + 2] = d.c;
-        red[4 * wv + 3] = d.d;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        double t[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int wv = 0; wv < THREADS / 64; ++wv)
-            for (int j = 0; j < 4; ++j) t[j] += red[4 * wv + j];
-        double2 *out = reinterpret_cast<double2 *>(parts) + 2 * blockIdx.x;
-        out[0] = make_double2(t[0], t[1]);
-        out[1] = make_double2(t[2], t[3]);
-    }
-}
-
 // ---------------------------------------------------------------------------------- dispatch
 struct Shape {
     int threads, cap;
@@ -323,12 +26,12 @@ constexpr int NSHAPES = sizeof(SHAPES) / sizeof(SHAPES[0]);
 // LDS (values and products) at most 64 KiB per workgroup.
 template <typename T> bool admissible(const Shape &S) { return 2 * S.cap * sizeof(T) <= 65536; }
 
-#define FEMD_SHAPES(X) X(256, 1024) X(256, 2048) X(512, 2048) X(512, 4096)
 
 template <typename T>
 void launch_shape(const SpmvPlan &p, const Shape &S, const int *ip, const int *ix, const T *v, const T *x, T *y, int64_t nnz,
-                  const int *stop, hipStream_t s)
+                  const int *stop, const int *mask, hipStream_t s)
 {
+    const int *tile_sys = mask ? p.tile_sys.as<int>() : nullptr;
     const int2 *blk = p.blocks.as<int2>();
     const int *uptr = p.uptr.as<int>(), *ucol = p.ucol.as<int>();
     const unsigned short *lidx = p.lidx.as<unsigned short>();
@@ -337,7 +40,7 @@ void launch_shape(const SpmvPlan &p, const Shape &S, const int *ip, const int *i
     {                                                                                                              \
         if constexpr (2 * CP * sizeof(T) <= 65536)                                                                 \
             hipLaunchKernelGGL((k_spmv_tile<T, TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx, ip, ix, v, \
-                               x, y, nnz, stop);                                                                   \
+                               x, y, nnz, stop, tile_sys, mask);                                                   \
         return;                                                                                                    \
     }
     FEMD_SHAPES(FEMD_CASE)
@@ -349,8 +52,24 @@ SpmvPlan make_plan(const DevPattern &P, int si, hipStream_t s)
     SpmvPlan p;
     p.variant = si;
     const Shape &S = SHAPES[si];
-    const std::vector<int> rows = spmv_row_blocks(P.n, P.h_ip.data(), S.threads, S.cap);
+    // tiles never straddle a segment boundary
+    std::vector<int64_t> seg = P.segments;
+    if (seg.empty()) seg = {0, P.n};
+    std::vector<int> rows{0}, tsys;
+    p.seg_tiles.assign(1, 0);
+    for (size_t k = 0; k + 1 < seg.size(); ++k)
+    {
+        const int r0 = static_cast<int>(seg[k]), nr = static_cast<int>(seg[k + 1] - seg[k]);
+        const std::vector<int> b = spmv_row_blocks(nr, P.h_ip.data() + r0, S.threads, S.cap);
+        for (size_t j = 1; j < b.size(); ++j)
+        {
+            rows.push_back(r0 + b[j]);
+            tsys.push_back(static_cast<int>(k));
+        }
+        p.seg_tiles.push_back(static_cast<int>(tsys.size()));
+    }
     p.ntiles = static_cast<int>(rows.size()) - 1;
+    if (!P.segments.empty()) p.tile_sys = upload(tsys.empty() ? std::vector<int>{0} : tsys, s);
     std::vector<int> b2(2 * rows.size());
     for (size_t k = 0; k < rows.size(); ++k)
     {
@@ -374,7 +93,7 @@ template <typename T>
 void run_plan(const SpmvPlan &p, const DevPattern &P, const T *v, const T *x, T *y, const int *stop, hipStream_t s)
 {
     if (P.n <= 0 || p.ntiles <= 0) return;
-    launch_shape<T>(p, SHAPES[p.variant], P.ip.as<int>(), P.ix.as<int>(), v, x, y, P.nnz, stop, s);
+    launch_shape<T>(p, SHAPES[p.variant], P.ip.as<int>(), P.ix.as<int>(), v, x, y, P.nnz, stop, nullptr, s);
 }
 
 // Time every admissible shape on the pattern (values v, x = 0) and keep the fastest.
@@ -412,9 +131,18 @@ SpmvPlan tune(const DevPattern &P, const T *v, hipStream_t s)
 
 } // namespace
 
+int shape_index(const char *name, int precision);
+
 const SpmvPlan &spmv_plan(DevPattern &P, int precision, const void *values, hipStream_t s)
 {
     std::unique_ptr<SpmvPlan> &slot = precision == FEMD_FP32 ? P.plan32 : P.plan64;
+    if (!slot && !P.segments.empty())
+    {
+        // segmented (batched) patterns use fixed shapes, so that a segment is tiled -- and its
+        // dot products summed -- the same way whatever else is in the batch
+        slot = std::make_unique<SpmvPlan>(make_plan(P, shape_index(precision == FEMD_FP32 ? "tile256x2048" : "tile256x1024",
+                                                                   precision), s));
+    }
     if (!slot)
     {
         slot = std::make_unique<SpmvPlan>(precision == FEMD_FP32 ? tune<float>(P, static_cast<const float *>(values), s)
@@ -435,28 +163,18 @@ void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const fl
     run_plan<float>(p, P, v, x, y, stop, s);
 }
 
-int scg_spmv(const SpmvPlan &p, const DevPattern &P, const float *v, const float *r, const float *pv, const float *sv, float *w,
-             double *parts, const SCGState *st, int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+void spmv_run64_masked(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, const int *mask,
+                       hipStream_t s)
 {
-    const Shape &S = SHAPES[p.variant];
-    const int2 *blk = p.blocks.as<int2>();
-    const int *uptr = p.uptr.as<int>(), *ucol = p.ucol.as<int>();
-    const unsigned short *lidx = p.lidx.as<unsigned short>();
-    if (P.n <= 0 || p.ntiles <= 0) return 0;
-#define FEMD_CASE(TH, CP)                                                                                        \
-    if (S.threads == TH && S.cap == CP)                                                                          \
-    {                                                                                                            \
-        if (e0)                                                                                                  \
-            hipExtLaunchKernelGGL((k_scg_spmv<TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, ucol, lidx, \
-                                  P.ip.as<int>(), P.ix.as<int>(), v, r, pv, sv, w, P.nnz, parts, st, k);         \
-        else                                                                                                     \
-            hipLaunchKernelGGL((k_scg_spmv<TH, CP>), dim3(p.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx,      \
-                               P.ip.as<int>(), P.ix.as<int>(), v, r, pv, sv, w, P.nnz, parts, st, k);            \
-        return p.ntiles;                                                                                         \
-    }
-    FEMD_SHAPES(FEMD_CASE)
-#undef FEMD_CASE
-    return 0;
+    if (P.n <= 0 || p.ntiles <= 0) return;
+    if (!p.tile_sys.p) fail(FEMD_EINVAL, "masked SpMV on a pattern without segments");
+    launch_shape<double>(p, SHAPES[p.variant], P.ip.as<int>(), P.ix.as<int>(), v, x, y, P.nnz, nullptr, mask, s);
+}
+
+void plan_shape(const SpmvPlan &p, int *threads, int *cap)
+{
+    *threads = SHAPES[p.variant].threads;
+    *cap = SHAPES[p.variant].cap;
 }
 
 const char *spmv_variant_name(const SpmvPlan &p) { return SHAPES[p.variant].name; }

@@ -1,0 +1,247 @@
+// tile.hpp -- the CSR SpMV tile shared by the single-matrix kernels (spmv.hip) and the batched
+// solver (batch.hip): device code only, instantiated in each translation unit that uses it.
+//
+// The tile: a run of whole rows whose entries fit the LDS buffer (CAP entries, at most THREADS
+// rows), one workgroup each.  Finite-element rows of one tile share most of their columns (a
+// P3 tile of 2048 entries touches about 480 distinct ones), so the tile's distinct columns are
+// listed ahead of time (sorted, per pattern) and every entry carries a 16-bit position in that
+// list instead of its 32-bit column.  The workgroup
+//   1. loads the list and gathers x once per distinct column into LDS (xs), while every lane
+//      loads groups of four consecutive entries -- values and local positions with vector loads;
+//   2. forms the products v[p] * xs[lidx[p]] into LDS;
+//   3. adds each row's products left to right from zero, one lane per row: the order of
+//      CSRMatrix::apply (sparse/csr.hpp:203-215), so the FP64 product is bit-identical to the
+//      reference's.
+// Per entry the matrix stream is 8 + 2 (FP64) or 4 + 2 (FP32) bytes, and the scattered gathers
+// shrink to one per distinct column: the per-lane gather, not HBM, limits a plain CSR tile here.
+#pragma once
+#include "internal.hpp"
+
+namespace femd_amd {
+namespace {
+
+// The launch shapes (THREADS, CAP) of the tuning space.
+#define FEMD_SHAPES(X) X(256, 1024) X(256, 2048) X(512, 2048) X(512, 4096)
+
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Vals4;
+template <> struct Vals4<float> {
+    f32x4 a;
+    __device__ void load(const float *v, int p) { a = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(v + p)); }
+    __device__ float get(int j) const { return a[j]; }
+    __device__ void set(int j, float t) { a[j] = t; }
+};
+template <> struct Vals4<double> {
+    f64x2 a, b;
+    __device__ void load(const double *v, int p)
+    {
+        a = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(v + p));
+        b = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(v + p + 2));
+    }
+    __device__ double get(int j) const { return j < 2 ? a[j] : b[j - 2]; }
+    __device__ void set(int j, double t) { if (j < 2) a[j] = t; else b[j - 2] = t; }
+};
+
+// One tile (workgroup blockIdx.x).  Op supplies gather(j) (the x value at global column j),
+// own(r) (what the row epilogue needs of row r) and store(r, s, own) (writes row r's result and
+// returns its share of the workgroup's partials, of type Op::Part).  Returns this thread's share.
+// `stop` (may be NULL) is read together with the tile's bounds: a set flag ends the workgroup
+// before it loads the matrix, and its latency hides behind theirs.  With `mask` (may be NULL) the
+// tile is skipped when mask[tile_sys[b]] is zero (the batched solver: tile b belongs to system
+// tile_sys[b]).  *stopped tells the caller.
+template <typename T, int THREADS, int CAP, class Op>
+__device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr, const int *__restrict__ ucol,
+                                     const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
+                                     const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op, T *xs,
+                                     T *prod, const int *stop, bool *stopped, const int *tile_sys = nullptr,
+                                     const int *mask = nullptr)
+{
+    constexpr int NGRP = (CAP / 4 + 1 + THREADS - 1) / THREADS;
+    constexpr int NU = (CAP + THREADS - 1) / THREADS;
+    const int b = blockIdx.x;
+    const int2 b0 = blk[b], b1 = blk[b + 1];
+    const int halt = (stop && *stop) || (mask && !mask[tile_sys[b]]);
+    *stopped = halt != 0;
+    if (halt) return typename Op::Part{};
+    const int r0 = b0.x, r1 = b1.x, p0 = b0.y, len = b1.y - p0;
+    const int r = r0 + static_cast<int>(threadIdx.x);
+    int rb = 0, re = 0;
+    typename Op::Own own{};
+    if (r < r1)
+    {
+        rb = ip[r] - p0;
+        re = ip[r + 1] - p0;
+        own = op.own(r);
+    }
+    if (len > CAP)
+    {
+        // one row longer than the buffer: one lane, from global memory, in order
+        typename Op::Part part{};
+        if (threadIdx.x == 0)
+        {
+            T s = 0;
+            for (int p = p0; p < p0 + len; ++p)
+            {
+                const T t = v[p] * op.gather(ix[p]);
+                s += t;
+            }
+            part = op.store(r0, s, own);
+        }
+        return part;
+    }
+    // 1. distinct columns and entry loads
+    const int u0 = uptr[b], nu = uptr[b + 1] - u0;
+    int uc[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i)
+    {
+        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+        uc[i] = u < nu ? __builtin_nontemporal_load(ucol + u0 + u) : 0;
+    }
+    const int a0 = p0 & ~3, e = p0 + len;
+    Vals4<T> vv[NGRP];
+    u16x4 cc[NGRP];
+#pragma unroll
+    for (int g = 0; g < NGRP; ++g)
+    {
+        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
+        if (a < e)
+        {
+            if (int64_t(a) + 4 <= nnz)
+            {
+                vv[g].load(v, a);
+                cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    const bool in = int64_t(a) + j < nnz;
+                    vv[g].set(j, in ? v[a + j] : T(0));
+                    cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
+                }
+            }
+        }
+        else cc[g] = u16x4{0, 0, 0, 0};
+    }
+    T xg[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i)
+    {
+        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+        xg[i] = u < nu ? op.gather(uc[i]) : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NU; ++i)
+    {
+        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+        if (u < nu) xs[u] = xg[i];
+    }
+    __syncthreads();
+    // 2. products
+#pragma unroll
+    for (int g = 0; g < NGRP; ++g)
+    {
+        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
+        if (a < e)
+        {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int k = a + j - p0;
+                if (k >= 0 && k < len) prod[k] = vv[g].get(j) * xs[cc[g][j]];
+            }
+        }
+    }
+    __syncthreads();
+    // 3. row sums, left to right
+    typename Op::Part part{};
+    if (r < r1)
+    {
+        T s = 0;
+        for (int k = rb; k < re; ++k) s += prod[k];
+        part = op.store(r, s, own);
+    }
+    return part;
+}
+
+// Sum of v over the workgroup in a fixed order (wave shuffles, then the waves in order); valid in
+// thread 0.
+template <int THREADS>
+__device__ inline double wg_sum(double v, double *red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < THREADS / 64; ++w) t += red[w];
+    return t;
+}
+
+template <typename T> struct PlainOp {
+    using Own = int;
+    using Part = int;
+    const T *x;
+    T *y;
+    __device__ T gather(int j) const { return x[j]; }
+    __device__ Own own(int) const { return 0; }
+    __device__ Part store(int r, T s, Own) const
+    {
+        y[r] = s;
+        return 0;
+    }
+};
+
+template <typename T, int THREADS, int CAP>
+__global__ __launch_bounds__(THREADS) void k_spmv_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr,
+                                                       const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
+                                                       const int *__restrict__ ip, const int *__restrict__ ix,
+                                                       const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y,
+                                                       int64_t nnz, const int *stop, const int *tile_sys, const int *mask)
+{
+    __shared__ T xs[CAP];
+    __shared__ T prod[CAP];
+    bool stopped = false;
+    (void)stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, PlainOp<T>{x, y}, xs, prod, stop, &stopped,
+                                       tile_sys, mask);
+}
+
+// ------------------------------------------------------------------ product of the inner CG
+// The CG product with the direction update folded in: p' = r + beta p is formed from the gathered
+// (r, p) at every distinct column and for the workgroup's own rows, which it writes to the other
+// direction buffer together with q = A p'; the workgroup's partial of p'.q is returned.
+template <typename T> struct PCGOwn {
+    T r = 0, p = 0;
+};
+template <typename T> struct PCGOp {
+    using Own = PCGOwn<T>;
+    using Part = double;
+    const T *r, *pold;
+    T *pnew, *q;
+    T beta;
+    __device__ T gather(int j) const { return fma(beta, pold[j], r[j]); }
+    __device__ Own own(int i) const
+    {
+        Own o;
+        o.r = r[i];
+        o.p = pold[i];
+        return o;
+    }
+    __device__ Part store(int i, T s, Own o) const
+    {
+        const T pn = fma(beta, o.p, o.r);
+        pnew[i] = pn;
+        q[i] = s;
+        return static_cast<double>(pn) * static_cast<double>(s);
+    }
+};
+
+} // namespace
+} // namespace femd_amd

@@ -6,12 +6,14 @@
 //   b = M u (FP64, the reference's row order), bb = b.b;
 //   `rounds` times: r = b - A u (FP64), rr = r.r; unless rr <= tol^2 bb, the FP32 scaled
 //   Jacobi-CG solves A d = r / sqrt(rr) from d = 0 (warm start: the step starts from u_n) to a
-//   relative residual of inner_tol within max_inner iterations, and u += sqrt(rr) d (FP64);
+//   relative residual of inner_tol within max_inner iterations (cg_engine.hpp, one system, no
+//   starting direction), and u += sqrt(rr) d (FP64);
 //   the step's inner iterations are added to a per-step counter on the device.
 // Every scalar lives on the device and every kernel of a finished solve returns at once, so the
 // whole step is one fixed launch sequence: captured once, replayed nsteps times back to back, and
 // the host reads the counters at the end.
-#include "cg32.hpp"
+#include "cg_engine.hpp"
+#include "kernels.hpp"
 
 #include <cmath>
 
@@ -30,7 +32,15 @@ __global__ void k_gate(StepScalars *sc, double tol2)
     if (threadIdx.x == 0 && blockIdx.x == 0 && sc->rr <= tol2 * sc->bb) sc->rr = 0.0;
 }
 
-__global__ void k_record(const SCGState *st, StepScalars *sc, int *iters, int nmax)
+// the norm the inner solve scales by (0: the round is skipped) and its target (tol ||b|| / rho)^2
+__global__ void k_rho(const StepScalars *sc, double tol2, double *rho, double *tgt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *rho = sc->rr > 0.0 ? sqrt(sc->rr) : 0.0;
+    *tgt = sc->rr > 0.0 ? tol2 * sc->bb / sc->rr : 0.0;
+}
+
+__global__ void k_record(const CGSys *st, StepScalars *sc, int *iters, int nmax)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int it = st->it;
@@ -48,8 +58,8 @@ struct Stepper {
     int n = 0, rounds = 1, max_inner = 1, nsteps = 0;
     double tol = 0.0, inner_tol = 0.0;
     const SpmvPlan *pM = nullptr, *pA = nullptr;
-    ScaledCG cg;
-    DevBuf b, t, r, parts, sc, iters;
+    CGEngine cg;
+    DevBuf b, t, r, parts, sc, iters, rho, tgt, sel;
     double *u = nullptr;
 
     StepScalars *scal() const { return sc.as<StepScalars>(); }
@@ -70,10 +80,11 @@ struct Stepper {
             launch_sub(n, b.as<double>(), t.as<double>(), r.as<double>(), s);
             norm2(r.as<double>(), &scal()->rr, s);
             hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, scal(), tol * tol);
-            cg.init(r.as<double>(), &scal()->rr, inner_tol, max_inner, s);
-            cg.iterate(0, max_inner, s);
-            cg.finish(u, s);
-            hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, cg.st.as<SCGState>(), scal(), iters.as<int>(), nsteps);
+            hipLaunchKernelGGL(k_rho, dim3(1), dim3(64), 0, s, scal(), tol * tol, rho.as<double>(), tgt.as<double>());
+            cg.start_device(r.as<double>(), rho.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol, max_inner, s);
+            cg.iterate(max_inner, s);
+            cg.finish_device(sel.as<int>(), u, s);
+            hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, cg.st.as<CGSys>(), scal(), iters.as<int>(), nsteps);
         }
         hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, s, scal());
     }
@@ -96,11 +107,16 @@ void heat_run(femd_csr_s *M, femd_csr_s *A, double *u, int nsteps, double tol, i
     // everything that tunes, allocates or synchronizes happens before the capture
     T.pM = &spmv_plan(*M->pat, FEMD_FP64, M->v.p, s);
     T.pA = &spmv_plan(*A->pat, FEMD_FP64, A->v.p, s);
-    T.cg.setup(A, s);
+    T.cg.setup(A, FEMD_FP32, s);
     const size_t N = size_t(n);
     T.b = DevBuf(8 * N); T.t = DevBuf(8 * N); T.r = DevBuf(8 * N);
     T.parts = DevBuf(8 * 2048 + 64);
     T.sc = DevBuf(sizeof(StepScalars));
+    T.rho = DevBuf(sizeof(double));
+    T.tgt = DevBuf(sizeof(double));
+    T.sel = DevBuf(sizeof(int));
+    const int restart = 2;      // every inner solve starts from no direction
+    HIPC(hipMemcpyAsync(T.sel.p, &restart, sizeof(int), hipMemcpyHostToDevice, s));
     T.iters = DevBuf(sizeof(int) * size_t(std::max(nsteps, 1)));
     const StepScalars z{};
     HIPC(hipMemcpyAsync(T.sc.p, &z, sizeof(z), hipMemcpyHostToDevice, s));

@@ -242,16 +242,28 @@ int femd_solve_refined_dev(femd_csr A, const double *b_dev, double *x_dev, doubl
  * so uneven sizes share one launch sequence, replayed from a captured hipGraph when use_graph.
  * femd_batch_create copies the members' patterns and values (the members must stay alive for
  * femd_batch_update_values, which re-copies their values after a re-assembly).
- * Vectors are laid out system after system (row_offsets, np + 1 entries).  The solve is FP32
- * Jacobi-PCG with FP64 refinement per system until ||b_p - A_p x_p|| / ||b_p|| <= tol;
- * iterations[p] (inner FP32 iterations) and residuals[p] may be NULL; info.iterations sums the
- * inner iterations, info.breakdown counts refinement rounds, info.residual is the worst system's. */
+ * Vectors are laid out system after system (row_offsets, np + 1 entries).  Every system runs
+ * Jacobi-preconditioned CG (on S A S, S = diag^(-1/2)) in FP32 with reliable FP64 updates: when
+ * its r.r has dropped by inner_tol^2 its correction goes into x and its FP64 residual replaces the
+ * FP32 one; it stops when ||b_p - A_p x_p|| / ||b_p|| <= tol.  iterations[p] (inner iterations)
+ * and residuals[p] may be NULL; info.iterations sums the inner iterations, info.breakdown counts
+ * the rounds of reliable updates, info.residual is the worst system's.  Each system's result does
+ * not depend on the other systems in the batch (fixed tiling, per-system reductions). */
 typedef struct femd_batch_s *femd_batch;
 int femd_batch_create(int nsystems, const femd_csr *matrices, femd_batch *out);
 int femd_batch_info(femd_batch B, int64_t *nrows, int64_t *nnz, int64_t *row_offsets);
 int femd_batch_update_values(femd_batch B);
 int femd_batch_solve_refined(femd_batch B, const double *b, double *x, double tol, double inner_tol, int max_outer,
                              int use_graph, int *iterations, double *residuals, femd_solve_info *info);
+/* The same with the inner iterations in FEMD_FP32 (the _refined entry points) or FEMD_FP64 (S A S,
+ * the vectors and every reduction in FP64: the batched counterpart of the plain FP64 path, for
+ * systems whose convergence FP32 arithmetic would slow down).  max_outer caps the reliable
+ * updates per system. */
+int femd_batch_solve(femd_batch B, const double *b, double *x, double tol, double inner_tol, int max_outer, int precision,
+                     int use_graph, int *iterations, double *residuals, femd_solve_info *info);
+int femd_batch_solve_dev(femd_batch B, const double *b_dev, double *x_dev, double tol, double inner_tol, int max_outer,
+                         int precision, int use_graph, int *iterations, double *residuals, femd_solve_info *info,
+                         femd_stream s);
 int femd_batch_solve_refined_dev(femd_batch B, const double *b_dev, double *x_dev, double tol, double inner_tol,
                                  int max_outer, int use_graph, int *iterations, double *residuals,
                                  femd_solve_info *info, femd_stream s);

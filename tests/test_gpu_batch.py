@@ -50,3 +50,23 @@ def test_batch_update_values_and_zero_rhs():
     B.update_values()
     ys, info = B.solve([b1, b2], tol=1e-10)
     assert np.linalg.norm(2.0 * ys[0] - xs[0]) <= 1e-8 * np.linalg.norm(xs[0])
+
+
+def test_batch_fp64_inner(orc):
+    print("\n[case] batch with FP64 inner iterations", flush=True)
+    specs = [(10, 2, 7), (23, 2, 8), (7, 3, 9)]
+    systems = [_system(*s) for s in specs]
+    B = fd.Batch([A for _, A, _ in systems])
+    xs, info = B.solve([b for _, _, b in systems], tol=1e-11, inner_tol=1e-11, precision="fp64")
+    assert info["converged"] and info["worst_residual"] <= 1e-11
+    for (V, A, b), x in zip(systems, xs):
+        ip, ix = V.pattern()
+        xo, _ = orc.cg(ip, ix, A.data, b, M="jacobi", tol=1e-13)
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-9
+    # FP64 inner iterations take about as many steps as the FP64 reference algorithm
+    for (V, A, b), it in zip(systems, info["iterations"]):
+        ip, ix = V.pattern()
+        _, io = orc.cg(ip, ix, A.data, b, M="jacobi", tol=1e-11)
+        assert it <= 1.25 * io["iterations"] + 5
+    with pytest.raises(ValueError):
+        B.solve([b for _, _, b in systems], precision="fp16")

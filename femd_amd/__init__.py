@@ -24,7 +24,7 @@ from .forms import (Const, Constant, Dx, TestFunction, TrialFunction, abs_, as_v
 from .sparse import Preconditioner, SparseMatrix, SparseSolver
 from .krylov import KrylovInfo, cg, gmres, lgmres, solve_refined
 from .linalg import solve
-from .batch import Batch
+from .batch import Batch, value_dictionary
 from .timestep import BackwardEuler
 
 __version__ = "0.9.0"

@@ -108,6 +108,8 @@ _SIGS = {
     "femd_event_timer_stop": (_i, [_p, C.POINTER(_d)]),
     "femd_inner_timer": (_i, [_i]),
     "femd_inner_timer_read": (_i, [C.POINTER(_i64), C.POINTER(_d)]),
+    "femd_value_dictionary": (_i, [_i]),
+    "femd_batch_dictionary": (_i, [_p, _i, C.POINTER(_i64), C.POINTER(_i)]),
 }
 
 _lib = None

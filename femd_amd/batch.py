@@ -69,3 +69,18 @@ class Batch:
                                           its.ctypes.data, res.ctypes.data, C.byref(info)))
         return self.split(x), dict(converged=bool(info.converged), iterations=its, residuals=res,
                                    refinement_rounds=int(info.breakdown), worst_residual=float(info.residual))
+
+    def dictionary(self, precision="fp32"):
+        """(distinct values over all tiles, most in one tile) of the value dictionary the last solve
+        in `precision` read its matrix through, or None when it read the values themselves."""
+        prec = {"fp32": L.FP32, "fp64": L.FP64}[precision]
+        d, m = C.c_int64(), C.c_int()
+        L.check(L.load().femd_batch_dictionary(self._h, prec, C.byref(d), C.byref(m)))
+        return None if d.value < 0 else (d.value, m.value)
+
+
+def value_dictionary(enable=None) -> bool:
+    """Whether the inner CG may read matrices through lossless per-tile value dictionaries (results
+    are bit-identical either way); with `enable` set it first switches them on or off.  Returns the
+    previous setting."""
+    return bool(L.load().femd_value_dictionary(-1 if enable is None else int(bool(enable))))

@@ -44,7 +44,12 @@ def main(argv=None):
                          "FP64 solutions is measured and reported")
     ap.add_argument("--inner-tol", type=float, default=0.3)
     ap.add_argument("--sweep", default="", help="comma-separated mixed-tol:inner-tol pairs to time as well")
+    ap.add_argument("--no-graph", action="store_true", help="batched solves with plain launches instead of graph replays")
+    ap.add_argument("--no-dict", action="store_true", help="batched solves read the matrix values, not value dictionaries")
+    ap.add_argument("--parts", default="plain,fp64,mixed",
+                    help="which solves to run (a profiling run may leave out the plain path; errors then are not measured)")
     args = ap.parse_args(argv)
+    parts = set(args.parts.split(","))
 
     import femd_amd as fd
     from femd_amd import _lib as L
@@ -52,6 +57,7 @@ def main(argv=None):
 
     lib = L.load()
     L.check(lib.femd_init(0))
+    lib.femd_value_dictionary(0 if args.no_dict else 1)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
     sizes = problem_sizes(args.count, args.nmin, args.nmax, args.seed)
@@ -93,16 +99,20 @@ def main(argv=None):
         L.check(lib.femd_stream_sync(None))
         return its
 
-    plain_round()                      # warm-up: tuning, graph and pool set-up
-    t1 = time.perf_counter()
-    its64 = plain_round()
-    t_plain = time.perf_counter() - t1
-    x64 = []
-    for (V, A, b), xd in zip(systems, xdev):
-        x = np.empty(b.size)
-        L.check(lib.femd_memcpy_dtoh(x.ctypes.data, xd, 8 * b.size))
-        x64.append(x)
-    log(f"plain FP64: {t_plain:.3f} s, {sum(its64)} iterations in total (max {max(its64)})")
+    x64 = None
+    t_plain, its64 = float("nan"), [0]
+    if "plain" in parts:
+        plain_round()                      # warm-up: tuning, graph and pool set-up
+        t1 = time.perf_counter()
+        its64 = plain_round()
+        t_plain = time.perf_counter() - t1
+        x64 = []
+        for (V, A, b), xd in zip(systems, xdev):
+            x = np.empty(b.size)
+            L.check(lib.femd_memcpy_dtoh(x.ctypes.data, xd, 8 * b.size))
+            x64.append(x)
+        log(f"plain FP64: {t_plain:.3f} s, {sum(its64)} iterations in total (max {max(its64)}), "
+            f"{sum(n * k for n, k in zip(dims, its64))} row-iterations")
 
     # ---- batched mixed precision ----------------------------------------------------------------
     B = fd.Batch([A for _, A, _ in systems])
@@ -118,29 +128,33 @@ def main(argv=None):
     def batched_round(tol, itol, prec=L.FP32):
         L.check(lib.femd_memcpy_htod(xd, zcat.ctypes.data, 8 * B.nrows))
         info = L.SolveInfo()
-        L.check(lib.femd_batch_solve_dev(B._h, bd, xd, tol, itol, 200, prec, 1, its32.ctypes.data, res.ctypes.data,
+        L.check(lib.femd_batch_solve_dev(B._h, bd, xd, tol, itol, 200, prec, 0 if args.no_graph else 1, its32.ctypes.data, res.ctypes.data,
                                          C.byref(info), None))
         L.check(lib.femd_stream_sync(None))
         return info
 
     def errors():
+        if x64 is None:
+            return [float("nan")]
         xb = np.empty(B.nrows)
         L.check(lib.femd_memcpy_dtoh(xb.ctypes.data, xd, 8 * B.nrows))
         return [float(np.linalg.norm(xs - x) / max(np.linalg.norm(x), 1e-300)) for xs, x in zip(B.split(xb), x64)]
 
     # the batched FP64 solve (same machinery, FP64 inner arithmetic)
-    batched_round64 = lambda: batched_round(args.tol, args.tol, L.FP64)
-    batched_round64()
-    t3 = time.perf_counter()
-    info64 = batched_round64()
-    t_b64 = time.perf_counter() - t3
-    errs64 = errors()
-    log(f"batched FP64: {t_b64:.3f} s, converged {bool(info64.converged)}, max error vs FP64 {max(errs64):.2e}, "
-        f"inner iterations max {int(its32.max())}")
-    b64 = {"seconds": round(t_b64, 4), "solves_per_s": round(len(systems) / t_b64, 3), "speedup": round(t_plain / t_b64, 3),
-           "converged": bool(info64.converged), "max_rel_err_vs_fp64": max(errs64), "inner_iterations_max": int(its32.max()),
-           "inner_iterations_total": int(info64.iterations)}
-
+    b64 = None
+    if "fp64" in parts:
+        batched_round(args.tol, args.tol, L.FP64)
+        t3 = time.perf_counter()
+        info64 = batched_round(args.tol, args.tol, L.FP64)
+        t_b64 = time.perf_counter() - t3
+        errs64 = errors()
+        log(f"batched FP64: {t_b64:.3f} s, converged {bool(info64.converged)}, max error vs FP64 {max(errs64):.2e}, "
+            f"inner iterations max {int(its32.max())}, {int(np.dot(dims, its32))} row-iterations")
+        b64 = {"seconds": round(t_b64, 4), "solves_per_s": round(len(systems) / t_b64, 3),
+               "speedup": round(t_plain / t_b64, 3), "converged": bool(info64.converged),
+               "max_rel_err_vs_fp64": max(errs64), "inner_iterations_max": int(its32.max()),
+               "inner_iterations_total": int(info64.iterations), "row_iterations": int(np.dot(dims, its32)),
+               "value_dictionary": B.dictionary("fp64")}
     sweep = []
     for pair in [p for p in args.sweep.split(",") if p]:
         mt, it = (float(v) for v in pair.split(":"))
@@ -153,22 +167,27 @@ def main(argv=None):
                       "max_rel_err_vs_fp64": max(e), "inner_iterations_max": int(its32.max()),
                       "inner_iterations_total": int(inf.iterations), "rounds": int(inf.breakdown)})
         log("sweep", sweep[-1])
-    batched_round(mtol, args.inner_tol)
-    t2 = time.perf_counter()
-    info = batched_round(mtol, args.inner_tol)
-    t_batch = time.perf_counter() - t2
-    errs = errors()
-    log(f"batched mixed: {t_batch:.3f} s, converged {bool(info.converged)}, refinement rounds {info.breakdown}, "
-        f"max error vs FP64 {max(errs):.2e}")
+    t_batch, info, errs = float("nan"), L.SolveInfo(), [float("nan")]
+    if "mixed" in parts:
+        batched_round(mtol, args.inner_tol)
+        t2 = time.perf_counter()
+        info = batched_round(mtol, args.inner_tol)
+        t_batch = time.perf_counter() - t2
+        errs = errors()
+        log(f"batched mixed: {t_batch:.3f} s, converged {bool(info.converged)}, refinement rounds {info.breakdown}, "
+            f"max error vs FP64 {max(errs):.2e}, {int(np.dot(dims, its32))} row-iterations")
     out = {
         "workload": f"{len(systems)} P2 Poisson systems, structured N x N, N log-uniform in [{args.nmin}, {args.nmax}]",
         "seed": args.seed, "systems": len(systems), "unknowns": int(sum(dims)), "nnz": int(sum(nnzs)),
         "N": sizes, "tol": args.tol, "mixed_tol": mtol, "inner_tol": args.inner_tol,
         "plain_fp64": {"seconds": round(t_plain, 4), "solves_per_s": round(len(systems) / t_plain, 3),
-                       "iterations_total": int(sum(its64)), "iterations_max": int(max(its64))},
+                       "iterations_total": int(sum(its64)), "iterations_max": int(max(its64)),
+                       "row_iterations": int(sum(n * k for n, k in zip(dims, its64)))},
         "batched_mixed": {"seconds": round(t_batch, 4), "solves_per_s": round(len(systems) / t_batch, 3),
                           "converged": bool(info.converged), "refinement_rounds": int(info.breakdown),
-                          "inner_iterations_max": int(its32.max()), "worst_residual": float(info.residual)},
+                          "inner_iterations_max": int(its32.max()), "worst_residual": float(info.residual),
+                          "row_iterations": int(np.dot(dims, its32)),
+                          "value_dictionary": B.dictionary("fp32") if "mixed" in parts else None},
         "speedup": round(t_plain / t_batch, 3), "max_rel_err_vs_fp64": max(errs),
         "batched_fp64": b64,
         "sweep": sweep,

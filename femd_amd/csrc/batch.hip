@@ -154,6 +154,18 @@ int femd_batch_solve_refined(femd_batch B, const double *b, double *x, double to
     return femd_batch_solve(B, b, x, tol, inner_tol, max_outer, FEMD_FP32, use_graph, iterations, residuals, info);
 }
 
+int femd_batch_dictionary(femd_batch B, int precision, int64_t *distinct, int *max_per_tile)
+{
+    return api([&] {
+        if (!B) fail(FEMD_EINVAL, "null batch");
+        if (precision != FEMD_FP32 && precision != FEMD_FP64) fail(FEMD_EINVAL, "batch: precision is FEMD_FP32 or FEMD_FP64");
+        const CGEngine &E = B->engine[precision == FEMD_FP32 ? 0 : 1];
+        const ValueDict &d = precision == FEMD_FP32 ? B->A->d32s : B->A->d64s;
+        if (distinct) *distinct = E.use_dict ? d.distinct : -1;
+        if (max_per_tile) *max_per_tile = d.valid ? d.max_distinct : -1;
+    });
+}
+
 void femd_batch_free(femd_batch B)
 {
     if (B && g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);

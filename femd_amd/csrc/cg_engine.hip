@@ -65,16 +65,19 @@ __global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, cons
 }
 
 // ---- iteration kernels -----------------------------------------------------------------------
-template <typename T, int THREADS, int CAP>
+template <typename T, int THREADS, int CAP, bool DICT>
 __global__ __launch_bounds__(THREADS) void k_cg_product(const int2 *__restrict__ blk, const int *__restrict__ uptr,
                                                         const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
                                                         const int *__restrict__ ip, const int *__restrict__ ix,
-                                                        const T *__restrict__ v, const T *__restrict__ r, T *P0, T *P1,
-                                                        T *__restrict__ q, int64_t nnz, const int *__restrict__ tile_sys,
-                                                        double *__restrict__ tparts, const CGSys *st)
+                                                        const T *__restrict__ v, const unsigned char *__restrict__ vidx,
+                                                        const int *__restrict__ doff, const T *__restrict__ dval,
+                                                        const T *__restrict__ r, T *P0, T *P1, T *__restrict__ q, int64_t nnz,
+                                                        const int *__restrict__ tile_sys, double *__restrict__ tparts,
+                                                        const CGSys *st)
 {
     __shared__ T xs[CAP];
     __shared__ T prod[CAP];
+    __shared__ T dv[DICT ? 256 : 1];
     __shared__ double red[THREADS / 64];
     const CGSys &S = st[tile_sys ? tile_sys[blockIdx.x] : 0];
     if (!S.active) return;
@@ -82,9 +85,13 @@ __global__ __launch_bounds__(THREADS) void k_cg_product(const int2 *__restrict__
     const T *pold = odd ? P1 : P0;
     T *pnew = odd ? P0 : P1;
     bool stopped = false;
-    const double part = stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz,
-                                                     PCGOp<T>{r, pold, pnew, q, static_cast<T>(S.beta)}, xs, prod, nullptr,
-                                                     &stopped);
+    const PCGOp<T> op{r, pold, pnew, q, static_cast<T>(S.beta)};
+    double part;
+    if constexpr (DICT)
+        part = stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, op, xs, prod, nullptr, &stopped, nullptr,
+                                            nullptr, TileDict<T>{vidx, doff, dval, dv});
+    else
+        part = stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, op, xs, prod, nullptr, &stopped);
     const double t = wg_sum<THREADS>(part, red);
     if (threadIdx.x == 0) tparts[blockIdx.x] = t;
 }
@@ -426,6 +433,7 @@ void scaled_values(femd_csr_s *A, hipStream_t s)
     if (badrow != big)
         fail(FEMD_ENOTSPD, "inner CG: diagonal entry of row " + std::to_string(badrow) + " is not positive");
     A->v32s_valid = true;
+    A->d32s.valid = false;
 }
 
 void scaled_values64(femd_csr_s *A, hipStream_t s)
@@ -438,6 +446,7 @@ void scaled_values64(femd_csr_s *A, hipStream_t s)
         hipLaunchKernelGGL(k_scaled_values<double>, dim3((P.n + BT - 1) / BT), dim3(BT), 0, s, P.n, P.ip.as<int>(),
                            P.ix.as<int>(), A->v.as<double>(), A->scale.as<double>(), A->v64s.as<double>());
     A->v64s_valid = true;
+    A->d64s.valid = false;
 }
 
 // ---- the engine ------------------------------------------------------------------------------
@@ -456,11 +465,23 @@ void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s)
     if (M->pat->n != M->pat->m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
     if (prec == FEMD_FP64) scaled_values64(M, s);
     else scaled_values(M, s);
-    if (A == M && precision == prec && plan) return;
-    drop_graph();
-    A = M;
-    precision = prec;
-    bind(M, prec, s);
+    if (!(A == M && precision == prec && plan))
+    {
+        drop_graph();
+        A = M;
+        precision = prec;
+        bind(M, prec, s);
+    }
+    // the value dictionary of the scaled values under this plan, when the values allow one
+    ValueDict &d = prec == FEMD_FP64 ? M->d64s : M->d32s;
+    const bool want = dict_enabled();
+    if (want && (!d.valid || d.plan != plan))
+        build_dict(*plan, prec == FEMD_FP64 ? M->v64s.p : M->v32s.p, prec, M->pat->nnz, d, s);
+    const bool use = want && d.valid && d.usable && d.plan == plan;
+    // a captured graph holds the format and the dictionary's buffers
+    if (use != use_dict || (use && d.gen != dict_gen)) drop_graph();
+    use_dict = use;
+    dict_gen = use ? d.gen : 0;
 }
 
 void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
@@ -508,8 +529,8 @@ void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
 
 namespace {
 
-template <typename T>
-void iterate_t(CGEngine &E, int count, hipStream_t s)
+template <typename T, bool DICT>
+void launch_product(CGEngine &E, hipStream_t s)
 {
     const DevPattern &P = *E.A->pat;
     const SpmvPlan &pl = *E.plan;
@@ -520,27 +541,41 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
     const unsigned short *lidx = pl.lidx.as<unsigned short>();
     const int *tsys = pl.tile_sys.p ? pl.tile_sys.as<int>() : nullptr;
     const T *v = sizeof(T) == 8 ? reinterpret_cast<const T *>(E.A->v64s.p) : reinterpret_cast<const T *>(E.A->v32s.p);
+    const ValueDict &d = sizeof(T) == 8 ? E.A->d64s : E.A->d32s;
+    const unsigned char *vidx = DICT ? d.vidx.as<unsigned char>() : nullptr;
+    const int *doff = DICT ? d.doff.as<int>() : nullptr;
+    const T *dval = DICT ? d.dval.as<T>() : nullptr;
+    const CGSys *st = E.st.as<CGSys>();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!timer_sample(s, &e0, &e1)) e0 = e1 = nullptr;
+    bool launched = false;
+#define FEMD_CASE(TH, CP)                                                                                             \
+    if constexpr (2 * CP * sizeof(T) + (DICT ? 256 * sizeof(T) : 0) <= 65536)                                        \
+        if (th == TH && cap == CP)                                                                                    \
+        {                                                                                                             \
+            launched = true;                                                                                          \
+            if (e0)                                                                                                   \
+                hipExtLaunchKernelGGL((k_cg_product<T, TH, CP, DICT>), dim3(pl.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, \
+                                      ucol, lidx, P.ip.as<int>(), P.ix.as<int>(), v, vidx, doff, dval, E.r.as<T>(),   \
+                                      E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(), st); \
+            else                                                                                                      \
+                hipLaunchKernelGGL((k_cg_product<T, TH, CP, DICT>), dim3(pl.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx, \
+                                   P.ip.as<int>(), P.ix.as<int>(), v, vidx, doff, dval, E.r.as<T>(), E.P0.as<T>(),     \
+                                   E.P1.as<T>(), E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(), st);                 \
+        }
+    FEMD_SHAPES(FEMD_CASE)
+#undef FEMD_CASE
+    if (!launched) fail(FEMD_EINVAL, "inner CG: no product kernel for the plan's launch shape");
+}
+
+template <typename T>
+void iterate_t(CGEngine &E, int count, hipStream_t s)
+{
     CGSys *st = E.st.as<CGSys>();
     for (int k = 0; k < count; ++k)
     {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (!timer_sample(s, &e0, &e1)) e0 = e1 = nullptr;
-#define FEMD_CASE(TH, CP)                                                                                             \
-        if constexpr (2 * CP * sizeof(T) <= 65536)                                                                    \
-            if (th == TH && cap == CP)                                                                                \
-            {                                                                                                         \
-                if (e0)                                                                                               \
-                    hipExtLaunchKernelGGL((k_cg_product<T, TH, CP>), dim3(pl.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, \
-                                          ucol, lidx, P.ip.as<int>(), P.ix.as<int>(), v, E.r.as<T>(), E.P0.as<T>(),   \
-                                          E.P1.as<T>(), E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(),              \
-                                          static_cast<const CGSys *>(st));                                            \
-                else                                                                                                  \
-                    hipLaunchKernelGGL((k_cg_product<T, TH, CP>), dim3(pl.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx, \
-                                       P.ip.as<int>(), P.ix.as<int>(), v, E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(),     \
-                                       E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(), static_cast<const CGSys *>(st)); \
-            }
-        FEMD_SHAPES(FEMD_CASE)
-#undef FEMD_CASE
+        if (E.use_dict) launch_product<T, true>(E, s);
+        else launch_product<T, false>(E, s);
         hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(BT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), E.y.as<T>(),
                            E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(),

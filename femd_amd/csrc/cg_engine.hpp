@@ -60,6 +60,8 @@ struct CGEngine {
     const SpmvPlan *plan = nullptr;
     DevBuf seg_tiles;
     DevBuf st, tparts, cparts, r, q, P0, P1, y, r64, t64, sel, retired, rho_new, tgt, cnt;
+    bool use_dict = false;              // the product reads the scaled values through their dictionary
+    int dict_gen = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;

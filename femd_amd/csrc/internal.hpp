@@ -123,6 +123,23 @@ struct SpmvPlan {
     double ms = 0.0;
 };
 
+// Lossless per-tile dictionary of one value array under one tile plan: tile b's distinct values
+// (at most 256) in dval[doff[b] .. doff[b + 1]) and every entry's 1-byte position in them (vidx),
+// 3 bytes per entry with the column positions instead of 6 (FP32) or 10 (FP64).
+struct ValueDict {
+    bool valid = false;            // built for the current values
+    bool usable = false;           // every tile has at most 256 distinct values
+    int gen = 0;                   // bumped by every build
+    int max_distinct = 0;
+    int64_t distinct = 0;          // dictionary entries over all tiles
+    const SpmvPlan *plan = nullptr;  // the tiling it was built for
+    DevBuf doff, dval, vidx;
+};
+// Build d for values v (FP32 or FP64 by `precision`) under plan p (spmv.hip).
+void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, ValueDict &d, hipStream_t s);
+// Whether the solvers use value dictionaries where a matrix allows them (femd_value_dictionary).
+bool dict_enabled();
+
 struct DevPattern {
     int n = 0, m = 0;
     int64_t nnz = 0;
@@ -187,6 +204,7 @@ struct femd_csr_s {
     bool v32s_valid = false;
     femd_amd::DevBuf v64s;               // S A S in FP64 (the batched FP64 CG)
     bool v64s_valid = false;
+    femd_amd::ValueDict d32s, d64s;      // dictionaries of v32s and v64s under the pattern's plans
     const femd_space_s *space = nullptr;
 };
 

@@ -9,6 +9,7 @@
 // The product of the inner CG (cg_engine.hip) is the same tile with the direction update fused in.
 #include "tile.hpp"
 
+#include <atomic>
 #include <cfloat>
 
 namespace femd_amd {
@@ -192,7 +193,184 @@ SpmvPlan shape_plan(const DevPattern &P, int si, int, hipStream_t s) { return ma
 
 } // namespace femd_amd
 
+// ---------------------------------------------------------------------------- value dictionaries
+// Built on the device in two passes over the tiles (one workgroup each): both insert the tile's
+// values, by bit pattern, into an LDS hash table (open addressing, 512 slots); the first counts
+// the distinct ones, the second -- once every tile is known to have at most 256 and the offsets
+// are set -- numbers the occupied slots in slot order, writes the dictionary and every entry's
+// position.  Values are compared by bit pattern, so the dictionary is exact (-0.0 and 0.0 stay
+// apart); the one pattern used to mark empty slots (all ones, a NaN) makes a tile unusable.
+namespace femd_amd {
+namespace {
+
+constexpr int DICT_SLOTS = 512;
+constexpr int DICT_THREADS = 256;
+constexpr unsigned long long DICT_EMPTY = ~0ull;
+constexpr int DICT_OVERFLOW = 1 << 20;
+
+__device__ inline unsigned long long dict_key(float v) { return __float_as_uint(v); }
+__device__ inline unsigned long long dict_key(double v) { return static_cast<unsigned long long>(__double_as_longlong(v)); }
+template <typename T> __device__ inline T dict_value(unsigned long long k);
+template <> __device__ inline float dict_value<float>(unsigned long long k) { return __uint_as_float(static_cast<unsigned>(k)); }
+template <> __device__ inline double dict_value<double>(unsigned long long k) { return __longlong_as_double(static_cast<long long>(k)); }
+__device__ inline int dict_hash(unsigned long long k) { return static_cast<int>((k * 0x9E3779B97F4A7C15ull) >> 55); }
+
+// Insert the values of tile b into keys; false (in every thread) when the table or a key cannot hold them.
+template <typename T>
+__device__ bool dict_fill(int p0, int p1, const T *__restrict__ v, unsigned long long *keys, int *bad)
+{
+    for (int i = threadIdx.x; i < DICT_SLOTS; i += DICT_THREADS) keys[i] = DICT_EMPTY;
+    if (threadIdx.x == 0) *bad = 0;
+    __syncthreads();
+    for (int p = p0 + static_cast<int>(threadIdx.x); p < p1; p += DICT_THREADS)
+    {
+        const unsigned long long k = dict_key(v[p]);
+        if (k == DICT_EMPTY) { *bad = 1; continue; }
+        int h = dict_hash(k);
+        for (int probe = 0;; ++probe)
+        {
+            if (probe == DICT_SLOTS) { *bad = 1; break; }
+            const unsigned long long prev = atomicCAS(&keys[h], DICT_EMPTY, k);
+            if (prev == DICT_EMPTY || prev == k) break;
+            h = (h + 1) & (DICT_SLOTS - 1);
+        }
+    }
+    __syncthreads();
+    const bool ok = *bad == 0;
+    __syncthreads();
+    return ok;
+}
+
+template <typename T>
+__global__ __launch_bounds__(DICT_THREADS) void k_dict_count(const int2 *__restrict__ blk, const T *__restrict__ v, int *counts)
+{
+    __shared__ unsigned long long keys[DICT_SLOTS];
+    __shared__ int bad;
+    const int b = blockIdx.x;
+    const bool ok = dict_fill(blk[b].y, blk[b + 1].y, v, keys, &bad);
+    int c = 0;
+    for (int i = threadIdx.x; i < DICT_SLOTS; i += DICT_THREADS) c += keys[i] != DICT_EMPTY ? 1 : 0;
+    const int total = __syncthreads_count(c >= 1) + __syncthreads_count(c >= 2);
+    if (threadIdx.x == 0) counts[b] = ok ? total : DICT_OVERFLOW;
+}
+
+template <typename T>
+__global__ __launch_bounds__(DICT_THREADS) void k_dict_write(const int2 *__restrict__ blk, const T *__restrict__ v,
+                                                             const int *__restrict__ doff, T *dval, unsigned char *vidx)
+{
+    __shared__ unsigned long long keys[DICT_SLOTS];
+    __shared__ int bad;
+    __shared__ int scan[DICT_THREADS];
+    __shared__ unsigned char pos[DICT_SLOTS];
+    const int b = blockIdx.x, p0 = blk[b].y, p1 = blk[b + 1].y;
+    if (!dict_fill(p0, p1, v, keys, &bad)) return;
+    // slots 2t and 2t + 1 belong to thread t; number the occupied ones in slot order
+    const int t = threadIdx.x;
+    const bool o0 = keys[2 * t] != DICT_EMPTY, o1 = keys[2 * t + 1] != DICT_EMPTY;
+    scan[t] = int(o0) + int(o1);
+    __syncthreads();
+    for (int off = 1; off < DICT_THREADS; off <<= 1)
+    {
+        const int add = t >= off ? scan[t - off] : 0;
+        __syncthreads();
+        scan[t] += add;
+        __syncthreads();
+    }
+    const int first = scan[t] - int(o0) - int(o1), d0 = doff[b];
+    if (o0)
+    {
+        pos[2 * t] = static_cast<unsigned char>(first);
+        dval[d0 + first] = dict_value<T>(keys[2 * t]);
+    }
+    if (o1)
+    {
+        pos[2 * t + 1] = static_cast<unsigned char>(first + int(o0));
+        dval[d0 + first + int(o0)] = dict_value<T>(keys[2 * t + 1]);
+    }
+    __syncthreads();
+    for (int p = p0 + t; p < p1; p += DICT_THREADS)
+    {
+        const unsigned long long k = dict_key(v[p]);
+        int h = dict_hash(k);
+        while (keys[h] != k) h = (h + 1) & (DICT_SLOTS - 1);
+        vidx[p] = pos[h];
+    }
+}
+
+std::atomic<int> g_dict_on{1};
+
+} // namespace
+
+bool dict_enabled() { return g_dict_on.load() != 0; }
+
+void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, ValueDict &d, hipStream_t s)
+{
+    d.valid = true;
+    d.usable = false;
+    d.gen += 1;
+    d.plan = &p;
+    d.max_distinct = 0;
+    d.distinct = 0;
+    d.doff.release();
+    d.dval.release();
+    d.vidx.release();
+    if (p.ntiles <= 0 || nnz <= 0) return;
+    const int nt = p.ntiles;
+    const int2 *blk = p.blocks.as<int2>();
+    DevBuf counts(4 * size_t(nt));
+    if (precision == FEMD_FP64)
+        hipLaunchKernelGGL(k_dict_count<double>, dim3(nt), dim3(DICT_THREADS), 0, s, blk, static_cast<const double *>(v),
+                           counts.as<int>());
+    else
+        hipLaunchKernelGGL(k_dict_count<float>, dim3(nt), dim3(DICT_THREADS), 0, s, blk, static_cast<const float *>(v),
+                           counts.as<int>());
+    std::vector<int> h(size_t(nt) + 1, 0);
+    HIPC(hipMemcpyAsync(h.data(), counts.p, 4 * size_t(nt), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    int mx = 0;
+    int64_t tot = 0;
+    for (int b = 0; b < nt; ++b)
+    {
+        mx = std::max(mx, h[size_t(b)]);
+        tot += h[size_t(b)];
+    }
+    d.max_distinct = mx;
+    if (mx > 256 || tot >= (int64_t(1) << 31)) return;
+    // exclusive offsets
+    int run = 0;
+    for (int b = 0; b < nt; ++b)
+    {
+        const int c = h[size_t(b)];
+        h[size_t(b)] = run;
+        run += c;
+    }
+    h[size_t(nt)] = run;
+    const size_t w = precision == FEMD_FP64 ? 8 : 4;
+    d.doff = upload(h, s);
+    d.dval = DevBuf(w * size_t(std::max(run, 1)));
+    d.vidx = DevBuf(size_t(nnz) + 8);
+    if (precision == FEMD_FP64)
+        hipLaunchKernelGGL(k_dict_write<double>, dim3(nt), dim3(DICT_THREADS), 0, s, blk, static_cast<const double *>(v),
+                           d.doff.as<int>(), d.dval.as<double>(), d.vidx.as<unsigned char>());
+    else
+        hipLaunchKernelGGL(k_dict_write<float>, dim3(nt), dim3(DICT_THREADS), 0, s, blk, static_cast<const float *>(v),
+                           d.doff.as<int>(), d.dval.as<float>(), d.vidx.as<unsigned char>());
+    HIPC(hipGetLastError());
+    d.distinct = run;
+    d.usable = true;
+}
+
+} // namespace femd_amd
+
 // --------------------------------------------------------------------------------- C ABI
+extern "C" int femd_value_dictionary(int mode)
+{
+    using namespace femd_amd;
+    const int prev = g_dict_on.load();
+    if (mode == 0 || mode == 1) g_dict_on.store(mode);
+    return prev;
+}
+
 extern "C" int femd_spmv_tuning(femd_csr A, int precision, char *name, int name_len, double *ms)
 {
     using namespace femd_amd;

@@ -46,6 +46,23 @@ template <> struct Vals4<double> {
     __device__ void set(int j, double t) { if (j < 2) a[j] = t; else b[j - 2] = t; }
 };
 
+// Where a tile's matrix values come from: the value array itself (NoDict), or a per-tile value
+// dictionary -- the tile's distinct values (at most 256) in dval[doff[b] .. doff[b + 1]), staged
+// in LDS (dv), and every entry's 1-byte position in them (vidx).  Both give every entry its exact
+// value, so the two formats give bit-identical products.
+struct NoDict {
+    static constexpr bool on = false;
+};
+template <typename T> struct TileDict {
+    static constexpr bool on = true;
+    const unsigned char *vidx;
+    const int *doff;
+    const T *dval;
+    T *dv;          // LDS, 256 entries
+};
+
+typedef unsigned int u32;
+
 // One tile (workgroup blockIdx.x).  Op supplies gather(j) (the x value at global column j),
 // own(r) (what the row epilogue needs of row r) and store(r, s, own) (writes row r's result and
 // returns its share of the workgroup's partials, of type Op::Part).  Returns this thread's share.
@@ -53,15 +70,16 @@ template <> struct Vals4<double> {
 // before it loads the matrix, and its latency hides behind theirs.  With `mask` (may be NULL) the
 // tile is skipped when mask[tile_sys[b]] is zero (the batched solver: tile b belongs to system
 // tile_sys[b]).  *stopped tells the caller.
-template <typename T, int THREADS, int CAP, class Op>
+template <typename T, int THREADS, int CAP, class Op, class VS = NoDict>
 __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr, const int *__restrict__ ucol,
                                      const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
                                      const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op, T *xs,
                                      T *prod, const int *stop, bool *stopped, const int *tile_sys = nullptr,
-                                     const int *mask = nullptr)
+                                     const int *mask = nullptr, const VS &vs = VS{})
 {
     constexpr int NGRP = (CAP / 4 + 1 + THREADS - 1) / THREADS;
     constexpr int NU = (CAP + THREADS - 1) / THREADS;
+    static_assert(!VS::on || THREADS >= 256, "a dictionary of 256 values is staged one per lane");
     const int b = blockIdx.x;
     const int2 b0 = blk[b], b1 = blk[b + 1];
     const int halt = (stop && *stop) || (mask && !mask[tile_sys[b]]);
@@ -86,14 +104,17 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
             T s = 0;
             for (int p = p0; p < p0 + len; ++p)
             {
-                const T t = v[p] * op.gather(ix[p]);
+                T a;
+                if constexpr (VS::on) a = vs.dval[vs.doff[b] + vs.vidx[p]];
+                else a = v[p];
+                const T t = a * op.gather(ix[p]);
                 s += t;
             }
             part = op.store(r0, s, own);
         }
         return part;
     }
-    // 1. distinct columns and entry loads
+    // 1. distinct columns (and the tile's value dictionary) and entry loads
     const int u0 = uptr[b], nu = uptr[b + 1] - u0;
     int uc[NU];
 #pragma unroll
@@ -102,8 +123,17 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
         const int u = static_cast<int>(threadIdx.x) + i * THREADS;
         uc[i] = u < nu ? __builtin_nontemporal_load(ucol + u0 + u) : 0;
     }
+    T dvv = 0;
+    int d0 = 0, nd = 0;
+    if constexpr (VS::on)
+    {
+        d0 = vs.doff[b];
+        nd = vs.doff[b + 1] - d0;
+        if (static_cast<int>(threadIdx.x) < nd) dvv = vs.dval[d0 + threadIdx.x];
+    }
     const int a0 = p0 & ~3, e = p0 + len;
     Vals4<T> vv[NGRP];
+    u32 di[NGRP];
     u16x4 cc[NGRP];
 #pragma unroll
     for (int g = 0; g < NGRP; ++g)
@@ -113,21 +143,28 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
         {
             if (int64_t(a) + 4 <= nnz)
             {
-                vv[g].load(v, a);
+                if constexpr (VS::on) di[g] = __builtin_nontemporal_load(reinterpret_cast<const u32 *>(vs.vidx + a));
+                else vv[g].load(v, a);
                 cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
             }
             else
             {
+                if constexpr (VS::on) di[g] = 0;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                 {
                     const bool in = int64_t(a) + j < nnz;
-                    vv[g].set(j, in ? v[a + j] : T(0));
+                    if constexpr (VS::on) di[g] |= (in ? u32(vs.vidx[a + j]) : 0u) << (8 * j);
+                    else vv[g].set(j, in ? v[a + j] : T(0));
                     cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
                 }
             }
         }
-        else cc[g] = u16x4{0, 0, 0, 0};
+        else
+        {
+            cc[g] = u16x4{0, 0, 0, 0};
+            if constexpr (VS::on) di[g] = 0;
+        }
     }
     T xg[NU];
 #pragma unroll
@@ -142,6 +179,8 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
         const int u = static_cast<int>(threadIdx.x) + i * THREADS;
         if (u < nu) xs[u] = xg[i];
     }
+    if constexpr (VS::on)
+        if (static_cast<int>(threadIdx.x) < nd) vs.dv[threadIdx.x] = dvv;
     __syncthreads();
     // 2. products
 #pragma unroll
@@ -154,7 +193,10 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
             for (int j = 0; j < 4; ++j)
             {
                 const int k = a + j - p0;
-                if (k >= 0 && k < len) prod[k] = vv[g].get(j) * xs[cc[g][j]];
+                T val;
+                if constexpr (VS::on) val = vs.dv[(di[g] >> (8 * j)) & 0xffu];
+                else val = vv[g].get(j);
+                if (k >= 0 && k < len) prod[k] = val * xs[cc[g][j]];
             }
         }
     }

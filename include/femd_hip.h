@@ -267,6 +267,10 @@ int femd_batch_solve_dev(femd_batch B, const double *b_dev, double *x_dev, doubl
 int femd_batch_solve_refined_dev(femd_batch B, const double *b_dev, double *x_dev, double tol, double inner_tol,
                                  int max_outer, int use_graph, int *iterations, double *residuals,
                                  femd_solve_info *info, femd_stream s);
+/* The value dictionary the last solve in `precision` used: *distinct its entries over all tiles
+ * (-1: none, the values did not allow one or it is turned off), *max_per_tile the most distinct
+ * values in one tile (-1: not examined). */
+int femd_batch_dictionary(femd_batch B, int precision, int64_t *distinct, int *max_per_tile);
 void femd_batch_free(femd_batch B);
 
 /* ---------------------------------------------------------------- time stepping
@@ -287,12 +291,21 @@ int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double 
 
 /* ---------------------------------------------------------------- timing
  * Kernel-time of the library's work on a stream, measured with HIP events around it. */
-/* Sampled timing of the FP32 inner CG's fused product kernel: while enabled, every 8th launch
- * outside graph capture is bracketed by HIP events on its stream; read returns the number of
- * timed launches and their total milliseconds.  femd_inner_timer resets the samples.  No
- * counterpart in the reference. */
+/* Sampled timing of the inner CG's product kernel (the refined solves, batches and the time
+ * stepper): while enabled, every 8th launch outside graph capture is bracketed by HIP events on
+ * its stream (solves then run without graphs); read returns the number of timed launches and
+ * their total milliseconds.  femd_inner_timer resets the samples.  No counterpart in the
+ * reference. */
 int femd_inner_timer(int enable);
 int femd_inner_timer_read(int64_t *launches, double *total_ms);
+
+/* ---------------------------------------------------------------- value dictionaries
+ * The inner CG (refined solves, batches, time stepping) reads its scaled matrix through a
+ * lossless per-tile value dictionary whenever every SpMV tile holds at most 256 distinct values
+ * (constant-coefficient problems on structured meshes): 1 byte per entry instead of 4 or 8, the
+ * same values bit for bit, so results do not change.  mode 1 allows it (the default), 0 turns it
+ * off, -1 only queries; returns the previous mode.  No counterpart in the reference. */
+int femd_value_dictionary(int mode);
 int femd_event_timer_start(femd_stream s);
 int femd_event_timer_stop(femd_stream s, double *ms);
 

@@ -26,5 +26,6 @@ from .krylov import KrylovInfo, cg, gmres, lgmres, solve_refined
 from .linalg import solve
 from .batch import Batch, value_dictionary
 from .timestep import BackwardEuler
+from .serve import Server
 
 __version__ = "0.9.0"

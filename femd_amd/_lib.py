@@ -110,6 +110,11 @@ _SIGS = {
     "femd_inner_timer_read": (_i, [C.POINTER(_i64), C.POINTER(_d)]),
     "femd_value_dictionary": (_i, [_i]),
     "femd_batch_dictionary": (_i, [_p, _i, C.POINTER(_i64), C.POINTER(_i)]),
+    "femd_server_create": (_i, [_i64, _i, _i, _d, _d, _i, C.POINTER(_p)]),
+    "femd_server_submit": (_i, [_p, _i64, _p, _p, _p, C.POINTER(_i64)]),
+    "femd_server_step": (_i, [_p, _i, _p, _p, _p, _p, C.POINTER(_i)]),
+    "femd_server_status": (_i, [_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i64)]),
+    "femd_server_free": (None, [_p]),
 }
 
 _lib = None

@@ -8,22 +8,13 @@
 // reliable FP64 updates, in FP32 or FP64; kernels return at once for tiles and chunks of systems
 // that are not running, so uneven sizes and iteration counts share one launch sequence, captured
 // once into a hipGraph and replayed.
-#include "cg_engine.hpp"
+#include "batch.hpp"
 
 #include <cmath>
 
 using namespace femd_amd;
 
-struct femd_batch_s {
-    int np = 0;
-    int64_t nrows = 0, nnz = 0, max_rows = 0;
-    std::vector<femd_csr> members;
-    std::vector<int64_t> roff, voff;            // row and value offsets (np + 1)
-    std::unique_ptr<femd_csr_s> A;              // the block-diagonal matrix, one segment per system
-    CGEngine engine[2];                         // FP32 and FP64 inner iterations
-};
-
-namespace {
+namespace femd_amd {
 
 void batch_copy_values(femd_batch_s *B, hipStream_t s)
 {
@@ -38,6 +29,48 @@ void batch_copy_values(femd_batch_s *B, hipStream_t s)
     B->A->v64s_valid = false;
     scaled_values(B->A.get(), s);
 }
+
+std::unique_ptr<femd_batch_s> batch_build(int np, const femd_csr *mats, hipStream_t s)
+{
+    if (np < 1) fail(FEMD_EINVAL, "batch: at least one system");
+    if (!mats) fail(FEMD_EINVAL, "batch: null argument");
+    auto B = std::make_unique<femd_batch_s>();
+    B->np = np;
+    B->members.assign(mats, mats + np);
+    B->roff.assign(np + 1, 0);
+    B->voff.assign(np + 1, 0);
+    for (int p = 0; p < np; ++p)
+    {
+        femd_csr_s *M = check_csr(mats[p]);
+        if (M->pat->n != M->pat->m) fail(FEMD_EINVAL, "batch: every system must be square");
+        B->roff[p + 1] = B->roff[p] + M->pat->n;
+        B->voff[p + 1] = B->voff[p] + M->pat->nnz;
+        B->max_rows = std::max<int64_t>(B->max_rows, M->pat->n);
+    }
+    B->nrows = B->roff[np];
+    B->nnz = B->voff[np];
+    if (B->nnz >= (int64_t(1) << 31) || B->nrows >= (int64_t(1) << 31)) fail(FEMD_EINVAL, "batch: more than 2^31 rows or entries");
+    // the block-diagonal pattern (host copy for the tile plans), one segment per system
+    std::vector<int> ip(size_t(B->nrows) + 1), ix(size_t(B->nnz));
+    for (int p = 0; p < np; ++p)
+    {
+        const DevPattern &P = *mats[p]->pat;
+        const int r0 = static_cast<int>(B->roff[p]), v0 = static_cast<int>(B->voff[p]);
+        for (int i = 0; i <= P.n; ++i) ip[size_t(r0 + i)] = v0 + P.h_ip[size_t(i)];
+        for (int64_t k = 0; k < P.nnz; ++k) ix[size_t(v0 + k)] = r0 + P.h_ix[size_t(k)];
+    }
+    B->A = std::make_unique<femd_csr_s>();
+    B->A->pat = make_pattern(static_cast<int>(B->nrows), static_cast<int>(B->nrows), std::move(ip), std::move(ix), s);
+    B->A->pat->segments = B->roff;
+    B->A->v = DevBuf(8 * size_t(std::max<int64_t>(B->nnz, 1)));
+    batch_copy_values(B.get(), s);
+    HIPC(hipStreamSynchronize(s));
+    return B;
+}
+
+} // namespace femd_amd
+
+namespace {
 
 void batch_solve(femd_batch_s *B, const double *b, double *x, double tol, double inner_tol, int max_outer, int precision,
                  int use_graph, int *iters, double *res, femd_solve_info *info, hipStream_t s)
@@ -63,41 +96,8 @@ extern "C" {
 int femd_batch_create(int np, const femd_csr *mats, femd_batch *out)
 {
     return api([&] {
-        if (np < 1) fail(FEMD_EINVAL, "batch: at least one system");
-        if (!mats || !out) fail(FEMD_EINVAL, "batch: null argument");
-        hipStream_t s = S(nullptr);
-        auto B = std::make_unique<femd_batch_s>();
-        B->np = np;
-        B->members.assign(mats, mats + np);
-        B->roff.assign(np + 1, 0);
-        B->voff.assign(np + 1, 0);
-        for (int p = 0; p < np; ++p)
-        {
-            femd_csr_s *M = check_csr(mats[p]);
-            if (M->pat->n != M->pat->m) fail(FEMD_EINVAL, "batch: every system must be square");
-            B->roff[p + 1] = B->roff[p] + M->pat->n;
-            B->voff[p + 1] = B->voff[p] + M->pat->nnz;
-            B->max_rows = std::max<int64_t>(B->max_rows, M->pat->n);
-        }
-        B->nrows = B->roff[np];
-        B->nnz = B->voff[np];
-        if (B->nnz >= (int64_t(1) << 31) || B->nrows >= (int64_t(1) << 31)) fail(FEMD_EINVAL, "batch: more than 2^31 rows or entries");
-        // the block-diagonal pattern (host copy for the tile plans), one segment per system
-        std::vector<int> ip(size_t(B->nrows) + 1), ix(size_t(B->nnz));
-        for (int p = 0; p < np; ++p)
-        {
-            const DevPattern &P = *mats[p]->pat;
-            const int r0 = static_cast<int>(B->roff[p]), v0 = static_cast<int>(B->voff[p]);
-            for (int i = 0; i <= P.n; ++i) ip[size_t(r0 + i)] = v0 + P.h_ip[size_t(i)];
-            for (int64_t k = 0; k < P.nnz; ++k) ix[size_t(v0 + k)] = r0 + P.h_ix[size_t(k)];
-        }
-        B->A = std::make_unique<femd_csr_s>();
-        B->A->pat = make_pattern(static_cast<int>(B->nrows), static_cast<int>(B->nrows), std::move(ip), std::move(ix), s);
-        B->A->pat->segments = B->roff;
-        B->A->v = DevBuf(8 * size_t(std::max<int64_t>(B->nnz, 1)));
-        batch_copy_values(B.get(), s);
-        HIPC(hipStreamSynchronize(s));
-        *out = B.release();
+        if (!out) fail(FEMD_EINVAL, "batch: null argument");
+        *out = batch_build(np, mats, S(nullptr)).release();
     });
 }
 

@@ -671,95 +671,172 @@ std::vector<double> CGEngine::norms2(const double *b, const double *x, hipStream
     return h;
 }
 
-CGEngine::Result CGEngine::solve(const double *b, double *x, double tol, double inner_tol, int max_updates, bool use_graph,
-                                 int *iters, double *res, hipStream_t s)
+void CGEngine::put_int(DevBuf &d, const std::vector<int> &v, hipStream_t s)
+{
+    HIPC(hipMemcpyAsync(d.p, v.data(), 4 * size_t(np), hipMemcpyHostToDevice, s));
+}
+
+void CGEngine::begin(const double *b, double *x, double tol, double inner_tol, int max_updates, bool use_graph, hipStream_t s)
 {
     if (!(tol >= 0.0) || !(inner_tol > 0.0 && inner_tol < 1.0) || max_updates < 1)
         fail(FEMD_EINVAL, "inner CG: need tol >= 0, 0 < inner_tol < 1, max_updates >= 1");
-    Result out;
-    const int max_iter = static_cast<int>(std::min<int64_t>(1 << 30, std::max<int64_t>(1000, 10 * max_rows)));
-    if (timer_on()) use_graph = false;          // sampled kernel timing needs plain launches
-    std::vector<int> selh(np, 1), retiredh(np, 0), conv(np, 0), updates(np, 0);
-    std::vector<double> rel(np, 0.0), rho(np, 0.0), tg(np, 0.0);
-    auto put = [&](DevBuf &d, const std::vector<int> &v) { HIPC(hipMemcpyAsync(d.p, v.data(), 4 * size_t(np), hipMemcpyHostToDevice, s)); };
-    put(sel, selh);
-    const std::vector<double> nb2 = norms2(b, nullptr, s);
+    Run &R = run_;
+    R = Run{};
+    R.b = b;
+    R.x = x;
+    R.tol = tol;
+    R.inner_tol = inner_tol;
+    R.max_updates = max_updates;
+    R.max_iter = static_cast<int>(std::min<int64_t>(1 << 30, std::max<int64_t>(1000, 10 * max_rows)));
+    R.use_graph = use_graph && !timer_on();       // sampled kernel timing needs plain launches
+    R.sel.assign(np, 1);
+    R.retired.assign(np, 0);
+    R.conv.assign(np, 0);
+    R.updates.assign(np, 0);
+    R.rel.assign(np, 0.0);
+    R.hs.assign(np, CGSys{});
+    std::vector<double> rho(np, 0.0), tg(np, 0.0);
+    put_int(sel, R.sel, s);
+    R.nb2 = norms2(b, nullptr, s);
     for (int p = 0; p < np; ++p)
-        if (nb2[p] == 0.0 && roff[p + 1] > roff[p]) launch_fill(roff[p + 1] - roff[p], 0.0, x + roff[p], s);
-    std::vector<double> rr = norms2(b, x, s);
-    int running = 0;
+        if (R.nb2[p] == 0.0 && roff[p + 1] > roff[p]) launch_fill(roff[p + 1] - roff[p], 0.0, x + roff[p], s);
+    const std::vector<double> rr = norms2(b, x, s);
     for (int p = 0; p < np; ++p)
     {
-        const double nb = std::sqrt(nb2[p]), rn = std::sqrt(rr[p]);
-        rel[p] = nb > 0.0 ? rn / nb : 0.0;
-        if (nb == 0.0 || rel[p] <= tol) { conv[p] = 1; retiredh[p] = 1; selh[p] = 0; }
-        else { selh[p] = 2; rho[p] = rn; tg[p] = (tol * nb / rn) * (tol * nb / rn); ++running; }
+        const double nb = std::sqrt(R.nb2[p]), rn = std::sqrt(rr[p]);
+        R.rel[p] = nb > 0.0 ? rn / nb : 0.0;
+        if (nb == 0.0 || R.rel[p] <= tol)
+        {
+            R.conv[p] = 1;
+            R.retired[p] = 1;
+            R.sel[p] = 0;
+        }
+        else
+        {
+            R.sel[p] = 2;
+            rho[p] = rn;
+            tg[p] = (tol * nb / rn) * (tol * nb / rn);
+            ++R.running;
+        }
     }
+    std::vector<CGSys> z(np);
+    std::memset(z.data(), 0, sizeof(CGSys) * np);
+    HIPC(hipMemcpyAsync(st.p, z.data(), sizeof(CGSys) * np, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(rho_new.p, rho.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(tgt.p, tg.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
+    put_int(retired, R.retired, s);
+    put_int(sel, R.sel, s);
+    resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol * inner_tol, R.max_iter, s);
+    if (!counts) counts = std::make_unique<Pinned>(2 * sizeof(int));
+    HIPC(hipStreamSynchronize(s));                // the host vectors above go out of scope
+}
+
+void CGEngine::launch(hipStream_t s)
+{
+    if (run_.running == 0) return;
+    replay(run_.use_graph, s);
+    hipLaunchKernelGGL(k_cg_count, dim3(1), dim3(BT), 0, s, np, static_cast<const CGSys *>(st.as<CGSys>()), retired.as<int>(),
+                       cnt.as<int>());
+    HIPC(hipMemcpyAsync(counts->p, cnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    run_.launched = true;
+}
+
+std::vector<int> CGEngine::complete(hipStream_t s)
+{
+    Run &R = run_;
+    std::vector<int> done;
+    if (!R.launched) return done;
+    R.launched = false;
+    HIPC(hipStreamSynchronize(s));
+    const int running_now = counts->as<int>()[0], paused = counts->as<int>()[1];
+    if (paused == 0 && running_now > 0) return done;
+    HIPC(hipMemcpyAsync(R.hs.data(), st.p, sizeof(CGSys) * np, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (paused == 0)
     {
-        std::vector<CGSys> z(np);
-        std::memset(z.data(), 0, sizeof(CGSys) * np);
-        HIPC(hipMemcpyAsync(st.p, z.data(), sizeof(CGSys) * np, hipMemcpyHostToDevice, s));
+        // nothing runs and nothing waits for an update: whatever is left cannot go on
+        for (int p = 0; p < np; ++p)
+            if (!R.retired[p])
+            {
+                R.retired[p] = 1;
+                done.push_back(p);
+            }
+        R.running = 0;
+        return done;
+    }
+    // reliable updates of the paused systems
+    ++R.rounds;
+    for (int p = 0; p < np; ++p) R.sel[p] = (!R.retired[p] && !R.hs[p].active && R.hs[p].reason) ? 1 : 0;
+    put_int(sel, R.sel, s);
+    finish_device(sel.as<int>(), R.x, s);
+    const std::vector<double> rr = norms2(R.b, R.x, s);
+    std::vector<double> rho(np, 0.0), tg(np, 0.0);
+    for (int p = 0; p < np; ++p)
+    {
+        if (!R.sel[p]) continue;
+        ++R.updates[p];
+        const double rn = std::sqrt(rr[p]), nb = std::sqrt(R.nb2[p]);
+        R.rel[p] = rn / nb;
+        if (R.rel[p] <= R.tol || R.hs[p].total >= R.max_iter || R.updates[p] >= R.max_updates)
+        {
+            R.conv[p] = R.rel[p] <= R.tol ? 1 : 0;
+            R.retired[p] = 1;
+            R.sel[p] = 0;
+            done.push_back(p);
+            continue;
+        }
+        R.sel[p] = R.hs[p].reason == 2 ? 2 : 1;
+        rho[p] = rn;
+        tg[p] = (R.tol * nb / rn) * (R.tol * nb / rn);
     }
     HIPC(hipMemcpyAsync(rho_new.p, rho.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
     HIPC(hipMemcpyAsync(tgt.p, tg.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
-    put(retired, retiredh);
-    put(sel, selh);
-    resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol * inner_tol, max_iter, s);
-    Pinned hc(2 * sizeof(int));
-    std::vector<CGSys> hs(np);
-    while (running > 0)
-    {
-        replay(use_graph, s);
-        hipLaunchKernelGGL(k_cg_count, dim3(1), dim3(BT), 0, s, np, static_cast<const CGSys *>(st.as<CGSys>()), retired.as<int>(),
-                           cnt.as<int>());
-        HIPC(hipMemcpyAsync(hc.p, cnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        const int run = hc.as<int>()[0], paused = hc.as<int>()[1];
-        if (paused == 0 && run > 0) continue;
-        if (paused == 0 && run == 0) break;
-        // reliable updates of the paused systems
-        ++out.rounds;
-        HIPC(hipMemcpyAsync(hs.data(), st.p, sizeof(CGSys) * np, hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        for (int p = 0; p < np; ++p) selh[p] = (!retiredh[p] && !hs[p].active && hs[p].reason) ? 1 : 0;
-        put(sel, selh);
-        finish_device(sel.as<int>(), x, s);
-        rr = norms2(b, x, s);
-        for (int p = 0; p < np; ++p)
-        {
-            rho[p] = 0.0;
-            if (!selh[p]) continue;
-            ++updates[p];
-            const double rn = std::sqrt(rr[p]);
-            rel[p] = rn / std::sqrt(nb2[p]);
-            if (rel[p] <= tol) { conv[p] = 1; retiredh[p] = 1; selh[p] = 0; continue; }
-            if (hs[p].total >= max_iter || updates[p] >= max_updates) { retiredh[p] = 1; selh[p] = 0; continue; }
-            selh[p] = hs[p].reason == 2 ? 2 : 1;
-            rho[p] = rn;
-            tg[p] = (tol * std::sqrt(nb2[p]) / rn) * (tol * std::sqrt(nb2[p]) / rn);
-        }
-        HIPC(hipMemcpyAsync(rho_new.p, rho.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(tgt.p, tg.data(), 8 * size_t(np), hipMemcpyHostToDevice, s));
-        put(retired, retiredh);
-        put(sel, selh);
-        resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol * inner_tol, max_iter, s);
-        running = 0;
-        for (int p = 0; p < np; ++p) running += retiredh[p] ? 0 : 1;
-    }
-    HIPC(hipMemcpyAsync(hs.data(), st.p, sizeof(CGSys) * np, hipMemcpyDeviceToHost, s));
+    put_int(retired, R.retired, s);
+    put_int(sel, R.sel, s);
+    resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), R.inner_tol * R.inner_tol, R.max_iter, s);
+    R.running = 0;
+    for (int p = 0; p < np; ++p) R.running += R.retired[p] ? 0 : 1;
+    HIPC(hipStreamSynchronize(s));                // rho, tg go out of scope
+    return done;
+}
+
+void CGEngine::system_result(int p, int *iters, double *res, int *converged) const
+{
+    if (iters) *iters = run_.hs[p].total;
+    if (res) *res = run_.rel[p];
+    if (converged) *converged = run_.conv[p];
+}
+
+CGEngine::Result CGEngine::result(int *iters, double *res, hipStream_t s)
+{
+    HIPC(hipMemcpyAsync(run_.hs.data(), st.p, sizeof(CGSys) * np, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    Result out;
     int nconv = 0;
     for (int p = 0; p < np; ++p)
     {
-        if (iters) iters[p] = hs[p].total;
-        if (res) res[p] = rel[p];
-        nconv += conv[p];
-        out.worst = std::max(out.worst, rel[p]);
-        out.iterations += hs[p].total;
-        out.updates += updates[p];
+        if (iters) iters[p] = run_.hs[p].total;
+        if (res) res[p] = run_.rel[p];
+        nconv += run_.conv[p];
+        out.worst = std::max(out.worst, run_.rel[p]);
+        out.iterations += run_.hs[p].total;
+        out.updates += run_.updates[p];
     }
+    out.rounds = run_.rounds;
     out.converged = nconv == np;
     return out;
+}
+
+CGEngine::Result CGEngine::solve(const double *b, double *x, double tol, double inner_tol, int max_updates, bool use_graph,
+                                 int *iters, double *res, hipStream_t s)
+{
+    begin(b, x, tol, inner_tol, max_updates, use_graph, s);
+    while (!finished())
+    {
+        launch(s);
+        (void)complete(s);
+    }
+    return result(iters, res, s);
 }
 
 } // namespace femd_amd

@@ -84,6 +84,18 @@ struct CGEngine {
     Result solve(const double *b, double *x, double tol, double inner_tol, int max_updates, bool use_graph, int *iters,
                  double *res, hipStream_t s);
 
+    // The same solve in steps, for a host loop that drives several engines at once: begin (b and x
+    // stay in use until the solve has finished), then launch (queues one graph replay and the
+    // read-back of the running / paused counts, returns at once) and complete (waits for it, runs
+    // the reliable updates of the paused systems and returns the systems retired by this step)
+    // until finished().  A retired system's x and system_result are final.
+    void begin(const double *b, double *x, double tol, double inner_tol, int max_updates, bool use_graph, hipStream_t s);
+    void launch(hipStream_t s);
+    std::vector<int> complete(hipStream_t s);
+    bool finished() const { return run_.running == 0 && !run_.launched; }
+    void system_result(int p, int *iters, double *res, int *converged) const;
+    Result result(int *iters, double *res, hipStream_t s);
+
     // Device-driven pieces for a captured loop (the time stepper), on the systems with
     // sel[sys] != 0: start from the FP64 residual r64_dev (norms in rho_dev; 0 = nothing to do)
     // with no direction, pausing at a drop by inner_tol or when the unscaled residual estimate
@@ -93,6 +105,18 @@ struct CGEngine {
     void finish_device(const int *sel_dev, double *x, hipStream_t s);
 
   private:
+    struct Run {
+        const double *b = nullptr;
+        double *x = nullptr;
+        double tol = 0.0, inner_tol = 0.5;
+        int max_updates = 1, max_iter = 1, running = 0, rounds = 0;
+        bool use_graph = true, launched = false;
+        std::vector<int> sel, retired, conv, updates;
+        std::vector<double> rel, nb2;
+        std::vector<CGSys> hs;
+    } run_;
+    std::unique_ptr<Pinned> counts;
+    void put_int(DevBuf &d, const std::vector<int> &v, hipStream_t s);
     void bind(femd_csr_s *A, int precision, hipStream_t s);
     std::vector<double> norms2(const double *b, const double *x, hipStream_t s);
     void resume(const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev, double tol2,

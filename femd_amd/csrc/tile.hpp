@@ -63,29 +63,25 @@ template <typename T> struct TileDict {
 
 typedef unsigned int u32;
 
-// One tile (workgroup blockIdx.x).  Op supplies gather(j) (the x value at global column j),
-// own(r) (what the row epilogue needs of row r) and store(r, s, own) (writes row r's result and
-// returns its share of the workgroup's partials, of type Op::Part).  Returns this thread's share.
-// `stop` (may be NULL) is read together with the tile's bounds: a set flag ends the workgroup
-// before it loads the matrix, and its latency hides behind theirs.  With `mask` (may be NULL) the
-// tile is skipped when mask[tile_sys[b]] is zero (the batched solver: tile b belongs to system
-// tile_sys[b]).  *stopped tells the caller.
+// A tile's bounds: rows [r0, r1), entries [p0, p1), distinct columns ucol[u0 .. u1).
+struct TileDesc {
+    int r0, p0, u0, r1, p1, u1;
+};
+
+// The work of one tile b (the whole workgroup) once its bounds are known.  Op supplies gather(j)
+// (the x value at global column j), own(r) (what the row epilogue needs of row r) and
+// store(r, s, own) (writes row r's result and returns its share of the workgroup's partials, of
+// type Op::Part).  Returns this thread's share.
 template <typename T, int THREADS, int CAP, class Op, class VS = NoDict>
-__device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr, const int *__restrict__ ucol,
-                                     const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
-                                     const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op, T *xs,
-                                     T *prod, const int *stop, bool *stopped, const int *tile_sys = nullptr,
-                                     const int *mask = nullptr, const VS &vs = VS{})
+__device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const int *__restrict__ ucol,
+                                              const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
+                                              const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op,
+                                              T *xs, T *prod, const VS &vs = VS{})
 {
     constexpr int NGRP = (CAP / 4 + 1 + THREADS - 1) / THREADS;
     constexpr int NU = (CAP + THREADS - 1) / THREADS;
     static_assert(!VS::on || THREADS >= 256, "a dictionary of 256 values is staged one per lane");
-    const int b = blockIdx.x;
-    const int2 b0 = blk[b], b1 = blk[b + 1];
-    const int halt = (stop && *stop) || (mask && !mask[tile_sys[b]]);
-    *stopped = halt != 0;
-    if (halt) return typename Op::Part{};
-    const int r0 = b0.x, r1 = b1.x, p0 = b0.y, len = b1.y - p0;
+    const int r0 = td.r0, r1 = td.r1, p0 = td.p0, len = td.p1 - td.p0;
     const int r = r0 + static_cast<int>(threadIdx.x);
     int rb = 0, re = 0;
     typename Op::Own own{};
@@ -115,7 +111,7 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
         return part;
     }
     // 1. distinct columns (and the tile's value dictionary) and entry loads
-    const int u0 = uptr[b], nu = uptr[b + 1] - u0;
+    const int u0 = td.u0, nu = td.u1 - td.u0;
     int uc[NU];
 #pragma unroll
     for (int i = 0; i < NU; ++i)
@@ -210,6 +206,28 @@ __device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, co
         part = op.store(r, s, own);
     }
     return part;
+}
+
+
+// One tile (workgroup blockIdx.x) of a plan's tile table: bounds from blk (row, entry) and uptr.
+// `stop` (may be NULL) is read together with the tile's bounds: a set flag ends the workgroup
+// before it loads the matrix, and its latency hides behind theirs.  With `mask` (may be NULL) the
+// tile is skipped when mask[tile_sys[b]] is zero (tile b belongs to system tile_sys[b]).
+// *stopped tells the caller.
+template <typename T, int THREADS, int CAP, class Op, class VS = NoDict>
+__device__ inline typename Op::Part stream_tile(const int2 *__restrict__ blk, const int *__restrict__ uptr, const int *__restrict__ ucol,
+                                     const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
+                                     const int *__restrict__ ix, const T *__restrict__ v, int64_t nnz, const Op &op, T *xs,
+                                     T *prod, const int *stop, bool *stopped, const int *tile_sys = nullptr,
+                                     const int *mask = nullptr, const VS &vs = VS{})
+{
+    const int b = blockIdx.x;
+    const int2 b0 = blk[b], b1 = blk[b + 1];
+    const int halt = (stop && *stop) || (mask && !mask[tile_sys[b]]);
+    *stopped = halt != 0;
+    if (halt) return typename Op::Part{};
+    const TileDesc td{b0.x, b0.y, uptr[b], b1.x, b1.y, uptr[b + 1]};
+    return tile_work<T, THREADS, CAP>(td, b, ucol, lidx, ip, ix, v, nnz, op, xs, prod, vs);
 }
 
 // Sum of v over the workgroup in a fixed order (wave shuffles, then the waves in order); valid in

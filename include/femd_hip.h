@@ -273,6 +273,29 @@ int femd_batch_solve_refined_dev(femd_batch B, const double *b_dev, double *x_de
 int femd_batch_dictionary(femd_batch B, int precision, int64_t *distinct, int *max_per_tile);
 void femd_batch_free(femd_batch B);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop for independent square SPD systems of uneven sizes (the serving pattern; no
+ * reference counterpart).  femd_server_submit queues a request: an assembled matrix A, a
+ * right-hand side b_dev and an output x_dev holding the initial guess, all on the device and kept
+ * alive by the caller until the request is reported done; *bytes (may be NULL) gets the device
+ * memory it will hold while admitted.  Each femd_server_step queues one round of iterations of
+ * every running group, admits in arrival order the waiting requests that fit hbm_budget and
+ * max_systems as a new group (its own batch, engine and stream, so groups overlap on the device),
+ * completes the round -- reliable FP64 updates of paused systems, and every system that has met
+ * tol retires at once, its solution copied to its x_dev -- and frees finished groups.  It returns
+ * up to max_done finished requests (tag, inner iterations, relative residual, converged); the
+ * caller keeps stepping while requests are queued or running.  Results are bit-identical whatever
+ * else is admitted alongside.  Inner iterations in `precision` (FEMD_FP32 with FP64 reliable
+ * updates, or FEMD_FP64), inner_tol and max_outer as femd_batch_solve. */
+typedef struct femd_server_s *femd_server;
+int femd_server_create(int64_t hbm_budget, int max_systems, int precision, double tol, double inner_tol, int max_outer,
+                       femd_server *out);
+int femd_server_submit(femd_server S, int64_t tag, femd_csr A, const double *b_dev, double *x_dev, int64_t *bytes);
+int femd_server_step(femd_server S, int max_done, int64_t *tags, int *iterations, double *residuals, int *converged,
+                     int *ndone);
+int femd_server_status(femd_server S, int *queued, int *running, int *groups, int64_t *bytes_in_use);
+void femd_server_free(femd_server S);
+
 /* ---------------------------------------------------------------- time stepping
  * Stepping with a fixed matrix pair: u_{k+1} solves A u_{k+1} = M u_k (backward Euler for
  * M u' + K u = 0 with A = M + dt K).  No reference counterpart as one call: the reference writes

@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="every solve with plain launches (for profilers that cannot follow graph replays)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -66,6 +68,8 @@ def main():
 
     lib = L.load()
     L.check(lib.femd_init(0))
+    if args.no_graph:
+        L.check(lib.femd_inner_timer(1))       # the sampled kernel timer runs solves without graphs
 
     V, a_form, L_form, pinfo = problems.p3_kappa_problem(args.h)
     P, T, min_ang = pinfo["points"], pinfo["triangles"], pinfo["min_angle"]

@@ -14,18 +14,29 @@ namespace femd_amd {
 namespace {
 
 constexpr int BT = 256;
-constexpr int BCHUNK = 4096;
+constexpr int BCHUNK = 1024;
+constexpr int RT = 1024;           // threads of the per-system reductions
 constexpr int GRAPH_ITERS = 16;
 
 inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + BT - 1) / BT)); }
 
-// The sum of a[0..n) (stride `stride`) in a fixed order -- lane-strided, wave shuffles, then the
-// waves in order -- returned to every thread of the workgroup.
+// The sum of a[0..n) (stride `stride`) in a fixed order -- four interleaved lane-strided
+// partial sums per lane (independent loads in flight), wave shuffles, then the waves in order --
+// returned to every thread of the workgroup.  The order depends only on n and THREADS.
 template <int THREADS>
 __device__ inline double wg_fixed_sum(const double *a, int n, double *red, int stride = 1)
 {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += THREADS) s += a[static_cast<int64_t>(i) * stride];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * THREADS < n; i += 4 * THREADS)
+    {
+        s0 += a[static_cast<int64_t>(i) * stride];
+        s1 += a[static_cast<int64_t>(i + THREADS) * stride];
+        s2 += a[static_cast<int64_t>(i + 2 * THREADS) * stride];
+        s3 += a[static_cast<int64_t>(i + 3 * THREADS) * stride];
+    }
+    for (; i < n; i += THREADS) s0 += a[static_cast<int64_t>(i) * stride];
+    double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -97,13 +108,13 @@ __global__ __launch_bounds__(THREADS) void k_cg_product(const int2 *__restrict__
 }
 
 // one workgroup per system: p'.q over its tiles, alpha
-__global__ __launch_bounds__(BT) void k_cg_alpha(const int *__restrict__ seg_tiles, const double *__restrict__ tparts, CGSys *st)
+__global__ __launch_bounds__(RT) void k_cg_alpha(const int *__restrict__ seg_tiles, const double *__restrict__ tparts, CGSys *st)
 {
     CGSys &S = st[blockIdx.x];
     if (!S.active) return;
-    __shared__ double red[BT / 64];
+    __shared__ double red[RT / 64];
     const int t0 = seg_tiles[blockIdx.x], t1 = seg_tiles[blockIdx.x + 1];
-    const double pq = wg_fixed_sum<BT>(tparts + t0, t1 - t0, red);
+    const double pq = wg_fixed_sum<RT>(tparts + t0, t1 - t0, red);
     if (threadIdx.x != 0) return;
     if (!(pq > 0.0))
     {
@@ -163,14 +174,14 @@ __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ cl
 }
 
 // one workgroup per system: the new r.r and r.D r over its chunks, the pause tests, beta
-__global__ __launch_bounds__(BT) void k_cg_beta(const int *__restrict__ cptr, const double *__restrict__ cparts, CGSys *st)
+__global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, const double *__restrict__ cparts, CGSys *st)
 {
     CGSys &S = st[blockIdx.x];
     if (!S.active) return;
-    __shared__ double red[BT / 64];
+    __shared__ double red[RT / 64];
     const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
-    const double rr = wg_fixed_sum<BT>(cparts + 2 * c0, c1 - c0, red, 2);
-    const double rt = wg_fixed_sum<BT>(cparts + 2 * c0 + 1, c1 - c0, red, 2);
+    const double rr = wg_fixed_sum<RT>(cparts + 2 * c0, c1 - c0, red, 2);
+    const double rt = wg_fixed_sum<RT>(cparts + 2 * c0 + 1, c1 - c0, red, 2);
     if (threadIdx.x != 0) return;
     S.it += 1;
     S.total += 1;
@@ -230,13 +241,13 @@ __global__ __launch_bounds__(BT) void k_cg_residual(const int64_t *__restrict__ 
 }
 
 // one workgroup per system: the sum of its chunks' partials
-__global__ __launch_bounds__(BT) void k_cg_sys_sum(const int *__restrict__ cptr, const int *__restrict__ sel,
+__global__ __launch_bounds__(RT) void k_cg_sys_sum(const int *__restrict__ cptr, const int *__restrict__ sel,
                                                    const double *__restrict__ cparts, double *out)
 {
     if (!sel[blockIdx.x]) return;
-    __shared__ double red[BT / 64];
+    __shared__ double red[RT / 64];
     const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
-    const double t = wg_fixed_sum<BT>(cparts + c0, c1 - c0, red);
+    const double t = wg_fixed_sum<RT>(cparts + c0, c1 - c0, red);
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
@@ -317,27 +328,27 @@ __global__ __launch_bounds__(BT) void k_cg_fresh_dots(const int64_t *__restrict_
 
 // one workgroup per system: r.q, p.q, r.r over its chunks; beta that makes the next direction
 // A-conjugate to the current one (0 on a restart), the segment's threshold, running again
-__global__ __launch_bounds__(BT) void k_cg_fresh_state(const int *__restrict__ cptr, const int *__restrict__ sel,
+__global__ __launch_bounds__(RT) void k_cg_fresh_state(const int *__restrict__ cptr, const int *__restrict__ sel,
                                                        const double *__restrict__ c3, const double *__restrict__ rho_new,
                                                        const double *__restrict__ tgt, double tol2, int max_iter, CGSys *st)
 {
     const int sys = blockIdx.x, mode = sel[sys];
     if (!mode) return;
-    __shared__ double red[BT / 64];
+    __shared__ double red[RT / 64];
     const int c0 = cptr[sys], c1 = cptr[sys + 1];
-    const double rq = wg_fixed_sum<BT>(c3 + 3 * c0, c1 - c0, red, 3);
-    const double pq = wg_fixed_sum<BT>(c3 + 3 * c0 + 1, c1 - c0, red, 3);
-    const double rr = wg_fixed_sum<BT>(c3 + 3 * c0 + 2, c1 - c0, red, 3);
+    const double rq = wg_fixed_sum<RT>(c3 + 3 * c0, c1 - c0, red, 3);
+    const double pq = wg_fixed_sum<RT>(c3 + 3 * c0 + 1, c1 - c0, red, 3);
+    const double rr = wg_fixed_sum<RT>(c3 + 3 * c0 + 2, c1 - c0, red, 3);
     if (threadIdx.x != 0) return;
     CGSys &S = st[sys];
     const double rn = rho_new[sys];
     if (mode == 2)
     {
         S.it = 0;
-        S.max_iter = max_iter;
         S.beta = 0.0;
     }
     else S.beta = pq > 0.0 ? -rq / pq : 0.0;
+    S.max_iter = S.it + max_iter;           // a cap per segment; it keeps counting (direction parity)
     S.tol2 = tol2;
     S.tgt = tgt ? tgt[sys] : 0.0;
     S.rr = rr;
@@ -576,11 +587,11 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
     {
         if (E.use_dict) launch_product<T, true>(E, s);
         else launch_product<T, false>(E, s);
-        hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(BT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
+        hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(RT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), E.y.as<T>(),
                            E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(),
                            E.cparts.as<double>(), static_cast<const CGSys *>(st));
-        hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(BT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), st);
+        hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), st);
     }
 }
 
@@ -594,7 +605,7 @@ void resume_t(CGEngine &E, const double *r64_dev, const double *rho_dev, const d
     hipLaunchKernelGGL(k_cg_fresh_dots<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev,
                        E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), static_cast<const CGSys *>(E.st.as<CGSys>()),
                        E.cparts.as<double>());
-    hipLaunchKernelGGL(k_cg_fresh_state, dim3(E.np), dim3(BT), 0, s, E.cptr.as<int>(), sel_dev, E.cparts.as<double>(), rho_dev,
+    hipLaunchKernelGGL(k_cg_fresh_state, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), sel_dev, E.cparts.as<double>(), rho_dev,
                        tgt_dev, tol2, max_iter, E.st.as<CGSys>());
 }
 
@@ -663,7 +674,7 @@ std::vector<double> CGEngine::norms2(const double *b, const double *x, hipStream
     }
     hipLaunchKernelGGL(k_cg_residual, dim3(nch), dim3(BT), 0, s, clo.as<int64_t>(), cpid.as<int>(), sel.as<int>(), b,
                        x ? t64.as<double>() : nullptr, r64.as<double>(), cparts.as<double>());
-    hipLaunchKernelGGL(k_cg_sys_sum, dim3(np), dim3(BT), 0, s, cptr.as<int>(), sel.as<int>(), cparts.as<double>(),
+    hipLaunchKernelGGL(k_cg_sys_sum, dim3(np), dim3(RT), 0, s, cptr.as<int>(), sel.as<int>(), cparts.as<double>(),
                        rho_new.as<double>());
     std::vector<double> h(np, 0.0);
     HIPC(hipMemcpyAsync(h.data(), rho_new.p, 8 * size_t(np), hipMemcpyDeviceToHost, s));

@@ -26,8 +26,8 @@
 
 namespace femd_amd {
 
-// The state of one system.  reason: 0 running, 1 r.r dropped by inner_tol^2 (a reliable update is
-// due), 2 breakdown, 3 iteration cap.
+// The state of one system.  reason: 0 running, 1 r.r dropped by inner_tol^2 or the residual
+// estimate met its target (a reliable update is due), 2 breakdown, 3 the segment's iteration cap.
 struct CGSys {
     double rr;           // r.r of the current residual (scaled system)
     double alpha, beta;  // alpha of the current iteration; beta for the next product
@@ -35,7 +35,7 @@ struct CGSys {
     double tgt;          // or once r.D r <= tgt  ((tol ||b|| / rho)^2; 0: no such test)
     double tol2;         // inner_tol^2
     double rho;          // the FP64 residual norm the system is scaled by
-    int it, total, max_iter, active, reason, pad;
+    int it, total, max_iter, active, reason, pad;   // it: since the last restart; total: all
 };
 
 // S = diag(A)^(-1/2) (FP64) and S A S (FP32 / FP64) on the matrix, cached until its values change;
@@ -54,7 +54,7 @@ struct CGEngine {
     int np = 0;
     int64_t n = 0, max_rows = 0;
     std::vector<int64_t> roff;          // system row offsets (np + 1)
-    int nch = 0;                        // vector chunks (4096 rows at most, one system each)
+    int nch = 0;                        // vector chunks (1024 rows at most, one system each)
     std::vector<int> h_cptr;
     DevBuf clo, cpid, cptr;
     const SpmvPlan *plan = nullptr;
@@ -97,9 +97,10 @@ struct CGEngine {
     Result result(int *iters, double *res, hipStream_t s);
 
     // Device-driven pieces for a captured loop (the time stepper), on the systems with
-    // sel[sys] != 0: start from the FP64 residual r64_dev (norms in rho_dev; 0 = nothing to do)
-    // with no direction, pausing at a drop by inner_tol or when the unscaled residual estimate
-    // reaches sqrt(tgt_dev[sys]) rho (tgt_dev may be NULL), and fold the solution into x.
+    // sel[sys] != 0: start a segment of at most max_iter iterations from the FP64 residual r64_dev
+    // (norms in rho_dev; 0 = nothing to do) -- with no direction (sel 2) or keeping the current one
+    // (sel 1, a reliable update) -- pausing at a drop by inner_tol or when the unscaled residual
+    // estimate reaches sqrt(tgt_dev[sys]) rho (tgt_dev may be NULL); fold the solution into x.
     void start_device(const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev,
                       double inner_tol, int max_iter, hipStream_t s);
     void finish_device(const int *sel_dev, double *x, hipStream_t s);

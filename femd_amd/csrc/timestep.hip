@@ -4,10 +4,12 @@
 //
 // Backward Euler for M u' + K u = 0 is  A u_{n+1} = M u_n  with A = M + dt K.  One step:
 //   b = M u (FP64, the reference's row order), bb = b.b;
-//   `rounds` times: r = b - A u (FP64), rr = r.r; unless rr <= tol^2 bb, the FP32 scaled
-//   Jacobi-CG solves A d = r / sqrt(rr) from d = 0 (warm start: the step starts from u_n) to a
-//   relative residual of inner_tol within max_inner iterations (cg_engine.hpp, one system, no
-//   starting direction), and u += sqrt(rr) d (FP64);
+//   `rounds` times: r = b - A u (FP64), rr = r.r; unless rr <= tol^2 bb, a segment of at most
+//   max_inner iterations of the scaled Jacobi-CG (cg_engine.hpp, one system, FP32 or FP64) on
+//   A d = r / sqrt(rr) from d = 0, until its residual has dropped by inner_tol or its estimate of
+//   ||b - A u|| meets tol ||b||, and u += sqrt(rr) d (FP64).  The first round of a step starts
+//   with no search direction (the step starts from u_n: a warm start); the later rounds keep the
+//   direction, conjugate to the next one -- reliable updates of one CG run, not restarts;
 //   the step's inner iterations are added to a per-step counter on the device.
 // Every scalar lives on the device and every kernel of a finished solve returns at once, so the
 // whole step is one fixed launch sequence: captured once, replayed nsteps times back to back, and
@@ -25,6 +27,8 @@ struct StepScalars {
     double rr;          // ||b - A u||^2 of the current round (0 when the round is skipped)
     int counter;        // steps done
     int total_inner;
+    int seen;           // the engine's iteration total at the last record
+    int pad;
 };
 
 __global__ void k_gate(StepScalars *sc, double tol2)
@@ -43,7 +47,8 @@ __global__ void k_rho(const StepScalars *sc, double tol2, double *rho, double *t
 __global__ void k_record(const CGSys *st, StepScalars *sc, int *iters, int nmax)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int it = st->it;
+    const int it = st->total - sc->seen;
+    sc->seen = st->total;
     if (iters && sc->counter < nmax) iters[sc->counter] += it;
     sc->total_inner += it;
 }
@@ -55,11 +60,11 @@ __global__ void k_next_step(StepScalars *sc)
 
 struct Stepper {
     femd_csr_s *M = nullptr, *A = nullptr;
-    int n = 0, rounds = 1, max_inner = 1, nsteps = 0;
+    int n = 0, rounds = 1, max_inner = 1, nsteps = 0, precision = FEMD_FP32;
     double tol = 0.0, inner_tol = 0.0;
     const SpmvPlan *pM = nullptr, *pA = nullptr;
     CGEngine cg;
-    DevBuf b, t, r, parts, sc, iters, rho, tgt, sel;
+    DevBuf b, t, r, parts, sc, iters, rho, tgt, sel;     // sel: {2, 1}, restart and resume
     double *u = nullptr;
 
     StepScalars *scal() const { return sc.as<StepScalars>(); }
@@ -81,7 +86,8 @@ struct Stepper {
             norm2(r.as<double>(), &scal()->rr, s);
             hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, scal(), tol * tol);
             hipLaunchKernelGGL(k_rho, dim3(1), dim3(64), 0, s, scal(), tol * tol, rho.as<double>(), tgt.as<double>());
-            cg.start_device(r.as<double>(), rho.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol, max_inner, s);
+            cg.start_device(r.as<double>(), rho.as<double>(), tgt.as<double>(), sel.as<int>() + (k == 0 ? 0 : 1), inner_tol,
+                            max_inner, s);
             cg.iterate(max_inner, s);
             cg.finish_device(sel.as<int>(), u, s);
             hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, cg.st.as<CGSys>(), scal(), iters.as<int>(), nsteps);
@@ -91,32 +97,33 @@ struct Stepper {
 };
 
 void heat_run(femd_csr_s *M, femd_csr_s *A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
-              int use_graph, int *iters_out, femd_solve_info *info, hipStream_t s)
+              int precision, int use_graph, int *iters_out, femd_solve_info *info, hipStream_t s)
 {
     const DevPattern &PM = *M->pat, &PA = *A->pat;
     const int n = PA.n;
     if (PA.m != n || PM.n != n || PM.m != n) fail(FEMD_EINVAL, "time stepping: M and A must be square and of one size");
-    if (nsteps < 0 || rounds < 1 || max_inner < 1 || !(tol >= 0.0) || !(inner_tol > 0.0))
-        fail(FEMD_EINVAL, "time stepping: need nsteps >= 0, rounds >= 1, max_inner >= 1, tol >= 0, inner_tol > 0");
+    if (nsteps < 0 || rounds < 1 || max_inner < 1 || !(tol >= 0.0) || !(inner_tol > 0.0 && inner_tol < 1.0))
+        fail(FEMD_EINVAL, "time stepping: need nsteps >= 0, rounds >= 1, max_inner >= 1, tol >= 0, 0 < inner_tol < 1");
+    if (precision != FEMD_FP32 && precision != FEMD_FP64) fail(FEMD_EINVAL, "time stepping: precision is FEMD_FP32 or FEMD_FP64");
     std::memset(info, 0, sizeof(*info));
     if (iters_out && nsteps > 0) std::memset(iters_out, 0, sizeof(int) * size_t(nsteps));
     if (n == 0) { info->converged = 1; return; }
     Stepper T;
     T.M = M; T.A = A; T.n = n; T.rounds = rounds; T.max_inner = max_inner; T.nsteps = nsteps;
-    T.tol = tol; T.inner_tol = inner_tol; T.u = u;
+    T.tol = tol; T.inner_tol = inner_tol; T.u = u; T.precision = precision;
     // everything that tunes, allocates or synchronizes happens before the capture
     T.pM = &spmv_plan(*M->pat, FEMD_FP64, M->v.p, s);
     T.pA = &spmv_plan(*A->pat, FEMD_FP64, A->v.p, s);
-    T.cg.setup(A, FEMD_FP32, s);
+    T.cg.setup(A, precision, s);
     const size_t N = size_t(n);
     T.b = DevBuf(8 * N); T.t = DevBuf(8 * N); T.r = DevBuf(8 * N);
     T.parts = DevBuf(8 * 2048 + 64);
     T.sc = DevBuf(sizeof(StepScalars));
     T.rho = DevBuf(sizeof(double));
     T.tgt = DevBuf(sizeof(double));
-    T.sel = DevBuf(sizeof(int));
-    const int restart = 2;      // every inner solve starts from no direction
-    HIPC(hipMemcpyAsync(T.sel.p, &restart, sizeof(int), hipMemcpyHostToDevice, s));
+    T.sel = DevBuf(2 * sizeof(int));
+    const int modes[2] = {2, 1};
+    HIPC(hipMemcpyAsync(T.sel.p, modes, sizeof(modes), hipMemcpyHostToDevice, s));
     T.iters = DevBuf(sizeof(int) * size_t(std::max(nsteps, 1)));
     const StepScalars z{};
     HIPC(hipMemcpyAsync(T.sc.p, &z, sizeof(z), hipMemcpyHostToDevice, s));
@@ -177,16 +184,17 @@ using namespace femd_amd;
 extern "C" {
 
 int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,
-                      int max_inner, int use_graph, int *iters, femd_solve_info *info, femd_stream st)
+                      int max_inner, int precision, int use_graph, int *iters, femd_solve_info *info, femd_stream st)
 {
     return api([&] {
         if (!info) fail(FEMD_EINVAL, "time stepping: info is NULL");
-        heat_run(check_csr(M), check_csr(A), u_dev, nsteps, tol, rounds, inner_tol, max_inner, use_graph, iters, info, S(st));
+        heat_run(check_csr(M), check_csr(A), u_dev, nsteps, tol, rounds, inner_tol, max_inner, precision, use_graph, iters,
+                 info, S(st));
     });
 }
 
 int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
-                  int use_graph, int *iters, femd_solve_info *info)
+                  int precision, int use_graph, int *iters, femd_solve_info *info)
 {
     return api([&] {
         if (!info) fail(FEMD_EINVAL, "time stepping: info is NULL");
@@ -194,7 +202,8 @@ int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int
         hipStream_t s = S(nullptr);
         const int n = a->pat->n;
         DevBuf ud = upload(u, size_t(n), s);
-        heat_run(check_csr(M), a, ud.as<double>(), nsteps, tol, rounds, inner_tol, max_inner, use_graph, iters, info, s);
+        heat_run(check_csr(M), a, ud.as<double>(), nsteps, tol, rounds, inner_tol, max_inner, precision, use_graph, iters,
+                 info, s);
         if (n) HIPC(hipMemcpyAsync(u, ud.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
     });

@@ -86,6 +86,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--h", type=float, default=None)
     ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--cg-tol", type=float, default=None,
+                    help="also run one mixed-precision solve to this relative residual (the inner CG product "
+                         "kernel, launched without graphs so that every launch is visible to a profiler)")
     args = ap.parse_args()
     from . import problems
     lib = L.load()
@@ -111,6 +114,24 @@ def main():
            "spmv_fp64": row(t64, B64, names[L.FP64]), "spmv_fp32": row(t32, B32, names[L.FP32]),
            "spmv_fp64_flushed": row(c64, B64, names[L.FP64]), "spmv_fp32_flushed": row(c32, B32, names[L.FP32]),
            "copy_64MiB": {"us": tc * 1e3, "bytes": Bc, "gbs": Bc / tc / 1e6}}
+    if args.cg_tol is not None:
+        bvec = Lf.assemble()
+        bd, xd = _dev(lib, 8 * n), _dev(lib, 8 * n)
+        L.check(lib.femd_memcpy_htod(bd, bvec.ctypes.data, 8 * n))
+        L.check(lib.femd_memcpy_htod(xd, np.zeros(n).ctypes.data, 8 * n))
+        L.check(lib.femd_inner_timer(1))
+        info = L.SolveInfo()
+        L.check(lib.femd_solve_refined_dev(A._h, bd, xd, args.cg_tol, 0.3, 60, C.byref(info), None))
+        L.check(lib.femd_stream_sync(None))
+        k, tot = C.c_int64(), C.c_double()
+        L.check(lib.femd_inner_timer_read(C.byref(k), C.byref(tot)))
+        L.check(lib.femd_inner_timer(0))
+        Bp = 8 * nnz + 20 * n + 4
+        tp = tot.value / max(k.value, 1)
+        res["cg_product"] = {"iterations": int(info.iterations), "timed_launches": k.value, "us": tp * 1e3, "bytes": Bp,
+                             "gbs": Bp / tp / 1e6 if tp > 0 else None}
+        lib.femd_dev_free(bd)
+        lib.femd_dev_free(xd)
     print(json.dumps(res), flush=True)
 
 

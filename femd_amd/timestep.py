@@ -1,12 +1,12 @@
 """Time stepping with a fixed matrix pair on the GPU (the time-stepping pattern).
 
 The reference steps in Python (example13.ipynb: ``b = M @ u`` then a solve with ``A = M + dt K``,
-every step).  Here the whole step -- the FP64 product ``M u``, FP64 refinement rounds around an
-FP32 scaled Jacobi-CG correction warm-started at the previous ``u`` -- is one launch sequence,
-captured once into a hipGraph and replayed for every step:
+every step).  Here the whole step -- the FP64 product ``M u`` and FP64 residual rounds of one scaled
+Jacobi-CG run (FP32 or FP64 iterations, reliable updates between rounds) warm-started at the
+previous ``u`` -- is one launch sequence, captured once into a hipGraph and replayed for every step:
 
     stepper = BackwardEuler(M, A)                  # SparseMatrix pair on one pattern size
-    u, info = stepper.run(u0, nsteps=500, tol=1e-10)
+    u, info = stepper.run(u0, nsteps=500, tol=1e-12)
 """
 from __future__ import annotations
 
@@ -28,18 +28,22 @@ class BackwardEuler:
             raise ValueError(f"BackwardEuler: M {M.shape} and A {A.shape} must be square and of one size")
         self.M, self.A = M, A
 
-    def run(self, u0, nsteps, *, tol=1e-10, rounds=3, inner_tol=1e-4, max_inner=None, graph=True):
-        """Advance u0 by nsteps steps.  Returns (u, info) with info a dict: per-step inner FP32
+    def run(self, u0, nsteps, *, tol=1e-10, rounds=8, inner_tol=1e-2, max_inner=64, precision="fp32", graph=True):
+        """Advance u0 by nsteps steps.  Each step runs up to `rounds` FP64 residual rounds, each a
+        segment of at most max_inner CG iterations in `precision` ("fp32" or "fp64") that ends when
+        its residual has dropped by inner_tol or the step's residual meets tol; later rounds continue
+        the same CG (reliable updates).  Returns (u, info) with info a dict: per-step inner
         iterations, their total, the last step's relative residual and whether it met tol."""
         n = self.A.shape[0]
         u = np.array(u0, dtype=np.float64).reshape(-1)
         if u.size != n:
             raise ValueError(f"BackwardEuler: u0 has {u.size} values, the system {n}")
-        if max_inner is None:
-            max_inner = max(50, min(2000, 4 * int(np.sqrt(n))))
+        prec = {"fp32": L.FP32, "fp64": L.FP64}.get(precision)
+        if prec is None:
+            raise ValueError(f"BackwardEuler: precision is 'fp32' or 'fp64', not {precision!r}")
         its = np.zeros(max(int(nsteps), 1), dtype=np.int32)
         info = L.SolveInfo()
         L.check(L.load().femd_heat_run(self.M._h, self.A._h, u, int(nsteps), float(tol), int(rounds), float(inner_tol),
-                                       int(max_inner), int(bool(graph)), its.ctypes.data, C.byref(info)))
+                                       int(max_inner), prec, int(bool(graph)), its.ctypes.data, C.byref(info)))
         return u, dict(converged=bool(info.converged), iterations=its[:int(nsteps)], total_iterations=int(info.iterations),
                        steps=int(info.breakdown), residual=float(info.residual))

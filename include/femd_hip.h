@@ -300,17 +300,20 @@ void femd_server_free(femd_server S);
  * Stepping with a fixed matrix pair: u_{k+1} solves A u_{k+1} = M u_k (backward Euler for
  * M u' + K u = 0 with A = M + dt K).  No reference counterpart as one call: the reference writes
  * this loop in Python (example13.ipynb: b = M u, then a solve with A, each step); this replaces
- * that loop's SpMV and solve.  Each step: b = M u; `rounds` FP64 refinement rounds r = b - A u,
- * each of which, unless ||r|| <= tol ||b||, runs the FP32 scaled Jacobi-CG for a correction
- * (started from zero, so the step is warm-started at u_k) to relative residual inner_tol within
- * max_inner iterations.  With use_graph the step is captured once into a hipGraph and replayed
- * nsteps times with no host synchronisation between steps.  iters[k] (may be NULL) gets step k's
- * inner iterations; info.iterations their sum, info.breakdown the number of steps done,
- * info.residual the last step's ||b - A u|| / ||b||, info.converged whether that is <= tol. */
+ * that loop's SpMV and solve.  Each step: b = M u; up to `rounds` FP64 rounds r = b - A u, each
+ * of which, unless ||r|| <= tol ||b||, runs a segment of at most max_inner iterations of the
+ * scaled Jacobi-CG in `precision` (FEMD_FP32 or FEMD_FP64) on the correction, until its residual
+ * has dropped by inner_tol (0 < inner_tol < 1) or its estimate of ||b - A u|| meets tol ||b||.
+ * The first round of a step starts from no search direction (u_k is the warm start), later
+ * rounds continue the same CG from the FP64 residual (reliable updates).  With use_graph the step
+ * is captured once into a hipGraph and replayed nsteps times with no host synchronisation between
+ * steps.  iters[k] (may be NULL) gets step k's inner iterations; info.iterations their sum,
+ * info.breakdown the number of steps done, info.residual the last step's ||b - A u|| / ||b||,
+ * info.converged whether that is <= tol. */
 int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
-                  int use_graph, int *iters, femd_solve_info *info);
+                  int precision, int use_graph, int *iters, femd_solve_info *info);
 int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,
-                      int max_inner, int use_graph, int *iters, femd_solve_info *info, femd_stream s);
+                      int max_inner, int precision, int use_graph, int *iters, femd_solve_info *info, femd_stream s);
 
 /* ---------------------------------------------------------------- timing
  * Kernel-time of the library's work on a stream, measured with HIP events around it. */

@@ -76,35 +76,56 @@ __global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, cons
 }
 
 // ---- iteration kernels -----------------------------------------------------------------------
+// The product over the list of tiles of the running systems (k_cg_list_fill): entry g is two
+// int4 -- the tile's first row, first entry, first distinct column and system, then the same
+// bounds of the next tile and the tile's index -- so a tile starts from one load.  The direction
+// buffers alternate with the iteration (odd: read P1, write P0), the same for every system, so
+// the own-row and column loads do not wait for the system's state; a system that has paused since
+// the list was built skips its tiles (PCGOp::ready) once those loads are in flight.
 template <typename T, int THREADS, int CAP, bool DICT>
-__global__ __launch_bounds__(THREADS) void k_cg_product(const int2 *__restrict__ blk, const int *__restrict__ uptr,
+__global__ __launch_bounds__(THREADS) void k_cg_product(const int4 *__restrict__ list, int count,
                                                         const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
                                                         const int *__restrict__ ip, const int *__restrict__ ix,
                                                         const T *__restrict__ v, const unsigned char *__restrict__ vidx,
                                                         const int *__restrict__ doff, const T *__restrict__ dval,
                                                         const T *__restrict__ r, T *P0, T *P1, T *__restrict__ q, int64_t nnz,
-                                                        const int *__restrict__ tile_sys, double *__restrict__ tparts,
-                                                        const CGSys *st)
+                                                        int odd, double *__restrict__ tparts, const CGSys *st)
 {
     __shared__ T xs[CAP];
     __shared__ T prod[CAP];
     __shared__ T dv[DICT ? 256 : 1];
     __shared__ double red[THREADS / 64];
-    const CGSys &S = st[tile_sys ? tile_sys[blockIdx.x] : 0];
-    if (!S.active) return;
-    const bool odd = S.it & 1;
-    const T *pold = odd ? P1 : P0;
-    T *pnew = odd ? P0 : P1;
-    bool stopped = false;
-    const PCGOp<T> op{r, pold, pnew, q, static_cast<T>(S.beta)};
+    const int g = blockIdx.x;
+    if (g >= count) return;
+    const int4 e0 = list[2 * g], e1 = list[2 * g + 1];
+    const CGSys &S = st[e0.w];
+    const PCGOp<T> op{r, odd ? P1 : P0, odd ? P0 : P1, q, static_cast<T>(S.beta), S.active};
+    const TileDesc td{e0.x, e0.y, e0.z, e1.x, e1.y, e1.z};
     double part;
     if constexpr (DICT)
-        part = stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, op, xs, prod, nullptr, &stopped, nullptr,
-                                            nullptr, TileDict<T>{vidx, doff, dval, dv});
+        part = tile_work<T, THREADS, CAP>(td, e1.w, ucol, lidx, ip, ix, v, nnz, op, xs, prod, TileDict<T>{vidx, doff, dval, dv});
     else
-        part = stream_tile<T, THREADS, CAP>(blk, uptr, ucol, lidx, ip, ix, v, nnz, op, xs, prod, nullptr, &stopped);
+        part = tile_work<T, THREADS, CAP>(td, e1.w, ucol, lidx, ip, ix, v, nnz, op, xs, prod);
+    if (!op.ready()) return;
     const double t = wg_sum<THREADS>(part, red);
-    if (threadIdx.x == 0) tparts[blockIdx.x] = t;
+    if (threadIdx.x == 0) tparts[e1.w] = t;
+}
+
+// one thread per tile: the list entry of every tile of a system with aoff[sys] >= 0, at
+// aoff[sys] + its position among the system's tiles
+__global__ __launch_bounds__(BT) void k_cg_list_fill(int ntiles, const int2 *__restrict__ blk, const int *__restrict__ uptr,
+                                                     const int *__restrict__ tile_sys, const int *__restrict__ seg_tiles,
+                                                     const int *__restrict__ aoff, int4 *list)
+{
+    const int b = blockIdx.x * BT + threadIdx.x;
+    if (b >= ntiles) return;
+    const int sys = tile_sys ? tile_sys[b] : 0;
+    const int a = aoff[sys];
+    if (a < 0) return;
+    const int k = a + b - seg_tiles[sys];
+    const int2 b0 = blk[b], b1 = blk[b + 1];
+    list[2 * k] = make_int4(b0.x, b0.y, uptr[b], sys);
+    list[2 * k + 1] = make_int4(b1.x, b1.y, uptr[b + 1], b);
 }
 
 // one workgroup per system: p'.q over its tiles, alpha
@@ -130,13 +151,13 @@ __global__ __launch_bounds__(RT) void k_cg_alpha(const int *__restrict__ seg_til
 template <typename T>
 __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ clo, const int *__restrict__ cpid, T *y, T *r,
                                                   const T *P0, const T *P1, const T *__restrict__ q,
-                                                  const float *__restrict__ dg, double *cparts, const CGSys *st)
+                                                  const float *__restrict__ dg, int odd, double *cparts, const CGSys *st)
 {
     __shared__ double red[2 * (BT / 64)];
     const int c = blockIdx.x;
     const CGSys &S = st[cpid[c]];
     if (!S.active) return;
-    const T *pnew = (S.it & 1) ? P0 : P1;
+    const T *pnew = odd ? P0 : P1;
     const T a = static_cast<T>(S.alpha);
     double s = 0.0, t = 0.0;
     for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
@@ -174,7 +195,8 @@ __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ cl
 }
 
 // one workgroup per system: the new r.r and r.D r over its chunks, the pause tests, beta
-__global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, const double *__restrict__ cparts, CGSys *st)
+__global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, const double *__restrict__ cparts, int odd,
+                                                CGSys *st)
 {
     CGSys &S = st[blockIdx.x];
     if (!S.active) return;
@@ -185,6 +207,7 @@ __global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, co
     if (threadIdx.x != 0) return;
     S.it += 1;
     S.total += 1;
+    S.dir = odd ^ 1;                        // the new direction's buffer
     int reason = 0;
     if (!(rr == rr) || !(rt == rt)) reason = 2;
     else if (rr <= S.thr || rt <= S.tgt) reason = 1;
@@ -257,7 +280,7 @@ template <typename T>
 __global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
                                                    const int *__restrict__ sel, const double *__restrict__ rho_new,
                                                    const double *__restrict__ sc, const double *__restrict__ r64, T *r, T *y,
-                                                   T *P0, T *P1, T *q, const CGSys *st)
+                                                   T *P0, T *P1, T *q, int next, const CGSys *st)
 {
     const int c = blockIdx.x, sys = cpid[c];
     const int mode = sel[sys];
@@ -265,7 +288,8 @@ __global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ c
     const double rn = rho_new[sys];
     const double scale = rn > 0.0 ? 1.0 / rn : 0.0;
     const T f = static_cast<T>(mode == 2 || !(rn > 0.0) ? 0.0 : st[sys].rho / rn);
-    T *pc = (st[sys].it & 1) ? P1 : P0;
+    const T *pc = st[sys].dir ? P1 : P0;    // the current direction ...
+    T *pn = next ? P1 : P0;                  // ... moves to where the next product reads it
     for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
     {
         r[i] = static_cast<T>(sc[i] * r64[i] * scale);
@@ -279,7 +303,7 @@ __global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ c
         }
         else
         {
-            pc[i] *= f;
+            pn[i] = pc[i] * f;
             q[i] *= f;
         }
     }
@@ -289,12 +313,12 @@ __global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ c
 template <typename T>
 __global__ __launch_bounds__(BT) void k_cg_fresh_dots(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
                                                       const int *__restrict__ sel, const T *__restrict__ r, const T *P0,
-                                                      const T *P1, const T *__restrict__ q, const CGSys *st, double *c3)
+                                                      const T *P1, const T *__restrict__ q, int next, double *c3)
 {
     __shared__ double red[3 * (BT / 64)];
     const int c = blockIdx.x, sys = cpid[c];
     if (!sel[sys]) return;
-    const T *pc = (st[sys].it & 1) ? P1 : P0;
+    const T *pc = next ? P1 : P0;
     double a = 0.0, b = 0.0, d = 0.0;
     for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
     {
@@ -330,7 +354,8 @@ __global__ __launch_bounds__(BT) void k_cg_fresh_dots(const int64_t *__restrict_
 // A-conjugate to the current one (0 on a restart), the segment's threshold, running again
 __global__ __launch_bounds__(RT) void k_cg_fresh_state(const int *__restrict__ cptr, const int *__restrict__ sel,
                                                        const double *__restrict__ c3, const double *__restrict__ rho_new,
-                                                       const double *__restrict__ tgt, double tol2, int max_iter, CGSys *st)
+                                                       const double *__restrict__ tgt, double tol2, int max_iter, int next,
+                                                       CGSys *st)
 {
     const int sys = blockIdx.x, mode = sel[sys];
     if (!mode) return;
@@ -348,7 +373,8 @@ __global__ __launch_bounds__(RT) void k_cg_fresh_state(const int *__restrict__ c
         S.beta = 0.0;
     }
     else S.beta = pq > 0.0 ? -rq / pq : 0.0;
-    S.max_iter = S.it + max_iter;           // a cap per segment; it keeps counting (direction parity)
+    S.max_iter = S.it + max_iter;           // a cap per segment
+    S.dir = next;
     S.tol2 = tol2;
     S.tgt = tgt ? tgt[sys] : 0.0;
     S.rr = rr;
@@ -532,31 +558,58 @@ void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
     rho_new = DevBuf(8 * size_t(std::max(np, 1)));
     tgt = DevBuf(8 * size_t(std::max(np, 1)));
     cnt = DevBuf(2 * sizeof(int));
+    list = DevBuf(32 * size_t(std::max(plan->ntiles, 1)));
+    aoff = DevBuf(4 * size_t(std::max(np, 1)));
+    parity = 0;
+    list_count = -1;
+    build_list(nullptr, s);
     std::vector<CGSys> z(size_t(std::max(np, 1)));
     std::memset(z.data(), 0, sizeof(CGSys) * z.size());
     HIPC(hipMemcpyAsync(st.p, z.data(), sizeof(CGSys) * z.size(), hipMemcpyHostToDevice, s));
     HIPC(hipStreamSynchronize(s));
 }
 
+// The product's tile list: the tiles of every system not marked in `leave_out` (NULL: all).
+void CGEngine::build_list(const std::vector<int> *leave_out, hipStream_t s)
+{
+    const std::vector<int> &stl = plan->seg_tiles;
+    std::vector<int> off(size_t(std::max(np, 1)), -1);
+    int run = 0;
+    for (int p = 0; p < np; ++p)
+    {
+        if (leave_out && (*leave_out)[size_t(p)]) continue;
+        off[size_t(p)] = run;
+        run += stl[size_t(p) + 1] - stl[size_t(p)];
+    }
+    HIPC(hipMemcpyAsync(aoff.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, s));
+    if (plan->ntiles > 0)
+        hipLaunchKernelGGL(k_cg_list_fill, dim3((plan->ntiles + BT - 1) / BT), dim3(BT), 0, s, plan->ntiles,
+                           plan->blocks.as<int2>(), plan->uptr.as<int>(), plan->tile_sys.p ? plan->tile_sys.as<int>() : nullptr,
+                           seg_tiles.as<int>(), aoff.as<int>(), list.as<int4>());
+    HIPC(hipStreamSynchronize(s));            // off goes out of scope
+    if (run != list_count) drop_graph();      // a captured graph holds the launch's grid
+    list_count = run;
+}
+
 namespace {
 
 template <typename T, bool DICT>
-void launch_product(CGEngine &E, hipStream_t s)
+void launch_product(CGEngine &E, int odd, hipStream_t s)
 {
     const DevPattern &P = *E.A->pat;
     const SpmvPlan &pl = *E.plan;
     int th = 0, cap = 0;
     plan_shape(pl, &th, &cap);
-    const int2 *blk = pl.blocks.as<int2>();
-    const int *uptr = pl.uptr.as<int>(), *ucol = pl.ucol.as<int>();
+    const int *ucol = pl.ucol.as<int>();
     const unsigned short *lidx = pl.lidx.as<unsigned short>();
-    const int *tsys = pl.tile_sys.p ? pl.tile_sys.as<int>() : nullptr;
     const T *v = sizeof(T) == 8 ? reinterpret_cast<const T *>(E.A->v64s.p) : reinterpret_cast<const T *>(E.A->v32s.p);
     const ValueDict &d = sizeof(T) == 8 ? E.A->d64s : E.A->d32s;
     const unsigned char *vidx = DICT ? d.vidx.as<unsigned char>() : nullptr;
     const int *doff = DICT ? d.doff.as<int>() : nullptr;
     const T *dval = DICT ? d.dval.as<T>() : nullptr;
     const CGSys *st = E.st.as<CGSys>();
+    const int4 *list = E.list.as<int4>();
+    const int count = E.list_count, grid = std::max(count, 1);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!timer_sample(s, &e0, &e1)) e0 = e1 = nullptr;
     bool launched = false;
@@ -566,13 +619,13 @@ void launch_product(CGEngine &E, hipStream_t s)
         {                                                                                                             \
             launched = true;                                                                                          \
             if (e0)                                                                                                   \
-                hipExtLaunchKernelGGL((k_cg_product<T, TH, CP, DICT>), dim3(pl.ntiles), dim3(TH), 0, s, e0, e1, 0, blk, uptr, \
+                hipExtLaunchKernelGGL((k_cg_product<T, TH, CP, DICT>), dim3(grid), dim3(TH), 0, s, e0, e1, 0, list, count, \
                                       ucol, lidx, P.ip.as<int>(), P.ix.as<int>(), v, vidx, doff, dval, E.r.as<T>(),   \
-                                      E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(), st); \
+                                      E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), P.nnz, odd, E.tparts.as<double>(), st); \
             else                                                                                                      \
-                hipLaunchKernelGGL((k_cg_product<T, TH, CP, DICT>), dim3(pl.ntiles), dim3(TH), 0, s, blk, uptr, ucol, lidx, \
+                hipLaunchKernelGGL((k_cg_product<T, TH, CP, DICT>), dim3(grid), dim3(TH), 0, s, list, count, ucol, lidx,  \
                                    P.ip.as<int>(), P.ix.as<int>(), v, vidx, doff, dval, E.r.as<T>(), E.P0.as<T>(),     \
-                                   E.P1.as<T>(), E.q.as<T>(), P.nnz, tsys, E.tparts.as<double>(), st);                 \
+                                   E.P1.as<T>(), E.q.as<T>(), P.nnz, odd, E.tparts.as<double>(), st);                 \
         }
     FEMD_SHAPES(FEMD_CASE)
 #undef FEMD_CASE
@@ -585,13 +638,15 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
     CGSys *st = E.st.as<CGSys>();
     for (int k = 0; k < count; ++k)
     {
-        if (E.use_dict) launch_product<T, true>(E, s);
-        else launch_product<T, false>(E, s);
+        const int odd = E.parity;
+        if (E.use_dict) launch_product<T, true>(E, odd, s);
+        else launch_product<T, false>(E, odd, s);
         hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(RT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), E.y.as<T>(),
-                           E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(),
+                           E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
                            E.cparts.as<double>(), static_cast<const CGSys *>(st));
-        hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), st);
+        hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), odd, st);
+        E.parity ^= 1;
     }
 }
 
@@ -599,14 +654,14 @@ template <typename T>
 void resume_t(CGEngine &E, const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev,
               double tol2, int max_iter, hipStream_t s)
 {
+    const int next = E.parity;
     hipLaunchKernelGGL(k_cg_rescale<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev, rho_dev,
                        E.A->scale.as<double>(), r64_dev, E.r.as<T>(), E.y.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(),
-                       static_cast<const CGSys *>(E.st.as<CGSys>()));
+                       next, static_cast<const CGSys *>(E.st.as<CGSys>()));
     hipLaunchKernelGGL(k_cg_fresh_dots<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev,
-                       E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), static_cast<const CGSys *>(E.st.as<CGSys>()),
-                       E.cparts.as<double>());
+                       E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), next, E.cparts.as<double>());
     hipLaunchKernelGGL(k_cg_fresh_state, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), sel_dev, E.cparts.as<double>(), rho_dev,
-                       tgt_dev, tol2, max_iter, E.st.as<CGSys>());
+                       tgt_dev, tol2, max_iter, next, E.st.as<CGSys>());
 }
 
 template <typename T>
@@ -738,6 +793,7 @@ void CGEngine::begin(const double *b, double *x, double tol, double inner_tol, i
     put_int(retired, R.retired, s);
     put_int(sel, R.sel, s);
     resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), inner_tol * inner_tol, R.max_iter, s);
+    build_list(&R.retired, s);
     if (!counts) counts = std::make_unique<Pinned>(2 * sizeof(int));
     HIPC(hipStreamSynchronize(s));                // the host vectors above go out of scope
 }
@@ -807,6 +863,11 @@ std::vector<int> CGEngine::complete(hipStream_t s)
     resume(r64.as<double>(), rho_new.as<double>(), tgt.as<double>(), sel.as<int>(), R.inner_tol * R.inner_tol, R.max_iter, s);
     R.running = 0;
     for (int p = 0; p < np; ++p) R.running += R.retired[p] ? 0 : 1;
+    // once a quarter of the listed tiles belong to retired systems, list only the others
+    int live = 0;
+    for (int p = 0; p < np; ++p)
+        if (!R.retired[p]) live += plan->seg_tiles[size_t(p) + 1] - plan->seg_tiles[size_t(p)];
+    if (4 * live <= 3 * list_count) build_list(&R.retired, s);
     HIPC(hipStreamSynchronize(s));                // rho, tg go out of scope
     return done;
 }

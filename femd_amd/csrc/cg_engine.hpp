@@ -35,7 +35,8 @@ struct CGSys {
     double tgt;          // or once r.D r <= tgt  ((tol ||b|| / rho)^2; 0: no such test)
     double tol2;         // inner_tol^2
     double rho;          // the FP64 residual norm the system is scaled by
-    int it, total, max_iter, active, reason, pad;   // it: since the last restart; total: all
+    int it, total, max_iter, active, reason;   // it: since the last restart; total: all
+    int dir;             // the direction buffer (P0 / P1) holding the current direction
 };
 
 // S = diag(A)^(-1/2) (FP64) and S A S (FP32 / FP64) on the matrix, cached until its values change;
@@ -61,6 +62,9 @@ struct CGEngine {
     DevBuf seg_tiles;
     DevBuf st, tparts, cparts, r, q, P0, P1, y, r64, t64, sel, retired, rho_new, tgt, cnt;
     bool use_dict = false;              // the product reads the scaled values through their dictionary
+    int parity = 0;                     // the next iteration reads its direction from P1 (1) or P0 (0)
+    DevBuf list, aoff;                  // the product's tile list (two int4 per tile) and its offsets
+    int list_count = -1;
     int dict_gen = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -119,6 +123,7 @@ struct CGEngine {
     std::unique_ptr<Pinned> counts;
     void put_int(DevBuf &d, const std::vector<int> &v, hipStream_t s);
     void bind(femd_csr_s *A, int precision, hipStream_t s);
+    void build_list(const std::vector<int> *leave_out, hipStream_t s);
     std::vector<double> norms2(const double *b, const double *x, hipStream_t s);
     void resume(const double *r64_dev, const double *rho_dev, const double *tgt_dev, const int *sel_dev, double tol2,
                 int max_iter, hipStream_t s);

@@ -69,9 +69,11 @@ struct TileDesc {
 };
 
 // The work of one tile b (the whole workgroup) once its bounds are known.  Op supplies gather(j)
-// (the x value at global column j), own(r) (what the row epilogue needs of row r) and
-// store(r, s, own) (writes row r's result and returns its share of the workgroup's partials, of
-// type Op::Part).  Returns this thread's share.
+// (the x value at global column j), own(r) (what the row epilogue needs of row r), store(r, s, own)
+// (writes row r's result and returns its share of the workgroup's partials, of type Op::Part) and
+// ready() (false: the tile has nothing to do -- asked once the tile's loads are in flight, so a
+// flag read ahead of them hides behind their latency; the same answer in every thread).  Returns
+// this thread's share.
 template <typename T, int THREADS, int CAP, class Op, class VS = NoDict>
 __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const int *__restrict__ ucol,
                                               const unsigned short *__restrict__ lidx, const int *__restrict__ ip,
@@ -95,6 +97,7 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
     {
         // one row longer than the buffer: one lane, from global memory, in order
         typename Op::Part part{};
+        if (!op.ready()) return part;
         if (threadIdx.x == 0)
         {
             T s = 0;
@@ -162,6 +165,8 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
             if constexpr (VS::on) di[g] = 0;
         }
     }
+    // every load above is in flight; the op may still decide that the tile has nothing to do
+    if (!op.ready()) return typename Op::Part{};
     T xg[NU];
 #pragma unroll
     for (int i = 0; i < NU; ++i)
@@ -250,6 +255,7 @@ template <typename T> struct PlainOp {
     using Part = int;
     const T *x;
     T *y;
+    __device__ bool ready() const { return true; }
     __device__ T gather(int j) const { return x[j]; }
     __device__ Own own(int) const { return 0; }
     __device__ Part store(int r, T s, Own) const
@@ -286,6 +292,8 @@ template <typename T> struct PCGOp {
     const T *r, *pold;
     T *pnew, *q;
     T beta;
+    int active;
+    __device__ bool ready() const { return active != 0; }
     __device__ T gather(int j) const { return fma(beta, pold[j], r[j]); }
     __device__ Own own(int i) const
     {

@@ -33,7 +33,8 @@ class BackwardEuler:
         segment of at most max_inner CG iterations in `precision` ("fp32" or "fp64") that ends when
         its residual has dropped by inner_tol or the step's residual meets tol; later rounds continue
         the same CG (reliable updates).  Returns (u, info) with info a dict: per-step inner
-        iterations, their total, the last step's relative residual and whether it met tol."""
+        iterations, their total, the largest relative residual any step ended with and whether
+        every step met tol."""
         n = self.A.shape[0]
         u = np.array(u0, dtype=np.float64).reshape(-1)
         if u.size != n:

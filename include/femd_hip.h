@@ -308,8 +308,9 @@ void femd_server_free(femd_server S);
  * rounds continue the same CG from the FP64 residual (reliable updates).  With use_graph the step
  * is captured once into a hipGraph and replayed nsteps times with no host synchronisation between
  * steps.  iters[k] (may be NULL) gets step k's inner iterations; info.iterations their sum,
- * info.breakdown the number of steps done, info.residual the last step's ||b - A u|| / ||b||,
- * info.converged whether that is <= tol. */
+ * info.breakdown the number of steps done, info.residual the largest ||b - A u|| / ||b|| that any
+ * step ended with (each step checks its own FP64 residual after its rounds), info.converged
+ * whether that is <= tol. */
 int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
                   int precision, int use_graph, int *iters, femd_solve_info *info);
 int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,

@@ -16,7 +16,7 @@ from . import _lib
 
 _lib.load()        # no CPU fallback: the HIP library must be built
 
-from .mesh import Mesh2D, rectangle_mesh, mesh_from_arrays
+from .mesh import Mesh2D, QuadMesh, rectangle_mesh, mesh_from_arrays, rectangle_quad_mesh, quad_mesh_from_arrays
 from .spaces import LagrangeSpace, LagrangeSpace2D
 from .forms import (Const, Constant, Dx, TestFunction, TrialFunction, abs_, as_vector, cos, cosh, div, dot, ds, dS, dx,
                     exp, form, grad, inner, log, quadrature_degree, sech, set_quadrature_degree, sign, sin, sinh, sqrt,

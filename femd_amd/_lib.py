@@ -52,6 +52,9 @@ _SIGS = {
     "femd_vec_copy_dev": (_i, [_i64, _p, _p, _p]),
     "femd_mesh_rectangle": (_i, [_d, _d, _d, _d, _i, _i, _i, _ip, C.POINTER(_p)]),
     "femd_mesh_from_arrays": (_i, [_i, _dp, _i, _ip, _i, _ip, _p, _i, C.POINTER(_p)]),
+    "femd_quad_mesh_rectangle": (_i, [_d, _d, _d, _d, _i, _i, _ip, C.POINTER(_p)]),
+    "femd_quad_mesh_from_arrays": (_i, [_i, _dp, _i, _ip, _i, _ip, _p, _i, C.POINTER(_p)]),
+    "femd_mesh_cell_vertices": (_i, [_p, C.POINTER(_i)]),
     "femd_mesh_sizes": (_i, [_p, _ip]),
     "femd_mesh_arrays": (_i, [_p, _dp, _ip, _ip, _ip, _ip]),
     "femd_mesh_free": (None, [_p]),
@@ -61,6 +64,7 @@ _SIGS = {
     "femd_space_free": (None, [_p]),
     "femd_pattern": (_i, [_p, C.POINTER(_i64), _p, _p]),
     "femd_quadrature": (_i, [_i, C.POINTER(_i), _dp, _dp, _dp]),
+    "femd_quadrature_square": (_i, [_i, C.POINTER(_i), _dp, _dp, _dp]),
     "femd_program_compact": (_i, [C.POINTER(Coefficient), _ip, C.POINTER(_i)]),
     "femd_csr_create_space": (_i, [_p, C.POINTER(_p)]),
     "femd_csr_create": (_i, [_i, _i, _ip, _ip, _dp, C.POINTER(_p)]),

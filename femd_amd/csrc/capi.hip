@@ -102,12 +102,18 @@ Quad &quad_of(femd_space_s *V, int degree, hipStream_t s)
     auto it = V->quads.find(degree);
     if (it != V->quads.end()) return *it->second;
     auto Q = std::make_unique<Quad>();
-    QuadRule R = triangle_rule(degree);
+    const bool quad = V->V.nv == 4;
+    QuadRule R = quad ? square_rule(degree) : triangle_rule(degree);
     Q->nq = R.size();
     Q->xi = R.xi; Q->eta = R.eta; Q->w = R.w;
     const int n = V->V.nloc;
     std::vector<double> tab(static_cast<size_t>(Q->nq) * 3 * n);
-    for (int q = 0; q < Q->nq; ++q) V->ref->tabulate(R.xi[q], R.eta[q], &tab[static_cast<size_t>(q) * 3 * n]);
+    for (int q = 0; q < Q->nq; ++q)
+    {
+        double *o = &tab[static_cast<size_t>(q) * 3 * n];
+        if (quad) V->refq->tabulate(R.xi[q], R.eta[q], o);
+        else V->ref->tabulate(R.xi[q], R.eta[q], o);
+    }
     Q->tab = upload(tab, s);
     Q->qw = upload(R.w, s);
     Q->qxi = upload(R.xi, s);
@@ -224,7 +230,7 @@ void eval_program(femd_space_s *V, Quad &Q, const femd_coefficient &c, double *o
     if (nreg > MAX_REGS) fail(FEMD_EINVAL, "coefficient: the program needs more than " + std::to_string(MAX_REGS) + " live registers");
     DevBuf code = upload(compact, s);
     DevBuf consts = upload(c.consts, static_cast<size_t>(std::max(c.nconst, 0)), s);
-    launch_coefficient(V->V.ncells, Q.nq, V->geom.as<double>(), Q.qxi.as<double>(), Q.qeta.as<double>(), c.ninstr,
+    launch_coefficient(V->V.ncells, Q.nq, V->V.nv, V->geom.as<double>(), Q.qxi.as<double>(), Q.qeta.as<double>(), c.ninstr,
                        code.as<int>(), nreg, consts.as<double>(), out, s);
     HIPC(hipStreamSynchronize(s));           // code and consts are released on return
 }
@@ -785,6 +791,37 @@ int femd_mesh_from_arrays(int np, const double *points, int nt, const int *trian
     });
 }
 
+int femd_quad_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, const int markers[4], femd_mesh *out)
+{
+    return api([&] {
+        const int dm[4] = {1, 2, 3, 4};
+        auto m = std::make_unique<femd_mesh_s>();
+        m->M = quad_mesh_rectangle(x0, y0, x1, y1, nx, ny, markers ? markers : dm);
+        *out = m.release();
+    });
+}
+
+int femd_quad_mesh_from_arrays(int np, const double *points, int nq, const int *quads, int ns, const int *segments,
+                               const int *smarks, int marker, femd_mesh *out)
+{
+    return api([&] {
+        if (np < 0 || nq < 0 || ns < 0) fail(FEMD_EINVAL, "quad_mesh_from_arrays: negative count");
+        auto m = std::make_unique<femd_mesh_s>();
+        m->M = quad_mesh_from_arrays(std::vector<double>(points, points + 2 * size_t(np)), std::vector<int>(quads, quads + 4 * size_t(nq)),
+                                     std::vector<int>(segments, segments + 2 * size_t(ns)),
+                                     smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{}, marker);
+        *out = m.release();
+    });
+}
+
+int femd_mesh_cell_vertices(femd_mesh m, int *nv)
+{
+    return api([&] {
+        if (!m || !nv) fail(FEMD_EINVAL, "null argument");
+        *nv = m->M.nv;
+    });
+}
+
 int femd_mesh_sizes(femd_mesh m, int out[3])
 {
     return api([&] {
@@ -816,10 +853,13 @@ int femd_space_create(femd_mesh m, int degree, int ndirichlet, const int *dirich
         if (!m) fail(FEMD_EINVAL, "null mesh");
         if (degree < 1) fail(FEMD_EINVAL, "LagrangeSpace2D: degree must be >= 1 (P0 is a DG space)");
         if (!equispaced && degree >= 3) fail(FEMD_EINVAL, "LagrangeSpace2D: only equispaced nodes are built for degree >= 3");
-        if (degree > 5) fail(FEMD_EINVAL, "LagrangeSpace2D: the GPU kernels cover degrees 1..5");
+        const bool quad = m->M.nv == 4;
+        if (!quad && degree > 5) fail(FEMD_EINVAL, "LagrangeSpace2D: the GPU kernels cover degrees 1..5");
+        if (quad && degree > 4) fail(FEMD_EINVAL, "LagrangeSpaceQ: the GPU kernels cover degrees 1..4 on quadrilaterals");
         auto V = std::make_unique<femd_space_s>();
         V->V = build_space(m->M, degree, std::vector<int>(dirichlet, dirichlet + std::max(ndirichlet, 0)), all_exterior != 0);
-        V->ref = std::make_unique<RefElement>(degree);
+        if (quad) V->refq = std::make_unique<RefQuad>(degree);
+        else V->ref = std::make_unique<RefElement>(degree);
         *out = V.release();
     });
 }
@@ -866,6 +906,17 @@ int femd_program_compact(const femd_coefficient *c, int *code_out, int *nreg_out
         check_program(*c);
         std::vector<int> code = compact_registers(*c, *nreg_out);
         std::copy(code.begin(), code.end(), code_out);
+    });
+}
+
+int femd_quadrature_square(int degree, int *nq, double *xi, double *eta, double *w)
+{
+    return api([&] {
+        QuadRule R = square_rule(degree);
+        *nq = R.size();
+        if (xi) std::copy(R.xi.begin(), R.xi.end(), xi);
+        if (eta) std::copy(R.eta.begin(), R.eta.end(), eta);
+        if (w) std::copy(R.w.begin(), R.w.end(), w);
     });
 }
 
@@ -1020,7 +1071,9 @@ int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a
         Coeffs C = make_coeffs(Vs, Q, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
-        launch_elem_matrix(n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(), C.T, E.as<double>(), s);
+        if (!launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
+                                Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s))
+            fail(FEMD_EINVAL, "assemble_matrix: quadrature degree " + std::to_string(quad_degree) + " is too high for the element kernel");
         launch_gather(Ks->pat->nnz, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(), false, s);
         Ks->v32_valid = false; Ks->v32s_valid = false; Ks->v64s_valid = false;
         HIPC(hipStreamSynchronize(s));
@@ -1038,7 +1091,9 @@ int femd_assemble_vector_dev(femd_space V, int quad_degree, int nterms, const in
         Coeffs C = make_coeffs(Vs, Q, nterms, a, nullptr, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf F(8 * std::max<size_t>(size_t(nc) * n, 1));
-        launch_elem_vector(n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(), C.T, F.as<double>(), s);
+        if (!launch_elem_vector(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
+                                Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, F.as<double>(), s))
+            fail(FEMD_EINVAL, "assemble_vector: quadrature degree " + std::to_string(quad_degree) + " is too high for the element kernel");
         launch_gather(Vs->V.dim, Vs->vptr.as<int>(), Vs->vlist.as<int>(), F.as<double>(), f_dev, true, s);
         HIPC(hipStreamSynchronize(s));
     });

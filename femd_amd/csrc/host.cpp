@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <memory>
 #include <numeric>
 #include <utility>
 #ifdef _OPENMP
@@ -189,6 +190,104 @@ MeshData mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int 
     for (int i = nx; i > 0; --i) side(ny * W + i, ny * W + i - 1, markers[2]);
     for (int j = ny; j > 0; --j) side(j * W, (j - 1) * W, markers[3]);
     return mesh_from_arrays(std::move(g), std::move(tri), segs, smk, markers[0]);
+}
+
+// mesh/quad_mesh.hpp:50-103.  Clockwise quads are re-oriented by swapping vertices 1 and 3; every
+// quad must be strictly convex; neighbours by sorting (edge key, slot) pairs; the listed
+// segments, then every other boundary edge in quad order with `marker`.
+MeshData quad_mesh_from_arrays(std::vector<double> xy, std::vector<int> quads, const std::vector<int> &segs,
+                               const std::vector<int> &smarks, int marker)
+{
+    const int np = static_cast<int>(xy.size() / 2), nq = static_cast<int>(quads.size() / 4);
+    if (np < 4 || nq < 1) bad("quad_mesh_from_arrays: need at least 4 points and 1 quad");
+    if (!smarks.empty() && smarks.size() * 2 != segs.size()) bad("quad_mesh_from_arrays: one marker per segment");
+    auto P = [&](int v, int d) { return xy[2 * static_cast<std::size_t>(v) + d]; };
+    for (int t = 0; t < nq; ++t)
+    {
+        int *Q = &quads[4 * static_cast<std::size_t>(t)];
+        for (int k = 0; k < 4; ++k)
+            if (Q[k] < 0 || Q[k] >= np) bad("quad_mesh_from_arrays: quad " + std::to_string(t) + " has a vertex out of range");
+        double s = 0.0;
+        for (int i = 0; i < 4; ++i) s += P(Q[i], 0) * P(Q[(i + 1) % 4], 1) - P(Q[(i + 1) % 4], 0) * P(Q[i], 1);
+        if (0.5 * s < 0.0) std::swap(Q[1], Q[3]);
+        for (int i = 0; i < 4; ++i)
+        {
+            const int a = Q[(i + 3) % 4], b = Q[i], c = Q[(i + 1) % 4];
+            const double o = (P(b, 0) - P(a, 0)) * (P(c, 1) - P(a, 1)) - (P(b, 1) - P(a, 1)) * (P(c, 0) - P(a, 0));
+            if (!(o > 0.0)) bad("quad_mesh_from_arrays: quad " + std::to_string(t) + " is degenerate or not convex");
+        }
+    }
+    MeshData M;
+    M.nv = 4;
+    M.xy = std::move(xy);
+    M.tri = std::move(quads);
+    M.nbr.assign(4 * static_cast<std::size_t>(nq), -1);
+    std::vector<std::pair<int64_t, int>> half(4 * static_cast<std::size_t>(nq));
+    for (int t = 0; t < nq; ++t)
+        for (int i = 0; i < 4; ++i)
+        {
+            int a, b;
+            M.edge(t, i, a, b);
+            half[4 * static_cast<std::size_t>(t) + i] = {edge_key(a, b), 4 * t + i};
+        }
+    std::sort(half.begin(), half.end());
+    for (std::size_t h = 0; h < half.size();)
+    {
+        std::size_t e = h;
+        while (e < half.size() && half[e].first == half[h].first) ++e;
+        if (e - h > 2) bad("quad_mesh_from_arrays: an edge is shared by more than two quads");
+        if (e - h == 2)
+        {
+            const int s0 = half[h].second, s1 = half[h + 1].second;
+            M.nbr[s0] = s1 / 4;
+            M.nbr[s1] = s0 / 4;
+        }
+        h = e;
+    }
+    std::vector<std::pair<int64_t, int>> keys;
+    auto add = [&](int a, int b, int m) {
+        M.seg.push_back(a); M.seg.push_back(b); M.smark.push_back(m);
+        keys.push_back({edge_key(a, b), static_cast<int>(M.smark.size()) - 1});
+    };
+    std::vector<int64_t> listed;
+    for (std::size_t s2 = 0; 2 * s2 < segs.size(); ++s2)
+    {
+        const int64_t k = edge_key(segs[2 * s2], segs[2 * s2 + 1]);
+        if (!std::binary_search(half.begin(), half.end(), std::make_pair(k, -1),
+                                [](const std::pair<int64_t, int> &x, const std::pair<int64_t, int> &y) { return x.first < y.first; }))
+            bad("quad_mesh_from_arrays: segment " + std::to_string(s2) + " is not an edge of the mesh");
+        add(segs[2 * s2], segs[2 * s2 + 1], smarks.empty() ? marker : smarks[s2]);
+        listed.push_back(k);
+    }
+    std::sort(listed.begin(), listed.end());
+    for (int t = 0; t < nq; ++t)
+        for (int i = 0; i < 4; ++i)
+            if (M.nbr[4 * static_cast<std::size_t>(t) + i] < 0)
+            {
+                int a, b;
+                M.edge(t, i, a, b);
+                if (!std::binary_search(listed.begin(), listed.end(), edge_key(a, b))) add(a, b, marker);
+            }
+    std::sort(keys.begin(), keys.end());
+    M.seg_key.resize(keys.size()); M.seg_of_key.resize(keys.size());
+    for (std::size_t i = 0; i < keys.size(); ++i) { M.seg_key[i] = keys[i].first; M.seg_of_key[i] = keys[i].second; }
+    return M;
+}
+
+MeshData quad_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, const int markers[4])
+{
+    const MeshData T = mesh_rectangle(x0, y0, x1, y1, nx, ny, 0, markers);
+    const int W = nx + 1;
+    std::vector<double> g(T.xy.begin(), T.xy.begin() + 2 * static_cast<std::size_t>(W) * (ny + 1));
+    std::vector<int> q;
+    q.reserve(4 * static_cast<std::size_t>(nx) * ny);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+        {
+            const int v = j * W + i;
+            q.insert(q.end(), {v, v + 1, v + W + 1, v + W});
+        }
+    return quad_mesh_from_arrays(std::move(g), std::move(q), T.seg, T.smark, markers[0]);
 }
 
 // -------------------------------------------------------------------------- reference element
@@ -395,32 +494,113 @@ QuadRule triangle_rule(int d)
     return Q;
 }
 
+RefQuad::RefQuad(int degree) : k(degree), n((degree + 1) * (degree + 1))
+{
+    std::vector<double> z;
+    for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+    t.resize(static_cast<std::size_t>(k) + 1);
+    for (int j = 0; j <= k; ++j) t[j] = 0.5 * (1.0 + 0.5 * (z[j] - z[k - j]));
+    t[0] = 0.0;
+    t[k] = 1.0;
+    w.assign(static_cast<std::size_t>(k) + 1, 1.0);
+    for (int j = 0; j <= k; ++j)
+        for (int m = 0; m <= k; ++m)
+            if (m != j) w[j] /= (t[j] - t[m]);
+    auto add = [&](int i, int j) { ii.push_back(i); jj.push_back(j); };
+    add(0, 0); add(k, 0); add(k, k); add(0, k);
+    for (int j = 1; j < k; ++j) add(j, 0);           // edge 0, bottom
+    for (int j = 1; j < k; ++j) add(k, j);           // edge 1, right
+    for (int j = 1; j < k; ++j) add(k - j, k);       // edge 2, top
+    for (int j = 1; j < k; ++j) add(0, k - j);       // edge 3, left
+    for (int j = 1; j < k; ++j)
+        for (int i = 1; i < k; ++i) add(i, j);
+    for (int l = 0; l < n; ++l) { nxi.push_back(t[ii[l]]); neta.push_back(t[jj[l]]); }
+}
+
+void RefQuad::tabulate(double xi, double eta, double *out) const
+{
+    std::vector<double> a(4 * (static_cast<std::size_t>(k) + 1));
+    double *L[2] = {a.data(), a.data() + 2 * (k + 1)}, *dL[2] = {a.data() + (k + 1), a.data() + 3 * (k + 1)};
+    const double x[2] = {xi, eta};
+    for (int d = 0; d < 2; ++d)
+        for (int j = 0; j <= k; ++j)
+        {
+            double p = 1.0, dp = 0.0;
+            for (int m = 0; m <= k; ++m)
+            {
+                if (m == j) continue;
+                dp = dp * (x[d] - t[m]) + p;       // product rule, one factor differentiated at a time
+                p *= (x[d] - t[m]);
+            }
+            L[d][j] = w[j] * p;
+            dL[d][j] = w[j] * dp;
+        }
+    for (int l = 0; l < n; ++l)
+    {
+        out[l] = L[0][ii[l]] * L[1][jj[l]];
+        out[n + l] = dL[0][ii[l]] * L[1][jj[l]];
+        out[2 * n + l] = L[0][ii[l]] * dL[1][jj[l]];
+    }
+}
+
+QuadRule square_rule(int d)
+{
+    std::vector<double> g, gw;
+    gauss_legendre(std::max(1, (d + 3) / 2), g, gw);
+    QuadRule Q;
+    const int m = static_cast<int>(g.size());
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i)
+        {
+            Q.xi.push_back(0.5 * (g[i] + 1.0));
+            Q.eta.push_back(0.5 * (g[j] + 1.0));
+            Q.w.push_back(0.25 * gw[i] * gw[j]);
+        }
+    return Q;
+}
+
 // ---------------------------------------------------------------------------------- spaces
 SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior)
 {
     std::sort(dirichlet.begin(), dirichlet.end());
     dirichlet.erase(std::unique(dirichlet.begin(), dirichlet.end()), dirichlet.end());
-    RefElement R(k);
+    const int nv = M.nv;
+    std::unique_ptr<RefElement> RT;
+    std::unique_ptr<RefQuad> RQ;
+    if (nv == 4) RQ = std::make_unique<RefQuad>(k);
+    else RT = std::make_unique<RefElement>(k);
+    const std::vector<double> &nxi = nv == 4 ? RQ->nxi : RT->nxi, &neta = nv == 4 ? RQ->neta : RT->neta;
     SpaceData V;
-    const int nc = M.nt(), np = M.np(), n = R.n, de = k - 1, nint = (k - 1) * (k - 2) / 2, f0 = 3 + 3 * de;
-    V.k = k; V.nloc = n; V.ncells = nc; V.npoints = np;
-    V.geom.resize(6 * static_cast<std::size_t>(nc));
+    const int nc = M.nt(), np = M.np(), n = nv == 4 ? RQ->n : RT->n, de = k - 1;
+    const int nint = nv == 4 ? (k - 1) * (k - 1) : (k - 1) * (k - 2) / 2, f0 = nv + nv * de;
+    V.nv = nv; V.k = k; V.nloc = n; V.ncells = nc; V.npoints = np;
+    const int gs = V.geom_stride();
+    V.geom.resize(gs * static_cast<std::size_t>(nc));
     for (int c = 0; c < nc; ++c)
     {
-        const int *t = &M.tri[3 * static_cast<std::size_t>(c)];
-        double *g = &V.geom[6 * static_cast<std::size_t>(c)];
+        const int *t = &M.tri[nv * static_cast<std::size_t>(c)];
+        double *g = &V.geom[gs * static_cast<std::size_t>(c)];
+        if (nv == 4)
+        {
+            for (int v = 0; v < 4; ++v) { g[2 * v] = M.xy[2 * t[v]]; g[2 * v + 1] = M.xy[2 * t[v] + 1]; }
+            continue;
+        }
         g[0] = M.xy[2 * t[0]]; g[1] = M.xy[2 * t[0] + 1];
         g[2] = M.xy[2 * t[1]] - M.xy[2 * t[0]]; g[3] = M.xy[2 * t[2]] - M.xy[2 * t[0]];
         g[4] = M.xy[2 * t[1] + 1] - M.xy[2 * t[0] + 1]; g[5] = M.xy[2 * t[2] + 1] - M.xy[2 * t[0] + 1];
         const double d = g[2] * g[5] - g[3] * g[4];
         if (!(d > 0.0)) bad("LagrangeSpace: cell " + std::to_string(c) + " is degenerate or clockwise");
     }
-    // edges in first-met order: slot s = 3c + i; an edge's id is the rank of its smallest slot
-    const std::size_t ns = 3 * static_cast<std::size_t>(nc);
+    // edges in first-met order: slot s = nv c + i; an edge's id is the rank of its smallest slot
+    const std::size_t ns = nv * static_cast<std::size_t>(nc);
     std::vector<std::pair<int64_t, int>> ks(ns);
     for (int c = 0; c < nc; ++c)
-        for (int i = 0; i < 3; ++i)
-            ks[3 * c + i] = {edge_key(M.tri[3 * c + (i + 1) % 3], M.tri[3 * c + (i + 2) % 3]), 3 * c + i};
+        for (int i = 0; i < nv; ++i)
+        {
+            int a, b;
+            M.edge(c, i, a, b);
+            ks[nv * static_cast<std::size_t>(c) + i] = {edge_key(a, b), nv * c + i};
+        }
     std::sort(ks.begin(), ks.end());
     std::vector<int> group_of_slot(ns), first_slot;
     for (std::size_t h = 0; h < ns;)
@@ -442,8 +622,8 @@ SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool
     for (std::size_t s = 0; s < ns; ++s) cedge[s] = edge_id[group_of_slot[s]];
     for (int g = 0; g < ne; ++g)
     {
-        const int s = first_slot[g], c = s / 3, i = s % 3;
-        const int a = M.tri[3 * c + (i + 1) % 3], b = M.tri[3 * c + (i + 2) % 3];
+        int a, b;
+        M.edge(first_slot[g] / nv, first_slot[g] % nv, a, b);
         edges[edge_id[g]] = {std::min(a, b), std::max(a, b)};
     }
     V.nedges = ne;
@@ -452,11 +632,13 @@ SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool
     for (int c = 0; c < nc; ++c)
     {
         int *d = &V.cell_raw[static_cast<std::size_t>(c) * n];
-        for (int v = 0; v < 3; ++v) d[v] = M.tri[3 * c + v];
-        for (int i = 0; i < 3; ++i)
+        for (int v = 0; v < nv; ++v) d[v] = M.tri[nv * static_cast<std::size_t>(c) + v];
+        for (int i = 0; i < nv; ++i)
         {
-            const bool flip = M.tri[3 * c + (i + 1) % 3] > M.tri[3 * c + (i + 2) % 3];
-            for (int j = 0; j < de; ++j) d[3 + i * de + j] = np + cedge[3 * c + i] * de + (flip ? de - 1 - j : j);
+            int a, b;
+            M.edge(c, i, a, b);
+            const bool flip = a > b;
+            for (int j = 0; j < de; ++j) d[nv + i * de + j] = np + cedge[nv * static_cast<std::size_t>(c) + i] * de + (flip ? de - 1 - j : j);
         }
         for (int j = 0; j < nint; ++j) d[f0 + j] = np + ne * de + c * nint + j;
     }
@@ -465,24 +647,37 @@ SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool
     for (int v = 0; v < np; ++v) { V.raw_xy[2 * v] = M.xy[2 * v]; V.raw_xy[2 * v + 1] = M.xy[2 * v + 1]; placed[v] = 1; }
     for (int c = 0; c < nc; ++c)
     {
-        const double *g = &V.geom[6 * static_cast<std::size_t>(c)];
-        for (int l = 3; l < n; ++l)
+        const double *g = &V.geom[gs * static_cast<std::size_t>(c)];
+        for (int l = nv; l < n; ++l)
         {
             const int r = V.cell_raw[static_cast<std::size_t>(c) * n + l];
             if (placed[r]) continue;
-            V.raw_xy[2 * r] = g[0] + g[2] * R.nxi[l] + g[3] * R.neta[l];
-            V.raw_xy[2 * r + 1] = g[1] + g[4] * R.nxi[l] + g[5] * R.neta[l];
+            const double xi = nxi[l], eta = neta[l];
+            if (nv == 4)
+            {
+                // the bilinear map (lagrange_space_quad.hpp:155-162)
+                const double a0 = (1 - xi) * (1 - eta), a1 = xi * (1 - eta), a2 = xi * eta, a3 = (1 - xi) * eta;
+                V.raw_xy[2 * r] = a0 * g[0] + a1 * g[2] + a2 * g[4] + a3 * g[6];
+                V.raw_xy[2 * r + 1] = a0 * g[1] + a1 * g[3] + a2 * g[5] + a3 * g[7];
+            }
+            else
+            {
+                V.raw_xy[2 * r] = g[0] + g[2] * xi + g[3] * eta;
+                V.raw_xy[2 * r + 1] = g[1] + g[4] * xi + g[5] * eta;
+            }
             placed[r] = 1;
         }
     }
     // Dirichlet elimination over the facets (exterior edges and segment edges, first met)
     std::vector<char> fixed(static_cast<std::size_t>(V.raw_dim), 0), seen(static_cast<std::size_t>(ne), 0);
     for (int c = 0; c < nc; ++c)
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < nv; ++i)
         {
-            const int g = cedge[3 * c + i];
-            const bool ext = M.nbr[3 * c + i] < 0;
-            const int m = M.marker_of(M.tri[3 * c + (i + 1) % 3], M.tri[3 * c + (i + 2) % 3]);
+            const int g = cedge[nv * static_cast<std::size_t>(c) + i];
+            const bool ext = M.nbr[nv * static_cast<std::size_t>(c) + i] < 0;
+            int a, b;
+            M.edge(c, i, a, b);
+            const int m = M.marker_of(a, b);
             if (!ext && (m < 0 || seen[g])) continue;
             seen[g] = 1;
             const int mark = m >= 0 ? m : 0;

@@ -19,21 +19,36 @@ struct Error : std::runtime_error {
 
 // A triangulation in the state mesh_from_arrays leaves it (mesh/structured.hpp:99-221):
 // counter-clockwise triangles, neighbour across the edge opposite each vertex, segments with
-// markers (user segments first, then the remaining boundary edges in loop order).
+// markers (user segments first, then the remaining boundary edges in loop order).  Or a
+// quadrilateral mesh as QuadMesh holds it (mesh/quad_mesh.hpp:4-103): counter-clockwise convex
+// quads, neighbour across edge i (from vertex i to vertex i+1), the listed segments first, then
+// the remaining boundary edges in quad order.
 struct MeshData {
+    int nv = 3;                      // vertices per cell
     std::vector<double> xy;          // np x 2
-    std::vector<int> tri, nbr;       // nt x 3
+    std::vector<int> tri, nbr;       // nt x nv
     std::vector<int> seg, smark;     // ns x 2, ns
     std::vector<int64_t> seg_key;    // sorted edge keys of the segments
     std::vector<int> seg_of_key;     // segment index for seg_key[i]
     int np() const { return static_cast<int>(xy.size() / 2); }
-    int nt() const { return static_cast<int>(tri.size() / 3); }
+    int nt() const { return static_cast<int>(tri.size() / nv); }
     int marker_of(int a, int b) const;        // the segment's marker, -1 when (a, b) is no segment
+    // the vertices of local edge i of cell c
+    void edge(int c, int i, int &a, int &b) const
+    {
+        const int *t = &tri[static_cast<std::size_t>(c) * nv];
+        if (nv == 3) { a = t[(i + 1) % 3]; b = t[(i + 2) % 3]; }
+        else { a = t[i]; b = t[(i + 1) % 4]; }
+    }
 };
 
 MeshData mesh_from_arrays(std::vector<double> xy, std::vector<int> tri, const std::vector<int> &segs,
                           const std::vector<int> &smarks, int marker);
 MeshData mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int diag, const int markers[4]);
+MeshData quad_mesh_from_arrays(std::vector<double> xy, std::vector<int> quads, const std::vector<int> &segs,
+                               const std::vector<int> &smarks, int marker);
+// python/femd/quadmesh.py:146-170: rectangle_mesh's grid and sides, quad (i, j) = j nx + i
+MeshData quad_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, const int markers[4]);
 
 // The nodal P_k basis on the reference triangle (fe/simplex_basis.hpp: Dubiner modes times the
 // inverse Vandermonde matrix, equispaced nodes, local order vertices / edges / interior).
@@ -44,18 +59,34 @@ struct RefElement {
     void tabulate(double xi, double eta, double *out3n) const;     // values, d/dxi, d/deta
 };
 
+// The nodal Q_k basis on the reference square [0,1]^2 (fe/lagrange_space_quad.hpp:28-110): 1D
+// Lagrange polynomials in barycentric form on k+1 symmetrized equispaced points, tensor products;
+// local order vertices, the nodes of each edge along it, interior row by row.
+struct RefQuad {
+    int k = 1, n = 4;
+    std::vector<double> t, w, nxi, neta;
+    std::vector<int> ii, jj;
+    explicit RefQuad(int degree);
+    void tabulate(double xi, double eta, double *out3n) const;     // values, d/dxi, d/deta
+};
+
 struct QuadRule {
     std::vector<double> xi, eta, w;
     int size() const { return static_cast<int>(w.size()); }
 };
 QuadRule triangle_rule(int degree);
+QuadRule square_rule(int degree);            // fe/cache_2d.hpp:75-84, tensor Gauss-Legendre
 
-// A continuous P_k space (fe/space_2d.hpp:390-526, fe/lagrange_space_2d.hpp:32-67).
+// A continuous P_k (triangles) or Q_k (quadrilaterals) space (fe/space_2d.hpp:390-526,
+// fe/lagrange_space_2d.hpp:32-67, fe/lagrange_space_quad.hpp:112-210).
 struct SpaceData {
-    int k = 1, nloc = 3, ncells = 0, npoints = 0, nedges = 0, raw_dim = 0, dim = 0;
+    int nv = 3, k = 1, nloc = 3, ncells = 0, npoints = 0, nedges = 0, raw_dim = 0, dim = 0;
     std::vector<int> cell_raw, cell_dof, ad2raw;
     std::vector<double> raw_xy;
-    std::vector<double> geom;        // per cell: x0, y0, J00, J01, J10, J11 (6 doubles)
+    // per cell: triangles x0, y0, J00, J01, J10, J11 (6 doubles); quadrilaterals the four
+    // vertices x0, y0, .., x3, y3 (8 doubles; J varies in the cell)
+    std::vector<double> geom;
+    int geom_stride() const { return nv == 4 ? 8 : 6; }
 };
 SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior);
 

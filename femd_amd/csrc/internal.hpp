@@ -192,7 +192,8 @@ struct femd_space_s {
     std::shared_ptr<femd_amd::DevPattern> pat;
     femd_amd::DevBuf mptr, mlist, vptr, vlist;
     std::map<int, std::unique_ptr<femd_amd::Quad>> quads;
-    std::unique_ptr<femd_amd::RefElement> ref;
+    std::unique_ptr<femd_amd::RefElement> ref;     // triangles
+    std::unique_ptr<femd_amd::RefQuad> refq;       // quadrilaterals
 };
 
 struct femd_csr_s {

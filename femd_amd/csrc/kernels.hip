@@ -111,11 +111,40 @@ __device__ inline double phys(int m, const double *R, int l, double j0, double j
     return j1 * R[N + l] + j3 * R[2 * N + l];
 }
 
+// The Jacobian J = [dx/dxi, dx/deta, dy/dxi, dy/deta] of cell geometry g at (xi, eta): constant
+// on a triangle (g = x0, y0, J), the bilinear map's on a quadrilateral (g = its four vertices,
+// fe/lagrange_space_quad.hpp:163-169); then det, |det| and J^{-1} as the cache forms them
+// (fe/cache_2d.hpp:106-115).
+struct CellJ {
+    double i0, i1, i2, i3, ad;
+};
+template <int NV>
+__device__ inline CellJ cell_jacobian(const double *g, double xi, double eta)
+{
+    double J0, J1, J2, J3;
+    if constexpr (NV == 4)
+    {
+        J0 = (g[2] - g[0]) * (1 - eta) + (g[4] - g[6]) * eta;
+        J1 = (g[6] - g[0]) * (1 - xi) + (g[4] - g[2]) * xi;
+        J2 = (g[3] - g[1]) * (1 - eta) + (g[5] - g[7]) * eta;
+        J3 = (g[7] - g[1]) * (1 - xi) + (g[5] - g[3]) * xi;
+    }
+    else
+    {
+        J0 = g[2]; J1 = g[3]; J2 = g[4]; J3 = g[5];
+    }
+    const double d = J0 * J3 - J1 * J2;
+    return CellJ{J3 / d, -J1 / d, -J2 / d, J0 / d, fabs(d)};
+}
+
 // One thread per (cell e, row i): the row Ke[i][:] of the element matrix, the terms in order, then
-// the points, each product skipped when it is exactly zero (forms/assembly_2d.hpp:102-126).
-template <int N>
+// the points, each product skipped when it is exactly zero (forms/assembly_2d.hpp:102-126).  NV
+// is the cell's vertex count (3: P_k on triangles, J constant; 4: Q_k on quadrilaterals, J at
+// every point).
+template <int N, int NV>
 __global__ __launch_bounds__(TPB) void k_elem_matrix(int nc, int nq, const double *__restrict__ geom,
                                                      const double *__restrict__ tab, const double *__restrict__ qw,
+                                                     const double *__restrict__ qxi, const double *__restrict__ qeta,
                                                      TermSet T, double *__restrict__ E)
 {
     extern __shared__ double sh[];
@@ -126,10 +155,8 @@ __global__ __launch_bounds__(TPB) void k_elem_matrix(int nc, int nq, const doubl
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= static_cast<int64_t>(nc) * N) return;
     const int e = static_cast<int>(idx / N), i = static_cast<int>(idx % N);
-    const double *g = geom + 6 * static_cast<int64_t>(e);
-    const double J0 = g[2], J1 = g[3], J2 = g[4], J3 = g[5];
-    const double d = J0 * J3 - J1 * J2;
-    const double i0 = J3 / d, i1 = -J1 / d, i2 = -J2 / d, i3 = J0 / d, ad = fabs(d);
+    const double *g = geom + (NV == 4 ? 8 : 6) * static_cast<int64_t>(e);
+    CellJ cj = cell_jacobian<NV>(g, 0.0, 0.0);
     double K[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) K[j] = 0.0;
@@ -140,7 +167,9 @@ __global__ __launch_bounds__(TPB) void k_elem_matrix(int nc, int nq, const doubl
         const double cv = T.cval[t];
         for (int q = 0; q < nq; ++q)
         {
-            const double w = sqw[q] * ad;
+            if constexpr (NV == 4) cj = cell_jacobian<NV>(g, qxi[q], qeta[q]);
+            const double i0 = cj.i0, i1 = cj.i1, i2 = cj.i2, i3 = cj.i3;
+            const double w = sqw[q] * cj.ad;
             const double s = w * (c ? c[static_cast<int64_t>(e) * nq + q] : cv);
             if (s == 0.0) continue;
             const double *R = stab + q * 3 * N;
@@ -156,9 +185,10 @@ __global__ __launch_bounds__(TPB) void k_elem_matrix(int nc, int nq, const doubl
 }
 
 // One thread per (cell e, local l): fe[l] of assemble_vector_2d (forms/assembly_2d.hpp:184-196).
-template <int N>
+template <int N, int NV>
 __global__ __launch_bounds__(TPB) void k_elem_vector(int nc, int nq, const double *__restrict__ geom,
                                                      const double *__restrict__ tab, const double *__restrict__ qw,
+                                                     const double *__restrict__ qxi, const double *__restrict__ qeta,
                                                      TermSet T, double *__restrict__ F)
 {
     extern __shared__ double sh[];
@@ -169,10 +199,8 @@ __global__ __launch_bounds__(TPB) void k_elem_vector(int nc, int nq, const doubl
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= static_cast<int64_t>(nc) * N) return;
     const int e = static_cast<int>(idx / N), l = static_cast<int>(idx % N);
-    const double *g = geom + 6 * static_cast<int64_t>(e);
-    const double J0 = g[2], J1 = g[3], J2 = g[4], J3 = g[5];
-    const double d = J0 * J3 - J1 * J2;
-    const double i0 = J3 / d, i1 = -J1 / d, i2 = -J2 / d, i3 = J0 / d, ad = fabs(d);
+    const double *g = geom + (NV == 4 ? 8 : 6) * static_cast<int64_t>(e);
+    CellJ cj = cell_jacobian<NV>(g, 0.0, 0.0);
     double f = 0.0;
     for (int t = 0; t < T.nt; ++t)
     {
@@ -180,7 +208,9 @@ __global__ __launch_bounds__(TPB) void k_elem_vector(int nc, int nq, const doubl
         const double cv = T.cval[t];
         for (int q = 0; q < nq; ++q)
         {
-            const double w = sqw[q] * ad;
+            if constexpr (NV == 4) cj = cell_jacobian<NV>(g, qxi[q], qeta[q]);
+            const double i0 = cj.i0, i1 = cj.i1, i2 = cj.i2, i3 = cj.i3;
+            const double w = sqw[q] * cj.ad;
             const double s = w * (c ? c[static_cast<int64_t>(e) * nq + q] : cv);
             if (s == 0.0) continue;
             f += s * phys<N>(T.a[t], stab + q * 3 * N, l, i0, i1, i2, i3);
@@ -201,8 +231,10 @@ __global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int *__restrict
 }
 
 // The Integrand register program (forms/integrand.hpp:300-342) at the quadrature points, the
-// point mapped as the cache does: x = x0 + J00 xi + J01 eta (fe/lagrange_space_2d.hpp:78-85).
-__global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, const double *__restrict__ geom,
+// point mapped as the cache does: x = x0 + J00 xi + J01 eta on a triangle
+// (fe/lagrange_space_2d.hpp:78-85), the bilinear map on a quadrilateral
+// (fe/lagrange_space_quad.hpp:155-162).
+__global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, const double *__restrict__ geom,
                                                      const double *__restrict__ qxi, const double *__restrict__ qeta,
                                                      int ninstr, const int *__restrict__ code, int nreg,
                                                      const double *__restrict__ consts, double *__restrict__ out)
@@ -210,9 +242,21 @@ __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, const doubl
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= static_cast<int64_t>(nc) * nq) return;
     const int e = static_cast<int>(k / nq), q = static_cast<int>(k % nq);
-    const double *g = geom + 6 * static_cast<int64_t>(e);
-    const double x = g[0] + g[2] * qxi[q] + g[3] * qeta[q];
-    const double y = g[1] + g[4] * qxi[q] + g[5] * qeta[q];
+    double x, y;
+    if (nv == 4)
+    {
+        const double *g = geom + 8 * static_cast<int64_t>(e);
+        const double xi = qxi[q], eta = qeta[q];
+        const double a0 = (1 - xi) * (1 - eta), a1 = xi * (1 - eta), a2 = xi * eta, a3 = (1 - xi) * eta;
+        x = a0 * g[0] + a1 * g[2] + a2 * g[4] + a3 * g[6];
+        y = a0 * g[1] + a1 * g[3] + a2 * g[5] + a3 * g[7];
+    }
+    else
+    {
+        const double *g = geom + 6 * static_cast<int64_t>(e);
+        x = g[0] + g[2] * qxi[q] + g[3] * qeta[q];
+        y = g[1] + g[4] * qxi[q] + g[5] * qeta[q];
+    }
     double reg[MAX_REGS];
     int res = 0;
     for (int s = 0; s < ninstr; ++s)
@@ -676,41 +720,68 @@ __global__ void k_lincomb(int64_t n, int k, const double *y, VecPtrs V, double *
 #define FEMD_LAUNCH(kernel, grid, block, shmem, stream, ...) \
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, __VA_ARGS__)
 
-#define ELEM_DISPATCH(KERNEL, N_)                                                                   \
+#define ELEM_DISPATCH(KERNEL, N_, NV_)                                                              \
     case N_:                                                                                         \
-        FEMD_LAUNCH(KERNEL<N_>, grid_for(static_cast<int64_t>(nc) * N_), TPB, sh, s, nc, nq, geom, tab, qw, T, out); \
+        FEMD_LAUNCH((KERNEL<N_, NV_>), grid_for(static_cast<int64_t>(nc) * N_), TPB, sh, s, nc, nq, geom, tab, qw, qxi, \
+                    qeta, T, out);                                                                   \
         break;
 
-void launch_elem_matrix(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
-                        const TermSet &T, double *out, hipStream_t s)
+// P1..P5 on triangles (nloc 3, 6, 10, 15, 21) and Q1..Q4 on quadrilaterals (4, 9, 16, 25): the
+// local sizes do not collide, and nv says which cell it is.
+bool launch_elem_matrix(int nv, int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s)
 {
-    if (nc == 0) return;
+    if (nc == 0) return true;
     const size_t sh = sizeof(double) * (static_cast<size_t>(nq) * 3 * nloc + nq);
-    switch (nloc)
-    {
-        ELEM_DISPATCH(k_elem_matrix, 3)
-        ELEM_DISPATCH(k_elem_matrix, 6)
-        ELEM_DISPATCH(k_elem_matrix, 10)
-        ELEM_DISPATCH(k_elem_matrix, 15)
-        ELEM_DISPATCH(k_elem_matrix, 21)
-    default: break;
-    }
+    if (sh > 65536) return false;
+    if (nv == 4)
+        switch (nloc)
+        {
+            ELEM_DISPATCH(k_elem_matrix, 4, 4)
+            ELEM_DISPATCH(k_elem_matrix, 9, 4)
+            ELEM_DISPATCH(k_elem_matrix, 16, 4)
+            ELEM_DISPATCH(k_elem_matrix, 25, 4)
+        default: return false;
+        }
+    else
+        switch (nloc)
+        {
+            ELEM_DISPATCH(k_elem_matrix, 3, 3)
+            ELEM_DISPATCH(k_elem_matrix, 6, 3)
+            ELEM_DISPATCH(k_elem_matrix, 10, 3)
+            ELEM_DISPATCH(k_elem_matrix, 15, 3)
+            ELEM_DISPATCH(k_elem_matrix, 21, 3)
+        default: return false;
+        }
+    return true;
 }
 
-void launch_elem_vector(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
-                        const TermSet &T, double *out, hipStream_t s)
+bool launch_elem_vector(int nv, int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s)
 {
-    if (nc == 0) return;
+    if (nc == 0) return true;
     const size_t sh = sizeof(double) * (static_cast<size_t>(nq) * 3 * nloc + nq);
-    switch (nloc)
-    {
-        ELEM_DISPATCH(k_elem_vector, 3)
-        ELEM_DISPATCH(k_elem_vector, 6)
-        ELEM_DISPATCH(k_elem_vector, 10)
-        ELEM_DISPATCH(k_elem_vector, 15)
-        ELEM_DISPATCH(k_elem_vector, 21)
-    default: break;
-    }
+    if (sh > 65536) return false;
+    if (nv == 4)
+        switch (nloc)
+        {
+            ELEM_DISPATCH(k_elem_vector, 4, 4)
+            ELEM_DISPATCH(k_elem_vector, 9, 4)
+            ELEM_DISPATCH(k_elem_vector, 16, 4)
+            ELEM_DISPATCH(k_elem_vector, 25, 4)
+        default: return false;
+        }
+    else
+        switch (nloc)
+        {
+            ELEM_DISPATCH(k_elem_vector, 3, 3)
+            ELEM_DISPATCH(k_elem_vector, 6, 3)
+            ELEM_DISPATCH(k_elem_vector, 10, 3)
+            ELEM_DISPATCH(k_elem_vector, 15, 3)
+            ELEM_DISPATCH(k_elem_vector, 21, 3)
+        default: return false;
+        }
+    return true;
 }
 
 void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s)
@@ -718,11 +789,11 @@ void launch_gather(int64_t n, const int *ptr, const int *list, const double *src
     if (n > 0) FEMD_LAUNCH(k_gather, grid_for(n), TPB, 0, s, n, ptr, list, src, dst, plus_zero ? 1 : 0);
 }
 
-void launch_coefficient(int nc, int nq, const double *geom, const double *qxi, const double *qeta, int ninstr,
+void launch_coefficient(int nc, int nq, int nv, const double *geom, const double *qxi, const double *qeta, int ninstr,
                         const int *code, int nreg, const double *consts, double *out, hipStream_t s)
 {
     const int64_t m = static_cast<int64_t>(nc) * nq;
-    if (m > 0) FEMD_LAUNCH(k_coefficient, grid_for(m), TPB, 0, s, nc, nq, geom, qxi, qeta, ninstr, code, nreg, consts, out);
+    if (m > 0) FEMD_LAUNCH(k_coefficient, grid_for(m), TPB, 0, s, nc, nq, nv, geom, qxi, qeta, ninstr, code, nreg, consts, out);
 }
 
 void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,

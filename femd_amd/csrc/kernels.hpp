@@ -20,12 +20,13 @@ struct TermSet {
 };
 
 // ---- assembly -------------------------------------------------------------------------------
-void launch_elem_matrix(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
-                        const TermSet &T, double *E, hipStream_t s);
-void launch_elem_vector(int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
-                        const TermSet &T, double *F, hipStream_t s);
+// false: no kernel for this local size, or the reference tables do not fit LDS
+bool launch_elem_matrix(int nv, int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s);
+bool launch_elem_vector(int nv, int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
+                        const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s);
 void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s);
-void launch_coefficient(int nc, int nq, const double *geom, const double *qxi, const double *qeta, int ninstr,
+void launch_coefficient(int nc, int nq, int nv, const double *geom, const double *qxi, const double *qeta, int ninstr,
                         const int *code, int nreg, const double *consts, double *out, hipStream_t s);
 
 // ---- SpMV -----------------------------------------------------------------------------------

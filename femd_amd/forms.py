@@ -383,6 +383,10 @@ def poly_degree(e):
 
 
 def _arg_degree(a):
+    """Total degree on triangles; on quadrilaterals the per-variable degree, which a derivative
+    does not lower (forms.py:654-662)."""
+    if getattr(a.space, "nverts", 3) == 4:
+        return a.space.degree
     return max(a.space.degree - sum(a.deriv), 0)
 
 

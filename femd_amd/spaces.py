@@ -25,7 +25,8 @@ def _markers(m, what):
 
 
 class LagrangeSpace2D:
-    """P_k (k >= 1) on a triangular Mesh2D.
+    """P_k (k >= 1) on a triangular Mesh2D, or Q_k (k = 1..4) on a QuadMesh
+    (fe/lagrange_space_quad.hpp:112-210: tensor Lagrange on the reference square, bilinear map).
 
     bc:        "free" (default) or "dirichlet" (every exterior side eliminated).
     dirichlet: a marker or list of markers whose sides are eliminated; combines with bc.
@@ -62,6 +63,8 @@ class LagrangeSpace2D:
         L.check(L.load().femd_space_create(mesh._h, k, len(marks), d, int(b == "dirichlet"), 1, C.byref(h)))
         self._h = h.value
         self.mesh = mesh
+        self.nverts = mesh.nverts
+        self.cell_name = "quadrilateral" if self.nverts == 4 else "triangle"
         self.degree = k
         self.dirichlet_markers = marks
         self.all_exterior = b == "dirichlet"
@@ -152,11 +155,12 @@ class LagrangeSpace2D:
         return "; ".join(parts) or "free (natural on every side)"
 
     def __repr__(self):
-        return f"LagrangeSpace2D(degree={self.degree}, ncells={self.ncells}, bc={self.bc_description}, dim={self.dim})"
+        name = "LagrangeSpaceQ" if self.nverts == 4 else "LagrangeSpace2D"
+        return f"{name}(degree={self.degree}, ncells={self.ncells}, bc={self.bc_description}, dim={self.dim})"
 
 
 def LagrangeSpace(mesh, degree, bc="free", dirichlet=None, nodes="equispaced"):
-    """fd.LagrangeSpace on a Mesh2D (the 2D branch of spaces.py:435-442)."""
+    """fd.LagrangeSpace on a Mesh2D (P_k) or a QuadMesh (Q_k) (the 2D branches of spaces.py:435-442)."""
     if not isinstance(mesh, Mesh2D):
         raise TypeError("LagrangeSpace: this package builds the 2D (Mesh2D) spaces only")
     return LagrangeSpace2D(mesh, degree, bc=bc, dirichlet=dirichlet, nodes=nodes)

@@ -81,17 +81,32 @@ int femd_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int 
                         const int markers[4], femd_mesh *out);
 int femd_mesh_from_arrays(int np, const double *points, int nt, const int *triangles, int ns,
                           const int *segments, const int *smarks, int marker, femd_mesh *out);
-/* out[3]: npoints, ntriangles, nsegments */
+/* Quadrilateral meshes (for Q_k spaces).  femd_quad_mesh_from_arrays replaces the QuadMesh
+ * constructor (mesh/quad_mesh.hpp:50-103, bound through python/femd/quadmesh.py:130-143):
+ * clockwise quads re-oriented, every quad strictly convex, neighbour i across the edge from
+ * vertex i to vertex i+1, the listed segments first, then every other boundary edge in quad
+ * order with `marker`.  femd_quad_mesh_rectangle replaces rectangle_quad_mesh
+ * (python/femd/quadmesh.py:161-170): rectangle_mesh's grid and sides, quad (i, j) = j nx + i
+ * with vertices (i, j), (i+1, j), (i+1, j+1), (i, j+1). */
+int femd_quad_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, const int markers[4],
+                             femd_mesh *out);
+int femd_quad_mesh_from_arrays(int np, const double *points, int nq, const int *quads, int ns, const int *segments,
+                               const int *smarks, int marker, femd_mesh *out);
+/* 3 for a triangle mesh, 4 for a quadrilateral one */
+int femd_mesh_cell_vertices(femd_mesh m, int *nv);
+/* out[3]: npoints, ncells, nsegments */
 int femd_mesh_sizes(femd_mesh m, int out[3]);
-/* points (np x 2), triangles and neighbours (nt x 3), segments (ns x 2), markers (ns) */
+/* points (np x 2), cells and neighbours (ncells x nv), segments (ns x 2), markers (ns) */
 int femd_mesh_arrays(femd_mesh m, double *points, int *triangles, int *neighbours, int *segments, int *smarks);
 void femd_mesh_free(femd_mesh m);
 
 /* ---------------------------------------------------------------- spaces
  * Replaces LagrangeSpace2D(mesh, k, dirichlet, all_exterior, equispaced)
- * (fe/lagrange_space_2d.hpp:32-67, bound at python/src/femd_bindings_2d.cpp:571-573): the
- * numbering of fe/space_2d.hpp:390-526, bit for bit.  Only equispaced nodes are built
- * (equispaced = 0 returns FEMD_EINVAL for k >= 3; for k <= 2 the two node sets coincide). */
+ * (fe/lagrange_space_2d.hpp:32-67, bound at python/src/femd_bindings_2d.cpp:571-573) on a
+ * triangle mesh (k = 1..5), and LagrangeSpaceQ(mesh, k, ...) (fe/lagrange_space_quad.hpp:112-210)
+ * on a quadrilateral one (k = 1..4): the numbering of fe/space_2d.hpp:390-526, bit for bit.
+ * Only equispaced nodes are built (equispaced = 0 returns FEMD_EINVAL for k >= 3; for k <= 2
+ * the two node sets coincide). */
 int femd_space_create(femd_mesh m, int degree, int ndirichlet, const int *dirichlet, int all_exterior,
                       int equispaced, femd_space *out);
 /* out[6]: ncells, nloc, raw_dim, dim, nedges, npoints */
@@ -127,6 +142,9 @@ int femd_program_compact(const femd_coefficient *c, int *code_out, int *nreg_out
 
 /* Quadrature of a degree: nq points (the rule of quadrature/triangle.hpp:22-57). */
 int femd_quadrature(int degree, int *nq, double *xi, double *eta, double *w);
+/* The quadrilateral rule of a degree (fe/cache_2d.hpp:75-84): tensor Gauss-Legendre with
+ * max(1, (degree+3)/2) points per direction on [0,1]^2, xi fastest. */
+int femd_quadrature_square(int degree, int *nq, double *xi, double *eta, double *w);
 
 /* ---------------------------------------------------------------- CSR matrices
  * femd_csr_create_space: zero values on the space's own pattern (the K of

@@ -131,8 +131,9 @@ class _Lib:
 class OracleProblem:
     """A mesh plus a P_k space held by one of the libraries."""
 
-    def __init__(self, owner, handle, free):
-        self.o, self.h, self._free = owner, handle, free
+    def __init__(self, owner, handle, free, quad=False):
+        self.o, self.h, self._free, self.quad = owner, handle, free, quad
+        self.nv = 4 if quad else 3
         s = self.sizes()
         self.ncells, self.nloc, self.raw_dim, self.dim = s[0], s[1], s[2], s[3]
 
@@ -182,6 +183,40 @@ class Oracle(_Lib):
         self._ie = self._fn("integrand_eval", _I, _I, _ip, _I, _dp, _L, _dp, _dp, _dp)
         self._ilu = self._fn("ilu0_factor", _I, _I, _ip, _ip, _dp, _I, _ip, _ip, _ip, _dp, C.POINTER(_D))
         self._marr = self._fn("mesh_arrays", _I, _P, _dp, _ip, _ip, _ip, _ip)
+        self._pqrect = self._fn("problem_quad_rectangle", _P, _D, _D, _D, _D, _I, _I, _I, _I, _ip, _I)
+        self._pqarr = self._fn("problem_quad_arrays", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I)
+        self._qquad = self._fn("quadrature_quad", _I, _I, C.POINTER(_I), _dp, _dp, _dp)
+        self._qbasis = self._fn("basis_eval_quad", _I, _I, _D, _D, _dp)
+
+    def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+        d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
+        h = self._pqrect(x0, y0, x1, y1, nx, ny, k, len(dirichlet), d, int(all_exterior))
+        if not h:
+            raise RuntimeError(self._err())
+        return OracleProblem(self, h, self._pfree, quad=True)
+
+    def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True):
+        pts, q = _f64(points).reshape(-1), _i32(quads).reshape(-1)
+        segs = _i32(segments).reshape(-1) if segments is not None else np.zeros(0, dtype=np.int32)
+        mk = _i32(markers) if markers is not None else None
+        d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
+        h = self._pqarr(pts.size // 2, pts, q.size // 4, q, segs.size // 2, segs, mk.ctypes.data if mk is not None else None,
+                        k, len(dirichlet), d, int(all_exterior))
+        if not h:
+            raise RuntimeError(self._err())
+        return OracleProblem(self, h, self._pfree, quad=True)
+
+    def quad_quadrature(self, degree):
+        xi, eta, w = np.empty(4096), np.empty(4096), np.empty(4096)
+        nq = _I(0)
+        self._check(self._qquad(int(degree), C.byref(nq), xi, eta, w))
+        return xi[: nq.value].copy(), eta[: nq.value].copy(), w[: nq.value].copy()
+
+    def quad_basis_eval(self, k, xi, eta):
+        n = (k + 1) ** 2
+        out = np.empty(3 * n)
+        self._check(self._qbasis(int(k), float(xi), float(eta), out))
+        return out.reshape(3, n)
 
     def rectangle(self, nx, ny, k, diagonal="right", dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
@@ -208,10 +243,10 @@ class Oracle(_Lib):
 
     def mesh_arrays(self, P):
         s = P.sizes()
-        pts, tris, nbr = np.empty(2 * s[5]), np.empty(3 * s[0], dtype=np.int32), np.empty(3 * s[0], dtype=np.int32)
+        pts, tris, nbr = np.empty(2 * s[5]), np.empty(P.nv * s[0], dtype=np.int32), np.empty(P.nv * s[0], dtype=np.int32)
         segs, mk = np.empty(2 * s[6], dtype=np.int32), np.empty(max(s[6], 1), dtype=np.int32)
         self._check(self._marr(P.h, pts, tris, nbr, segs, mk))
-        return pts.reshape(-1, 2), tris.reshape(-1, 3), nbr.reshape(-1, 3), segs.reshape(-1, 2), mk[: s[6]]
+        return pts.reshape(-1, 2), tris.reshape(-1, P.nv), nbr.reshape(-1, P.nv), segs.reshape(-1, 2), mk[: s[6]]
 
     def _numbering(self, P):
         cd = np.empty(P.ncells * P.nloc, dtype=np.int32)
@@ -241,7 +276,7 @@ class Oracle(_Lib):
         return out.reshape(3, n)
 
     def _npts(self, P, degree):
-        return P.ncells * len(self.quadrature(degree)[2])
+        return P.ncells * len((self.quad_quadrature if P.quad else self.quadrature)(degree)[2])
 
     def _cache_points(self, P, degree):
         m = self._npts(P, degree)
@@ -301,6 +336,55 @@ class Ref(_Lib):
         self._avec = self._fn("assemble_vector", _I, _P, _I, _I, _ip, _dp, _dp)
         self._ie = self._fn("integrand_eval", _I, _I, _ip, _I, _I, _dp, _L, _dp, _dp, _dp)
         self._asym = self._fn("asymmetry", _D, _I, _ip, _ip, _dp)
+        self._qcreate = self._fn("quad_space_create", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I)
+        self._qquads = self._fn("quad_mesh_quads", _I, _P, _ip)
+        self._qquad = self._fn("quad_quadrature", _I, _I, C.POINTER(_I), _dp, _dp, _dp)
+        self._qbasis = self._fn("quad_basis_eval", _I, _I, _D, _D, _dp)
+
+    def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True):
+        pts, q = _f64(points).reshape(-1), _i32(quads).reshape(-1)
+        segs = _i32(segments).reshape(-1) if segments is not None else np.zeros(0, dtype=np.int32)
+        mk = _i32(markers) if markers is not None else None
+        d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
+        h = self._qcreate(pts.size // 2, pts, q.size // 4, q, segs.size // 2, segs, mk.ctypes.data if mk is not None else None,
+                          int(k), len(dirichlet), d, int(all_exterior))
+        if not h:
+            raise RuntimeError(self._err())
+        return OracleProblem(self, h, self._sfree, quad=True)
+
+    def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+        """rectangle_quad_mesh (python/femd/quadmesh.py:146-170): the grid points and the sides of
+        the reference's own rectangle_mesh, quad (i, j) with vertices (i,j), (i+1,j), (i+1,j+1), (i,j+1)."""
+        m = self._mrect(x0, y0, x1, y1, nx, ny, b"right")
+        if not m:
+            raise RuntimeError(self._err())
+        s = np.zeros(3, dtype=np.int32)
+        self._check(self._msizes(m, s))
+        pts, tris, nbr = np.empty(2 * s[0]), np.empty(3 * s[1], dtype=np.int32), np.empty(3 * s[1], dtype=np.int32)
+        segs, mk = np.empty(2 * s[2], dtype=np.int32), np.empty(max(s[2], 1), dtype=np.int32)
+        self._check(self._mget(m, pts, tris, nbr, segs, mk))
+        self._mfree(m)
+        j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        v = (j * (nx + 1) + i).ravel()
+        quads = np.stack([v, v + 1, v + nx + 2, v + nx + 1], axis=1)
+        return self.quad_from_arrays(pts[: 2 * (nx + 1) * (ny + 1)], quads, segs, mk[: s[2]], k, dirichlet, all_exterior)
+
+    def quad_quadrature(self, degree):
+        xi, eta, w = np.empty(4096), np.empty(4096), np.empty(4096)
+        nq = _I(0)
+        self._check(self._qquad(int(degree), C.byref(nq), xi, eta, w))
+        return xi[: nq.value].copy(), eta[: nq.value].copy(), w[: nq.value].copy()
+
+    def quad_basis_eval(self, k, xi, eta):
+        n = (k + 1) ** 2
+        out = np.empty(3 * n)
+        self._check(self._qbasis(int(k), float(xi), float(eta), out))
+        return out.reshape(3, n)
+
+    def quad_mesh_quads(self, P):
+        out = np.empty(4 * P.ncells, dtype=np.int32)
+        self._check(self._qquads(P.h, out))
+        return out.reshape(-1, 4)
 
     def _space(self, mesh, k, dirichlet, all_exterior):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)

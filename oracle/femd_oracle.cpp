@@ -47,14 +47,23 @@ inline I64 ekey(int a, int b)
 // ---------------------------------------------------------------------------------- mesh
 // mesh/structured.hpp:99-221 (mesh_from_arrays), 260-351 (structured_grid_mesh, rectangle_mesh)
 struct Mesh {
+    int nv = 3;                             // vertices per cell: 3 triangles, 4 quadrilaterals
     std::vector<double> xy;                 // np x 2
-    std::vector<int> tri;                   // nt x 3, counter-clockwise
-    std::vector<int> nbr;                   // nt x 3, across the edge opposite vertex i
+    std::vector<int> tri;                   // nt x nv, counter-clockwise
+    std::vector<int> nbr;                   // nt x nv, across local edge i
     std::vector<int> seg;                   // ns x 2
     std::vector<int> smark;                 // ns
     std::unordered_map<I64, int> segidx;    // edge key -> segment index
     int np() const { return static_cast<int>(xy.size() / 2); }
-    int nt() const { return static_cast<int>(tri.size() / 3); }
+    int nt() const { return static_cast<int>(tri.size() / nv); }
+    // the vertices of local edge i of cell c: opposite vertex i on a triangle (mesh2d.hpp),
+    // from vertex i to vertex i+1 on a quadrilateral (quad_mesh.hpp:4-9)
+    void edge(int c, int i, int &a, int &b) const
+    {
+        const int *t = &tri[static_cast<std::size_t>(c) * nv];
+        if (nv == 3) { a = t[(i + 1) % 3]; b = t[(i + 2) % 3]; }
+        else { a = t[i]; b = t[(i + 1) % 4]; }
+    }
     int segment_marker(int a, int b) const
     {
         auto it = segidx.find(ekey(a, b));
@@ -185,6 +194,83 @@ Mesh rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int d
     for (int i = nx; i > 0; --i) { segs.push_back(V(i, ny)); segs.push_back(V(i - 1, ny)); smk.push_back(markers[2]); }
     for (int j = ny; j > 0; --j) { segs.push_back(V(0, j)); segs.push_back(V(0, j - 1)); smk.push_back(markers[3]); }
     return build_mesh(std::move(g), std::move(tri), segs, smk, markers[0]);
+}
+
+// mesh/quad_mesh.hpp:50-103 (the QuadMesh constructor from arrays): clockwise quads re-oriented
+// by swapping vertices 1 and 3, strict convexity required, neighbours across each edge, the
+// listed segments first, then every other boundary edge (in quad order) with `marker`.  The
+// convexity test is the sign of the plain cross product (the reference's exact predicate agrees
+// on every quad that is not nearly degenerate).
+Mesh build_quad_mesh(std::vector<double> xy, std::vector<int> q, const std::vector<int> &segs, const std::vector<int> &smarks,
+                     int marker)
+{
+    Mesh M;
+    M.nv = 4;
+    const int np = static_cast<int>(xy.size() / 2), nq = static_cast<int>(q.size() / 4);
+    auto P = [&](int v, int d) { return xy[2 * static_cast<std::size_t>(v) + d]; };
+    for (int t = 0; t < nq; ++t)
+    {
+        int *Q = &q[4 * static_cast<std::size_t>(t)];
+        for (int k = 0; k < 4; ++k)
+            if (Q[k] < 0 || Q[k] >= np) throw std::invalid_argument("quad mesh: vertex index out of range");
+        double s = 0.0;
+        for (int i = 0; i < 4; ++i) s += P(Q[i], 0) * P(Q[(i + 1) % 4], 1) - P(Q[(i + 1) % 4], 0) * P(Q[i], 1);
+        if (0.5 * s < 0.0) std::swap(Q[1], Q[3]);
+        for (int i = 0; i < 4; ++i)
+        {
+            const int a = Q[(i + 3) % 4], b = Q[i], c = Q[(i + 1) % 4];
+            const double o = (P(b, 0) - P(a, 0)) * (P(c, 1) - P(a, 1)) - (P(b, 1) - P(a, 1)) * (P(c, 0) - P(a, 0));
+            if (!(o > 0.0)) throw std::invalid_argument("quad mesh: a quad is degenerate or not convex");
+        }
+    }
+    M.xy = std::move(xy);
+    M.tri = std::move(q);
+    M.nbr.assign(static_cast<std::size_t>(nq) * 4, -1);
+    std::unordered_map<I64, std::pair<int, int>> first;
+    for (int t = 0; t < nq; ++t)
+        for (int i = 0; i < 4; ++i)
+        {
+            int a, b;
+            M.edge(t, i, a, b);
+            auto it = first.find(ekey(a, b));
+            if (it == first.end()) { first.emplace(ekey(a, b), std::make_pair(t, i)); continue; }
+            const int u = it->second.first, j = it->second.second;
+            if (M.nbr[4 * u + j] >= 0) throw std::invalid_argument("quad mesh: edge shared by more than two quads");
+            M.nbr[4 * t + i] = u; M.nbr[4 * u + j] = t;
+        }
+    const int ns = static_cast<int>(segs.size() / 2);
+    for (int s2 = 0; s2 < ns; ++s2)
+    {
+        if (first.find(ekey(segs[2 * s2], segs[2 * s2 + 1])) == first.end())
+            throw std::invalid_argument("quad mesh: a segment is not an edge");
+        M.add_segment(segs[2 * s2], segs[2 * s2 + 1], smarks.empty() ? marker : smarks[s2]);
+    }
+    for (int t = 0; t < nq; ++t)
+        for (int i = 0; i < 4; ++i)
+            if (M.nbr[4 * t + i] < 0)
+            {
+                int a, b;
+                M.edge(t, i, a, b);
+                if (M.segidx.find(ekey(a, b)) == M.segidx.end()) M.add_segment(a, b, marker);
+            }
+    return M;
+}
+
+// python/femd/quadmesh.py:146-170 (rectangle_quad_mesh): the grid and the sides of
+// rectangle_mesh, quad (i, j) = j nx + i with vertices (i, j), (i+1, j), (i+1, j+1), (i, j+1)
+Mesh quad_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, const int markers[4])
+{
+    const Mesh T = rectangle(x0, y0, x1, y1, nx, ny, 0, markers);
+    std::vector<double> g(T.xy.begin(), T.xy.begin() + 2 * static_cast<std::size_t>(nx + 1) * (ny + 1));
+    std::vector<int> q;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+        {
+            const int v = j * (nx + 1) + i;
+            const int c4[4] = {v, v + 1, v + nx + 2, v + nx + 1};
+            q.insert(q.end(), c4, c4 + 4);
+        }
+    return build_quad_mesh(std::move(g), std::move(q), T.seg, T.smark, markers[0]);
 }
 
 // ------------------------------------------------------------------------ reference element
@@ -327,6 +413,63 @@ struct RefElement {
     }
 };
 
+// fe/lagrange_space_quad.hpp:28-110 (ReferenceQuad, equispaced nodes): the tensor product of
+// 1D Lagrange polynomials in barycentric form on k+1 symmetrized points; local order vertices,
+// edge nodes along each edge, interior row by row.
+struct RefQuad {
+    int k = 1, n = 4;
+    std::vector<double> t, w, nx, ny;
+    std::vector<int> ii, jj;
+    explicit RefQuad(int deg) : k(deg), n((deg + 1) * (deg + 1))
+    {
+        std::vector<double> z;
+        for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+        t.resize(static_cast<std::size_t>(k) + 1);
+        for (int j = 0; j <= k; ++j) t[j] = 0.5 * (1.0 + 0.5 * (z[j] - z[k - j]));
+        t[0] = 0.0; t[k] = 1.0;
+        w.assign(static_cast<std::size_t>(k) + 1, 1.0);
+        for (int j = 0; j <= k; ++j)
+            for (int m = 0; m <= k; ++m) if (m != j) w[j] /= (t[j] - t[m]);
+        auto add = [&](int i, int j) { ii.push_back(i); jj.push_back(j); };
+        add(0, 0); add(k, 0); add(k, k); add(0, k);
+        for (int j = 1; j < k; ++j) add(j, 0);
+        for (int j = 1; j < k; ++j) add(k, j);
+        for (int j = 1; j < k; ++j) add(k - j, k);
+        for (int j = 1; j < k; ++j) add(0, k - j);
+        for (int j = 1; j < k; ++j)
+            for (int i = 1; i < k; ++i) add(i, j);
+        for (int l = 0; l < n; ++l) { nx.push_back(t[ii[l]]); ny.push_back(t[jj[l]]); }
+    }
+    void basis_1d(double x, double *L, double *dL) const
+    {
+        for (int j = 0; j <= k; ++j)
+        {
+            double p = 1.0, dp = 0.0;
+            for (int m = 0; m <= k; ++m)
+            {
+                if (m == j) continue;
+                dp = dp * (x - t[m]) + p;
+                p *= (x - t[m]);
+            }
+            L[j] = w[j] * p;
+            dL[j] = w[j] * dp;
+        }
+    }
+    void eval(double xi, double eta, double *out) const
+    {
+        std::vector<double> a(static_cast<std::size_t>(4) * (k + 1));
+        double *Lx = a.data(), *dLx = Lx + (k + 1), *Ly = dLx + (k + 1), *dLy = Ly + (k + 1);
+        basis_1d(xi, Lx, dLx);
+        basis_1d(eta, Ly, dLy);
+        for (int l = 0; l < n; ++l)
+        {
+            out[l] = Lx[ii[l]] * Ly[jj[l]];
+            out[n + l] = dLx[ii[l]] * Ly[jj[l]];
+            out[2 * n + l] = Lx[ii[l]] * dLy[jj[l]];
+        }
+    }
+};
+
 // ------------------------------------------------------------------------------ quadrature
 // quadrature/gauss_legendre.hpp:30-137 and quadrature/triangle.hpp:22-57
 void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w)
@@ -401,6 +544,23 @@ void triangle_rule(int d, std::vector<double> &qx, std::vector<double> &qy, std:
     }
 }
 
+// fe/cache_2d.hpp:75-84: the tensor Gauss-Legendre rule with max(1, (d+3)/2) points per direction
+// on [0,1]^2, xi running fastest
+void quad_rule(int d, std::vector<double> &qx, std::vector<double> &qy, std::vector<double> &qw)
+{
+    qx.clear(); qy.clear(); qw.clear();
+    std::vector<double> g, gw;
+    gauss_legendre(std::max(1, (d + 3) / 2), g, gw);
+    const int m = static_cast<int>(g.size());
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i)
+        {
+            qx.push_back(0.5 * (g[i] + 1.0));
+            qy.push_back(0.5 * (g[j] + 1.0));
+            qw.push_back(0.25 * gw[i] * gw[j]);
+        }
+}
+
 // --------------------------------------------------------------------------------- space
 // fe/space_2d.hpp:390-526 (edges in first-met order, raw numbering, Dirichlet elimination,
 // adapted numbering) and fe/lagrange_space_2d.hpp:32-67 (per-cell J, J^{-1}, det).
@@ -410,7 +570,30 @@ struct Problem {
     std::vector<int> cell_raw, cell_dof, ad2raw, raw2ad;
     std::vector<double> raw_xy, J, Ji, det;
     std::unique_ptr<RefElement> ref;
+    std::unique_ptr<RefQuad> refq;            // quadrilateral meshes
+    bool quad() const { return mesh.nv == 4; }
+    void ref_eval(double xi, double eta, double *out) const { if (refq) refq->eval(xi, eta, out); else ref->eval(xi, eta, out); }
+    double node_xi(int l) const { return refq ? refq->nx[l] : ref->nx[l]; }
+    double node_eta(int l) const { return refq ? refq->ny[l] : ref->ny[l]; }
 };
+
+// fe/lagrange_space_quad.hpp:155-172: the bilinear map of quad c and its Jacobian
+// [dx/dxi, dx/deta, dy/dxi, dy/deta]
+void quad_map(const Mesh &M, int c, double xi, double eta, double &x, double &y, double *J)
+{
+    const int *q = &M.tri[4 * static_cast<std::size_t>(c)];
+    const double *P0 = &M.xy[2 * q[0]], *P1 = &M.xy[2 * q[1]], *P2 = &M.xy[2 * q[2]], *P3 = &M.xy[2 * q[3]];
+    const double a0 = (1 - xi) * (1 - eta), a1 = xi * (1 - eta), a2 = xi * eta, a3 = (1 - xi) * eta;
+    x = a0 * P0[0] + a1 * P1[0] + a2 * P2[0] + a3 * P3[0];
+    y = a0 * P0[1] + a1 * P1[1] + a2 * P2[1] + a3 * P3[1];
+    if (J)
+    {
+        J[0] = (P1[0] - P0[0]) * (1 - eta) + (P2[0] - P3[0]) * eta;
+        J[1] = (P3[0] - P0[0]) * (1 - xi) + (P2[0] - P1[0]) * xi;
+        J[2] = (P1[1] - P0[1]) * (1 - eta) + (P2[1] - P3[1]) * eta;
+        J[3] = (P3[1] - P0[1]) * (1 - xi) + (P2[1] - P1[1]) * xi;
+    }
+}
 
 void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool all_exterior)
 {
@@ -419,13 +602,14 @@ void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool a
     std::sort(dirichlet.begin(), dirichlet.end());
     dirichlet.erase(std::unique(dirichlet.begin(), dirichlet.end()), dirichlet.end());
     const Mesh &M = P.mesh;
-    const int nc = M.nt(), np = M.np();
+    const int nc = M.nt(), np = M.np(), nv = M.nv;
     P.k = k;
-    P.ref = std::make_unique<RefElement>(k);
-    P.nloc = P.ref->n;
-    const int n = P.nloc, de = k - 1, nint = (k - 1) * (k - 2) / 2, f0 = 3 + 3 * (k - 1);
+    if (nv == 4) P.refq = std::make_unique<RefQuad>(k);
+    else P.ref = std::make_unique<RefElement>(k);
+    P.nloc = nv == 4 ? P.refq->n : P.ref->n;
+    const int n = P.nloc, de = k - 1, nint = nv == 4 ? (k - 1) * (k - 1) : (k - 1) * (k - 2) / 2, f0 = nv + nv * (k - 1);
     P.J.resize(4 * static_cast<std::size_t>(nc)); P.Ji.resize(P.J.size()); P.det.resize(nc);
-    for (int c = 0; c < nc; ++c)
+    for (int c = 0; c < nc && nv == 3; ++c)
     {
         const int *t = &M.tri[3 * c];
         double *J = &P.J[4 * c], *Ji = &P.Ji[4 * c];
@@ -436,20 +620,21 @@ void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool a
         P.det[c] = d;
         Ji[0] = J[3] / d; Ji[1] = -J[1] / d; Ji[2] = -J[2] / d; Ji[3] = J[0] / d;
     }
-    // edges, first met (cell c, local edge i from vertex (i+1)%3 to (i+2)%3)
-    std::vector<int> cedge(3 * static_cast<std::size_t>(nc)), cflip(3 * static_cast<std::size_t>(nc));
+    // edges, first met (cell c, local edge i: Mesh::edge)
+    std::vector<int> cedge(nv * static_cast<std::size_t>(nc)), cflip(nv * static_cast<std::size_t>(nc));
     std::vector<std::array<int, 2>> edges;
     std::unordered_map<I64, int> emap;
     emap.reserve(2 * static_cast<std::size_t>(nc));
     for (int c = 0; c < nc; ++c)
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < nv; ++i)
         {
-            const int a = M.tri[3 * c + (i + 1) % 3], b = M.tri[3 * c + (i + 2) % 3];
+            int a, b;
+            M.edge(c, i, a, b);
             auto it = emap.find(ekey(a, b));
             int g;
             if (it == emap.end()) { g = static_cast<int>(edges.size()); edges.push_back({std::min(a, b), std::max(a, b)}); emap.emplace(ekey(a, b), g); }
             else g = it->second;
-            cedge[3 * c + i] = g; cflip[3 * c + i] = a > b;
+            cedge[nv * c + i] = g; cflip[nv * c + i] = a > b;
         }
     P.nedges = static_cast<int>(edges.size());
     P.raw_dim = np + P.nedges * de + nc * nint;
@@ -457,10 +642,10 @@ void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool a
     for (int c = 0; c < nc; ++c)
     {
         int *d = &P.cell_raw[static_cast<std::size_t>(c) * n];
-        for (int v = 0; v < 3; ++v) d[v] = M.tri[3 * c + v];
-        for (int i = 0; i < 3; ++i)
+        for (int v = 0; v < nv; ++v) d[v] = M.tri[nv * c + v];
+        for (int i = 0; i < nv; ++i)
             for (int j = 0; j < de; ++j)
-                d[3 + i * de + j] = np + cedge[3 * c + i] * de + (cflip[3 * c + i] ? de - 1 - j : j);
+                d[nv + i * de + j] = np + cedge[nv * c + i] * de + (cflip[nv * c + i] ? de - 1 - j : j);
         for (int j = 0; j < nint; ++j) d[f0 + j] = np + P.nedges * de + c * nint + j;
     }
     // node coordinates (space_2d.hpp:475-490)
@@ -468,24 +653,30 @@ void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool a
     std::vector<char> done(static_cast<std::size_t>(P.raw_dim), 0);
     for (int v = 0; v < np; ++v) { P.raw_xy[2 * v] = M.xy[2 * v]; P.raw_xy[2 * v + 1] = M.xy[2 * v + 1]; done[v] = 1; }
     for (int c = 0; c < nc; ++c)
-        for (int l = 3; l < n; ++l)
+        for (int l = nv; l < n; ++l)
         {
             const int r = P.cell_raw[static_cast<std::size_t>(c) * n + l];
             if (done[r]) continue;
-            const double *J = &P.J[4 * c];
-            const double ox = M.xy[2 * M.tri[3 * c]], oy = M.xy[2 * M.tri[3 * c] + 1];
-            P.raw_xy[2 * r] = ox + J[0] * P.ref->nx[l] + J[1] * P.ref->ny[l];
-            P.raw_xy[2 * r + 1] = oy + J[2] * P.ref->nx[l] + J[3] * P.ref->ny[l];
+            if (nv == 4)
+                quad_map(M, c, P.node_xi(l), P.node_eta(l), P.raw_xy[2 * r], P.raw_xy[2 * r + 1], nullptr);
+            else
+            {
+                const double *J = &P.J[4 * c];
+                const double ox = M.xy[2 * M.tri[3 * c]], oy = M.xy[2 * M.tri[3 * c] + 1];
+                P.raw_xy[2 * r] = ox + J[0] * P.ref->nx[l] + J[1] * P.ref->ny[l];
+                P.raw_xy[2 * r + 1] = oy + J[2] * P.ref->nx[l] + J[3] * P.ref->ny[l];
+            }
             done[r] = 1;
         }
     // Dirichlet facets: exterior edges and segment edges, in cell order (space_2d.hpp:428-446, 491-498)
     std::vector<char> fixed(static_cast<std::size_t>(P.raw_dim), 0), seen(edges.size(), 0);
     for (int c = 0; c < nc; ++c)
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < nv; ++i)
         {
-            const int g = cedge[3 * c + i];
-            const int a = M.tri[3 * c + (i + 1) % 3], b = M.tri[3 * c + (i + 2) % 3];
-            const bool ext = M.nbr[3 * c + i] < 0;
+            const int g = cedge[nv * c + i];
+            int a, b;
+            M.edge(c, i, a, b);
+            const bool ext = M.nbr[nv * c + i] < 0;
             const int m = M.segment_marker(a, b);
             if (!ext && m < 0) continue;
             if (!ext && seen[g]) continue;
@@ -581,18 +772,38 @@ struct Quad {
     int nq = 0;
     std::vector<double> tab;              // nq x 3n reference tabulation
     std::vector<double> x, y, w;          // nc x nq (cache_2d.hpp:98-118)
+    std::vector<double> Ji;               // nc x nq x 4 on quadrilaterals (J varies in the cell)
 };
 
 Quad make_quad(const Problem &P, int degree)
 {
     Quad Q;
     std::vector<double> qx, qy, qw;
-    triangle_rule(degree, qx, qy, qw);
+    if (P.quad()) quad_rule(degree, qx, qy, qw);
+    else triangle_rule(degree, qx, qy, qw);
     Q.nq = static_cast<int>(qw.size());
     const int n = P.nloc, nc = P.mesh.nt();
     Q.tab.resize(static_cast<std::size_t>(Q.nq) * 3 * n);
-    for (int q = 0; q < Q.nq; ++q) P.ref->eval(qx[q], qy[q], &Q.tab[static_cast<std::size_t>(q) * 3 * n]);
+    for (int q = 0; q < Q.nq; ++q) P.ref_eval(qx[q], qy[q], &Q.tab[static_cast<std::size_t>(q) * 3 * n]);
     Q.x.resize(static_cast<std::size_t>(nc) * Q.nq); Q.y.resize(Q.x.size()); Q.w.resize(Q.x.size());
+    if (P.quad())
+    {
+        // cache_2d.hpp:98-118 with the bilinear map: J, det, w, J^{-1} per point
+        Q.Ji.resize(4 * Q.x.size());
+        for (int c = 0; c < nc; ++c)
+            for (int q = 0; q < Q.nq; ++q)
+            {
+                const std::size_t k = static_cast<std::size_t>(c) * Q.nq + q;
+                double J[4];
+                quad_map(P.mesh, c, qx[q], qy[q], Q.x[k], Q.y[k], J);
+                const double d = J[0] * J[3] - J[1] * J[2];
+                Q.w[k] = qw[q] * std::abs(d);
+                double *Ji = &Q.Ji[4 * k];
+                Ji[0] = J[3] / d; Ji[1] = -J[1] / d;
+                Ji[2] = -J[2] / d; Ji[3] = J[0] / d;
+            }
+        return Q;
+    }
     for (int c = 0; c < nc; ++c)
     {
         const double *J = &P.J[4 * c];
@@ -615,9 +826,14 @@ void phys_row(const Problem &P, const Quad &Q, int c, int q, int code, double *o
     const int n = P.nloc;
     const double *R = &Q.tab[static_cast<std::size_t>(q) * 3 * n];
     if (code == 0) { std::copy(R, R + n, out); return; }
-    const double *J = &P.J[4 * c];
-    const double d = J[0] * J[3] - J[1] * J[2];
-    const double Ji[4] = {J[3] / d, -J[1] / d, -J[2] / d, J[0] / d};
+    double Ji[4];
+    if (P.quad()) std::copy_n(&Q.Ji[4 * (static_cast<std::size_t>(c) * Q.nq + q)], 4, Ji);
+    else
+    {
+        const double *J = &P.J[4 * c];
+        const double d = J[0] * J[3] - J[1] * J[2];
+        Ji[0] = J[3] / d; Ji[1] = -J[1] / d; Ji[2] = -J[2] / d; Ji[3] = J[0] / d;
+    }
     const double a = code == 1 ? Ji[0] : Ji[1], b = code == 1 ? Ji[2] : Ji[3];
     for (int l = 0; l < n; ++l) out[l] = a * R[n + l] + b * R[2 * n + l];
 }
@@ -1309,6 +1525,37 @@ void *orc_problem_arrays(int np, const double *pts, int nt, const int *tris, int
     return out;
 }
 
+/// A structured quadrilateral mesh of the rectangle (quadmesh.py:161-170) and a Q_k space on it.
+void *orc_problem_quad_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int k, int ndir, const int *dir,
+                                 int all_exterior)
+{
+    Problem *out = nullptr;
+    guard([&] {
+        auto P = std::make_unique<Problem>();
+        const int mk[4] = {1, 2, 3, 4};
+        P->mesh = quad_rectangle(x0, y0, x1, y1, nx, ny, mk);
+        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0);
+        out = P.release();
+    });
+    return out;
+}
+
+/// A quadrilateral mesh from arrays (quad_mesh.hpp:50-103) and a Q_k space on it.
+void *orc_problem_quad_arrays(int np, const double *pts, int nq, const int *quads, int ns, const int *segs, const int *smarks,
+                              int k, int ndir, const int *dir, int all_exterior)
+{
+    Problem *out = nullptr;
+    guard([&] {
+        auto P = std::make_unique<Problem>();
+        P->mesh = build_quad_mesh(std::vector<double>(pts, pts + 2 * np), std::vector<int>(quads, quads + 4 * nq),
+                                  std::vector<int>(segs, segs + 2 * ns), smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{},
+                                  1);
+        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0);
+        out = P.release();
+    });
+    return out;
+}
+
 void orc_problem_free(void *h) { delete static_cast<Problem *>(h); }
 
 /// out: ncells, nloc, raw_dim, dim, nedges, npoints, nsegments
@@ -1374,6 +1621,23 @@ int orc_quadrature(int degree, int *nq, double *xi, double *eta, double *w)
 int orc_basis_eval(int k, double xi, double eta, double *out)
 {
     return guard([&] { RefElement R(k); R.eval(xi, eta, out); });
+}
+
+/// The quadrilateral rule of `degree` (tensor Gauss-Legendre on [0,1]^2).
+int orc_quadrature_quad(int degree, int *nq, double *xi, double *eta, double *w)
+{
+    return guard([&] {
+        std::vector<double> a, b, c;
+        quad_rule(degree, a, b, c);
+        *nq = static_cast<int>(c.size());
+        std::copy(a.begin(), a.end(), xi); std::copy(b.begin(), b.end(), eta); std::copy(c.begin(), c.end(), w);
+    });
+}
+
+/// Values, d/dxi, d/deta of the Q_k basis at (xi, eta): 3 (k+1)^2 numbers.
+int orc_basis_eval_quad(int k, double xi, double eta, double *out)
+{
+    return guard([&] { RefQuad R(k); R.eval(xi, eta, out); });
 }
 
 int orc_cache_points(void *h, int degree, double *x, double *y, double *w)

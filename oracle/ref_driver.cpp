@@ -13,6 +13,8 @@
 //   ref_mesh_rectangle      mesh/structured.hpp:338-351 (rectangle_mesh)
 //   ref_mesh_from_arrays    mesh/structured.hpp:99-221  (mesh_from_arrays)
 //   ref_space_create        fe/lagrange_space_2d.hpp:32-67, fe/space_2d.hpp:372-526
+//   ref_quad_space_create   mesh/quad_mesh.hpp:50-103, fe/lagrange_space_quad.hpp:112-210
+//   ref_quad_quadrature, ref_quad_basis_eval   fe/cache_2d.hpp:75-84, lagrange_space_quad.hpp:28-110
 //   ref_pattern             forms/assembly_2d.hpp:36-41, sparse/csr.hpp:84-125
 //   ref_assemble_matrix     fe/cache_2d.hpp:67-121, forms/assembly_2d.hpp:136-164,219-305
 //   ref_assemble_vector     forms/assembly_2d.hpp:173-211
@@ -23,7 +25,9 @@
 //   ref_rcm, ref_precond_apply  sparse/amd.hpp:166-234, sparse/precond.hpp:82-205
 //   ref_asymmetry           sparse/csr.hpp:631-644
 #include "femd/femd.hpp"
+#include "femd/fe/lagrange_space_quad.hpp"
 #include "femd/krylov/adapters.hpp"
+#include "femd/mesh/quad_mesh.hpp"
 
 #include <cstring>
 #include <omp.h>
@@ -45,7 +49,8 @@ template <class F> int guard(F &&f)
 
 struct Space {
     std::shared_ptr<Mesh2D> mesh;
-    std::unique_ptr<LagrangeSpace2D> V;
+    std::shared_ptr<QuadMesh> qmesh;
+    std::unique_ptr<Space2D> V;
 };
 
 CSRMatrix make_csr(int n, int m, const int *ip, const int *ix, const double *v)
@@ -129,13 +134,70 @@ void *ref_space_create(void *mh, int k, int ndir, const int *dir, int all_exteri
     return out;
 }
 
+/// A QuadMesh from arrays (quads counter-clockwise or clockwise; the listed segments with their
+/// markers, other boundary edges marker 1) and a Q_k space on it.
+void *ref_quad_space_create(int np, const double *pts, int nq, const int *quads, int ns, const int *segs, const int *smarks,
+                            int k, int ndir, const int *dir, int all_exterior)
+{
+    Space *out = nullptr;
+    guard([&] {
+        auto S = std::make_unique<Space>();
+        std::vector<Point> P(static_cast<std::size_t>(np));
+        for (int i = 0; i < np; ++i) P[i] = Point{pts[2 * i], pts[2 * i + 1]};
+        std::vector<std::array<int, 4>> Q(static_cast<std::size_t>(nq));
+        for (int q = 0; q < nq; ++q) Q[q] = {quads[4 * q], quads[4 * q + 1], quads[4 * q + 2], quads[4 * q + 3]};
+        std::vector<std::array<int, 2>> G(static_cast<std::size_t>(ns));
+        for (int e = 0; e < ns; ++e) G[e] = {segs[2 * e], segs[2 * e + 1]};
+        std::vector<int> M = smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{};
+        S->qmesh = std::make_shared<QuadMesh>(std::move(P), std::move(Q), std::move(G), std::move(M), std::vector<int>{}, 1);
+        S->V = std::make_unique<LagrangeSpaceQ>(*S->qmesh, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, true);
+        out = S.release();
+    });
+    return out;
+}
+
+/// The quads of a space's mesh after orientation (nq x 4).
+int ref_quad_mesh_quads(void *sh, int *quads)
+{
+    return guard([&] {
+        const QuadMesh &M = *static_cast<Space *>(sh)->qmesh;
+        for (int q = 0; q < M.nquads(); ++q)
+            for (int i = 0; i < 4; ++i) quads[4 * q + i] = M.quad(q)[i];
+    });
+}
+
+/// The quadrilateral rule Cache2D uses for `degree` (up to 64 points).
+int ref_quad_quadrature(int degree, int *nq, double *xi, double *eta, double *w)
+{
+    return guard([&] {
+        GaussLegendre g(std::max(1, (degree + 3) / 2));
+        int k = 0;
+        for (int j = 0; j < g.size(); ++j)
+            for (int i = 0; i < g.size(); ++i, ++k)
+            {
+                xi[k] = 0.5 * (g.node(i) + 1.0); eta[k] = 0.5 * (g.node(j) + 1.0);
+                w[k] = 0.25 * g.weight(i) * g.weight(j);
+            }
+        *nq = k;
+    });
+}
+
+/// Values, d/dxi, d/deta of the Q_k basis at (xi, eta): 3 (k+1)^2 numbers.
+int ref_quad_basis_eval(int k, double xi, double eta, double *out)
+{
+    return guard([&] {
+        ReferenceQuad R(k, true);
+        R.eval(xi, eta, 1, out);
+    });
+}
+
 void ref_space_free(void *s) { delete static_cast<Space *>(s); }
 
 /// out: ncells, nloc, raw_dim, dim, nedges, npoints
 int ref_space_sizes(void *sh, int *out)
 {
     return guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         out[0] = V.ncells(); out[1] = V.nloc(); out[2] = V.raw_dim(); out[3] = V.dim(); out[4] = V.nedges(); out[5] = V.npoints();
     });
 }
@@ -143,7 +205,7 @@ int ref_space_sizes(void *sh, int *out)
 int ref_space_numbering(void *sh, int *cell_dof, int *cell_raw, int *ad2raw, double *raw_xy)
 {
     return guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         std::copy(V.cell_dofs().begin(), V.cell_dofs().end(), cell_dof);
         std::copy(V.cell_raw_dofs().begin(), V.cell_raw_dofs().end(), cell_raw);
         for (int a = 0; a < V.dim(); ++a) ad2raw[a] = V.adapted_to_raw(a);
@@ -155,7 +217,7 @@ long ref_pattern_nnz(void *sh)
 {
     long n = -1;
     guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         n = cell_pattern(V, V)->nnz();
     });
     return n;
@@ -164,7 +226,7 @@ long ref_pattern_nnz(void *sh)
 int ref_pattern(void *sh, int *indptr, int *indices)
 {
     return guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         auto P = cell_pattern(V, V);
         std::copy(P->indptr.begin(), P->indptr.end(), indptr);
         std::copy(P->indices.begin(), P->indices.end(), indices);
@@ -194,7 +256,7 @@ int ref_basis_eval(int k, int equispaced, double xi, double eta, double *out)
 int ref_cache_points(void *sh, int degree, double *x, double *y, double *w)
 {
     return guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         Cache2D Q(V, degree);
         std::copy(Q.x().begin(), Q.x().end(), x);
         std::copy(Q.y().begin(), Q.y().end(), y);
@@ -207,7 +269,7 @@ int ref_cache_points(void *sh, int degree, double *x, double *y, double *w)
 int ref_assemble_matrix(void *sh, int degree, int nterms, const int *a, const int *b, const double *coeffs, double *data)
 {
     return guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         Cache2D Q(V, degree);
         const std::size_t np = static_cast<std::size_t>(Q.nent()) * Q.nq();
         std::vector<std::vector<double>> c(static_cast<std::size_t>(nterms));
@@ -223,7 +285,7 @@ int ref_assemble_matrix(void *sh, int degree, int nterms, const int *a, const in
 int ref_assemble_vector(void *sh, int degree, int nterms, const int *a, const double *coeffs, double *f)
 {
     return guard([&] {
-        const LagrangeSpace2D &V = *static_cast<Space *>(sh)->V;
+        const Space2D &V = *static_cast<Space *>(sh)->V;
         Cache2D Q(V, degree);
         const std::size_t np = static_cast<std::size_t>(Q.nent()) * Q.nq();
         std::vector<std::vector<double>> c(static_cast<std::size_t>(nterms));

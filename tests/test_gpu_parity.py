@@ -265,3 +265,57 @@ def test_breakdown_and_errors(orc):
     assert not info.converged and info.iterations == 3 and np.array_equal(x, xo)
     x, info = fd.cg(Ag, np.zeros_like(bc))
     assert info.converged and not x.any()
+
+
+# ------------------------------------------------------------------- Q_k on quadrilaterals
+def _distorted_quads(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1))
+    pts = np.stack([xs.ravel(), ys.ravel()], 1)
+    inner = (pts[:, 0] > 0) & (pts[:, 0] < 1) & (pts[:, 1] > 0) & (pts[:, 1] < 1)
+    pts[inner] += rng.uniform(-0.25, 0.25, (inner.sum(), 2)) / max(nx, ny)
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    v = (j * (nx + 1) + i).ravel()
+    q = np.stack([v, v + 1, v + nx + 2, v + nx + 1], 1)
+    q[::3] = q[::3, ::-1]
+    return pts, q
+
+
+@pytest.mark.parametrize("k,deg", [(1, 2), (2, 4), (3, 7), (4, 9)])
+def test_quad_assembly_bit_exact(orc, k, deg):
+    _name(f"Q{k} assembly deg={deg}, rectangle and distorted meshes")
+    pts, q = _distorted_quads(7, 6, k)
+    cases = [(fd.LagrangeSpace(fd.rectangle_quad_mesh(0, 0, 1, 1, 7, 6), k, bc="dirichlet"), orc.quad_rectangle(7, 6, k)),
+             (fd.LagrangeSpace(fd.quad_mesh_from_arrays(pts, q), k), orc.quad_from_arrays(pts, q, k=k, all_exterior=False))]
+    for V, P in cases:
+        npt = P.ncells * len(orc.quad_quadrature(deg)[2])
+        rng = np.random.default_rng(k)
+        c = rng.standard_normal((4, npt))
+        c[2, ::5] = 0.0
+        a, b = [1, 2, 0, 1], [1, 2, 0, 2]
+        K = _assemble_values(V, deg, a, b, c)
+        assert np.array_equal(K.data, P.assemble_matrix(deg, a, b, c))
+        assert np.array_equal(_vector_values(V, deg, [0, 1, 2], c[:3]), P.assemble_vector(deg, [0, 1, 2], c[:3]))
+
+
+def test_quad_forms_and_solve(orc):
+    _name("Q2 forms with device coefficients on a distorted mesh, then Jacobi-CG")
+    pts, q = _distorted_quads(12, 10, 5)
+    V = fd.LagrangeSpace(fd.quad_mesh_from_arrays(pts, q), 2, bc="dirichlet")
+    P = orc.quad_from_arrays(pts, q, k=2)
+    kap, _ = problems.kappa_field(1)
+    f, _ = problems.fourier_source(3)
+    a, L_ = problems.poisson_forms(V, f, kappa=kap)
+    A, b = a.assemble(), L_.assemble()
+    code, nreg, consts = fd.forms.Program(kap).arrays()
+    x, y, _ = P.cache_points(a.quad_degree)
+    ck = orc.integrand_eval(code, nreg, consts, x, y)
+    Ao = P.assemble_matrix(a.quad_degree, [1, 2], [1, 2], np.stack([ck, ck]))
+    assert np.abs(A.data - Ao).max() <= 1e-13 * np.abs(Ao).max()
+    # fed the oracle's coefficient values the assembly is bit for bit
+    K = _assemble_values(V, a.quad_degree, [1, 2], [1, 2], [ck, ck])
+    assert np.array_equal(K.data, Ao)
+    ip, ix = V.pattern()
+    xg, info = fd.cg(K, b, M=K.preconditioner("jacobi"), tol=1e-10)
+    xo, io = orc.cg(ip, ix, Ao, b, M="jacobi", tol=1e-10)
+    assert info.iterations == io["iterations"] and np.array_equal(xg, xo)

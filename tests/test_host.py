@@ -43,6 +43,54 @@ def test_rectangle_numbering_vs_oracle(orc, k, diag):
             assert np.array_equal(a, b)
 
 
+def _distorted_quads(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1))
+    pts = np.stack([xs.ravel(), ys.ravel()], 1)
+    inner = (pts[:, 0] > 0) & (pts[:, 0] < 1) & (pts[:, 1] > 0) & (pts[:, 1] < 1)
+    pts[inner] += rng.uniform(-0.25, 0.25, (inner.sum(), 2)) / max(nx, ny)
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    v = (j * (nx + 1) + i).ravel()
+    q = np.stack([v, v + 1, v + nx + 2, v + nx + 1], 1)
+    q[::3] = q[::3, ::-1]
+    return pts, q
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_quad_numbering_vs_oracle(orc, k):
+    m = fd.rectangle_quad_mesh(0, 0, 1, 1, 6, 4)
+    assert m.nverts == 4 and m.nquads == 24 and m.quads.shape == (24, 4)
+    pts, q = _distorted_quads(5, 6, k)
+    mq = fd.quad_mesh_from_arrays(pts, q)
+    cases = [(fd.LagrangeSpace(m, k, bc="dirichlet"), orc.quad_rectangle(6, 4, k)),
+             (fd.LagrangeSpace(m, k, dirichlet=[2, 3]), orc.quad_rectangle(6, 4, k, dirichlet=[2, 3], all_exterior=False)),
+             (fd.LagrangeSpace(mq, k), orc.quad_from_arrays(pts, q, k=k, all_exterior=False))]
+    for V, P in cases:
+        num = P.numbering()
+        assert np.array_equal(V.cell_dofs(), num["cell_dof"])
+        assert np.array_equal(V.cell_raw_dofs(), num["cell_raw"])
+        assert np.array_equal(V.adapted_to_raw(), num["ad2raw"])
+        assert np.array_equal(V.raw_coordinates(), num["raw_xy"], equal_nan=True)
+        for a, b in zip(V.pattern(), P.pattern()):
+            assert np.array_equal(a, b)
+    po = orc.quad_from_arrays(pts, q, k=1)
+    opts, oq, onbr, oseg, omk = orc.mesh_arrays(po)
+    assert np.array_equal(mq.quads, oq) and np.array_equal(mq.neighbours, onbr)
+    assert np.array_equal(mq.segments, oseg) and np.array_equal(mq.segment_markers, omk)
+    n = __import__("ctypes").c_int()
+    xi, eta, w = np.empty(64), np.empty(64), np.empty(64)
+    _lib.check(_lib.load().femd_quadrature_square(7, __import__("ctypes").byref(n), xi, eta, w))
+    for a, b in zip((xi[: n.value], eta[: n.value], w[: n.value]), orc.quad_quadrature(7)):
+        assert np.array_equal(a, b)
+
+
+def test_quad_mesh_errors():
+    with pytest.raises(ValueError):                          # a bow-tie is not convex
+        fd.quad_mesh_from_arrays([[0, 0], [1, 0], [0, 1], [1, 1]], [[0, 1, 2, 3]])
+    with pytest.raises(ValueError):
+        fd.LagrangeSpace(fd.rectangle_quad_mesh(0, 0, 1, 1, 2, 2), 5)     # Q5 is not built
+
+
 def test_mesh_arrays_vs_oracle(orc):
     P, T, _ = problems.delaunay_square(0.15, seed=4)
     T = T.copy()

@@ -323,3 +323,125 @@ def test_example13_heat_known_answer(orc):
     ue = np.exp(-2 * np.pi ** 2 * 0.1) * np.sin(np.pi * xq) * np.sin(np.pi * yq)
     err = math.sqrt(float((wq * (uh - ue) ** 2).sum()))
     assert abs(err - 1.35e-3) < 0.01e-3, err
+
+
+# ------------------------------------------------------------------- Q_k on quadrilaterals
+def _distorted_quads(nx, ny, seed, flip_every=3):
+    """A structured grid with its interior points moved (every quad stays convex) and every
+    flip_every-th quad given clockwise (the mesh re-orients it)."""
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1))
+    pts = np.stack([xs.ravel(), ys.ravel()], 1)
+    inner = (pts[:, 0] > 0) & (pts[:, 0] < 1) & (pts[:, 1] > 0) & (pts[:, 1] < 1)
+    pts[inner] += rng.uniform(-0.25, 0.25, (inner.sum(), 2)) / max(nx, ny)
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    v = (j * (nx + 1) + i).ravel()
+    q = np.stack([v, v + 1, v + nx + 2, v + nx + 1], 1)
+    q[::flip_every] = q[::flip_every, ::-1]
+    return pts, q
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_quad_space_vs_reference(orc, ref, k):
+    pts, q = _distorted_quads(6, 5, k)
+    cases = [(orc.quad_rectangle(5, 4, k, **kw), ref.quad_rectangle(5, 4, k, **kw))
+             for kw in (dict(all_exterior=True), dict(dirichlet=[2, 3], all_exterior=False), dict(all_exterior=False))]
+    cases.append((orc.quad_from_arrays(pts, q, k=k), ref.quad_from_arrays(pts, q, k=k)))
+    for po, pr in cases:
+        assert list(po.sizes()[:6]) == list(pr.sizes())
+        no, nr = po.numbering(), pr.numbering()
+        for key in ("cell_dof", "cell_raw", "ad2raw"):
+            assert np.array_equal(no[key], nr[key]), key
+        assert np.array_equal(no["raw_xy"], nr["raw_xy"])
+        for a, b in zip(po.pattern(), pr.pattern()):
+            assert np.array_equal(a, b)
+        rng = np.random.default_rng(k)
+        for deg in (2 * k, 2 * k + 3):
+            for a, b in zip(po.cache_points(deg), pr.cache_points(deg)):
+                assert np.array_equal(a, b)
+            npt = po.ncells * len(orc.quad_quadrature(deg)[2])
+            c = rng.standard_normal((4, npt))
+            c[1, ::6] = 0.0
+            A, B = [1, 2, 0, 1], [1, 2, 0, 2]
+            assert np.array_equal(po.assemble_matrix(deg, A, B, c), pr.assemble_matrix(deg, A, B, c))
+            assert np.array_equal(po.assemble_vector(deg, [0, 2], c[:2]), pr.assemble_vector(deg, [0, 2], c[:2]))
+    for xi, eta in [(0.3, 0.7), (0.0, 1.0), (0.91, 0.05)]:
+        assert np.array_equal(orc.quad_basis_eval(k, xi, eta), ref.quad_basis_eval(k, xi, eta))
+
+
+def test_quad_rule_exactness(orc, ref):
+    for d in range(0, 16):
+        xi, eta, w = orc.quad_quadrature(d)
+        for a, b in zip((xi, eta, w), ref.quad_quadrature(d)):
+            assert np.array_equal(a, b)
+        g = max(1, (d + 3) // 2)
+        assert len(w) == g * g and abs(w.sum() - 1.0) < 1e-15
+        for a in range(2 * g):                                # exact per direction to degree 2g-1
+            for b in range(2 * g):
+                assert abs((w * xi ** a * eta ** b).sum() - 1.0 / ((a + 1) * (b + 1))) < 1e-14, (d, a, b)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_quad_basis_nodal_and_span(orc, k):
+    t = [j / k for j in range(k + 1)]
+    ij = [(0, 0), (k, 0), (k, k), (0, k)] + [(j, 0) for j in range(1, k)] + [(k, j) for j in range(1, k)] \
+        + [(k - j, k) for j in range(1, k)] + [(0, k - j) for j in range(1, k)] + [(i, j) for j in range(1, k) for i in range(1, k)]
+    nodes = [(t[i], t[j]) for i, j in ij]
+    n = (k + 1) ** 2
+    for l, (xi, eta) in enumerate(nodes):
+        B = orc.quad_basis_eval(k, xi, eta)
+        e = np.zeros(n); e[l] = 1.0
+        assert np.abs(B[0] - e).max() < 1e-12
+    # xi^k eta^k is in the span: the basis reproduces it and its derivatives
+    vals = np.array([x ** k * y ** k for x, y in nodes])
+    for xi, eta in np.random.default_rng(k).random((5, 2)):
+        B = orc.quad_basis_eval(k, xi, eta)
+        assert abs(B[0].sum() - 1) < 1e-12 and abs(B[1].sum()) < 1e-11
+        assert abs(B[0] @ vals - xi ** k * eta ** k) < 1e-11
+        assert abs(B[1] @ vals - k * xi ** (k - 1) * eta ** k) < 1e-10
+
+
+def test_q1_element_matrices(orc):
+    """The Q1 mass matrix of an h1 x h2 rectangle is h1 h2 / 36 [4 2 1 2; 2 4 2 1; 1 2 4 2; 2 1 2 4];
+    its stiffness annihilates constants and has row sums zero."""
+    P = orc.quad_from_arrays([[0.0, 0.0], [2.0, 0.0], [2.0, 0.5], [0.0, 0.5]], [[0, 1, 2, 3]], k=1, all_exterior=False)
+    ip, ix = P.pattern()
+    npt = P.ncells * len(orc.quad_quadrature(2)[2])
+    rows = np.repeat(np.arange(4), np.diff(ip))
+    M = np.zeros((4, 4)); M[rows, ix] = P.assemble_matrix(2, [0], [0], np.ones((1, npt)))
+    ref_m = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) / 36.0
+    assert np.abs(M - 1.0 * ref_m).max() < 1e-15
+    K = np.zeros((4, 4)); K[rows, ix] = P.assemble_matrix(2, [1, 2], [1, 2], np.ones((2, npt)))
+    # Q1 stiffness of a rectangle a x b: (b/6a)[2 -2 -1 1; ...] + (a/6b)[2 1 -1 -2; ...]
+    a, b = 2.0, 0.5
+    Kx = b / (6 * a) * np.array([[2, -2, -1, 1], [-2, 2, 1, -1], [-1, 1, 2, -2], [1, -1, -2, 2]])
+    Ky = a / (6 * b) * np.array([[2, 1, -1, -2], [1, 2, -2, -1], [-1, -2, 2, 1], [-2, -1, 1, 2]])
+    assert np.abs(K - (Kx + Ky)).max() < 1e-14
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_quad_counts_and_patch(orc, k):
+    N = 5
+    P = orc.quad_rectangle(N, N, k)
+    s = P.sizes()
+    assert s[2] == (k * N + 1) ** 2 and s[3] == (k * N - 1) ** 2
+    pts, q = _distorted_quads(N, N, 7)
+    F = orc.quad_from_arrays(pts, q, k=k, all_exterior=False)
+    ip, ix = F.pattern()
+    deg = 2 * k + 2
+    npt = F.ncells * len(orc.quad_quadrature(deg)[2])
+    assert abs(F.assemble_matrix(deg, [0], [0], np.ones((1, npt))).sum() - 1.0) < 1e-13     # area
+    Kv = F.assemble_matrix(deg, [1, 2], [1, 2], np.ones((2, npt)))
+    assert np.abs(orc.spmv(ip, ix, Kv, np.ones(F.dim))).max() < 1e-12
+    # patch test on the distorted mesh: u = 1 + 2x - y is in every Q_k (the map is bilinear), and
+    # the Galerkin solution of -lap u = 0 with u's boundary values reproduces it
+    num = F.numbering()
+    xy = num["raw_xy"][num["ad2raw"]]
+    Pd = orc.quad_from_arrays(pts, q, k=k)
+    bd = np.setdiff1d(np.arange(F.dim), np.searchsorted(num["ad2raw"], Pd.numbering()["ad2raw"]))
+    rows = np.repeat(np.arange(F.dim), np.diff(ip))
+    D = np.zeros((F.dim, F.dim)); D[rows, ix] = Kv
+    free = np.setdiff1d(np.arange(F.dim), bd)
+    ue = 1 + 2 * xy[:, 0] - xy[:, 1]
+    sol = np.linalg.solve(D[np.ix_(free, free)], -D[np.ix_(free, bd)] @ ue[bd])
+    assert np.abs(sol - ue[free]).max() < 1e-11

@@ -852,14 +852,14 @@ int femd_space_create(femd_mesh m, int degree, int ndirichlet, const int *dirich
     return api([&] {
         if (!m) fail(FEMD_EINVAL, "null mesh");
         if (degree < 1) fail(FEMD_EINVAL, "LagrangeSpace2D: degree must be >= 1 (P0 is a DG space)");
-        if (!equispaced && degree >= 3) fail(FEMD_EINVAL, "LagrangeSpace2D: only equispaced nodes are built for degree >= 3");
         const bool quad = m->M.nv == 4;
         if (!quad && degree > 5) fail(FEMD_EINVAL, "LagrangeSpace2D: the GPU kernels cover degrees 1..5");
         if (quad && degree > 4) fail(FEMD_EINVAL, "LagrangeSpaceQ: the GPU kernels cover degrees 1..4 on quadrilaterals");
         auto V = std::make_unique<femd_space_s>();
-        V->V = build_space(m->M, degree, std::vector<int>(dirichlet, dirichlet + std::max(ndirichlet, 0)), all_exterior != 0);
-        if (quad) V->refq = std::make_unique<RefQuad>(degree);
-        else V->ref = std::make_unique<RefElement>(degree);
+        V->V = build_space(m->M, degree, std::vector<int>(dirichlet, dirichlet + std::max(ndirichlet, 0)), all_exterior != 0,
+                           equispaced != 0);
+        if (quad) V->refq = std::make_unique<RefQuad>(degree, equispaced != 0);
+        else V->ref = std::make_unique<RefElement>(degree, equispaced != 0);
         *out = V.release();
     });
 }

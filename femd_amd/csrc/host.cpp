@@ -292,6 +292,93 @@ MeshData quad_mesh_rectangle(double x0, double y0, double x1, double y1, int nx,
 
 // -------------------------------------------------------------------------- reference element
 namespace {
+// Legendre P_n, P_n', P_n'' at x for n = 0..p by the three-term recurrence with derivatives
+// n P_n^(m) = (2n-1)(x P_{n-1}^(m) + m P_{n-1}^(m-1)) - (n-1) P_{n-2}^(m) (fe/dg_space.hpp:38-58),
+// in that evaluation order so the Newton iterates below carry the reference's bits.
+void legendre_d2(int p, double x, double *rows)
+{
+    const int nl = p + 1;
+    for (int m = 0; m < 3; ++m)
+    {
+        double *R = rows + m * nl;
+        const double *D = m > 0 ? rows + (m - 1) * nl : nullptr;
+        R[0] = m == 0 ? 1.0 : 0.0;
+        for (int n = 1; n <= p; ++n)
+        {
+            const double lower = D ? D[n - 1] : 0.0, prev2 = n >= 2 ? R[n - 2] : 0.0;
+            R[n] = ((2 * n - 1) * (x * R[n - 1] + m * lower) - (n - 1) * prev2) / n;
+        }
+    }
+}
+
+// The p+1 Gauss-Lobatto points on [-1, 1]: the ends and the roots of P_p', each by Newton
+// (step P_p'/P_p'') from the Chebyshev-Lobatto guess -cos(pi i / p), at most 100 steps, stopping
+// after a step below 1e-16 (fe/dg_space.hpp:60-80).
+std::vector<double> lobatto(int p)
+{
+    std::vector<double> x(static_cast<std::size_t>(p) + 1), rows(3 * (static_cast<std::size_t>(p) + 1));
+    x[0] = -1.0;
+    x[p] = 1.0;
+    for (int i = 1; i < p; ++i)
+    {
+        double z = -std::cos(3.14159265358979323846 * i / p);
+        for (int it = 0; it < 100; ++it)
+        {
+            legendre_d2(p, z, rows.data());
+            const double step = rows[(p + 1) + p] / rows[2 * (p + 1) + p];
+            z -= step;
+            if (std::fabs(step) < 1e-16) break;
+        }
+        x[i] = z;
+    }
+    return x;
+}
+
+// Hesthaven's 1D warp at r: the Lobatto-minus-uniform node shift interpolated on the uniform
+// nodes, divided by 1 - r^2, zero within 1e-10 of the ends (simplex_basis.hpp:150-166).
+double warp(int N, double r, const std::vector<double> &lgl)
+{
+    double w = 0.0;
+    for (int i = 0; i <= N; ++i)
+    {
+        const double ri = -1.0 + 2.0 * i / N;
+        double ell = 1.0;
+        for (int j = 0; j <= N; ++j)
+            if (j != i) ell *= (r - (-1.0 + 2.0 * j / N)) / (ri - (-1.0 + 2.0 * j / N));
+        w += ell * (lgl[i] - ri);
+    }
+    if (std::abs(r) < 1.0 - 1e-10) return w / (1.0 - r * r);
+    return 0.0;
+}
+
+// Warburton's warp-and-blend nodes of degree N (simplex_basis.hpp:168-193): the lattice on the
+// equilateral triangle, each edge's warp blended inwards with the per-degree optimum alpha (5/3
+// from degree 16), then mapped to the reference triangle (xi, eta).
+void warp_blend(int N, std::vector<double> &xs, std::vector<double> &ys)
+{
+    static const double alpha_opt[15] = {0.0000, 0.0000, 1.4152, 0.1001, 0.2751, 0.9800, 1.0999, 1.2832,
+                                         1.3648, 1.4773, 1.4959, 1.5743, 1.5770, 1.6223, 1.6258};
+    const double al = N < 16 ? alpha_opt[N - 1] : 5.0 / 3.0;
+    const std::vector<double> lgl = lobatto(N);
+    const double pi = 3.14159265358979323846, s3 = std::sqrt(3.0);
+    const double c2 = std::cos(2.0 * pi / 3.0), c4 = std::cos(4.0 * pi / 3.0);
+    const double s2 = std::sin(2.0 * pi / 3.0), s4 = std::sin(4.0 * pi / 3.0);
+    for (int n = 0; n <= N; ++n)
+        for (int m = 0; m <= N - n; ++m)
+        {
+            const double L1 = static_cast<double>(n) / N, L3 = static_cast<double>(m) / N, L2 = 1.0 - L1 - L3;
+            double x = -L2 + L3, y = (-L2 - L3 + 2.0 * L1) / s3;
+            const double w1 = 4.0 * L2 * L3 * warp(N, L3 - L2, lgl) * (1.0 + (al * L1) * (al * L1));
+            const double w2 = 4.0 * L1 * L3 * warp(N, L1 - L3, lgl) * (1.0 + (al * L2) * (al * L2));
+            const double w3 = 4.0 * L1 * L2 * warp(N, L2 - L1, lgl) * (1.0 + (al * L3) * (al * L3));
+            x += w1 + c2 * w2 + c4 * w3;
+            y += s2 * w2 + s4 * w3;
+            const double l1 = (s3 * y + 1.0) / 3.0, l2 = (-3.0 * x - s3 * y + 2.0) / 6.0, l3 = (3.0 * x - s3 * y + 2.0) / 6.0;
+            xs.push_back(0.5 * ((-l2 + l3 - l1) + 1.0));
+            ys.push_back(0.5 * ((-l2 - l3 + l1) + 1.0));
+        }
+}
+
 // Orthonormal Jacobi P_0..P_N^{(a,b)} (simplex_basis.hpp:38-55, Hesthaven's JacobiP).
 void jacobi_table(double x, double al, double be, int N, double *P)
 {
@@ -353,13 +440,16 @@ void modes(int k, double xi, double eta, double *v, double *dx, double *dy)
 }
 } // namespace
 
-RefElement::RefElement(int degree) : k(degree), n((degree + 1) * (degree + 2) / 2)
+RefElement::RefElement(int degree, bool equispaced) : k(degree), n((degree + 1) * (degree + 2) / 2)
 {
     if (k < 1 || k > 20) bad("LagrangeSpace: degree must be in 1..20");
     static const double vx[3] = {0.0, 1.0, 0.0}, vy[3] = {0.0, 0.0, 1.0};
     nxi.assign(vx, vx + 3); neta.assign(vy, vy + 3);
     std::vector<double> z(static_cast<std::size_t>(k) + 1);
-    for (int j = 0; j <= k; ++j) z[j] = -1.0 + 2.0 * j / k;
+    if (equispaced)
+        for (int j = 0; j <= k; ++j) z[j] = -1.0 + 2.0 * j / k;
+    else
+        z = lobatto(k);
     for (int j = 0; j <= k; ++j) z[j] = 0.5 * (z[j] - z[k - j]);          // symmetrized in place, as the reference does
     for (int e = 0; e < 3; ++e)
         for (int j = 1; j < k; ++j)
@@ -368,8 +458,20 @@ RefElement::RefElement(int degree) : k(degree), n((degree + 1) * (degree + 2) / 
             nxi.push_back((1.0 - t) * vx[(e + 1) % 3] + t * vx[(e + 2) % 3]);
             neta.push_back((1.0 - t) * vy[(e + 1) % 3] + t * vy[(e + 2) % 3]);
         }
-    for (int j = 1; j < k; ++j)
-        for (int i = 1; i + j < k; ++i) { nxi.push_back(static_cast<double>(i) / k); neta.push_back(static_cast<double>(j) / k); }
+    if (equispaced)
+    {
+        for (int j = 1; j < k; ++j)
+            for (int i = 1; i + j < k; ++i) { nxi.push_back(static_cast<double>(i) / k); neta.push_back(static_cast<double>(j) / k); }
+    }
+    else if (k >= 3)
+    {
+        // the warp-and-blend points strictly inside (simplex_basis.hpp:274-282)
+        std::vector<double> px, py;
+        warp_blend(k, px, py);
+        for (std::size_t p = 0; p < px.size(); ++p)
+            if (px[p] > 1e-10 && py[p] > 1e-10 && 1.0 - px[p] - py[p] > 1e-10) { nxi.push_back(px[p]); neta.push_back(py[p]); }
+    }
+    if (static_cast<int>(nxi.size()) != n) bad("reference element: wrong node count");
     // Vandermonde V_im = psi_m(node_i), inverted by Gauss-Jordan with partial pivoting
     std::vector<double> A(static_cast<std::size_t>(n) * n), X(static_cast<std::size_t>(n) * n, 0.0);
     for (int i = 0; i < n; ++i) modes(k, nxi[i], neta[i], &A[static_cast<std::size_t>(i) * n], nullptr, nullptr);
@@ -494,10 +596,14 @@ QuadRule triangle_rule(int d)
     return Q;
 }
 
-RefQuad::RefQuad(int degree) : k(degree), n((degree + 1) * (degree + 1))
+RefQuad::RefQuad(int degree, bool equispaced) : k(degree), n((degree + 1) * (degree + 1))
 {
+    if (k < 1 || k > 20) bad("LagrangeSpaceQ: degree must be in 1..20");
     std::vector<double> z;
-    for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+    if (equispaced)
+        for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+    else
+        z = lobatto(k);
     t.resize(static_cast<std::size_t>(k) + 1);
     for (int j = 0; j <= k; ++j) t[j] = 0.5 * (1.0 + 0.5 * (z[j] - z[k - j]));
     t[0] = 0.0;
@@ -560,15 +666,15 @@ QuadRule square_rule(int d)
 }
 
 // ---------------------------------------------------------------------------------- spaces
-SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior)
+SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior, bool equispaced)
 {
     std::sort(dirichlet.begin(), dirichlet.end());
     dirichlet.erase(std::unique(dirichlet.begin(), dirichlet.end()), dirichlet.end());
     const int nv = M.nv;
     std::unique_ptr<RefElement> RT;
     std::unique_ptr<RefQuad> RQ;
-    if (nv == 4) RQ = std::make_unique<RefQuad>(k);
-    else RT = std::make_unique<RefElement>(k);
+    if (nv == 4) RQ = std::make_unique<RefQuad>(k, equispaced);
+    else RT = std::make_unique<RefElement>(k, equispaced);
     const std::vector<double> &nxi = nv == 4 ? RQ->nxi : RT->nxi, &neta = nv == 4 ? RQ->neta : RT->neta;
     SpaceData V;
     const int nc = M.nt(), np = M.np(), n = nv == 4 ? RQ->n : RT->n, de = k - 1;

@@ -51,22 +51,24 @@ MeshData quad_mesh_from_arrays(std::vector<double> xy, std::vector<int> quads, c
 MeshData quad_mesh_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, const int markers[4]);
 
 // The nodal P_k basis on the reference triangle (fe/simplex_basis.hpp: Dubiner modes times the
-// inverse Vandermonde matrix, equispaced nodes, local order vertices / edges / interior).
+// inverse Vandermonde matrix; equispaced nodes, or symmetrized Gauss-Lobatto edge nodes and
+// warp-and-blend interior nodes; local order vertices / edges / interior).
 struct RefElement {
     int k = 1, n = 3;
     std::vector<double> nxi, neta, vinv;
-    explicit RefElement(int degree);
+    explicit RefElement(int degree, bool equispaced = true);
     void tabulate(double xi, double eta, double *out3n) const;     // values, d/dxi, d/deta
 };
 
 // The nodal Q_k basis on the reference square [0,1]^2 (fe/lagrange_space_quad.hpp:28-110): 1D
-// Lagrange polynomials in barycentric form on k+1 symmetrized equispaced points, tensor products;
+// Lagrange polynomials in barycentric form on k+1 symmetrized equispaced or Gauss-Lobatto points,
+// tensor products;
 // local order vertices, the nodes of each edge along it, interior row by row.
 struct RefQuad {
     int k = 1, n = 4;
     std::vector<double> t, w, nxi, neta;
     std::vector<int> ii, jj;
-    explicit RefQuad(int degree);
+    explicit RefQuad(int degree, bool equispaced = true);
     void tabulate(double xi, double eta, double *out3n) const;     // values, d/dxi, d/deta
 };
 
@@ -88,7 +90,7 @@ struct SpaceData {
     std::vector<double> geom;
     int geom_stride() const { return nv == 4 ? 8 : 6; }
 };
-SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior);
+SpaceData build_space(const MeshData &M, int k, std::vector<int> dirichlet, bool all_exterior, bool equispaced = true);
 
 // The pattern of a space with itself and, for every stored entry, the element-matrix entries
 // that add into it in ascending element order (the order assemble_matrix_2d adds them,

@@ -86,6 +86,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--h", type=float, default=None)
     ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--shape", default=None,
+                    help="pin the SpMV (and inner CG product) launch shape in both precisions, e.g. tile256x2048, "
+                         "so that separate profiler runs measure the same kernels")
     ap.add_argument("--cg-tol", type=float, default=None,
                     help="also run one mixed-precision solve to this relative residual (the inner CG product "
                          "kernel, launched without graphs so that every launch is visible to a profiler)")
@@ -95,6 +98,9 @@ def main():
     L.check(lib.femd_init(0))
     V, a, Lf, info = problems.p3_kappa_problem(args.h or problems.C1_H)
     A = a.assemble()
+    if args.shape:
+        for prec in (L.FP64, L.FP32):
+            L.check(lib.femd_spmv_select(A._h, prec, args.shape.encode()))
     ip, ix = V.pattern()
     n, nnz = V.dim, int(ix.size)
     t64, t32 = spmv_times(A, args.reps)

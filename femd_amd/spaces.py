@@ -30,7 +30,10 @@ class LagrangeSpace2D:
 
     bc:        "free" (default) or "dirichlet" (every exterior side eliminated).
     dirichlet: a marker or list of markers whose sides are eliminated; combines with bc.
-    nodes:     "equispaced" (default).  For k <= 2 "lobatto" gives the same nodes."""
+    nodes:     "equispaced" (default) or "lobatto": Warburton's warp-and-blend points on triangles,
+               Gauss-Lobatto tensor points on quadrilaterals (fe/simplex_basis.hpp:150-285,
+               lagrange_space_quad.hpp:31-50); the same space with another basis, identical to the
+               equispaced one, to rounding, for k <= 2."""
 
     family = "lagrange"
     tdim = 2
@@ -56,11 +59,10 @@ class LagrangeSpace2D:
         nd = str(nodes).lower()
         if nd not in ("equispaced", "lobatto"):
             raise ValueError(f"LagrangeSpace2D: nodes is 'equispaced' or 'lobatto', got {nodes!r}")
-        if nd == "lobatto" and k >= 3:
-            raise NotImplementedError("LagrangeSpace2D: warp-and-blend (lobatto) nodes are not built for k >= 3")
         h = C.c_void_p()
         d = L.i32(marks) if marks else np.zeros(1, dtype=np.int32)
-        L.check(L.load().femd_space_create(mesh._h, k, len(marks), d, int(b == "dirichlet"), 1, C.byref(h)))
+        L.check(L.load().femd_space_create(mesh._h, k, len(marks), d, int(b == "dirichlet"),
+                                           int(nd == "equispaced"), C.byref(h)))
         self._h = h.value
         self.mesh = mesh
         self.nverts = mesh.nverts

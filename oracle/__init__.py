@@ -62,6 +62,18 @@ class _Lib:
         f.restype = C.c_char_p
         return f().decode()
 
+    def ref_nodes(self, k, equispaced=True, quad=False):
+        """(xi, eta) of the local nodes of the P_k (or, quad=True, Q_k) reference element."""
+        n = (k + 1) ** 2 if quad else (k + 1) * (k + 2) // 2
+        xi, eta = np.empty(n), np.empty(n)
+        self._check(self._nodes(int(k), int(equispaced), int(quad), xi, eta))
+        return xi, eta
+
+    def lobatto_points(self, p):
+        out = np.empty(p + 1)
+        self._check(self._lgl(int(p), out))
+        return out
+
     # -- shared CSR / Krylov entry points (same signatures in both libraries) ------------
     def _declare(self):
         self._spmv = self._fn("spmv", _I, _I, _ip, _ip, _dp, _dp, _dp)
@@ -168,40 +180,43 @@ class Oracle(_Lib):
 
     def __init__(self, path=ORACLE_SO):
         super().__init__(path)
-        self._prect = self._fn("problem_rectangle", _P, _D, _D, _D, _D, _I, _I, _I, _I, _I, _ip, _I)
-        self._parr = self._fn("problem_arrays", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I)
+        self._prect = self._fn("problem_rectangle", _P, _D, _D, _D, _D, _I, _I, _I, _I, _I, _ip, _I, _I)
+        self._parr = self._fn("problem_arrays", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I, _I)
         self._pfree = self._fn("problem_free", None, _P)
         self._psizes = self._fn("problem_sizes", _I, _P, _ip)
         self._pnum = self._fn("numbering", _I, _P, _ip, _ip, _ip, _dp)
         self._pnnz = self._fn("pattern_nnz", _L, _P)
         self._ppat = self._fn("pattern", _I, _P, _ip, _ip)
         self._quad = self._fn("quadrature", _I, _I, C.POINTER(_I), _dp, _dp, _dp)
-        self._basis = self._fn("basis_eval", _I, _I, _D, _D, _dp)
+        self._basis = self._fn("basis_eval", _I, _I, _I, _D, _D, _dp)
         self._cpts = self._fn("cache_points", _I, _P, _I, _dp, _dp, _dp)
         self._amat = self._fn("assemble_matrix", _I, _P, _I, _I, _ip, _ip, _dp, _dp)
         self._avec = self._fn("assemble_vector", _I, _P, _I, _I, _ip, _dp, _dp)
         self._ie = self._fn("integrand_eval", _I, _I, _ip, _I, _dp, _L, _dp, _dp, _dp)
         self._ilu = self._fn("ilu0_factor", _I, _I, _ip, _ip, _dp, _I, _ip, _ip, _ip, _dp, C.POINTER(_D))
         self._marr = self._fn("mesh_arrays", _I, _P, _dp, _ip, _ip, _ip, _ip)
-        self._pqrect = self._fn("problem_quad_rectangle", _P, _D, _D, _D, _D, _I, _I, _I, _I, _ip, _I)
-        self._pqarr = self._fn("problem_quad_arrays", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I)
+        self._pqrect = self._fn("problem_quad_rectangle", _P, _D, _D, _D, _D, _I, _I, _I, _I, _ip, _I, _I)
+        self._pqarr = self._fn("problem_quad_arrays", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I, _I)
         self._qquad = self._fn("quadrature_quad", _I, _I, C.POINTER(_I), _dp, _dp, _dp)
-        self._qbasis = self._fn("basis_eval_quad", _I, _I, _D, _D, _dp)
+        self._qbasis = self._fn("basis_eval_quad", _I, _I, _I, _D, _D, _dp)
+        self._nodes = self._fn("ref_nodes", _I, _I, _I, _I, _dp, _dp)
+        self._lgl = self._fn("lobatto_points", _I, _I, _dp)
 
-    def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+    def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0, equispaced=True):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
-        h = self._pqrect(x0, y0, x1, y1, nx, ny, k, len(dirichlet), d, int(all_exterior))
+        h = self._pqrect(x0, y0, x1, y1, nx, ny, k, len(dirichlet), d, int(all_exterior), int(equispaced))
         if not h:
             raise RuntimeError(self._err())
         return OracleProblem(self, h, self._pfree, quad=True)
 
-    def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True):
+    def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True,
+                         equispaced=True):
         pts, q = _f64(points).reshape(-1), _i32(quads).reshape(-1)
         segs = _i32(segments).reshape(-1) if segments is not None else np.zeros(0, dtype=np.int32)
         mk = _i32(markers) if markers is not None else None
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
         h = self._pqarr(pts.size // 2, pts, q.size // 4, q, segs.size // 2, segs, mk.ctypes.data if mk is not None else None,
-                        k, len(dirichlet), d, int(all_exterior))
+                        k, len(dirichlet), d, int(all_exterior), int(equispaced))
         if not h:
             raise RuntimeError(self._err())
         return OracleProblem(self, h, self._pfree, quad=True)
@@ -212,26 +227,28 @@ class Oracle(_Lib):
         self._check(self._qquad(int(degree), C.byref(nq), xi, eta, w))
         return xi[: nq.value].copy(), eta[: nq.value].copy(), w[: nq.value].copy()
 
-    def quad_basis_eval(self, k, xi, eta):
+    def quad_basis_eval(self, k, xi, eta, equispaced=True):
         n = (k + 1) ** 2
         out = np.empty(3 * n)
-        self._check(self._qbasis(int(k), float(xi), float(eta), out))
+        self._check(self._qbasis(int(k), int(equispaced), float(xi), float(eta), out))
         return out.reshape(3, n)
 
-    def rectangle(self, nx, ny, k, diagonal="right", dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+    def rectangle(self, nx, ny, k, diagonal="right", dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0,
+                  equispaced=True):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
-        h = self._prect(x0, y0, x1, y1, nx, ny, DIAG[diagonal], k, len(dirichlet), d, int(all_exterior))
+        h = self._prect(x0, y0, x1, y1, nx, ny, DIAG[diagonal], k, len(dirichlet), d, int(all_exterior), int(equispaced))
         if not h:
             raise RuntimeError(self._err())
         return OracleProblem(self, h, self._pfree)
 
-    def from_arrays(self, points, triangles, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True):
+    def from_arrays(self, points, triangles, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True,
+                    equispaced=True):
         pts, tris = _f64(points).reshape(-1), _i32(triangles).reshape(-1)
         segs = _i32(segments).reshape(-1) if segments is not None else np.zeros(0, dtype=np.int32)
         mk = _i32(markers) if markers is not None else None
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
         h = self._parr(pts.size // 2, pts, tris.size // 3, tris, segs.size // 2, segs,
-                       mk.ctypes.data if mk is not None else None, k, len(dirichlet), d, int(all_exterior))
+                       mk.ctypes.data if mk is not None else None, k, len(dirichlet), d, int(all_exterior), int(equispaced))
         if not h:
             raise RuntimeError(self._err())
         return OracleProblem(self, h, self._pfree)
@@ -269,10 +286,10 @@ class Oracle(_Lib):
         self._check(self._quad(int(degree), C.byref(nq), xi, eta, w))
         return xi[: nq.value].copy(), eta[: nq.value].copy(), w[: nq.value].copy()
 
-    def basis_eval(self, k, xi, eta):
+    def basis_eval(self, k, xi, eta, equispaced=True):
         n = (k + 1) * (k + 2) // 2
         out = np.empty(3 * n)
-        self._check(self._basis(int(k), float(xi), float(eta), out))
+        self._check(self._basis(int(k), int(equispaced), float(xi), float(eta), out))
         return out.reshape(3, n)
 
     def _npts(self, P, degree):
@@ -336,23 +353,26 @@ class Ref(_Lib):
         self._avec = self._fn("assemble_vector", _I, _P, _I, _I, _ip, _dp, _dp)
         self._ie = self._fn("integrand_eval", _I, _I, _ip, _I, _I, _dp, _L, _dp, _dp, _dp)
         self._asym = self._fn("asymmetry", _D, _I, _ip, _ip, _dp)
-        self._qcreate = self._fn("quad_space_create", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I)
+        self._qcreate = self._fn("quad_space_create", _P, _I, _dp, _I, _ip, _I, _ip, C.c_void_p, _I, _I, _ip, _I, _I)
         self._qquads = self._fn("quad_mesh_quads", _I, _P, _ip)
         self._qquad = self._fn("quad_quadrature", _I, _I, C.POINTER(_I), _dp, _dp, _dp)
-        self._qbasis = self._fn("quad_basis_eval", _I, _I, _D, _D, _dp)
+        self._qbasis = self._fn("quad_basis_eval", _I, _I, _I, _D, _D, _dp)
+        self._nodes = self._fn("nodes", _I, _I, _I, _I, _dp, _dp)
+        self._lgl = self._fn("lobatto_points", _I, _I, _dp)
 
-    def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True):
+    def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True,
+                         equispaced=True):
         pts, q = _f64(points).reshape(-1), _i32(quads).reshape(-1)
         segs = _i32(segments).reshape(-1) if segments is not None else np.zeros(0, dtype=np.int32)
         mk = _i32(markers) if markers is not None else None
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
         h = self._qcreate(pts.size // 2, pts, q.size // 4, q, segs.size // 2, segs, mk.ctypes.data if mk is not None else None,
-                          int(k), len(dirichlet), d, int(all_exterior))
+                          int(k), len(dirichlet), d, int(all_exterior), int(equispaced))
         if not h:
             raise RuntimeError(self._err())
         return OracleProblem(self, h, self._sfree, quad=True)
 
-    def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+    def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0, equispaced=True):
         """rectangle_quad_mesh (python/femd/quadmesh.py:146-170): the grid points and the sides of
         the reference's own rectangle_mesh, quad (i, j) with vertices (i,j), (i+1,j), (i+1,j+1), (i,j+1)."""
         m = self._mrect(x0, y0, x1, y1, nx, ny, b"right")
@@ -367,7 +387,8 @@ class Ref(_Lib):
         j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
         v = (j * (nx + 1) + i).ravel()
         quads = np.stack([v, v + 1, v + nx + 2, v + nx + 1], axis=1)
-        return self.quad_from_arrays(pts[: 2 * (nx + 1) * (ny + 1)], quads, segs, mk[: s[2]], k, dirichlet, all_exterior)
+        return self.quad_from_arrays(pts[: 2 * (nx + 1) * (ny + 1)], quads, segs, mk[: s[2]], k, dirichlet, all_exterior,
+                                     equispaced)
 
     def quad_quadrature(self, degree):
         xi, eta, w = np.empty(4096), np.empty(4096), np.empty(4096)
@@ -375,10 +396,10 @@ class Ref(_Lib):
         self._check(self._qquad(int(degree), C.byref(nq), xi, eta, w))
         return xi[: nq.value].copy(), eta[: nq.value].copy(), w[: nq.value].copy()
 
-    def quad_basis_eval(self, k, xi, eta):
+    def quad_basis_eval(self, k, xi, eta, equispaced=True):
         n = (k + 1) ** 2
         out = np.empty(3 * n)
-        self._check(self._qbasis(int(k), float(xi), float(eta), out))
+        self._check(self._qbasis(int(k), int(equispaced), float(xi), float(eta), out))
         return out.reshape(3, n)
 
     def quad_mesh_quads(self, P):
@@ -386,28 +407,30 @@ class Ref(_Lib):
         self._check(self._qquads(P.h, out))
         return out.reshape(-1, 4)
 
-    def _space(self, mesh, k, dirichlet, all_exterior):
+    def _space(self, mesh, k, dirichlet, all_exterior, equispaced=True):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
-        h = self._screate(mesh, int(k), len(dirichlet), d, int(all_exterior), 1)
+        h = self._screate(mesh, int(k), len(dirichlet), d, int(all_exterior), int(equispaced))
         self._mfree(mesh)
         if not h:
             raise RuntimeError(self._err())
         return OracleProblem(self, h, self._sfree)
 
-    def rectangle(self, nx, ny, k, diagonal="right", dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+    def rectangle(self, nx, ny, k, diagonal="right", dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0,
+                  equispaced=True):
         m = self._mrect(x0, y0, x1, y1, nx, ny, diagonal.encode())
         if not m:
             raise RuntimeError(self._err())
-        return self._space(m, k, dirichlet, all_exterior)
+        return self._space(m, k, dirichlet, all_exterior, equispaced)
 
-    def from_arrays(self, points, triangles, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True):
+    def from_arrays(self, points, triangles, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True,
+                    equispaced=True):
         pts, tris = _f64(points).reshape(-1), _i32(triangles).reshape(-1)
         segs = _i32(segments).reshape(-1) if segments is not None else np.zeros(0, dtype=np.int32)
         mk = _i32(markers) if markers is not None else None
         m = self._marr(pts.size // 2, pts, tris.size // 3, tris, segs.size // 2, segs, mk.ctypes.data if mk is not None else None)
         if not m:
             raise RuntimeError(self._err())
-        return self._space(m, k, dirichlet, all_exterior)
+        return self._space(m, k, dirichlet, all_exterior, equispaced)
 
     def _sizes(self, h):
         out = np.zeros(6, dtype=np.int32)
@@ -430,10 +453,10 @@ class Ref(_Lib):
     _assemble_matrix = Oracle._assemble_matrix
     _assemble_vector = Oracle._assemble_vector
 
-    def basis_eval(self, k, xi, eta):
+    def basis_eval(self, k, xi, eta, equispaced=True):
         n = (k + 1) * (k + 2) // 2
         out = np.empty(3 * n)
-        self._check(self._basis(int(k), 1, float(xi), float(eta), out))
+        self._check(self._basis(int(k), int(equispaced), float(xi), float(eta), out))
         return out.reshape(3, n)
 
     def integrand_eval(self, code, nreg, consts, xs, ys):

@@ -275,7 +275,7 @@ Mesh quad_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, 
 
 // ------------------------------------------------------------------------ reference element
 // fe/simplex_basis.hpp:38-55 (orthonormal Jacobi), 77-114 (Dubiner modes), 117-148 (inverse),
-// 202-296 (ReferenceTriangle, equispaced nodes).
+// 202-296 (ReferenceTriangle; equispaced or warp-and-blend nodes).
 void jacobi_p(double x, double al, double be, int N, double *P)
 {
     const double ab = al + be;
@@ -363,15 +363,98 @@ std::vector<double> gauss_jordan_inverse(std::vector<double> A, int n)
     return X;
 }
 
+// fe/dg_space.hpp:38-58: P_n^(m)(x), n = 0..p, m = 0..nder, by
+// n P_n^(m) = (2n-1)(x P_{n-1}^(m) + m P_{n-1}^(m-1)) - (n-1) P_{n-2}^(m); out[m (p+1) + n].
+void legendre_rows(int p, int nder, double x, double *out)
+{
+    const int nl = p + 1;
+    for (int m = 0; m <= nder; ++m)
+        for (int n = 0; n <= p; ++n)
+        {
+            if (n == 0) { out[m * nl] = m == 0 ? 1.0 : 0.0; continue; }
+            const double a = out[m * nl + n - 1];
+            const double b = m > 0 ? out[(m - 1) * nl + n - 1] : 0.0;
+            const double c = n >= 2 ? out[m * nl + n - 2] : 0.0;
+            out[m * nl + n] = ((2 * n - 1) * (x * a + m * b) - (n - 1) * c) / n;
+        }
+}
+
+// fe/dg_space.hpp:60-80: the p+1 Gauss-Lobatto points on [-1, 1], the interior ones by Newton on
+// P_p' / P_p'' from -cos(pi i / p), at most 100 steps, stopping once a step is below 1e-16.
+std::vector<double> lobatto_points(int p)
+{
+    std::vector<double> x(static_cast<std::size_t>(p) + 1), t(3 * (static_cast<std::size_t>(p) + 1));
+    x[0] = -1.0; x[p] = 1.0;
+    for (int i = 1; i < p; ++i)
+    {
+        double z = -std::cos(3.14159265358979323846 * i / p);
+        for (int it = 0; it < 100; ++it)
+        {
+            legendre_rows(p, 2, z, t.data());
+            const double d = t[(p + 1) + p] / t[2 * (p + 1) + p];
+            z -= d;
+            if (std::fabs(d) < 1e-16) break;
+        }
+        x[i] = z;
+    }
+    return x;
+}
+
+// simplex_basis.hpp:150-166: Hesthaven's warp, the Lobatto-minus-uniform displacement
+// interpolated at r (uniform nodes) and divided by 1 - r^2 (zero at the ends).
+double warp_of(int N, double r, const std::vector<double> &lgl)
+{
+    double w = 0.0;
+    for (int i = 0; i <= N; ++i)
+    {
+        const double ri = -1.0 + 2.0 * i / N;
+        double L = 1.0;
+        for (int j = 0; j <= N; ++j)
+            if (j != i) L *= (r - (-1.0 + 2.0 * j / N)) / (ri - (-1.0 + 2.0 * j / N));
+        w += L * (lgl[i] - ri);
+    }
+    return std::abs(r) < 1.0 - 1e-10 ? w / (1.0 - r * r) : 0.0;
+}
+
+// simplex_basis.hpp:168-193: Warburton's warp-and-blend points of degree N on the equilateral
+// triangle (blend optimum alpha per degree, 5/3 above 15), mapped back to (xi, eta), lattice order.
+void warp_blend_points(int N, std::vector<double> &px, std::vector<double> &py)
+{
+    static const double aopt[15] = {0.0, 0.0, 1.4152, 0.1001, 0.2751, 0.9800, 1.0999, 1.2832,
+                                    1.3648, 1.4773, 1.4959, 1.5743, 1.5770, 1.6223, 1.6258};
+    const double al = N < 16 ? aopt[N - 1] : 5.0 / 3.0;
+    const std::vector<double> lgl = lobatto_points(N);
+    const double pi = 3.14159265358979323846, s3 = std::sqrt(3.0);
+    for (int n = 0; n <= N; ++n)
+        for (int m = 0; m <= N - n; ++m)
+        {
+            const double L1 = static_cast<double>(n) / N, L3 = static_cast<double>(m) / N, L2 = 1.0 - L1 - L3;
+            double x = -L2 + L3, y = (-L2 - L3 + 2.0 * L1) / s3;
+            const double w1 = 4.0 * L2 * L3 * warp_of(N, L3 - L2, lgl) * (1.0 + (al * L1) * (al * L1));
+            const double w2 = 4.0 * L1 * L3 * warp_of(N, L1 - L3, lgl) * (1.0 + (al * L2) * (al * L2));
+            const double w3 = 4.0 * L1 * L2 * warp_of(N, L2 - L1, lgl) * (1.0 + (al * L3) * (al * L3));
+            x += w1 + std::cos(2.0 * pi / 3.0) * w2 + std::cos(4.0 * pi / 3.0) * w3;
+            y += std::sin(2.0 * pi / 3.0) * w2 + std::sin(4.0 * pi / 3.0) * w3;
+            const double l1 = (s3 * y + 1.0) / 3.0, l2 = (-3.0 * x - s3 * y + 2.0) / 6.0, l3 = (3.0 * x - s3 * y + 2.0) / 6.0;
+            const double r = -l2 + l3 - l1, s = -l2 - l3 + l1;
+            px.push_back(0.5 * (r + 1.0));
+            py.push_back(0.5 * (s + 1.0));
+        }
+}
+
+// simplex_basis.hpp:253-285: equispaced nodes, or (lobatto) symmetrized Gauss-Lobatto edge nodes
+// and the warp-and-blend points strictly inside the triangle.
 struct RefElement {
     int k = 1, n = 3;
+    bool equi = true;
     std::vector<double> nx, ny, Vinv;
-    explicit RefElement(int deg) : k(deg), n((deg + 1) * (deg + 2) / 2)
+    explicit RefElement(int deg, bool equispaced = true) : k(deg), n((deg + 1) * (deg + 2) / 2), equi(equispaced)
     {
         const double vx[3] = {0.0, 1.0, 0.0}, vy[3] = {0.0, 0.0, 1.0};
         for (int v = 0; v < 3; ++v) { nx.push_back(vx[v]); ny.push_back(vy[v]); }
         std::vector<double> z;
-        for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+        if (equi) for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+        else z = lobatto_points(k);
         for (int j = 0; j <= k; ++j) z[j] = 0.5 * (z[j] - z[k - j]);     // simplex_basis.hpp:259, symmetrized
         for (int e = 0; e < 3; ++e)
         {
@@ -383,8 +466,19 @@ struct RefElement {
                 ny.push_back((1.0 - t) * vy[a] + t * vy[b]);
             }
         }
-        for (int j = 1; j < k; ++j)
-            for (int i = 1; i + j < k; ++i) { nx.push_back(static_cast<double>(i) / k); ny.push_back(static_cast<double>(j) / k); }
+        if (equi)
+        {
+            for (int j = 1; j < k; ++j)
+                for (int i = 1; i + j < k; ++i) { nx.push_back(static_cast<double>(i) / k); ny.push_back(static_cast<double>(j) / k); }
+        }
+        else if (k >= 3)
+        {
+            std::vector<double> px, py;
+            warp_blend_points(k, px, py);
+            for (std::size_t p = 0; p < px.size(); ++p)
+                if (px[p] > 1e-10 && py[p] > 1e-10 && 1.0 - px[p] - py[p] > 1e-10) { nx.push_back(px[p]); ny.push_back(py[p]); }
+        }
+        if (static_cast<int>(nx.size()) != n) throw std::logic_error("reference element: node count mismatch");
         std::vector<double> V(static_cast<std::size_t>(n) * n), psi(static_cast<std::size_t>(n));
         for (int i = 0; i < n; ++i)
         {
@@ -413,17 +507,18 @@ struct RefElement {
     }
 };
 
-// fe/lagrange_space_quad.hpp:28-110 (ReferenceQuad, equispaced nodes): the tensor product of
+// fe/lagrange_space_quad.hpp:28-110 (ReferenceQuad, equispaced or Gauss-Lobatto nodes): the tensor product of
 // 1D Lagrange polynomials in barycentric form on k+1 symmetrized points; local order vertices,
 // edge nodes along each edge, interior row by row.
 struct RefQuad {
     int k = 1, n = 4;
     std::vector<double> t, w, nx, ny;
     std::vector<int> ii, jj;
-    explicit RefQuad(int deg) : k(deg), n((deg + 1) * (deg + 1))
+    explicit RefQuad(int deg, bool equispaced = true) : k(deg), n((deg + 1) * (deg + 1))
     {
         std::vector<double> z;
-        for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+        if (equispaced) for (int j = 0; j <= k; ++j) z.push_back(-1.0 + 2.0 * j / k);
+        else z = lobatto_points(k);
         t.resize(static_cast<std::size_t>(k) + 1);
         for (int j = 0; j <= k; ++j) t[j] = 0.5 * (1.0 + 0.5 * (z[j] - z[k - j]));
         t[0] = 0.0; t[k] = 1.0;
@@ -595,7 +690,7 @@ void quad_map(const Mesh &M, int c, double xi, double eta, double &x, double &y,
     }
 }
 
-void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool all_exterior)
+void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool all_exterior, bool equispaced)
 {
     if (k < 1) throw std::invalid_argument("space: degree must be >= 1");
     std::vector<int> dirichlet = dirichlet_in;
@@ -604,8 +699,8 @@ void build_space(Problem &P, int k, const std::vector<int> &dirichlet_in, bool a
     const Mesh &M = P.mesh;
     const int nc = M.nt(), np = M.np(), nv = M.nv;
     P.k = k;
-    if (nv == 4) P.refq = std::make_unique<RefQuad>(k);
-    else P.ref = std::make_unique<RefElement>(k);
+    if (nv == 4) P.refq = std::make_unique<RefQuad>(k, equispaced);
+    else P.ref = std::make_unique<RefElement>(k, equispaced);
     P.nloc = nv == 4 ? P.refq->n : P.ref->n;
     const int n = P.nloc, de = k - 1, nint = nv == 4 ? (k - 1) * (k - 1) : (k - 1) * (k - 2) / 2, f0 = nv + nv * (k - 1);
     P.J.resize(4 * static_cast<std::size_t>(nc)); P.Ji.resize(P.J.size()); P.det.resize(nc);
@@ -1497,14 +1592,14 @@ void orc_set_threads(int n)
 
 /// A structured rectangle mesh and a P_k space on it (diag: 0 right, 1 left, 2 alternate, 3 crossed).
 void *orc_problem_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int diag, int k,
-                            int ndir, const int *dir, int all_exterior)
+                            int ndir, const int *dir, int all_exterior, int equispaced)
 {
     Problem *out = nullptr;
     guard([&] {
         auto P = std::make_unique<Problem>();
         const int mk[4] = {1, 2, 3, 4};
         P->mesh = rectangle(x0, y0, x1, y1, nx, ny, diag, mk);
-        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0);
+        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, equispaced != 0);
         out = P.release();
     });
     return out;
@@ -1512,14 +1607,14 @@ void *orc_problem_rectangle(double x0, double y0, double x1, double y1, int nx, 
 
 /// A mesh from arrays (structured.hpp:99-221) and a P_k space on it.
 void *orc_problem_arrays(int np, const double *pts, int nt, const int *tris, int ns, const int *segs, const int *smarks,
-                         int k, int ndir, const int *dir, int all_exterior)
+                         int k, int ndir, const int *dir, int all_exterior, int equispaced)
 {
     Problem *out = nullptr;
     guard([&] {
         auto P = std::make_unique<Problem>();
         P->mesh = build_mesh(std::vector<double>(pts, pts + 2 * np), std::vector<int>(tris, tris + 3 * nt),
                              std::vector<int>(segs, segs + 2 * ns), smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{}, 1);
-        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0);
+        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, equispaced != 0);
         out = P.release();
     });
     return out;
@@ -1527,14 +1622,14 @@ void *orc_problem_arrays(int np, const double *pts, int nt, const int *tris, int
 
 /// A structured quadrilateral mesh of the rectangle (quadmesh.py:161-170) and a Q_k space on it.
 void *orc_problem_quad_rectangle(double x0, double y0, double x1, double y1, int nx, int ny, int k, int ndir, const int *dir,
-                                 int all_exterior)
+                                 int all_exterior, int equispaced)
 {
     Problem *out = nullptr;
     guard([&] {
         auto P = std::make_unique<Problem>();
         const int mk[4] = {1, 2, 3, 4};
         P->mesh = quad_rectangle(x0, y0, x1, y1, nx, ny, mk);
-        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0);
+        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, equispaced != 0);
         out = P.release();
     });
     return out;
@@ -1542,7 +1637,7 @@ void *orc_problem_quad_rectangle(double x0, double y0, double x1, double y1, int
 
 /// A quadrilateral mesh from arrays (quad_mesh.hpp:50-103) and a Q_k space on it.
 void *orc_problem_quad_arrays(int np, const double *pts, int nq, const int *quads, int ns, const int *segs, const int *smarks,
-                              int k, int ndir, const int *dir, int all_exterior)
+                              int k, int ndir, const int *dir, int all_exterior, int equispaced)
 {
     Problem *out = nullptr;
     guard([&] {
@@ -1550,7 +1645,7 @@ void *orc_problem_quad_arrays(int np, const double *pts, int nq, const int *quad
         P->mesh = build_quad_mesh(std::vector<double>(pts, pts + 2 * np), std::vector<int>(quads, quads + 4 * nq),
                                   std::vector<int>(segs, segs + 2 * ns), smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{},
                                   1);
-        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0);
+        build_space(*P, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, equispaced != 0);
         out = P.release();
     });
     return out;
@@ -1618,9 +1713,9 @@ int orc_quadrature(int degree, int *nq, double *xi, double *eta, double *w)
     });
 }
 
-int orc_basis_eval(int k, double xi, double eta, double *out)
+int orc_basis_eval(int k, int equispaced, double xi, double eta, double *out)
 {
-    return guard([&] { RefElement R(k); R.eval(xi, eta, out); });
+    return guard([&] { RefElement R(k, equispaced != 0); R.eval(xi, eta, out); });
 }
 
 /// The quadrilateral rule of `degree` (tensor Gauss-Legendre on [0,1]^2).
@@ -1635,9 +1730,36 @@ int orc_quadrature_quad(int degree, int *nq, double *xi, double *eta, double *w)
 }
 
 /// Values, d/dxi, d/deta of the Q_k basis at (xi, eta): 3 (k+1)^2 numbers.
-int orc_basis_eval_quad(int k, double xi, double eta, double *out)
+int orc_basis_eval_quad(int k, int equispaced, double xi, double eta, double *out)
 {
-    return guard([&] { RefQuad R(k); R.eval(xi, eta, out); });
+    return guard([&] { RefQuad R(k, equispaced != 0); R.eval(xi, eta, out); });
+}
+
+/// The local nodes (xi, eta) of the P_k (quad = 0) or Q_k (quad = 1) reference element.
+int orc_ref_nodes(int k, int equispaced, int quad, double *xi, double *eta)
+{
+    return guard([&] {
+        if (quad)
+        {
+            RefQuad R(k, equispaced != 0);
+            std::copy(R.nx.begin(), R.nx.end(), xi); std::copy(R.ny.begin(), R.ny.end(), eta);
+        }
+        else
+        {
+            RefElement R(k, equispaced != 0);
+            std::copy(R.nx.begin(), R.nx.end(), xi); std::copy(R.ny.begin(), R.ny.end(), eta);
+        }
+    });
+}
+
+/// The p+1 Gauss-Lobatto points on [-1, 1].
+int orc_lobatto_points(int p, double *out)
+{
+    return guard([&] {
+        if (p < 1) throw std::invalid_argument("Gauss-Lobatto points need p >= 1");
+        const std::vector<double> x = lobatto_points(p);
+        std::copy(x.begin(), x.end(), out);
+    });
 }
 
 int orc_cache_points(void *h, int degree, double *x, double *y, double *w)

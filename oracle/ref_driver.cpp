@@ -137,7 +137,7 @@ void *ref_space_create(void *mh, int k, int ndir, const int *dir, int all_exteri
 /// A QuadMesh from arrays (quads counter-clockwise or clockwise; the listed segments with their
 /// markers, other boundary edges marker 1) and a Q_k space on it.
 void *ref_quad_space_create(int np, const double *pts, int nq, const int *quads, int ns, const int *segs, const int *smarks,
-                            int k, int ndir, const int *dir, int all_exterior)
+                            int k, int ndir, const int *dir, int all_exterior, int equispaced)
 {
     Space *out = nullptr;
     guard([&] {
@@ -150,7 +150,7 @@ void *ref_quad_space_create(int np, const double *pts, int nq, const int *quads,
         for (int e = 0; e < ns; ++e) G[e] = {segs[2 * e], segs[2 * e + 1]};
         std::vector<int> M = smarks ? std::vector<int>(smarks, smarks + ns) : std::vector<int>{};
         S->qmesh = std::make_shared<QuadMesh>(std::move(P), std::move(Q), std::move(G), std::move(M), std::vector<int>{}, 1);
-        S->V = std::make_unique<LagrangeSpaceQ>(*S->qmesh, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, true);
+        S->V = std::make_unique<LagrangeSpaceQ>(*S->qmesh, k, std::vector<int>(dir, dir + ndir), all_exterior != 0, equispaced != 0);
         out = S.release();
     });
     return out;
@@ -183,10 +183,10 @@ int ref_quad_quadrature(int degree, int *nq, double *xi, double *eta, double *w)
 }
 
 /// Values, d/dxi, d/deta of the Q_k basis at (xi, eta): 3 (k+1)^2 numbers.
-int ref_quad_basis_eval(int k, double xi, double eta, double *out)
+int ref_quad_basis_eval(int k, int equispaced, double xi, double eta, double *out)
 {
     return guard([&] {
-        ReferenceQuad R(k, true);
+        ReferenceQuad R(k, equispaced != 0);
         R.eval(xi, eta, 1, out);
     });
 }
@@ -249,6 +249,32 @@ int ref_basis_eval(int k, int equispaced, double xi, double eta, double *out)
     return guard([&] {
         ReferenceTriangle T(k, equispaced != 0);
         T.eval(xi, eta, 1, out);
+    });
+}
+
+/// The local nodes of ReferenceTriangle (quad = 0) or ReferenceQuad (quad = 1).
+int ref_nodes(int k, int equispaced, int quad, double *xi, double *eta)
+{
+    return guard([&] {
+        if (quad)
+        {
+            ReferenceQuad R(k, equispaced != 0);
+            for (int l = 0; l < R.size(); ++l) { xi[l] = R.node_xi(l); eta[l] = R.node_eta(l); }
+        }
+        else
+        {
+            ReferenceTriangle R(k, equispaced != 0);
+            for (int l = 0; l < R.size(); ++l) { xi[l] = R.node_xi(l); eta[l] = R.node_eta(l); }
+        }
+    });
+}
+
+/// detail::gauss_lobatto_nodes (fe/dg_space.hpp:60-80).
+int ref_lobatto_points(int p, double *out)
+{
+    return guard([&] {
+        const std::vector<double> x = detail::gauss_lobatto_nodes(p);
+        std::copy(x.begin(), x.end(), out);
     });
 }
 

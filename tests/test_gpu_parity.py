@@ -319,3 +319,23 @@ def test_quad_forms_and_solve(orc):
     xg, info = fd.cg(K, b, M=K.preconditioner("jacobi"), tol=1e-10)
     xo, io = orc.cg(ip, ix, Ao, b, M="jacobi", tol=1e-10)
     assert info.iterations == io["iterations"] and np.array_equal(xg, xo)
+
+
+@pytest.mark.parametrize("k,quad", [(3, False), (5, False), (3, True), (4, True)])
+def test_lobatto_assembly_bit_exact(orc, k, quad):
+    _name(f"nodes=lobatto assembly k={k} quad={quad}")
+    if quad:
+        pts, q = _distorted_quads(6, 5, k)
+        V = fd.LagrangeSpace(fd.quad_mesh_from_arrays(pts, q), k, bc="dirichlet", nodes="lobatto")
+        P = orc.quad_from_arrays(pts, q, k=k, equispaced=False)
+        rule = orc.quad_quadrature
+    else:
+        V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 6, 5, diagonal="alternate"), k, bc="dirichlet", nodes="lobatto")
+        P = orc.rectangle(6, 5, k, "alternate", equispaced=False)
+        rule = orc.quadrature
+    deg = 2 * k
+    c = np.random.default_rng(k).standard_normal((3, P.ncells * len(rule(deg)[2])))
+    a, b = [1, 2, 0], [1, 2, 0]
+    K = _assemble_values(V, deg, a, b, c)
+    assert np.array_equal(K.data, P.assemble_matrix(deg, a, b, c))
+    assert np.array_equal(_vector_values(V, deg, [0, 1], c[:2]), P.assemble_vector(deg, [0, 1], c[:2]))

@@ -43,6 +43,25 @@ def test_rectangle_numbering_vs_oracle(orc, k, diag):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_lobatto_numbering_vs_oracle(orc, k):
+    """nodes="lobatto": warp-and-blend nodes on triangles, Gauss-Lobatto tensor nodes on quads;
+    the numbering is that of the equispaced space and the node coordinates match bit for bit."""
+    pts, q = _distorted_quads(4, 5, k)
+    cases = [(fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 5, 3, diagonal="crossed"), k, bc="dirichlet", nodes="lobatto"),
+              orc.rectangle(5, 3, k, "crossed", equispaced=False))]
+    if k <= 4:
+        cases.append((fd.LagrangeSpace(fd.quad_mesh_from_arrays(pts, q), k, nodes="lobatto"),
+                      orc.quad_from_arrays(pts, q, k=k, all_exterior=False, equispaced=False)))
+    for V, P in cases:
+        assert V.node_set == "lobatto"
+        num = P.numbering()
+        assert np.array_equal(V.cell_dofs(), num["cell_dof"])
+        assert np.array_equal(V.raw_coordinates(), num["raw_xy"], equal_nan=True)
+    with pytest.raises(ValueError):
+        fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 2, 2), 3, nodes="chebyshev")
+
+
 def _distorted_quads(nx, ny, seed):
     rng = np.random.default_rng(seed)
     xs, ys = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1))

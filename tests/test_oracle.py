@@ -445,3 +445,94 @@ def test_quad_counts_and_patch(orc, k):
     ue = 1 + 2 * xy[:, 0] - xy[:, 1]
     sol = np.linalg.solve(D[np.ix_(free, free)], -D[np.ix_(free, bd)] @ ue[bd])
     assert np.abs(sol - ue[free]).max() < 1e-11
+
+
+# ------------------------------------------------------------- nodes="lobatto" (both cell kinds)
+def test_lobatto_points_closed_forms(orc):
+    """Gauss-Lobatto points: p = 2..5 in closed form; for every p the interior points are roots
+    of P_p' (numpy's Legendre series) and the set is symmetric about 0."""
+    s7 = math.sqrt(7.0)
+    closed = {2: [0.0], 3: [-1 / math.sqrt(5), 1 / math.sqrt(5)], 4: [-math.sqrt(3 / 7), 0.0, math.sqrt(3 / 7)],
+              5: sorted([s * math.sqrt(1 / 3 + q * 2 * s7 / 21) for s in (-1, 1) for q in (-1, 1)])}
+    for p, inner in closed.items():
+        x = orc.lobatto_points(p)
+        assert x[0] == -1.0 and x[-1] == 1.0
+        assert np.abs(x[1:-1] - np.array(inner)).max() < 1e-15, p
+    for p in range(2, 16):
+        x = orc.lobatto_points(p)
+        dP = np.polynomial.legendre.Legendre.basis(p).deriv()
+        assert np.abs(dP(x[1:-1])).max() < 1e-11 * p ** 2 and np.all(np.diff(x) > 0)
+        assert np.abs(x + x[::-1]).max() < 1e-15
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 6])
+def test_warp_blend_nodes(orc, k):
+    """Warp-and-blend nodes: the edge nodes are the Gauss-Lobatto points mapped to each edge, the
+    node set is invariant under the six symmetries of the triangle (to rounding), the single
+    interior node of P3 is the centroid, and the basis is nodal there and spans P_k."""
+    xi, eta = orc.ref_nodes(k, equispaced=False)
+    n = (k + 1) * (k + 2) // 2
+    assert xi.size == n
+    t = 0.5 * (1 + orc.lobatto_points(k)[1:-1])
+    e0 = np.stack([xi[3:3 + k - 1], eta[3:3 + k - 1]], 1)                 # edge 0 runs (1,0) -> (0,1)
+    assert np.abs(e0 - np.stack([1 - t, t], 1)).max() < 1e-15
+    b = np.stack([1 - xi - eta, xi, eta], 1)
+    for perm in ((1, 0, 2), (0, 2, 1), (2, 1, 0), (1, 2, 0), (2, 0, 1)):
+        d = np.abs(b[:, None, :] - b[None, :, perm]).max(axis=2)
+        assert d.min(axis=1).max() < 1e-14
+    if k == 3:
+        assert abs(xi[-1] - 1 / 3) < 1e-15 and abs(eta[-1] - 1 / 3) < 1e-15
+    for j in range(n):
+        B = orc.basis_eval(k, xi[j], eta[j], equispaced=False)
+        e = np.zeros(n); e[j] = 1.0
+        assert np.abs(B[0] - e).max() < 1e-11
+    vals = xi ** (k - 1) * eta
+    for p, q in np.random.default_rng(k).random((4, 2)) * 0.5:
+        B = orc.basis_eval(k, p, q, equispaced=False)
+        assert abs(B[0] @ vals - p ** (k - 1) * q) < 1e-10
+        assert abs(B[1] @ vals - (k - 1) * p ** (k - 2) * q) < 1e-9
+    # the same space as the equispaced basis: every equispaced basis function is reproduced
+    xe, ye = orc.ref_nodes(k)
+    Be = np.array([orc.basis_eval(k, a, c)[0] for a, c in zip(xi, eta)])     # equispaced phi at lobatto nodes
+    p, q = 0.21, 0.37
+    assert np.abs(orc.basis_eval(k, p, q, equispaced=False)[0] @ Be - orc.basis_eval(k, p, q)[0]).max() < 1e-10
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_quad_lobatto_nodes(orc, k):
+    xi, eta = orc.ref_nodes(k, equispaced=False, quad=True)
+    t = 0.5 * (1 + orc.lobatto_points(k))
+    assert np.abs(np.sort(np.unique(np.round(xi, 14))) - t).max() < 1e-14
+    n = (k + 1) ** 2
+    for j in range(n):
+        B = orc.quad_basis_eval(k, xi[j], eta[j], equispaced=False)
+        e = np.zeros(n); e[j] = 1.0
+        assert np.abs(B[0] - e).max() < 1e-12
+    vals = xi ** k * eta ** k
+    B = orc.quad_basis_eval(k, 0.3, 0.8, equispaced=False)
+    assert abs(B[0] @ vals - 0.3 ** k * 0.8 ** k) < 1e-12
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_lobatto_nodes_vs_reference(orc, ref, k):
+    for quad in (False, True):
+        for a, b in zip(orc.ref_nodes(k, False, quad), ref.ref_nodes(k, False, quad)):
+            assert np.array_equal(a, b)
+    for p in range(1, 12):
+        assert np.array_equal(orc.lobatto_points(p), ref.lobatto_points(p))
+    Pts, T = _small_delaunay(3)
+    cases = [(orc.rectangle(5, 4, k, "alternate", equispaced=False), ref.rectangle(5, 4, k, "alternate", equispaced=False)),
+             (orc.from_arrays(Pts, T, k=k, equispaced=False), ref.from_arrays(Pts, T, k=k, equispaced=False))]
+    if k <= 4:
+        pq, qq = _distorted_quads(5, 4, k)
+        cases.append((orc.quad_from_arrays(pq, qq, k=k, equispaced=False), ref.quad_from_arrays(pq, qq, k=k, equispaced=False)))
+    for po, pr in cases:
+        no, nr = po.numbering(), pr.numbering()
+        for key in ("cell_dof", "cell_raw", "ad2raw", "raw_xy"):
+            assert np.array_equal(no[key], nr[key]), key
+        deg = 2 * k + 1
+        npt = po.ncells * len((orc.quad_quadrature if po.quad else orc.quadrature)(deg)[2])
+        c = np.random.default_rng(k).standard_normal((3, npt))
+        A, B = [1, 2, 0], [1, 2, 0]
+        assert np.array_equal(po.assemble_matrix(deg, A, B, c), pr.assemble_matrix(deg, A, B, c))
+        assert np.array_equal(po.assemble_vector(deg, [0, 1], c[:2]), pr.assemble_vector(deg, [0, 1], c[:2]))

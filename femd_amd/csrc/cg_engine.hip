@@ -20,31 +20,48 @@ constexpr int GRAPH_ITERS = 16;
 
 inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + BT - 1) / BT)); }
 
-// The sum of a[0..n) (stride `stride`) in a fixed order -- four interleaved lane-strided
-// partial sums per lane (independent loads in flight), wave shuffles, then the waves in order --
-// returned to every thread of the workgroup.  The order depends only on n and THREADS.
-template <int THREADS>
-__device__ inline double wg_fixed_sum(const double *a, int n, double *red, int stride = 1)
+// The W sums out[w] = sum of a[i * W + w] over i in [0, n), in a fixed order, returned to every
+// thread of the workgroup: lane l adds its elements i = l + k THREADS in k order, loading K of
+// them per round all at once (one memory latency per round, not per element), then the wave
+// (shuffles) and the waves in order.  The order depends only on n and THREADS.  red holds
+// W THREADS / 64 doubles.
+template <int THREADS, int W>
+__device__ inline void wg_fixed_sums(const double *a, int n, double *red, double (&out)[W])
 {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3 * THREADS < n; i += 4 * THREADS)
-    {
-        s0 += a[static_cast<int64_t>(i) * stride];
-        s1 += a[static_cast<int64_t>(i + THREADS) * stride];
-        s2 += a[static_cast<int64_t>(i + 2 * THREADS) * stride];
-        s3 += a[static_cast<int64_t>(i + 3 * THREADS) * stride];
-    }
-    for (; i < n; i += THREADS) s0 += a[static_cast<int64_t>(i) * stride];
-    double s = (s0 + s1) + (s2 + s3);
+    constexpr int K = W == 1 ? 16 : 8;
+    double s[W];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    for (int w = 0; w < W; ++w) s[w] = 0.0;
+    for (int base = 0; base < n; base += K * THREADS)
+    {
+        double v[K][W];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            const int i = base + k * THREADS + static_cast<int>(threadIdx.x);
+#pragma unroll
+            for (int w = 0; w < W; ++w) v[k][w] = i < n ? a[static_cast<int64_t>(i) * W + w] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int w = 0; w < W; ++w) s[w] += v[k][w];
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+    {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s[w] += __shfl_xor(s[w], off, 64);
+        if ((threadIdx.x & 63) == 0) red[W * (threadIdx.x >> 6) + w] = s[w];
+    }
     __syncthreads();
-    double t = 0.0;
-    for (int w = 0; w < THREADS / 64; ++w) t += red[w];
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+    {
+        out[w] = 0.0;
+        for (int q = 0; q < THREADS / 64; ++q) out[w] += red[W * q + w];
+    }
     __syncthreads();
-    return t;
 }
 
 // ---- scaled values ---------------------------------------------------------------------------
@@ -135,7 +152,9 @@ __global__ __launch_bounds__(RT) void k_cg_alpha(const int *__restrict__ seg_til
     if (!S.active) return;
     __shared__ double red[RT / 64];
     const int t0 = seg_tiles[blockIdx.x], t1 = seg_tiles[blockIdx.x + 1];
-    const double pq = wg_fixed_sum<RT>(tparts + t0, t1 - t0, red);
+    double o[1];
+    wg_fixed_sums<RT, 1>(tparts + t0, t1 - t0, red, o);
+    const double pq = o[0];
     if (threadIdx.x != 0) return;
     if (!(pq > 0.0))
     {
@@ -200,10 +219,11 @@ __global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, co
 {
     CGSys &S = st[blockIdx.x];
     if (!S.active) return;
-    __shared__ double red[RT / 64];
+    __shared__ double red[2 * (RT / 64)];
     const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
-    const double rr = wg_fixed_sum<RT>(cparts + 2 * c0, c1 - c0, red, 2);
-    const double rt = wg_fixed_sum<RT>(cparts + 2 * c0 + 1, c1 - c0, red, 2);
+    double o[2];
+    wg_fixed_sums<RT, 2>(cparts + 2 * c0, c1 - c0, red, o);
+    const double rr = o[0], rt = o[1];
     if (threadIdx.x != 0) return;
     S.it += 1;
     S.total += 1;
@@ -270,7 +290,9 @@ __global__ __launch_bounds__(RT) void k_cg_sys_sum(const int *__restrict__ cptr,
     if (!sel[blockIdx.x]) return;
     __shared__ double red[RT / 64];
     const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
-    const double t = wg_fixed_sum<RT>(cparts + c0, c1 - c0, red);
+    double o[1];
+    wg_fixed_sums<RT, 1>(cparts + c0, c1 - c0, red, o);
+    const double t = o[0];
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
@@ -359,11 +381,11 @@ __global__ __launch_bounds__(RT) void k_cg_fresh_state(const int *__restrict__ c
 {
     const int sys = blockIdx.x, mode = sel[sys];
     if (!mode) return;
-    __shared__ double red[RT / 64];
+    __shared__ double red[3 * (RT / 64)];
     const int c0 = cptr[sys], c1 = cptr[sys + 1];
-    const double rq = wg_fixed_sum<RT>(c3 + 3 * c0, c1 - c0, red, 3);
-    const double pq = wg_fixed_sum<RT>(c3 + 3 * c0 + 1, c1 - c0, red, 3);
-    const double rr = wg_fixed_sum<RT>(c3 + 3 * c0 + 2, c1 - c0, red, 3);
+    double o[3];
+    wg_fixed_sums<RT, 3>(c3 + 3 * c0, c1 - c0, red, o);
+    const double rq = o[0], pq = o[1], rr = o[2];
     if (threadIdx.x != 0) return;
     CGSys &S = st[sys];
     const double rn = rho_new[sys];

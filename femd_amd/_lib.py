@@ -79,6 +79,15 @@ _SIGS = {
     "femd_assemble_vector": (_i, [_p, _i, _i, _ip, C.POINTER(Coefficient), _dp]),
     "femd_assemble_vector_dev": (_i, [_p, _i, _i, _ip, C.POINTER(Coefficient), _p, _p]),
     "femd_eval_coefficient": (_i, [_p, _i, C.POINTER(Coefficient), _dp]),
+    "femd_assemble_matrix_add": (_i, [_p, _i, _i, _ip, _ip, C.POINTER(Coefficient), _p]),
+    "femd_assemble_vector_add": (_i, [_p, _i, _i, _ip, C.POINTER(Coefficient), _dp]),
+    "femd_facets_create": (_i, [_p, _i, _i, _ip, _i, C.POINTER(_p)]),
+    "femd_facets_sizes": (_i, [_p, _ip]),
+    "femd_facets_points": (_i, [_p, _dp, _dp, _dp, _dp, _dp]),
+    "femd_facets_free": (None, [_p]),
+    "femd_assemble_matrix_facets": (_i, [_p, _i, _ip, _ip, C.POINTER(Coefficient), _p, _i]),
+    "femd_assemble_vector_facets": (_i, [_p, _i, _ip, C.POINTER(Coefficient), _dp, _i]),
+    "femd_eval_coefficient_facets": (_i, [_p, C.POINTER(Coefficient), _dp]),
     "femd_spmv": (_i, [_p, _dp, _dp]),
     "femd_spmv_dev": (_i, [_p, _p, _p, _i, _p]),
     "femd_spmv_tuning": (_i, [_p, _i, C.c_char_p, _i, C.POINTER(_d)]),

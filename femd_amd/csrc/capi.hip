@@ -154,7 +154,7 @@ struct Coeffs {
     TermSet T{};
 };
 
-void check_program(const femd_coefficient &c)
+void check_program(const femd_coefficient &c, bool normals = false)
 {
     if (c.ninstr < 1) fail(FEMD_EINVAL, "coefficient: empty program");
     if (c.nreg < 1) fail(FEMD_EINVAL, "coefficient: at least one register");
@@ -163,8 +163,10 @@ void check_program(const femd_coefficient &c)
         const int op = c.code[4 * k], o = c.code[4 * k + 1], a = c.code[4 * k + 2], b = c.code[4 * k + 3];
         if (o < 0 || o >= c.nreg) fail(FEMD_EINVAL, "coefficient: register out of range");
         const bool unary = op >= 9 && op <= 19;
-        if (op == 3 || op == 4 || op == 5 || op < 0 || op > 19)
-            fail(FEMD_EINVAL, "coefficient: instruction " + std::to_string(op) + " is not supported on the GPU (fields and normals are not)");
+        if (((op == 3 || op == 4) && !normals) || op == 5 || op < 0 || op > 19)
+            fail(FEMD_EINVAL, "coefficient: instruction " + std::to_string(op) +
+                                  (op == 5 ? " (a known field) is not supported on the GPU"
+                                           : " (a facet normal) needs a ds term"));
         if (op == 0 && (a < 0 || a >= c.nconst)) fail(FEMD_EINVAL, "coefficient: constant out of range");
         if ((op == 6 || op == 7) && (a < 0 || a >= c.nreg || b < 0 || b >= c.nreg)) fail(FEMD_EINVAL, "coefficient: register out of range");
         if (op == 8 && (a < 0 || a >= c.nreg || b < 0 || b >= c.nconst)) fail(FEMD_EINVAL, "coefficient: pow operands out of range");
@@ -856,6 +858,7 @@ int femd_space_create(femd_mesh m, int degree, int ndirichlet, const int *dirich
         if (!quad && degree > 5) fail(FEMD_EINVAL, "LagrangeSpace2D: the GPU kernels cover degrees 1..5");
         if (quad && degree > 4) fail(FEMD_EINVAL, "LagrangeSpaceQ: the GPU kernels cover degrees 1..4 on quadrilaterals");
         auto V = std::make_unique<femd_space_s>();
+        V->M = m->M;
         V->V = build_space(m->M, degree, std::vector<int>(dirichlet, dirichlet + std::max(ndirichlet, 0)), all_exterior != 0,
                            equispaced != 0);
         if (quad) V->refq = std::make_unique<RefQuad>(degree, equispaced != 0);
@@ -1058,10 +1061,12 @@ void femd_csr_free(femd_csr A)
 }
 
 // ---- assembly ----------------------------------------------------------------------------
-int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
-                         femd_csr K)
+extern "C++" {
+namespace {
+void assemble_matrix_dx(femd_space V, int quad_degree, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
+                        femd_csr K, int mode)
 {
-    return api([&] {
+    {
         femd_space_s *Vs = check_space(V);
         femd_csr_s *Ks = check_csr(K);
         hipStream_t s = S(nullptr);
@@ -1074,27 +1079,59 @@ int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a
         if (!launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
                                 Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s))
             fail(FEMD_EINVAL, "assemble_matrix: quadrature degree " + std::to_string(quad_degree) + " is too high for the element kernel");
-        launch_gather(Ks->pat->nnz, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(), false, s);
+        launch_gather_mode(Ks->pat->nnz, nullptr, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(),
+                           mode, s);
         Ks->v32_valid = false; Ks->v32s_valid = false; Ks->v64s_valid = false;
         HIPC(hipStreamSynchronize(s));
-    });
+    }
+}
+
+void assemble_vector_dx(femd_space V, int quad_degree, int nterms, const int *a, const femd_coefficient *coeffs, double *f_dev,
+                        hipStream_t s, int mode)
+{
+    femd_space_s *Vs = check_space(V);
+    ensure_pattern(Vs, s);
+    Quad &Q = quad_of(Vs, quad_degree, s);
+    Coeffs C = make_coeffs(Vs, Q, nterms, a, nullptr, coeffs, s);
+    const int n = Vs->V.nloc, nc = Vs->V.ncells;
+    DevBuf F(8 * std::max<size_t>(size_t(nc) * n, 1));
+    if (!launch_elem_vector(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
+                            Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, F.as<double>(), s))
+        fail(FEMD_EINVAL, "assemble_vector: quadrature degree " + std::to_string(quad_degree) + " is too high for the element kernel");
+    launch_gather_mode(Vs->V.dim, nullptr, Vs->vptr.as<int>(), Vs->vlist.as<int>(), F.as<double>(), f_dev, mode, s);
+    HIPC(hipStreamSynchronize(s));
+}
+} // namespace
+} // extern "C++"
+
+int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
+                         femd_csr K)
+{
+    return api([&] { assemble_matrix_dx(V, quad_degree, nterms, a, b, coeffs, K, GATHER_SET); });
+}
+
+int femd_assemble_matrix_add(femd_space V, int quad_degree, int nterms, const int *a, const int *b,
+                             const femd_coefficient *coeffs, femd_csr K)
+{
+    return api([&] { assemble_matrix_dx(V, quad_degree, nterms, a, b, coeffs, K, GATHER_CONTINUE); });
 }
 
 int femd_assemble_vector_dev(femd_space V, int quad_degree, int nterms, const int *a, const femd_coefficient *coeffs,
                              double *f_dev, femd_stream st)
 {
+    return api([&] { assemble_vector_dx(V, quad_degree, nterms, a, coeffs, f_dev, S(st), GATHER_SET_PZ); });
+}
+
+int femd_assemble_vector_add(femd_space V, int quad_degree, int nterms, const int *a, const femd_coefficient *coeffs, double *f)
+{
     return api([&] {
         femd_space_s *Vs = check_space(V);
-        hipStream_t s = S(st);
-        ensure_pattern(Vs, s);
-        Quad &Q = quad_of(Vs, quad_degree, s);
-        Coeffs C = make_coeffs(Vs, Q, nterms, a, nullptr, coeffs, s);
-        const int n = Vs->V.nloc, nc = Vs->V.ncells;
-        DevBuf F(8 * std::max<size_t>(size_t(nc) * n, 1));
-        if (!launch_elem_vector(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
-                                Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, F.as<double>(), s))
-            fail(FEMD_EINVAL, "assemble_vector: quadrature degree " + std::to_string(quad_degree) + " is too high for the element kernel");
-        launch_gather(Vs->V.dim, Vs->vptr.as<int>(), Vs->vlist.as<int>(), F.as<double>(), f_dev, true, s);
+        hipStream_t s = S(nullptr);
+        const int dim = Vs->V.dim;
+        DevBuf fd(8 * size_t(std::max(dim, 1)));
+        if (dim) HIPC(hipMemcpyAsync(fd.p, f, 8 * size_t(dim), hipMemcpyHostToDevice, s));
+        assemble_vector_dx(V, quad_degree, nterms, a, coeffs, fd.as<double>(), s, GATHER_ADD);
+        if (dim) HIPC(hipMemcpyAsync(f, fd.p, 8 * size_t(dim), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
     });
 }
@@ -1123,6 +1160,188 @@ int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient 
         const size_t np = size_t(Vs->V.ncells) * Q.nq;
         DevBuf d(8 * std::max<size_t>(np, 1));
         eval_program(Vs, Q, *c, d.as<double>(), s);
+        if (np) HIPC(hipMemcpyAsync(out, d.p, 8 * np, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+// ---- boundary facets (ds) -----------------------------------------------------------------
+extern "C++" {
+namespace {
+femd_facets_s *check_facets(femd_facets F)
+{
+    if (!F || !F->space) fail(FEMD_EINVAL, "null facet set");
+    return F;
+}
+
+template <class T> DevBuf upload_nz(const std::vector<T> &v, hipStream_t s)
+{
+    return v.empty() ? DevBuf(sizeof(T)) : upload(v, s);
+}
+
+// The facet set's arrays on the device, uploaded on first use (creation is host work only).
+void facets_device(femd_facets_s *Fs, hipStream_t s)
+{
+    if (Fs->var.p) return;
+    const FacetData &D = Fs->F;
+    Fs->var = upload_nz(D.var, s); Fs->w = upload_nz(D.w, s); Fs->Ji = upload_nz(D.Ji, s); Fs->tab = upload_nz(D.tab, s);
+    Fs->x = upload_nz(D.x, s); Fs->y = upload_nz(D.y, s); Fs->nx = upload_nz(D.nx, s); Fs->ny = upload_nz(D.ny, s);
+    Fs->mpos = upload_nz(D.mpos, s); Fs->mptr = upload_nz(D.mptr, s); Fs->mlist = upload_nz(D.mlist, s);
+    Fs->vdof = upload_nz(D.vdof, s); Fs->vptr = upload_nz(D.vptr, s); Fs->vlist = upload_nz(D.vlist, s);
+    HIPC(hipStreamSynchronize(s));
+}
+
+// A program at the facet points, the outward normal available (Integrand NX / NY)
+void eval_program_facets(femd_facets_s *Fs, const femd_coefficient &c, double *out, hipStream_t s)
+{
+    check_program(c, true);
+    int nreg = 0;
+    std::vector<int> compact = compact_registers(c, nreg);
+    if (nreg > MAX_REGS) fail(FEMD_EINVAL, "coefficient: the program needs more than " + std::to_string(MAX_REGS) + " live registers");
+    DevBuf code = upload(compact, s);
+    DevBuf consts = upload(c.consts, static_cast<size_t>(std::max(c.nconst, 0)), s);
+    launch_coefficient_pts(int64_t(Fs->F.nent) * Fs->F.nq, Fs->x.as<double>(), Fs->y.as<double>(), Fs->nx.as<double>(),
+                           Fs->ny.as<double>(), c.ninstr, code.as<int>(), consts.as<double>(), out, s);
+    HIPC(hipStreamSynchronize(s));
+}
+
+Coeffs facet_coeffs(femd_facets_s *Fs, int nt, const int *a, const int *b, const femd_coefficient *c, hipStream_t s)
+{
+    if (nt < 1 || nt > MAX_TERMS) fail(FEMD_EINVAL, "assembly: 1.." + std::to_string(MAX_TERMS) + " terms per call");
+    Coeffs C;
+    C.T.nt = nt;
+    const size_t np = size_t(Fs->F.nent) * Fs->F.nq;
+    for (int t = 0; t < nt; ++t)
+    {
+        if (a[t] < 0 || a[t] > 2 || (b && (b[t] < 0 || b[t] > 2))) fail(FEMD_EINVAL, "assembly_2d: derivative code is not in 0..2");
+        C.T.a[t] = a[t];
+        C.T.b[t] = b ? b[t] : 0;
+        C.T.c[t] = nullptr;
+        C.T.cval[t] = 0.0;
+        const femd_coefficient &k = c[t];
+        if (k.values)
+        {
+            C.bufs.push_back(np ? upload(k.values, np, s) : DevBuf(sizeof(double)));
+            C.T.c[t] = C.bufs.back().as<double>();
+        }
+        else if (k.ninstr == 1 && k.code[0] == 0)
+        {
+            if (k.code[2] < 0 || k.code[2] >= k.nconst) fail(FEMD_EINVAL, "coefficient: constant out of range");
+            C.T.cval[t] = k.consts[k.code[2]];
+        }
+        else
+        {
+            C.bufs.emplace_back(std::max<size_t>(np, 1) * sizeof(double));
+            if (np) eval_program_facets(Fs, k, C.bufs.back().as<double>(), s);
+            C.T.c[t] = C.bufs.back().as<double>();
+        }
+    }
+    return C;
+}
+} // namespace
+} // extern "C++"
+
+int femd_facets_create(femd_space V, int degree, int nmarkers, const int *markers, int all, femd_facets *out)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        if (!all && (nmarkers < 1 || !markers)) fail(FEMD_EINVAL, "ds: name the markers, or ask for every exterior facet");
+        ensure_host_pattern(Vs);
+        auto F = std::make_unique<femd_facets_s>();
+        F->space = Vs;
+        F->F = boundary_facets(Vs->M, Vs->V, Vs->P, Vs->ref.get(), Vs->refq.get(), std::max(degree, 0),
+                               all ? std::vector<int>{} : std::vector<int>(markers, markers + nmarkers), all != 0);
+        *out = F.release();
+    });
+}
+
+int femd_facets_sizes(femd_facets F, int out[2])
+{
+    return api([&] {
+        femd_facets_s *Fs = check_facets(F);
+        out[0] = Fs->F.nent; out[1] = Fs->F.nq;
+    });
+}
+
+int femd_facets_points(femd_facets F, double *x, double *y, double *w, double *nx, double *ny)
+{
+    return api([&] {
+        const FacetData &D = check_facets(F)->F;
+        if (x) std::copy(D.x.begin(), D.x.end(), x);
+        if (y) std::copy(D.y.begin(), D.y.end(), y);
+        if (w) std::copy(D.w.begin(), D.w.end(), w);
+        if (nx) std::copy(D.nx.begin(), D.nx.end(), nx);
+        if (ny) std::copy(D.ny.begin(), D.ny.end(), ny);
+    });
+}
+
+void femd_facets_free(femd_facets F)
+{
+    if (F && F->var.p && g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
+    delete F;
+}
+
+int femd_assemble_matrix_facets(femd_facets F, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
+                                femd_csr K, int mode)
+{
+    return api([&] {
+        femd_facets_s *Fs = check_facets(F);
+        femd_space_s *Vs = Fs->space;
+        femd_csr_s *Ks = check_csr(K);
+        hipStream_t s = S(nullptr);
+        ensure_pattern(Vs, s);
+        if (Ks->pat != Vs->pat) fail(FEMD_EPATTERN, "assemble_matrix: the matrix is not on the space's pattern");
+        if (mode != 0 && mode != 1) fail(FEMD_EINVAL, "assemble_matrix_facets: mode is 0 (zero K first) or 1 (add)");
+        facets_device(Fs, s);
+        const FacetData &D = Fs->F;
+        Coeffs C = facet_coeffs(Fs, nterms, a, b, coeffs, s);
+        if (mode == 0) launch_fill(Ks->pat->nnz, 0.0, Ks->v.as<double>(), s);
+        const int n = D.nloc;
+        DevBuf E(8 * std::max<size_t>(size_t(D.nent) * n * n, 1));
+        if (!launch_facet_matrix(n, D.nent, D.nq, Fs->var.as<int>(), Fs->w.as<double>(), Fs->Ji.as<double>(), Fs->tab.as<double>(),
+                                 C.T, E.as<double>(), s))
+            fail(FEMD_EINVAL, "assemble_matrix_facets: no facet kernel for this element");
+        launch_gather_mode(static_cast<int64_t>(D.mpos.size()), Fs->mpos.as<int>(), Fs->mptr.as<int>(), Fs->mlist.as<int>(),
+                           E.as<double>(), Ks->v.as<double>(), GATHER_CONTINUE, s);
+        Ks->v32_valid = false; Ks->v32s_valid = false; Ks->v64s_valid = false;
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_assemble_vector_facets(femd_facets F, int nterms, const int *a, const femd_coefficient *coeffs, double *f, int mode)
+{
+    return api([&] {
+        femd_facets_s *Fs = check_facets(F);
+        femd_space_s *Vs = Fs->space;
+        hipStream_t s = S(nullptr);
+        if (mode != 0 && mode != 1) fail(FEMD_EINVAL, "assemble_vector_facets: mode is 0 (zero f first) or 1 (add)");
+        facets_device(Fs, s);
+        const FacetData &D = Fs->F;
+        const int dim = Vs->V.dim, n = D.nloc;
+        Coeffs C = facet_coeffs(Fs, nterms, a, nullptr, coeffs, s);
+        DevBuf fd(8 * size_t(std::max(dim, 1)));
+        if (mode == 0) launch_fill(dim, 0.0, fd.as<double>(), s);
+        else if (dim) HIPC(hipMemcpyAsync(fd.p, f, 8 * size_t(dim), hipMemcpyHostToDevice, s));
+        DevBuf Fe(8 * std::max<size_t>(size_t(D.nent) * n, 1));
+        if (!launch_facet_vector(n, D.nent, D.nq, Fs->var.as<int>(), Fs->w.as<double>(), Fs->Ji.as<double>(), Fs->tab.as<double>(),
+                                 C.T, Fe.as<double>(), s))
+            fail(FEMD_EINVAL, "assemble_vector_facets: no facet kernel for this element");
+        launch_gather_mode(static_cast<int64_t>(D.vdof.size()), Fs->vdof.as<int>(), Fs->vptr.as<int>(), Fs->vlist.as<int>(),
+                           Fe.as<double>(), fd.as<double>(), GATHER_ADD, s);
+        if (dim) HIPC(hipMemcpyAsync(f, fd.p, 8 * size_t(dim), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+    });
+}
+
+int femd_eval_coefficient_facets(femd_facets F, const femd_coefficient *c, double *out)
+{
+    return api([&] {
+        femd_facets_s *Fs = check_facets(F);
+        hipStream_t s = S(nullptr);
+        facets_device(Fs, s);
+        const size_t np = size_t(Fs->F.nent) * Fs->F.nq;
+        DevBuf d(8 * std::max<size_t>(np, 1));
+        if (np) eval_program_facets(Fs, *c, d.as<double>(), s);
         if (np) HIPC(hipMemcpyAsync(out, d.p, 8 * np, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
     });

@@ -870,6 +870,145 @@ PatternData build_pattern(const SpaceData &V)
     return P;
 }
 
+// ------------------------------------------------------------------------ boundary facets
+FacetData boundary_facets(const MeshData &M, const SpaceData &V, const PatternData &P, const RefElement *RT,
+                          const RefQuad *RQ, int degree, std::vector<int> markers, bool all)
+{
+    std::sort(markers.begin(), markers.end());
+    const int nv = M.nv, nl = V.nloc, nc = M.nt();
+    FacetData F;
+    F.nvar = nv;
+    F.nloc = nl;
+    std::vector<double> g, gw;
+    gauss_legendre(std::max(1, (degree + (nv == 4 ? 3 : 2)) / 2), g, gw);
+    F.nq = static_cast<int>(g.size());
+    const int nq = F.nq;
+    // the reference points of every local edge and the basis there (fe/cache_2d.hpp:135-149)
+    static const double tvx[3] = {0.0, 1.0, 0.0}, tvy[3] = {0.0, 0.0, 1.0};
+    static const double qvx[4] = {0.0, 1.0, 1.0, 0.0}, qvy[4] = {0.0, 0.0, 1.0, 1.0};
+    std::vector<double> fx(static_cast<std::size_t>(nv) * nq), fy(fx.size());
+    F.tab.assign(static_cast<std::size_t>(nv) * nq * 3 * nl, 0.0);
+    for (int i = 0; i < nv; ++i)
+    {
+        const int va = nv == 4 ? i : (i + 1) % 3, vb = nv == 4 ? (i + 1) % 4 : (i + 2) % 3;
+        const double ax = nv == 4 ? qvx[va] : tvx[va], ay = nv == 4 ? qvy[va] : tvy[va];
+        const double bx = nv == 4 ? qvx[vb] : tvx[vb], by = nv == 4 ? qvy[vb] : tvy[vb];
+        for (int q = 0; q < nq; ++q)
+        {
+            const double t = 0.5 * (g[q] + 1.0);
+            const std::size_t k = static_cast<std::size_t>(i) * nq + q;
+            fx[k] = (1.0 - t) * ax + t * bx;
+            fy[k] = (1.0 - t) * ay + t * by;
+            double *o = &F.tab[k * 3 * nl];
+            if (nv == 4) RQ->tabulate(fx[k], fy[k], o);
+            else RT->tabulate(fx[k], fy[k], o);
+        }
+    }
+    for (int c = 0; c < nc; ++c)
+        for (int i = 0; i < nv; ++i)
+        {
+            if (M.nbr[static_cast<std::size_t>(c) * nv + i] >= 0) continue;
+            int a = 0, b = 0;
+            M.edge(c, i, a, b);
+            const int m = M.marker_of(a, b);
+            if (!all && !std::binary_search(markers.begin(), markers.end(), m >= 0 ? m : 0)) continue;
+            F.cell.push_back(c);
+            F.var.push_back(i);
+        }
+    F.nent = static_cast<int>(F.cell.size());
+    const std::size_t np = static_cast<std::size_t>(F.nent) * nq;
+    F.x.resize(np); F.y.resize(np); F.w.resize(np); F.nx.resize(np); F.ny.resize(np); F.Ji.resize(4 * np);
+    const int gs = V.geom_stride();
+    for (int e = 0; e < F.nent; ++e)
+    {
+        const int c = F.cell[e], i = F.var[e];
+        int a = 0, b = 0;
+        M.edge(c, i, a, b);
+        const double dx = M.xy[2 * b] - M.xy[2 * a], dy = M.xy[2 * b + 1] - M.xy[2 * a + 1], len = std::hypot(dx, dy);
+        const double *gc = &V.geom[gs * static_cast<std::size_t>(c)];
+        for (int q = 0; q < nq; ++q)
+        {
+            const std::size_t k = static_cast<std::size_t>(e) * nq + q;
+            const double xi = fx[static_cast<std::size_t>(i) * nq + q], eta = fy[static_cast<std::size_t>(i) * nq + q];
+            double J[4];
+            if (nv == 4)
+            {
+                // the bilinear map and its Jacobian (fe/lagrange_space_quad.hpp:155-172)
+                const double a0 = (1 - xi) * (1 - eta), a1 = xi * (1 - eta), a2 = xi * eta, a3 = (1 - xi) * eta;
+                F.x[k] = a0 * gc[0] + a1 * gc[2] + a2 * gc[4] + a3 * gc[6];
+                F.y[k] = a0 * gc[1] + a1 * gc[3] + a2 * gc[5] + a3 * gc[7];
+                J[0] = (gc[2] - gc[0]) * (1 - eta) + (gc[4] - gc[6]) * eta;
+                J[1] = (gc[6] - gc[0]) * (1 - xi) + (gc[4] - gc[2]) * xi;
+                J[2] = (gc[3] - gc[1]) * (1 - eta) + (gc[5] - gc[7]) * eta;
+                J[3] = (gc[7] - gc[1]) * (1 - xi) + (gc[5] - gc[3]) * xi;
+            }
+            else
+            {
+                // the affine map (fe/lagrange_space_2d.hpp:82-89)
+                F.x[k] = gc[0] + gc[2] * xi + gc[3] * eta;
+                F.y[k] = gc[1] + gc[4] * xi + gc[5] * eta;
+                J[0] = gc[2]; J[1] = gc[3]; J[2] = gc[4]; J[3] = gc[5];
+            }
+            F.w[k] = 0.5 * gw[q] * len;
+            F.nx[k] = dy / len;
+            F.ny[k] = -dx / len;
+            const double d = J[0] * J[3] - J[1] * J[2];
+            double *Ji = &F.Ji[4 * k];
+            Ji[0] = J[3] / d; Ji[1] = -J[1] / d;
+            Ji[2] = -J[2] / d; Ji[3] = J[0] / d;
+        }
+    }
+    // gather lists: per dof the (entry, local) pairs in ascending entry order, then per row the
+    // (column, element entry) pairs stably sorted by column
+    const int n = V.dim;
+    std::vector<int> cnt(static_cast<std::size_t>(n) + 1, 0);
+    for (int e = 0; e < F.nent; ++e)
+        for (int l = 0; l < nl; ++l) { const int d = V.cell_dof[static_cast<std::size_t>(F.cell[e]) * nl + l]; if (d >= 0) ++cnt[d + 1]; }
+    for (int d = 0; d < n; ++d) cnt[d + 1] += cnt[d];
+    std::vector<int> lst(cnt[n]);
+    {
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < F.nent; ++e)
+            for (int l = 0; l < nl; ++l)
+            {
+                const int d = V.cell_dof[static_cast<std::size_t>(F.cell[e]) * nl + l];
+                if (d >= 0) lst[fill[d]++] = e * nl + l;
+            }
+    }
+    std::vector<std::pair<int, int>> buf;
+    for (int d = 0; d < n; ++d)
+    {
+        if (cnt[d] == cnt[d + 1]) continue;
+        F.vdof.push_back(d);
+        F.vptr.push_back(static_cast<int>(F.vlist.size()));
+        for (int q = cnt[d]; q < cnt[d + 1]; ++q) F.vlist.push_back(lst[q]);
+        buf.clear();
+        for (int q = cnt[d]; q < cnt[d + 1]; ++q)
+        {
+            const int e = lst[q] / nl, l = lst[q] % nl;
+            const int *cd = &V.cell_dof[static_cast<std::size_t>(F.cell[e]) * nl];
+            for (int j = 0; j < nl; ++j)
+                if (cd[j] >= 0) buf.emplace_back(cd[j], (e * nl + l) * nl + j);
+        }
+        std::stable_sort(buf.begin(), buf.end(), [](const auto &u, const auto &v) { return u.first < v.first; });
+        const int *row = P.indices.data() + P.indptr[d], len = P.indptr[d + 1] - P.indptr[d];
+        for (std::size_t t = 0; t < buf.size(); ++t)
+        {
+            if (t == 0 || buf[t].first != buf[t - 1].first)
+            {
+                const int *hit = std::lower_bound(row, row + len, buf[t].first);
+                if (hit == row + len || *hit != buf[t].first) bad("boundary facets: an entry outside the pattern");
+                F.mptr.push_back(static_cast<int>(F.mlist.size()));
+                F.mpos.push_back(P.indptr[d] + static_cast<int>(hit - row));
+            }
+            F.mlist.push_back(buf[t].second);
+        }
+    }
+    F.vptr.push_back(static_cast<int>(F.vlist.size()));
+    F.mptr.push_back(static_cast<int>(F.mlist.size()));
+    return F;
+}
+
 std::vector<int> spmv_row_blocks(int n, const int *indptr, int max_rows, int cap)
 {
     std::vector<int> b{0};

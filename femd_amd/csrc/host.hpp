@@ -106,6 +106,25 @@ struct PatternData {
 };
 PatternData build_pattern(const SpaceData &V);
 
+// The exterior facets of a space whose side marker is in `markers` (every exterior facet with
+// `all`), in the order of the space's facet list (fe/space_2d.hpp:432-448: cells ascending, local
+// edges ascending), with Gauss-Legendre points along each edge and the geometry of
+// fe/cache_2d.hpp:126-183 at every point: physical point, weight (reference weight times half the
+// edge length), outward unit normal and J^{-1} of the cell's map; the basis tabulated once per
+// local edge; and the gather lists that add the facet element matrices (vectors) into a pattern's
+// stored entries (dofs) in ascending facet order.
+struct FacetData {
+    int nq = 0, nent = 0, nvar = 0, nloc = 0;
+    std::vector<int> cell, var;                  // entry e: its cell and local edge
+    std::vector<double> x, y, w, nx, ny, Ji;     // per (entry, point), [e nq + q]; Ji 4 per point
+    std::vector<double> tab;                     // [(variant nq + q) 3 nloc]: values, d/dxi, d/deta
+    // stored entry mpos[k] receives E[mlist[j]], j in [mptr[k], mptr[k+1]), E laid out [e][i][j];
+    // dof vdof[k] receives F[vlist[j]], j in [vptr[k], vptr[k+1]), F laid out [e][i]
+    std::vector<int> mpos, mptr, mlist, vdof, vptr, vlist;
+};
+FacetData boundary_facets(const MeshData &M, const SpaceData &V, const PatternData &P, const RefElement *RT,
+                          const RefQuad *RQ, int degree, std::vector<int> markers, bool all);
+
 // Row blocks of the CSR-stream SpMV: block b covers rows [rows[b], rows[b+1]) whose entries
 // number at most cap; a single row longer than cap gets a block of its own (flagged long).
 std::vector<int> spmv_row_blocks(int n, const int *indptr, int max_rows, int cap);

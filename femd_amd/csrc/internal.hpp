@@ -185,6 +185,7 @@ struct femd_mesh_s {
 };
 
 struct femd_space_s {
+    femd_amd::MeshData M;                          // the mesh it was built on (its boundary facets)
     femd_amd::SpaceData V;
     femd_amd::DevBuf geom;
     femd_amd::PatternData P;
@@ -194,6 +195,12 @@ struct femd_space_s {
     std::map<int, std::unique_ptr<femd_amd::Quad>> quads;
     std::unique_ptr<femd_amd::RefElement> ref;     // triangles
     std::unique_ptr<femd_amd::RefQuad> refq;       // quadrilaterals
+};
+
+struct femd_facets_s {
+    femd_space_s *space = nullptr;
+    femd_amd::FacetData F;
+    femd_amd::DevBuf var, w, Ji, tab, x, y, nx, ny, mpos, mptr, mlist, vdof, vptr, vlist;
 };
 
 struct femd_csr_s {

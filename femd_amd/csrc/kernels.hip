@@ -219,44 +219,99 @@ __global__ __launch_bounds__(TPB) void k_elem_vector(int nc, int nq, const doubl
     F[idx] = f;
 }
 
-// dst[p] = ((0 + src[list[ptr[p]]]) + src[list[ptr[p]+1]]) + ...: the element-order sum.
-__global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int *__restrict__ ptr, const int *__restrict__ list,
-                                                const double *__restrict__ src, double *__restrict__ dst, int plus_zero)
+// One thread per (boundary facet e, row i): the facet's element matrix as assemble_matrix_2d forms
+// it on a facet cache (forms/assembly_2d.hpp:102-126 on fe/cache_2d.hpp:126-183): the cell's basis
+// at the facet's points (the table of its local edge), J^{-1} and the weight of each point as the
+// cache stores them; the loop order and zero skips of k_elem_matrix.
+template <int N>
+__global__ __launch_bounds__(TPB) void k_facet_matrix(int ne, int nq, const int *__restrict__ var, const double *__restrict__ w,
+                                                      const double *__restrict__ Ji, const double *__restrict__ tab, TermSet T,
+                                                      double *__restrict__ E)
 {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    double acc = 0.0;
-    for (int q = ptr[p]; q < ptr[p + 1]; ++q) acc += src[list[q]];
-    dst[p] = plus_zero ? 0.0 + acc : acc;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<int64_t>(ne) * N) return;
+    const int e = static_cast<int>(idx / N), i = static_cast<int>(idx % N);
+    const double *tv = tab + static_cast<int64_t>(var[e]) * nq * 3 * N;
+    double K[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) K[j] = 0.0;
+    for (int t = 0; t < T.nt; ++t)
+    {
+        const int ca = T.a[t], cb = T.b[t];
+        const double *c = T.c[t];
+        const double cv = T.cval[t];
+        for (int q = 0; q < nq; ++q)
+        {
+            const int64_t k = static_cast<int64_t>(e) * nq + q;
+            const double i0 = Ji[4 * k], i1 = Ji[4 * k + 1], i2 = Ji[4 * k + 2], i3 = Ji[4 * k + 3];
+            const double s = w[k] * (c ? c[k] : cv);
+            if (s == 0.0) continue;
+            const double *R = tv + q * 3 * N;
+            const double sa = s * phys<N>(ca, R, i, i0, i1, i2, i3);
+            if (sa == 0.0) continue;
+#pragma unroll
+            for (int j = 0; j < N; ++j) K[j] += sa * phys<N>(cb, R, j, i0, i1, i2, i3);
+        }
+    }
+    double *out = E + (static_cast<int64_t>(e) * N + i) * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = K[j];
+}
+
+// One thread per (boundary facet e, local l): the facet's element vector (assemble_vector_2d on a
+// facet cache).
+template <int N>
+__global__ __launch_bounds__(TPB) void k_facet_vector(int ne, int nq, const int *__restrict__ var, const double *__restrict__ w,
+                                                      const double *__restrict__ Ji, const double *__restrict__ tab, TermSet T,
+                                                      double *__restrict__ F)
+{
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<int64_t>(ne) * N) return;
+    const int e = static_cast<int>(idx / N), l = static_cast<int>(idx % N);
+    const double *tv = tab + static_cast<int64_t>(var[e]) * nq * 3 * N;
+    double f = 0.0;
+    for (int t = 0; t < T.nt; ++t)
+    {
+        const double *c = T.c[t];
+        const double cv = T.cval[t];
+        for (int q = 0; q < nq; ++q)
+        {
+            const int64_t k = static_cast<int64_t>(e) * nq + q;
+            const double i0 = Ji[4 * k], i1 = Ji[4 * k + 1], i2 = Ji[4 * k + 2], i3 = Ji[4 * k + 3];
+            const double s = w[k] * (c ? c[k] : cv);
+            if (s == 0.0) continue;
+            f += s * phys<N>(T.a[t], tv + q * 3 * N, l, i0, i1, i2, i3);
+        }
+    }
+    F[idx] = f;
+}
+
+// Slot k (dst index pos[k], or k without pos) receives the element-order sum of its
+// contributions src[list[ptr[k]]], src[list[ptr[k]+1]], ...:
+//   GATHER_SET      dst = ((0 + c0) + c1) + ...       (a matrix, or a group's own vector block)
+//   GATHER_SET_PZ   dst = 0 + (((0 + c0) + c1) + ...) (the block added onto a zeroed vector)
+//   GATHER_ADD      dst = dst + (((0 + c0) + c1) + ...) (a later group's vector block)
+//   GATHER_CONTINUE dst = ((dst + c0) + c1) + ...     (a later group's matrix entries: every
+//                   contribution is added onto the stored value, as assemble_matrix_2d does)
+__global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int *__restrict__ pos, const int *__restrict__ ptr,
+                                                const int *__restrict__ list, const double *__restrict__ src,
+                                                double *__restrict__ dst, int mode)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t p = pos ? pos[k] : k;
+    double acc = mode == GATHER_CONTINUE ? dst[p] : 0.0;
+    for (int q = ptr[k]; q < ptr[k + 1]; ++q) acc += src[list[q]];
+    dst[p] = mode == GATHER_SET_PZ ? 0.0 + acc : mode == GATHER_ADD ? dst[p] + acc : acc;
 }
 
 // The Integrand register program (forms/integrand.hpp:300-342) at the quadrature points, the
 // point mapped as the cache does: x = x0 + J00 xi + J01 eta on a triangle
 // (fe/lagrange_space_2d.hpp:78-85), the bilinear map on a quadrilateral
 // (fe/lagrange_space_quad.hpp:155-162).
-__global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, const double *__restrict__ geom,
-                                                     const double *__restrict__ qxi, const double *__restrict__ qeta,
-                                                     int ninstr, const int *__restrict__ code, int nreg,
-                                                     const double *__restrict__ consts, double *__restrict__ out)
+__device__ inline double run_program(double x, double y, double nx, double ny, int ninstr, const int *__restrict__ code,
+                                     const double *__restrict__ consts)
 {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= static_cast<int64_t>(nc) * nq) return;
-    const int e = static_cast<int>(k / nq), q = static_cast<int>(k % nq);
-    double x, y;
-    if (nv == 4)
-    {
-        const double *g = geom + 8 * static_cast<int64_t>(e);
-        const double xi = qxi[q], eta = qeta[q];
-        const double a0 = (1 - xi) * (1 - eta), a1 = xi * (1 - eta), a2 = xi * eta, a3 = (1 - xi) * eta;
-        x = a0 * g[0] + a1 * g[2] + a2 * g[4] + a3 * g[6];
-        y = a0 * g[1] + a1 * g[3] + a2 * g[5] + a3 * g[7];
-    }
-    else
-    {
-        const double *g = geom + 6 * static_cast<int64_t>(e);
-        x = g[0] + g[2] * qxi[q] + g[3] * qeta[q];
-        y = g[1] + g[4] * qxi[q] + g[5] * qeta[q];
-    }
     double reg[MAX_REGS];
     int res = 0;
     for (int s = 0; s < ninstr; ++s)
@@ -268,6 +323,8 @@ __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, con
         case 0: v = consts[a]; break;
         case 1: v = x; break;
         case 2: v = y; break;
+        case 3: v = (a < 0 ? -1.0 : 1.0) * nx; break;
+        case 4: v = (a < 0 ? -1.0 : 1.0) * ny; break;
         case 6: v = reg[a] + reg[b]; break;
         case 7: v = reg[a] * reg[b]; break;
         case 8:
@@ -292,7 +349,46 @@ __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, con
         reg[o] = v;
         res = o;
     }
-    out[k] = reg[res];
+    return reg[res];
+}
+
+// The program at given points (boundary facets: x, y and the outward normal per point).
+__global__ __launch_bounds__(TPB) void k_coefficient_pts(int64_t np, const double *__restrict__ xs,
+                                                         const double *__restrict__ ys, const double *__restrict__ nxs,
+                                                         const double *__restrict__ nys, int ninstr,
+                                                         const int *__restrict__ code, const double *__restrict__ consts,
+                                                         double *__restrict__ out)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    out[k] = run_program(xs[k], ys[k], nxs[k], nys[k], ninstr, code, consts);
+}
+
+__global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, const double *__restrict__ geom,
+                                                     const double *__restrict__ qxi, const double *__restrict__ qeta,
+                                                     int ninstr, const int *__restrict__ code, int nreg,
+                                                     const double *__restrict__ consts, double *__restrict__ out)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<int64_t>(nc) * nq) return;
+    const int e = static_cast<int>(k / nq), q = static_cast<int>(k % nq);
+    double x, y;
+    if (nv == 4)
+    {
+        const double *g = geom + 8 * static_cast<int64_t>(e);
+        const double xi = qxi[q], eta = qeta[q];
+        const double a0 = (1 - xi) * (1 - eta), a1 = xi * (1 - eta), a2 = xi * eta, a3 = (1 - xi) * eta;
+        x = a0 * g[0] + a1 * g[2] + a2 * g[4] + a3 * g[6];
+        y = a0 * g[1] + a1 * g[3] + a2 * g[5] + a3 * g[7];
+    }
+    else
+    {
+        const double *g = geom + 6 * static_cast<int64_t>(e);
+        x = g[0] + g[2] * qxi[q] + g[3] * qeta[q];
+        y = g[1] + g[4] * qxi[q] + g[5] * qeta[q];
+    }
+    (void)nreg;
+    out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts);
 }
 
 // ============================================================================ SpMV
@@ -786,7 +882,67 @@ bool launch_elem_vector(int nv, int nloc, int nc, int nq, const double *geom, co
 
 void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s)
 {
-    if (n > 0) FEMD_LAUNCH(k_gather, grid_for(n), TPB, 0, s, n, ptr, list, src, dst, plus_zero ? 1 : 0);
+    if (n > 0)
+        FEMD_LAUNCH(k_gather, grid_for(n), TPB, 0, s, n, static_cast<const int *>(nullptr), ptr, list, src, dst,
+                    plus_zero ? GATHER_SET_PZ : GATHER_SET);
+}
+
+void launch_gather_mode(int64_t n, const int *pos, const int *ptr, const int *list, const double *src, double *dst, int mode,
+                        hipStream_t s)
+{
+    if (n > 0) FEMD_LAUNCH(k_gather, grid_for(n), TPB, 0, s, n, pos, ptr, list, src, dst, mode);
+}
+
+#define FACET_DISPATCH(KERNEL, N_)                                                                    \
+    case N_:                                                                                         \
+        FEMD_LAUNCH((KERNEL<N_>), grid_for(static_cast<int64_t>(ne) * N_), TPB, 0, s, ne, nq, var, w, Ji, tab, T, out); \
+        break;
+
+bool launch_facet_matrix(int nloc, int ne, int nq, const int *var, const double *w, const double *Ji, const double *tab,
+                         const TermSet &T, double *out, hipStream_t s)
+{
+    if (ne == 0) return true;
+    switch (nloc)
+    {
+        FACET_DISPATCH(k_facet_matrix, 3)
+        FACET_DISPATCH(k_facet_matrix, 4)
+        FACET_DISPATCH(k_facet_matrix, 6)
+        FACET_DISPATCH(k_facet_matrix, 9)
+        FACET_DISPATCH(k_facet_matrix, 10)
+        FACET_DISPATCH(k_facet_matrix, 15)
+        FACET_DISPATCH(k_facet_matrix, 16)
+        FACET_DISPATCH(k_facet_matrix, 21)
+        FACET_DISPATCH(k_facet_matrix, 25)
+    default: return false;
+    }
+    return true;
+}
+
+bool launch_facet_vector(int nloc, int ne, int nq, const int *var, const double *w, const double *Ji, const double *tab,
+                         const TermSet &T, double *out, hipStream_t s)
+{
+    if (ne == 0) return true;
+    switch (nloc)
+    {
+        FACET_DISPATCH(k_facet_vector, 3)
+        FACET_DISPATCH(k_facet_vector, 4)
+        FACET_DISPATCH(k_facet_vector, 6)
+        FACET_DISPATCH(k_facet_vector, 9)
+        FACET_DISPATCH(k_facet_vector, 10)
+        FACET_DISPATCH(k_facet_vector, 15)
+        FACET_DISPATCH(k_facet_vector, 16)
+        FACET_DISPATCH(k_facet_vector, 21)
+        FACET_DISPATCH(k_facet_vector, 25)
+    default: return false;
+    }
+    return true;
+}
+#undef FACET_DISPATCH
+
+void launch_coefficient_pts(int64_t np, const double *x, const double *y, const double *nx, const double *ny, int ninstr,
+                            const int *code, const double *consts, double *out, hipStream_t s)
+{
+    if (np > 0) FEMD_LAUNCH(k_coefficient_pts, grid_for(np), TPB, 0, s, np, x, y, nx, ny, ninstr, code, consts, out);
 }
 
 void launch_coefficient(int nc, int nq, int nv, const double *geom, const double *qxi, const double *qeta, int ninstr,

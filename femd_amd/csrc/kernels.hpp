@@ -26,6 +26,18 @@ bool launch_elem_matrix(int nv, int nloc, int nc, int nq, const double *geom, co
 bool launch_elem_vector(int nv, int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
                         const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s);
 void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s);
+// How a gather slot combines its element-order sum with the destination (k_gather).
+enum { GATHER_SET = 0, GATHER_SET_PZ = 1, GATHER_ADD = 2, GATHER_CONTINUE = 3 };
+void launch_gather_mode(int64_t n, const int *pos, const int *ptr, const int *list, const double *src, double *dst, int mode,
+                        hipStream_t s);
+// Boundary-facet element matrices / vectors (false: no kernel for this local size).
+bool launch_facet_matrix(int nloc, int ne, int nq, const int *var, const double *w, const double *Ji, const double *tab,
+                         const TermSet &T, double *out, hipStream_t s);
+bool launch_facet_vector(int nloc, int ne, int nq, const int *var, const double *w, const double *Ji, const double *tab,
+                         const TermSet &T, double *out, hipStream_t s);
+// A coefficient program at np given points with their normals.
+void launch_coefficient_pts(int64_t np, const double *x, const double *y, const double *nx, const double *ny, int ninstr,
+                            const int *code, const double *consts, double *out, hipStream_t s);
 void launch_coefficient(int nc, int nq, int nv, const double *geom, const double *qxi, const double *qeta, int ninstr,
                         const int *code, int nreg, const double *consts, double *out, hipStream_t s);
 

@@ -90,6 +90,18 @@ x = Spatial(0)
 y = Spatial(1)
 
 
+class Normal(Expr):
+    """A component of the outward unit normal on the exterior facets, for ds terms
+    (python/femd/forms.py:101-104, 634-639)."""
+    def __init__(self, index: int): self.index = int(index)
+    def __repr__(self): return f"n[{self.index}]"
+
+
+def FacetNormal(mesh=None):
+    """The outward unit normal (n[0], n[1]) on the exterior facets, for ds terms."""
+    return ListTensor([Normal(0), Normal(1)])
+
+
 class Sum(Expr):
     def __init__(self, terms): self.terms = list(terms)
     def __repr__(self): return "(" + " + ".join(map(repr, self.terms)) + ")"
@@ -246,31 +258,48 @@ inner = dot
 
 # ============================================================================ measures
 class Measure:
-    def __init__(self, kind, quad_degree=None):
-        self.kind, self.quad_degree = kind, quad_degree
+    """dx (cells), ds (exterior facets: every one, or those with the given side markers) and dS
+    (interior facets, outside this path); forms.py:1269-1309."""
+    def __init__(self, kind, where=None, quad_degree=None):
+        self.kind, self.where, self.quad_degree = kind, where, quad_degree
 
-    def __call__(self, quad_degree=None, **kw):
-        if self.kind != "dx":
-            raise NotImplementedError("boundary (ds) and interior-facet (dS) integrals are outside this path")
+    def __call__(self, *args, quad_degree=None, **kw):
+        if self.kind == "dS":
+            raise NotImplementedError("interior-facet (dS) integrals are outside this path")
         if kw:
-            raise NotImplementedError("dx options other than the degree are outside this path")
-        if quad_degree is not None and int(quad_degree) < 1:
-            raise ValueError(f"dx(n): n must be at least 1, got {quad_degree}")
-        return Measure("dx", quad_degree)
+            raise NotImplementedError(f"{self.kind} options other than the degree are outside this path")
+        if self.kind == "dx":
+            if args:
+                quad_degree = args[0]
+            if quad_degree is not None and int(quad_degree) < 1:
+                raise ValueError(f"dx(n): n must be at least 1, got {quad_degree}")
+            return Measure("dx", None, quad_degree)
+        where = args[0] if args else None
+        if len(args) > 1:
+            where = tuple(args)
+        if isinstance(where, (int, np.integer)) and not isinstance(where, bool):
+            return Measure("ds", (int(where),), quad_degree)
+        if isinstance(where, (tuple, list)) and where and all(isinstance(m, (int, np.integer)) for m in where):
+            return Measure("ds", tuple(sorted({int(m) for m in where})), quad_degree)
+        if where in ("left", "right"):
+            raise TypeError(f"ds('{where}') is for 1D meshes; on a 2D mesh use ds(marker), ds((m1, m2)) or ds")
+        if where not in (None, "both"):
+            raise ValueError("ds takes a marker or markers on a 2D mesh")
+        return Measure("ds", "both", quad_degree)
 
     def __rmul__(self, e):
         return Integral(_lift(e), self)
 
 
 dx = Measure("dx")
-ds = Measure("ds")
+ds = Measure("ds", "both")
 dS = Measure("dS")
 
 
 class Integral:
     def __init__(self, expr, measure):
-        if measure.kind != "dx":
-            raise NotImplementedError("boundary (ds) and interior-facet (dS) integrals are outside this path")
+        if measure.kind == "dS":
+            raise NotImplementedError("interior-facet (dS) integrals are outside this path")
         self.expr, self.measure = expr, measure
 
     def __add__(self, o): return FormExpr([self]) + o
@@ -334,7 +363,9 @@ class Term:
     def rank(self): return (self.test is not None) + (self.trial is not None)
 
     def __repr__(self):
-        return "*".join([repr(self.coeff)] + [repr(a) for a in (self.test, self.trial) if a is not None]) + "*dx"
+        m = self.measure
+        where = "" if m.kind == "dx" or m.where == "both" else f"({', '.join(map(str, m.where))})"
+        return "*".join([repr(self.coeff)] + [repr(a) for a in (self.test, self.trial) if a is not None]) + f"*{m.kind}{where}"
 
 
 def _trivially_zero(e):
@@ -368,6 +399,7 @@ def poly_degree(e):
     """Polynomial degree of a pointwise expression, None if not polynomial (forms.py:1429-1448)."""
     if isinstance(e, (Const, Constant)): return 0
     if isinstance(e, Spatial): return 1
+    if isinstance(e, Normal): return 0
     if isinstance(e, Argument): return _arg_degree(e)
     if isinstance(e, Sum):
         ds_ = [poly_degree(t) for t in e.terms]
@@ -459,6 +491,8 @@ class Program:
             r = self._reg(); self.code.append((self.CONST, r, self._const(e), 0)); return r
         if isinstance(e, Spatial):
             r = self._reg(); self.code.append((self.X if e.index == 0 else self.Y, r, 0, 0)); return r
+        if isinstance(e, Normal):
+            r = self._reg(); self.code.append((self.NX if e.index == 0 else self.NY, r, 1, 0)); return r
         if isinstance(e, Sum):
             if not e.terms:
                 r = self._reg(); self.code.append((self.CONST, r, self._const(0.0), 0)); return r
@@ -485,6 +519,52 @@ class Program:
         """(code int32 (n, 4), nreg, consts float64) with the Constants' current values."""
         consts = [c.value if isinstance(c, Constant) else c for c in self.consts]
         return np.asarray(self.code, dtype=np.int32).reshape(-1, 4), max(self.nreg, 1), np.asarray(consts, dtype=np.float64)
+
+
+_NP_FUNCS = {"sin": np.sin, "cos": np.cos, "exp": np.exp, "log": np.log, "tanh": np.tanh, "sqrt": np.sqrt,
+             "sinh": np.sinh, "cosh": np.cosh, "sech": lambda v: 1.0 / np.cosh(v), "abs": np.abs, "sign": np.sign}
+
+
+def evaluate(e, x, y):
+    """An expression in fd.x, fd.y, numbers and Constants at the points (x, y), with NumPy (the
+    boundary data of V.lift; python/femd/forms.py evaluate)."""
+    if isinstance(e, (Const, Constant)): return e.value
+    if isinstance(e, Spatial): return x if e.index == 0 else y
+    if isinstance(e, Sum):
+        out = 0.0
+        for t in e.terms:
+            out = out + evaluate(t, x, y)
+        return out
+    if isinstance(e, Prod):
+        out = 1.0
+        for f in e.factors:
+            out = out * evaluate(f, x, y)
+        return out
+    if isinstance(e, Pow): return np.power(evaluate(e.base, x, y), e.n)
+    if isinstance(e, Func): return _NP_FUNCS[e.name](evaluate(e.arg, x, y))
+    raise TypeError(f"boundary data: an expression may contain fd.x, fd.y, numbers and Constants, not {e!r}")
+
+
+def _has_normal(e):
+    if isinstance(e, Normal): return True
+    if isinstance(e, Sum): return any(_has_normal(t) for t in e.terms)
+    if isinstance(e, Prod): return any(_has_normal(f) for f in e.factors)
+    if isinstance(e, Pow): return _has_normal(e.base)
+    if isinstance(e, Func): return _has_normal(e.arg)
+    return False
+
+
+def _rebind(f, U):
+    """The same form with its test and trial functions on space U (same mesh and degree)."""
+    def arg(a):
+        return None if a is None else Argument(a.kind, U, a.deriv)
+    g = Form.__new__(Form)
+    g.__dict__.update(f.__dict__)
+    g.terms = [Term(arg(t.test), arg(t.trial), t.coeff, t.measure) for t in f.terms]
+    g.test_space = U
+    g.trial_space = U if f.trial_space is not None else None
+    g._result = None
+    return g
 
 
 # ============================================================================ the Form
@@ -517,56 +597,103 @@ class Form:
                 if a is not None and sum(a.deriv) > 1:
                     raise ValueError(f"{a!r}: derivatives of order {sum(a.deriv)} are not available on a C^0 "
                                      "Lagrange space; integrate by parts (only first derivatives enter a form)")
-        # quadrature degree (forms.py:2487-2498)
-        kmax = max(sp.degree for sp in spaces.values())
-        deg = 0
         for t in self.terms:
+            if t.measure.kind == "dx" and _has_normal(t.coeff):
+                raise TypeError("the facet normal n appears in a dx term; it exists on ds terms only")
+        present = set(np.asarray(self.test_space.mesh.segment_markers).tolist())
+        for t in self.terms:
+            if t.measure.kind == "ds" and t.measure.where != "both":
+                missing = [m for m in t.measure.where if m not in present]
+                if missing:
+                    raise ValueError(f"ds({', '.join(str(m) for m in t.measure.where)}): no side of the mesh carries "
+                                     f"marker(s) {missing}; the markers present are {sorted(present)}")
+        # quadrature degrees per measure (forms.py:2487-2500)
+        kmax = max(sp.degree for sp in spaces.values())
+        deg = {"dx": 0, "ds": 0}
+        for t in self.terms:
+            k = t.measure.kind
             if t.measure.quad_degree is not None:
-                deg = max(deg, int(t.measure.quad_degree)); continue
+                deg[k] = max(deg[k], int(t.measure.quad_degree)); continue
             if _default_quad_degree is not None:
-                deg = max(deg, _default_quad_degree); continue
+                deg[k] = max(deg[k], _default_quad_degree); continue
             d = poly_degree(t.coeff)
             arg_d = sum(_arg_degree(a) for a in (t.test, t.trial) if a is not None)
             if d is None:
                 d = quad_degree_estimate(t.coeff, kmax + 1)
-            deg = max(deg, arg_d + d)
-        self.quad_degree = max(deg, 1)
+            deg[k] = max(deg[k], arg_d + d)
+        self.quad_degree = max(deg["dx"], 1)
+        self.facet_degree = max(deg["ds"], 1)
+        # the point sets in order of first use: dx, or ds(where) (forms.py:2313-2336); the groups
+        # add into one matrix (vector) in this order, as FormAssembler2D does
+        self.groups = {}
+        for i, t in enumerate(self.terms):
+            key = ("dx",) if t.measure.kind == "dx" else ("ds", t.measure.where)
+            self.groups.setdefault(key, []).append(i)
         self.programs = [Program(t.coeff) for t in self.terms]
         self.constant = not any(isinstance(c, Constant) for p in self.programs for c in p.consts)
         self._result = None
 
-    def _coefficients(self):
+    def _coefficients(self, idx=None):
         items = []
-        for p in self.programs:
-            code, nreg, consts = p.arrays()
+        for i in (range(len(self.terms)) if idx is None else idx):
+            code, nreg, consts = self.programs[i].arrays()
             items.append(((code, nreg, consts), None))
         return L.coefficient_array(items)
 
     def assemble(self, reuse: bool = True):
         """Rank 2: a SparseMatrix on the space's pattern (symmetric flag set as forms.py:2093-2094);
-        rank 1: the vector (numpy, float64).  Assembled on the GPU.  A form without Constants is
-        assembled once and the result reused, as the reference does (forms.py:2752-2760), unless
-        reuse=False."""
+        rank 1: the vector (numpy, float64).  Assembled on the GPU, one group of terms per point
+        set (dx, ds(markers)) in the form's order, each adding into the same matrix (vector) as
+        FormAssembler2D does (forms/assembler_2d.hpp:166-188, 249-266).  A form without Constants
+        is assembled once and the result reused, as the reference does (forms.py:2752-2760),
+        unless reuse=False."""
         if reuse and self.constant and self._result is not None:
             return self._result
         V = self.test_space
-        coeffs, keep = self._coefficients()
-        a = L.i32([t.test.code() for t in self.terms])
+        lib = L.load()
         if self.rank == 1:
-            f = np.empty(V.dim)
-            L.check(L.load().femd_assemble_vector(V._h, self.quad_degree, len(self.terms), a, coeffs, f))
-            out = f
+            out = np.empty(V.dim)
         else:
             from .sparse import SparseMatrix
-            b = L.i32([t.trial.code() for t in self.terms])
-            K = SparseMatrix._on_space(V)
-            L.check(L.load().femd_assemble_matrix(V._h, self.quad_degree, len(self.terms), a, b, coeffs, K._h))
-            K.symmetric = bool(K.asymmetry() <= 1e-13)
-            out = K
-        del keep
+            out = SparseMatrix._on_space(V)
+        for g, (key, idx) in enumerate(self.groups.items()):
+            coeffs, keep = self._coefficients(idx)
+            a = L.i32([self.terms[i].test.code() for i in idx])
+            b = L.i32([self.terms[i].trial.code() for i in idx]) if self.rank == 2 else None
+            if key[0] == "dx":
+                if self.rank == 1:
+                    fn = lib.femd_assemble_vector if g == 0 else lib.femd_assemble_vector_add
+                    L.check(fn(V._h, self.quad_degree, len(idx), a, coeffs, out))
+                else:
+                    fn = lib.femd_assemble_matrix if g == 0 else lib.femd_assemble_matrix_add
+                    L.check(fn(V._h, self.quad_degree, len(idx), a, b, coeffs, out._h))
+            else:
+                F = V.facet_set(self.facet_degree, None if key[1] == "both" else key[1])
+                if self.rank == 1:
+                    L.check(lib.femd_assemble_vector_facets(F, len(idx), a, coeffs, out, 0 if g == 0 else 1))
+                else:
+                    L.check(lib.femd_assemble_matrix_facets(F, len(idx), a, b, coeffs, out._h, 0 if g == 0 else 1))
+            del keep
+        if self.rank == 2:
+            out.symmetric = bool(out.asymmetry() <= 1e-13)
         if self.constant:
             self._result = out
         return out
+
+    def action(self, u):
+        """The linear form v -> a(u, v) of a bilinear form for a known function u given by its
+        values on the space's unconstrained nodes (raw numbering), assembled: a vector over the
+        free nodes.  The reference assembles a.action(u) with u as a field; this forms the same
+        sums through the matrix of the unconstrained space (equal to rounding)."""
+        if self.rank != 2:
+            raise TypeError("action() is for bilinear forms")
+        V = self.test_space
+        U = V.unconstrained
+        u = np.asarray(u, dtype=np.float64).reshape(-1)
+        if u.size != U.dim:
+            raise ValueError(f"action(): u has {u.size} values, the unconstrained space has {U.dim}")
+        KU = _rebind(self, U).assemble()
+        return KU.matvec(u)[V.adapted_to_raw()]
 
     def __repr__(self):
         return f"Form(rank {self.rank}, {len(self.terms)} terms, quadrature degree {self.quad_degree}: {self.terms})"

@@ -81,9 +81,36 @@ class LagrangeSpace2D:
         h, self._h = getattr(self, "_h", None), None
         if h:
             try:
-                L.load().femd_space_free(h)
+                lib = L.load()
+                for f in getattr(self, "_facet_sets", {}).values():
+                    lib.femd_facets_free(f)
+                lib.femd_space_free(h)
             except Exception:
                 pass
+
+    def facet_set(self, degree, markers=None):
+        """The library's ds point set of this space (cached): exterior facets with a marker in
+        `markers`, every exterior facet for None, Gauss-Legendre points exact to `degree`
+        (Cache2D on facets, fe/cache_2d.hpp:124-183; boundary_cache, spaces2d.py:265-270)."""
+        cache = self.__dict__.setdefault("_facet_sets", {})
+        key = (int(degree), None if markers is None else tuple(sorted({int(m) for m in markers})))
+        h = cache.get(key)
+        if h is None:
+            m = L.i32(key[1]) if key[1] else np.zeros(1, dtype=np.int32)
+            out = C.c_void_p()
+            L.check(L.load().femd_facets_create(self._h, key[0], len(key[1]) if key[1] else 0, m, int(key[1] is None),
+                                                C.byref(out)))
+            h = cache[key] = out.value
+        return h
+
+    def boundary_points(self, degree, markers=None):
+        """(x, y, w, nx, ny) per (facet, point) of facet_set(degree, markers)."""
+        h = self.facet_set(degree, markers)
+        sz = np.zeros(2, dtype=np.int32)
+        L.check(L.load().femd_facets_sizes(h, sz))
+        out = [np.empty(int(sz[0]) * int(sz[1])) for _ in range(5)]
+        L.check(L.load().femd_facets_points(h, *out))
+        return tuple(out)
 
     @property
     def n_constraints(self) -> int:
@@ -129,6 +156,97 @@ class LagrangeSpace2D:
     def restrict(self, raw) -> np.ndarray:
         return np.asarray(raw, dtype=np.float64)[self.adapted_to_raw()]
 
+    # ---- Dirichlet data (spaces2d.py:212-220, 239-248, 373-414) -------------------------------
+    @property
+    def unconstrained(self):
+        """The same mesh, degree and nodes with nothing eliminated (created once); self if nothing is."""
+        if self.n_constraints == 0:
+            return self
+        U = self.__dict__.get("_unconstrained")
+        if U is None:
+            U = type(self)(self.mesh, self.degree, nodes=self.node_set)
+            self._unconstrained = U
+        return U
+
+    def constrained(self) -> np.ndarray:
+        """Raw indices of the eliminated nodes."""
+        mask = np.ones(self.raw_dim, dtype=bool)
+        mask[self.adapted_to_raw()] = False
+        return np.flatnonzero(mask)
+
+    def facets(self):
+        """The facet list of fe/space_2d.hpp:432-448 -- every exterior edge and every interior edge
+        that is a marked segment (once), cells ascending, local edges ascending: arrays cell,
+        local edge, marker (0 for none), exterior."""
+        m = self.mesh
+        T = m.triangles
+        nv = T.shape[1]
+        va = np.arange(nv) if nv == 4 else (np.arange(3) + 1) % 3
+        vb = (np.arange(nv) + 1) % 4 if nv == 4 else (np.arange(3) + 2) % 3
+        A, B = T[:, va], T[:, vb]
+        lo, hi = np.minimum(A, B).astype(np.int64), np.maximum(A, B).astype(np.int64)
+        key = (lo * m.npoints + hi).ravel()
+        seg = m.segments.astype(np.int64)
+        skey = np.minimum(seg[:, 0], seg[:, 1]) * m.npoints + np.maximum(seg[:, 0], seg[:, 1])
+        mark = np.full(key.size, -1, dtype=np.int64)
+        if skey.size:
+            order = np.argsort(skey, kind="stable")
+            sk = skey[order]
+            pos = np.minimum(np.searchsorted(sk, key), sk.size - 1)
+            hit = sk[pos] == key
+            mark[hit] = np.asarray(m.segment_markers)[order][pos[hit]]
+        ext = (m.neighbours < 0).ravel()
+        keep = ext | (mark >= 0)
+        first = np.zeros(key.size, dtype=bool)                       # an interior segment once, first met
+        _, idx = np.unique(key, return_index=True)
+        first[idx] = True
+        keep &= ext | first
+        sel = np.flatnonzero(keep)
+        return sel // nv, sel % nv, np.where(mark[sel] >= 0, mark[sel], 0), ext[sel]
+
+    def boundary_nodes(self, marker=None) -> np.ndarray:
+        """Raw indices of the nodes on the sides with this marker (every exterior side for None):
+        the edges' vertices and edge nodes."""
+        cell, local, mark, ext = self.facets()
+        sel = ext if marker is None else np.isin(mark, _markers(marker, "marker"))
+        nv, k = self.nverts, self.degree
+        cr = self.cell_raw_dofs()
+        va = local[sel] if nv == 4 else (local[sel] + 1) % 3
+        vb = (local[sel] + 1) % 4 if nv == 4 else (local[sel] + 2) % 3
+        parts = [cr[cell[sel], va], cr[cell[sel], vb]]
+        if k > 1:
+            l = nv + local[sel][:, None] * (k - 1) + np.arange(k - 1)[None, :]
+            parts.append(cr[cell[sel][:, None], l].ravel())
+        return np.unique(np.concatenate(parts)).astype(np.int64)
+
+    def lift(self, g=None):
+        """The raw vector (a function of V.unconstrained) carrying Dirichlet data on the eliminated
+        nodes: a number or a callable g(x, y) / expression in fd.x, fd.y for every Dirichlet side,
+        or a dict {marker: ...}; sides left out get zero data, and where two sides meet the corner
+        takes the data of the larger marker (spaces2d.py:373-414)."""
+        raw = np.zeros(self.raw_dim)
+        cons = self.constrained()
+        if g is None or cons.size == 0:
+            if g is not None and cons.size == 0 and not _is_zero(g):
+                raise ValueError("lift(): this space has no Dirichlet condition built in; natural data enter "
+                                 "through ds(marker)")
+            return raw
+        xy = self.raw_coordinates()
+        if isinstance(g, dict):
+            allowed = set(self.dirichlet_markers)
+            if self.all_exterior:
+                allowed |= set(np.asarray(self.mesh.segment_markers).tolist()) | {0}
+            for mk in sorted(g):
+                if int(mk) not in allowed:
+                    raise ValueError(f"lift(): marker {mk} carries no Dirichlet condition in this space "
+                                     f"(Dirichlet markers {sorted(allowed)}); natural data enter through ds({mk})")
+                nodes = self.boundary_nodes(int(mk))
+                nodes = nodes[np.isin(nodes, cons)]
+                raw[nodes] = _values(g[mk], xy[nodes])
+        else:
+            raw[cons] = _values(g, xy[cons])
+        return raw
+
     def interpolate(self, f) -> np.ndarray:
         """Nodal values of a callable f(x, y) at the degrees of freedom (the coefficients)."""
         xy = self.dof_coordinates()
@@ -159,6 +277,21 @@ class LagrangeSpace2D:
     def __repr__(self):
         name = "LagrangeSpaceQ" if self.nverts == 4 else "LagrangeSpace2D"
         return f"{name}(degree={self.degree}, ncells={self.ncells}, bc={self.bc_description}, dim={self.dim})"
+
+
+def _is_zero(g):
+    return isinstance(g, (int, float)) and g == 0
+
+
+def _values(g, xy):
+    """Boundary data at the nodes xy: an expression in fd.x, fd.y and Constants, a callable g(x, y),
+    or a number (spaces2d.py:540-555)."""
+    from .forms import Expr, evaluate
+    if isinstance(g, Expr):
+        return np.broadcast_to(np.asarray(evaluate(g, xy[:, 0], xy[:, 1]), dtype=np.float64), (xy.shape[0],))
+    if callable(g):
+        return np.broadcast_to(np.asarray(g(xy[:, 0], xy[:, 1]), dtype=np.float64), (xy.shape[0],))
+    return float(g)
 
 
 def LagrangeSpace(mesh, degree, bc="free", dirichlet=None, nodes="equispaced"):

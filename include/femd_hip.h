@@ -50,6 +50,7 @@ typedef struct femd_mesh_s *femd_mesh;
 typedef struct femd_space_s *femd_space;
 typedef struct femd_csr_s *femd_csr;
 typedef struct femd_precond_s *femd_precond;
+typedef struct femd_facets_s *femd_facets;
 typedef void *femd_stream;          /* a hipStream_t */
 
 /* The thread's last error message (empty after success). */
@@ -181,6 +182,40 @@ int femd_assemble_vector_dev(femd_space V, int quad_degree, int nterms, const in
                              const femd_coefficient *coeffs, double *f_dev, femd_stream s);
 /* The coefficient program at the quadrature points of the space (ncells x nq, host). */
 int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient *c, double *out);
+/* femd_assemble_matrix / femd_assemble_vector for a dx group that follows another group of the
+ * same form: every stored entry continues its sum (K[p] += each element entry in order), the
+ * vector adds the group's own sum (f[i] += block[i]) -- how FormAssembler2D combines its groups
+ * (forms/assembler_2d.hpp:166-188, 249-266).  No separate entry point in the reference. */
+int femd_assemble_matrix_add(femd_space V, int quad_degree, int nterms, const int *a, const int *b,
+                             const femd_coefficient *coeffs, femd_csr K);
+int femd_assemble_vector_add(femd_space V, int quad_degree, int nterms, const int *a,
+                             const femd_coefficient *coeffs, double *f);
+
+/* ---------------------------------------------------------------- boundary facets (ds)
+ * The exterior facets of V whose side marker is one of markers[0..nmarkers) (every exterior facet
+ * with all != 0), in the space's facet order, with Gauss-Legendre points exact to `degree` along
+ * each edge: the point set of Cache2D(V, degree, markers, all) (fe/cache_2d.hpp:124-183), which
+ * LagrangeSpace.boundary_cache builds (python/femd/spaces2d.py:265-270) for ds(markers) terms
+ * (python/femd/forms.py:2502-2524).  The set keeps V's address: free it before V. */
+int femd_facets_create(femd_space V, int degree, int nmarkers, const int *markers, int all, femd_facets *out);
+/* out[0] facets, out[1] points per facet. */
+int femd_facets_sizes(femd_facets F, int out[2]);
+/* Per (facet, point), facet-major: physical point, weight (reference weight x half the edge
+ * length), outward unit normal.  Any pointer may be NULL. */
+int femd_facets_points(femd_facets F, double *x, double *y, double *w, double *nx, double *ny);
+void femd_facets_free(femd_facets F);
+/* K += sum_t integral over the facets of c_t (D^{a_t} phi_i)(D^{b_t} phi_j) ds, each facet's
+ * element matrix formed and added in ascending facet order (assemble_matrix_2d on a facet cache,
+ * forms/assembler_2d.hpp:249-266).  mode 0 zeroes K first (the form's first group), 1 continues
+ * K's sums (a later group).  Coefficient programs may read the outward normal (NX, NY). */
+int femd_assemble_matrix_facets(femd_facets F, int nterms, const int *a, const int *b,
+                                const femd_coefficient *coeffs, femd_csr K, int mode);
+/* f (dim, host) += the group's vector sum_t integral c_t D^{a_t} phi_i ds (assemble_vector_2d on a
+ * facet cache, then added as forms/assembler_2d.hpp:166-188 does); mode 0 zeroes f first. */
+int femd_assemble_vector_facets(femd_facets F, int nterms, const int *a, const femd_coefficient *coeffs,
+                                double *f, int mode);
+/* The coefficient program at the facet points (nfacets x nq, host), normals available. */
+int femd_eval_coefficient_facets(femd_facets F, const femd_coefficient *c, double *out);
 
 /* ---------------------------------------------------------------- SpMV
  * y = A x (CSRMatrix::apply, sparse/csr.hpp:203-215, bound as CSRMatrix.matvec): each row

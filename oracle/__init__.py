@@ -74,6 +74,50 @@ class _Lib:
         self._check(self._lgl(int(p), out))
         return out
 
+    @staticmethod
+    def _marks(markers):
+        all_ = markers is None
+        m = _i32(list(markers)) if markers is not None and len(markers) else np.zeros(1, dtype=np.int32)
+        return (0 if all_ else len(markers)), m, int(all_)
+
+    def facet_points(self, P, degree, markers=None):
+        """(x, y, w, nx, ny) of the ds point set: markers None = every exterior facet."""
+        nm, m, all_ = self._marks(markers)
+        sz = np.zeros(2, dtype=np.int32)
+        self._check(self._fsz(P.h, int(degree), nm, m, all_, sz))
+        out = [np.empty(int(sz[0]) * int(sz[1])) for _ in range(5)]
+        self._check(self._fpts(P.h, int(degree), nm, m, all_, *out))
+        return tuple(out)
+
+    def _groups(self, P, groups):
+        """groups: [(kind 'dx' | 'ds', degree, markers or None, a, b or None, coeffs (nterms x npts))]."""
+        kind = _i32([0 if g[0] == "dx" else 1 for g in groups])
+        deg = _i32([g[1] for g in groups])
+        nmark, marks, all_ = [], [], []
+        for g in groups:
+            if g[0] == "ds" and g[2] is not None:
+                nmark.append(len(g[2])); marks.extend(g[2]); all_.append(0)
+            else:
+                nmark.append(0); all_.append(1 if g[0] == "ds" else 0)
+        nterms = _i32([len(g[3]) for g in groups])
+        a = _i32([x for g in groups for x in g[3]])
+        b = _i32([x for g in groups for x in (g[4] if g[4] is not None else [0] * len(g[3]))])
+        co = _f64(np.concatenate([np.asarray(g[5], dtype=np.float64).reshape(-1) for g in groups]))
+        return (len(groups), kind, deg, _i32(nmark), _i32(marks) if marks else np.zeros(1, dtype=np.int32), _i32(all_), nterms,
+                a, b, co)
+
+    def assemble_matrix_groups(self, P, groups):
+        n, kind, deg, nmark, marks, all_, nterms, a, b, co = self._groups(P, groups)
+        out = np.empty(int(self._pnnz(P.h)))
+        self._check(self._amg(P.h, n, kind, deg, nmark, marks, all_, nterms, a, b, co, out))
+        return out
+
+    def assemble_vector_groups(self, P, groups):
+        n, kind, deg, nmark, marks, all_, nterms, a, b, co = self._groups(P, groups)
+        out = np.empty(P.dim)
+        self._check(self._avg(P.h, n, kind, deg, nmark, marks, all_, nterms, a, co, out))
+        return out
+
     # -- shared CSR / Krylov entry points (same signatures in both libraries) ------------
     def _declare(self):
         self._spmv = self._fn("spmv", _I, _I, _ip, _ip, _dp, _dp, _dp)
@@ -201,6 +245,11 @@ class Oracle(_Lib):
         self._qbasis = self._fn("basis_eval_quad", _I, _I, _I, _D, _D, _dp)
         self._nodes = self._fn("ref_nodes", _I, _I, _I, _I, _dp, _dp)
         self._lgl = self._fn("lobatto_points", _I, _I, _dp)
+        self._fsz = self._fn("facet_sizes", _I, _P, _I, _I, _ip, _I, _ip)
+        self._fpts = self._fn("facet_points", _I, _P, _I, _I, _ip, _I, _dp, _dp, _dp, _dp, _dp)
+        self._amg = self._fn("assemble_matrix_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
+        self._avg = self._fn("assemble_vector_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
+        self._ien = self._fn("integrand_eval_n", _I, _I, _ip, _I, _dp, _L, _dp, _dp, _dp, _dp, _dp)
 
     def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0, equispaced=True):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
@@ -321,6 +370,14 @@ class Oracle(_Lib):
         self._check(self._ie(code.size // 4, code, int(nreg), _f64(consts) if len(consts) else np.zeros(1), xs.size, xs, ys, out))
         return out
 
+    def integrand_eval_n(self, code, nreg, consts, xs, ys, nxs, nys):
+        code = _i32(code).reshape(-1)
+        xs, ys, nxs, nys = _f64(xs), _f64(ys), _f64(nxs), _f64(nys)
+        out = np.empty(xs.size)
+        self._check(self._ien(code.size // 4, code, int(nreg), _f64(consts) if len(consts) else np.zeros(1), xs.size, xs, ys,
+                              nxs, nys, out))
+        return out
+
     def ilu0_factor(self, ip, ix, v, ordering="rcm"):
         ip, ix, v = _i32(ip), _i32(ix), _f64(v)
         n = ip.size - 1
@@ -359,6 +416,11 @@ class Ref(_Lib):
         self._qbasis = self._fn("quad_basis_eval", _I, _I, _I, _D, _D, _dp)
         self._nodes = self._fn("nodes", _I, _I, _I, _I, _dp, _dp)
         self._lgl = self._fn("lobatto_points", _I, _I, _dp)
+        self._fsz = self._fn("facet_sizes", _I, _P, _I, _I, _ip, _I, _ip)
+        self._fpts = self._fn("facet_points", _I, _P, _I, _I, _ip, _I, _dp, _dp, _dp, _dp, _dp)
+        self._amg = self._fn("assemble_matrix_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
+        self._avg = self._fn("assemble_vector_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
+        self._ien = self._fn("integrand_eval_n", _I, _I, _ip, _I, _I, _dp, _L, _dp, _dp, _dp, _dp, _dp)
 
     def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True,
                          equispaced=True):
@@ -465,6 +527,14 @@ class Ref(_Lib):
         out = np.empty(xs.size)
         self._check(self._ie(code.size // 4, code, int(nreg), len(consts), _f64(consts) if len(consts) else np.zeros(1),
                              xs.size, xs, ys, out))
+        return out
+
+    def integrand_eval_n(self, code, nreg, consts, xs, ys, nxs, nys):
+        code = _i32(code).reshape(-1)
+        xs, ys, nxs, nys = _f64(xs), _f64(ys), _f64(nxs), _f64(nys)
+        out = np.empty(xs.size)
+        self._check(self._ien(code.size // 4, code, int(nreg), len(consts), _f64(consts) if len(consts) else np.zeros(1),
+                              xs.size, xs, ys, nxs, nys, out))
         return out
 
     def asymmetry(self, ip, ix, v):

@@ -822,7 +822,8 @@ void pattern(const Problem &P, std::vector<int> &ip, std::vector<int> &ix)
 
 // ------------------------------------------------------------------------------- integrand
 // forms/integrand.hpp:33-46 (op codes), 300-342 (run), 159-163 (sech)
-double run_program(int ninstr, const int *code, int nreg, const double *consts, double x, double y, std::vector<double> &reg)
+double run_program(int ninstr, const int *code, int nreg, const double *consts, double x, double y, std::vector<double> &reg,
+                   double nx = 0.0, double ny = 0.0)
 {
     reg.assign(static_cast<std::size_t>(nreg), 0.0);
     int res = -1;
@@ -835,6 +836,8 @@ double run_program(int ninstr, const int *code, int nreg, const double *consts, 
         case 0: v = consts[a]; break;
         case 1: v = x; break;
         case 2: v = y; break;
+        case 3: v = (a < 0 ? -1.0 : 1.0) * nx; break;     // integrand.hpp:314-315: the normal, a the side's sign
+        case 4: v = (a < 0 ? -1.0 : 1.0) * ny; break;
         case 6: v = reg[a] + reg[b]; break;
         case 7: v = reg[a] * reg[b]; break;
         case 8:
@@ -936,11 +939,11 @@ void phys_row(const Problem &P, const Quad &Q, int c, int q, int code, double *o
 // forms/assembly_2d.hpp:102-126 (element matrix: terms, points, rows, columns, zero products
 // skipped) and 219-305 (the entries added to the CSR values in ascending element order).
 void assemble_matrix(const Problem &P, const Quad &Q, int nt, const int *a, const int *b, const double *coef,
-                     const std::vector<int> &ip, const std::vector<int> &ix, std::vector<double> &data)
+                     const std::vector<int> &ip, const std::vector<int> &ix, std::vector<double> &data, bool zero = true)
 {
     const int n = P.nloc, nq = Q.nq, nc = P.mesh.nt();
     const std::size_t np = static_cast<std::size_t>(nc) * nq;
-    data.assign(static_cast<std::size_t>(ip.back()), 0.0);
+    if (zero) data.assign(static_cast<std::size_t>(ip.back()), 0.0);
     std::vector<double> Ke(static_cast<std::size_t>(n) * n), RT(static_cast<std::size_t>(3) * nq * n);
     for (int c = 0; c < nc; ++c)
     {
@@ -979,7 +982,7 @@ void assemble_matrix(const Problem &P, const Quad &Q, int nt, const int *a, cons
 }
 
 // forms/assembly_2d.hpp:173-211 and assembler_2d.hpp:166-188 (f = 0 + the group's block)
-void assemble_vector(const Problem &P, const Quad &Q, int nt, const int *a, const double *coef, double *f)
+void assemble_vector(const Problem &P, const Quad &Q, int nt, const int *a, const double *coef, double *f, bool add = false)
 {
     const int n = P.nloc, nq = Q.nq, nc = P.mesh.nt();
     const std::size_t np = static_cast<std::size_t>(nc) * nq;
@@ -999,7 +1002,175 @@ void assemble_vector(const Problem &P, const Quad &Q, int nt, const int *a, cons
         const int *d = &P.cell_dof[static_cast<std::size_t>(c) * n];
         for (int l = 0; l < n; ++l) if (d[l] >= 0) block[d[l]] += fe[l];
     }
-    for (int i = 0; i < P.dim; ++i) f[i] = 0.0 + block[i];
+    for (int i = 0; i < P.dim; ++i) f[i] = add ? f[i] + block[i] : 0.0 + block[i];
+}
+
+// --------------------------------------------------------------------- boundary facets (ds)
+// fe/cache_2d.hpp:124-183: the exterior facets with a marker in `markers` (all of them with
+// `all`), taken from the facet list of fe/space_2d.hpp:432-448 (cells ascending, local edges
+// ascending); Gauss-Legendre points along each edge, max(1, (d + 2) / 2) of them on triangles,
+// (d + 3) / 2 on quadrilaterals; per point the mapped point, 0.5 w_q |edge|, the outward normal
+// (dy, -dx) / |edge| and J^{-1}; the basis tabulated once per local edge.
+struct FacetQuad {
+    int nq = 0, nent = 0;
+    std::vector<int> cell, var;
+    std::vector<double> tab;                      // (edge, point) x 3n
+    std::vector<double> x, y, w, nx, ny, Ji;      // nent x nq (Ji x 4)
+};
+
+FacetQuad make_facet_quad(const Problem &P, int degree, std::vector<int> markers, bool all)
+{
+    std::sort(markers.begin(), markers.end());
+    const Mesh &M = P.mesh;
+    const int nv = M.nv, n = P.nloc;
+    FacetQuad F;
+    std::vector<double> g, gw;
+    gauss_legendre(std::max(1, (degree + (nv == 4 ? 3 : 2)) / 2), g, gw);
+    F.nq = static_cast<int>(g.size());
+    const int nq = F.nq;
+    // reference vertices (lagrange_space_2d.hpp:72-76, lagrange_space_quad.hpp:146-150) and the
+    // local edges' end vertices (edge_vertices)
+    const double tv[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+    const double qv[4][2] = {{0.0, 0.0}, {1.0, 0.0}, {1.0, 1.0}, {0.0, 1.0}};
+    std::vector<double> fx(static_cast<std::size_t>(nv) * nq), fy(fx.size());
+    F.tab.resize(static_cast<std::size_t>(nv) * nq * 3 * n);
+    for (int i = 0; i < nv; ++i)
+    {
+        const int va = nv == 4 ? i : (i + 1) % 3, vb = nv == 4 ? (i + 1) % 4 : (i + 2) % 3;
+        const double *A = nv == 4 ? qv[va] : tv[va], *B = nv == 4 ? qv[vb] : tv[vb];
+        for (int q = 0; q < nq; ++q)
+        {
+            const double t = 0.5 * (g[q] + 1.0);
+            const std::size_t k = static_cast<std::size_t>(i) * nq + q;
+            fx[k] = (1.0 - t) * A[0] + t * B[0];
+            fy[k] = (1.0 - t) * A[1] + t * B[1];
+            P.ref_eval(fx[k], fy[k], &F.tab[k * 3 * n]);
+        }
+    }
+    for (int c = 0; c < M.nt(); ++c)
+        for (int i = 0; i < nv; ++i)
+        {
+            if (M.nbr[static_cast<std::size_t>(c) * nv + i] >= 0) continue;
+            int a = 0, b = 0;
+            M.edge(c, i, a, b);
+            const int m = M.segment_marker(a, b);
+            if (!all && !std::binary_search(markers.begin(), markers.end(), m >= 0 ? m : 0)) continue;
+            F.cell.push_back(c);
+            F.var.push_back(i);
+        }
+    F.nent = static_cast<int>(F.cell.size());
+    const std::size_t np = static_cast<std::size_t>(F.nent) * nq;
+    F.x.resize(np); F.y.resize(np); F.w.resize(np); F.nx.resize(np); F.ny.resize(np); F.Ji.resize(4 * np);
+    for (int e = 0; e < F.nent; ++e)
+    {
+        const int c = F.cell[e], i = F.var[e];
+        int a = 0, b = 0;
+        M.edge(c, i, a, b);
+        const double dx = M.xy[2 * b] - M.xy[2 * a], dy = M.xy[2 * b + 1] - M.xy[2 * a + 1];
+        const double len = std::hypot(dx, dy);
+        for (int q = 0; q < nq; ++q)
+        {
+            const std::size_t k = static_cast<std::size_t>(e) * nq + q;
+            const double xi = fx[static_cast<std::size_t>(i) * nq + q], eta = fy[static_cast<std::size_t>(i) * nq + q];
+            double J[4];
+            if (P.quad()) quad_map(M, c, xi, eta, F.x[k], F.y[k], J);
+            else
+            {
+                std::copy_n(&P.J[4 * static_cast<std::size_t>(c)], 4, J);
+                const double ox = M.xy[2 * M.tri[3 * c]], oy = M.xy[2 * M.tri[3 * c] + 1];
+                F.x[k] = ox + J[0] * xi + J[1] * eta;
+                F.y[k] = oy + J[2] * xi + J[3] * eta;
+            }
+            F.w[k] = 0.5 * gw[q] * len;
+            F.nx[k] = dy / len;
+            F.ny[k] = -dx / len;
+            const double d = J[0] * J[3] - J[1] * J[2];
+            double *Ji = &F.Ji[4 * k];
+            Ji[0] = J[3] / d; Ji[1] = -J[1] / d;
+            Ji[2] = -J[2] / d; Ji[3] = J[0] / d;
+        }
+    }
+    return F;
+}
+
+void facet_row(const Problem &P, const FacetQuad &F, int e, int q, int code, double *out)
+{
+    const int n = P.nloc;
+    const double *R = &F.tab[(static_cast<std::size_t>(F.var[e]) * F.nq + q) * 3 * n];
+    if (code == 0) { std::copy(R, R + n, out); return; }
+    const double *Ji = &F.Ji[4 * (static_cast<std::size_t>(e) * F.nq + q)];
+    const double a = code == 1 ? Ji[0] : Ji[1], b = code == 1 ? Ji[2] : Ji[3];
+    for (int l = 0; l < n; ++l) out[l] = a * R[n + l] + b * R[2 * n + l];
+}
+
+// assemble_matrix_2d on a facet cache: each facet's element matrix (the loop order and zero skips
+// of assembly_2d.hpp:102-126) added into data in ascending facet order (data is not zeroed: a
+// form's groups add into one matrix, assembler_2d.hpp:249-266)
+void assemble_matrix_facets(const Problem &P, const FacetQuad &F, int nt, const int *a, const int *b, const double *coef,
+                            const std::vector<int> &ip, const std::vector<int> &ix, std::vector<double> &data)
+{
+    const int n = P.nloc, nq = F.nq;
+    const std::size_t np = static_cast<std::size_t>(F.nent) * nq;
+    std::vector<double> Ke(static_cast<std::size_t>(n) * n), RT(static_cast<std::size_t>(3) * nq * n);
+    for (int e = 0; e < F.nent; ++e)
+    {
+        for (int m = 0; m < 3; ++m)
+            for (int q = 0; q < nq; ++q) facet_row(P, F, e, q, m, &RT[(static_cast<std::size_t>(m) * nq + q) * n]);
+        std::fill(Ke.begin(), Ke.end(), 0.0);
+        for (int t = 0; t < nt; ++t)
+            for (int q = 0; q < nq; ++q)
+            {
+                const std::size_t kk = static_cast<std::size_t>(e) * nq + q;
+                const double s = F.w[kk] * coef[t * np + kk];
+                if (s == 0.0) continue;
+                const double *ra = &RT[(static_cast<std::size_t>(a[t]) * nq + q) * n];
+                const double *rb = &RT[(static_cast<std::size_t>(b[t]) * nq + q) * n];
+                for (int i = 0; i < n; ++i)
+                {
+                    const double sa = s * ra[i];
+                    if (sa == 0.0) continue;
+                    for (int j = 0; j < n; ++j) Ke[i * n + j] += sa * rb[j];
+                }
+            }
+        const int *d = &P.cell_dof[static_cast<std::size_t>(F.cell[e]) * n];
+        for (int i = 0; i < n; ++i)
+        {
+            if (d[i] < 0) continue;
+            for (int j = 0; j < n; ++j)
+            {
+                if (d[j] < 0) continue;
+                const int *lo = ix.data() + ip[d[i]], *hi = ix.data() + ip[d[i] + 1];
+                const int *p = std::lower_bound(lo, hi, d[j]);
+                if (p == hi || *p != d[j]) throw std::logic_error("assembly: entry outside the pattern");
+                data[p - ix.data()] += Ke[i * n + j];
+            }
+        }
+    }
+}
+
+// assemble_vector_2d on a facet cache into the group's own block, then f += block
+// (assembler_2d.hpp:166-188)
+void assemble_vector_facets(const Problem &P, const FacetQuad &F, int nt, const int *a, const double *coef, double *f)
+{
+    const int n = P.nloc, nq = F.nq;
+    const std::size_t np = static_cast<std::size_t>(F.nent) * nq;
+    std::vector<double> block(static_cast<std::size_t>(P.dim), 0.0), fe(static_cast<std::size_t>(n)), r(static_cast<std::size_t>(n));
+    for (int e = 0; e < F.nent; ++e)
+    {
+        std::fill(fe.begin(), fe.end(), 0.0);
+        for (int t = 0; t < nt; ++t)
+            for (int q = 0; q < nq; ++q)
+            {
+                const std::size_t kk = static_cast<std::size_t>(e) * nq + q;
+                const double s = F.w[kk] * coef[t * np + kk];
+                if (s == 0.0) continue;
+                facet_row(P, F, e, q, a[t], r.data());
+                for (int l = 0; l < n; ++l) fe[l] += s * r[l];
+            }
+        const int *d = &P.cell_dof[static_cast<std::size_t>(F.cell[e]) * n];
+        for (int l = 0; l < n; ++l) if (d[l] >= 0) block[d[l]] += fe[l];
+    }
+    for (int i = 0; i < P.dim; ++i) f[i] += block[i];
 }
 
 // ------------------------------------------------------------------------- vectors, sums
@@ -1798,6 +1969,96 @@ int orc_assemble_vector(void *h, int degree, int nterms, const int *a, const dou
         const Problem &P = *static_cast<Problem *>(h);
         Quad Q = make_quad(P, degree);
         assemble_vector(P, Q, nterms, a, coeffs, f);
+    });
+}
+
+/// The point set of ds(markers) (every exterior facet with all): out = (facets, points per facet).
+int orc_facet_sizes(void *h, int degree, int nmark, const int *marks, int all, int *out)
+{
+    return guard([&] {
+        FacetQuad F = make_facet_quad(*static_cast<Problem *>(h), degree, std::vector<int>(marks, marks + nmark), all != 0);
+        out[0] = F.nent; out[1] = F.nq;
+    });
+}
+
+int orc_facet_points(void *h, int degree, int nmark, const int *marks, int all, double *x, double *y, double *w, double *nx,
+                     double *ny)
+{
+    return guard([&] {
+        FacetQuad F = make_facet_quad(*static_cast<Problem *>(h), degree, std::vector<int>(marks, marks + nmark), all != 0);
+        std::copy(F.x.begin(), F.x.end(), x); std::copy(F.y.begin(), F.y.end(), y); std::copy(F.w.begin(), F.w.end(), w);
+        std::copy(F.nx.begin(), F.nx.end(), nx); std::copy(F.ny.begin(), F.ny.end(), ny);
+    });
+}
+
+/// A form's groups in order (forms/assembler_2d.hpp:166-188, 249-266): group g is dx (kind 0) or
+/// ds (kind 1, markers marks[...] of count nmark[g], or every exterior facet with all[g]) at
+/// degree[g], with nterms[g] terms (codes a, b) whose coefficient arrays (entries x points of the
+/// group's point set) follow one another over all groups.  K is zeroed, then every group adds.
+int orc_assemble_matrix_groups(void *h, int ngroups, const int *kind, const int *degree, const int *nmark, const int *marks,
+                               const int *all, const int *nterms, const int *a, const int *b, const double *coeffs, double *data)
+{
+    return guard([&] {
+        const Problem &P = *static_cast<Problem *>(h);
+        std::vector<int> ip, ix;
+        pattern(P, ip, ix);
+        std::vector<double> v(static_cast<std::size_t>(ip.back()), 0.0);
+        std::size_t mo = 0, to = 0, co = 0;
+        for (int g = 0; g < ngroups; ++g)
+        {
+            if (kind[g] == 0)
+            {
+                Quad Q = make_quad(P, degree[g]);
+                assemble_matrix(P, Q, nterms[g], a + to, b + to, coeffs + co, ip, ix, v, false);
+                co += static_cast<std::size_t>(nterms[g]) * P.mesh.nt() * Q.nq;
+            }
+            else
+            {
+                FacetQuad F = make_facet_quad(P, degree[g], std::vector<int>(marks + mo, marks + mo + nmark[g]), all[g] != 0);
+                assemble_matrix_facets(P, F, nterms[g], a + to, b + to, coeffs + co, ip, ix, v);
+                co += static_cast<std::size_t>(nterms[g]) * F.nent * F.nq;
+            }
+            mo += nmark[g];
+            to += nterms[g];
+        }
+        std::copy(v.begin(), v.end(), data);
+    });
+}
+
+int orc_assemble_vector_groups(void *h, int ngroups, const int *kind, const int *degree, const int *nmark, const int *marks,
+                               const int *all, const int *nterms, const int *a, const double *coeffs, double *f)
+{
+    return guard([&] {
+        const Problem &P = *static_cast<Problem *>(h);
+        std::fill(f, f + P.dim, 0.0);
+        std::size_t mo = 0, to = 0, co = 0;
+        for (int g = 0; g < ngroups; ++g)
+        {
+            if (kind[g] == 0)
+            {
+                Quad Q = make_quad(P, degree[g]);
+                assemble_vector(P, Q, nterms[g], a + to, coeffs + co, f, true);
+                co += static_cast<std::size_t>(nterms[g]) * P.mesh.nt() * Q.nq;
+            }
+            else
+            {
+                FacetQuad F = make_facet_quad(P, degree[g], std::vector<int>(marks + mo, marks + mo + nmark[g]), all[g] != 0);
+                assemble_vector_facets(P, F, nterms[g], a + to, coeffs + co, f);
+                co += static_cast<std::size_t>(nterms[g]) * F.nent * F.nq;
+            }
+            mo += nmark[g];
+            to += nterms[g];
+        }
+    });
+}
+
+/// The program at points with their normals (ds terms).
+int orc_integrand_eval_n(int ninstr, const int *code, int nreg, const double *consts, long npts, const double *xs,
+                         const double *ys, const double *nxs, const double *nys, double *out)
+{
+    return guard([&] {
+        std::vector<double> reg;
+        for (long i = 0; i < npts; ++i) out[i] = run_program(ninstr, code, nreg, consts, xs[i], ys[i], reg, nxs[i], nys[i]);
     });
 }
 

@@ -322,12 +322,95 @@ int ref_assemble_vector(void *sh, int degree, int nterms, const int *a, const do
     });
 }
 
+/// The reference's ds point set: Cache2D(V, degree, markers, all) (fe/cache_2d.hpp:124-183).
+int ref_facet_sizes(void *sh, int degree, int nmark, const int *marks, int all, int *out)
+{
+    return guard([&] {
+        Cache2D Q(*static_cast<Space *>(sh)->V, degree, std::vector<int>(marks, marks + nmark), all != 0);
+        out[0] = Q.nent(); out[1] = Q.nq();
+    });
+}
+
+int ref_facet_points(void *sh, int degree, int nmark, const int *marks, int all, double *x, double *y, double *w, double *nx,
+                     double *ny)
+{
+    return guard([&] {
+        Cache2D Q(*static_cast<Space *>(sh)->V, degree, std::vector<int>(marks, marks + nmark), all != 0);
+        std::copy(Q.x().begin(), Q.x().end(), x);
+        std::copy(Q.y().begin(), Q.y().end(), y);
+        std::copy(Q.weights().begin(), Q.weights().end(), w);
+        std::copy(Q.normal_x().begin(), Q.normal_x().end(), nx);
+        std::copy(Q.normal_y().begin(), Q.normal_y().end(), ny);
+    });
+}
+
+/// A form's dx / ds groups added into one matrix (vector) as FormAssembler2D does
+/// (forms/assembler_2d.hpp:166-188, 249-266), each group through the reference's own cache,
+/// scatter map and assembly routine.
+int ref_assemble_matrix_groups(void *sh, int ngroups, const int *kind, const int *degree, const int *nmark, const int *marks,
+                               const int *all, const int *nterms, const int *a, const int *b, const double *coeffs, double *data)
+{
+    return guard([&] {
+        const Space2D &V = *static_cast<Space *>(sh)->V;
+        CSRMatrix K(cell_pattern(V, V));
+        std::size_t mo = 0, to = 0, co = 0;
+        for (int g = 0; g < ngroups; ++g)
+        {
+            std::unique_ptr<Cache2D> Q = kind[g] == 0 ? std::make_unique<Cache2D>(V, degree[g])
+                                                      : std::make_unique<Cache2D>(V, degree[g], std::vector<int>(marks + mo, marks + mo + nmark[g]), all[g] != 0);
+            const std::size_t np = static_cast<std::size_t>(Q->nent()) * Q->nq();
+            std::vector<std::vector<double>> c(static_cast<std::size_t>(nterms[g]));
+            for (int t = 0; t < nterms[g]; ++t) c[t].assign(coeffs + co + t * np, coeffs + co + (t + 1) * np);
+            std::vector<int> A(a + to, a + to + nterms[g]), B(b + to, b + to + nterms[g]);
+            std::vector<int> map = scatter_map_2d(*Q, *Q, *K.pattern, 0, 0);
+            assemble_matrix_2d(*Q, *Q, A, B, c, K, 0, 0, map.data());
+            co += static_cast<std::size_t>(nterms[g]) * np;
+            mo += nmark[g];
+            to += nterms[g];
+        }
+        std::copy(K.data.begin(), K.data.end(), data);
+    });
+}
+
+int ref_assemble_vector_groups(void *sh, int ngroups, const int *kind, const int *degree, const int *nmark, const int *marks,
+                               const int *all, const int *nterms, const int *a, const double *coeffs, double *f)
+{
+    return guard([&] {
+        const Space2D &V = *static_cast<Space *>(sh)->V;
+        for (int i = 0; i < V.dim(); ++i) f[i] = 0.0;
+        std::size_t mo = 0, to = 0, co = 0;
+        for (int g = 0; g < ngroups; ++g)
+        {
+            std::unique_ptr<Cache2D> Q = kind[g] == 0 ? std::make_unique<Cache2D>(V, degree[g])
+                                                      : std::make_unique<Cache2D>(V, degree[g], std::vector<int>(marks + mo, marks + mo + nmark[g]), all[g] != 0);
+            const std::size_t np = static_cast<std::size_t>(Q->nent()) * Q->nq();
+            std::vector<std::vector<double>> c(static_cast<std::size_t>(nterms[g]));
+            for (int t = 0; t < nterms[g]; ++t) c[t].assign(coeffs + co + t * np, coeffs + co + (t + 1) * np);
+            std::vector<double> block(static_cast<std::size_t>(V.dim()), 0.0);
+            assemble_vector_2d(*Q, std::vector<int>(a + to, a + to + nterms[g]), c, block);
+            for (int i = 0; i < V.dim(); ++i) f[i] += block[i];
+            co += static_cast<std::size_t>(nterms[g]) * np;
+            mo += nmark[g];
+            to += nterms[g];
+        }
+    });
+}
+
 int ref_integrand_eval(int ninstr, const int *code, int nreg, int nconst, const double *consts, long npts,
                        const double *xs, const double *ys, double *out)
 {
     return guard([&] {
         Integrand P(std::vector<int>(code, code + 4 * ninstr), nreg, nconst, 0);
         P.evaluate(static_cast<std::size_t>(npts), xs, ys, nullptr, nullptr, {}, std::vector<double>(consts, consts + nconst), out);
+    });
+}
+
+int ref_integrand_eval_n(int ninstr, const int *code, int nreg, int nconst, const double *consts, long npts, const double *xs,
+                         const double *ys, const double *nxs, const double *nys, double *out)
+{
+    return guard([&] {
+        Integrand P(std::vector<int>(code, code + 4 * ninstr), nreg, nconst, 0);
+        P.evaluate(static_cast<std::size_t>(npts), xs, ys, nxs, nys, {}, std::vector<double>(consts, consts + nconst), out);
     });
 }
 

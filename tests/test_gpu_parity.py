@@ -339,3 +339,118 @@ def test_lobatto_assembly_bit_exact(orc, k, quad):
     K = _assemble_values(V, deg, a, b, c)
     assert np.array_equal(K.data, P.assemble_matrix(deg, a, b, c))
     assert np.array_equal(_vector_values(V, deg, [0, 1], c[:2]), P.assemble_vector(deg, [0, 1], c[:2]))
+
+
+# ------------------------------------------------------------------- boundary facets and lifts
+def _facet_set(V, deg, markers=None):
+    return V.facet_set(deg, markers)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_facet_groups_bit_exact(orc, k):
+    """A form's ds and dx groups through the C ABI, added in the form's order: bit for bit with the
+    oracle (pinned to the reference's facet cache and group combination)."""
+    _name(f"ds/dx groups k={k}")
+    lib = L.load()
+    pts, q = _distorted_quads(5, 4, k)
+    cases = [(fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 5, 4, diagonal="alternate"), k, dirichlet=[4]),
+              orc.rectangle(5, 4, k, "alternate", dirichlet=[4], all_exterior=False), orc.quadrature),
+             (fd.LagrangeSpace(fd.quad_mesh_from_arrays(pts, q), k), orc.quad_from_arrays(pts, q, k=k, all_exterior=False),
+              orc.quad_quadrature)]
+    rng = np.random.default_rng(k)
+    for V, P, rule in cases:
+        npf = lambda d, mk: orc.facet_points(P, d, mk)[0].size
+        gs = ("ds", 2 * k + 1, [1, 3], [0, 1], [0, 0], rng.standard_normal((2, npf(2 * k + 1, [1, 3]))))
+        gd = ("dx", 2 * k, None, [1, 2, 0], [1, 2, 0], rng.standard_normal((3, P.ncells * len(rule(2 * k)[2]))))
+        ga = ("ds", 2, None, [0], [0], rng.standard_normal((1, npf(2, None))))
+        gs[5][0, ::4] = 0.0
+        for order in ([gs, gd, ga], [gd, gs, ga]):
+            K = fd.SparseMatrix._on_space(V)
+            f = np.empty(V.dim)
+            for g, (kind, deg, mk, a, b, c) in enumerate(order):
+                arr, keep = L.coefficient_array([(None, row) for row in c])
+                if kind == "dx":
+                    fm = lib.femd_assemble_matrix if g == 0 else lib.femd_assemble_matrix_add
+                    fv = lib.femd_assemble_vector if g == 0 else lib.femd_assemble_vector_add
+                    L.check(fm(V._h, deg, len(a), L.i32(a), L.i32(b), arr, K._h))
+                    L.check(fv(V._h, deg, len(a), L.i32(a), arr, f))
+                else:
+                    F = _facet_set(V, deg, mk)
+                    L.check(lib.femd_assemble_matrix_facets(F, len(a), L.i32(a), L.i32(b), arr, K._h, int(g > 0)))
+                    L.check(lib.femd_assemble_vector_facets(F, len(a), L.i32(a), arr, f, int(g > 0)))
+            assert np.array_equal(K.data, orc.assemble_matrix_groups(P, order))
+            assert np.array_equal(f, orc.assemble_vector_groups(P, [(g[0], g[1], g[2], g[3], None, g[5]) for g in order]))
+
+
+def test_facet_forms_with_normals(orc):
+    _name("ds forms: normal programs on the GPU, Robin matrix")
+    V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 6, 6), 2, dirichlet=[4])
+    P = orc.rectangle(6, 6, 2, dirichlet=[4], all_exterior=False)
+    u, v, n = fd.TrialFunction(V), fd.TestFunction(V), fd.n
+    g = fd.sin(fd.Const(2.0) * fd.x) * n[0] + fd.y * n[1]
+    Lf = fd.form(g * v * fd.ds((2, 3)) + fd.x * v * fd.dx)
+    fg = Lf.assemble()
+    x, y, w, nx, ny = orc.facet_points(P, Lf.facet_degree, [2, 3])
+    cg_ = orc.integrand_eval_n(*fd.forms.Program(g).arrays(), x, y, nx, ny)
+    cd = orc.integrand_eval(*fd.forms.Program(fd.x).arrays(), *P.cache_points(Lf.quad_degree)[:2])
+    fo = orc.assemble_vector_groups(P, [("ds", Lf.facet_degree, [2, 3], [0], None, cg_[None, :]),
+                                        ("dx", Lf.quad_degree, None, [0], None, cd[None, :])])
+    assert np.abs(fg - fo).max() <= 1e-14 * np.abs(fo).max()
+    # the GPU's program at the facet points against the oracle's libm
+    out = np.empty(x.size)
+    arr, keep = L.coefficient_array([(fd.forms.Program(g).arrays(), None)])
+    L.check(L.load().femd_eval_coefficient_facets(V.facet_set(Lf.facet_degree, (2, 3)), arr, out))
+    assert np.abs(out - cg_).max() <= 4e-15
+    # Robin: grad.grad + alpha u v on sides 1, 2 (a constant coefficient: bit for bit)
+    A = fd.form(fd.dot(fd.grad(u), fd.grad(v)) * fd.dx + fd.Const(3.0) * u * v * fd.ds((1, 2))).assemble()
+    npd = P.ncells * len(orc.quadrature(2)[2])                      # grad.grad of P2: degree 2
+    nf = orc.facet_points(P, 4, [1, 2])[0].size                     # u v of P2 on the sides: degree 4
+    Ao = orc.assemble_matrix_groups(P, [("dx", 2, None, [1, 2], [1, 2], np.ones((2, npd))),
+                                        ("ds", 4, [1, 2], [0], [0], np.full((1, nf), 3.0))])
+    assert np.array_equal(A.data, Ao) and A.symmetric
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_solve_dirichlet_lift(orc, k):
+    """fd.solve with Dirichlet data: the patch test (u in P_k is reproduced to the solver's
+    tolerance) with every side Dirichlet, and with Neumann data g = grad(u).n on three sides
+    through ds; and against the same lift solved with the oracle."""
+    _name(f"solve with dirichlet= k={k}")
+    Pts, T, _ = problems.delaunay_square(0.15, seed=3)
+    ue = (lambda x, y: 1 + 2 * x - y) if k == 1 else (lambda x, y: 1 + 2 * x - y + 0.5 * x * y)
+    V = fd.LagrangeSpace(fd.mesh_from_arrays(Pts, T), k, bc="dirichlet")
+    u, v = fd.TrialFunction(V), fd.TestFunction(V)
+    a = fd.dot(fd.grad(u), fd.grad(v)) * fd.dx
+    L0 = fd.Const(0.0) * v * fd.dx
+    uh = fd.solve(a, L0, backend="cg", M="jacobi", tol=1e-13, dirichlet=ue)
+    xy = V.raw_coordinates()
+    assert uh.size == V.raw_dim and np.abs(uh - ue(xy[:, 0], xy[:, 1])).max() < 1e-9
+    # Dirichlet on the left side only, g = grad(u).n on the others (P1: grad u = (2, -1))
+    if k == 1:
+        W = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 8, 8), 1, dirichlet=[4])
+        u, v, n = fd.TrialFunction(W), fd.TestFunction(W), fd.n
+        aw = fd.dot(fd.grad(u), fd.grad(v)) * fd.dx
+        Lw = (fd.Const(2.0) * n[0] - n[1]) * v * fd.ds((1, 2, 3))
+        uw = fd.solve(aw, Lw, backend="cg", M="jacobi", tol=1e-13, dirichlet={4: ue})
+        xw = W.raw_coordinates()
+        assert np.abs(uw - ue(xw[:, 0], xw[:, 1])).max() < 1e-9
+    # the same lift through the oracle: unconstrained matrix, b_f - (A_U u_g)_f, Jacobi-CG
+    f, _ = problems.fourier_source(2)
+    V2 = V
+    u2, v2 = fd.TrialFunction(V2), fd.TestFunction(V2)
+    a2, L2 = fd.dot(fd.grad(u2), fd.grad(v2)) * fd.dx, f * v2 * fd.dx
+    got = fd.solve(a2, L2, backend="cg", M="jacobi", tol=1e-12, dirichlet=ue)
+    Po, Pu = orc.from_arrays(Pts, T, k=k), orc.from_arrays(Pts, T, k=k, all_exterior=False)
+    deg_a, deg_l = fd.form(a2).quad_degree, fd.form(L2).quad_degree
+    Ao = Po.assemble_matrix(deg_a, [1, 2], [1, 2], np.ones((2, Po.ncells * len(orc.quadrature(deg_a)[2]))))
+    cf = orc.integrand_eval(*fd.forms.Program(f).arrays(), *Po.cache_points(deg_l)[:2])
+    bo = Po.assemble_vector(deg_l, [0], cf[None, :])
+    Au = Pu.assemble_matrix(deg_a, [1, 2], [1, 2], np.ones((2, Pu.ncells * len(orc.quadrature(deg_a)[2]))))
+    ipu, ixu = Pu.pattern()
+    ug = V2.lift(ue)
+    num = Po.numbering()
+    bo = bo - orc.spmv(ipu, ixu, Au, ug)[num["ad2raw"]]
+    ip, ix = Po.pattern()
+    xo, _ = orc.cg(ip, ix, Ao, bo, M="jacobi", tol=1e-12)
+    want = V2.prolongate(xo) + ug
+    assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want)

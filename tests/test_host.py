@@ -189,7 +189,19 @@ def test_form_canonicalization_and_degree():
     with pytest.raises(TypeError):
         fd.form(u * fd.dx)
     with pytest.raises(NotImplementedError):
-        fd.form(u * v * fd.ds)
+        fd.form(u * v * fd.dS)
+    # ds terms: their own degree (forms.py:2487-2500) and point set per marker set, in the form's order
+    nv = fd.FacetNormal()
+    F = fd.form(u * v * fd.ds(3) + fd.dot(fd.grad(u), fd.grad(v)) * fd.dx + fd.x * u * v * fd.ds((1, 3)))
+    assert F.quad_degree == 2 and F.facet_degree == 5
+    assert list(F.groups) == [("ds", (3,)), ("dx",), ("ds", (1, 3))]
+    assert fd.form(nv[0] * v * fd.ds).facet_degree == 2
+    with pytest.raises(TypeError):                                   # the normal exists on facets only
+        fd.form(nv[0] * v * fd.dx)
+    with pytest.raises(ValueError):                                  # no side carries marker 7
+        fd.form(v * fd.ds(7))
+    with pytest.raises(TypeError):
+        fd.ds("left")
 
 
 def test_program_matches_reference_instruction_order():
@@ -238,3 +250,35 @@ def test_register_compaction_keeps_the_arithmetic():
         assert nr.value <= min(nreg, 16)
         assert np.array_equal(_interpret(comp, consts, x, y), _interpret(code, consts, x, y))
         assert np.array_equal(comp[:, 0], code[:, 0])                      # same operations, same order
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_boundary_facets_vs_oracle(orc, k):
+    """The library's ds point sets (host side of the facet set) bit for bit with the oracle's, and
+    the Dirichlet-data helpers: boundary nodes, constrained nodes, the lift."""
+    pts, q = _distorted_quads(5, 4, k)
+    P, T, _ = problems.delaunay_square(0.2, seed=1)
+    cases = [(fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 5, 4, diagonal="alternate"), k, dirichlet=[4]),
+              orc.rectangle(5, 4, k, "alternate", dirichlet=[4], all_exterior=False)),
+             (fd.LagrangeSpace(fd.quad_mesh_from_arrays(pts, q), k), orc.quad_from_arrays(pts, q, k=k, all_exterior=False)),
+             (fd.LagrangeSpace(fd.mesh_from_arrays(P, T), k, bc="dirichlet"), orc.from_arrays(P, T, k=k))]
+    for V, Po in cases:
+        for mk in (None, [1], [2, 3]):
+            for deg in (1, 4):
+                for a, b in zip(V.boundary_points(deg, mk), orc.facet_points(Po, deg, mk)):
+                    assert np.array_equal(a, b)
+    V = cases[0][0]
+    xy = V.raw_coordinates()
+    assert np.array_equal(V.boundary_nodes(1), np.flatnonzero(xy[:, 1] == 0.0))
+    assert np.array_equal(V.boundary_nodes(4), np.flatnonzero(xy[:, 0] == 0.0))
+    assert np.array_equal(V.constrained(), V.boundary_nodes(4))
+    g = V.lift(lambda x, y: 1 + y)
+    assert np.array_equal(np.flatnonzero(g), V.constrained()) and np.array_equal(g[V.constrained()], 1 + xy[V.constrained(), 1])
+    assert np.array_equal(V.lift({4: 2.5})[V.constrained()], np.full(V.n_constraints, 2.5))
+    assert np.array_equal(V.lift(fd.x + 2 * fd.y)[V.constrained()], 2 * xy[V.constrained(), 1])
+    U = V.unconstrained
+    assert U.dim == V.raw_dim and U.n_constraints == 0 and U.unconstrained is U
+    with pytest.raises(ValueError):
+        V.lift({1: 1.0})                         # side 1 is natural here
+    with pytest.raises(ValueError):
+        U.lift(1.0)                              # nothing eliminated

@@ -536,3 +536,65 @@ def test_lobatto_nodes_vs_reference(orc, ref, k):
         A, B = [1, 2, 0], [1, 2, 0]
         assert np.array_equal(po.assemble_matrix(deg, A, B, c), pr.assemble_matrix(deg, A, B, c))
         assert np.array_equal(po.assemble_vector(deg, [0, 1], c[:2]), pr.assemble_vector(deg, [0, 1], c[:2]))
+
+
+# ------------------------------------------------------------------- boundary facets (ds groups)
+def test_facet_groups_vs_reference(orc, ref):
+    """ds point sets and a form's dx / ds groups added in either order: bit for bit with the
+    reference's Cache2D on facets and FormAssembler2D's group combination."""
+    rng = np.random.default_rng(0)
+    Pts, T = _small_delaunay(1)
+    pq, qq = _distorted_quads(5, 4, 2)
+    for k in (1, 2, 3):
+        cases = [(orc.rectangle(5, 4, k, "alternate", all_exterior=False, dirichlet=[4]),
+                  ref.rectangle(5, 4, k, "alternate", all_exterior=False, dirichlet=[4]), orc.quadrature),
+                 (orc.quad_from_arrays(pq, qq, k=k, all_exterior=False), ref.quad_from_arrays(pq, qq, k=k, all_exterior=False),
+                  orc.quad_quadrature),
+                 (orc.from_arrays(Pts, T, k=k), ref.from_arrays(Pts, T, k=k), orc.quadrature)]
+        for po, pr, rule in cases:
+            for mk in (None, [1], [2, 3]):
+                for deg in (1, 3, 6):
+                    for u, v in zip(orc.facet_points(po, deg, mk), ref.facet_points(pr, deg, mk)):
+                        assert np.array_equal(u, v)
+            npf = lambda d, mk: orc.facet_points(po, d, mk)[0].size
+            gd = ("dx", 2 * k, None, [1, 2, 0], [1, 2, 0], rng.standard_normal((3, po.ncells * len(rule(2 * k)[2]))))
+            gs = ("ds", 2 * k + 1, [1, 3], [0, 1], [0, 0], rng.standard_normal((2, npf(2 * k + 1, [1, 3]))))
+            ga = ("ds", 2, None, [0], [0], rng.standard_normal((1, npf(2, None))))
+            for groups in ([gd, gs, ga], [gs, gd, ga]):
+                assert np.array_equal(orc.assemble_matrix_groups(po, groups), ref.assemble_matrix_groups(pr, groups))
+                vg = [(g[0], g[1], g[2], g[3], None, g[5]) for g in groups]
+                assert np.array_equal(orc.assemble_vector_groups(po, vg), ref.assemble_vector_groups(pr, vg))
+    x, y, nx, ny = rng.random((4, 40))
+    code = np.array([[3, 0, 1, 0], [4, 1, -1, 0], [1, 2, 0, 0], [7, 3, 0, 2], [6, 4, 3, 1]], dtype=np.int32)
+    assert np.array_equal(orc.integrand_eval_n(code, 5, [], x, y, nx, ny), ref.integrand_eval_n(code, 5, [], x, y, nx, ny))
+
+
+def test_facet_closed_forms(orc):
+    """Facet weights sum to the boundary length, the divergence theorem for x n_x and n_x, the P1
+    boundary mass of one side (h/6 [2 1; 1 2] per edge), and ds vectors of constants."""
+    N = 4
+    P = orc.rectangle(N, N, 1, all_exterior=False)
+    x, y, w, nx, ny = orc.facet_points(P, 3)
+    assert abs(w.sum() - 4.0) < 1e-14
+    assert abs((w * nx).sum()) < 1e-15 and abs((w * ny).sum()) < 1e-15
+    assert abs((w * x * nx).sum() - 1.0) < 1e-14 and abs((w * y * ny).sum() - 1.0) < 1e-14
+    bottom = orc.facet_points(P, 3, [1])
+    assert np.all(bottom[1] == 0.0) and np.all(bottom[3] == 0.0) and np.all(bottom[4] == -1.0)
+    assert abs(bottom[2].sum() - 1.0) < 1e-15
+    # P1 on the bottom side: the vector of 1 is h/2, h, ..., h, h/2 at the side's nodes
+    nb = bottom[0].size
+    f = orc.assemble_vector_groups(P, [("ds", 2, [1], [0], None, np.ones((1, nb)))])
+    num = P.numbering()
+    xy = num["raw_xy"][num["ad2raw"]]
+    on = np.flatnonzero(xy[:, 1] == 0.0)
+    h = 1.0 / N
+    expect = np.where((xy[on, 0] == 0.0) | (xy[on, 0] == 1.0), h / 2, h)
+    assert np.abs(f[on] - expect).max() < 1e-15 and np.abs(np.delete(f, on)).max() == 0.0
+    ip, ix = P.pattern()
+    M = orc.assemble_matrix_groups(P, [("ds", 2, [1], [0], [0], np.ones((1, nb)))])
+    rows = np.repeat(np.arange(P.dim), np.diff(ip))
+    D = np.zeros((P.dim, P.dim)); D[rows, ix] = M
+    i0 = on[np.argsort(xy[on, 0])]
+    for a, b in zip(i0[:-1], i0[1:]):
+        assert abs(D[a, b] - h / 6) < 1e-15
+    assert np.abs(D.sum(1) - f).max() < 1e-15

@@ -64,6 +64,32 @@ __device__ inline void wg_fixed_sums(const double *a, int n, double *red, double
     __syncthreads();
 }
 
+// Four consecutive values of a vector (16-byte aligned): one float4, or two double2.
+template <typename T> struct Vec4;
+template <> struct Vec4<float> {
+    float v[4];
+    __device__ void load(const float *p)
+    {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+    __device__ void store(float *p) const { *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]); }
+};
+template <> struct Vec4<double> {
+    double v[4];
+    __device__ void load(const double *p)
+    {
+        const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+    __device__ void store(double *p) const
+    {
+        *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+    }
+};
+static_assert(BCHUNK == 4 * BT, "a whole chunk is four rows per lane");
+
 // ---- scaled values ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BT) void k_scale_of_diag(int n, const int *diag, const double *v, double *sc, float *dg,
                                                       int *bad_row)
@@ -179,15 +205,38 @@ __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ cl
     const T *pnew = odd ? P0 : P1;
     const T a = static_cast<T>(S.alpha);
     double s = 0.0, t = 0.0;
-    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
+    const int64_t lo = clo[c], hi = clo[c + 1];
+    if ((lo & 3) == 0 && hi - lo == BCHUNK)
     {
-        y[i] = fma(a, pnew[i], y[i]);
-        const T ri = fma(-a, q[i], r[i]);
-        r[i] = ri;
-        const double e = static_cast<double>(ri) * static_cast<double>(ri);
-        s += e;
-        t += e * static_cast<double>(dg[i]);
+        // a whole aligned chunk: four consecutive rows per lane, vector loads and stores
+        const int64_t i = lo + 4 * static_cast<int64_t>(threadIdx.x);
+        Vec4<T> yv, pv, rv, qv;
+        yv.load(y + i); pv.load(pnew + i); rv.load(r + i); qv.load(q + i);
+        const float4 dv = *reinterpret_cast<const float4 *>(dg + i);
+        const float dd[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            yv.v[j] = fma(a, pv.v[j], yv.v[j]);
+            const T ri = fma(-a, qv.v[j], rv.v[j]);
+            rv.v[j] = ri;
+            const double e = static_cast<double>(ri) * static_cast<double>(ri);
+            s += e;
+            t += e * static_cast<double>(dd[j]);
+        }
+        yv.store(y + i);
+        rv.store(r + i);
     }
+    else
+        for (int64_t i = lo + threadIdx.x; i < hi; i += BT)
+        {
+            y[i] = fma(a, pnew[i], y[i]);
+            const T ri = fma(-a, q[i], r[i]);
+            r[i] = ri;
+            const double e = static_cast<double>(ri) * static_cast<double>(ri);
+            s += e;
+            t += e * static_cast<double>(dg[i]);
+        }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
     {

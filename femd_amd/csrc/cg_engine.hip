@@ -17,6 +17,7 @@ constexpr int BT = 256;
 constexpr int BCHUNK = 1024;
 constexpr int RT = 1024;           // threads of the per-system reductions
 constexpr int GRAPH_ITERS = 16;
+constexpr int64_t FUSE_ALPHA_MAX = int64_t(1) << 20;   // tiles x chunks of the largest system (8 MiB of extra reads)
 
 inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + BT - 1) / BT)); }
 
@@ -25,10 +26,9 @@ inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + 
 // them per round all at once (one memory latency per round, not per element), then the wave
 // (shuffles) and the waves in order.  The order depends only on n and THREADS.  red holds
 // W THREADS / 64 doubles.
-template <int THREADS, int W>
+template <int THREADS, int W, int K = (W == 1 ? 16 : 8)>
 __device__ inline void wg_fixed_sums(const double *a, int n, double *red, double (&out)[W])
 {
-    constexpr int K = W == 1 ? 16 : 8;
     double s[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) s[w] = 0.0;
@@ -171,15 +171,17 @@ __global__ __launch_bounds__(BT) void k_cg_list_fill(int ntiles, const int2 *__r
     list[2 * k + 1] = make_int4(b1.x, b1.y, uptr[b + 1], b);
 }
 
-// one workgroup per system: p'.q over its tiles, alpha
-__global__ __launch_bounds__(RT) void k_cg_alpha(const int *__restrict__ seg_tiles, const double *__restrict__ tparts, CGSys *st)
+// one workgroup per system: p'.q over its tiles, alpha.  The sum runs in the order of the update
+// kernels' own (BT lanes), so alpha has the same bits whether it is formed here or there; 48 loads
+// per lane per round keep a system of 12 k tiles to one round.
+__global__ __launch_bounds__(BT) void k_cg_alpha(const int *__restrict__ seg_tiles, const double *__restrict__ tparts, CGSys *st)
 {
     CGSys &S = st[blockIdx.x];
     if (!S.active) return;
-    __shared__ double red[RT / 64];
+    __shared__ double red[BT / 64];
     const int t0 = seg_tiles[blockIdx.x], t1 = seg_tiles[blockIdx.x + 1];
     double o[1];
-    wg_fixed_sums<RT, 1>(tparts + t0, t1 - t0, red, o);
+    wg_fixed_sums<BT, 1, 48>(tparts + t0, t1 - t0, red, o);
     const double pq = o[0];
     if (threadIdx.x != 0) return;
     if (!(pq > 0.0))
@@ -193,17 +195,38 @@ __global__ __launch_bounds__(RT) void k_cg_alpha(const int *__restrict__ seg_til
 
 // y += alpha p', r -= alpha q over one chunk of one system, and the chunk's partials of r.r and
 // r.D r (D = diag(A): rho^2 r.D r is the squared norm of the unscaled residual r stands for)
-template <typename T>
+// With FUSE (small systems) every workgroup forms alpha = r.r / p'.q itself from the system's tile
+// partials, in one fixed order (the same bits in every workgroup), and k_cg_alpha is not launched:
+// the system's first chunk records alpha, or a breakdown (p'.q <= 0) in S.broke, which no update
+// workgroup reads (k_cg_beta pauses the system).
+template <typename T, bool FUSE>
 __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ clo, const int *__restrict__ cpid, T *y, T *r,
                                                   const T *P0, const T *P1, const T *__restrict__ q,
-                                                  const float *__restrict__ dg, int odd, double *cparts, const CGSys *st)
+                                                  const float *__restrict__ dg, int odd, double *cparts,
+                                                  const int *__restrict__ seg_tiles, const int *__restrict__ cptr,
+                                                  const double *tparts, CGSys *st)
 {
     __shared__ double red[2 * (BT / 64)];
-    const int c = blockIdx.x;
-    const CGSys &S = st[cpid[c]];
+    const int c = blockIdx.x, sys = cpid[c];
+    CGSys &S = st[sys];
     if (!S.active) return;
+    double alpha = FUSE ? 0.0 : S.alpha;
+    if constexpr (FUSE)
+    {
+        double o[1];
+        const int t0 = seg_tiles[sys];
+        wg_fixed_sums<BT, 1>(tparts + t0, seg_tiles[sys + 1] - t0, red, o);
+        const bool lead = c == cptr[sys] && threadIdx.x == 0;
+        if (!(o[0] > 0.0))
+        {
+            if (lead) S.broke = 1;
+            return;
+        }
+        alpha = S.rr / o[0];
+        if (lead) S.alpha = alpha;
+    }
     const T *pnew = odd ? P0 : P1;
-    const T a = static_cast<T>(S.alpha);
+    const T a = static_cast<T>(alpha);
     double s = 0.0, t = 0.0;
     const int64_t lo = clo[c], hi = clo[c + 1];
     if ((lo & 3) == 0 && hi - lo == BCHUNK)
@@ -268,6 +291,16 @@ __global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, co
 {
     CGSys &S = st[blockIdx.x];
     if (!S.active) return;
+    if (S.broke)
+    {
+        if (threadIdx.x == 0)
+        {
+            S.active = 0;
+            S.reason = 2;
+            S.broke = 0;
+        }
+        return;
+    }
     __shared__ double red[2 * (RT / 64)];
     const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
     double o[2];
@@ -452,6 +485,7 @@ __global__ __launch_bounds__(RT) void k_cg_fresh_state(const int *__restrict__ c
     S.thr = tol2 * rr;
     S.rho = rn;
     S.reason = 0;
+    S.broke = 0;
     S.active = (rn > 0.0 && rr > 0.0) ? 1 : 0;
 }
 
@@ -619,6 +653,13 @@ void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
     cptr = upload(cptr_h, s);
     plan = &spmv_plan(P, prec, prec == FEMD_FP64 ? static_cast<const void *>(M->v64s.p) : M->v32s.p, s);
     seg_tiles = upload(plan->seg_tiles, s);
+    // alpha folded into the update when every workgroup's extra reads stay small: each of a
+    // system's chunks reads all of its tile partials
+    int64_t most = 0;
+    for (int p = 0; p < np; ++p)
+        most = std::max(most, int64_t(plan->seg_tiles[size_t(p) + 1] - plan->seg_tiles[size_t(p)]) *
+                                  int64_t(cptr_h[size_t(p) + 1] - cptr_h[size_t(p)]));
+    fuse_alpha = most <= FUSE_ALPHA_MAX;
     const size_t N = size_t(std::max<int64_t>(n, 1)), w = prec == FEMD_FP64 ? 8 : 4;
     st = DevBuf(sizeof(CGSys) * size_t(std::max(np, 1)));
     tparts = DevBuf(8 * size_t(std::max(plan->ntiles, 1)));
@@ -712,10 +753,17 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
         const int odd = E.parity;
         if (E.use_dict) launch_product<T, true>(E, odd, s);
         else launch_product<T, false>(E, odd, s);
-        hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(RT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), E.y.as<T>(),
-                           E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
-                           E.cparts.as<double>(), static_cast<const CGSys *>(st));
+        if (E.fuse_alpha)
+            hipLaunchKernelGGL((k_cg_update<T, true>), dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(),
+                               E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
+                               E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(), E.tparts.as<double>(), st);
+        else
+        {
+            hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(BT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
+            hipLaunchKernelGGL((k_cg_update<T, false>), dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(),
+                               E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
+                               E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(), E.tparts.as<double>(), st);
+        }
         hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), odd, st);
         E.parity ^= 1;
     }

@@ -37,6 +37,8 @@ struct CGSys {
     double rho;          // the FP64 residual norm the system is scaled by
     int it, total, max_iter, active, reason;   // it: since the last restart; total: all
     int dir;             // the direction buffer (P0 / P1) holding the current direction
+    int broke;           // p'.q <= 0 seen by the update (alpha folded into it): beta pauses the system
+    int pad;
 };
 
 // S = diag(A)^(-1/2) (FP64) and S A S (FP32 / FP64) on the matrix, cached until its values change;
@@ -62,6 +64,7 @@ struct CGEngine {
     DevBuf seg_tiles;
     DevBuf st, tparts, cparts, r, q, P0, P1, y, r64, t64, sel, retired, rho_new, tgt, cnt;
     bool use_dict = false;              // the product reads the scaled values through their dictionary
+    bool fuse_alpha = false;            // small systems: every update workgroup forms alpha itself
     int parity = 0;                     // the next iteration reads its direction from P1 (1) or P0 (0)
     DevBuf list, aoff;                  // the product's tile list (two int4 per tile) and its offsets
     int list_count = -1;

@@ -141,9 +141,10 @@ const SpmvPlan &spmv_plan(DevPattern &P, int precision, const void *values, hipS
     {
         // segmented (batched) patterns use fixed shapes, so that a segment is tiled -- and its
         // dot products summed -- the same way whatever else is in the batch
-        const char *e32 = std::getenv("FEMD_SEG_SHAPE32"), *e64 = std::getenv("FEMD_SEG_SHAPE64");
-        const char *nm = precision == FEMD_FP32 ? (e32 ? e32 : "tile256x2048") : (e64 ? e64 : "tile256x1024");
-        slot = std::make_unique<SpmvPlan>(make_plan(P, shape_index(nm, precision), s));
+        // (measured on the 256-system P2 batch: FP64 256x1024 2.36 s, 512x2048 2.36 s, 256x2048
+        // 2.55 s; FP32 256x2048 4.78 s, 512x4096 5.07 s, 512x2048 6.32 s)
+        slot = std::make_unique<SpmvPlan>(make_plan(P, shape_index(precision == FEMD_FP32 ? "tile256x2048" : "tile256x1024",
+                                                                   precision), s));
     }
     if (!slot)
     {

@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--original-order", action="store_true",
+                    help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
     ap.add_argument("--no-graph", action="store_true",
                     help="every solve with plain launches (for profilers that cannot follow graph replays)")
     args = ap.parse_args()
@@ -74,6 +76,7 @@ def main():
 
     lib = L.load()
     L.check(lib.femd_init(0))
+    lib.femd_inner_order(0 if args.original_order else 1)
     if args.no_graph:
         L.check(lib.femd_inner_timer(1))       # the sampled kernel timer runs solves without graphs
 
@@ -144,6 +147,9 @@ def main():
     step()
     launches, tot = C.c_int64(), C.c_double()
     L.check(lib.femd_inner_timer_read(C.byref(launches), C.byref(tot)))
+    inner_shape = C.create_string_buffer(64)
+    L.check(lib.femd_inner_timer_shape(inner_shape, 64))
+    inner_shape = inner_shape.value.decode()
     L.check(lib.femd_inner_timer(0 if not args.no_graph else 1))
     value = 1e3 / ms
 
@@ -213,7 +219,7 @@ def main():
                           "flushed_us": round(c32 * 1e3, 2), "flushed_gbs": round(gbs(B32, c32), 1),
                           "flushed_frac": round(gbs(B32, c32) / HBM_PEAK_GBS, 4)},
                  "bytes_fp64": B64, "bytes_fp32": B32},
-        "roofline": {"kernel": f"k_cg_product (product of the FP32 inner CG with the direction update, {variants['fp32']})",
+        "roofline": {"kernel": f"k_cg_product (product of the FP32 inner CG with the direction update, {inner_shape})",
                      "bound": "hbm",
                      "achieved": round(gbs_fused, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs_fused / HBM_PEAK_GBS, 4), "traffic": traffic,

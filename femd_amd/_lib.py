@@ -121,6 +121,8 @@ _SIGS = {
     "femd_event_timer_stop": (_i, [_p, C.POINTER(_d)]),
     "femd_inner_timer": (_i, [_i]),
     "femd_inner_timer_read": (_i, [C.POINTER(_i64), C.POINTER(_d)]),
+    "femd_inner_timer_shape": (_i, [C.c_char_p, _i]),
+    "femd_inner_order": (_i, [_i]),
     "femd_value_dictionary": (_i, [_i]),
     "femd_batch_dictionary": (_i, [_p, _i, C.POINTER(_i64), C.POINTER(_i)]),
     "femd_server_create": (_i, [_i64, _i, _i, _d, _d, _i, C.POINTER(_p)]),

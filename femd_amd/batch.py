@@ -79,6 +79,13 @@ class Batch:
         return None if d.value < 0 else (d.value, m.value)
 
 
+def inner_order(enable=None) -> bool:
+    """Whether the inner CG works in its own order of the unknowns (each system's unknowns in the
+    order its rows first meet them as columns, which keeps an SpMV tile's gathers close together);
+    with `enable` set it first switches it on or off.  Returns the previous setting."""
+    return bool(L.load().femd_inner_order(-1 if enable is None else int(bool(enable))))
+
+
 def value_dictionary(enable=None) -> bool:
     """Whether the inner CG may read matrices through lossless per-tile value dictionaries (results
     are bit-identical either way); with `enable` set it first switches them on or off.  Returns the

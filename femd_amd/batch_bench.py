@@ -46,6 +46,8 @@ def main(argv=None):
     ap.add_argument("--sweep", default="", help="comma-separated mixed-tol:inner-tol pairs to time as well")
     ap.add_argument("--no-graph", action="store_true", help="batched solves with plain launches instead of graph replays")
     ap.add_argument("--no-dict", action="store_true", help="batched solves read the matrix values, not value dictionaries")
+    ap.add_argument("--original-order", action="store_true",
+                    help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
     ap.add_argument("--parts", default="plain,fp64,mixed",
                     help="which solves to run (a profiling run may leave out the plain path; errors then are not measured)")
     args = ap.parse_args(argv)
@@ -58,6 +60,7 @@ def main(argv=None):
     lib = L.load()
     L.check(lib.femd_init(0))
     lib.femd_value_dictionary(0 if args.no_dict else 1)
+    lib.femd_inner_order(0 if args.original_order else 1)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
     sizes = problem_sizes(args.count, args.nmin, args.nmax, args.seed)

@@ -27,15 +27,16 @@ void batch_copy_values(femd_batch_s *B, hipStream_t s)
     B->A->v32_valid = false;
     B->A->v32s_valid = false;
     B->A->v64s_valid = false;
-    scaled_values(B->A.get(), s);
+    scaled_values(B->A.get(), s, B->reorder);
 }
 
-std::unique_ptr<femd_batch_s> batch_build(int np, const femd_csr *mats, hipStream_t s)
+std::unique_ptr<femd_batch_s> batch_build(int np, const femd_csr *mats, hipStream_t s, bool reorder)
 {
     if (np < 1) fail(FEMD_EINVAL, "batch: at least one system");
     if (!mats) fail(FEMD_EINVAL, "batch: null argument");
     auto B = std::make_unique<femd_batch_s>();
     B->np = np;
+    B->reorder = reorder;
     B->members.assign(mats, mats + np);
     B->roff.assign(np + 1, 0);
     B->voff.assign(np + 1, 0);
@@ -80,7 +81,7 @@ void batch_solve(femd_batch_s *B, const double *b, double *x, double tol, double
     if (!(tol >= 0.0) || !(inner_tol > 0.0 && inner_tol < 1.0) || max_outer < 1)
         fail(FEMD_EINVAL, "batch: need tol >= 0, 0 < inner_tol < 1, max_outer >= 1");
     CGEngine &E = B->engine[precision == FEMD_FP32 ? 0 : 1];
-    E.setup(B->A.get(), precision, s);
+    E.setup(B->A.get(), precision, s, B->reorder);
     const CGEngine::Result r = E.solve(b, x, tol, inner_tol, max_outer, use_graph != 0, iters, res, s);
     info->converged = r.converged;
     info->iterations = r.iterations;

@@ -91,14 +91,17 @@ template <> struct Vec4<double> {
 static_assert(BCHUNK == 4 * BT, "a whole chunk is four rows per lane");
 
 // ---- scaled values ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BT) void k_scale_of_diag(int n, const int *diag, const double *v, double *sc, float *dg,
-                                                      int *bad_row)
+// one thread per position k of the inner CG's order (row i = perm[k]; perm NULL: i = k):
+// sc[i] = a_ii^(-1/2) in the original order, dg[k] = a_ii in the inner CG's
+__global__ __launch_bounds__(BT) void k_scale_of_diag(int n, const int *__restrict__ perm, const int *diag, const double *v,
+                                                      double *sc, float *dg, int *bad_row)
 {
-    const int i = blockIdx.x * BT + threadIdx.x;
-    if (i >= n) return;
+    const int k = blockIdx.x * BT + threadIdx.x;
+    if (k >= n) return;
+    const int i = perm ? perm[k] : k;
     const int d = diag[i];
     const double a = d >= 0 ? v[d] : 0.0;
-    dg[i] = static_cast<float>(a);
+    dg[k] = static_cast<float>(a);
     if (!(a > 0.0))
     {
         atomicMin(bad_row, i);
@@ -107,15 +110,24 @@ __global__ __launch_bounds__(BT) void k_scale_of_diag(int n, const int *diag, co
     else sc[i] = 1.0 / std::sqrt(a);
 }
 
-// out[p] = v[p] s_i s_j for the entries of row i (one lane per row: a set-up pass).
+// S A S in the inner CG's order: entry p of the reordered pattern (ip, ix; row k) is entry emap[p]
+// of A, in row perm[k] and column perm[ix[p]], so out[p] = v[emap[p]] s_i s_j -- the same product,
+// in the same order, as for the entry in place (one lane per row: a set-up pass).  perm and emap
+// NULL: the original order.
 template <typename T>
-__global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, const int *ix, const double *v, const double *sc,
-                                                      T *out)
+__global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, const int *ix, const int *__restrict__ emap,
+                                                      const int *__restrict__ perm, const double *v, const double *sc, T *out)
 {
-    const int i = blockIdx.x * BT + threadIdx.x;
-    if (i >= n) return;
-    const double si = sc[i];
-    for (int p = ip[i]; p < ip[i + 1]; ++p) out[p] = static_cast<T>(v[p] * si * sc[ix[p]]);
+    const int k = blockIdx.x * BT + threadIdx.x;
+    if (k >= n) return;
+    if (!perm)
+    {
+        const double si = sc[k];
+        for (int p = ip[k]; p < ip[k + 1]; ++p) out[p] = static_cast<T>(v[p] * si * sc[ix[p]]);
+        return;
+    }
+    const double si = sc[perm[k]];
+    for (int p = ip[k]; p < ip[k + 1]; ++p) out[p] = static_cast<T>(v[emap[p]] * si * sc[perm[ix[p]]]);
 }
 
 // ---- iteration kernels -----------------------------------------------------------------------
@@ -326,16 +338,23 @@ __global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, co
 }
 
 // ---- maintenance kernels (on the systems with sel[sys] != 0) ---------------------------------
+// x (original order) += rho S y (y in the inner CG's order: position i is unknown perm[i], or i
+// when perm is NULL)
 template <typename T>
 __global__ __launch_bounds__(BT) void k_cg_finish(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
-                                                  const int *__restrict__ sel, const double *__restrict__ sc,
-                                                  const T *__restrict__ y, const CGSys *st, double *x)
+                                                  const int *__restrict__ sel, const int *__restrict__ perm,
+                                                  const double *__restrict__ sc, const T *__restrict__ y, const CGSys *st,
+                                                  double *x)
 {
     const int c = blockIdx.x, sys = cpid[c];
     if (!sel[sys]) return;
     const double rho = st[sys].rho;
     if (!(rho > 0.0)) return;
-    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT) x[i] += rho * (sc[i] * static_cast<double>(y[i]));
+    for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
+    {
+        const int64_t j = perm ? perm[i] : i;
+        x[j] += rho * (sc[j] * static_cast<double>(y[i]));
+    }
 }
 
 // r64 = b - t64 and the chunk's partial of r64.r64 (of b.b when t64 is NULL)
@@ -378,13 +397,15 @@ __global__ __launch_bounds__(RT) void k_cg_sys_sum(const int *__restrict__ cptr,
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-// Resume (sel 1) or restart with no direction (sel 2): r = S r64 / rho_new, y = 0, and the
-// current direction p and q = A p scaled by rho / rho_new (or zeroed).
+// Resume (sel 1) or restart with no direction (sel 2): r = S r64 / rho_new (r64 in the original
+// order, r in the inner CG's), y = 0, and the current direction p and q = A p scaled by
+// rho / rho_new (or zeroed).
 template <typename T>
 __global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ clo, const int *__restrict__ cpid,
                                                    const int *__restrict__ sel, const double *__restrict__ rho_new,
-                                                   const double *__restrict__ sc, const double *__restrict__ r64, T *r, T *y,
-                                                   T *P0, T *P1, T *q, int next, const CGSys *st)
+                                                   const int *__restrict__ perm, const double *__restrict__ sc,
+                                                   const double *__restrict__ r64, T *r, T *y, T *P0, T *P1, T *q, int next,
+                                                   const CGSys *st)
 {
     const int c = blockIdx.x, sys = cpid[c];
     const int mode = sel[sys];
@@ -396,7 +417,8 @@ __global__ __launch_bounds__(BT) void k_cg_rescale(const int64_t *__restrict__ c
     T *pn = next ? P1 : P0;                  // ... moves to where the next product reads it
     for (int64_t i = clo[c] + threadIdx.x; i < clo[c + 1]; i += BT)
     {
-        r[i] = static_cast<T>(sc[i] * r64[i] * scale);
+        const int64_t j = perm ? perm[i] : i;
+        r[i] = static_cast<T>(sc[j] * r64[j] * scale);
         y[i] = T(0);
         // a restart stores zeros (the buffers may hold anything, NaN included, before the first)
         if (mode == 2)
@@ -516,11 +538,12 @@ struct InnerTimer {
     int64_t launches = 0;
     std::vector<Sample> samples;
     size_t used = 0;
+    std::string shape;             // launch shape of the last timed launch
 };
 InnerTimer g_timer;
 
 // Events for this product launch, or false (timer off, every 8th launch only, never in capture).
-bool timer_sample(hipStream_t s, hipEvent_t *e0, hipEvent_t *e1)
+bool timer_sample(hipStream_t s, hipEvent_t *e0, hipEvent_t *e1, const char *shape)
 {
     std::lock_guard<std::mutex> lk(g_timer.mu);
     if (!g_timer.on) return false;
@@ -535,6 +558,7 @@ bool timer_sample(hipStream_t s, hipEvent_t *e0, hipEvent_t *e1)
         g_timer.samples.push_back(x);
     }
     const Sample &x = g_timer.samples[g_timer.used++];
+    g_timer.shape = shape;
     *e0 = x.e0;
     *e1 = x.e1;
     return true;
@@ -548,14 +572,106 @@ bool timer_on()
 
 } // namespace
 
-// ---- scaled values (cached on the matrix) --------------------------------------------------
-void scaled_values(femd_csr_s *A, hipStream_t s)
+// ---- the inner CG's order (cached on the pattern) --------------------------------------------
+const LocalOrder &local_order(DevPattern &P, hipStream_t s)
 {
-    if (A->v32s_valid) return;
+    if (P.local) return *P.local;
+    if (P.n != P.m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
+    const int n = P.n;
+    const std::vector<int64_t> seg = P.segments.empty() ? std::vector<int64_t>{0, int64_t(n)} : P.segments;
+    const int nseg = static_cast<int>(seg.size()) - 1;
+    const int *ip = P.h_ip.data(), *ix = P.h_ix.data();
+    std::vector<int> perm(size_t(std::max(n, 1))), iperm(size_t(std::max(n, 1)), -1);
+    int outside = 0;
+    // within each segment: columns in the order the rows first meet them, then the unmet ones
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : outside)
+    for (int k = 0; k < nseg; ++k)
+    {
+        const int r0 = static_cast<int>(seg[size_t(k)]), r1 = static_cast<int>(seg[size_t(k) + 1]);
+        int next = r0;
+        for (int e = ip[r0]; e < ip[r1]; ++e)
+        {
+            const int j = ix[e];
+            if (j < r0 || j >= r1)
+            {
+                ++outside;
+                break;
+            }
+            if (iperm[size_t(j)] < 0)
+            {
+                iperm[size_t(j)] = next;
+                perm[size_t(next++)] = j;
+            }
+        }
+        for (int j = r0; j < r1; ++j)
+            if (iperm[size_t(j)] < 0)
+            {
+                iperm[size_t(j)] = next;
+                perm[size_t(next++)] = j;
+            }
+    }
+    if (outside) fail(FEMD_EINVAL, "inner CG: a system's rows have columns outside it");
+    std::vector<int> ip2(size_t(n) + 1, 0);
+    for (int k = 0; k < n; ++k) ip2[size_t(k) + 1] = ip2[size_t(k)] + (ip[perm[size_t(k)] + 1] - ip[perm[size_t(k)]]);
+    std::vector<int> ix2(size_t(std::max<int64_t>(P.nnz, 1))), emap(size_t(std::max<int64_t>(P.nnz, 1)));
+    // row k of P A P^T is row perm[k] with its columns renumbered, in ascending order
+#pragma omp parallel
+    {
+        std::vector<std::pair<int, int>> buf;
+#pragma omp for schedule(dynamic, 4096)
+        for (int k = 0; k < n; ++k)
+        {
+            const int b = ip[perm[size_t(k)]], len = ip[perm[size_t(k)] + 1] - b, o = ip2[size_t(k)];
+            buf.resize(size_t(len));
+            for (int t = 0; t < len; ++t) buf[size_t(t)] = {iperm[size_t(ix[b + t])], b + t};
+            std::sort(buf.begin(), buf.end());
+            for (int t = 0; t < len; ++t)
+            {
+                ix2[size_t(o + t)] = buf[size_t(t)].first;
+                emap[size_t(o + t)] = buf[size_t(t)].second;
+            }
+        }
+    }
+    auto L = std::make_unique<LocalOrder>();
+    L->perm = upload(perm.data(), size_t(n), s);
+    L->emap = upload(emap.data(), size_t(P.nnz), s);
+    ix2.resize(size_t(P.nnz));
+    L->pat = make_pattern(n, n, std::move(ip2), std::move(ix2), s);
+    L->pat->segments = P.segments;
+    HIPC(hipStreamSynchronize(s));            // perm, emap go out of scope
+    P.local = std::move(L);
+    return *P.local;
+}
+
+// The LocalOrder the inner CG uses for P, or NULL (femd_inner_order off: the original order).
+const LocalOrder *inner_order(DevPattern &P, hipStream_t s)
+{
+    return inner_order_enabled() ? &local_order(P, s) : nullptr;
+}
+
+// The launch plan of the inner CG's pattern Q (P itself, or its LocalOrder's pattern): the shape
+// femd_spmv_select set on P, or Q's own tuned one.
+const SpmvPlan &local_plan(DevPattern &P, DevPattern &Q, int prec, const void *values, hipStream_t s)
+{
+    if (&Q == &P) return spmv_plan(P, prec, values, s);
+    const int f = P.forced[prec == FEMD_FP32 ? 1 : 0];
+    std::unique_ptr<SpmvPlan> &slot = prec == FEMD_FP32 ? Q.plan32 : Q.plan64;
+    if (f >= 0 && (!slot || slot->variant != f)) slot = std::make_unique<SpmvPlan>(shape_plan(Q, f, prec, s));
+    return spmv_plan(Q, prec, values, s);
+}
+
+// ---- scaled values (cached on the matrix) --------------------------------------------------
+void scaled_values(femd_csr_s *A, hipStream_t s, bool reorder)
+{
     DevPattern &P = *A->pat;
+    const LocalOrder *L = reorder ? inner_order(P, s) : nullptr;
+    if (A->v32s_valid && A->scaled_reordered == (L != nullptr)) return;
+    A->v64s_valid = false;
     if (P.n != P.m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
     const int n = P.n;
     const int *diag = diag_of(P, s);
+    const DevPattern &Q = L ? *L->pat : P;
+    const int *perm = L ? L->perm.as<int>() : nullptr, *emap = L ? L->emap.as<int>() : nullptr;
     A->scale.ensure(sizeof(double) * std::max(n, 1));
     A->diagf.ensure(sizeof(float) * std::max(n, 1));
     A->v32s.ensure(sizeof(float) * std::max<int64_t>(P.nnz, 1));
@@ -564,10 +680,10 @@ void scaled_values(femd_csr_s *A, hipStream_t s)
     HIPC(hipMemcpyAsync(bad.p, &big, sizeof(int), hipMemcpyHostToDevice, s));
     if (n > 0)
     {
-        hipLaunchKernelGGL(k_scale_of_diag, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, diag, A->v.as<double>(),
+        hipLaunchKernelGGL(k_scale_of_diag, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, perm, diag, A->v.as<double>(),
                            A->scale.as<double>(), A->diagf.as<float>(), bad.as<int>());
-        hipLaunchKernelGGL(k_scaled_values<float>, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, P.ip.as<int>(), P.ix.as<int>(),
-                           A->v.as<double>(), A->scale.as<double>(), A->v32s.as<float>());
+        hipLaunchKernelGGL(k_scaled_values<float>, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, Q.ip.as<int>(), Q.ix.as<int>(),
+                           emap, perm, A->v.as<double>(), A->scale.as<double>(), A->v32s.as<float>());
     }
     int badrow = big;
     HIPC(hipMemcpyAsync(&badrow, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -575,18 +691,21 @@ void scaled_values(femd_csr_s *A, hipStream_t s)
     if (badrow != big)
         fail(FEMD_ENOTSPD, "inner CG: diagonal entry of row " + std::to_string(badrow) + " is not positive");
     A->v32s_valid = true;
+    A->scaled_reordered = L != nullptr;
     A->d32s.valid = false;
 }
 
-void scaled_values64(femd_csr_s *A, hipStream_t s)
+void scaled_values64(femd_csr_s *A, hipStream_t s, bool reorder)
 {
-    scaled_values(A, s);
+    scaled_values(A, s, reorder);
     if (A->v64s_valid) return;
-    const DevPattern &P = *A->pat;
-    A->v64s.ensure(sizeof(double) * std::max<int64_t>(P.nnz, 1));
-    if (P.n > 0)
-        hipLaunchKernelGGL(k_scaled_values<double>, dim3((P.n + BT - 1) / BT), dim3(BT), 0, s, P.n, P.ip.as<int>(),
-                           P.ix.as<int>(), A->v.as<double>(), A->scale.as<double>(), A->v64s.as<double>());
+    const LocalOrder *L = A->scaled_reordered ? &local_order(*A->pat, s) : nullptr;
+    const DevPattern &Q = L ? *L->pat : *A->pat;
+    A->v64s.ensure(sizeof(double) * std::max<int64_t>(Q.nnz, 1));
+    if (Q.n > 0)
+        hipLaunchKernelGGL(k_scaled_values<double>, dim3((Q.n + BT - 1) / BT), dim3(BT), 0, s, Q.n, Q.ip.as<int>(),
+                           Q.ix.as<int>(), L ? L->emap.as<int>() : nullptr, L ? L->perm.as<int>() : nullptr,
+                           A->v.as<double>(), A->scale.as<double>(), A->v64s.as<double>());
     A->v64s_valid = true;
     A->d64s.valid = false;
 }
@@ -601,13 +720,17 @@ void CGEngine::drop_graph()
     graph_stream = nullptr;
 }
 
-void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s)
+void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s, bool reorder)
 {
     if (prec != FEMD_FP32 && prec != FEMD_FP64) fail(FEMD_EINVAL, "inner CG: precision is FEMD_FP32 or FEMD_FP64");
     if (M->pat->n != M->pat->m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
-    if (prec == FEMD_FP64) scaled_values64(M, s);
-    else scaled_values(M, s);
-    if (!(A == M && precision == prec && plan))
+    if (prec == FEMD_FP64) scaled_values64(M, s, reorder);
+    else scaled_values(M, s, reorder);
+    // the plan of the pattern in the inner CG's order (femd_spmv_select may have replaced it, or
+    // femd_inner_order switched the order, since the last bind)
+    const LocalOrder *L = M->scaled_reordered ? &local_order(*M->pat, s) : nullptr;
+    const SpmvPlan &want = local_plan(*M->pat, L ? *L->pat : *M->pat, prec, prec == FEMD_FP64 ? M->v64s.p : M->v32s.p, s);
+    if (!(A == M && precision == prec && plan && plan_id == want.id))
     {
         drop_graph();
         A = M;
@@ -616,10 +739,10 @@ void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s)
     }
     // the value dictionary of the scaled values under this plan, when the values allow one
     ValueDict &d = prec == FEMD_FP64 ? M->d64s : M->d32s;
-    const bool want = dict_enabled();
-    if (want && (!d.valid || d.plan != plan))
+    const bool want_dict = dict_enabled();
+    if (want_dict && (!d.valid || d.plan_id != plan_id))
         build_dict(*plan, prec == FEMD_FP64 ? M->v64s.p : M->v32s.p, prec, M->pat->nnz, d, s);
-    const bool use = want && d.valid && d.usable && d.plan == plan;
+    const bool use = want_dict && d.valid && d.usable && d.plan_id == plan_id;
     // a captured graph holds the format and the dictionary's buffers
     if (use != use_dict || (use && d.gen != dict_gen)) drop_graph();
     use_dict = use;
@@ -628,7 +751,10 @@ void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s)
 
 void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
 {
-    DevPattern &P = *M->pat;
+    const LocalOrder *L = M->scaled_reordered ? &local_order(*M->pat, s) : nullptr;
+    DevPattern &P = L ? *L->pat : *M->pat;
+    pat = &P;
+    perm = L ? L->perm.as<int>() : nullptr;
     n = P.n;
     roff = P.segments.empty() ? std::vector<int64_t>{0, int64_t(P.n)} : P.segments;
     np = static_cast<int>(roff.size()) - 1;
@@ -651,7 +777,8 @@ void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
     clo = upload(lo, s);
     cpid = upload(cp.empty() ? std::vector<int>{0} : cp, s);
     cptr = upload(cptr_h, s);
-    plan = &spmv_plan(P, prec, prec == FEMD_FP64 ? static_cast<const void *>(M->v64s.p) : M->v32s.p, s);
+    plan = &local_plan(*M->pat, P, prec, prec == FEMD_FP64 ? M->v64s.p : M->v32s.p, s);
+    plan_id = plan->id;
     seg_tiles = upload(plan->seg_tiles, s);
     // alpha folded into the update when every workgroup's extra reads stay small: each of a
     // system's chunks reads all of its tile partials
@@ -708,7 +835,7 @@ namespace {
 template <typename T, bool DICT>
 void launch_product(CGEngine &E, int odd, hipStream_t s)
 {
-    const DevPattern &P = *E.A->pat;
+    const DevPattern &P = *E.pat;
     const SpmvPlan &pl = *E.plan;
     int th = 0, cap = 0;
     plan_shape(pl, &th, &cap);
@@ -723,7 +850,7 @@ void launch_product(CGEngine &E, int odd, hipStream_t s)
     const int4 *list = E.list.as<int4>();
     const int count = E.list_count, grid = std::max(count, 1);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (!timer_sample(s, &e0, &e1)) e0 = e1 = nullptr;
+    if (!timer_sample(s, &e0, &e1, spmv_variant_name(pl))) e0 = e1 = nullptr;
     bool launched = false;
 #define FEMD_CASE(TH, CP)                                                                                             \
     if constexpr (2 * CP * sizeof(T) + (DICT ? 256 * sizeof(T) : 0) <= 65536)                                        \
@@ -775,7 +902,7 @@ void resume_t(CGEngine &E, const double *r64_dev, const double *rho_dev, const d
 {
     const int next = E.parity;
     hipLaunchKernelGGL(k_cg_rescale<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev, rho_dev,
-                       E.A->scale.as<double>(), r64_dev, E.r.as<T>(), E.y.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(),
+                       E.perm, E.A->scale.as<double>(), r64_dev, E.r.as<T>(), E.y.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(),
                        next, static_cast<const CGSys *>(E.st.as<CGSys>()));
     hipLaunchKernelGGL(k_cg_fresh_dots<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev,
                        E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), next, E.cparts.as<double>());
@@ -787,7 +914,7 @@ template <typename T>
 void finish_t(CGEngine &E, const int *sel_dev, double *x, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cg_finish<T>, dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(), sel_dev,
-                       E.A->scale.as<double>(), E.y.as<T>(), static_cast<const CGSys *>(E.st.as<CGSys>()), x);
+                       E.perm, E.A->scale.as<double>(), E.y.as<T>(), static_cast<const CGSys *>(E.st.as<CGSys>()), x);
 }
 
 } // namespace
@@ -1042,6 +1169,7 @@ int femd_inner_timer(int enable)
         g_timer.on = enable != 0;
         g_timer.used = 0;
         g_timer.launches = 0;
+        g_timer.shape.clear();
     });
 }
 
@@ -1067,6 +1195,18 @@ int femd_inner_timer_read(int64_t *launches, double *total_ms)
         }
         if (launches) *launches = k;
         if (total_ms) *total_ms = t;
+    });
+}
+
+int femd_inner_timer_shape(char *name, int len)
+{
+    using namespace femd_amd;
+    return api([&] {
+        if (!name || len < 1) fail(FEMD_EINVAL, "femd_inner_timer_shape: need a buffer");
+        std::lock_guard<std::mutex> lk(g_timer.mu);
+        const size_t k = std::min(g_timer.shape.size(), size_t(len - 1));
+        std::memcpy(name, g_timer.shape.data(), k);
+        name[k] = '\0';
     });
 }
 

@@ -41,10 +41,11 @@ struct CGSys {
     int pad;
 };
 
-// S = diag(A)^(-1/2) (FP64) and S A S (FP32 / FP64) on the matrix, cached until its values change;
-// FEMD_ENOTSPD when a diagonal entry is not positive.
-void scaled_values(femd_csr_s *A, hipStream_t s);
-void scaled_values64(femd_csr_s *A, hipStream_t s);
+// S = diag(A)^(-1/2) (FP64) and S A S (FP32 / FP64) on the matrix, in the pattern's LocalOrder when
+// `reorder` (and femd_inner_order allows it), else in place; cached until the values or the order
+// change; FEMD_ENOTSPD when a diagonal entry is not positive.
+void scaled_values(femd_csr_s *A, hipStream_t s, bool reorder = true);
+void scaled_values64(femd_csr_s *A, hipStream_t s, bool reorder = true);
 
 struct CGEngine {
     struct Result {
@@ -60,7 +61,10 @@ struct CGEngine {
     int nch = 0;                        // vector chunks (1024 rows at most, one system each)
     std::vector<int> h_cptr;
     DevBuf clo, cpid, cptr;
-    const SpmvPlan *plan = nullptr;
+    DevPattern *pat = nullptr;          // A's pattern in the inner CG's order (its LocalOrder)
+    const int *perm = nullptr;          // position -> unknown of A (device)
+    const SpmvPlan *plan = nullptr;     // pat's plan
+    uint64_t plan_id = 0;
     DevBuf seg_tiles;
     DevBuf st, tparts, cparts, r, q, P0, P1, y, r64, t64, sel, retired, rho_new, tgt, cnt;
     bool use_dict = false;              // the product reads the scaled values through their dictionary
@@ -79,8 +83,9 @@ struct CGEngine {
     ~CGEngine();
 
     // Bind to A (square; its pattern's segments are the systems, none means one system) in a
-    // precision; allocates the work space.  Re-binding to the same matrix keeps it.
-    void setup(femd_csr_s *A, int precision, hipStream_t s);
+    // precision, working in the pattern's LocalOrder when `reorder` (and femd_inner_order allows
+    // it); allocates the work space.  Re-binding to the same matrix keeps it.
+    void setup(femd_csr_s *A, int precision, hipStream_t s, bool reorder = true);
     // count iterations of every running system (graph capturable)
     void iterate(int count, hipStream_t s);
     // GRAPH_ITERS iterations, from a captured graph when use_graph

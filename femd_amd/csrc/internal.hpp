@@ -114,6 +114,7 @@ inline double read_scalar(const double *d, hipStream_t s)
 // One SpMV launch shape ready to launch on one pattern (spmv.hip): its tile table, the tiles'
 // distinct-column lists, and the time the autotuner measured.
 struct SpmvPlan {
+    uint64_t id = 0;               // unique per built plan (a holder of a plan's buffers can tell a new one)
     int variant = 0;
     int ntiles = 0;
     DevBuf blocks;                 // int2 per tile boundary: (first row, first entry)
@@ -132,13 +133,28 @@ struct ValueDict {
     int gen = 0;                   // bumped by every build
     int max_distinct = 0;
     int64_t distinct = 0;          // dictionary entries over all tiles
-    const SpmvPlan *plan = nullptr;  // the tiling it was built for
+    uint64_t plan_id = 0;          // the tiling it was built for (SpmvPlan::id)
     DevBuf doff, dval, vidx;
 };
 // Build d for values v (FP32 or FP64 by `precision`) under plan p (spmv.hip).
 void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, ValueDict &d, hipStream_t s);
 // Whether the solvers use value dictionaries where a matrix allows them (femd_value_dictionary).
 bool dict_enabled();
+// Whether the inner CG works in its own order of the unknowns (femd_inner_order).
+bool inner_order_enabled();
+
+struct DevPattern;
+// The inner CG's order of a square pattern's unknowns (cg_engine.hip): within every segment, the
+// order in which a row-by-row scan of the segment first meets them as columns, those it never meets
+// last in their own order.  Consecutive rows then share most of their columns, so a tile of the
+// reordered pattern gathers fewer distinct columns, from fewer cache lines, than one of a pattern
+// numbered vertices first, then edges, then cell interiors (a P3 tile of 2048 entries: 229 instead
+// of 478 columns, 18 instead of 35 lines of 128 bytes).
+struct LocalOrder {
+    DevBuf perm;                        // new position -> old (n)
+    DevBuf emap;                        // entry of the reordered pattern -> entry of the original (nnz)
+    std::shared_ptr<DevPattern> pat;    // P A P^T's pattern, each row's columns ascending, same segments
+};
 
 struct DevPattern {
     int n = 0, m = 0;
@@ -148,6 +164,8 @@ struct DevPattern {
     DevBuf ip, ix, diag;
     bool have_diag = false;
     std::unique_ptr<SpmvPlan> plan64, plan32;    // tuned launch shapes per precision
+    int forced[2] = {-1, -1};                    // shape set by femd_spmv_select (FP64, FP32); -1: tuned
+    std::unique_ptr<LocalOrder> local;           // built by the inner CG on first use
 };
 
 // The tuned SpMV of a pattern (values give the tuning runs realistic data).  Every shape sums each
@@ -207,12 +225,17 @@ struct femd_csr_s {
     std::shared_ptr<femd_amd::DevPattern> pat;
     femd_amd::DevBuf v, v32;
     bool v32_valid = false;
-    femd_amd::DevBuf v32s, scale;        // S A S in FP32 and S = diag(A)^(-1/2) (FP64), for the inner CG
-    femd_amd::DevBuf diagf;              // diag(A) in FP32 (the inner CG's estimate of the true residual)
+    // The inner CG's copies, in the pattern's LocalOrder (entries of the reordered pattern, rows
+    // in its order): S A S in FP32, S = diag(A)^(-1/2) (FP64, original order), diag(A) in FP32 (the
+    // inner CG's estimate of the true residual), S A S in FP64 (the FP64 inner CG), and the
+    // dictionaries of the scaled values under the reordered pattern's plans.
+    femd_amd::DevBuf v32s, scale;
+    femd_amd::DevBuf diagf;
     bool v32s_valid = false;
-    femd_amd::DevBuf v64s;               // S A S in FP64 (the batched FP64 CG)
+    femd_amd::DevBuf v64s;
     bool v64s_valid = false;
-    femd_amd::ValueDict d32s, d64s;      // dictionaries of v32s and v64s under the pattern's plans
+    bool scaled_reordered = false;       // v32s, diagf (and v64s) are in the LocalOrder (else in place)
+    femd_amd::ValueDict d32s, d64s;
     const femd_space_s *space = nullptr;
 };
 

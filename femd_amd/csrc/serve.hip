@@ -104,7 +104,10 @@ void admit(femd_server_s *S)
     G->left = static_cast<int>(take.size());
     std::vector<femd_csr> mats;
     for (const Request &r : take) mats.push_back(r.A);
-    G->B = batch_build(static_cast<int>(mats.size()), mats.data(), G->stream);
+    // the group's engine keeps the original order: building the reordered pattern on the host for
+    // every group sits on the admission path, and measured it cost the stream more than the faster
+    // iterations gave back (200 requests: 15.3 against 16.0 solves/s, p50 0.89 against 0.50 s)
+    G->B = batch_build(static_cast<int>(mats.size()), mats.data(), G->stream, false);
     const int64_t nr = G->B->nrows;
     G->bcat = DevBuf(8 * size_t(std::max<int64_t>(nr, 1)));
     G->xcat = DevBuf(8 * size_t(std::max<int64_t>(nr, 1)));
@@ -116,7 +119,7 @@ void admit(femd_server_s *S)
         HIPC(hipMemcpyAsync(G->xcat.as<double>() + r0, take[p].x, 8 * size_t(m), hipMemcpyDeviceToDevice, G->stream));
     }
     CGEngine &E = G->B->engine[S->precision == FEMD_FP32 ? 0 : 1];
-    E.setup(G->B->A.get(), S->precision, G->stream);
+    E.setup(G->B->A.get(), S->precision, G->stream, false);
     E.begin(G->bcat.as<double>(), G->xcat.as<double>(), S->tol, S->inner_tol, S->max_outer, true, G->stream);
     S->used += bytes;
     S->running += static_cast<int>(take.size());

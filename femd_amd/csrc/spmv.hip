@@ -50,7 +50,9 @@ void launch_shape(const SpmvPlan &p, const Shape &S, const int *ip, const int *i
 
 SpmvPlan make_plan(const DevPattern &P, int si, hipStream_t s)
 {
+    static std::atomic<uint64_t> built{0};
     SpmvPlan p;
+    p.id = ++built;
     p.variant = si;
     const Shape &S = SHAPES[si];
     // tiles never straddle a segment boundary
@@ -304,13 +306,15 @@ std::atomic<int> g_dict_on{1};
 } // namespace
 
 bool dict_enabled() { return g_dict_on.load() != 0; }
+std::atomic<int> g_order_on{1};
+bool inner_order_enabled() { return g_order_on.load() != 0; }
 
 void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, ValueDict &d, hipStream_t s)
 {
     d.valid = true;
     d.usable = false;
     d.gen += 1;
-    d.plan = &p;
+    d.plan_id = p.id;
     d.max_distinct = 0;
     d.distinct = 0;
     d.doff.release();
@@ -365,6 +369,14 @@ void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, Va
 } // namespace femd_amd
 
 // --------------------------------------------------------------------------------- C ABI
+extern "C" int femd_inner_order(int mode)
+{
+    using namespace femd_amd;
+    const int prev = g_order_on.load();
+    if (mode == 0 || mode == 1) g_order_on.store(mode);
+    return prev;
+}
+
 extern "C" int femd_value_dictionary(int mode)
 {
     using namespace femd_amd;
@@ -410,9 +422,13 @@ extern "C" int femd_spmv_select(femd_csr A, int precision, const char *name)
         femd_csr_s *a = check_csr(A);
         hipStream_t s = S(nullptr);
         std::unique_ptr<SpmvPlan> &slot = precision == FEMD_FP32 ? a->pat->plan32 : a->pat->plan64;
-        if (!name || !*name) { slot.reset(); return; }
-        const int si = shape_index(name, precision);
-        if (si < 0) fail(FEMD_EINVAL, std::string("SpMV shape '") + name + "' is unknown or not available in this precision");
+        const int si = name && *name ? shape_index(name, precision) : -1;
+        if (name && *name && si < 0)
+            fail(FEMD_EINVAL, std::string("SpMV shape '") + name + "' is unknown or not available in this precision");
+        // the inner CG's reordered pattern takes the same shape (or is tuned again) at its next use
+        a->pat->forced[precision == FEMD_FP32 ? 1 : 0] = si;
+        if (a->pat->local) (precision == FEMD_FP32 ? a->pat->local->pat->plan32 : a->pat->local->pat->plan64).reset();
+        if (si < 0) { slot.reset(); return; }
         slot = std::make_unique<SpmvPlan>(shape_plan(*a->pat, si, precision, s));
     });
 }

@@ -39,6 +39,8 @@ def main(argv=None):
     ap.add_argument("--inner-tol", type=float, default=1e-2)
     ap.add_argument("--max-inner", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--original-order", action="store_true",
+                    help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
     args = ap.parse_args(argv)
 
     import femd_amd as fd
@@ -46,6 +48,7 @@ def main(argv=None):
     from femd_amd import problems
     lib = L.load()
     L.check(lib.femd_init(0))
+    lib.femd_inner_order(0 if args.original_order else 1)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()

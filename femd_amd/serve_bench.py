@@ -72,12 +72,15 @@ def main(argv=None):
     ap.add_argument("--inner-tol", type=float, default=0.3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 baseline and the error check")
+    ap.add_argument("--original-order", action="store_true",
+                    help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
     args = ap.parse_args(argv)
 
     import femd_amd as fd
     from femd_amd import _lib as L
     lib = L.load()
     L.check(lib.femd_init(0))
+    lib.femd_inner_order(0 if args.original_order else 1)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
     reqs = draw_requests(args.count, args.rate, args.dof_min, args.dof_max, args.seed)

@@ -378,6 +378,9 @@ int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double 
  * reference. */
 int femd_inner_timer(int enable);
 int femd_inner_timer_read(int64_t *launches, double *total_ms);
+/* The launch shape (e.g. "tile256x2048") of the last timed product launch, NUL-terminated in
+ * name[0 .. len); empty before the first.  No counterpart in the reference. */
+int femd_inner_timer_shape(char *name, int len);
 
 /* ---------------------------------------------------------------- value dictionaries
  * The inner CG (refined solves, batches, time stepping) reads its scaled matrix through a
@@ -386,6 +389,13 @@ int femd_inner_timer_read(int64_t *launches, double *total_ms);
  * same values bit for bit, so results do not change.  mode 1 allows it (the default), 0 turns it
  * off, -1 only queries; returns the previous mode.  No counterpart in the reference. */
 int femd_value_dictionary(int mode);
+/* The inner CG (refined solves, batches, time stepping, the server) works on P A P^T, P numbering
+ * every system's unknowns in the order its rows first meet them as columns, so that an SpMV tile's
+ * rows and the columns it gathers lie close together; vectors at the interface stay in the
+ * caller's order.  mode 1 (the default) uses that order, 0 the original one, -1 only queries;
+ * returns the previous mode.  Results differ between the two only by the rounding of the inner
+ * iterations.  No counterpart in the reference. */
+int femd_inner_order(int mode);
 int femd_event_timer_start(femd_stream s);
 int femd_event_timer_stop(femd_stream s, double *ms);
 

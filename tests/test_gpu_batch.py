@@ -70,3 +70,41 @@ def test_batch_fp64_inner(orc):
         assert it <= 1.25 * io["iterations"] + 5
     with pytest.raises(ValueError):
         B.solve([b for _, _, b in systems], precision="fp16")
+
+
+def test_inner_order_switch(orc):
+    """The inner CG's own order of the unknowns and the original one: both within 1e-8 of the FP64
+    oracle, alone and in the mix the same bits in either, and switching the order back gives the
+    first result again (the scaled values are rebuilt for the order in use)."""
+    print("\n[case] inner order on / off / on, batch of 3 and a refined solve", flush=True)
+    systems = [_system(*s) for s in [(14, 2, 7), (9, 3, 8), (21, 1, 9)]]
+    mats = [A for _, A, _ in systems]
+    bs = [b for _, _, b in systems]
+    refs = []
+    for V, A, b in systems:
+        ip, ix = V.pattern()
+        refs.append(orc.cg(ip, ix, A.data, b, M="jacobi", tol=1e-13)[0])
+    prev = fd.inner_order(True)
+    try:
+        B = fd.Batch(mats)
+        runs = {}
+        for mode in (True, False, True):
+            fd.inner_order(mode)
+            xs, info = B.solve(bs, tol=1e-11, inner_tol=1e-4)
+            assert info["converged"], mode
+            for x, xo in zip(xs, refs):
+                assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8, mode
+            alone, _ = fd.Batch([mats[1]]).solve([bs[1]], tol=1e-11, inner_tol=1e-4)
+            assert np.array_equal(alone[0], xs[1]), mode
+            if mode in runs:
+                for a, b in zip(runs[mode], xs):
+                    assert np.array_equal(a, b)
+            runs[mode] = xs
+        # a single refined solve on one matrix, both orders
+        V, A, b = systems[0]
+        for mode in (False, True):
+            fd.inner_order(mode)
+            x, info = fd.solve_refined(A, b, tol=1e-11, inner_tol=1e-4)
+            assert info.converged and np.linalg.norm(x - refs[0]) / np.linalg.norm(refs[0]) <= 1e-8, mode
+    finally:
+        fd.inner_order(prev)

@@ -107,7 +107,13 @@ def test_fused_inner_cg_every_shape(orc):
     for name in _shapes():
         print(f"\n[case] refined solve with FP32 shape {name}", flush=True)
         L.check(lib.femd_spmv_select(A._h, L.FP32, name.encode()))
+        # the inner CG works on its reordered copy of the pattern, which takes the selected shape
+        L.check(lib.femd_inner_timer(1))
         x, info = fd.solve_refined(A, b, tol=1e-11, inner_tol=1e-4)
+        used = C.create_string_buffer(64)
+        L.check(lib.femd_inner_timer_shape(used, 64))
+        L.check(lib.femd_inner_timer(0))
+        assert used.value.decode() == name
         assert info.converged, name
         assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8, name
         # run to run: the reductions have a fixed order

@@ -17,7 +17,8 @@ steps at the stricter rtol --strict-rtol (1e-10) are reported beside it.
 Also reported: the roofline of the dominant kernel, the FP32 inner CG's product (its
 algorithmic bytes 8 nnz + 20 n + 4 over its average duration, sampled with HIP events on its stream
 in one extra step after the timed ones); the plain SpMV kernels' HBM throughput (12 nnz + 20 n + 4 bytes in FP64,
-8 nnz + 12 n + 4 in FP32), back to back and with the Infinity Cache flushed; the plain FP64 GPU path
+8 nnz + 12 n + 4 in FP32), back to back and with the Infinity Cache flushed, on the matrix as
+numbered and on its copy in the inner CG's order of the unknowns; the plain FP64 GPU path
 (assembly + Jacobi-CG to the same rtol) timed once; p50 / p99 step latency; and the CPU baseline:
 the reference's own code (oracle/_ref) on the host cores, on a bounded sample (the full assembly and
 100 + 300 of its Jacobi-CG iterations, scaled to one solve with its iteration count to the same
@@ -190,6 +191,15 @@ def main():
     gbs = lambda B, t: B / (t * 1e-3) / 1e9
     log(f"SpMV FP64 {t64 * 1e3:.1f} us {gbs(B64, t64):.0f} GB/s (flushed {c64 * 1e3:.1f} us {gbs(B64, c64):.0f} GB/s); "
         f"FP32 {t32 * 1e3:.1f} us {gbs(B32, t32):.0f} GB/s (flushed {c32 * 1e3:.1f} us {gbs(B32, c32):.0f} GB/s)")
+    # the same kernels on the matrix in the inner CG's order of the unknowns (the SpMV its
+    # iterations run); the products above keep the reference's order, whose row sums are its own
+    from femd_amd.roofline import reordered
+    Ar = reordered(A)
+    o64, o32 = spmv_times(Ar, args.spmv_reps)
+    q64, q32 = spmv_times(Ar, max(5, args.spmv_reps // 5), flush=True)
+    del Ar
+    log(f"SpMV in the inner CG's order: FP64 {o64 * 1e3:.1f} us {gbs(B64, o64):.0f} GB/s (flushed {gbs(B64, q64):.0f}); "
+        f"FP32 {o32 * 1e3:.1f} us {gbs(B32, o32):.0f} GB/s (flushed {gbs(B32, q32):.0f})")
     traffic = measured_traffic("k_cg_product")
 
     # ---- CPU baseline: the reference's own code, bounded sample ---------------------------
@@ -218,6 +228,10 @@ def main():
                  "fp32": {"us": round(t32 * 1e3, 2), "gbs": round(gbs(B32, t32), 1), "shape": variants["fp32"],
                           "flushed_us": round(c32 * 1e3, 2), "flushed_gbs": round(gbs(B32, c32), 1),
                           "flushed_frac": round(gbs(B32, c32) / HBM_PEAK_GBS, 4)},
+                 "inner_order": {"fp64_us": round(o64 * 1e3, 2), "fp64_gbs": round(gbs(B64, o64), 1),
+                                 "fp64_flushed_frac": round(gbs(B64, q64) / HBM_PEAK_GBS, 4),
+                                 "fp32_us": round(o32 * 1e3, 2), "fp32_gbs": round(gbs(B32, o32), 1),
+                                 "fp32_flushed_frac": round(gbs(B32, q32) / HBM_PEAK_GBS, 4)},
                  "bytes_fp64": B64, "bytes_fp32": B32},
         "roofline": {"kernel": f"k_cg_product (product of the FP32 inner CG with the direction update, {inner_shape})",
                      "bound": "hbm",

@@ -72,6 +72,23 @@ def spmv_times(A, reps=50, flush=False):
     return out[L.FP64], out[L.FP32]
 
 
+def reordered(A):
+    """A copy of the square matrix A in the inner CG's order of the unknowns (each unknown numbered
+    where a row-by-row scan first meets it as a column, those it never meets last), for timing the
+    SpMV the inner iterations run.  Host-side (NumPy / SciPy) plumbing for measurement."""
+    import scipy.sparse as sp
+    from .sparse import SparseMatrix
+    n = A.shape[0]
+    ip, ix, v = A.indptr, A.indices, A.data
+    cols, first = np.unique(ix, return_index=True)
+    seen = np.zeros(n, dtype=bool)
+    seen[cols] = True
+    perm = np.concatenate([cols[np.argsort(first, kind="stable")], np.flatnonzero(~seen)])
+    S = sp.csr_matrix((v, ix, ip), shape=(n, n))[perm][:, perm].tocsr()
+    S.sort_indices()
+    return SparseMatrix.from_arrays(n, n, S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data)
+
+
 def copy_time(nbytes=64 << 20, reps=50):
     lib = L.load()
     a, b = _dev(lib, nbytes), _dev(lib, nbytes)

@@ -137,7 +137,9 @@ __global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, cons
 // buffers alternate with the iteration (odd: read P1, write P0), the same for every system, so
 // the own-row and column loads do not wait for the system's state; a system that has paused since
 // the list was built skips its tiles (PCGOp::ready) once those loads are in flight.
-template <typename T, int THREADS, int CAP, bool DICT>
+// DICT: 0 the value array, 1 per-tile dictionaries, 2 dictionaries where a tile has one (the
+// others read the value array)
+template <typename T, int THREADS, int CAP, int DICT>
 __global__ __launch_bounds__(THREADS) void k_cg_product(const int4 *__restrict__ list, int count,
                                                         const int *__restrict__ ucol, const unsigned short *__restrict__ lidx,
                                                         const int *__restrict__ ip, const int *__restrict__ ix,
@@ -157,7 +159,10 @@ __global__ __launch_bounds__(THREADS) void k_cg_product(const int4 *__restrict__
     const PCGOp<T> op{r, odd ? P1 : P0, odd ? P0 : P1, q, static_cast<T>(S.beta), S.active};
     const TileDesc td{e0.x, e0.y, e0.z, e1.x, e1.y, e1.z};
     double part;
-    if constexpr (DICT)
+    if constexpr (DICT == 2)
+        part = tile_work<T, THREADS, CAP>(td, e1.w, ucol, lidx, ip, ix, v, nnz, op, xs, prod,
+                                          TileDict<T, true>{vidx, doff, dval, dv});
+    else if constexpr (DICT == 1)
         part = tile_work<T, THREADS, CAP>(td, e1.w, ucol, lidx, ip, ix, v, nnz, op, xs, prod, TileDict<T>{vidx, doff, dval, dv});
     else
         part = tile_work<T, THREADS, CAP>(td, e1.w, ucol, lidx, ip, ix, v, nnz, op, xs, prod);
@@ -746,6 +751,7 @@ void CGEngine::setup(femd_csr_s *M, int prec, hipStream_t s, bool reorder)
     // a captured graph holds the format and the dictionary's buffers
     if (use != use_dict || (use && d.gen != dict_gen)) drop_graph();
     use_dict = use;
+    dict_raw = use && d.raw_tiles > 0;
     dict_gen = use ? d.gen : 0;
 }
 
@@ -832,7 +838,7 @@ void CGEngine::build_list(const std::vector<int> *leave_out, hipStream_t s)
 
 namespace {
 
-template <typename T, bool DICT>
+template <typename T, int DICT>
 void launch_product(CGEngine &E, int odd, hipStream_t s)
 {
     const DevPattern &P = *E.pat;
@@ -878,8 +884,9 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
     for (int k = 0; k < count; ++k)
     {
         const int odd = E.parity;
-        if (E.use_dict) launch_product<T, true>(E, odd, s);
-        else launch_product<T, false>(E, odd, s);
+        if (E.use_dict && E.dict_raw) launch_product<T, 2>(E, odd, s);
+        else if (E.use_dict) launch_product<T, 1>(E, odd, s);
+        else launch_product<T, 0>(E, odd, s);
         if (E.fuse_alpha)
             hipLaunchKernelGGL((k_cg_update<T, true>), dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(),
                                E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,

@@ -68,6 +68,7 @@ struct CGEngine {
     DevBuf seg_tiles;
     DevBuf st, tparts, cparts, r, q, P0, P1, y, r64, t64, sel, retired, rho_new, tgt, cnt;
     bool use_dict = false;              // the product reads the scaled values through their dictionary
+    bool dict_raw = false;              // ... and some tiles, with too many values, read the values
     bool fuse_alpha = false;            // small systems: every update workgroup forms alpha itself
     int parity = 0;                     // the next iteration reads its direction from P1 (1) or P0 (0)
     DevBuf list, aoff;                  // the product's tile list (two int4 per tile) and its offsets

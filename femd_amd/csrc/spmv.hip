@@ -267,6 +267,7 @@ __global__ __launch_bounds__(DICT_THREADS) void k_dict_write(const int2 *__restr
     __shared__ int scan[DICT_THREADS];
     __shared__ unsigned char pos[DICT_SLOTS];
     const int b = blockIdx.x, p0 = blk[b].y, p1 = blk[b + 1].y;
+    if (doff[b + 1] == doff[b]) return;        // a tile with too many values reads the value array
     if (!dict_fill(p0, p1, v, keys, &bad)) return;
     // slots 2t and 2t + 1 belong to thread t; number the occupied ones in slot order
     const int t = threadIdx.x;
@@ -331,17 +332,28 @@ void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, Va
         hipLaunchKernelGGL(k_dict_count<float>, dim3(nt), dim3(DICT_THREADS), 0, s, blk, static_cast<const float *>(v),
                            counts.as<int>());
     std::vector<int> h(size_t(nt) + 1, 0);
+    std::vector<int2> hb(size_t(nt) + 1);
     HIPC(hipMemcpyAsync(h.data(), counts.p, 4 * size_t(nt), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(hb.data(), blk, sizeof(int2) * (size_t(nt) + 1), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    // a tile with more than 256 distinct values keeps reading the value array (an empty
+    // dictionary); the format pays while such tiles hold at most a quarter of the entries
     int mx = 0;
-    int64_t tot = 0;
+    int64_t tot = 0, raw_entries = 0;
+    d.raw_tiles = 0;
     for (int b = 0; b < nt; ++b)
     {
         mx = std::max(mx, h[size_t(b)]);
+        if (h[size_t(b)] > 256)
+        {
+            h[size_t(b)] = 0;
+            raw_entries += hb[size_t(b) + 1].y - hb[size_t(b)].y;
+            ++d.raw_tiles;
+        }
         tot += h[size_t(b)];
     }
     d.max_distinct = mx;
-    if (mx > 256 || tot >= (int64_t(1) << 31)) return;
+    if (4 * raw_entries > nnz || tot >= (int64_t(1) << 31)) return;
     // exclusive offsets
     int run = 0;
     for (int b = 0; b < nt; ++b)

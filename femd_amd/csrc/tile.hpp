@@ -48,13 +48,17 @@ template <> struct Vals4<double> {
 
 // Where a tile's matrix values come from: the value array itself (NoDict), or a per-tile value
 // dictionary -- the tile's distinct values (at most 256) in dval[doff[b] .. doff[b + 1]), staged
-// in LDS (dv), and every entry's 1-byte position in them (vidx).  Both give every entry its exact
-// value, so the two formats give bit-identical products.
+// in LDS (dv), and every entry's 1-byte position in them (vidx).  A tile with more distinct values
+// has an empty dictionary and reads the value array itself.  Every entry gets its exact value
+// either way, so the formats give bit-identical products.
 struct NoDict {
     static constexpr bool on = false;
+    static constexpr bool raw_tiles = false;
 };
-template <typename T> struct TileDict {
+// RAW: some tiles have empty dictionaries and read the value array (a branch per workgroup)
+template <typename T, bool RAW = false> struct TileDict {
     static constexpr bool on = true;
+    static constexpr bool raw_tiles = RAW;
     const unsigned char *vidx;
     const int *doff;
     const T *dval;
@@ -85,6 +89,14 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
     static_assert(!VS::on || THREADS >= 256, "a dictionary of 256 values is staged one per lane");
     const int r0 = td.r0, r1 = td.r1, p0 = td.p0, len = td.p1 - td.p0;
     const int r = r0 + static_cast<int>(threadIdx.x);
+    int d0 = 0, nd = 0;
+    if constexpr (VS::on)
+    {
+        d0 = vs.doff[b];
+        nd = vs.doff[b + 1] - d0;
+    }
+    // the tile reads the value array: no dictionary, or (where the format allows it) too many values
+    const bool raw = !VS::on || (VS::raw_tiles && nd == 0);
     int rb = 0, re = 0;
     typename Op::Own own{};
     if (r < r1)
@@ -104,7 +116,7 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
             for (int p = p0; p < p0 + len; ++p)
             {
                 T a;
-                if constexpr (VS::on) a = vs.dval[vs.doff[b] + vs.vidx[p]];
+                if constexpr (VS::on) a = raw ? v[p] : vs.dval[d0 + vs.vidx[p]];
                 else a = v[p];
                 const T t = a * op.gather(ix[p]);
                 s += t;
@@ -123,13 +135,8 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
         uc[i] = u < nu ? __builtin_nontemporal_load(ucol + u0 + u) : 0;
     }
     T dvv = 0;
-    int d0 = 0, nd = 0;
     if constexpr (VS::on)
-    {
-        d0 = vs.doff[b];
-        nd = vs.doff[b + 1] - d0;
         if (static_cast<int>(threadIdx.x) < nd) dvv = vs.dval[d0 + threadIdx.x];
-    }
     const int a0 = p0 & ~3, e = p0 + len;
     Vals4<T> vv[NGRP];
     u32 di[NGRP];
@@ -142,8 +149,8 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
         {
             if (int64_t(a) + 4 <= nnz)
             {
-                if constexpr (VS::on) di[g] = __builtin_nontemporal_load(reinterpret_cast<const u32 *>(vs.vidx + a));
-                else vv[g].load(v, a);
+                if (raw) vv[g].load(v, a);
+                else if constexpr (VS::on) di[g] = __builtin_nontemporal_load(reinterpret_cast<const u32 *>(vs.vidx + a));
                 cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
             }
             else
@@ -153,8 +160,8 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
                 for (int j = 0; j < 4; ++j)
                 {
                     const bool in = int64_t(a) + j < nnz;
-                    if constexpr (VS::on) di[g] |= (in ? u32(vs.vidx[a + j]) : 0u) << (8 * j);
-                    else vv[g].set(j, in ? v[a + j] : T(0));
+                    if (raw) vv[g].set(j, in ? v[a + j] : T(0));
+                    else if constexpr (VS::on) di[g] |= (in ? u32(vs.vidx[a + j]) : 0u) << (8 * j);
                     cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
                 }
             }
@@ -195,8 +202,8 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
             {
                 const int k = a + j - p0;
                 T val;
-                if constexpr (VS::on) val = vs.dv[(di[g] >> (8 * j)) & 0xffu];
-                else val = vv[g].get(j);
+                if (raw) val = vv[g].get(j);
+                else if constexpr (VS::on) val = vs.dv[(di[g] >> (8 * j)) & 0xffu];
                 if (k >= 0 && k < len) prod[k] = val * xs[cc[g][j]];
             }
         }

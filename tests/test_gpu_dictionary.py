@@ -1,7 +1,8 @@
 """Per-tile value dictionaries in the inner CG: where a matrix allows one (at most 256 distinct
 values per SpMV tile) the product reads 1-byte positions into an LDS table instead of the values.
-The table holds the values bit for bit, so every solver gives bit-identical results with the
-dictionary on and off; a matrix with too many distinct values keeps the plain format."""
+A tile with more distinct values reads the values themselves; a matrix where such tiles hold more
+than a quarter of the entries keeps the plain format.  The table holds the values bit for bit, so
+every solver gives bit-identical results with the dictionary on and off."""
 import numpy as np
 import pytest
 
@@ -75,11 +76,37 @@ def test_too_many_values_keep_plain_format(dictionary_switch):
     xs, info = B.solve([b], tol=1e-10)
     assert info["converged"]
     assert B.dictionary("fp32") is None
-    # one FP64 tile of this matrix holds about 310 distinct scaled values; FP32 rounding merges them
+    # FP64 tiles of this matrix hold up to about 310 distinct scaled values, FP32 rounding merges
+    # them: whatever the format, the FP64 solve gives the same bits as without dictionaries
     V, A, b = _system(10, 3, 16)
     B = fd.Batch([A])
     x64, _ = B.solve([b], tol=1e-10, precision="fp64", inner_tol=0.5)
-    assert B.dictionary("fp64") is None
+    fd.value_dictionary(False)
+    y64, _ = B.solve([b], tol=1e-10, precision="fp64", inner_tol=0.5)
+    fd.value_dictionary(True)
+    assert np.array_equal(x64[0], y64[0])
     x32, _ = B.solve([b], tol=1e-10)
     assert B.dictionary("fp32") is not None
     assert np.linalg.norm(x32[0] - x64[0]) <= 1e-8 * np.linalg.norm(x64[0])
+
+
+def test_tiles_with_many_values_read_the_values(dictionary_switch):
+    """A batch whose variable-coefficient member has tiles of more than 256 distinct values: those
+    tiles read the values themselves, the others their dictionaries, and every result has the same
+    bits as with dictionaries off."""
+    print("\n[case] batch: constant-coefficient P2 systems and a small variable-coefficient P1", flush=True)
+    kap, _ = problems.kappa_field(2)
+    systems = [_system(34, 2, 17), _system(26, 2, 18), _system(12, 1, 19, kappa=kap)]
+    mats, bs = [A for _, A, _ in systems], [b for _, _, b in systems]
+    B = fd.Batch(mats)
+    for prec, kw in {"fp32": dict(tol=1e-10, inner_tol=0.3), "fp64": dict(tol=1e-10, inner_tol=0.5)}.items():
+        fd.value_dictionary(True)
+        xs, info = B.solve(bs, precision=prec, **kw)
+        assert info["converged"]
+        d = B.dictionary(prec)
+        assert d is not None and d[1] > 256, d
+        fd.value_dictionary(False)
+        ys, infoy = B.solve(bs, precision=prec, **kw)
+        assert B.dictionary(prec) is None
+        assert all(np.array_equal(a, c) for a, c in zip(xs, ys)), prec
+        assert np.array_equal(info["iterations"], infoy["iterations"])

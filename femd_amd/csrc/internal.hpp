@@ -131,7 +131,7 @@ struct ValueDict {
     bool valid = false;            // built for the current values
     bool usable = false;           // tiles of more than 256 distinct values hold at most a quarter of the entries
     int gen = 0;                   // bumped by every build
-    int max_distinct = 0;          // most distinct values in one tile
+    int max_distinct = 0;          // most distinct values in one tile (385: more than 384, not counted on)
     int raw_tiles = 0;             // tiles with more than 256, which read the value array
     int64_t distinct = 0;          // dictionary entries over all tiles
     uint64_t plan_id = 0;          // the tiling it was built for (SpmvPlan::id)

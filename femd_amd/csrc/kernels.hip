@@ -309,6 +309,17 @@ __global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int *__restrict
 // point mapped as the cache does: x = x0 + J00 xi + J01 eta on a triangle
 // (fe/lagrange_space_2d.hpp:78-85), the bilinear map on a quadrilateral
 // (fe/lagrange_space_quad.hpp:155-162).
+// The library functions of the programs, called out of line: inlined into every case of the
+// interpreter's switch they drive its register demand (and so its occupancy) for all programs.
+__device__ __noinline__ double op_pow(double r, double n) { return pow(r, n); }
+__device__ __noinline__ double op_sin(double r) { return sin(r); }
+__device__ __noinline__ double op_cos(double r) { return cos(r); }
+__device__ __noinline__ double op_exp(double r) { return exp(r); }
+__device__ __noinline__ double op_log(double r) { return log(r); }
+__device__ __noinline__ double op_tanh(double r) { return tanh(r); }
+__device__ __noinline__ double op_sinh(double r) { return sinh(r); }
+__device__ __noinline__ double op_cosh(double r) { return cosh(r); }
+
 __device__ inline double run_program(double x, double y, double nx, double ny, int ninstr, const int *__restrict__ code,
                                      const double *__restrict__ consts)
 {
@@ -330,18 +341,18 @@ __device__ inline double run_program(double x, double y, double nx, double ny, i
         case 8:
         {
             const double n = consts[b], r = reg[a];
-            v = n == 2.0 ? r * r : n == -1.0 ? 1.0 / r : n == 0.5 ? sqrt(r) : pow(r, n);
+            v = n == 2.0 ? r * r : n == -1.0 ? 1.0 / r : n == 0.5 ? sqrt(r) : op_pow(r, n);
             break;
         }
-        case 9: v = sin(reg[a]); break;
-        case 10: v = cos(reg[a]); break;
-        case 11: v = exp(reg[a]); break;
-        case 12: v = log(reg[a]); break;
-        case 13: v = tanh(reg[a]); break;
+        case 9: v = op_sin(reg[a]); break;
+        case 10: v = op_cos(reg[a]); break;
+        case 11: v = op_exp(reg[a]); break;
+        case 12: v = op_log(reg[a]); break;
+        case 13: v = op_tanh(reg[a]); break;
         case 14: v = sqrt(reg[a]); break;
-        case 15: v = sinh(reg[a]); break;
-        case 16: v = cosh(reg[a]); break;
-        case 17: { const double t = exp(-fabs(reg[a])); v = 2.0 * t / (1.0 + t * t); break; }
+        case 15: v = op_sinh(reg[a]); break;
+        case 16: v = op_cosh(reg[a]); break;
+        case 17: { const double t = op_exp(-fabs(reg[a])); v = 2.0 * t / (1.0 + t * t); break; }
         case 18: v = fabs(reg[a]); break;
         case 19: v = reg[a] > 0.0 ? 1.0 : (reg[a] < 0.0 ? -1.0 : 0.0); break;
         default: v = 0.0; break;

@@ -211,6 +211,7 @@ constexpr int DICT_SLOTS = 512;
 constexpr int DICT_THREADS = 256;
 constexpr unsigned long long DICT_EMPTY = ~0ull;
 constexpr int DICT_OVERFLOW = 1 << 20;
+constexpr int DICT_FULL = 384;             // a tile with more distinct values stops inserting (overflow)
 
 __device__ inline unsigned long long dict_key(float v) { return __float_as_uint(v); }
 __device__ inline unsigned long long dict_key(double v) { return static_cast<unsigned long long>(__double_as_longlong(v)); }
@@ -219,15 +220,22 @@ template <> __device__ inline float dict_value<float>(unsigned long long k) { re
 template <> __device__ inline double dict_value<double>(unsigned long long k) { return __longlong_as_double(static_cast<long long>(k)); }
 __device__ inline int dict_hash(unsigned long long k) { return static_cast<int>((k * 0x9E3779B97F4A7C15ull) >> 55); }
 
-// Insert the values of tile b into keys; false (in every thread) when the table or a key cannot hold them.
+// Insert the values of tile b into keys; false (in every thread) when the table or a key cannot hold
+// them, or when the tile has more than DICT_FULL distinct values (the inserts stop there: a nearly
+// full table makes every further insert probe most of it, and such a tile is beyond any dictionary).
 template <typename T>
-__device__ bool dict_fill(int p0, int p1, const T *__restrict__ v, unsigned long long *keys, int *bad)
+__device__ bool dict_fill(int p0, int p1, const T *__restrict__ v, unsigned long long *keys, int *bad, int *filled)
 {
     for (int i = threadIdx.x; i < DICT_SLOTS; i += DICT_THREADS) keys[i] = DICT_EMPTY;
-    if (threadIdx.x == 0) *bad = 0;
+    if (threadIdx.x == 0)
+    {
+        *bad = 0;
+        *filled = 0;
+    }
     __syncthreads();
     for (int p = p0 + static_cast<int>(threadIdx.x); p < p1; p += DICT_THREADS)
     {
+        if (*static_cast<volatile int *>(bad)) break;
         const unsigned long long k = dict_key(v[p]);
         if (k == DICT_EMPTY) { *bad = 1; continue; }
         int h = dict_hash(k);
@@ -235,7 +243,12 @@ __device__ bool dict_fill(int p0, int p1, const T *__restrict__ v, unsigned long
         {
             if (probe == DICT_SLOTS) { *bad = 1; break; }
             const unsigned long long prev = atomicCAS(&keys[h], DICT_EMPTY, k);
-            if (prev == DICT_EMPTY || prev == k) break;
+            if (prev == DICT_EMPTY)
+            {
+                if (atomicAdd(filled, 1) >= DICT_FULL) *bad = 1;
+                break;
+            }
+            if (prev == k) break;
             h = (h + 1) & (DICT_SLOTS - 1);
         }
     }
@@ -249,9 +262,9 @@ template <typename T>
 __global__ __launch_bounds__(DICT_THREADS) void k_dict_count(const int2 *__restrict__ blk, const T *__restrict__ v, int *counts)
 {
     __shared__ unsigned long long keys[DICT_SLOTS];
-    __shared__ int bad;
+    __shared__ int bad, filled;
     const int b = blockIdx.x;
-    const bool ok = dict_fill(blk[b].y, blk[b + 1].y, v, keys, &bad);
+    const bool ok = dict_fill(blk[b].y, blk[b + 1].y, v, keys, &bad, &filled);
     int c = 0;
     for (int i = threadIdx.x; i < DICT_SLOTS; i += DICT_THREADS) c += keys[i] != DICT_EMPTY ? 1 : 0;
     const int total = __syncthreads_count(c >= 1) + __syncthreads_count(c >= 2);
@@ -263,12 +276,12 @@ __global__ __launch_bounds__(DICT_THREADS) void k_dict_write(const int2 *__restr
                                                              const int *__restrict__ doff, T *dval, unsigned char *vidx)
 {
     __shared__ unsigned long long keys[DICT_SLOTS];
-    __shared__ int bad;
+    __shared__ int bad, filled;
     __shared__ int scan[DICT_THREADS];
     __shared__ unsigned char pos[DICT_SLOTS];
     const int b = blockIdx.x, p0 = blk[b].y, p1 = blk[b + 1].y;
     if (doff[b + 1] == doff[b]) return;        // a tile with too many values reads the value array
-    if (!dict_fill(p0, p1, v, keys, &bad)) return;
+    if (!dict_fill(p0, p1, v, keys, &bad, &filled)) return;
     // slots 2t and 2t + 1 belong to thread t; number the occupied ones in slot order
     const int t = threadIdx.x;
     const bool o0 = keys[2 * t] != DICT_EMPTY, o1 = keys[2 * t + 1] != DICT_EMPTY;
@@ -343,7 +356,7 @@ void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, Va
     d.raw_tiles = 0;
     for (int b = 0; b < nt; ++b)
     {
-        mx = std::max(mx, h[size_t(b)]);
+        mx = std::max(mx, std::min(h[size_t(b)], DICT_FULL + 1));     // DICT_FULL + 1: more than DICT_FULL
         if (h[size_t(b)] > 256)
         {
             h[size_t(b)] = 0;

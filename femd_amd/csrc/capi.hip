@@ -1444,6 +1444,36 @@ int femd_precond_info(femd_precond M, int out[3])
     });
 }
 
+int femd_inner_order_pattern(int n, const int *indptr, const int *indices, int nseg, const int64_t *segments, int *perm,
+                             int *indptr2, int *indices2, int *emap)
+{
+    return api([&] {
+        if (n < 0 || nseg < 0 || !indptr || (n > 0 && !indices) || !perm || !indptr2 || (nseg > 0 && !segments))
+            fail(FEMD_EINVAL, "femd_inner_order_pattern: bad arguments");
+        if (indptr[0] != 0) fail(FEMD_EINVAL, "femd_inner_order_pattern: indptr[0] must be 0");
+        for (int i = 0; i < n; ++i)
+            if (indptr[i + 1] < indptr[i]) fail(FEMD_EINVAL, "femd_inner_order_pattern: indptr decreases");
+        if (n > 0 && indptr[n] > 0 && (!indices2 || !emap)) fail(FEMD_EINVAL, "femd_inner_order_pattern: null output");
+        std::vector<int64_t> seg;
+        if (nseg > 0)
+        {
+            seg.assign(segments, segments + nseg + 1);
+            if (seg.front() != 0 || seg.back() != n) fail(FEMD_EINVAL, "femd_inner_order_pattern: segments must cover the rows");
+            for (int k = 0; k < nseg; ++k)
+                if (seg[size_t(k) + 1] < seg[size_t(k)]) fail(FEMD_EINVAL, "femd_inner_order_pattern: segments decrease");
+        }
+        for (int p = 0; p < (n > 0 ? indptr[n] : 0); ++p)
+            if (indices[p] < 0 || indices[p] >= n) fail(FEMD_EINVAL, "femd_inner_order_pattern: column index out of range");
+        std::vector<int> pm, ip2, ix2, em;
+        if (!first_met_order(n, indptr, indices, seg, pm, ip2, ix2, em))
+            fail(FEMD_EINVAL, "femd_inner_order_pattern: a segment's rows have columns outside it");
+        std::copy(pm.begin(), pm.end(), perm);
+        std::copy(ip2.begin(), ip2.end(), indptr2);
+        std::copy(ix2.begin(), ix2.end(), indices2);
+        std::copy(em.begin(), em.end(), emap);
+    });
+}
+
 int femd_rcm(int n, const int *indptr, const int *indices, int *perm)
 {
     return api([&] {

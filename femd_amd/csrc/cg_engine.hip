@@ -583,64 +583,14 @@ const LocalOrder &local_order(DevPattern &P, hipStream_t s)
     if (P.local) return *P.local;
     if (P.n != P.m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
     const int n = P.n;
-    const std::vector<int64_t> seg = P.segments.empty() ? std::vector<int64_t>{0, int64_t(n)} : P.segments;
-    const int nseg = static_cast<int>(seg.size()) - 1;
-    const int *ip = P.h_ip.data(), *ix = P.h_ix.data();
-    std::vector<int> perm(size_t(std::max(n, 1))), iperm(size_t(std::max(n, 1)), -1);
-    int outside = 0;
-    // within each segment: columns in the order the rows first meet them, then the unmet ones
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : outside)
-    for (int k = 0; k < nseg; ++k)
-    {
-        const int r0 = static_cast<int>(seg[size_t(k)]), r1 = static_cast<int>(seg[size_t(k) + 1]);
-        int next = r0;
-        for (int e = ip[r0]; e < ip[r1]; ++e)
-        {
-            const int j = ix[e];
-            if (j < r0 || j >= r1)
-            {
-                ++outside;
-                break;
-            }
-            if (iperm[size_t(j)] < 0)
-            {
-                iperm[size_t(j)] = next;
-                perm[size_t(next++)] = j;
-            }
-        }
-        for (int j = r0; j < r1; ++j)
-            if (iperm[size_t(j)] < 0)
-            {
-                iperm[size_t(j)] = next;
-                perm[size_t(next++)] = j;
-            }
-    }
-    if (outside) fail(FEMD_EINVAL, "inner CG: a system's rows have columns outside it");
-    std::vector<int> ip2(size_t(n) + 1, 0);
-    for (int k = 0; k < n; ++k) ip2[size_t(k) + 1] = ip2[size_t(k)] + (ip[perm[size_t(k)] + 1] - ip[perm[size_t(k)]]);
-    std::vector<int> ix2(size_t(std::max<int64_t>(P.nnz, 1))), emap(size_t(std::max<int64_t>(P.nnz, 1)));
-    // row k of P A P^T is row perm[k] with its columns renumbered, in ascending order
-#pragma omp parallel
-    {
-        std::vector<std::pair<int, int>> buf;
-#pragma omp for schedule(dynamic, 4096)
-        for (int k = 0; k < n; ++k)
-        {
-            const int b = ip[perm[size_t(k)]], len = ip[perm[size_t(k)] + 1] - b, o = ip2[size_t(k)];
-            buf.resize(size_t(len));
-            for (int t = 0; t < len; ++t) buf[size_t(t)] = {iperm[size_t(ix[b + t])], b + t};
-            std::sort(buf.begin(), buf.end());
-            for (int t = 0; t < len; ++t)
-            {
-                ix2[size_t(o + t)] = buf[size_t(t)].first;
-                emap[size_t(o + t)] = buf[size_t(t)].second;
-            }
-        }
-    }
+    std::vector<int> perm, ip2, ix2, emap;
+    if (!first_met_order(n, P.h_ip.data(), P.h_ix.data(), P.segments, perm, ip2, ix2, emap))
+        fail(FEMD_EINVAL, "inner CG: a system's rows have columns outside it");
+    perm.resize(size_t(std::max(n, 1)));
+    emap.resize(size_t(std::max<int64_t>(P.nnz, 1)));
     auto L = std::make_unique<LocalOrder>();
     L->perm = upload(perm.data(), size_t(n), s);
     L->emap = upload(emap.data(), size_t(P.nnz), s);
-    ix2.resize(size_t(P.nnz));
     L->pat = make_pattern(n, n, std::move(ip2), std::move(ix2), s);
     L->pat->segments = P.segments;
     HIPC(hipStreamSynchronize(s));            // perm, emap go out of scope

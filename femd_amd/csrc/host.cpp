@@ -1054,6 +1054,69 @@ void spmv_tile_columns(const int *indptr, const int *indices, const std::vector<
         std::copy(lists[b].begin(), lists[b].end(), ucol.begin() + uptr[b]);
 }
 
+// --------------------------------------------------------------------------- the inner CG's order
+bool first_met_order(int n, const int *ip, const int *ix, const std::vector<int64_t> &segments, std::vector<int> &perm,
+                     std::vector<int> &ip2, std::vector<int> &ix2, std::vector<int> &emap)
+{
+    const std::vector<int64_t> seg = segments.empty() ? std::vector<int64_t>{0, int64_t(n)} : segments;
+    const int nseg = static_cast<int>(seg.size()) - 1;
+    const int nnz = n > 0 ? ip[n] : 0;
+    perm.assign(size_t(n), 0);
+    std::vector<int> iperm(size_t(n), -1);
+    int outside = 0;
+    // within each segment: columns in the order the rows first meet them, then the unmet ones
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : outside)
+    for (int k = 0; k < nseg; ++k)
+    {
+        const int r0 = static_cast<int>(seg[size_t(k)]), r1 = static_cast<int>(seg[size_t(k) + 1]);
+        int next = r0;
+        for (int e = ip[r0]; e < ip[r1]; ++e)
+        {
+            const int j = ix[e];
+            if (j < r0 || j >= r1)
+            {
+                ++outside;
+                break;
+            }
+            if (iperm[size_t(j)] < 0)
+            {
+                iperm[size_t(j)] = next;
+                perm[size_t(next++)] = j;
+            }
+        }
+        for (int j = r0; j < r1; ++j)
+            if (iperm[size_t(j)] < 0)
+            {
+                iperm[size_t(j)] = next;
+                perm[size_t(next++)] = j;
+            }
+    }
+    if (outside) return false;
+    ip2.assign(size_t(n) + 1, 0);
+    for (int k = 0; k < n; ++k) ip2[size_t(k) + 1] = ip2[size_t(k)] + (ip[perm[size_t(k)] + 1] - ip[perm[size_t(k)]]);
+    ix2.assign(size_t(nnz), 0);
+    emap.assign(size_t(nnz), 0);
+    // row k of P A P^T is row perm[k] with its columns renumbered, in ascending order
+#pragma omp parallel
+    {
+        std::vector<std::pair<int, int>> buf;
+#pragma omp for schedule(dynamic, 4096)
+        for (int k = 0; k < n; ++k)
+        {
+            const int b = ip[perm[size_t(k)]], len = ip[perm[size_t(k)] + 1] - b, o = ip2[size_t(k)];
+            buf.resize(size_t(len));
+            for (int t = 0; t < len; ++t) buf[size_t(t)] = {iperm[size_t(ix[b + t])], b + t};
+            std::sort(buf.begin(), buf.end());
+            for (int t = 0; t < len; ++t)
+            {
+                ix2[size_t(o + t)] = buf[size_t(t)].first;
+                emap[size_t(o + t)] = buf[size_t(t)].second;
+            }
+        }
+    }
+    return true;
+}
+
 // --------------------------------------------------------------------------------- orderings
 // sparse/amd.hpp:166-234: symmetric adjacency without the diagonal; seeds by increasing degree
 // (stable); George-Liu pseudo-peripheral search (at most 32 sweeps); breadth first with the new

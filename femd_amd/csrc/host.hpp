@@ -137,6 +137,13 @@ void spmv_tile_columns(const int *indptr, const int *indices, const std::vector<
 
 // sparse/amd.hpp:166-234 semantics: p[k] is the original row that becomes row k.
 std::vector<int> rcm_order(int n, const int *indptr, const int *indices);
+// The inner CG's order of a square pattern's unknowns: within every segment (row ranges; none: one),
+// the order in which a row-by-row scan first meets them as columns, those it never meets last in
+// their own order (perm: new -> old), and the reordered pattern P A P^T (ip2, ix2; each row's
+// columns ascending) with emap[p] the entry of the original behind entry p of the reordered one.
+// false when a segment's rows have columns outside it.
+bool first_met_order(int n, const int *ip, const int *ix, const std::vector<int64_t> &segments, std::vector<int> &perm,
+                     std::vector<int> &ip2, std::vector<int> &ix2, std::vector<int> &emap);
 
 // B = P A P^T on the pattern level (csr.hpp:286-313): the new pattern and, per new entry,
 // the position of its value in A.

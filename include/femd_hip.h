@@ -396,6 +396,14 @@ int femd_value_dictionary(int mode);
  * returns the previous mode.  Results differ between the two only by the rounding of the inner
  * iterations.  No counterpart in the reference. */
 int femd_inner_order(int mode);
+/* The order femd_inner_order uses, computed on the host (no device needed): perm[k] (n) the
+ * unknown at position k -- within every segment (row ranges segments[0..nseg], nseg 0: all rows),
+ * the order a row-by-row scan first meets the unknowns as columns, unmet ones last -- and the
+ * reordered pattern P A P^T: indptr2 (n + 1), indices2 (nnz, each row ascending), emap[p] (nnz)
+ * the entry of the original pattern behind entry p.  FEMD_EINVAL when a segment's rows have
+ * columns outside it.  No counterpart in the reference. */
+int femd_inner_order_pattern(int n, const int *indptr, const int *indices, int nseg, const int64_t *segments, int *perm,
+                             int *indptr2, int *indices2, int *emap);
 int femd_event_timer_start(femd_stream s);
 int femd_event_timer_stop(femd_stream s, double *ms);
 

@@ -282,3 +282,69 @@ def test_boundary_facets_vs_oracle(orc, k):
         V.lift({1: 1.0})                         # side 1 is natural here
     with pytest.raises(ValueError):
         U.lift(1.0)                              # nothing eliminated
+
+
+def _first_met(ip, ix, segments):
+    """The inner CG's order restated from its definition: per segment, the unknowns in the order a
+    row-by-row scan of the segment's entries first meets them as columns, then the unmet ones."""
+    perm = []
+    for r0, r1 in zip(segments[:-1], segments[1:]):
+        seen = []
+        met = set()
+        for j in ix[ip[r0]:ip[r1]]:
+            if j not in met:
+                met.add(int(j))
+                seen.append(int(j))
+        perm += seen + [j for j in range(r0, r1) if j not in met]
+    return np.array(perm, dtype=np.int64)
+
+
+def test_inner_order_pattern_first_met():
+    """The host construction of the inner CG's order and of P A P^T: the permutation against its
+    definition, the reordered pattern and the entry map against SciPy's permutation of a matrix
+    whose values are the entry numbers; an isolated unknown goes last in its segment; a segment
+    whose rows reach outside it is refused."""
+    import scipy.sparse as sp
+    lib = _lib.load()
+    rng = np.random.default_rng(11)
+    sizes = [7, 12, 1, 5, 9]
+    blocks = []
+    for k, m in enumerate(sizes):
+        a = (rng.random((m, m)) < 0.3).astype(float)
+        a = a + a.T + np.eye(m)
+        if k == 3:
+            a[2, :] = 0.0
+            a[:, 2] = 0.0                     # unknown 2 of this segment: no entries at all
+        blocks.append(sp.csr_matrix(a))
+    S = sp.block_diag(blocks, format="csr")
+    S.sort_indices()
+    n, nnz = S.shape[0], S.nnz
+    ip, ix = S.indptr.astype(np.int32), S.indices.astype(np.int32)
+    segments = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    perm = np.empty(n, np.int32)
+    ip2 = np.empty(n + 1, np.int32)
+    ix2 = np.empty(max(nnz, 1), np.int32)
+    emap = np.empty(max(nnz, 1), np.int32)
+    _lib.check(lib.femd_inner_order_pattern(n, ip, ix, len(sizes), segments.ctypes.data, perm, ip2, ix2, emap))
+    want = _first_met(ip, ix, segments)
+    assert np.array_equal(perm, want)
+    assert sorted(perm.tolist()) == list(range(n))
+    iso = segments[3] + 2
+    assert perm[segments[4] - 1] == iso
+    E = sp.csr_matrix((np.arange(nnz, dtype=np.float64), ix, ip), shape=(n, n))
+    Q = E[want][:, want].tocsr()
+    Q.sort_indices()
+    assert np.array_equal(ip2, Q.indptr) and np.array_equal(ix2[:nnz], Q.indices)
+    assert np.array_equal(emap[:nnz], Q.data.astype(np.int64))
+    # one segment (no segment list): the whole matrix
+    _lib.check(lib.femd_inner_order_pattern(n, ip, ix, 0, None, perm, ip2, ix2, emap))
+    assert np.array_equal(perm, _first_met(ip, ix, [0, n]))
+    # rows of one segment with a column in another
+    bad = S.tolil()
+    bad[0, n - 1] = 1.0
+    bad = bad.tocsr()
+    bad.sort_indices()
+    with pytest.raises(ValueError):
+        _lib.check(lib.femd_inner_order_pattern(n, bad.indptr.astype(np.int32), bad.indices.astype(np.int32), len(sizes),
+                                                segments.ctypes.data, perm, ip2, np.empty(bad.nnz, np.int32),
+                                                np.empty(bad.nnz, np.int32)))

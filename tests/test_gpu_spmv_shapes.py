@@ -42,6 +42,7 @@ def _cases(orc):
     out["ragged 5003"] = (*_csr(rng, lens, 5003), 5003)
     lens = rng.integers(5, 30, 700)
     lens[[3, 350, 699]] = [3000, 9000, 5000]                 # beyond the staging buffer of some or all shapes
+    lens[[100, 200]] = [300, 600]                            # within it, with hundreds of distinct columns
     out["long rows"] = (*_csr(rng, lens, 12000), 12000)
     P = orc.rectangle(32, 32, 2)
     ip, ix = P.pattern()

@@ -144,7 +144,9 @@ const SpmvPlan &spmv_plan(DevPattern &P, int precision, const void *values, hipS
         // segmented (batched) patterns use fixed shapes, so that a segment is tiled -- and its
         // dot products summed -- the same way whatever else is in the batch
         // (measured on the 256-system P2 batch: FP64 256x1024 2.36 s, 512x2048 2.36 s, 256x2048
-        // 2.55 s; FP32 256x2048 4.78 s, 512x4096 5.07 s, 512x2048 6.32 s)
+        // 2.55 s; FP32 256x2048 4.78 s, 512x4096 5.07 s, 512x2048 6.32 s; in the inner CG's order
+        // of the unknowns FP64 256x1024 1.99 s, 512x2048 2.06 s, 256x2048 2.28 s, FP32 256x2048
+        // 4.0 s, 512x4096 4.27 s, 256x1024 5.47 s)
         slot = std::make_unique<SpmvPlan>(make_plan(P, shape_index(precision == FEMD_FP32 ? "tile256x2048" : "tile256x1024",
                                                                    precision), s));
     }

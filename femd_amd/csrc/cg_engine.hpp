@@ -21,6 +21,12 @@
 // recomputed and the system either retired or resumed from it (reliable updates): r = S r_64 /
 // rho_new, y = 0, the direction kept (scaled by rho / rho_new) with the beta that makes the next
 // direction A-conjugate to it, -(r.q)/(p.q).  A breakdown restarts the system from p = 0.
+//
+// The inner vectors (r, p, q, y), the scaled values and diag(A) are kept in the pattern's
+// LocalOrder (every system's unknowns in the order its rows first meet them as columns) unless
+// the caller or femd_inner_order asks for the original order; the FP64 vectors (b, x, the
+// residual) stay in the caller's order and meet the inner ones only in the rescale and fold-back
+// kernels.
 #pragma once
 #include "internal.hpp"
 

@@ -1028,9 +1028,16 @@ int femd_csr_asymmetry(femd_csr A, double *out)
         femd_csr_s *a = check_csr(A);
         if (a->pat->n != a->pat->m) { *out = 1.0; return; }
         hipStream_t s = S(nullptr);
+        DevPattern &P = *a->pat;
+        if (!P.have_tpos)
+        {
+            P.tpos = DevBuf(4 * size_t(std::max<int64_t>(P.nnz, 1)));
+            launch_transpose_positions(P.n, P.ip.as<int>(), P.ix.as<int>(), P.tpos.as<int>(), s);
+            P.have_tpos = true;
+        }
         DevBuf r(16);
         launch_fill(2, 0.0, r.as<double>(), s);
-        launch_asymmetry(a->pat->n, a->pat->ip.as<int>(), a->pat->ix.as<int>(), a->v.as<double>(), r.as<double>(), s);
+        launch_asymmetry(P.nnz, P.tpos.as<int>(), a->v.as<double>(), r.as<double>(), s);
         double h[2];
         HIPC(hipMemcpyAsync(h, r.p, 16, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));

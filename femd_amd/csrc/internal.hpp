@@ -164,6 +164,8 @@ struct DevPattern {
     std::vector<int> h_ip, h_ix;
     DevBuf ip, ix, diag;
     bool have_diag = false;
+    DevBuf tpos;                       // every entry's transpose position (or -1), for the asymmetry
+    bool have_tpos = false;
     std::unique_ptr<SpmvPlan> plan64, plan32;    // tuned launch shapes per precision
     int forced[2] = {-1, -1};                    // shape set by femd_spmv_select (FP64, FP32); -1: tuned
     std::unique_ptr<LocalOrder> local;           // built by the inner CG on first use

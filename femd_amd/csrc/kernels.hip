@@ -519,20 +519,35 @@ __global__ void k_inv_diag(int n, const int *diag, const double *v, double *inv,
     if (inv) inv[i] = 1.0 / v[d];
 }
 // max |a_ij| and max |a_ij - a_ji| (sparse/csr.hpp:631-644); maxima are order independent.
-__global__ __launch_bounds__(TPB) void k_asymmetry(int n, const int *ip, const int *ix, const double *v, double *out2)
+// The position of every entry's transpose (j, i) in row j, or -1 (a binary search in the sorted
+// row; once per pattern).
+__global__ __launch_bounds__(TPB) void k_transpose_positions(int n, const int *ip, const int *ix, int *tpos)
 {
-    __shared__ double sb[TPB], sd[TPB];
-    double big = 0.0, dif = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         for (int p = ip[i]; p < ip[i + 1]; ++p)
         {
             const int j = ix[p];
-            big = fmax(big, fabs(v[p]));
             int lo = ip[j], hi = ip[j + 1];
             while (lo < hi) { const int m = (lo + hi) >> 1; if (ix[m] < i) lo = m + 1; else hi = m; }
-            const double t = (lo < ip[j + 1] && ix[lo] == i) ? v[lo] : 0.0;
-            dif = fmax(dif, fabs(v[p] - t));
+            tpos[p] = (lo < ip[j + 1] && ix[lo] == i) ? lo : -1;
         }
+}
+
+// max |a_ij| and max |a_ij - a_ji| (a missing transpose counts as 0), one pass over the entries
+// with the pattern's transpose positions
+__global__ __launch_bounds__(TPB) void k_asymmetry(int64_t nnz, const int *__restrict__ tpos, const double *__restrict__ v,
+                                                   double *out2)
+{
+    __shared__ double sb[TPB], sd[TPB];
+    double big = 0.0, dif = 0.0;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    {
+        const double a = v[p];
+        const int q = tpos[p];
+        big = fmax(big, fabs(a));
+        dif = fmax(dif, fabs(a - (q >= 0 ? v[q] : 0.0)));
+    }
     sb[threadIdx.x] = big; sd[threadIdx.x] = dif;
     __syncthreads();
     for (int s = TPB / 2; s > 0; s >>= 1)
@@ -1027,9 +1042,13 @@ void launch_inv_diag(int n, const int *diag, const double *v, double *inv, int *
 {
     if (n > 0) FEMD_LAUNCH(k_inv_diag, grid_for(n), TPB, 0, s, n, diag, v, inv, bad_row);
 }
-void launch_asymmetry(int n, const int *ip, const int *ix, const double *v, double *out2, hipStream_t s)
+void launch_transpose_positions(int n, const int *ip, const int *ix, int *tpos, hipStream_t s)
 {
-    if (n > 0) FEMD_LAUNCH(k_asymmetry, std::min(grid_for(n), 2048), TPB, 0, s, n, ip, ix, v, out2);
+    if (n > 0) FEMD_LAUNCH(k_transpose_positions, std::min(grid_for(n), 2048), TPB, 0, s, n, ip, ix, tpos);
+}
+void launch_asymmetry(int64_t nnz, const int *tpos, const double *v, double *out2, hipStream_t s)
+{
+    if (nnz > 0) FEMD_LAUNCH(k_asymmetry, std::min(grid_for(nnz), 4096), TPB, 0, s, nnz, tpos, v, out2);
 }
 
 void launch_cg_alpha(const double *parts, int np, CGState *st, hipStream_t s) { FEMD_LAUNCH(k_cg_alpha, 1, TPB, 0, s, parts, np, st); }

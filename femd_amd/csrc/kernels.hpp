@@ -63,7 +63,8 @@ void launch_inv_diag(int n, const int *diag, const double *v, double *inv, int *
 void launch_diag_positions(int n, const int *ip, const int *ix, int *diag, hipStream_t s);
 void launch_take(int64_t n, const int *idx, const double *src, double *dst, hipStream_t s);         // dst[i] = src[idx[i]]
 void launch_to_float(int64_t n, const double *a, float *b, hipStream_t s);
-void launch_asymmetry(int n, const int *ip, const int *ix, const double *v, double *out2, hipStream_t s);
+void launch_transpose_positions(int n, const int *ip, const int *ix, int *tpos, hipStream_t s);
+void launch_asymmetry(int64_t nnz, const int *tpos, const double *v, double *out2, hipStream_t s);
 
 // ---- CG (krylov/cg.hpp) -------------------------------------------------------------------------
 struct CGState {

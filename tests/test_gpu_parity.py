@@ -454,3 +454,45 @@ def test_solve_dirichlet_lift(orc, k):
     xo, _ = orc.cg(ip, ix, Ao, bo, M="jacobi", tol=1e-12)
     want = V2.prolongate(xo) + ug
     assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want)
+
+
+def test_asymmetry_matches_definition():
+    """CSR asymmetry (csr.hpp:631-644): max |a_ij - a_ji| / max |a_ij|, a missing transpose counting
+    as 0, exact maxima; the transpose positions are kept with the pattern, so a second call after
+    new values reuses them.  Non-square matrices give 1."""
+    import scipy.sparse as sp
+    print("\n[case] asymmetry of random square patterns, new values, symmetric and non-square", flush=True)
+    rng = np.random.default_rng(21)
+
+    def ref(S):
+        S = S.tocsr()
+        S.sort_indices()
+        big = np.abs(S.data).max() if S.nnz else 0.0
+        T = S.T.tocsr()
+        d = 0.0
+        for i in range(S.shape[0]):
+            for p in range(S.indptr[i], S.indptr[i + 1]):
+                j = S.indices[p]
+                d = max(d, abs(S.data[p] - T[i, j]))     # T[i, j] = a_ji, 0 when not stored
+        return d / big if big > 0 else 0.0
+
+    n = 300
+    S = sp.random(n, n, density=0.03, random_state=1, format="csr") + sp.eye(n)
+    S.sort_indices()
+    A = fd.SparseMatrix.from_arrays(n, n, S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy(),
+                                    symmetric=False)
+    assert A.asymmetry() == ref(S)
+    S2 = S.copy()
+    S2.data = rng.standard_normal(S.nnz)
+    A.set_data(S2.data)
+    assert A.asymmetry() == ref(S2)
+    Y = (S + S.T).tocsr()
+    Y.sort_indices()
+    B = fd.SparseMatrix.from_arrays(n, n, Y.indptr.astype(np.int32), Y.indices.astype(np.int32), Y.data.copy(),
+                                    symmetric=False)
+    assert B.asymmetry() == 0.0
+    R = sp.random(20, 30, density=0.2, random_state=2, format="csr")
+    R.sort_indices()
+    C = fd.SparseMatrix.from_arrays(20, 30, R.indptr.astype(np.int32), R.indices.astype(np.int32), R.data.copy(),
+                                    symmetric=False)
+    assert C.asymmetry() == 1.0

@@ -320,10 +320,22 @@ __device__ __noinline__ double op_tanh(double r) { return tanh(r); }
 __device__ __noinline__ double op_sinh(double r) { return sinh(r); }
 __device__ __noinline__ double op_cosh(double r) { return cosh(r); }
 
+// The program's register file: an array of MAX_REGS per lane (scratch memory), or nreg rows of the
+// workgroup's LDS, register i of lane t at [i * TPB + t] (consecutive lanes, consecutive banks).
+template <bool LDS> struct RegFile;
+template <> struct RegFile<false> {
+    double r[MAX_REGS];
+    __device__ double &operator[](int i) { return r[i]; }
+};
+template <> struct RegFile<true> {
+    double *base;
+    __device__ double &operator[](int i) { return base[i * TPB]; }
+};
+
+template <class RF>
 __device__ inline double run_program(double x, double y, double nx, double ny, int ninstr, const int *__restrict__ code,
-                                     const double *__restrict__ consts)
+                                     const double *__restrict__ consts, RF &reg)
 {
-    double reg[MAX_REGS];
     int res = 0;
     for (int s = 0; s < ninstr; ++s)
     {
@@ -372,9 +384,12 @@ __global__ __launch_bounds__(TPB) void k_coefficient_pts(int64_t np, const doubl
 {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= np) return;
-    out[k] = run_program(xs[k], ys[k], nxs[k], nys[k], ninstr, code, consts);
+    RegFile<false> rf;
+    out[k] = run_program(xs[k], ys[k], nxs[k], nys[k], ninstr, code, consts, rf);
 }
 
+// LDS: the register file in dynamic LDS (nreg * TPB doubles), else in scratch memory
+template <bool LDS>
 __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, const double *__restrict__ geom,
                                                      const double *__restrict__ qxi, const double *__restrict__ qeta,
                                                      int ninstr, const int *__restrict__ code, int nreg,
@@ -399,7 +414,17 @@ __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, con
         y = g[1] + g[4] * qxi[q] + g[5] * qeta[q];
     }
     (void)nreg;
-    out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts);
+    if constexpr (LDS)
+    {
+        extern __shared__ double sh[];
+        RegFile<true> rf{sh + threadIdx.x};
+        out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts, rf);
+    }
+    else
+    {
+        RegFile<false> rf;
+        out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts, rf);
+    }
 }
 
 // ============================================================================ SpMV
@@ -975,7 +1000,14 @@ void launch_coefficient(int nc, int nq, int nv, const double *geom, const double
                         const int *code, int nreg, const double *consts, double *out, hipStream_t s)
 {
     const int64_t m = static_cast<int64_t>(nc) * nq;
-    if (m > 0) FEMD_LAUNCH(k_coefficient, grid_for(m), TPB, 0, s, nc, nq, nv, geom, qxi, qeta, ninstr, code, nreg, consts, out);
+    if (m <= 0) return;
+    // a small register file lives in LDS (8 KiB per register row of the workgroup at most 16 rows),
+    // a larger one in scratch memory; the arithmetic is the same
+    if (nreg >= 1 && nreg <= 16)
+        FEMD_LAUNCH(k_coefficient<true>, grid_for(m), TPB, size_t(nreg) * TPB * sizeof(double), s, nc, nq, nv, geom, qxi,
+                    qeta, ninstr, code, nreg, consts, out);
+    else
+        FEMD_LAUNCH(k_coefficient<false>, grid_for(m), TPB, 0, s, nc, nq, nv, geom, qxi, qeta, ninstr, code, nreg, consts, out);
 }
 
 void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,

@@ -15,7 +15,7 @@ namespace {
 
 constexpr int BT = 256;
 constexpr int BCHUNK = 1024;
-constexpr int RT = 1024;           // threads of the per-system reductions
+constexpr int RT = 256;            // threads of the per-system reductions
 constexpr int GRAPH_ITERS = 16;
 constexpr int64_t FUSE_ALPHA_MAX = int64_t(1) << 20;   // tiles x chunks of the largest system (8 MiB of extra reads)
 

@@ -126,7 +126,9 @@ void admit(femd_server_s *S)
     S->groups.push_back(std::move(G));
 }
 
-// hand out system p of group G: its solution to the request's output, its result to the list
+// hand out system p of group G: its solution to the request's output (copied on the group's
+// stream; the caller synchronizes that stream before the result can be handed out), its result to
+// the list
 void retire(femd_server_s *S, Group &G, int p)
 {
     const CGEngine &E = G.B->engine[S->precision == FEMD_FP32 ? 0 : 1];
@@ -164,7 +166,11 @@ void step(femd_server_s *S)
     for (auto &G : S->groups)
     {
         CGEngine &E = G->B->engine[eng];
+        const size_t ndone = S->done.size();
         for (int p : E.complete(G->stream)) retire(S, *G, p);
+        // a retired request's output must hold its solution before step() reports it, while the
+        // group's other systems run on: wait for the copies retire() queued
+        if (S->done.size() != ndone) HIPC(hipStreamSynchronize(G->stream));
     }
     for (auto it = S->groups.begin(); it != S->groups.end();)
     {

@@ -12,7 +12,9 @@ def solve(a, L, backend="cg", dirichlet=None, **options):
     "ilu0"), tol=, maxiter=).  With Dirichlet data -- dirichlet= (a number, a callable g(x, y), an
     expression in fd.x, fd.y, or a dict {marker: ...}, as V.lift takes) -- the solution is
     u = u_0 + u_g with u_g = V.lift(dirichlet) and u_0 in V solving a(u_0, v) = L(v) - a(u_g, v); it
-    is returned as the raw vector of V.unconstrained.  Without data the adapted vector of V."""
+    is returned as the raw vector of V.unconstrained.  Without data in the call, the data given with the
+    space (LagrangeSpace(m, k, dirichlet={marker: data}) or data=) are used; without any, the adapted
+    vector of V."""
     a = a if isinstance(a, Form) else form(a)
     L = L if isinstance(L, Form) else form(L)
     if a.rank != 2 or L.rank != 1:
@@ -24,6 +26,9 @@ def solve(a, L, backend="cg", dirichlet=None, **options):
         raise TypeError("solve(a, L): the form must be square (trial space = test space)")
     if str(backend).lower() == "auto":
         backend = "cg"
+    if dirichlet is None:
+        # the data given with the space, when the call gives none (linalg.py:697-706)
+        dirichlet = getattr(V, "boundary_data", None)
     A = a.assemble()
     b = np.array(L.assemble(), dtype=np.float64)
     S = A.solver(backend, **options)

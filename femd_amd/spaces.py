@@ -29,17 +29,26 @@ class LagrangeSpace2D:
     (fe/lagrange_space_quad.hpp:112-210: tensor Lagrange on the reference square, bilinear map).
 
     bc:        "free" (default) or "dirichlet" (every exterior side eliminated).
-    dirichlet: a marker or list of markers whose sides are eliminated; combines with bc.
+    dirichlet: a marker or list of markers whose sides are eliminated; combines with bc.  A dict
+               {marker: data} names the sides and gives their data at once (spaces2d.py:115-119).
     nodes:     "equispaced" (default) or "lobatto": Warburton's warp-and-blend points on triangles,
                Gauss-Lobatto tensor points on quadrilaterals (fe/simplex_basis.hpp:150-285,
                lagrange_space_quad.hpp:31-50); the same space with another basis, identical to the
-               equispaced one, to rounding, for k <= 2."""
+               equispaced one, to rounding, for k <= 2.
+    data:      Dirichlet data kept with the space (a number, a callable g(x, y), an expression in
+               fd.x, fd.y, or a dict {marker: ...}, as lift() takes); lift() and fd.solve use it
+               when they are given none (spaces2d.py:161-172)."""
 
     family = "lagrange"
     tdim = 2
     nverts = 3
 
-    def __init__(self, mesh: Mesh2D, degree: int, bc="free", dirichlet=None, nodes="equispaced"):
+    def __init__(self, mesh: Mesh2D, degree: int, bc="free", dirichlet=None, nodes="equispaced", data=None):
+        if isinstance(dirichlet, dict):
+            if data is not None:
+                raise TypeError(f"{type(self).__name__}: give the data either in dirichlet={{marker: data}} or in "
+                                "data=, not both")
+            data, dirichlet = dict(dirichlet), sorted(dirichlet)
         if not isinstance(mesh, Mesh2D):
             raise TypeError(f"LagrangeSpace2D: mesh must be a Mesh2D, got {type(mesh).__name__}")
         if isinstance(degree, bool) or int(degree) != degree:
@@ -76,6 +85,20 @@ class LagrangeSpace2D:
         self.ncells, self.nloc, self.raw_dim, self.dim, self.nedges, self.npoints = map(int, s)
         self._num = None
         self._pattern = None
+        self._data = None
+        if data is not None:
+            if self.n_constraints == 0:
+                raise ValueError(f"{type(self).__name__}: data= gives Dirichlet data, and this space has no Dirichlet "
+                                 "side; name them with dirichlet=[markers] or bc='dirichlet'")
+            # evaluated once now, so that a marker without a Dirichlet side or a callable that does
+            # not evaluate fails here rather than at the first solve
+            self._lift(data)
+            self._data = data
+
+    @property
+    def boundary_data(self):
+        """The Dirichlet data given with the space (dirichlet={marker: data} or data=), or None."""
+        return self._data
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
@@ -223,7 +246,14 @@ class LagrangeSpace2D:
         """The raw vector (a function of V.unconstrained) carrying Dirichlet data on the eliminated
         nodes: a number or a callable g(x, y) / expression in fd.x, fd.y for every Dirichlet side,
         or a dict {marker: ...}; sides left out get zero data, and where two sides meet the corner
-        takes the data of the larger marker (spaces2d.py:373-414)."""
+        takes the data of the larger marker (spaces2d.py:373-414).  Without data, the data given
+        with the space (boundary_data), if any."""
+        if g is None and self._data is not None:
+            g = self._data
+        return self._lift(g)
+
+    def _lift(self, g=None):
+        """lift() with exactly the data given: None is zero; the space's own data are not used."""
         raw = np.zeros(self.raw_dim)
         cons = self.constrained()
         if g is None or cons.size == 0:
@@ -294,8 +324,8 @@ def _values(g, xy):
     return float(g)
 
 
-def LagrangeSpace(mesh, degree, bc="free", dirichlet=None, nodes="equispaced"):
+def LagrangeSpace(mesh, degree, bc="free", dirichlet=None, nodes="equispaced", data=None):
     """fd.LagrangeSpace on a Mesh2D (P_k) or a QuadMesh (Q_k) (the 2D branches of spaces.py:435-442)."""
     if not isinstance(mesh, Mesh2D):
         raise TypeError("LagrangeSpace: this package builds the 2D (Mesh2D) spaces only")
-    return LagrangeSpace2D(mesh, degree, bc=bc, dirichlet=dirichlet, nodes=nodes)
+    return LagrangeSpace2D(mesh, degree, bc=bc, dirichlet=dirichlet, nodes=nodes, data=data)

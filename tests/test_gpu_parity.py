@@ -434,6 +434,12 @@ def test_solve_dirichlet_lift(orc, k):
         uw = fd.solve(aw, Lw, backend="cg", M="jacobi", tol=1e-13, dirichlet={4: ue})
         xw = W.raw_coordinates()
         assert np.abs(uw - ue(xw[:, 0], xw[:, 1])).max() < 1e-9
+        # the same data kept with the space: fd.solve takes them when the call gives none
+        Wd = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 8, 8), 1, dirichlet={4: ue})
+        u, v = fd.TrialFunction(Wd), fd.TestFunction(Wd)
+        ud = fd.solve(fd.dot(fd.grad(u), fd.grad(v)) * fd.dx, (fd.Const(2.0) * n[0] - n[1]) * v * fd.ds((1, 2, 3)),
+                      backend="cg", M="jacobi", tol=1e-13)
+        assert np.array_equal(ud, uw)
     # the same lift through the oracle: unconstrained matrix, b_f - (A_U u_g)_f, Jacobi-CG
     f, _ = problems.fourier_source(2)
     V2 = V

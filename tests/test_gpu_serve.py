@@ -34,13 +34,28 @@ def test_serve_mix_is_alone_and_matches_oracle(orc):
     for t in range(5):
         S.submit(t, reqs[t][1], reqs[t][2])
     mix = {}
-    for _ in range(3):                    # the first wave runs before the second arrives
-        for r in S.step():
+    early = 0
+
+    def take(rs):
+        # each result as step() hands it out, while the rest of its group keeps running: its
+        # output must already hold the solution (true residual at the tolerance), not the guess
+        nonlocal early
+        running = S.status()["running"]
+        for r in rs:
+            V, A, b = reqs[r.tag]
+            res = np.linalg.norm(b - A.tocsr() @ r.x) / np.linalg.norm(b)
+            assert res <= 1.01e-10, (r.tag, res)
+            early += running > 0
             mix[r.tag] = r
+
+    for _ in range(3):                    # the first wave runs before the second arrives
+        take(S.step())
     assert S.status()["running"] > 0
     for t in range(5, len(reqs)):
         S.submit(t, reqs[t][1], reqs[t][2])
-    mix.update(S.run())
+    while S.busy():
+        take(S.step())
+    assert early > 0
     assert sorted(mix) == list(range(len(reqs)))
     st = S.status()
     assert st["running"] == 0 and st["queued"] == 0 and st["bytes_in_use"] == 0

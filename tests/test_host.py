@@ -348,3 +348,26 @@ def test_inner_order_pattern_first_met():
         _lib.check(lib.femd_inner_order_pattern(n, bad.indptr.astype(np.int32), bad.indices.astype(np.int32), len(sizes),
                                                 segments.ctypes.data, perm, ip2, np.empty(bad.nnz, np.int32),
                                                 np.empty(bad.nnz, np.int32)))
+
+
+def test_dirichlet_data_kept_with_the_space():
+    """LagrangeSpace(m, k, dirichlet={marker: data}) names the Dirichlet sides and keeps their data
+    (spaces2d.py:115-119, 161-172): boundary_data holds it, lift() without data uses it, and the
+    eliminated nodes of the marked side carry it; data= does the same; giving both, or data for a
+    space with no Dirichlet side, is an error."""
+    m = fd.rectangle_mesh(0, 0, 1, 1, 4, 4)
+    V = fd.LagrangeSpace(m, 2, dirichlet={1: 5.0})
+    assert V.dirichlet_markers == [1] and V.boundary_data == {1: 5.0}
+    ug = V.lift()
+    assert np.array_equal(ug, V.lift({1: 5.0}))
+    cons = V.constrained()
+    assert cons.size == 9 and np.all(ug[cons] == 5.0) and not np.delete(ug, cons).any()
+    W = fd.LagrangeSpace(m, 2, dirichlet=[1, 3], data=lambda x, y: x + 2 * y)
+    xy = W.raw_coordinates()
+    assert np.array_equal(W.lift()[W.constrained()], (xy[:, 0] + 2 * xy[:, 1])[W.constrained()])
+    with pytest.raises(TypeError):
+        fd.LagrangeSpace(m, 2, dirichlet={1: 5.0}, data=1.0)
+    with pytest.raises(ValueError):
+        fd.LagrangeSpace(m, 2, data=1.0)
+    with pytest.raises(ValueError):
+        fd.LagrangeSpace(m, 2, dirichlet=[1], data={3: 1.0})

@@ -104,6 +104,7 @@ _SIGS = {
     "femd_precond_free": (None, [_p]),
     "femd_cg": (_i, [_p, _p, _dp, _dp, _i, _d, C.POINTER(SolveInfo)]),
     "femd_cg_dev": (_i, [_p, _p, _p, _p, _i, _d, C.POINTER(SolveInfo), _p]),
+    "femd_cg_dev_noalloc": (_i, [_p, _p, _p, _p, _i, _d, C.POINTER(SolveInfo), _p]),
     "femd_krylov": (_i, [_p, _p, _dp, _dp, _i, _i, _i, _d, _i, C.POINTER(SolveInfo)]),
     "femd_solve_refined": (_i, [_p, _dp, _dp, _d, _d, _i, C.POINTER(SolveInfo)]),
     "femd_solve_refined_dev": (_i, [_p, _p, _p, _d, _d, _i, C.POINTER(SolveInfo), _p]),

@@ -13,6 +13,35 @@ thread_local std::string g_err;
 Device g_dev;
 std::mutex g_init_mu;
 
+// ---------------------------------------------------------------------- no-allocation work space
+// The device work space of femd_cg_dev_noalloc: the CG's four vectors of at most NOALLOC_MAX_ROWS
+// entries, its two arrays of chunk partials, eight scalars and the loop state (32 MiB and a few KiB
+// per GPU, held from the library's first use on that GPU until the process ends).  In host code
+// `scratch` only names the array: its device address on each GPU comes from hipGetSymbolAddress.
+constexpr int64_t NOALLOC_MAX_ROWS = int64_t(1) << 20;
+constexpr int64_t NOALLOC_MAX_CHUNKS = (NOALLOC_MAX_ROWS + CHUNK - 1) / CHUNK;
+constexpr int64_t NOALLOC_STATE = (sizeof(CGState) + 7) / 8 + 8;
+constexpr int64_t NOALLOC_DOUBLES = 4 * NOALLOC_MAX_ROWS + 2 * NOALLOC_MAX_CHUNKS + 8 + NOALLOC_STATE;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};              // scratch's device address on each GPU
+std::string g_scratch_err[MAX_DEVICES];           // why it is missing, when hipGetSymbolAddress failed
+
+// Called once per device, with that device current (from the library's set-up).
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES || g_scratch[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+    {
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch[device] = static_cast<double *>(ptr);
+}
+
 void ensure_device()
 {
     if (g_dev.ready) return;
@@ -22,6 +51,7 @@ void ensure_device()
     if (hipGetDeviceCount(&n) != hipSuccess || n < 1) fail(FEMD_EDEVICE, "no HIP device is visible");
     HIPC(hipSetDevice(g_dev.id));
     HIPC(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    noalloc_setup(g_dev.id);
     g_dev.ready = true;
 }
 
@@ -321,42 +351,52 @@ void precond_apply(femd_precond_s *M, const double *r, double *z, const int *sto
 // krylov/cg.hpp:245-350.  The set-up and the final residual are driven from the host (a few
 // read-backs per solve); the iterations run from device-side scalars (CGState), a batch of
 // iterations enqueued between read-backs, every kernel a no-op once `stop` is set.
-void cg_device(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int max_iter, double tol,
-               femd_solve_info *info, hipStream_t s)
+// The device work space of one CG solve: four vectors of n (z only with a general preconditioner),
+// two arrays of chunk partials, eight scalars and the loop state.
+struct CGWork {
+    double *r, *p, *Ap, *z, *prr, *prz, *scal;
+    CGState *st;
+};
+
+void cg_check(femd_csr_s *A, femd_precond_s *M)
 {
     const int n = A->pat->n;
     if (A->pat->m != n) fail(FEMD_EINVAL, "cg: the matrix must be square");
     if (M && M->n != n) fail(FEMD_EINVAL, "krylov: preconditioner and right-hand side differ in size");
+}
+
+// The solve on a given work space; hst receives the loop state at every read-back (pinned or not).
+void cg_run(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int max_iter, double tol,
+            femd_solve_info *info, const CGWork &w, CGState *hst, hipStream_t s)
+{
+    const int n = A->pat->n;
     std::memset(info, 0, sizeof(*info));
     const bool jac = M && M->kind == FEMD_PC_JACOBI, general = M && !jac;
     const int np = nchunks(n);
-    DevBuf r(8 * size_t(n)), p(8 * size_t(n)), Ap(8 * size_t(n)), z(general ? 8 * size_t(n) : 0);
-    DevBuf prr(8 * size_t(std::max(np, 1))), prz(8 * size_t(std::max(np, 1))), scal(64), stb(sizeof(CGState));
-    Pinned hst(sizeof(CGState));
-    double *rd = r.as<double>(), *pd = p.as<double>(), *Apd = Ap.as<double>(), *sc = scal.as<double>();
-    CGState *st = stb.as<CGState>();
+    double *rd = w.r, *pd = w.p, *Apd = w.Ap, *zd = w.z, *prr = w.prr, *prz = w.prz, *sc = w.scal;
+    CGState *st = w.st;
     const double *inv = jac ? M->inv.as<double>() : nullptr;
 
-    ddot_dev(n, b, b, prr.as<double>(), sc, s);
+    ddot_dev(n, b, b, prr, sc, s);
     const double nb = std::sqrt(read_scalar(sc, s));
     if (nb == 0.0) { launch_fill(n, 0.0, x, s); HIPC(hipStreamSynchronize(s)); info->converged = 1; return; }
     spmv64(A, x, Apd, nullptr, s);
     launch_sub(n, b, Apd, rd, s);
-    ddot_dev(n, rd, rd, prr.as<double>(), sc, s);
+    ddot_dev(n, rd, rd, prr, sc, s);
     double rn = std::sqrt(read_scalar(sc, s));
     if (rn <= tol * nb) { info->converged = 1; info->residual = rn / nb; info->precond_residual = info->residual; return; }
     double rz;
     if (general)
     {
-        precond_apply(M, rd, z.as<double>(), nullptr, s);
-        launch_copy(n, z.as<double>(), pd, s);
-        ddot_dev(n, rd, z.as<double>(), prr.as<double>(), sc, s);
+        precond_apply(M, rd, zd, nullptr, s);
+        launch_copy(n, zd, pd, s);
+        ddot_dev(n, rd, zd, prr, sc, s);
         rz = read_scalar(sc, s);
     }
     else if (jac)
     {
         launch_mul(n, inv, rd, pd, s);
-        ddot_dev(n, rd, pd, prr.as<double>(), sc, s);
+        ddot_dev(n, rd, pd, prr, sc, s);
         rz = read_scalar(sc, s);
     }
     else { launch_copy(n, rd, pd, s); rz = rn * rn; }
@@ -373,28 +413,28 @@ void cg_device(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int
         for (int k = 0; k < batch; ++k)
         {
             spmv64(A, pd, Apd, stop, s);
-            launch_dot_parts(n, pd, Apd, prr.as<double>(), stop, s);
-            launch_cg_alpha(prr.as<double>(), np, st, s);
-            launch_cg_update(n, x, rd, pd, Apd, inv, prr.as<double>(), prz.as<double>(), st, s);
-            launch_cg_beta(prr.as<double>(), prz.as<double>(), np, general ? 2 : (jac ? 1 : 0), st, s);
+            launch_dot_parts(n, pd, Apd, prr, stop, s);
+            launch_cg_alpha(prr, np, st, s);
+            launch_cg_update(n, x, rd, pd, Apd, inv, prr, prz, st, s);
+            launch_cg_beta(prr, prz, np, general ? 2 : (jac ? 1 : 0), st, s);
             if (general)
             {
-                precond_apply(M, rd, z.as<double>(), stop, s);
-                launch_dot_parts(n, rd, z.as<double>(), prr.as<double>(), stop, s);
-                launch_cg_rz(prr.as<double>(), np, st, s);
-                launch_cg_pupdate(n, pd, z.as<double>(), nullptr, st, s);
+                precond_apply(M, rd, zd, stop, s);
+                launch_dot_parts(n, rd, zd, prr, stop, s);
+                launch_cg_rz(prr, np, st, s);
+                launch_cg_pupdate(n, pd, zd, nullptr, st, s);
             }
             else launch_cg_pupdate(n, pd, rd, inv, st, s);
         }
-        HIPC(hipMemcpyAsync(hst.p, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hst, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
-        std::memcpy(&h, hst.p, sizeof(h));
+        std::memcpy(&h, hst, sizeof(h));
         batch = std::min(64, batch * 2);
     }
     // the true residual and the verdict (cg.hpp:222-237)
     spmv64(A, x, Apd, nullptr, s);
     launch_sub(n, b, Apd, rd, s);
-    ddot_dev(n, rd, rd, prr.as<double>(), sc, s);
+    ddot_dev(n, rd, rd, prr, sc, s);
     const double res = std::sqrt(read_scalar(sc, s)) / nb;
     info->iterations = h.it;
     info->breakdown = h.breakdown;
@@ -402,6 +442,44 @@ void cg_device(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int
     info->precond_residual = res;
     info->converged = !h.breakdown && res <= tol * (1.0 + 1e-6) + 1e-15;
     if (!info->converged && !h.breakdown && res <= 10.0 * tol && h.it < max_iter) info->converged = 1;
+}
+
+// The solve on buffers of the library's pool.
+void cg_device(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int max_iter, double tol,
+               femd_solve_info *info, hipStream_t s)
+{
+    cg_check(A, M);
+    const int n = A->pat->n, np = nchunks(n);
+    const bool general = M && M->kind != FEMD_PC_JACOBI;
+    DevBuf r(8 * size_t(n)), p(8 * size_t(n)), Ap(8 * size_t(n)), z(general ? 8 * size_t(n) : 0);
+    DevBuf prr(8 * size_t(std::max(np, 1))), prz(8 * size_t(std::max(np, 1))), scal(64), stb(sizeof(CGState));
+    Pinned hst(sizeof(CGState));
+    const CGWork w{r.as<double>(), p.as<double>(), Ap.as<double>(), z.as<double>(), prr.as<double>(), prz.as<double>(),
+                   scal.as<double>(), stb.as<CGState>()};
+    cg_run(A, M, b, x, max_iter, tol, info, w, hst.as<CGState>(), s);
+}
+
+// The solve on the device's `scratch` array: no allocation of any kind (the loop state is read
+// back into host stack memory).
+void cg_noalloc(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int max_iter, double tol,
+                femd_solve_info *info, hipStream_t s)
+{
+    cg_check(A, M);
+    const int64_t n = A->pat->n;
+    if (n > NOALLOC_MAX_ROWS)
+        fail(FEMD_EINVAL, "cg_dev_noalloc: " + std::to_string(n) + " rows, more than its work space holds (" +
+                              std::to_string(NOALLOC_MAX_ROWS) + ")");
+    if (n > 0 && !A->pat->plan64)
+        fail(FEMD_EINVAL, "cg_dev_noalloc: the matrix has no FP64 SpMV launch shape yet; call "
+                          "femd_spmv_tuning(A, FEMD_FP64, ...) (or any FP64 product or solve on A) once first");
+    double *base = g_scratch[g_dev.id];
+    if (!base) fail(FEMD_EDEVICE, "cg_dev_noalloc: no work space on this device: " + g_scratch_err[g_dev.id]);
+    double *parts = base + 4 * NOALLOC_MAX_ROWS;
+    const CGWork w{base, base + NOALLOC_MAX_ROWS, base + 2 * NOALLOC_MAX_ROWS, base + 3 * NOALLOC_MAX_ROWS, parts,
+                   parts + NOALLOC_MAX_CHUNKS, parts + 2 * NOALLOC_MAX_CHUNKS,
+                   reinterpret_cast<CGState *>(parts + 2 * NOALLOC_MAX_CHUNKS + 8)};
+    CGState hst{};
+    cg_run(A, M, b, x, max_iter, tol, info, w, &hst, s);
 }
 
 // --------------------------------------------------------------------------- GMRES / LGMRES
@@ -732,6 +810,7 @@ int femd_init(int device)
         g_dev.id = device;
         HIPC(hipSetDevice(device));
         HIPC(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+        noalloc_setup(device);
         g_dev.ready = true;
     });
 }
@@ -1500,6 +1579,15 @@ int femd_cg_dev(femd_csr A, femd_precond M, const double *b, double *x, int max_
                 femd_stream st)
 {
     return api([&] { cg_device(check_csr(A), M, b, x, max_iter, tol, info, S(st)); });
+}
+
+int femd_cg_dev_noalloc(femd_csr A, femd_precond M, const double *b, double *x, int max_iter, double tol,
+                        femd_solve_info *info, femd_stream st)
+{
+    return api([&] {
+        if (!info) fail(FEMD_EINVAL, "cg_dev_noalloc: null info");
+        cg_noalloc(check_csr(A), M, b, x, max_iter, tol, info, S(st));
+    });
 }
 
 int femd_cg(femd_csr A, femd_precond M, const double *b, double *x, int max_iter, double tol, femd_solve_info *info)

@@ -272,6 +272,19 @@ int femd_cg(femd_csr A, femd_precond M, const double *b, double *x, int max_iter
 int femd_cg_dev(femd_csr A, femd_precond M, const double *b_dev, double *x_dev, int max_iter, double tol,
                 femd_solve_info *info, femd_stream s);
 
+/* femd_cg_dev with no allocation of any kind (no counterpart in the reference: for callers that
+ * cannot allocate device or pinned memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and the same iterates, bit for bit, on a
+ * work space held in a static device array of 32 MiB per GPU (kept until the process ends), so:
+ *   - the matrix has at most 1 048 576 rows (FEMD_EINVAL beyond);
+ *   - A's FP64 SpMV launch shape must exist: call femd_spmv_tuning(A, FEMD_FP64, ...), or any
+ *     FP64 product or solve on A, once beforehand (FEMD_EINVAL otherwise);
+ *   - M is NULL or any preconditioner (its buffers are made by femd_precond_create);
+ *   - every call on one GPU uses that GPU's one work space: calls that overlap on a GPU (other
+ *     threads, other streams) race and must be serialized by the caller. */
+int femd_cg_dev_noalloc(femd_csr A, femd_precond M, const double *b_dev, double *x_dev, int max_iter, double tol,
+                        femd_solve_info *info, femd_stream s);
+
 /* GMRES(restart) / LGMRES(restart, k_aug), left preconditioned (fd.gmres / fd.lgmres,
  * krylov.py:148-175 -> krylov_solve, femd_bindings.cpp:784-798 -> krylov::solve,
  * krylov/adapters.hpp:116-147 -> csnewton gmres / lgmres, vendor/csnewton/csnewton.hpp:174-474). */

@@ -100,7 +100,9 @@ _SIGS = {
     "femd_precond_apply": (_i, [_p, _dp, _dp]),
     "femd_precond_apply_dev": (_i, [_p, _p, _p, _p]),
     "femd_precond_info": (_i, [_p, _ip]),
+    "femd_precond_stats": (_i, [_p, _p]),
     "femd_rcm": (_i, [_i, _ip, _ip, _ip]),
+    "femd_sweep_host_apply": (_i, [_i, _ip, _ip, _dp, _i, _d, _dp, _dp, _ip]),
     "femd_precond_free": (None, [_p]),
     "femd_cg": (_i, [_p, _p, _dp, _dp, _i, _d, C.POINTER(SolveInfo)]),
     "femd_cg_dev": (_i, [_p, _p, _p, _p, _i, _d, C.POINTER(SolveInfo), _p]),

@@ -1,8 +1,10 @@
 // capi.hip -- the C ABI of include/femd_hip.h: handles, the device memory pool, and the
 // orchestration of the kernels in kernels.hip.
 #include "cg_engine.hpp"
+#include "sweep.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 
 using namespace femd_amd;
@@ -330,6 +332,17 @@ void precond_apply(femd_precond_s *M, const double *r, double *z, const int *sto
     const int n = M->n;
     if (M->kind == FEMD_PC_JACOBI) { launch_mul(n, M->inv.as<double>(), r, z, s); return; }
     const bool perm = M->ordering == FEMD_ORDER_RCM;
+    if (M->sweep)
+    {
+        // the planned sweeps: r (permuted) into the forward stream, the forward sweep writes y into
+        // the backward stream, the backward sweep writes z (scattered back through the order)
+        const int ilu = M->kind == FEMD_PC_ILU0;
+        launch_sweep_fill(n, M->inposF.as<int>(), perm ? M->perm.as<int>() : nullptr, r, M->streamF.as<double>(), stop, s);
+        launch_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, M->nstepsF, M->streamF.p, M->wbF.as<int>(), M->omega,
+                     M->streamB.as<double>(), stop, s);
+        launch_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, M->nstepsB, M->streamB.p, M->wbB.as<int>(), M->omega, z, stop, s);
+        return;
+    }
     const double *rin = r;
     double *zout = z;
     if (perm)
@@ -715,6 +728,68 @@ void refined_device(femd_csr_s *A, const double *b, double *x, double tol, doubl
     info->precond_residual = res;
 }
 
+// Plan the one-workgroup sweeps of an ILU(0) / SSOR preconditioner on its (permuted) pattern and
+// pack its values (M->lu: the factor, or A's values for SSOR) into the streams.  When the schedule
+// does not fit the LDS the level-set kernels stay in use (M->sweep false, the reason kept).
+void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vector<int> &pix, const std::vector<int> &perm,
+                 hipStream_t s)
+{
+    const int n = M->n;
+    if (n == 0 || std::getenv("FEMD_LEVEL_SWEEPS")) return;
+    std::vector<int> dg(n, -1);
+    for (int i = 0; i < n; ++i)
+        for (int p = pip[i]; p < pip[i + 1]; ++p)
+            if (pix[p] == i) dg[i] = p;
+    const bool ilu = M->kind == FEMD_PC_ILU0;
+    std::vector<int> dest(n);
+    for (int i = 0; i < n; ++i) dest[i] = perm.empty() ? i : perm[i];
+    SweepPlan B, F;
+    std::string why;
+    int rows = SW_ROWS;
+    bool ok = false;
+    for (; rows >= 64 && !ok; rows /= 2)
+    {
+        ok = plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, pip.data(), pix.data(), dg.data(), dest, rows, B, why);
+        if (!ok) continue;
+        std::vector<int> fdest(n);
+        for (int i = 0; i < n; ++i)
+        {
+            if (B.in_pos[i] > INT32_MAX) { ok = false; why = "stream too large"; break; }
+            fdest[i] = int(B.in_pos[i]);
+        }
+        ok = ok && plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, pip.data(), pix.data(), dg.data(), fdest, rows, F, why);
+    }
+    if (ok && (F.stream.size() >= (size_t(1) << 31) || B.stream.size() >= (size_t(1) << 31)))
+        ok = false, why = "a stream of 2 GiB or more";
+    if (!ok) { M->sweep_note = why; return; }
+    const int64_t nnz = pip[n];
+    std::vector<int> fmap(nnz), bmap(nnz), inpos(n);
+    for (int64_t p = 0; p < nnz; ++p)
+    {
+        fmap[p] = int(F.vmap[p]);
+        bmap[p] = int(B.vmap[p]);
+    }
+    for (int i = 0; i < n; ++i) inpos[i] = int(F.in_pos[i]);
+    M->streamF = upload(F.stream, s);
+    M->streamB = upload(B.stream, s);
+    M->wbF = upload(F.wb, s);
+    M->wbB = upload(B.wb, s);
+    M->inposF = upload(inpos, s);
+    {
+        DevBuf df = upload(fmap, s), db = upload(bmap, s);
+        launch_sweep_pack(nnz, df.as<int>(), db.as<int>(), M->lu.as<double>(), M->streamF.as<double>(), M->streamB.as<double>(), s);
+        HIPC(hipStreamSynchronize(s));              // the maps go out of scope
+    }
+    M->nstepsF = F.nsteps;
+    M->nstepsB = B.nsteps;
+    M->nlevF = F.nlevels;
+    M->nlevB = B.nlevels;
+    M->bytesF = int64_t(F.stream.size());
+    M->bytesB = int64_t(B.stream.size());
+    M->entriesLU = F.entries + B.entries + n;
+    M->sweep = true;
+}
+
 // Build the preconditioner's device data from A.
 void build_precond(femd_precond_s *M, femd_csr_s *A, hipStream_t s)
 {
@@ -787,6 +862,7 @@ void build_precond(femd_precond_s *M, femd_csr_s *A, hipStream_t s)
         for (int i = 0; i < n; ++i) mp = std::min(mp, lu[dg[i]]);
         M->min_pivot = mp;
     }
+    plan_sweeps(M, pip, pix, perm, s);
     HIPC(hipStreamSynchronize(s));
 }
 
@@ -1522,6 +1598,21 @@ int femd_precond_apply(femd_precond M, const double *r, double *z)
     });
 }
 
+int femd_precond_stats(femd_precond M, int64_t out[8])
+{
+    return api([&] {
+        if (!M || !out) fail(FEMD_EINVAL, "precond_stats: null argument");
+        out[0] = M->sweep;
+        out[1] = M->nstepsF;
+        out[2] = M->nstepsB;
+        out[3] = M->nlevF;
+        out[4] = M->nlevB;
+        out[5] = M->bytesF;
+        out[6] = M->bytesB;
+        out[7] = M->entriesLU;
+    });
+}
+
 int femd_precond_info(femd_precond M, int out[3])
 {
     return api([&] {
@@ -1557,6 +1648,49 @@ int femd_inner_order_pattern(int n, const int *indptr, const int *indices, int n
         std::copy(ip2.begin(), ip2.end(), indptr2);
         std::copy(ix2.begin(), ix2.end(), indices2);
         std::copy(em.begin(), em.end(), emap);
+    });
+}
+
+int femd_sweep_host_apply(int n, const int *indptr, const int *indices, const double *values, int kind, double omega,
+                          const double *r, double *z, int stats[6])
+{
+    return api([&] {
+        if (n < 0 || !indptr || (n > 0 && (!indices || !values || !r || !z)))
+            fail(FEMD_EINVAL, "sweep_host_apply: null argument");
+        if (kind != FEMD_PC_ILU0 && kind != FEMD_PC_SSOR) fail(FEMD_EINVAL, "sweep_host_apply: kind is ILU0 or SSOR");
+        if (n == 0) return;
+        std::vector<int> dg(n, -1);
+        for (int i = 0; i < n; ++i)
+            for (int p = indptr[i]; p < indptr[i + 1]; ++p)
+                if (indices[p] == i) dg[i] = p;
+        for (int i = 0; i < n; ++i)
+            if (dg[i] < 0) fail(FEMD_EZEROPIVOT, "sweep_host_apply: no diagonal entry in row " + std::to_string(i));
+        const bool ilu = kind == FEMD_PC_ILU0;
+        std::vector<int> dest(n);
+        for (int i = 0; i < n; ++i) dest[i] = i;
+        SweepPlan B, F;
+        std::string why;
+        if (!plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, indptr, indices, dg.data(), dest, SW_ROWS, B, why))
+            fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        std::vector<int> fdest(n);
+        for (int i = 0; i < n; ++i) fdest[i] = int(B.in_pos[i]);
+        if (!plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, indptr, indices, dg.data(), fdest, SW_ROWS, F, why))
+            fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        std::vector<uint8_t> fs = F.stream, bs = B.stream;
+        double *f64 = reinterpret_cast<double *>(fs.data()), *b64 = reinterpret_cast<double *>(bs.data());
+        for (int64_t p = 0; p < indptr[n]; ++p)
+        {
+            if (F.vmap[p] >= 0) f64[F.vmap[p]] = values[p];
+            if (B.vmap[p] >= 0) b64[B.vmap[p]] = values[p];
+        }
+        for (int i = 0; i < n; ++i) f64[F.in_pos[i]] = r[i];
+        if (!emulate_sweep(F, fs, omega, b64, int64_t(bs.size() / 8), why)) fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        if (!emulate_sweep(B, bs, omega, z, n, why)) fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        if (stats)
+        {
+            stats[0] = F.nsteps; stats[1] = B.nsteps; stats[2] = F.nlevels; stats[3] = B.nlevels;
+            stats[4] = F.nslots; stats[5] = B.nslots;
+        }
     });
 }
 

@@ -250,6 +250,12 @@ struct femd_precond_s {
     femd_amd::DevBuf lptrL, lrowsL, lptrU, lrowsU;
     int nlevL = 0, nlevU = 0;
     femd_amd::DevBuf rp, zp;                            // permuted work vectors
+    // the planned one-workgroup sweeps (sweep.hpp); when planning fails, the level-set kernels above
+    bool sweep = false;
+    int nstepsF = 0, nstepsB = 0, nlevF = 0, nlevB = 0;
+    int64_t bytesF = 0, bytesB = 0, entriesLU = 0;
+    femd_amd::DevBuf streamF, streamB, wbF, wbB, inposF;
+    std::string sweep_note;                             // why the sweep was not planned, when it was not
 };
 
 namespace femd_amd {

@@ -217,6 +217,18 @@ class Preconditioner:
         """Smallest ILU(0) pivot (ILU(0) only)."""
         return self._min_pivot
 
+    def stats(self) -> dict:
+        """How the triangular sweeps run (no reference counterpart): planned (one workgroup's program
+        per sweep) or on the level-set kernels, steps and dependency levels per sweep, stream bytes
+        and the stored entries the two sweeps read."""
+        out = np.zeros(8, dtype=np.int64)
+        L.check(L.load().femd_precond_stats(self._h, out.ctypes.data))
+        keys = ("planned", "steps_forward", "steps_backward", "levels_forward", "levels_backward", "bytes_forward",
+                "bytes_backward", "entries")
+        d = {k: int(v) for k, v in zip(keys, out)}
+        d["planned"] = bool(d["planned"])
+        return d
+
     def solve(self, r):
         v = L.f64(getattr(r, "vector", r)).reshape(-1)
         z = np.empty_like(v)

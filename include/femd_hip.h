@@ -251,6 +251,20 @@ int femd_precond_apply(femd_precond M, const double *r, double *z);
 int femd_precond_apply_dev(femd_precond M, const double *r_dev, double *z_dev, femd_stream s);
 /* kind, ordering, n */
 int femd_precond_info(femd_precond M, int out[3]);
+/* How an ILU(0) / SSOR preconditioner runs its triangular sweeps (no counterpart in the reference):
+ * out[0] 1 when each sweep is one workgroup's planned program (else the level-set kernels), out[1..2]
+ * the forward / backward steps, out[3..4] their dependency levels, out[5..6] the bytes of their
+ * streams, out[7] the stored entries the two sweeps read (L, U and the diagonal). */
+int femd_precond_stats(femd_precond M, int64_t out[8]);
+/* The triangular sweeps of an ILU(0) (values = its factor L\U) or SSOR(omega) (values = the matrix)
+ * preconditioner, z = M^{-1} r on a pattern in the order it is applied, run on the HOST exactly as the
+ * GPU runs them: the one-workgroup schedule femd_precond_create plans (dependency levels cut into
+ * steps, LDS slots for the z values, the streamed ring) emulated step by step.  stats (may be NULL)
+ * receives the forward and backward steps, dependency levels and LDS slots.  FEMD_EINVAL when the
+ * schedule does not fit the LDS.  No counterpart in the reference: a check of the planner without a
+ * GPU. */
+int femd_sweep_host_apply(int n, const int *indptr, const int *indices, const double *values, int kind, double omega,
+                          const double *r, double *z, int stats[6]);
 /* The RCM order of a pattern: perm[k] = original index of row k (ordering::rcm_order). */
 int femd_rcm(int n, const int *indptr, const int *indices, int *perm);
 void femd_precond_free(femd_precond M);

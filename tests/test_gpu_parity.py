@@ -195,6 +195,36 @@ def test_precond_apply_bit_exact(orc, kind, ordering):
     assert np.array_equal(z, zo)
 
 
+@pytest.mark.parametrize("kind,ordering", [("ilu0", "rcm"), ("ilu0", "natural"), ("ssor", "rcm"), ("ssor", "natural")])
+def test_planned_sweeps_bit_exact_unstructured_p3(orc, kind, ordering):
+    """The one-workgroup planned sweeps on a P3 kappa matrix of an unstructured mesh (~26k rows,
+    about a thousand dependency levels, several steps per level): the application, and a whole CG
+    solve (iterates and iteration count), equal the oracle's bit for bit."""
+    _name(f"planned sweeps P3 unstructured {kind} {ordering}")
+    Pts, T, _ = problems.delaunay_square(0.02, seed=4)
+    Po = orc.from_arrays(Pts, T, k=3)
+    kap, _ = problems.kappa_field(1)
+    x, y, _ = Po.cache_points(8)
+    ck = orc.integrand_eval(*fd.forms.Program(kap).arrays(), x, y)
+    ip, ix = Po.pattern()
+    Ao = Po.assemble_matrix(8, [1, 2], [1, 2], np.stack([ck, ck]))
+    n = ip.size - 1
+    Ag = fd.SparseMatrix.from_arrays(n, n, ip, ix, Ao, symmetric=True)
+    om = 1.2 if kind == "ssor" else 1.0
+    pc = Ag.preconditioner(kind, omega=om, ordering=ordering)
+    st = pc.stats()
+    assert st["planned"] and st["levels_forward"] > 100 and st["steps_forward"] >= st["levels_forward"], st
+    r = np.random.default_rng(5).standard_normal(n)
+    zo, _ = orc.precond_apply(ip, ix, Ao, r, kind, ordering, om)
+    assert np.array_equal(pc.solve(r), zo)
+    assert np.array_equal(pc.solve(2.0 * r), orc.precond_apply(ip, ix, Ao, 2.0 * r, kind, ordering, om)[0])
+    b = r[::-1].copy()
+    xg, info = fd.cg(Ag, b, M=pc, tol=1e-9)
+    xo, io = orc.cg(ip, ix, Ao, b, M=kind, ordering=ordering, omega=om, tol=1e-9)
+    assert info.iterations == io["iterations"] and info.residual == io["residual"]
+    assert np.array_equal(xg, xo)
+
+
 def test_ilu_factor_bit_exact(orc):
     _name("ilu factor values")
     P, ip, ix, A, b, _ = _oracle_c0(orc, 16, 2)

@@ -371,3 +371,62 @@ def test_dirichlet_data_kept_with_the_space():
         fd.LagrangeSpace(m, 2, data=1.0)
     with pytest.raises(ValueError):
         fd.LagrangeSpace(m, 2, dirichlet=[1], data={3: 1.0})
+
+
+def _kappa_p3(orc, N):
+    """A P3 matrix with a variable coefficient on an N x N grid (oracle assembly) and a right-hand side."""
+    P = orc.rectangle(N, N, 3, "crossed")
+    ip, ix = P.pattern()
+    deg = 8
+    xq, yq, _ = P.cache_points(deg)
+    kap = np.exp(0.5 * np.sin(6 * xq) * np.cos(5 * yq))
+    A = P.assemble_matrix(deg, [1, 2], [1, 2], np.vstack([kap, kap]))
+    b = np.random.default_rng(N).standard_normal(ip.size - 1)
+    return ip, ix, A, b
+
+
+@pytest.mark.parametrize("kind", ["ilu0", "ssor"])
+@pytest.mark.parametrize("ordering", ["natural", "rcm"])
+def test_sweep_plan_matches_oracle(orc, kind, ordering):
+    """The one-workgroup sweep program that femd_precond_create plans -- dependency levels cut into
+    steps, LDS slots for the z values, the streamed ring and its producer schedule -- run on the host
+    step by step (femd_sweep_host_apply) gives the oracle's z = M^{-1} r bit for bit, for ILU(0) and
+    SSOR(1.3) in both orderings, on a P3 pattern whose sweeps need several steps per level."""
+    import ctypes as C
+    import scipy.sparse as sp
+    ip, ix, A, r = _kappa_p3(orc, 14)
+    n = ip.size - 1
+    omega = 1.3 if kind == "ssor" else 1.0
+    want, _ = orc.precond_apply(ip, ix, A, r, kind, ordering, omega)
+    if ordering == "rcm":
+        perm = orc.rcm(ip, ix)
+    else:
+        perm = np.arange(n, dtype=np.int32)
+    if kind == "ilu0":
+        _, ip2, ix2, vals, _ = orc.ilu0_factor(ip, ix, A, ordering)
+    else:
+        B = sp.csr_matrix((A, ix, ip), shape=(n, n))[perm][:, perm].tocsr()
+        B.sort_indices()
+        ip2, ix2, vals = B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.copy()
+    zp = np.empty(n)
+    stats = np.zeros(6, dtype=np.int32)
+    lib = _lib.load()
+    _lib.check(lib.femd_sweep_host_apply(n, _lib.i32(ip2), _lib.i32(ix2), np.ascontiguousarray(vals),
+                                         _lib.PC_ILU0 if kind == "ilu0" else _lib.PC_SSOR, omega,
+                                         np.ascontiguousarray(r[perm]), zp, stats))
+    got = np.empty(n)
+    got[perm] = zp
+    assert np.array_equal(got, want)
+    assert stats[0] >= stats[2] > 1 and stats[1] >= stats[3] > 1
+
+
+def test_sweep_plan_refuses_bad_input():
+    lib = _lib.load()
+    ip = np.array([0, 1, 2], dtype=np.int32)
+    ix = np.array([1, 0], dtype=np.int32)                    # no diagonal
+    z = np.empty(2)
+    st = np.zeros(6, dtype=np.int32)
+    rc = lib.femd_sweep_host_apply(2, ip, ix, np.ones(2), _lib.PC_ILU0, 1.0, np.ones(2), z, st)
+    assert rc == _lib.EZEROPIVOT
+    rc = lib.femd_sweep_host_apply(2, ip, ix, np.ones(2), _lib.PC_JACOBI, 1.0, np.ones(2), z, st)
+    assert rc == _lib.EINVAL

@@ -4,12 +4,14 @@
     A = form((dot(grad(u), grad(v)) + u*v) * dx).assemble()      # SparseMatrix, on the GPU
     b = form(f * v * dx).assemble()                             # numpy vector
 
-A form is canonicalized into terms  c(x) (D^a v)(D^b u)  exactly as the reference does
-(forms.py:1364-1415: the same expansion and term order), its quadrature degree inferred by the
-same rules (forms.py:2443-2500), and each coefficient compiled into the same register program
-(_Program, forms.py:1822-1938), which the GPU evaluates at the quadrature points.  Known fields
-(Function arguments inside coefficients), ds / dS measures and 1D meshes are outside this path
-and raise NotImplementedError.
+A form becomes a list of terms  c(x) (D^a v)(D^b u)  over dx or ds(markers), in the order the
+reference's canonicalization gives them (python/femd/forms.py:1364-1415), with the quadrature
+degree its rules infer (forms.py:2443-2500); each coefficient becomes a register program of the
+Integrand instruction set (forms/integrand.hpp:33-46) in the reference's emission order
+(forms.py:1822-1909), which the GPU evaluates at the quadrature points.  Because the expression
+trees have the reference's shapes, the assembled values are the reference's bit for bit.  Known
+fields inside coefficients, dS measures and 1D meshes are outside this path and raise
+NotImplementedError.
 """
 from __future__ import annotations
 
@@ -20,41 +22,149 @@ import numpy as np
 from . import _lib as L
 
 
-# ============================================================================ expressions
+# ============================================================================ expression nodes
+# An expression is a tree of nodes.  Its shape is part of the contract: the terms a form expands into,
+# their order, and the instruction order of each coefficient's program follow from it, and the
+# assembled values follow from those bit for bit.  So a + b is always a two-term sum, a - b the sum
+# of a and (-1) * b, -a the product (-1) * a, a / b the product of a and b ** -1, and sums and
+# products are never flattened (python/femd/forms.py builds the same trees).  The algorithms over
+# trees (expansion, derivatives, degrees, program emission, NumPy evaluation) are tables keyed by
+# node class, below.
+
 class Expr:
-    def __add__(self, o):  return Sum([self, _lift(o)])
-    def __radd__(self, o): return Sum([_lift(o), self])
-    def __sub__(self, o):  return Sum([self, Prod([Const(-1.0), _lift(o)])])
-    def __rsub__(self, o): return Sum([_lift(o), Prod([Const(-1.0), self])])
-    def __neg__(self):     return Prod([Const(-1.0), self])
-    def __pos__(self):     return self
+    """Base of the scalar expression nodes; operators build nodes (see _OPERATORS)."""
+    __array_priority__ = 1000
 
-    def __mul__(self, o):
-        if isinstance(o, Measure):
-            return Integral(self, o)
-        if isinstance(o, ListTensor):
-            return o.__rmul__(self)
-        return Prod([self, _lift(o)])
+    def __pos__(self):
+        return self
 
-    def __rmul__(self, o): return Prod([_lift(o), self])
-
-    def __truediv__(self, o):
-        o = _lift(o)
-        if _has_arguments(o):
-            raise TypeError("cannot divide by an expression containing a test or trial function")
-        return Prod([self, Pow(o, -1)])
-
-    def __rtruediv__(self, o): return Prod([_lift(o), Pow(self, -1)])
+    def __neg__(self):
+        return _negated(self)
 
     def __pow__(self, n):
         if not isinstance(n, (int, float)):
             raise TypeError("exponent must be a number")
         return Pow(self, n)
 
-    __array_priority__ = 1000
+
+class Const(Expr):
+    """A number fixed when the form is made."""
+    def __init__(self, value):
+        self.value = float(value)
+
+    def __repr__(self):
+        return repr(self.value)
 
 
-def _lift(o):
+class Constant(Expr):
+    """A named number read each time the form is assembled (python/femd/forms.py:108-137)."""
+    def __init__(self, value=0.0, name="c"):
+        self.value, self.name = float(value), str(name)
+
+    def assign(self, value):
+        self.value = float(value)
+
+    def __float__(self):
+        return self.value
+
+    def __repr__(self):
+        return f"Constant({self.name}={self.value!r})"
+
+
+class Spatial(Expr):
+    """A coordinate: index 0 is x, 1 is y."""
+    def __init__(self, index):
+        self.index = int(index)
+
+    def __repr__(self):
+        return ("x", "y")[self.index]
+
+
+x = Spatial(0)
+y = Spatial(1)
+
+
+class Normal(Expr):
+    """A component of the outward unit normal on exterior facets (ds terms only;
+    python/femd/forms.py:101-104)."""
+    def __init__(self, index: int):
+        self.index = int(index)
+
+    def __repr__(self):
+        return ("n_x", "n_y")[self.index]
+
+
+class Sum(Expr):
+    """terms[0] + terms[1] + ..., added left to right."""
+    def __init__(self, terms):
+        self.terms = list(terms)
+
+    def __repr__(self):
+        return "Sum(" + ", ".join(repr(t) for t in self.terms) + ")"
+
+
+class Prod(Expr):
+    """factors[0] * factors[1] * ..., multiplied left to right."""
+    def __init__(self, factors):
+        self.factors = list(factors)
+
+    def __repr__(self):
+        return "Prod(" + ", ".join(repr(f) for f in self.factors) + ")"
+
+
+class Pow(Expr):
+    """base ** n for a number n."""
+    def __init__(self, base, n):
+        self.base, self.n = base, n
+
+    def __repr__(self):
+        return f"Pow({self.base!r}, {self.n!r})"
+
+
+# The pointwise functions of the instruction set (forms/integrand.hpp:33-46) with their opcodes.
+_OPCODE = {"sin": 9, "cos": 10, "exp": 11, "log": 12, "tanh": 13, "sqrt": 14, "sinh": 15, "cosh": 16,
+           "sech": 17, "abs": 18, "sign": 19}
+
+
+class Func(Expr):
+    """name(arg) for a function of the instruction set."""
+    def __init__(self, name, arg):
+        if name not in _OPCODE:
+            raise ValueError(f"unknown function {name}")
+        self.name, self.arg = name, arg
+
+    def __repr__(self):
+        return f"{self.name}({self.arg!r})"
+
+
+class Argument(Expr):
+    """The test or trial function of a 2D space, with a partial-derivative multi-index."""
+    def __init__(self, kind, space, deriv=(0, 0)):
+        self.kind, self.space, self.deriv = kind, space, tuple(deriv)
+
+    def __repr__(self):
+        name = "v" if self.kind == "test" else "u"
+        dx_, dy_ = self.deriv
+        return name if dx_ + dy_ == 0 else f"d{name}/{'dx' * dx_}{'dy' * dy_}"
+
+    def code(self):
+        """The kernels' derivative code of a first-order argument: 0 value, 1 d/dx, 2 d/dy."""
+        dx_, dy_ = self.deriv
+        if dx_ + dy_ == 0:
+            return 0
+        return 1 if dx_ else 2
+
+
+def TestFunction(V):
+    return Argument("test", V)
+
+
+def TrialFunction(V):
+    return Argument("trial", V)
+
+
+def _as_expr(o):
+    """A node for an operand: nodes as they are, numbers as Const."""
     if isinstance(o, Expr):
         return o
     if isinstance(o, ListTensor):
@@ -64,173 +174,195 @@ def _lift(o):
     raise TypeError(f"cannot use {o!r} in a form")
 
 
-class Const(Expr):
-    def __init__(self, value): self.value = float(value)
-    def __repr__(self): return f"{self.value:g}"
+def _negated(e):
+    return Prod([Const(-1.0), e])
 
 
-class Constant(Expr):
-    """A named number read when the form is assembled (forms.py:108-137)."""
-    def __init__(self, value=0.0, name="c"):
-        self.value, self.name = float(value), str(name)
-
-    def assign(self, value):
-        self.value = float(value)
-
-    def __float__(self): return self.value
-    def __repr__(self): return self.name
+def _mul(a, b):
+    if isinstance(b, Measure):
+        return Integral(a, b)
+    if isinstance(b, ListTensor):
+        return ListTensor([a * c for c in b])
+    return Prod([a, _as_expr(b)])
 
 
-class Spatial(Expr):
-    def __init__(self, index): self.index = int(index)
-    def __repr__(self): return "xy"[self.index]
+def _div(a, b):
+    b = _as_expr(b)
+    if _mentions(b, Argument):
+        raise TypeError("cannot divide by an expression containing a test or trial function")
+    return Prod([a, Pow(b, -1)])
 
 
-x = Spatial(0)
-y = Spatial(1)
+# operator -> node builder (self, other); "r" variants have the operand order swapped
+_OPERATORS = {
+    "__add__": lambda a, b: Sum([a, _as_expr(b)]),
+    "__radd__": lambda a, b: Sum([_as_expr(b), a]),
+    "__sub__": lambda a, b: Sum([a, _negated(_as_expr(b))]),
+    "__rsub__": lambda a, b: Sum([_as_expr(b), _negated(a)]),
+    "__mul__": _mul,
+    "__rmul__": lambda a, b: Prod([_as_expr(b), a]),
+    "__truediv__": _div,
+    "__rtruediv__": lambda a, b: Prod([_as_expr(b), Pow(a, -1)]),
+}
+for _name, _build in _OPERATORS.items():
+    setattr(Expr, _name, _build)
+del _name, _build
 
 
-class Normal(Expr):
-    """A component of the outward unit normal on the exterior facets, for ds terms
-    (python/femd/forms.py:101-104, 634-639)."""
-    def __init__(self, index: int): self.index = int(index)
-    def __repr__(self): return f"n[{self.index}]"
+def _function(name):
+    def make(e):
+        return Func(name, _as_expr(e))
+    make.__name__ = name
+    make.__doc__ = f"{name}(e) in a coefficient (evaluated at the quadrature points on the GPU)."
+    return make
 
 
-def FacetNormal(mesh=None):
-    """The outward unit normal (n[0], n[1]) on the exterior facets, for ds terms."""
-    return ListTensor([Normal(0), Normal(1)])
-
-
-class Sum(Expr):
-    def __init__(self, terms): self.terms = list(terms)
-    def __repr__(self): return "(" + " + ".join(map(repr, self.terms)) + ")"
-
-
-class Prod(Expr):
-    def __init__(self, factors): self.factors = list(factors)
-    def __repr__(self): return "*".join(map(repr, self.factors))
-
-
-class Pow(Expr):
-    def __init__(self, base, n): self.base, self.n = base, n
-    def __repr__(self): return f"({self.base!r})**{self.n}"
-
-
-class Func(Expr):
-    _NAMES = ("sin", "cos", "exp", "log", "tanh", "sqrt", "sinh", "cosh", "sech", "abs", "sign")
-
-    def __init__(self, name, arg):
-        if name not in self._NAMES:
-            raise ValueError(f"unknown function {name}")
-        self.name, self.arg = name, arg
-
-    def __repr__(self): return f"{self.name}({self.arg!r})"
-
-
-def sin(e):  return Func("sin", _lift(e))
-def cos(e):  return Func("cos", _lift(e))
-def exp(e):  return Func("exp", _lift(e))
-def log(e):  return Func("log", _lift(e))
-def tanh(e): return Func("tanh", _lift(e))
-def sqrt(e): return Func("sqrt", _lift(e))
-def sinh(e): return Func("sinh", _lift(e))
-def cosh(e): return Func("cosh", _lift(e))
-def sech(e): return Func("sech", _lift(e))
-def sign(e): return Func("sign", _lift(e))
-def abs_(e): return Func("abs", _lift(e))
-
-
-class Argument(Expr):
-    """Test or trial function of a 2D space, with a partial-derivative multi-index."""
-    def __init__(self, kind, space, deriv=(0, 0)):
-        self.kind, self.space, self.deriv = kind, space, tuple(deriv)
-
-    def __repr__(self):
-        base = {"test": "v", "trial": "u"}[self.kind]
-        if any(self.deriv):
-            base = "D" + "x" * self.deriv[0] + "y" * self.deriv[1] + f"({base})"
-        return base
-
-    def code(self):
-        """The kernel's derivative code (forms.py:181-184): 0 value, 1 d/dx, 2 d/dy."""
-        d = self.deriv
-        return 0 if d[0] + d[1] == 0 else (1 if d[0] else 2)
-
-
-def TestFunction(V):  return Argument("test", V)
-def TrialFunction(V): return Argument("trial", V)
+sin, cos, exp, log, tanh, sqrt, sinh, cosh, sech, sign = (_function(f) for f in
+                                                           ("sin", "cos", "exp", "log", "tanh", "sqrt", "sinh",
+                                                            "cosh", "sech", "sign"))
+abs_ = _function("abs")
 
 
 class ListTensor:
-    """A vector of scalar expressions (grad(u))."""
-    def __init__(self, components): self.components = [_lift(c) for c in components]
-    def __len__(self): return len(self.components)
-    def __getitem__(self, i): return self.components[i]
-    def __iter__(self): return iter(self.components)
-    def __repr__(self): return "[" + ", ".join(map(repr, self.components)) + "]"
-    def __add__(self, o): return ListTensor([a + b for a, b in zip(self, o)])
-    def __sub__(self, o): return ListTensor([a - b for a, b in zip(self, o)])
-    def __neg__(self): return ListTensor([-a for a in self])
-    def __mul__(self, o): return ListTensor([a * o for a in self])
-    def __rmul__(self, o): return ListTensor([o * a for a in self])
+    """A vector of scalar expressions (grad(u), the facet normal)."""
+    def __init__(self, components):
+        self.components = [_as_expr(c) for c in components]
+
+    def __len__(self):
+        return len(self.components)
+
+    def __getitem__(self, i):
+        return self.components[i]
+
+    def __iter__(self):
+        return iter(self.components)
+
+    def __repr__(self):
+        return "Vector(" + ", ".join(repr(c) for c in self.components) + ")"
+
+    def _zip(self, o, fn):
+        return ListTensor([fn(a, b) for a, b in zip(self.components, o)])
+
+    def __add__(self, o): return self._zip(o, lambda a, b: a + b)
+    def __sub__(self, o): return self._zip(o, lambda a, b: a - b)
+    def __neg__(self): return ListTensor([-a for a in self.components])
+    def __mul__(self, o): return ListTensor([a * o for a in self.components])
+    def __rmul__(self, o): return ListTensor([o * a for a in self.components])
 
 
-def as_vector(c): return c if isinstance(c, ListTensor) else ListTensor(c)
+def as_vector(c):
+    return c if isinstance(c, ListTensor) else ListTensor(c)
 
 
-def _dfunc(f):
-    """d f(g) / dg for Dx of a function (forms.py:368-377)."""
-    a = f.arg
-    table = {
-        "sin": lambda: cos(a), "cos": lambda: Prod([Const(-1.0), sin(a)]), "exp": lambda: exp(a),
-        "log": lambda: Pow(a, -1), "sqrt": lambda: Prod([Const(0.5), Pow(sqrt(a), -1)]),
-        "sinh": lambda: cosh(a), "cosh": lambda: sinh(a),
-        "tanh": lambda: Sum([Const(1.0), Prod([Const(-1.0), Pow(tanh(a), 2)])]),
-    }
-    if f.name not in table:
-        raise TypeError(f"cannot differentiate {f.name}")
-    return table[f.name]()
+def FacetNormal(mesh=None):
+    """The outward unit normal (n_x, n_y) on the exterior facets, for ds terms."""
+    return ListTensor([Normal(0), Normal(1)])
+
+
+# ============================================================================ tree walks
+def _children(e):
+    if isinstance(e, Sum):
+        return e.terms
+    if isinstance(e, Prod):
+        return e.factors
+    if isinstance(e, Pow):
+        return (e.base,)
+    if isinstance(e, Func):
+        return (e.arg,)
+    return ()
+
+
+def _mentions(e, kind):
+    """Whether the tree holds a node of class `kind` (iteratively: deep coefficient sums)."""
+    todo = [e]
+    while todo:
+        node = todo.pop()
+        if isinstance(node, kind):
+            return True
+        todo.extend(_children(node))
+    return False
+
+
+def _has_arguments(e):
+    return _mentions(e, Argument)
+
+
+def _has_normal(e):
+    return _mentions(e, Normal)
+
+
+# ---- derivatives (python/femd/forms.py:368-416 give the same rules and the same tree shapes)
+# d f(g) / dg for the differentiable functions
+_OUTER_DERIVATIVE = {
+    "sin": lambda g: cos(g),
+    "cos": lambda g: _negated(sin(g)),
+    "exp": lambda g: exp(g),
+    "log": lambda g: Pow(g, -1),
+    "sqrt": lambda g: Prod([Const(0.5), Pow(sqrt(g), -1)]),
+    "sinh": lambda g: cosh(g),
+    "cosh": lambda g: sinh(g),
+    "tanh": lambda g: Sum([Const(1.0), _negated(Pow(tanh(g), 2))]),
+}
+
+
+def _d_product(e, i):
+    # constant factors stay in front; one product per non-constant factor, that factor differentiated
+    fixed = [f for f in e.factors if isinstance(f, Const)]
+    varying = [f for f in e.factors if not isinstance(f, Const)]
+    if not varying:
+        return Const(0.0)
+    if len(varying) == 1:
+        return Prod(fixed + [Dx(varying[0], i)])
+    parts = []
+    for j, f in enumerate(varying):
+        parts.append(Prod(fixed + varying[:j] + [Dx(f, i)] + varying[j + 1:]))
+    return Sum(parts)
+
+
+def _d_function(e, i):
+    outer = _OUTER_DERIVATIVE.get(e.name)
+    if outer is None:
+        raise TypeError(f"cannot differentiate {e.name}")
+    return Prod([outer(e.arg), Dx(e.arg, i)])
+
+
+def _d_argument(e, i):
+    d = list(e.deriv)
+    d[i] += 1
+    return Argument(e.kind, e.space, tuple(d))
+
+
+_DERIVATIVE = {
+    Argument: _d_argument,
+    Sum: lambda e, i: Sum([Dx(t, i) for t in e.terms]),
+    Prod: _d_product,
+    Const: lambda e, i: Const(0.0),
+    Constant: lambda e, i: Const(0.0),
+    Spatial: lambda e, i: Const(1.0 if e.index == i else 0.0),
+    Pow: lambda e, i: Prod([Const(float(e.n)), Pow(e.base, e.n - 1), Dx(e.base, i)]),
+    Func: _d_function,
+}
 
 
 def Dx(e, i: int):
-    """d e / d x_i, linear with the product and chain rules (forms.py:380-416)."""
+    """d e / d x_i by the sum, product and chain rules."""
     if i not in (0, 1):
         raise ValueError(f"Dx(e, i): i is 0 (x) or 1 (y), got {i!r}")
     if isinstance(e, ListTensor):
         return ListTensor([Dx(c, i) for c in e.components])
     if isinstance(e, (int, float, np.floating, np.integer)):
         return Const(0.0)
-    if isinstance(e, Argument):
-        d = list(e.deriv)
-        d[i] += 1
-        return Argument(e.kind, e.space, tuple(d))
-    if isinstance(e, Sum):
-        return Sum([Dx(t, i) for t in e.terms])
-    if isinstance(e, Prod):
-        consts = [f for f in e.factors if isinstance(f, Const)]
-        rest = [f for f in e.factors if not isinstance(f, Const)]
-        if not rest:
-            return Const(0.0)
-        if len(rest) == 1:
-            return Prod(consts + [Dx(rest[0], i)])
-        return Sum([Prod(consts + rest[:j] + [Dx(f, i)] + rest[j + 1:]) for j, f in enumerate(rest)])
-    if isinstance(e, (Const, Constant)):
-        return Const(0.0)
-    if isinstance(e, Spatial):
-        return Const(1.0 if e.index == i else 0.0)
-    if isinstance(e, Pow):
-        return Prod([Const(float(e.n)), Pow(e.base, e.n - 1), Dx(e.base, i)])
-    if isinstance(e, Func):
-        return Prod([_dfunc(e), Dx(e.arg, i)])
-    raise TypeError(f"cannot differentiate {e!r}")
+    rule = _DERIVATIVE.get(type(e))
+    if rule is None:
+        raise TypeError(f"cannot differentiate {e!r}")
+    return rule(e, i)
 
 
 def grad(e):
-    """(Dx(e, 0), Dx(e, 1)) of a scalar (forms.py:521-532)."""
+    """(Dx(e, 0), Dx(e, 1)) of a scalar."""
     if isinstance(e, ListTensor):
         raise NotImplementedError("grad() of a vector is outside this path")
-    e = _lift(e)
+    e = _as_expr(e)
     return ListTensor([Dx(e, 0), Dx(e, 1)])
 
 
@@ -240,14 +372,14 @@ def div(v):
 
 
 def dot(a, b):
-    """a . b: a b for scalars, sum_i a_i b_i for vectors (forms.py:565-583)."""
+    """a . b: the product of two scalars, or sum_i a_i b_i of two vectors of one length."""
     a = as_vector(a) if isinstance(a, (list, tuple)) else a
     b = as_vector(b) if isinstance(b, (list, tuple)) else b
-    ra, rb = isinstance(a, ListTensor), isinstance(b, ListTensor)
-    if not ra and not rb:
-        return _lift(a) * _lift(b)
-    if ra != rb:
+    va, vb = isinstance(a, ListTensor), isinstance(b, ListTensor)
+    if va != vb:
         raise TypeError("dot(a, b): a scalar and a vector; use a * b for scaling")
+    if not va:
+        return _as_expr(a) * _as_expr(b)
     if len(a) != len(b):
         raise TypeError(f"dot(a, b): lengths {len(a)} and {len(b)} differ")
     return Sum([p * q for p, q in zip(a.components, b.components)])
@@ -258,8 +390,9 @@ inner = dot
 
 # ============================================================================ measures
 class Measure:
-    """dx (cells), ds (exterior facets: every one, or those with the given side markers) and dS
-    (interior facets, outside this path); forms.py:1269-1309."""
+    """dx (cells), ds (exterior facets: all of them, or those with given side markers) and dS
+    (interior facets, outside this path).  where: None for dx, "both" for every exterior facet,
+    else a sorted tuple of markers; quad_degree overrides the inferred degree."""
     def __init__(self, kind, where=None, quad_degree=None):
         self.kind, self.where, self.quad_degree = kind, where, quad_degree
 
@@ -269,26 +402,29 @@ class Measure:
         if kw:
             raise NotImplementedError(f"{self.kind} options other than the degree are outside this path")
         if self.kind == "dx":
-            if args:
-                quad_degree = args[0]
-            if quad_degree is not None and int(quad_degree) < 1:
-                raise ValueError(f"dx(n): n must be at least 1, got {quad_degree}")
-            return Measure("dx", None, quad_degree)
-        where = args[0] if args else None
-        if len(args) > 1:
-            where = tuple(args)
-        if isinstance(where, (int, np.integer)) and not isinstance(where, bool):
-            return Measure("ds", (int(where),), quad_degree)
-        if isinstance(where, (tuple, list)) and where and all(isinstance(m, (int, np.integer)) for m in where):
-            return Measure("ds", tuple(sorted({int(m) for m in where})), quad_degree)
-        if where in ("left", "right"):
-            raise TypeError(f"ds('{where}') is for 1D meshes; on a 2D mesh use ds(marker), ds((m1, m2)) or ds")
-        if where not in (None, "both"):
-            raise ValueError("ds takes a marker or markers on a 2D mesh")
-        return Measure("ds", "both", quad_degree)
+            degree = args[0] if args else quad_degree
+            if degree is not None and int(degree) < 1:
+                raise ValueError(f"dx(n): n must be at least 1, got {degree}")
+            return Measure("dx", None, degree)
+        return Measure("ds", _facet_selection(args), quad_degree)
 
     def __rmul__(self, e):
-        return Integral(_lift(e), self)
+        return Integral(_as_expr(e), self)
+
+
+def _facet_selection(args):
+    """ds(m), ds((m1, m2)), ds(m1, m2) -> the sorted markers; ds() or ds("both") -> every facet."""
+    sel = args[0] if len(args) == 1 else (tuple(args) if args else None)
+    if sel is None or sel == "both":
+        return "both"
+    if isinstance(sel, str):
+        if sel in ("left", "right"):
+            raise TypeError(f"ds('{sel}') is for 1D meshes; on a 2D mesh use ds(marker), ds((m1, m2)) or ds")
+        raise ValueError("ds takes a marker or markers on a 2D mesh")
+    markers = (sel,) if not isinstance(sel, (tuple, list)) else tuple(sel)
+    if not markers or not all(isinstance(m, (int, np.integer)) and not isinstance(m, bool) for m in markers):
+        raise ValueError("ds takes a marker or markers on a 2D mesh")
+    return tuple(sorted({int(m) for m in markers}))
 
 
 dx = Measure("dx")
@@ -297,6 +433,7 @@ dS = Measure("dS")
 
 
 class Integral:
+    """expr * measure."""
     def __init__(self, expr, measure):
         if measure.kind == "dS":
             raise NotImplementedError("interior-facet (dS) integrals are outside this path")
@@ -305,135 +442,165 @@ class Integral:
     def __add__(self, o): return FormExpr([self]) + o
     def __radd__(self, o): return FormExpr([self]) + o
     def __sub__(self, o): return FormExpr([self]) - o
-    def __neg__(self): return Integral(Prod([Const(-1.0), self.expr]), self.measure)
-    def __mul__(self, c): return Integral(Prod([_lift(c), self.expr]), self.measure)
+    def __neg__(self): return Integral(_negated(self.expr), self.measure)
+    def __mul__(self, c): return Integral(Prod([_as_expr(c), self.expr]), self.measure)
     __rmul__ = __mul__
 
 
 class FormExpr:
-    def __init__(self, integrals): self.integrals = list(integrals)
+    """A sum of integrals, kept in the order written."""
+    def __init__(self, integrals):
+        self.integrals = list(integrals)
 
     def __add__(self, o):
-        if isinstance(o, Integral): return FormExpr(self.integrals + [o])
-        if isinstance(o, FormExpr): return FormExpr(self.integrals + o.integrals)
-        if o == 0: return self
+        if isinstance(o, Integral):
+            return FormExpr(self.integrals + [o])
+        if isinstance(o, FormExpr):
+            return FormExpr(self.integrals + o.integrals)
+        if o == 0:
+            return self
         raise TypeError("can only add integrals to a form")
+
     __radd__ = __add__
 
     def __sub__(self, o):
         o = o if isinstance(o, FormExpr) else FormExpr([o])
         return FormExpr(self.integrals + [-i for i in o.integrals])
 
-    def __neg__(self): return FormExpr([-i for i in self.integrals])
+    def __neg__(self):
+        return FormExpr([-i for i in self.integrals])
 
 
 # ============================================================================ analysis
-def _has_arguments(e):
-    if isinstance(e, Argument): return True
-    if isinstance(e, Sum): return any(_has_arguments(t) for t in e.terms)
-    if isinstance(e, Prod): return any(_has_arguments(f) for f in e.factors)
-    if isinstance(e, Pow): return _has_arguments(e.base)
-    if isinstance(e, Func): return _has_arguments(e.arg)
-    return False
-
-
 def _expand(e):
-    """Sum of products over the test / trial structure (forms.py:1364-1381)."""
-    if not _has_arguments(e) or isinstance(e, Argument):
+    """The integrand as a list of products (lists of factors), distributing over the sums that hold
+    test or trial functions: sums in term order, products as the ordered cross product of their
+    factors' expansions (the reference's order, python/femd/forms.py:1364-1381)."""
+    if isinstance(e, Argument) or not _has_arguments(e):
         return [[e]]
     if isinstance(e, Sum):
-        return [t for term in e.terms for t in _expand(term)]
-    if isinstance(e, Prod):
-        out = [[]]
-        for f in e.factors:
-            out = [a + b for a in out for b in _expand(f)]
+        out = []
+        for t in e.terms:
+            out.extend(_expand(t))
         return out
+    if isinstance(e, Prod):
+        rows = [[]]
+        for f in e.factors:
+            rows = [head + tail for head in rows for tail in _expand(f)]
+        return rows
+    if isinstance(e, Pow) and e.n == 1:
+        return _expand(e.base)
     if isinstance(e, Pow):
-        if e.n == 1:
-            return _expand(e.base)
         raise TypeError(f"{e!r}: a form must be linear in each test and trial function")
     raise TypeError(f"cannot expand {e!r}")
 
 
 class Term:
+    """coeff * (test) * (trial) * measure; test and trial are Arguments or None."""
     def __init__(self, test, trial, coeff, measure):
         self.test, self.trial, self.coeff, self.measure = test, trial, coeff, measure
 
     @property
-    def rank(self): return (self.test is not None) + (self.trial is not None)
+    def rank(self):
+        return int(self.test is not None) + int(self.trial is not None)
 
     def __repr__(self):
         m = self.measure
-        where = "" if m.kind == "dx" or m.where == "both" else f"({', '.join(map(str, m.where))})"
-        return "*".join([repr(self.coeff)] + [repr(a) for a in (self.test, self.trial) if a is not None]) + f"*{m.kind}{where}"
+        where = "" if m.kind == "dx" or m.where == "both" else str(m.where)
+        parts = [repr(self.coeff)] + [repr(a) for a in (self.test, self.trial) if a is not None]
+        return " * ".join(parts) + f" {m.kind}{where}"
 
 
-def _trivially_zero(e):
-    if isinstance(e, Const): return e.value == 0.0
-    if isinstance(e, Prod): return any(_trivially_zero(f) for f in e.factors)
-    if isinstance(e, Sum): return all(_trivially_zero(t) for t in e.terms)
+def _is_zero(e):
+    """Zero by construction: a zero constant, a product with one, a sum of them."""
+    if isinstance(e, Const):
+        return e.value == 0.0
+    if isinstance(e, Prod):
+        return any(_is_zero(f) for f in e.factors)
+    if isinstance(e, Sum):
+        return all(_is_zero(t) for t in e.terms)
     return False
 
 
 def canonicalize(fe):
-    """Terms coeff * test * trial in the reference's order (forms.py:1398-1415)."""
+    """The form's terms coeff * test * trial, in expansion order (python/femd/forms.py:1398-1415):
+    a product's factors other than the arguments form its coefficient (one factor as itself,
+    several as their product, none as 1); terms with no arguments whose coefficient is zero by
+    construction are dropped unless nothing else is left."""
     terms = []
     for integral in fe.integrals:
         for factors in _expand(integral.expr):
-            tests = [f for f in factors if isinstance(f, Argument) and f.kind == "test"]
-            trials = [f for f in factors if isinstance(f, Argument) and f.kind == "trial"]
-            rest = [f for f in factors if not isinstance(f, Argument)]
-            if len(tests) > 1 or len(trials) > 1:
+            args = {"test": [], "trial": []}
+            rest = []
+            for f in factors:
+                (args[f.kind] if isinstance(f, Argument) else rest).append(f)
+            if len(args["test"]) > 1 or len(args["trial"]) > 1:
                 raise TypeError(f"term {factors!r} is not linear in the test/trial functions")
-            if trials and not tests:
+            if args["trial"] and not args["test"]:
                 raise TypeError("a term with a trial function but no test function has no row index")
-            coeff = Prod(rest) if len(rest) != 1 else rest[0]
-            if not rest:
-                coeff = Const(1.0)
-            terms.append(Term(tests[0] if tests else None, trials[0] if trials else None, coeff, integral.measure))
-    nonzero = [t for t in terms if not (t.test is None and t.trial is None and _trivially_zero(t.coeff))]
-    return nonzero if nonzero else terms
+            coeff = Const(1.0) if not rest else (rest[0] if len(rest) == 1 else Prod(rest))
+            test = args["test"][0] if args["test"] else None
+            trial = args["trial"][0] if args["trial"] else None
+            terms.append(Term(test, trial, coeff, integral.measure))
+    kept = [t for t in terms if t.rank or not _is_zero(t.coeff)]
+    return kept or terms
 
 
-def poly_degree(e):
-    """Polynomial degree of a pointwise expression, None if not polynomial (forms.py:1429-1448)."""
-    if isinstance(e, (Const, Constant)): return 0
-    if isinstance(e, Spatial): return 1
-    if isinstance(e, Normal): return 0
-    if isinstance(e, Argument): return _arg_degree(e)
-    if isinstance(e, Sum):
-        ds_ = [poly_degree(t) for t in e.terms]
-        return None if any(d is None for d in ds_) else max(ds_, default=0)
-    if isinstance(e, Prod):
-        ds_ = [poly_degree(f) for f in e.factors]
-        return None if any(d is None for d in ds_) else sum(ds_)
-    if isinstance(e, Pow):
-        d = poly_degree(e.base)
-        if d is None or not float(e.n).is_integer() or e.n < 0: return None
-        return int(e.n) * d
-    return None
-
-
+# ---- degrees (python/femd/forms.py:1429-1501, 654-662)
 def _arg_degree(a):
-    """Total degree on triangles; on quadrilaterals the per-variable degree, which a derivative
-    does not lower (forms.py:654-662)."""
+    """Total degree of an argument on triangles (a derivative lowers it); on quadrilaterals the
+    per-variable degree, which a derivative does not lower."""
     if getattr(a.space, "nverts", 3) == 4:
         return a.space.degree
     return max(a.space.degree - sum(a.deriv), 0)
 
 
+def _pow_degree(n, d):
+    if d is None or not float(n).is_integer() or n < 0:
+        return None
+    return int(n) * d
+
+
+_POLY_DEGREE = {
+    Const: lambda e: 0,
+    Constant: lambda e: 0,
+    Normal: lambda e: 0,
+    Spatial: lambda e: 1,
+    Argument: _arg_degree,
+    Sum: lambda e: _combine([poly_degree(t) for t in e.terms], max),
+    Prod: lambda e: _combine([poly_degree(f) for f in e.factors], sum),
+    Pow: lambda e: _pow_degree(e.n, poly_degree(e.base)),
+}
+
+
+def _combine(degrees, how):
+    if any(d is None for d in degrees):
+        return None
+    return how(degrees) if degrees else 0
+
+
+def poly_degree(e):
+    """Polynomial degree of a pointwise expression, None when it is not a polynomial."""
+    rule = _POLY_DEGREE.get(type(e))
+    return rule(e) if rule else None
+
+
 def quad_degree_estimate(e, kmax):
-    """forms.py:1481-1501: non-polynomial data counts as degree kmax."""
+    """The degree a quadrature rule needs for e: its polynomial degree, with every non-polynomial
+    part that holds no argument counted as degree kmax."""
     d = poly_degree(e)
-    if d is not None: return d
-    if not _has_arguments(e): return kmax
-    if isinstance(e, Argument): return _arg_degree(e)
-    if isinstance(e, Sum): return max((quad_degree_estimate(t, kmax) for t in e.terms), default=0)
-    if isinstance(e, Prod): return sum(quad_degree_estimate(f, kmax) for f in e.factors)
-    if isinstance(e, Pow):
-        if float(e.n).is_integer() and e.n >= 0:
-            return int(e.n) * quad_degree_estimate(e.base, kmax)
+    if d is not None:
+        return d
+    if not _has_arguments(e):
         return kmax
+    if isinstance(e, Argument):
+        return _arg_degree(e)
+    if isinstance(e, Sum):
+        return max((quad_degree_estimate(t, kmax) for t in e.terms), default=0)
+    if isinstance(e, Prod):
+        return sum(quad_degree_estimate(f, kmax) for f in e.factors)
+    if isinstance(e, Pow) and float(e.n).is_integer() and e.n >= 0:
+        return int(e.n) * quad_degree_estimate(e.base, kmax)
     return kmax
 
 
@@ -441,7 +608,8 @@ _default_quad_degree = None
 
 
 def set_quadrature_degree(degree=None):
-    """One rule for every form made from now on (forms.py:1451-1466)."""
+    """One rule for every form made from now on (python/femd/forms.py:1451-1466); None restores
+    the inferred degrees."""
     global _default_quad_degree
     if degree is None:
         _default_quad_degree = None
@@ -458,100 +626,135 @@ def quadrature_degree():
 
 # ============================================================================ programs
 class Program:
-    """A coefficient compiled into the register program of forms/integrand.hpp (forms.py:1822-1909):
-    post-order, one register per node, shared subexpressions emitted once (by identity)."""
-    _FUNCS = {"sin": 9, "cos": 10, "exp": 11, "log": 12, "tanh": 13, "sqrt": 14, "sinh": 15, "cosh": 16,
-              "sech": 17, "abs": 18, "sign": 19}
+    """A coefficient as a register program of the instruction set of forms/integrand.hpp:33-46
+    (4 ints per instruction: op, out, a, b), which the GPU runs at every quadrature point.
+
+    Emission order (the reference's, python/femd/forms.py:1822-1909): a node's operands before
+    the node; a sum (product) emits its first term, then each further term followed at once by an
+    ADD (MUL) of the running value and that term; a power emits its base, then POW with the
+    exponent in the constant table; a node met again (the same object) reuses its register.
+    Every instruction writes a new register, so register k holds instruction k's result."""
     CONST, X, Y, NX, NY, INPUT, ADD, MUL, POW = range(9)
+    _FUNCS = dict(_OPCODE)
 
     def __init__(self, expr):
-        self.code, self.consts, self.nreg, self.memo = [], [], 0, {}
-        self._emit(expr)
+        self.code, self.consts, self.memo = [], [], {}
+        self._compile(expr)
 
-    def _reg(self):
-        self.nreg += 1
-        return self.nreg - 1
+    @property
+    def nreg(self):
+        return len(self.code)
 
-    def _const(self, c):
+    def _put(self, op, a=0, b=0):
+        self.code.append((op, len(self.code), a, b))
+        return len(self.code) - 1
+
+    def _constant(self, c):
         self.consts.append(c)
         return len(self.consts) - 1
 
-    def _emit(self, e):
-        got = self.memo.get(id(e))
-        if got is not None and got[0] is e:
-            return got[1]
-        r = self._node(e)
-        self.memo[id(e)] = (e, r)
-        return r
-
-    def _node(self, e):
-        if isinstance(e, Const):
-            r = self._reg(); self.code.append((self.CONST, r, self._const(e.value), 0)); return r
-        if isinstance(e, Constant):
-            r = self._reg(); self.code.append((self.CONST, r, self._const(e), 0)); return r
+    def _leaf(self, e):
+        if isinstance(e, (Const, Constant)):
+            # the register is taken before the constant's slot, as each instruction takes it
+            reg = len(self.code)
+            self.code.append((self.CONST, reg, self._constant(e.value if isinstance(e, Const) else e), 0))
+            return reg
         if isinstance(e, Spatial):
-            r = self._reg(); self.code.append((self.X if e.index == 0 else self.Y, r, 0, 0)); return r
+            return self._put(self.X if e.index == 0 else self.Y)
         if isinstance(e, Normal):
-            r = self._reg(); self.code.append((self.NX if e.index == 0 else self.NY, r, 1, 0)); return r
-        if isinstance(e, Sum):
-            if not e.terms:
-                r = self._reg(); self.code.append((self.CONST, r, self._const(0.0), 0)); return r
-            acc = self._emit(e.terms[0])
-            for t in e.terms[1:]:
-                b = self._emit(t); r = self._reg(); self.code.append((self.ADD, r, acc, b)); acc = r
-            return acc
-        if isinstance(e, Prod):
-            if not e.factors:
-                r = self._reg(); self.code.append((self.CONST, r, self._const(1.0), 0)); return r
-            acc = self._emit(e.factors[0])
-            for f in e.factors[1:]:
-                b = self._emit(f); r = self._reg(); self.code.append((self.MUL, r, acc, b)); acc = r
-            return acc
-        if isinstance(e, Pow):
-            a = self._emit(e.base); r = self._reg(); self.code.append((self.POW, r, a, self._const(float(e.n)))); return r
-        if isinstance(e, Func):
-            a = self._emit(e.arg); r = self._reg(); self.code.append((self._FUNCS[e.name], r, a, 0)); return r
+            return self._put(self.NX if e.index == 0 else self.NY, 1)
         if isinstance(e, Argument):
             raise NotImplementedError("known fields inside coefficients are outside this path")
         raise NotImplementedError(f"cannot compile {e!r}")
 
+    def _compile(self, root):
+        """Post-order emission with an explicit stack of frames [node, operands taken, running
+        register]; a node finished (or met again) hands its register to the frame below."""
+        stack = [[root, 0, None]]
+        while stack:
+            frame = stack[-1]
+            node, taken, _ = frame
+            reg = None
+            if taken == 0:
+                seen = self.memo.get(id(node))
+                if seen is not None and seen[0] is node:
+                    stack.pop()
+                    self._handover(stack, seen[1])
+                    continue
+            ops = _children(node)
+            if taken < len(ops):
+                frame[1] = taken + 1
+                stack.append([ops[taken], 0, None])
+                continue
+            if ops:
+                reg = frame[2]                              # the folded value
+            elif isinstance(node, (Sum, Prod)):             # an empty sum (product) is 0 (1)
+                reg = len(self.code)
+                self.code.append((self.CONST, reg, self._constant(0.0 if isinstance(node, Sum) else 1.0), 0))
+            else:
+                reg = self._leaf(node)
+            self.memo[id(node)] = (node, reg)
+            stack.pop()
+            self._handover(stack, reg)
+
+    def _handover(self, stack, reg):
+        if stack:
+            self._absorb(stack[-1], reg)
+        else:
+            self.root = reg
+
+    def _absorb(self, frame, reg):
+        """An operand of frame's node is done (in register reg): fold it into the node's value."""
+        node, k, acc = frame
+        if isinstance(node, (Sum, Prod)):
+            if acc is None:
+                frame[2] = reg
+            else:
+                frame[2] = self._put(self.ADD if isinstance(node, Sum) else self.MUL, acc, reg)
+        elif isinstance(node, Pow):
+            out = len(self.code)
+            self.code.append((self.POW, out, reg, self._constant(float(node.n))))
+            frame[2] = out
+        else:
+            frame[2] = self._put(self._FUNCS[node.name], reg)
+
     def arrays(self):
-        """(code int32 (n, 4), nreg, consts float64) with the Constants' current values."""
+        """(code int32 (n, 4), registers, constants float64 with the Constants' current values)."""
         consts = [c.value if isinstance(c, Constant) else c for c in self.consts]
-        return np.asarray(self.code, dtype=np.int32).reshape(-1, 4), max(self.nreg, 1), np.asarray(consts, dtype=np.float64)
+        code = np.asarray(self.code, dtype=np.int32).reshape(-1, 4)
+        return code, max(self.nreg, 1), np.asarray(consts, dtype=np.float64)
 
 
-_NP_FUNCS = {"sin": np.sin, "cos": np.cos, "exp": np.exp, "log": np.log, "tanh": np.tanh, "sqrt": np.sqrt,
-             "sinh": np.sinh, "cosh": np.cosh, "sech": lambda v: 1.0 / np.cosh(v), "abs": np.abs, "sign": np.sign}
+# ============================================================================ NumPy evaluation
+_NUMPY = {"sin": np.sin, "cos": np.cos, "exp": np.exp, "log": np.log, "tanh": np.tanh, "sqrt": np.sqrt,
+          "sinh": np.sinh, "cosh": np.cosh, "sech": lambda v: 1.0 / np.cosh(v), "abs": np.abs, "sign": np.sign}
+
+
+def _eval_fold(values, op, start):
+    out = start
+    for v in values:
+        out = op(out, v)
+    return out
+
+
+_EVALUATE = {
+    Const: lambda e, x_, y_: e.value,
+    Constant: lambda e, x_, y_: e.value,
+    Spatial: lambda e, x_, y_: x_ if e.index == 0 else y_,
+    Sum: lambda e, x_, y_: _eval_fold((evaluate(t, x_, y_) for t in e.terms), lambda a, b: a + b, 0.0),
+    Prod: lambda e, x_, y_: _eval_fold((evaluate(f, x_, y_) for f in e.factors), lambda a, b: a * b, 1.0),
+    Pow: lambda e, x_, y_: np.power(evaluate(e.base, x_, y_), e.n),
+    Func: lambda e, x_, y_: _NUMPY[e.name](evaluate(e.arg, x_, y_)),
+}
 
 
 def evaluate(e, x, y):
     """An expression in fd.x, fd.y, numbers and Constants at the points (x, y), with NumPy (the
-    boundary data of V.lift; python/femd/forms.py evaluate)."""
-    if isinstance(e, (Const, Constant)): return e.value
-    if isinstance(e, Spatial): return x if e.index == 0 else y
-    if isinstance(e, Sum):
-        out = 0.0
-        for t in e.terms:
-            out = out + evaluate(t, x, y)
-        return out
-    if isinstance(e, Prod):
-        out = 1.0
-        for f in e.factors:
-            out = out * evaluate(f, x, y)
-        return out
-    if isinstance(e, Pow): return np.power(evaluate(e.base, x, y), e.n)
-    if isinstance(e, Func): return _NP_FUNCS[e.name](evaluate(e.arg, x, y))
-    raise TypeError(f"boundary data: an expression may contain fd.x, fd.y, numbers and Constants, not {e!r}")
-
-
-def _has_normal(e):
-    if isinstance(e, Normal): return True
-    if isinstance(e, Sum): return any(_has_normal(t) for t in e.terms)
-    if isinstance(e, Prod): return any(_has_normal(f) for f in e.factors)
-    if isinstance(e, Pow): return _has_normal(e.base)
-    if isinstance(e, Func): return _has_normal(e.arg)
-    return False
+    Dirichlet data of V.lift)."""
+    rule = _EVALUATE.get(type(e))
+    if rule is None:
+        raise TypeError(f"boundary data: an expression may contain fd.x, fd.y, numbers and Constants, not {e!r}")
+    return rule(e, x, y)
 
 
 def _rebind(f, U):

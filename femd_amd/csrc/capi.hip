@@ -496,219 +496,9 @@ void cg_noalloc(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, in
 }
 
 // --------------------------------------------------------------------------- GMRES / LGMRES
-// vendor/csnewton/csnewton.hpp:174-260 (gmres) and 298-474 (lgmres) with the adapter of
-// krylov/adapters.hpp:116-147.  Vectors and products on the GPU; the Hessenberg matrix, the
-// Givens rotations and the small triangular solves on the host.
-struct Krylov {
-    femd_csr_s *A;
-    femd_precond_s *M;
-    hipStream_t s;
-    int n;
-    DevBuf parts, scal, tmp;
-    Krylov(femd_csr_s *A_, femd_precond_s *M_, hipStream_t s_) : A(A_), M(M_), s(s_), n(A_->pat->n),
-        parts(8 * 1024 + 64), scal(8 * 512), tmp(8 * size_t(std::max(1, A_->pat->n))) {}
-    void matvec(const double *x, double *y) { spmv64(A, x, y, nullptr, s); }
-    void precon(const double *r, double *z)
-    {
-        if (M) precond_apply(M, r, z, nullptr, s);
-        else launch_copy(n, r, z, s);
-    }
-    void dot_into(const double *x, const double *y, double *out)
-    {
-        launch_fdot(n, x, y, parts.as<double>(), s);
-        launch_fsum(fast_blocks(n), parts.as<double>(), out, s);
-    }
-    double norm(const double *x)
-    {
-        dot_into(x, x, scal.as<double>());
-        return std::sqrt(read_scalar(scal.as<double>(), s));
-    }
-    // MGS of w against V[0..j] into column j of H; returns h_{j+1,j}
-    double arnoldi(double *w, const std::vector<double *> &V, int j, std::vector<double> &H, int ldh)
-    {
-        double *sd = scal.as<double>();
-        for (int i = 0; i <= j; ++i)
-        {
-            dot_into(w, V[i], sd + i);
-            launch_axpy_dev(n, sd + i, -1.0, V[i], w, s);
-        }
-        dot_into(w, w, sd + j + 1);
-        std::vector<double> h(static_cast<size_t>(j) + 2);
-        HIPC(hipMemcpyAsync(h.data(), sd, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        for (int i = 0; i <= j; ++i) H[i + static_cast<size_t>(j) * ldh] = h[i];
-        const double hn = std::sqrt(h[j + 1]);
-        H[(j + 1) + static_cast<size_t>(j) * ldh] = hn;
-        if (hn > 0.0)
-        {
-            launch_copy(n, w, V[j + 1], s);
-            launch_scale(n, 1.0 / hn, V[j + 1], s);
-        }
-        return hn;
-    }
-};
-
-void givens_gen(double dx, double dy, double &cs, double &sn)
-{
-    const double ax = std::abs(dx), ay = std::abs(dy);
-    if (ay == 0.0) { cs = 1.0; sn = 0.0; }
-    else if (ax == 0.0) { cs = 0.0; sn = 1.0; }
-    else { const double tau = std::sqrt(ax * ax + ay * ay); cs = ax / tau; sn = (dx / ax) * (ay / tau); }
-}
-void givens_apply(double &dx, double &dy, double cs, double sn)
-{
-    const double t = cs * dx + sn * dy;
-    dy = -sn * dx + cs * dy;
-    dx = t;
-}
-void rotate_column(std::vector<double> &H, int ldh, int j, std::vector<double> &cs, std::vector<double> &sn, std::vector<double> &g)
-{
-    for (int i = 0; i < j; ++i) givens_apply(H[i + j * ldh], H[(i + 1) + j * ldh], cs[i], sn[i]);
-    givens_gen(H[j + j * ldh], H[(j + 1) + j * ldh], cs[j], sn[j]);
-    givens_apply(H[j + j * ldh], H[(j + 1) + j * ldh], cs[j], sn[j]);
-    givens_apply(g[j], g[j + 1], cs[j], sn[j]);
-}
-std::vector<double> back_solve(const std::vector<double> &H, int ldh, const std::vector<double> &g, int k)
-{
-    std::vector<double> y(g.begin(), g.begin() + k + 1);
-    for (int i = k; i >= 0; --i)
-    {
-        y[i] /= H[i + i * ldh];
-        for (int j = 0; j < i; ++j) y[j] -= H[j + i * ldh] * y[i];
-    }
-    return y;
-}
-
-struct GRes { bool converged; int iterations; double residual; };
-
-GRes gmres_dev(Krylov &K, double *x, const double *b, int restart, int max_iter, double tol)
-{
-    const int n = K.n, ldh = restart + 1;
-    DevBuf Vb(8 * size_t(n) * (restart + 1)), w(8 * size_t(n)), r(8 * size_t(n)), z(8 * size_t(n)), yb(8 * size_t(restart + 1));
-    std::vector<double *> V(restart + 1);
-    for (int i = 0; i <= restart; ++i) V[i] = Vb.as<double>() + size_t(i) * n;
-    std::vector<double> H(size_t(ldh) * restart), cs(restart), sn(restart), g(restart + 1);
-    auto update = [&](int k) {
-        std::vector<double> y = back_solve(H, ldh, g, k);
-        HIPC(hipMemcpyAsync(yb.p, y.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, K.s));
-        launch_lincomb(n, k + 1, yb.as<double>(), V.data(), x, K.s);
-        HIPC(hipStreamSynchronize(K.s));
-    };
-    K.matvec(x, w.as<double>());
-    launch_sub(n, b, w.as<double>(), r.as<double>(), K.s);
-    K.precon(r.as<double>(), z.as<double>());
-    double normb = K.norm(b);
-    if (normb == 0.0) normb = 1.0;
-    double beta = K.norm(z.as<double>());
-    if (beta / normb < tol) return {true, 0, beta / normb};
-    int total = 0;
-    while (total < max_iter)
-    {
-        std::fill(H.begin(), H.end(), 0.0);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        launch_copy(n, z.as<double>(), V[0], K.s);
-        launch_scale(n, 1.0 / beta, V[0], K.s);
-        int i = 0;
-        for (; i < restart && total < max_iter; ++i, ++total)
-        {
-            K.matvec(V[i], r.as<double>());
-            K.precon(r.as<double>(), w.as<double>());
-            const double hn = K.arnoldi(w.as<double>(), V, i, H, ldh);
-            if (hn == 0.0) { update(i); return {true, total + 1, 0.0}; }
-            rotate_column(H, ldh, i, cs, sn, g);
-            const double est = std::abs(g[i + 1]) / normb;
-            if (est < tol) { update(i); return {true, total + 1, est}; }
-        }
-        update(i - 1);
-        K.matvec(x, w.as<double>());
-        launch_sub(n, b, w.as<double>(), r.as<double>(), K.s);
-        K.precon(r.as<double>(), z.as<double>());
-        beta = K.norm(z.as<double>());
-        if (beta / normb < tol) return {true, total, beta / normb};
-    }
-    return {false, total, beta / normb};
-}
-
-GRes lgmres_dev(Krylov &K, double *x, const double *b, int inner, int k_aug, int max_iter, double tol)
-{
-    const int n = K.n;
-    if (k_aug < 0) fail(FEMD_EINVAL, "lgmres: k_aug must be non-negative");
-    const int mmax = inner + k_aug;
-    DevBuf Vb(8 * size_t(n) * (mmax + 1)), Zb(8 * size_t(n) * std::max(k_aug, 1)), MZb(8 * size_t(n) * std::max(k_aug, 1));
-    DevBuf Ax(8 * size_t(n)), r(8 * size_t(n)), q(8 * size_t(n)), w(8 * size_t(n)), zn(8 * size_t(n)), yb(8 * size_t(mmax + 1));
-    std::vector<double *> V(mmax + 1), Zs, MZs;            // Zs[0] is the newest correction
-    for (int i = 0; i <= mmax; ++i) V[i] = Vb.as<double>() + size_t(i) * n;
-    std::vector<double *> zslot(std::max(k_aug, 1)), mslot(std::max(k_aug, 1));
-    for (int i = 0; i < std::max(k_aug, 1); ++i) { zslot[i] = Zb.as<double>() + size_t(i) * n; mslot[i] = MZb.as<double>() + size_t(i) * n; }
-    double normb = K.norm(b);
-    if (normb == 0.0) normb = 1.0;
-    int total = 0;
-    auto residual = [&]() {
-        K.matvec(x, Ax.as<double>());
-        launch_sub(n, b, Ax.as<double>(), r.as<double>(), K.s);
-        K.precon(r.as<double>(), q.as<double>());
-        return K.norm(q.as<double>());
-    };
-    while (total < max_iter)
-    {
-        const int k = static_cast<int>(Zs.size()), m = inner + k, ldh = m + 1;
-        const double beta = residual();
-        if (beta / normb < tol) return {true, total, beta / normb};
-        std::vector<double> H(size_t(ldh) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
-        launch_copy(n, q.as<double>(), V[0], K.s);
-        launch_scale(n, 1.0 / beta, V[0], K.s);
-        g[0] = beta;
-        int final_j = m - 1, built = 0;
-        bool conv = false;
-        const int inner_lim = std::min(inner, max_iter - total);
-        for (int j = 0; j < inner_lim; ++j, ++total)
-        {
-            K.matvec(V[j], r.as<double>());
-            K.precon(r.as<double>(), w.as<double>());
-            const double hn = K.arnoldi(w.as<double>(), V, j, H, ldh);
-            rotate_column(H, ldh, j, cs, sn, g);
-            built = j + 1;
-            if (std::abs(g[j + 1]) / normb < tol || hn == 0.0) { final_j = j; conv = true; ++total; break; }
-        }
-        if (!conv)
-        {
-            const int aug_lim = std::min(k, max_iter - total);
-            for (int aug = 0; aug < aug_lim; ++aug, ++total)
-            {
-                const int j = inner + aug;
-                launch_copy(n, MZs[aug], w.as<double>(), K.s);
-                const double hn = K.arnoldi(w.as<double>(), V, j, H, ldh);
-                rotate_column(H, ldh, j, cs, sn, g);
-                built = j + 1;
-                if (std::abs(g[j + 1]) / normb < tol || hn == 0.0) { final_j = j; conv = true; ++total; break; }
-            }
-        }
-        if (!conv) final_j = built - 1;
-        std::vector<double> y = back_solve(H, ldh, g, final_j);
-        HIPC(hipMemcpyAsync(yb.p, y.data(), sizeof(double) * (final_j + 1), hipMemcpyHostToDevice, K.s));
-        std::vector<const double *> basis(final_j + 1);
-        for (int j = 0; j <= final_j; ++j) basis[j] = j < inner ? V[j] : Zs[j - inner];
-        launch_fill(n, 0.0, zn.as<double>(), K.s);
-        launch_lincomb(n, final_j + 1, yb.as<double>(), basis.data(), zn.as<double>(), K.s);
-        launch_axpy(n, 1.0, zn.as<double>(), x, K.s);
-        if (k_aug > 0)
-        {
-            // the oldest slot is recycled for the newest correction
-            double *zs = nullptr, *ms = nullptr;
-            if (static_cast<int>(Zs.size()) < k_aug) { zs = zslot[Zs.size()]; ms = mslot[Zs.size()]; }
-            else { zs = Zs.back(); ms = MZs.back(); Zs.pop_back(); MZs.pop_back(); }
-            launch_copy(n, zn.as<double>(), zs, K.s);
-            K.matvec(zs, Ax.as<double>());
-            K.precon(Ax.as<double>(), ms);
-            Zs.insert(Zs.begin(), zs);
-            MZs.insert(MZs.begin(), ms);
-        }
-        HIPC(hipStreamSynchronize(K.s));
-        if (conv) return {true, total, residual() / normb};
-    }
-    return {false, total, residual() / normb};
-}
+// krylov.hip: the solvers with their Arnoldi steps on the device
+void krylov_device(femd_csr_s *A, femd_precond_s *M, const double *b, double *x, int lgmres, int restart, int max_iter,
+                   double tol, int k_aug, femd_solve_info *info, hipStream_t s);
 
 // ------------------------------------------------------------------------ mixed precision
 void refined_device(femd_csr_s *A, const double *b, double *x, double tol, double inner_tol, int max_outer,
@@ -1746,31 +1536,10 @@ int femd_krylov(femd_csr A, femd_precond M, const double *b, double *x, int lgmr
         if (M && M->n != a->pat->n) fail(FEMD_EINVAL, "krylov: preconditioner and right-hand side differ in size");
         if (restart <= 0 || max_iter <= 0) fail(FEMD_EINVAL, "gmres: restart and max_iter must be positive");
         if (restart + (lgmres ? std::max(k_aug, 0) : 0) + 2 > 512) fail(FEMD_EINVAL, "gmres: restart (+ k_aug) must stay below 510");
-        std::memset(info, 0, sizeof(*info));
         hipStream_t s = S(nullptr);
         const int n = a->pat->n;
-        DevBuf bd = upload(b, size_t(n), s), xd = upload(x, size_t(n), s), Mb(8 * size_t(std::max(n, 1)));
-        Krylov K(a, M, s);
-        const double nb = K.norm(bd.as<double>());
-        if (nb == 0.0)
-        {
-            std::fill(x, x + n, 0.0);
-            info->converged = 1;
-            return;
-        }
-        K.precon(bd.as<double>(), Mb.as<double>());
-        const double nMb = K.norm(Mb.as<double>());
-        const double scaled = nMb > 0.0 ? tol * nMb / nb : tol;
-        GRes g = lgmres ? lgmres_dev(K, xd.as<double>(), bd.as<double>(), restart, k_aug, max_iter, scaled)
-                        : gmres_dev(K, xd.as<double>(), bd.as<double>(), restart, max_iter, scaled);
-        DevBuf Ax(8 * size_t(n)), r(8 * size_t(n)), Mr(8 * size_t(n));
-        K.matvec(xd.as<double>(), Ax.as<double>());
-        launch_sub(n, bd.as<double>(), Ax.as<double>(), r.as<double>(), s);
-        K.precon(r.as<double>(), Mr.as<double>());
-        info->iterations = g.iterations;
-        info->residual = K.norm(r.as<double>()) / nb;
-        info->precond_residual = nMb > 0.0 ? K.norm(Mr.as<double>()) / nMb : 0.0;
-        info->converged = info->precond_residual <= tol * (1.0 + 1e-8) || (g.converged && info->precond_residual <= 10.0 * tol);
+        DevBuf bd = upload(b, size_t(n), s), xd = upload(x, size_t(n), s);
+        krylov_device(a, M, bd.as<double>(), xd.as<double>(), lgmres, restart, max_iter, tol, k_aug, info, s);
         if (n) HIPC(hipMemcpyAsync(x, xd.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
     });

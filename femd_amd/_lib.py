@@ -118,8 +118,8 @@ _SIGS = {
     "femd_batch_solve": (_i, [_p, _dp, _dp, _d, _d, _i, _i, _i, _p, _p, C.POINTER(SolveInfo)]),
     "femd_batch_solve_dev": (_i, [_p, _p, _p, _d, _d, _i, _i, _i, _p, _p, C.POINTER(SolveInfo), _p]),
     "femd_batch_free": (None, [_p]),
-    "femd_heat_run": (_i, [_p, _p, _dp, _i, _d, _i, _d, _i, _i, _i, _p, C.POINTER(SolveInfo)]),
-    "femd_heat_run_dev": (_i, [_p, _p, _p, _i, _d, _i, _d, _i, _i, _i, _p, C.POINTER(SolveInfo), _p]),
+    "femd_heat_run": (_i, [_p, _p, _dp, _i, _d, _i, _d, _i, _i, _i, _p, C.POINTER(_i), C.POINTER(SolveInfo)]),
+    "femd_heat_run_dev": (_i, [_p, _p, _p, _i, _d, _i, _d, _i, _i, _i, _p, C.POINTER(_i), C.POINTER(SolveInfo), _p]),
     "femd_event_timer_start": (_i, [_p]),
     "femd_event_timer_stop": (_i, [_p, C.POINTER(_d)]),
     "femd_inner_timer": (_i, [_i]),

@@ -28,9 +28,11 @@ struct StepScalars {
     int counter;        // steps done
     int total_inner;
     int seen;           // the engine's iteration total at the last record
-    int pad;
+    int step_inner;     // the current step's inner iterations
     double rr_end;      // ||b - A u||^2 once the step's rounds are done
     double worst;       // the largest ||b - A u||^2 / ||b||^2 of any step so far
+    int failed;         // steps the verdict of krylov/cg.hpp:222-237 does not accept
+    int pad;
 };
 
 __global__ void k_gate(StepScalars *sc, double tol2)
@@ -53,13 +55,21 @@ __global__ void k_record(const CGSys *st, StepScalars *sc, int *iters, int nmax)
     sc->seen = st->total;
     if (iters && sc->counter < nmax) iters[sc->counter] += it;
     sc->total_inner += it;
+    sc->step_inner += it;
 }
 
-__global__ void k_next_step(StepScalars *sc)
+// The step's verdict, the reference's for one CG solve (krylov/cg.hpp:222-237): its true relative
+// residual within tol (1 + 1e-6) + 1e-15, or within 10 tol when its rounds stopped before their
+// iteration budget ran out (an FP64 residual near its attainable accuracy).  Then the next step.
+__global__ void k_next_step(StepScalars *sc, double tol, int budget)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double rel2 = sc->bb > 0.0 ? sc->rr_end / sc->bb : sc->rr_end;
     if (!(rel2 <= sc->worst)) sc->worst = rel2;        // NaN sticks
+    const double rel = sqrt(rel2);
+    const bool ok = rel <= tol * (1.0 + 1e-6) + 1e-15 || (rel <= 10.0 * tol && sc->step_inner < budget);
+    if (!ok) sc->failed += 1;
+    sc->step_inner = 0;
     sc->counter += 1;
 }
 
@@ -101,12 +111,12 @@ struct Stepper {
         spmv_run64(*pA, *A->pat, A->v.as<double>(), u, t.as<double>(), nullptr, s);
         launch_sub(n, b.as<double>(), t.as<double>(), r.as<double>(), s);
         norm2(r.as<double>(), &scal()->rr_end, s);
-        hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, s, scal());
+        hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, s, scal(), tol, rounds * max_inner);
     }
 };
 
 void heat_run(femd_csr_s *M, femd_csr_s *A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
-              int precision, int use_graph, int *iters_out, femd_solve_info *info, hipStream_t s)
+              int precision, int use_graph, int *iters_out, int *failed_out, femd_solve_info *info, hipStream_t s)
 {
     const DevPattern &PM = *M->pat, &PA = *A->pat;
     const int n = PA.n;
@@ -116,6 +126,7 @@ void heat_run(femd_csr_s *M, femd_csr_s *A, double *u, int nsteps, double tol, i
     if (precision != FEMD_FP32 && precision != FEMD_FP64) fail(FEMD_EINVAL, "time stepping: precision is FEMD_FP32 or FEMD_FP64");
     std::memset(info, 0, sizeof(*info));
     if (iters_out && nsteps > 0) std::memset(iters_out, 0, sizeof(int) * size_t(nsteps));
+    if (failed_out) *failed_out = 0;
     if (n == 0) { info->converged = 1; return; }
     Stepper T;
     T.M = M; T.A = A; T.n = n; T.rounds = rounds; T.max_inner = max_inner; T.nsteps = nsteps;
@@ -179,7 +190,8 @@ void heat_run(femd_csr_s *M, femd_csr_s *A, double *u, int nsteps, double tol, i
     info->breakdown = hs.counter;
     info->residual = res;
     info->precond_residual = res;
-    info->converged = (hs.counter == nsteps && res <= std::max(tol, 0.0) * (1.0 + 1e-6) + 1e-15) ? 1 : 0;
+    info->converged = (hs.counter == nsteps && hs.failed == 0) ? 1 : 0;
+    if (failed_out) *failed_out = hs.failed;
 }
 
 } // namespace
@@ -190,17 +202,18 @@ using namespace femd_amd;
 extern "C" {
 
 int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,
-                      int max_inner, int precision, int use_graph, int *iters, femd_solve_info *info, femd_stream st)
+                      int max_inner, int precision, int use_graph, int *iters, int *failed, femd_solve_info *info,
+                      femd_stream st)
 {
     return api([&] {
         if (!info) fail(FEMD_EINVAL, "time stepping: info is NULL");
         heat_run(check_csr(M), check_csr(A), u_dev, nsteps, tol, rounds, inner_tol, max_inner, precision, use_graph, iters,
-                 info, S(st));
+                 failed, info, S(st));
     });
 }
 
 int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
-                  int precision, int use_graph, int *iters, femd_solve_info *info)
+                  int precision, int use_graph, int *iters, int *failed, femd_solve_info *info)
 {
     return api([&] {
         if (!info) fail(FEMD_EINVAL, "time stepping: info is NULL");
@@ -209,7 +222,7 @@ int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int
         const int n = a->pat->n;
         DevBuf ud = upload(u, size_t(n), s);
         heat_run(check_csr(M), a, ud.as<double>(), nsteps, tol, rounds, inner_tol, max_inner, precision, use_graph, iters,
-                 info, s);
+                 failed, info, s);
         if (n) HIPC(hipMemcpyAsync(u, ud.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
     });

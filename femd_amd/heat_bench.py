@@ -113,7 +113,7 @@ def main(argv=None):
         out[f"graph_stepper_{prec}"] = {
             "seconds": round(t_graph, 4), "steps_per_s": round(args.steps / t_graph, 2),
             "speedup_vs_plain": round(t_plain / t_graph, 3), "inner_iterations": info["total_iterations"],
-            "every_step_converged": info["converged"], "worst_step_residual": info["residual"], "final_rel_diff_vs_reference": rel(u),
+            "every_step_converged": info["converged"], "failed_steps": info["failed_steps"], "worst_step_residual": info["residual"], "final_rel_diff_vs_reference": rel(u),
             "no_graph_seconds": round(t_nograph, 4), "bit_identical_without_graph": bool(np.array_equal(u, u2))}
     lib.femd_precond_free(Mj)
     for p in (ud, bd):

@@ -33,8 +33,9 @@ class BackwardEuler:
         segment of at most max_inner CG iterations in `precision` ("fp32" or "fp64") that ends when
         its residual has dropped by inner_tol or the step's residual meets tol; later rounds continue
         the same CG (reliable updates).  Returns (u, info) with info a dict: per-step inner
-        iterations, their total, the largest relative residual any step ended with and whether
-        every step met tol."""
+        iterations, their total, the largest relative residual any step ended with, the steps
+        whose residual the reference's CG verdict would not accept (krylov/cg.hpp:222-237: tol, or
+        10 tol within the iteration budget) and whether there were none."""
         n = self.A.shape[0]
         u = np.array(u0, dtype=np.float64).reshape(-1)
         if u.size != n:
@@ -44,7 +45,9 @@ class BackwardEuler:
             raise ValueError(f"BackwardEuler: precision is 'fp32' or 'fp64', not {precision!r}")
         its = np.zeros(max(int(nsteps), 1), dtype=np.int32)
         info = L.SolveInfo()
+        failed = C.c_int(0)
         L.check(L.load().femd_heat_run(self.M._h, self.A._h, u, int(nsteps), float(tol), int(rounds), float(inner_tol),
-                                       int(max_inner), prec, int(bool(graph)), its.ctypes.data, C.byref(info)))
+                                       int(max_inner), prec, int(bool(graph)), its.ctypes.data, C.byref(failed),
+                                       C.byref(info)))
         return u, dict(converged=bool(info.converged), iterations=its[:int(nsteps)], total_iterations=int(info.iterations),
-                       steps=int(info.breakdown), residual=float(info.residual))
+                       steps=int(info.breakdown), residual=float(info.residual), failed_steps=int(failed.value))

@@ -389,12 +389,16 @@ void femd_server_free(femd_server S);
  * is captured once into a hipGraph and replayed nsteps times with no host synchronisation between
  * steps.  iters[k] (may be NULL) gets step k's inner iterations; info.iterations their sum,
  * info.breakdown the number of steps done, info.residual the largest ||b - A u|| / ||b|| that any
- * step ended with (each step checks its own FP64 residual after its rounds), info.converged
- * whether that is <= tol. */
+ * step ended with (each step checks its own FP64 residual after its rounds).  Each step gets the
+ * verdict of one reference CG solve (krylov/cg.hpp:222-237): accepted when its residual is within
+ * tol (1 + 1e-6) + 1e-15, or within 10 tol when its rounds stopped before using all
+ * rounds x max_inner iterations; *failed (may be NULL) receives the steps not accepted and
+ * info.converged says whether there were none. */
 int femd_heat_run(femd_csr M, femd_csr A, double *u, int nsteps, double tol, int rounds, double inner_tol, int max_inner,
-                  int precision, int use_graph, int *iters, femd_solve_info *info);
+                  int precision, int use_graph, int *iters, int *failed, femd_solve_info *info);
 int femd_heat_run_dev(femd_csr M, femd_csr A, double *u_dev, int nsteps, double tol, int rounds, double inner_tol,
-                      int max_inner, int precision, int use_graph, int *iters, femd_solve_info *info, femd_stream s);
+                      int max_inner, int precision, int use_graph, int *iters, int *failed, femd_solve_info *info,
+                      femd_stream s);
 
 /* ---------------------------------------------------------------- timing
  * Kernel-time of the library's work on a stream, measured with HIP events around it. */

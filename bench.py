@@ -1,9 +1,9 @@
 """Benchmark of the 2D assemble -> SpMV -> Krylov path on one MI355X.
 
 Workload (the P3 configuration): -div(kappa grad u) = f on the unit square, a seeded
-unstructured Delaunay mesh (h = 0.00265: ~1.48M P3 unknowns, ~25M stored entries),
-kappa = exp(0.5 g) with g a sum of 16 seeded Fourier modes, f the seeded 8-mode sine series,
-homogeneous Dirichlet sides.
+unstructured Delaunay mesh refined to a minimum angle of 28 degrees (h = 0.00265: ~1.48M P3
+unknowns, ~25M stored entries), kappa = exp(0.5 g) with g a sum of 16 seeded Fourier modes, f the
+seeded 8-mode sine series, homogeneous Dirichlet sides.
 
 One step = assembly of A and b in FP64 on the GPU (coefficient programs evaluated at the 25 / 20
 quadrature points of every cell on the device) + the mixed-precision solve: FP32 Jacobi-PCG inner
@@ -18,11 +18,14 @@ Also reported: the roofline of the dominant kernel, the FP32 inner CG's product 
 algorithmic bytes 8 nnz + 20 n + 4 over its average duration, sampled with HIP events on its stream
 in one extra step after the timed ones); the plain SpMV kernels' HBM throughput (12 nnz + 20 n + 4 bytes in FP64,
 8 nnz + 12 n + 4 in FP32), back to back and with the Infinity Cache flushed, on the matrix as
-numbered and on its copy in the inner CG's order of the unknowns; the plain FP64 GPU path
-(assembly + Jacobi-CG to the same rtol) timed once; p50 / p99 step latency; and the CPU baseline:
-the reference's own code (oracle/_ref) on the host cores, on a bounded sample (the full assembly and
-100 + 300 of its Jacobi-CG iterations, scaled to one solve with its iteration count to the same
-rtol, which the plain FP64 GPU path reproduces bit for bit).
+numbered and on its copy in the inner CG's order of the unknowns; the plain FP64 GPU paths to the
+same rtol, each timed once with its assembly: Jacobi-CG, and ILU(0)-RCM CG (the configuration's
+FP64 solver) with its factorisation and the throughput of one preconditioner application
+(12 nnz + 28 n algorithmic bytes); the matrix assembly's throughput (8 nnz + 4 nloc ncells +
+16 npoints + 12 ncells algorithmic bytes); p50 / p99 step latency; and the CPU baseline: the
+reference's own code (oracle/_ref) on the host cores, the full assembly and full ILU(0)-RCM CG and
+Jacobi-CG solves to the same rtol, measured, their iteration counts and solutions compared with the
+GPU's FP64 solves, plus a sampled 1-thread figure.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--h 0.00265] [--rtol 1e-8] [--no-cpu]
 """
@@ -61,6 +64,9 @@ def main():
                     help="two extra steps at this rtol, reported beside the headline (0: none)")
     ap.add_argument("--inner-tol", type=float, default=0.3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full-limit", type=float, default=150.0,
+                    help="run a CPU reference solve in full when a short sample predicts it within this many "
+                         "seconds (otherwise it is reported from the sample)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--original-order", action="store_true",
                     help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
@@ -121,10 +127,63 @@ def main():
     t1 = time.perf_counter()
     A64 = a_form.assemble(reuse=False)
     b64 = L_form.assemble()
-    _, ip64 = fd.cg(A64, b64, M=A64.preconditioner("jacobi"), tol=args.rtol, maxiter=200000)
+    xj64, ip64 = fd.cg(A64, b64, M=A64.preconditioner("jacobi"), tol=args.rtol, maxiter=200000)
     t_fp64 = time.perf_counter() - t1
-    del A64
     log(f"plain FP64 GPU path (assembly + Jacobi-CG to {args.rtol:g}): {ip64.iterations} iterations, {t_fp64:.2f} s")
+
+    # ---- the FP64 ILU(0)-RCM CG leg: assembly, factor, solve to rtol on the GPU --------------
+    t1 = time.perf_counter()
+    Ai = a_form.assemble(reuse=False)
+    bi = L_form.assemble()
+    t2 = time.perf_counter()
+    Milu = Ai.preconditioner("ilu0", ordering="rcm")
+    t3 = time.perf_counter()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        xilu, iilu = fd.cg(Ai, bi, M=Milu, tol=args.rtol, maxiter=20000)
+    t4 = time.perf_counter()
+    from femd_amd.roofline import precond_apply_time
+    t_apply = precond_apply_time(Milu, n, reps=5)
+    sw = Milu.stats()
+    # one application reads the factor's values and columns (12 bytes per stored entry of L + U,
+    # which has A's pattern) and per unknown r, the permutation both ways, the inverse pivot and the
+    # intermediate y (28 bytes): the two sweeps' algorithmic bytes
+    B_apply = 12 * nnz + 28 * n
+    ilu = {"iterations": iilu.iterations, "converged": bool(iilu.converged), "residual": iilu.residual,
+           "assembly_s": round(t2 - t1, 3), "factor_s": round(t3 - t2, 3), "solve_s": round(t4 - t3, 3),
+           "seconds": round(t4 - t1, 3), "solves_per_s": round(1.0 / (t4 - t1), 4),
+           "ms_per_iteration": round(1e3 * (t4 - t3) / max(iilu.iterations, 1), 3),
+           "apply": {"ms": round(t_apply, 3), "algorithmic_bytes": B_apply,
+                     "gbs": round(B_apply / (t_apply * 1e-3) / 1e9, 1),
+                     "frac": round(B_apply / (t_apply * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                     "stream_bytes": sw["bytes_forward"] + sw["bytes_backward"], "planned": sw["planned"],
+                     "levels": [sw["levels_forward"], sw["levels_backward"]],
+                     "steps": [sw["steps_forward"], sw["steps_backward"]]}}
+    del Milu, Ai
+    log(f"FP64 ILU(0)-RCM CG on the GPU: {iilu.iterations} iterations, residual {iilu.residual:.2e}, assembly "
+        f"{t2 - t1:.2f} s + factor {t3 - t2:.2f} s + solve {t4 - t3:.2f} s; apply {t_apply:.2f} ms "
+        f"({ilu['apply']['gbs']:.0f} GB/s of {B_apply} algorithmic bytes)")
+
+    # ---- assembly throughput: the matrix assembly kernels into the existing CSR values ---------
+    coeffs_a, keep_a = a_form._coefficients()
+    aa = L.i32([t.test.code() for t in a_form.terms])
+    ab = L.i32([t.trial.code() for t in a_form.terms])
+    asm = lambda: L.check(lib.femd_assemble_matrix(V._h, a_form.quad_degree, len(a_form.terms), aa, ab, coeffs_a, A64._h))
+    asm()
+    L.check(lib.femd_stream_sync(None))
+    ta = time.perf_counter()
+    for _ in range(5):
+        asm()
+    L.check(lib.femd_stream_sync(None))
+    t_asm = (time.perf_counter() - ta) / 5
+    del keep_a
+    nloc = (V.degree + 1) * (V.degree + 2) // 2
+    # the least an assembly moves: the values it writes (8 nnz), each cell's local-to-global map
+    # (4 nloc per cell), the vertices (16 per point) and the cells' vertex indices (12 per cell)
+    B_asm = 8 * nnz + 4 * nloc * int(T.shape[0]) + 16 * int(P.shape[0]) + 12 * int(T.shape[0])
+    assembly = {"ms": round(1e3 * t_asm, 3), "algorithmic_bytes": B_asm, "gbs": round(B_asm / t_asm / 1e9, 1),
+                "frac": round(B_asm / t_asm / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"matrix assembly: {1e3 * t_asm:.1f} ms, {assembly['gbs']:.0f} GB/s of {B_asm} algorithmic bytes")
 
     for _ in range(args.warmup):
         step()
@@ -205,7 +264,18 @@ def main():
     # ---- CPU baseline: the reference's own code, bounded sample ---------------------------
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(P, T, a_form, L_form, ip64.iterations, args.rtol, log)
+        cpu, xref = cpu_baseline(P, T, a_form, L_form, {"ilu0": iilu.iterations, "jacobi": ip64.iterations},
+                                 args.rtol, args.cpu_full_limit, log)
+        # the GPU's FP64 solves against the reference's on the same inputs
+        for key, xg, gi, leg in (("ilu0", xilu, iilu.iterations, "ilu0_rcm_cg"), ("jacobi", xj64, ip64.iterations, "jacobi_cg")):
+            c = cpu["legs"][leg]
+            if key in xref:
+                c["gpu_iterations"] = gi
+                c["iterations_equal"] = bool(gi == c["iterations"])
+                c["gpu_rel_diff"] = float(np.linalg.norm(xg - xref[key]) / np.linalg.norm(xref[key]))
+                log(f"{leg}: GPU {gi} / reference {c['iterations']} iterations, solutions differ by "
+                    f"{c['gpu_rel_diff']:.2e} relative")
+    del A64
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -222,6 +292,8 @@ def main():
         "strict_rtol": strict,
         "plain_fp64_gpu": {"iterations": ip64.iterations, "seconds": round(t_fp64, 3),
                            "solves_per_s": round(1.0 / t_fp64, 4), "speedup_of_value": round(value * t_fp64, 2)},
+        "fp64_ilu0_rcm_gpu": ilu,
+        "assembly": assembly,
         "spmv": {"fp64": {"us": round(t64 * 1e3, 2), "gbs": round(gbs(B64, t64), 1), "shape": variants["fp64"],
                           "flushed_us": round(c64 * 1e3, 2), "flushed_gbs": round(gbs(B64, c64), 1),
                           "flushed_frac": round(gbs(B64, c64) / HBM_PEAK_GBS, 4)},
@@ -259,11 +331,13 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(P, T, a_form, L_form, fp64_iters, rtol, log):
-    """The reference (oracle/_ref, the reference's own C++ compiled in place) on the host cores:
-    the full assembly (cache, coefficient programs, element matrices, scatter) and a bounded sample
-    of its Jacobi-CG (cg_jacobi, the algorithm of the plain FP64 GPU path, whose iteration count to
-    rtol on this matrix is known from that run, bit-identical to the reference), scaled to one solve."""
+def cpu_baseline(P, T, a_form, L_form, expected, rtol, limit, log):
+    """The reference (oracle/_ref, the reference's own C++ compiled in place) on the host cores: the
+    full assembly (cache, coefficient programs, element matrices, scatter), then its ILU(0)-RCM CG
+    and its Jacobi-CG, each solved in full to rtol and timed when a sample of its iterations
+    predicts the full solve within `limit` seconds (else reported from the sample, and marked so);
+    then a 1-thread figure from a sample: the assembly and 20 ILU(0)-CG iterations at one thread,
+    scaled to the full solve's iteration count.  Returns (the baseline dict, the solutions)."""
     import oracle
     R = oracle.reference()
     kind = "reference"
@@ -271,33 +345,71 @@ def cpu_baseline(P, T, a_form, L_form, fp64_iters, rtol, log):
         R = oracle.oracle()
         kind = "port"
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    R.set_threads(cores)
     from femd_amd.forms import Program
     Pr = R.from_arrays(P, T, k=3)
-    t0 = time.perf_counter()
-    x, y, _ = Pr.cache_points(a_form.quad_degree)
-    ck = R.integrand_eval(*Program(a_form.terms[0].coeff).arrays(), x, y)
-    Av = Pr.assemble_matrix(a_form.quad_degree, [1, 2], [1, 2], np.stack([ck, ck]))
-    xb, yb, _ = Pr.cache_points(L_form.quad_degree)
-    cf = R.integrand_eval(*Program(L_form.terms[0].coeff).arrays(), xb, yb)
-    b = Pr.assemble_vector(L_form.quad_degree, [0], cf[None, :])
-    t_asm = time.perf_counter() - t0
+
+    def assemble():
+        t0 = time.perf_counter()
+        x, y, _ = Pr.cache_points(a_form.quad_degree)
+        ck = R.integrand_eval(*Program(a_form.terms[0].coeff).arrays(), x, y)
+        Av = Pr.assemble_matrix(a_form.quad_degree, [1, 2], [1, 2], np.stack([ck, ck]))
+        xb, yb, _ = Pr.cache_points(L_form.quad_degree)
+        cf = R.integrand_eval(*Program(L_form.terms[0].coeff).arrays(), xb, yb)
+        b = Pr.assemble_vector(L_form.quad_degree, [0], cf[None, :])
+        return Av, b, time.perf_counter() - t0
+
+    R.set_threads(cores)
+    Av, b, t_asm = assemble()
     ip, ix = Pr.pattern()
-    ts = []
-    for it in (100, 300):
+    log(f"CPU reference ({cores} threads): assembly {t_asm:.2f} s")
+    legs, xs = {}, {}
+    for key, leg, sample in (("ilu0", "ilu0_rcm_cg", 20), ("jacobi", "jacobi_cg", 200)):
         t1 = time.perf_counter()
-        R.cg(ip, ix, Av, b, M="jacobi", maxiter=it)
+        R.cg(ip, ix, Av, b, M=key, maxiter=sample)
+        t_sample = time.perf_counter() - t1
+        predicted = t_sample * expected[key] / sample
+        if predicted <= limit:
+            t1 = time.perf_counter()
+            x, io = R.cg(ip, ix, Av, b, M=key, tol=rtol, maxiter=100000)
+            t_solve = time.perf_counter() - t1
+            xs[key] = x
+            legs[leg] = {"measured": True, "iterations": int(io["iterations"]), "converged": bool(io["converged"]),
+                         "residual": float(io["residual"]), "solve_s": round(t_solve, 3),
+                         "seconds": round(t_asm + t_solve, 3)}
+        else:
+            legs[leg] = {"measured": False, "iterations": int(expected[key]), "sample_iterations": sample,
+                         "sample_s": round(t_sample, 3), "solve_s": round(predicted, 3),
+                         "seconds": round(t_asm + predicted, 3)}
+        log(f"CPU reference {leg}: {legs[leg]}")
+    # 1 thread: a sample
+    R.set_threads(1)
+    Av1, b1, t_asm1 = assemble()
+    ts = []
+    for it in (10, 30):
+        t1 = time.perf_counter()
+        R.cg(ip, ix, Av1, b1, M="ilu0", maxiter=it)
         ts.append(time.perf_counter() - t1)
-    t_iter = (ts[1] - ts[0]) / 200
-    t_setup = max(ts[0] - 100 * t_iter, 0.0)
-    t_solve = t_asm + t_setup + fp64_iters * t_iter
-    log(f"CPU reference ({cores} threads): assembly {t_asm:.2f} s, Jacobi-CG {1e3 * t_iter:.2f} ms per iteration "
-        f"-> {t_solve:.1f} s per solve of {fp64_iters} iterations")
-    return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": kind,
-            "sample": f"the full assembly ({t_asm:.2f} s) and 100 + 300 Jacobi-CG iterations "
-                      f"({1e3 * t_iter:.2f} ms each, set-up {t_setup:.2f} s), scaled to the {fp64_iters} iterations "
-                      f"the reference's cg_jacobi takes on this matrix to rtol {rtol:g} (bit-identical count on the GPU)",
-            "seconds_per_solve": round(t_solve, 3)}
+    t_it1 = (ts[1] - ts[0]) / 20
+    t_set1 = max(ts[0] - 10 * t_it1, 0.0)
+    del Av1, b1
+    R.set_threads(cores)
+    it_ilu = legs["ilu0_rcm_cg"]["iterations"]
+    one = {"cores": 1, "assembly_s": round(t_asm1, 3), "ilu0_factor_s": round(t_set1, 3),
+           "ilu0_rcm_cg_ms_per_iteration": round(1e3 * t_it1, 3),
+           "seconds": round(t_asm1 + t_set1 + it_ilu * t_it1, 3),
+           "sample": f"the assembly and 10 + 30 ILU(0)-RCM CG iterations at one thread, scaled to {it_ilu} iterations"}
+    log(f"CPU reference at 1 thread: assembly {t_asm1:.2f} s, ILU(0)-CG {1e3 * t_it1:.1f} ms per iteration "
+        f"-> {one['seconds']:.1f} s per solve")
+    best = min(legs, key=lambda k: legs[k]["seconds"])
+    full = all(legs[k]["measured"] for k in legs)
+    return ({"value": round(1.0 / legs[best]["seconds"], 6), "unit": "solves/s", "cores": cores, "kind": kind,
+             "sample": (f"full solves, measured: the assembly ({t_asm:.2f} s) and the reference's ILU(0)-RCM CG and "
+                        f"Jacobi-CG to rtol {rtol:g}; value is the faster, {best}" if full else
+                        f"the assembly ({t_asm:.2f} s) and the reference's ILU(0)-RCM CG and Jacobi-CG to rtol "
+                        f"{rtol:g}, those predicted beyond {limit:g} s from a sample of their iterations; value is "
+                        f"the faster, {best}"),
+             "seconds_per_solve": legs[best]["seconds"], "assembly_s": round(t_asm, 3), "legs": legs,
+             "one_thread": one}, xs)
 
 
 if __name__ == "__main__":

@@ -48,9 +48,91 @@ def kappa_field(seed: int = 1, nmodes: int = 16, wmax: int = 8):
     return F.exp(F.Const(0.5) * g), dict(c=c, w=w, phi=phi)
 
 
-def delaunay_square(h: float, seed: int = 0, jitter: float = 0.12):
-    """(points (n, 2), triangles (m, 3), min angle in degrees) of the unit square."""
+def _triangles(P, h):
+    """The Delaunay triangles of P (Qhull), without the slivers of collinear boundary nodes."""
     from scipy.spatial import Delaunay
+    tri = Delaunay(P).simplices.astype(np.int64)
+    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+    area = 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    return tri[np.abs(area) > 1e-12 * h * h]
+
+
+def _angles(P, tri):
+    """The three angles (degrees) of every triangle."""
+    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+
+    def at(u, v, w):
+        e1, e2 = v - u, w - u
+        cs = (e1 * e2).sum(1) / (np.linalg.norm(e1, axis=1) * np.linalg.norm(e2, axis=1))
+        return np.degrees(np.arccos(np.clip(cs, -1.0, 1.0)))
+    return np.stack([at(a, b, c), at(b, c, a), at(c, a, b)], 1)
+
+
+def _circumcentres(P, tri):
+    """Circumcentres (m, 2) and circumradii (m,) of triangles."""
+    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+    d = 2.0 * (a[:, 0] * (b[:, 1] - c[:, 1]) + b[:, 0] * (c[:, 1] - a[:, 1]) + c[:, 0] * (a[:, 1] - b[:, 1]))
+    sa, sb, sc = (a * a).sum(1), (b * b).sum(1), (c * c).sum(1)
+    cc = np.stack([(sa * (b[:, 1] - c[:, 1]) + sb * (c[:, 1] - a[:, 1]) + sc * (a[:, 1] - b[:, 1])) / d,
+                   (sa * (c[:, 0] - b[:, 0]) + sb * (a[:, 0] - c[:, 0]) + sc * (b[:, 0] - a[:, 0])) / d], 1)
+    return cc, np.linalg.norm(cc - a, axis=1)
+
+
+def _refine_square(P, h, min_angle, max_rounds=200):
+    """Delaunay refinement of a point set of the unit square whose boundary nodes lie on its sides
+    (Ruppert's scheme, which the reference's mesher follows: triangulate/refine.hpp).  Every round
+    each triangle with an angle below min_angle proposes its circumcentre -- or, when that point
+    lies outside the square or inside the diametral circle of a boundary segment, the segment's
+    midpoint -- and the proposals are inserted as an independent set: worst triangle first, a
+    proposal is dropped for this round when it lies within its own radius (circumradius, or half
+    the segment) of an accepted one or within an accepted one's radius of it.  The proposals of one
+    round then change disjoint cavities, as one-at-a-time insertion would.  Deterministic."""
+    from scipy.spatial import cKDTree
+    for _ in range(max_rounds):
+        tri = _triangles(P, h)
+        worst = _angles(P, tri).min(axis=1)
+        bad = np.flatnonzero(worst < min_angle)
+        if bad.size == 0:
+            return P
+        bad = bad[np.argsort(worst[bad], kind="stable")]
+        x, rad = _circumcentres(P, tri[bad])
+        split = np.full(bad.size, -1, dtype=np.int64)
+        # the boundary segments: consecutive boundary nodes along each side; a proposal that is
+        # outside the square or encroaches a segment becomes that segment's midpoint
+        for si, (axis, val) in enumerate(((1, 0.0), (0, 1.0), (1, 1.0), (0, 0.0))):
+            along = np.sort(P[np.abs(P[:, axis] - val) <= 1e-14, 1 - axis])
+            t = np.clip(x[:, 1 - axis], 0.0, 1.0)
+            j = np.clip(np.searchsorted(along, t, side="right") - 1, 0, along.size - 2)
+            lo, hi = along[j], along[j + 1]
+            mid = np.empty_like(x)
+            mid[:, axis], mid[:, 1 - axis] = val, 0.5 * (lo + hi)
+            outside = x[:, axis] < 0.0 if val == 0.0 else x[:, axis] > 1.0
+            hit = (split < 0) & (outside | (np.linalg.norm(x - mid, axis=1) < 0.5 * (hi - lo)))
+            split[hit] = si * (1 << 40) + j[hit]
+            x[hit], rad[hit] = mid[hit], 0.5 * (hi - lo)[hit]
+        # one proposal per split segment (its worst triangle's)
+        keep = np.ones(bad.size, dtype=bool)
+        _, first = np.unique(split[split >= 0], return_index=True)
+        keep[split >= 0] = False
+        keep[np.flatnonzero(split >= 0)[first]] = True
+        x, rad = x[keep], rad[keep]
+        tree = cKDTree(x)
+        blocked = np.zeros(x.shape[0], dtype=bool)
+        accepted = np.zeros(x.shape[0], dtype=bool)
+        near = tree.query_ball_point(x, rad)
+        for k in range(x.shape[0]):
+            if blocked[k] or accepted[near[k]].any():
+                continue
+            accepted[k] = True
+            blocked[near[k]] = True
+        P = np.concatenate([P, x[accepted]])
+    raise RuntimeError(f"mesh refinement did not reach {min_angle} degrees in {max_rounds} rounds")
+
+
+def delaunay_square(h: float, seed: int = 0, jitter: float = 0.12, min_angle_deg: float = None):
+    """(points (n, 2), triangles (m, 3), min angle in degrees) of the unit square: boundary nodes
+    every h, a seeded jittered lattice inside, triangulated by Qhull and, with min_angle_deg,
+    refined until no angle is smaller (Ruppert's scheme)."""
     rng = np.random.default_rng(seed)
     nb = max(2, int(round(1.0 / h)))
     t = np.linspace(0.0, 1.0, nb + 1)[:-1]
@@ -68,11 +150,9 @@ def delaunay_square(h: float, seed: int = 0, jitter: float = 0.12):
     P = P + rng.uniform(-jitter * h, jitter * h, P.shape)
     inside = (P[:, 0] > 0.45 * h) & (P[:, 0] < 1 - 0.45 * h) & (P[:, 1] > 0.45 * h) & (P[:, 1] < 1 - 0.45 * h)
     P = np.concatenate([bnd, P[inside]])
-    tri = Delaunay(P).simplices.astype(np.int64)
-    # drop slivers on the straight boundary (three collinear boundary nodes)
-    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
-    area = 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
-    tri = tri[np.abs(area) > 1e-12 * h * h]
+    if min_angle_deg:
+        P = _refine_square(P, h, float(min_angle_deg))
+    tri = _triangles(P, h)
     cen = P[tri].mean(axis=1)
     order = np.lexsort((cen[:, 0], np.floor(cen[:, 1] / (4 * h))))
     tri = tri[order]
@@ -93,13 +173,7 @@ def delaunay_square(h: float, seed: int = 0, jitter: float = 0.12):
 
 
 def min_angle(P, tri) -> float:
-    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
-
-    def ang(u, v, w):
-        e1, e2 = v - u, w - u
-        cs = (e1 * e2).sum(1) / (np.linalg.norm(e1, axis=1) * np.linalg.norm(e2, axis=1))
-        return np.degrees(np.arccos(np.clip(cs, -1, 1)))
-    return float(min(ang(a, b, c).min(), ang(b, c, a).min(), ang(c, a, b).min()))
+    return float(_angles(P, tri).min())
 
 
 def poisson_forms(V, f_expr, kappa=None, reaction=0.0):
@@ -115,14 +189,17 @@ def poisson_forms(V, f_expr, kappa=None, reaction=0.0):
 C1_H = 0.00265
 
 
-def p3_kappa_problem(h: float = C1_H, seed: int = 0):
-    """The P3 configuration: seeded Delaunay mesh of the unit square, P3 with every side Dirichlet,
-    a = kappa grad u . grad v dx, L = f v dx.  Returns (space, a, L, info)."""
+C1_MIN_ANGLE = 28.0
+
+
+def p3_kappa_problem(h: float = C1_H, seed: int = 0, min_angle_deg: float = C1_MIN_ANGLE):
+    """The P3 configuration: seeded Delaunay mesh of the unit square refined to min_angle_deg, P3
+    with every side Dirichlet, a = kappa grad u . grad v dx, L = f v dx.  Returns (space, a, L, info)."""
     import time
     from .mesh import mesh_from_arrays
     from .spaces import LagrangeSpace
     t0 = time.perf_counter()
-    P, T, ma = delaunay_square(h, seed=seed)
+    P, T, ma = delaunay_square(h, seed=seed, min_angle_deg=min_angle_deg)
     V = LagrangeSpace(mesh_from_arrays(P, T), 3, bc="dirichlet")
     kap, _ = kappa_field(1)
     f, _ = fourier_source(0)

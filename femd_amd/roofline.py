@@ -72,6 +72,20 @@ def spmv_times(A, reps=50, flush=False):
     return out[L.FP64], out[L.FP32]
 
 
+def precond_apply_time(M, n, reps=10):
+    """Average time (ms) of one application z = M^-1 r of a preconditioner on the device, from HIP
+    events on the library stream, r a seeded random vector of n values."""
+    lib = L.load()
+    rb, zb = _dev(lib, 8 * n), _dev(lib, 8 * n)
+    try:
+        r = np.random.default_rng(0).standard_normal(n)
+        L.check(lib.femd_memcpy_htod(rb, r.ctypes.data, r.nbytes))
+        return _timed(lib, lambda: L.check(lib.femd_precond_apply_dev(M._h, rb, zb, None)), reps, warm=2)
+    finally:
+        lib.femd_dev_free(rb)
+        lib.femd_dev_free(zb)
+
+
 def reordered(A):
     """A copy of the square matrix A in the inner CG's order of the unknowns (each unknown numbered
     where a row-by-row scan first meets it as a column, those it never meets last), for timing the

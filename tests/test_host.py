@@ -126,6 +126,34 @@ def test_mesh_arrays_vs_oracle(orc):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("h,seed", [(0.1, 0), (0.04, 3), (0.015, 7)])
+def test_refined_delaunay_square(h, seed):
+    """The quality mesh of the unit square: every angle at least 28 degrees, a conforming
+    triangulation that tiles the square (areas sum to 1, every interior edge shared by two
+    triangles, boundary edges only on the sides, one per boundary node), Delaunay (no vertex
+    strictly inside any circumcircle, by brute force), and reproducible from its seed."""
+    P, T, ma = problems.delaunay_square(h, seed=seed, min_angle_deg=28)
+    assert ma >= 28.0 and ma == problems.min_angle(P, T)
+    a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
+    area = 0.5 * np.abs((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    assert abs(area.sum() - 1.0) < 1e-12 and area.min() > 0
+    E = np.sort(np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]]), 1)
+    edges, count = np.unique(E, axis=0, return_counts=True)
+    assert count.max() == 2
+    side = lambda q: (np.abs(q[:, 0]) < 1e-14) | (np.abs(q[:, 0] - 1) < 1e-14) | (np.abs(q[:, 1]) < 1e-14) | (np.abs(q[:, 1] - 1) < 1e-14)
+    bnd = edges[count == 1]
+    assert side(P[bnd[:, 0]]).all() and side(P[bnd[:, 1]]).all() and bnd.shape[0] == side(P).sum()
+    assert np.unique(T).size == P.shape[0]
+    cc, rad = problems._circumcentres(P, T)
+    for lo in range(0, T.shape[0], 512):
+        d = np.linalg.norm(P[None, :, :] - cc[lo:lo + 512, None, :], axis=2)
+        assert (d >= rad[lo:lo + 512, None] * (1 - 1e-9)).all()
+    P2, T2, _ = problems.delaunay_square(h, seed=seed, min_angle_deg=28)
+    assert np.array_equal(P, P2) and np.array_equal(T, T2)
+    # the unrefined lattice of the same seed is worse: refinement did the work
+    assert problems.delaunay_square(h, seed=seed)[2] < 28.0
+
+
 def test_user_segments_and_markers(orc):
     pts = np.array([[0, 0], [1, 0], [1, 1], [0, 1], [0.5, 0.5]], dtype=float)
     tris = np.array([[0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4]])

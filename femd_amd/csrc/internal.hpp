@@ -253,6 +253,7 @@ struct femd_precond_s {
     // the planned one-workgroup sweeps (sweep.hpp); when planning fails, the level-set kernels above
     bool sweep = false;
     int nstepsF = 0, nstepsB = 0, nlevF = 0, nlevB = 0;
+    int zbytesF = 0, zbytesB = 0, rbytesF = 0, rbytesB = 0;   // the sweeps' LDS splits
     int64_t bytesF = 0, bytesB = 0, entriesLU = 0;
     femd_amd::DevBuf streamF, streamB, wbF, wbB, inposF;
     std::string sweep_note;                             // why the sweep was not planned, when it was not

@@ -89,8 +89,8 @@ void launch_ssor_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, in
                        const double *r, double *z, const int *stop, hipStream_t s);
 
 // ---- the planned one-workgroup sweeps (sweep.hip, sweep.hpp) --------------------------------------
-void launch_sweep(int kind, int nsteps, const void *stream, const int *wb, double omega, double *out, const int *stop,
-                  hipStream_t s);
+void launch_sweep(int kind, int nsteps, const void *stream, const int *wb, int zbytes, int rbytes, double omega, double *out,
+                  const int *stop, hipStream_t s);
 void launch_sweep_fill(int n, const int *in_pos, const int *perm, const double *src, double *stream, const int *stop,
                        hipStream_t s);
 void launch_sweep_pack(int64_t nnz, const int *fmap, const int *bmap, const double *v, double *fwd, double *bwd, hipStream_t s);

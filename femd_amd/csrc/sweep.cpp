@@ -90,11 +90,13 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
         }
     }
     P.nslots = next;
+    P.zbytes = (8 * next + SW_BLOCK - 1) / SW_BLOCK * SW_BLOCK;
+    P.rbytes = (SW_LDS - P.zbytes) / SW_BLOCK * SW_BLOCK;
 
     // step sizes (header and wave blocks)
     std::vector<int64_t> ssize(nsteps, SW_HDR);
     std::vector<int> emax(size_t(nsteps) * SW_CWAVES, 0), lanes(size_t(nsteps) * SW_CWAVES, 0);
-    std::vector<int64_t> tot(size_t(nsteps) * SW_CWAVES, 0);
+    std::vector<int64_t> t2(size_t(nsteps) * SW_CWAVES, 0), t8(size_t(nsteps) * SW_CWAVES, 0);
     int max_emax = 0;
     for (int s = 0; s < nsteps; ++s)
     {
@@ -102,66 +104,96 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
         for (int w = 0; w < nw; ++w)
         {
             int e = 0;
-            int64_t t = 0;
+            int64_t a2 = 0, a8 = 0;
             const int a0 = sptr[s] + 64 * w, a1 = std::min(a0 + 64, sptr[s + 1]);
-            for (int a = a0; a < a1; ++a) e = std::max(e, hi(rows[a]) - lo(rows[a])), t += hi(rows[a]) - lo(rows[a]);
+            for (int a = a0; a < a1; ++a)
+            {
+                const int c = hi(rows[a]) - lo(rows[a]);
+                e = std::max(e, c);
+                a2 += sw_npairs(c);
+                a8 += sw_noct(c);
+            }
+            if (sw_npre(e) > 64)
+            {
+                why = "a row with " + std::to_string(e) + " entries in the triangle";
+                return false;
+            }
             const size_t k = size_t(s) * SW_CWAVES + w;
             emax[k] = e;
-            tot[k] = t;
+            t2[k] = a2;
+            t8[k] = a8;
             lanes[k] = (a1 - a0 + 7) / 8 * 8;
             max_emax = std::max(max_emax, e);
-            ssize[s] += sw_wave_bytes(kind, lanes[k], e, t);
+            ssize[s] += sw_wave_bytes(kind, lanes[k], e, a2, a8);
+        }
+        // a lane's reads reach up to 63 units past its wave's block: a step stays two blocks short
+        // of the ring, so that every read lands inside it
+        if (ssize[s] + SW_HDR > P.rbytes - 2 * SW_BLOCK)
+        {
+            why = "a step of " + std::to_string(ssize[s]) + " bytes for a ring of " + std::to_string(P.rbytes);
+            return false;
         }
     }
     P.max_emax = max_emax;
 
-    // The producers' schedule, simulated: avail blocks are in the ring when a step begins (the
-    // prologue fills the ring); a step may begin when its bytes and the next step's header are
-    // there; during a step the producers store up to SW_PMAX more blocks, never past the ring's
-    // room (the step's own bytes stay).  When a step's bytes are not there yet, idle steps (a
-    // header with no waves) let the producers catch up: the sweep is then bound by what one
-    // workgroup can stream.
+    // The producers' schedule, simulated.  W(x): the blocks in the ring while executed step x runs
+    // (the prologue fills W(-1) = W(0)); during step x the producers store up to SW_PMAX more,
+    // never over a byte at or past step x's offset: W(x + 1) = min(W(x) + SW_PMAX, off(x) / BLOCK +
+    // ring blocks).  While step x runs the consumers read step x + 1 whole (its header, rows and
+    // entries, into registers), so step x + 1 may follow step x only if its bytes end within W(x).
+    // When they do not, idle steps (a header with no waves) let the producers catch up: the sweep
+    // is then bound by what one workgroup can stream.  The last step reads one more header (zero
+    // waves) past the end.
     std::vector<int64_t> off;            // every executed step's offset; real[k]: the real step run there
     std::vector<int> real, wb;
-    int64_t avail = SW_RING / SW_BLOCK, cur = 0;
+    const int64_t NB = P.rbytes / SW_BLOCK;
     auto blocks = [](int64_t bytes) { return (bytes + SW_BLOCK - 1) / SW_BLOCK; };
-    auto advance = [&](int64_t start) {
-        avail = std::min<int64_t>(avail + SW_PMAX, (start + SW_RING) / SW_BLOCK);
+    int64_t prev = NB, cur = 0;          // W of the last executed step (W(-1) before any)
+    auto place = [&](int r, int64_t size) {
+        const int64_t w = off.empty() ? prev : std::min<int64_t>(prev + SW_PMAX, off.back() / SW_BLOCK + NB);
+        off.push_back(cur);
+        real.push_back(r);
+        wb.push_back(int(w));
+        prev = w;
+        cur += size;
     };
-    for (int k = 0; k < nsteps; ++k)
+    for (int k = 0; k <= nsteps; ++k)
     {
-        while (blocks(cur + ssize[k] + SW_HDR) > avail)
+        // step k (or the closing header) is read during the step before it; a real step also
+        // leaves room for an idle step's header after it, so that one can always follow
+        const int64_t need = k < nsteps ? ssize[k] + SW_HDR : SW_HDR;
+        int stuck = 0;                   // idle steps in a row that let no block in
+        while (blocks(cur + need) > prev)
         {
-            // an idle step needs its header and the next one
-            if (blocks(cur + 2 * SW_HDR) > avail)
+            const int64_t before = prev;
+            if (blocks(cur + SW_HDR) > prev)
             {
                 why = "no room for an idle step before step " + std::to_string(k);
                 return false;
             }
-            const int64_t before = avail;
-            off.push_back(cur);
-            real.push_back(-1);
-            wb.push_back(int(avail));
-            advance(cur);
-            cur += SW_HDR;
-            if (avail <= before && blocks(cur + ssize[k] + SW_HDR) > avail)
+            place(-1, SW_HDR);
+            // an idle step moves the oldest byte in use by one header; the ring's room follows
+            // within a block's worth of them
+            stuck = prev > before ? 0 : stuck + 1;
+            if (stuck > SW_BLOCK / SW_HDR + 1)
             {
-                why = "step " + std::to_string(k) + " (" + std::to_string(ssize[k]) + " bytes) does not fit the ring";
+                why = "step " + std::to_string(k) + " (" + std::to_string(need) + " bytes) does not fit the ring";
                 return false;
             }
         }
-        off.push_back(cur);
-        real.push_back(k);
-        wb.push_back(int(avail));
-        advance(cur);
-        cur += ssize[k];
+        if (k < nsteps) place(k, ssize[k]);
     }
-    const int64_t total = cur + SW_HDR;                 // a last header's room, never read
+    const int64_t total = cur + SW_HDR;                 // the closing header (zero waves)
     const int64_t nblk = blocks(total);
     const int nexec = static_cast<int>(off.size());
     P.nsteps = nexec;
-    P.wb.assign(size_t(nexec) + 2 * SW_AHEAD + 2, int(std::min<int64_t>(avail, nblk)));
+    P.wb.assign(size_t(nexec) + 2 * SW_AHEAD + 2, int(nblk));
     for (int s = 0; s < nexec; ++s) P.wb[s] = int(std::min<int64_t>(wb[s], nblk));
+    {
+        // after the last step's stores nothing else is needed: the tail of wb is the stream's end
+        const int64_t w = nexec ? std::min<int64_t>(prev + SW_PMAX, off.back() / SW_BLOCK + NB) : prev;
+        P.wb[nexec] = int(std::min<int64_t>(w, nblk));
+    }
 
     // the stream, and a spare block past its end (a producer turn with no blocks loads it)
     P.stream.assign(size_t(nblk + 1) * SW_BLOCK, 0);
@@ -187,50 +219,59 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
         for (int w = 0; w < nw; ++w)
         {
             const size_t kw = size_t(s) * SW_CWAVES + w;
-            const int e = emax[kw], L = lanes[kw];
-            const int64_t T = tot[kw];
+            const int e = emax[kw], L = lanes[kw], np2 = sw_npairs(e), no8 = sw_noct(e);
             put<uint32_t>(S, o + 16 + 16 * w, uint32_t(wo));
             put<uint32_t>(S, o + 20 + 16 * w, uint32_t(e));
             put<uint32_t>(S, o + 24 + 16 * w, uint32_t(L));
-            put<uint32_t>(S, o + 28 + 16 * w, uint32_t(T));
-            const int64_t b = o + wo, po = b + sw_off_pre(kind, L), eo = b + sw_off_ev(kind, L, e),
-                          so = b + sw_off_es(kind, L, e, T);
+            put<uint32_t>(S, o + 28 + 16 * w, uint32_t(sw_npre(e)));
+            const int64_t b = o + wo, po = b + sw_off_pre(kind, L);
             const int a0 = sptr[s] + 64 * w, a1 = std::min(a0 + 64, sptr[s + 1]);
-            // the jagged-diagonal prefix: the rows are longest first
-            std::vector<int64_t> pre(size_t(e) + 1, 0);
-            for (int k = 0; k < e; ++k)
+            auto cnt = [&](int j) { return a0 + j < a1 ? hi(rows[a0 + j]) - lo(rows[a0 + j]) : 0; };
+            // the jagged-diagonal starts: the rows are longest first
+            std::vector<int64_t> pre(size_t(np2 + no8), 0);
+            int64_t u = sw_unit_pairs(kind, L, e);
+            for (int q = 0; q < np2; ++q)
             {
-                int64_t with = 0;
-                for (int a = a0; a < a1; ++a) with += hi(rows[a]) - lo(rows[a]) > k;
-                pre[k + 1] = pre[k] + with;
+                pre[q] = u;
+                for (int j = 0; j < L; ++j) u += cnt(j) > 2 * q;
             }
-            for (int k = 0; k <= e; ++k) put<uint32_t>(S, po + 4 * k, uint32_t(pre[k]));
+            for (int r = 0; r < no8; ++r)
+            {
+                pre[np2 + r] = u;
+                for (int j = 0; j < L; ++j) u += cnt(j) > 8 * r;
+            }
+            if (16 * u != sw_wave_bytes(kind, L, e, t2[kw], t8[kw]))
+            {
+                why = "wave block size mismatch";
+                return false;
+            }
+            for (int q = 0; q < np2 + no8; ++q) put<uint32_t>(S, po + 4 * q, uint32_t(pre[q]));
             for (int j = 0; j < L; ++j)
             {
                 const int a = a0 + j;
                 if (a < a1)
                 {
                     const int i = rows[a];
+                    const int c = hi(i) - lo(i);
                     put<int32_t>(S, b + sw_off_dest(L) + 4 * j, dest.empty() ? -1 : dest[i]);
-                    put<uint16_t>(S, b + sw_off_slot(L) + 2 * j, uint16_t(slot[i]));
-                    put<uint16_t>(S, b + sw_off_cnt(L) + 2 * j, uint16_t(hi(i) - lo(i)));
+                    put<uint32_t>(S, b + sw_off_sc(L) + 4 * j, uint32_t(slot[i]) | uint32_t(c) << 16);
                     P.in_pos[i] = (b + 8 * j) / 8;
                     if (sweep_has_diag(kind)) P.vmap[diag[i]] = (b + sw_off_diag(L) + 8 * j) / 8;
                     int k = 0;
                     for (int p = lo(i); p < hi(i); ++p, ++k)
                     {
-                        P.vmap[p] = (eo + 8 * (pre[k] + j)) / 8;
-                        put<uint16_t>(S, so + 2 * (pre[k] + j), uint16_t(slot[ix[p]]));
+                        P.vmap[p] = (b + 16 * (pre[k / 2] + j) + 8 * (k & 1)) / 8;
+                        put<uint16_t>(S, b + 16 * (pre[np2 + k / 8] + j) + 2 * (k & 7), uint16_t(slot[ix[p]]));
                     }
                 }
                 else
                 {
                     put<int32_t>(S, b + sw_off_dest(L) + 4 * j, -1);
-                    put<uint16_t>(S, b + sw_off_slot(L) + 2 * j, uint16_t(SW_TRASH));
+                    put<uint32_t>(S, b + sw_off_sc(L) + 4 * j, uint32_t(SW_TRASH));
                     if (sweep_has_diag(kind)) put<double>(S, b + sw_off_diag(L) + 8 * j, 1.0);
                 }
             }
-            wo += sw_wave_bytes(kind, L, e, T);
+            wo += sw_wave_bytes(kind, L, e, t2[kw], t8[kw]);
         }
     }
     for (int i = 0; i < n; ++i) P.entries += hi(i) - lo(i);
@@ -240,37 +281,54 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
 
 bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, double *out, int64_t nout, std::string &why)
 {
-    const int NB = SW_RING / SW_BLOCK;
+    const int64_t NB = P.rbytes / SW_BLOCK;
+    if (P.zbytes < 8 * P.nslots || P.zbytes + P.rbytes > SW_LDS || NB < 4)
+    {
+        why = "the plan's LDS split does not fit";
+        return false;
+    }
     std::vector<int64_t> held(NB, -1), stored_at(NB, -2);
     std::vector<double> z(SW_SLOTS, 0.0);
     const int kind = P.kind;
-    auto store = [&](int64_t b, int64_t step) { held[b % NB] = b; stored_at[b % NB] = step; };
     const int nsteps = P.nsteps;
     if (nsteps == 0) return true;
-    for (int64_t b = 0; b < P.wb[0]; ++b) store(b, -1);
     bool ok = true;
-    // a ring read of stream bytes [x, x + len) at step s
-    auto rd = [&](int64_t x, int len, int s) -> const uint8_t * {
+    auto u32 = [&](int64_t x) { uint32_t v; std::memcpy(&v, S.data() + x, 4); return v; };
+    auto i32 = [&](int64_t x) { int32_t v; std::memcpy(&v, S.data() + x, 4); return v; };
+    auto u16 = [&](int64_t x) { uint16_t v; std::memcpy(&v, S.data() + x, 2); return v; };
+    auto f64 = [&](int64_t x) { double v; std::memcpy(&v, S.data() + x, 8); return v; };
+    // every block of [x, x + len) is in the ring at step s, stored before it
+    auto present = [&](int64_t x, int64_t len, int s) {
         for (int64_t b = x / SW_BLOCK; b <= (x + len - 1) / SW_BLOCK; ++b)
             if (held[b % NB] != b || stored_at[b % NB] >= s)
             {
                 if (ok) why = "step " + std::to_string(s) + " reads block " + std::to_string(b) + " before it is stored";
                 ok = false;
             }
-        return S.data() + x;
     };
-    auto f64 = [&](int64_t x, int s) { double v; std::memcpy(&v, rd(x, 8, s), 8); return v; };
-    auto u32 = [&](int64_t x, int s) { uint32_t v; std::memcpy(&v, rd(x, 4, s), 4); return v; };
-    auto i32 = [&](int64_t x, int s) { int32_t v; std::memcpy(&v, rd(x, 4, s), 4); return v; };
-    auto u16 = [&](int64_t x, int s) { uint16_t v; std::memcpy(&v, rd(x, 2, s), 2); return v; };
+    for (int64_t b = 0; b < P.wb[0]; ++b) held[b % NB] = b, stored_at[b % NB] = -1;
+    // the prologue reads step 0 whole
+    present(0, u32(0) ? u32(0) : SW_HDR, 0);
     const double w = omega, c = (2.0 - w) / w;
     int64_t o = 0;
     for (int s = 0; s < nsteps && ok; ++s)
     {
-        for (int64_t b = P.wb[s]; b < P.wb[s + 1]; ++b) store(b, s);
-        const uint32_t size = u32(o, s), nw = u32(o + 4, s);
-        (void)rd(o + size, SW_HDR, s);            // the GPU reads the next step's header during this one
-        if (u32(o + 8, s) != uint32_t(P.wb[s + SW_AHEAD]) || u32(o + 12, s) != uint32_t(P.wb[s + SW_AHEAD + 1]))
+        for (int64_t b = P.wb[s]; b < P.wb[s + 1]; ++b)
+        {
+            // the block it replaces must lie wholly before this step's bytes
+            if ((b - NB + 1) * SW_BLOCK > o)
+            {
+                why = "step " + std::to_string(s) + " stores block " + std::to_string(b) + " over bytes still needed";
+                return false;
+            }
+            held[b % NB] = b;
+            stored_at[b % NB] = s;
+        }
+        const uint32_t size = u32(o), nw = u32(o + 4);
+        // during this step the consumers read the next step whole (the closing header after the last)
+        const uint32_t size2 = u32(o + size);
+        present(o + size, size2 ? size2 : SW_HDR, s);
+        if (u32(o + 8) != uint32_t(P.wb[s + SW_AHEAD]) || u32(o + 12) != uint32_t(P.wb[s + SW_AHEAD + 1]))
         {
             why = "step " + std::to_string(s) + ": the header's block range differs from the schedule";
             return false;
@@ -279,33 +337,32 @@ bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, do
         std::vector<std::pair<int, double>> written;
         for (uint32_t wv = 0; wv < nw; ++wv)
         {
-            const int e = int(u32(o + 20 + 16 * wv, s)), L = int(u32(o + 24 + 16 * wv, s));
-            const int64_t b = o + u32(o + 16 + 16 * wv, s), po = b + sw_off_pre(kind, L);
-            const int64_t T = u32(o + 28 + 16 * wv, s);
-            if (T != u32(po + 4 * e, s)) { why = "inconsistent header"; return false; }
-            const int64_t eo = b + sw_off_ev(kind, L, e), so = b + sw_off_es(kind, L, e, T);
+            const int e = int(u32(o + 20 + 16 * wv)), L = int(u32(o + 24 + 16 * wv));
+            const int np2 = sw_npairs(e);
+            const int64_t b = o + u32(o + 16 + 16 * wv), po = b + sw_off_pre(kind, L);
+            if (int(u32(o + 28 + 16 * wv)) != sw_npre(e)) { why = "inconsistent header"; return false; }
             for (int j = 0; j < L; ++j)
             {
-                double acc = f64(b + 8 * j, s);
-                const int cnt = u16(b + sw_off_cnt(L) + 2 * j, s);
+                double acc = f64(b + 8 * j);
+                const uint32_t sc = u32(b + sw_off_sc(L) + 4 * j);
+                const int cnt = int(sc >> 16);
                 for (int k = 0; k < cnt; ++k)
                 {
-                    const int64_t at = u32(po + 4 * k, s) + j;
-                    const double v = f64(eo + 8 * at, s);
-                    const int sl = u16(so + 2 * at, s);
+                    const double v = f64(b + 16 * (int64_t(u32(po + 4 * (k / 2))) + j) + 8 * (k & 1));
+                    const int sl = u16(b + 16 * (int64_t(u32(po + 4 * (np2 + k / 8))) + j) + 2 * (k & 7));
                     const double prod = v * z[sl];
                     acc = acc - prod;
                 }
                 double zv = acc, yv = 0.0;
-                if (kind == SW_ILU_BWD) zv = acc / f64(b + sw_off_diag(L) + 8 * j, s);
+                if (kind == SW_ILU_BWD) zv = acc / f64(b + sw_off_diag(L) + 8 * j);
                 else if (kind == SW_SSOR_FWD || kind == SW_SSOR_BWD)
                 {
-                    const double d = f64(b + sw_off_diag(L) + 8 * j, s);
+                    const double d = f64(b + sw_off_diag(L) + 8 * j);
                     zv = acc * w / d;
                     if (kind == SW_SSOR_FWD) yv = zv * (c * d / w);
                 }
-                written.emplace_back(u16(b + sw_off_slot(L) + 2 * j, s), zv);
-                const int32_t dst = i32(b + sw_off_dest(L) + 4 * j, s);
+                written.emplace_back(int(sc & 0xffffu), zv);
+                const int32_t dst = i32(b + sw_off_dest(L) + 4 * j);
                 if (dst >= 0)
                 {
                     if (dst >= nout) { why = "destination out of range"; return false; }

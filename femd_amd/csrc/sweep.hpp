@@ -9,33 +9,39 @@
 // the schedule.
 //
 // The z values live in LDS slots: a row's slot is taken at its step and given back after the last
-// step that reads it (on C1's RCM order at most ~5.5k values are live at once).  Everything a step
+// step that reads it (on C1's RCM order at most ~5.7k values are live at once); the LDS the slots
+// leave is the ring.  Everything a step
 // reads besides those slots -- its header, the rows' records and their entries (value, slot of the
 // column) -- is one contiguous piece of a static byte stream in HBM, which producer waves copy into
-// an LDS ring ahead of use while the consumer waves compute:
+// an LDS ring ahead of use while the consumer waves compute.  A consumer wave reads the next
+// step's piece into registers during the current step, so that after a barrier it only gathers the
+// z values its rows need and sums:
 //
 //   step:  header (SW_HDR bytes): u32 size (bytes of the step), u32 waves, u32 wb[s + 4],
 //          u32 wb[s + 5] (the blocks the producer of step s loads for step s + 4), then per wave w a
-//          record of four u32 (woff, emax, L, T); then one block per consumer wave with rows, at
+//          record of four u32 (woff, emax, L, npre); then one block per consumer wave with rows, at
 //          the step's offset + woff, for its L lanes (its rows, longest first, rounded up to a
-//          multiple of 8):
+//          multiple of 8), every section a multiple of 16 bytes:
 //            f64 vin[L]    the row's input: r (forward, filled per application), y (backward,
 //                          written by the forward sweep)
 //            i32 dest[L]   where the result goes: forward -- the backward stream's vin field of the
 //                          row (index in doubles), backward -- the output vector's index; -1: none
-//            u16 slot[L]   the row's own LDS slot (SW_TRASH when no later row reads it)
-//            u16 cnt[L]    the row's entries
+//            u32 sc[L]     the row's own LDS slot (SW_TRASH when no later row reads it) | its
+//                          entry count << 16
 //            f64 diag[L]   the row's diagonal (kinds that divide by it)
-//            u32 pre[emax + 1], padded to a multiple of 8: pre[e] = the rows with more than e'
-//                          entries, summed over e' < e (a jagged-diagonal layout: entry e of every
-//                          row that has one, then entry e + 1, ...)
-//            f64 ev[T]     entry e of lane j at ev[pre[e] + j] (T = pre[emax], rounded up to even)
-//            u16 es[T]     the LDS slot of that entry's column, at es[pre[e] + j] (padded to 16 B)
+//            u32 pre[npre] padded to a multiple of 4: where (in 16-byte units from the block's start)
+//                          pair q of the entries starts (q < npairs = ceil(emax / 2)), then where
+//                          slot octet r starts (r < ceil(emax / 8))
+//            pairs         entries 2q and 2q + 1 of lane j as two f64 at pre[q] + j (only the
+//                          rows with more than 2q entries: a jagged-diagonal layout by pairs)
+//            octets        the LDS slots of entries 8r .. 8r + 7 of lane j as eight u16 at
+//                          pre[npairs + r] + j (rows with more than 8r entries)
 //
 // Producer wave p handles the steps s with s % 4 == p: at step s it stores the blocks it loaded
 // four steps earlier -- blocks [wb[s], wb[s+1]) of SW_BLOCK bytes -- into the ring and loads those
-// of step s + 4, whose range step s's header holds (the first four ranges come from wb in HBM).  The plan places wb so that every step's bytes and the next step's header are in
-// the ring before the step, and no store overwrites bytes not yet read.
+// of step s + 4, whose range step s's header holds (the first four ranges come from wb in HBM).
+// The plan places wb so that, while step s runs, the ring holds step s's bytes and all of step
+// s + 1's, and no store overwrites bytes the step may still read.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -48,8 +54,8 @@ constexpr int SW_PWAVES = 4;                   // producer waves
 constexpr int SW_THREADS = 64 * (SW_CWAVES + SW_PWAVES);
 constexpr int SW_ROWS = 64 * SW_CWAVES;        // rows per step at most
 constexpr int SW_BLOCK = 1024;                 // bytes per producer block (64 lanes x 16 B)
-constexpr int SW_RING = 65536;                 // LDS ring bytes (a power of two)
-constexpr int SW_SLOTS = 8192;                 // LDS z slots
+constexpr int SW_LDS = 152 * 1024;             // LDS bytes: the z slots, then the ring
+constexpr int SW_SLOTS = 8192;                 // LDS z slots at most
 constexpr int SW_ZERO = 0, SW_TRASH = 1;       // slot 0 holds 0.0; slot 1 takes values nobody reads
 constexpr int SW_PMAX = 20;                    // blocks a producer wave carries per turn
 constexpr int SW_HDR = 144;
@@ -57,16 +63,25 @@ constexpr int SW_AHEAD = SW_PWAVES;            // steps between a block's load a
 
 enum { SW_ILU_FWD = 0, SW_ILU_BWD = 1, SW_SSOR_FWD = 2, SW_SSOR_BWD = 3 };
 
+constexpr int SW_EREG = 24;                    // entries per row a consumer keeps in registers
+constexpr int SW_OREG = SW_EREG / 8;           // slot octets in registers
+
 constexpr bool sweep_has_diag(int kind) { return kind != SW_ILU_FWD; }
+constexpr int sw_npairs(int emax) { return (emax + 1) / 2; }
+constexpr int sw_noct(int emax) { return (emax + 7) / 8; }
+constexpr int sw_npre(int emax) { return sw_npairs(emax) + sw_noct(emax); }
 // byte offsets inside a wave block of L lanes
 constexpr int64_t sw_off_dest(int L) { return 8 * int64_t(L); }
-constexpr int64_t sw_off_slot(int L) { return 12 * int64_t(L); }
-constexpr int64_t sw_off_cnt(int L) { return 14 * int64_t(L); }
+constexpr int64_t sw_off_sc(int L) { return 12 * int64_t(L); }
 constexpr int64_t sw_off_diag(int L) { return 16 * int64_t(L); }
 constexpr int64_t sw_off_pre(int kind, int L) { return (sweep_has_diag(kind) ? 24 : 16) * int64_t(L); }
-constexpr int64_t sw_off_ev(int kind, int L, int emax) { return sw_off_pre(kind, L) + 4 * ((int64_t(emax) + 8) / 8 * 8); }
-constexpr int64_t sw_off_es(int kind, int L, int emax, int64_t T) { return sw_off_ev(kind, L, emax) + 8 * ((T + 1) / 2 * 2); }
-constexpr int64_t sw_wave_bytes(int kind, int L, int emax, int64_t T) { return sw_off_es(kind, L, emax, T) + (2 * T + 15) / 16 * 16; }
+// the first pair (16-byte units from the block's start)
+constexpr int64_t sw_unit_pairs(int kind, int L, int emax) { return (sw_off_pre(kind, L) + 4 * ((sw_npre(emax) + 3) / 4 * 4)) / 16; }
+// a wave block with T2 pair units and T8 octet units
+constexpr int64_t sw_wave_bytes(int kind, int L, int emax, int64_t T2, int64_t T8)
+{
+    return 16 * (sw_unit_pairs(kind, L, emax) + T2 + T8);
+}
 
 struct SweepPlan {
     int kind = 0;
@@ -76,6 +91,8 @@ struct SweepPlan {
     int nlevels = 0;
     int nslots = 0;                  // slots used (incl. SW_ZERO, SW_TRASH)
     int max_emax = 0;
+    int zbytes = 0;                  // LDS bytes of the z slots (whole blocks)
+    int rbytes = 0;                  // LDS bytes of the ring (whole blocks): the rest
     std::vector<uint8_t> stream;     // the static bytes (values, vin: zero), padded to whole blocks,
                                      // and one spare block
     std::vector<int> wb;             // nsteps + 2 * SW_AHEAD + 2 entries: blocks in the ring before step s

@@ -228,3 +228,18 @@ def heat_problem(N: int = 256, k: int = 2, dt: float = 1e-3, seed: int = 0):
     _, info = fourier_source(seed)
     u0 = V.interpolate(lambda xs, ys: fourier_values(info, xs, ys))
     return V, M, A, u0
+
+
+def convection_problem(N: int = 256, k: int = 2, eps: float = 0.01, beta=(1.0, 0.5), seed: int = 0):
+    """A nonsymmetric system for GMRES / LGMRES: -eps Laplace u + beta . grad u = f on an N x N
+    structured triangulation, P_k with every side Dirichlet, f the seeded random-Fourier series.
+    a = (eps grad u . grad v + (beta . grad u) v) dx, L = f v dx.  Returns (space, a, L)."""
+    from .mesh import rectangle_mesh
+    from .spaces import LagrangeSpace
+    V = LagrangeSpace(rectangle_mesh(0, 0, 1, 1, N, N), k, bc="dirichlet")
+    u, v = F.TrialFunction(V), F.TestFunction(V)
+    b = F.as_vector([F.Const(float(beta[0])), F.Const(float(beta[1]))])
+    a = F.form((F.Const(float(eps)) * F.dot(F.grad(u), F.grad(v)) + F.dot(b, F.grad(u)) * v) * F.dx)
+    f, _ = fourier_source(seed)
+    return V, a, F.form(f * v * F.dx)
+

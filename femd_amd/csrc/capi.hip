@@ -338,10 +338,9 @@ void precond_apply(femd_precond_s *M, const double *r, double *z, const int *sto
         // the backward stream, the backward sweep writes z (scattered back through the order)
         const int ilu = M->kind == FEMD_PC_ILU0;
         launch_sweep_fill(n, M->inposF.as<int>(), perm ? M->perm.as<int>() : nullptr, r, M->streamF.as<double>(), stop, s);
-        launch_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, M->nstepsF, M->streamF.p, M->wbF.as<int>(), M->zbytesF, M->rbytesF,
-                     M->omega, M->streamB.as<double>(), stop, s);
-        launch_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, M->nstepsB, M->streamB.p, M->wbB.as<int>(), M->zbytesB, M->rbytesB,
-                     M->omega, z, stop, s);
+        launch_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, M->nstepsF, M->streamF.p, M->firstF.p, M->zunitF, M->omega,
+                     M->streamB.as<double>(), stop, s);
+        launch_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, M->nstepsB, M->streamB.p, M->firstB.p, M->zunitB, M->omega, z, stop, s);
         return;
     }
     const double *rin = r;
@@ -563,8 +562,8 @@ void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vect
     for (int i = 0; i < n; ++i) inpos[i] = int(F.in_pos[i]);
     M->streamF = upload(F.stream, s);
     M->streamB = upload(B.stream, s);
-    M->wbF = upload(F.wb, s);
-    M->wbB = upload(B.wb, s);
+    M->firstF = upload(F.first, s);
+    M->firstB = upload(B.first, s);
     M->inposF = upload(inpos, s);
     {
         DevBuf df = upload(fmap, s), db = upload(bmap, s);
@@ -573,10 +572,8 @@ void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vect
     }
     M->nstepsF = F.nsteps;
     M->nstepsB = B.nsteps;
-    M->zbytesF = F.zbytes;
-    M->zbytesB = B.zbytes;
-    M->rbytesF = F.rbytes;
-    M->rbytesB = B.rbytes;
+    M->zunitF = F.zunit;
+    M->zunitB = B.zunit;
     M->nlevF = F.nlevels;
     M->nlevB = B.nlevels;
     M->bytesF = int64_t(F.stream.size());

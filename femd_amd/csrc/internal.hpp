@@ -253,9 +253,9 @@ struct femd_precond_s {
     // the planned one-workgroup sweeps (sweep.hpp); when planning fails, the level-set kernels above
     bool sweep = false;
     int nstepsF = 0, nstepsB = 0, nlevF = 0, nlevB = 0;
-    int zbytesF = 0, zbytesB = 0, rbytesF = 0, rbytesB = 0;   // the sweeps' LDS splits
+    uint32_t zunitF = 0, zunitB = 0;     // the streams' zeros
     int64_t bytesF = 0, bytesB = 0, entriesLU = 0;
-    femd_amd::DevBuf streamF, streamB, wbF, wbB, inposF;
+    femd_amd::DevBuf streamF, streamB, firstF, firstB, inposF;
     std::string sweep_note;                             // why the sweep was not planned, when it was not
 };
 

@@ -89,7 +89,7 @@ void launch_ssor_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, in
                        const double *r, double *z, const int *stop, hipStream_t s);
 
 // ---- the planned one-workgroup sweeps (sweep.hip, sweep.hpp) --------------------------------------
-void launch_sweep(int kind, int nsteps, const void *stream, const int *wb, int zbytes, int rbytes, double omega, double *out,
+void launch_sweep(int kind, int nsteps, const void *stream, const void *first, uint32_t zunit, double omega, double *out,
                   const int *stop, hipStream_t s);
 void launch_sweep_fill(int n, const int *in_pos, const int *perm, const double *src, double *stream, const int *stop,
                        hipStream_t s);

@@ -1,20 +1,14 @@
 // sweep.hip -- the triangular sweeps of ILU(0) and SSOR (sparse/precond.hpp:125-144, 189-205) as one
-// workgroup's program over a planned stream (sweep.hpp): eight consumer waves compute the rows of a
-// step and store their results; four producer waves keep an LDS ring filled from HBM, each
-// carrying the blocks of every fourth step in registers for four steps.
-//
-// A consumer wave holds the next step's rows in registers before the step begins -- each lane its
-// row's input, destination, slot, count, diagonal, up to SW_EREG values and their slots, read from
-// the ring during the step before -- so that after a barrier a step costs one round of LDS gathers
-// of z values, the products subtracted in stored order, and the stores of its results; the reads
-// of the step after it are issued while those products run.  Rows longer than SW_EREG read their
-// remaining entries from the ring during their own step.
+// workgroup's program over a planned stream (sweep.hpp).  The workgroup's twelve waves are four teams
+// of three; team s % 4 runs step s.  Each wave holds its team's coming step in registers -- per lane
+// its row's input, destination, slot, count, diagonal, the first SW_EREG entry values and the slots
+// of their columns -- loaded from HBM during the step after its team's last one, three steps ahead
+// of use, so that no data but the z values passes through the LDS: after a barrier the team gathers
+// the z values its rows need (one round of LDS reads), subtracts the products in stored order,
+// and stores its results.  Rows longer than SW_EREG load their remaining entries when they run.
 //
 // The steps are separated by a bare s_barrier after the LDS traffic has drained: a __syncthreads
-// would also wait for the producers' loads in flight, which are meant to stay in flight across
-// four barriers.  Consumer waves issue no global loads and producer waves no global stores, so a
-// producer's wait before its ring stores covers only its own loads of four steps before, and the
-// consumers' stores never hold anybody up.
+// would also wait for the loads in flight, which are meant to stay in flight across three barriers.
 
 #include "sweep.hpp"
 
@@ -29,200 +23,170 @@ namespace {
 // 16 bytes as the compiler's own vector type (kept in registers, unlike HIP's uint4 struct)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// One wave's part of a sweep for its team: after team barriers (the steps of the teams before
+// it), the team's steps in groups of SW_TEAMS, straight-line within a group (compute, load the next
+// block, wait, wait), so that the registers it loads are the ones it later reads, with no copy
+// between them that would wait for a load in flight.
+//
 // KIND: SW_ILU_FWD (z = r - sum L z), SW_ILU_BWD (z = (y - sum U z) / d), SW_SSOR_FWD (z = (r - sum L z) w / d,
 // the row's output scaled by (c d) / w for the backward sweep), SW_SSOR_BWD (z = (y - sum U z) w / d).
 // out: where the results go (the backward stream as doubles for a forward sweep, the output vector for a
-// backward one).  The LDS holds the z slots (zbytes) and then the ring (rbytes, whole blocks); ring
-// offsets are kept below rbytes, and an offset at most one ring past it is brought back by wrap().
+// backward one).
 template <int KIND>
-__global__ __launch_bounds__(SW_THREADS) void k_sweep(int nsteps, const u32x4 *__restrict__ stream, const int *__restrict__ wb,
-                                                      uint32_t zbytes, uint32_t rbytes, double w, double c, double *out,
-                                                      const int *stop)
+__device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__restrict__ stream, uint32_t zunit, u32x4 rec,
+                                           double w, double c, double *out, const uint8_t *zsb, double *zs, int lane)
 {
-    __shared__ u32x4 lds4[SW_LDS / 16];
-    if (stop && *stop) return;
-    double *zs = reinterpret_cast<double *>(lds4);
-    uint8_t *ring = reinterpret_cast<uint8_t *>(lds4) + zbytes;
-    const uint32_t R = rbytes, NB = rbytes / SW_BLOCK;
-    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
-    const bool producer = wv >= SW_CWAVES;
-    const int p = wv - SW_CWAVES;
-    auto wrap = [R](uint32_t at) { return min(at, at - R); };
-    auto f64 = [&](uint32_t at) { return *reinterpret_cast<const double *>(ring + wrap(at)); };
-    auto u32 = [&](uint32_t at) { return *reinterpret_cast<const uint32_t *>(ring + wrap(at)); };
-    auto v4 = [&](uint32_t at) { return *reinterpret_cast<const u32x4 *>(ring + wrap(at)); };
-
-    if (t == 0) zs[SW_ZERO] = 0.0;
-    const int b0 = wb[0];
-    for (int b = wv; b < b0; b += SW_CWAVES + SW_PWAVES)
-        reinterpret_cast<u32x4 *>(ring + uint32_t(b) * SW_BLOCK)[lane] = stream[int64_t(b) * (SW_BLOCK / 16) + lane];
-    __syncthreads();
-
-    if (producer)
-    {
-        // A turn's loads are straight-line code, every one issued (past the turn's blocks it loads
-        // its last one again; the stream ends with a spare block, so a turn with none loads that):
-        // a load under a branch of its own makes the compiler wait for the loads before it.
-        u32x4 buf[SW_PMAX];
-        auto load_turn = [&](int first, int count) {
-            const int last = count > 0 ? count - 1 : 0;
-            const u32x4 *src = stream + int64_t(first) * (SW_BLOCK / 16) + lane;
-#pragma unroll
-            for (int k = 0; k < SW_PMAX; ++k) buf[k] = src[int64_t(k < last ? k : last) * (SW_BLOCK / 16)];
-        };
-        int bfirst = wb[p], nbuf = wb[p + 1] - wb[p];
-        load_turn(bfirst, nbuf);
-        uint32_t o = 0;
-        for (int s = 0; s < nsteps; ++s)
-        {
-            if ((s & (SW_PWAVES - 1)) == p)
-            {
-                uint32_t at = (uint32_t(bfirst) % NB) * SW_BLOCK;
-#pragma unroll
-                for (int k = 0; k < SW_PMAX; ++k)
-                {
-                    if (k < nbuf) reinterpret_cast<u32x4 *>(ring + at)[lane] = buf[k];
-                    at = wrap(at + SW_BLOCK);
-                }
-                // the blocks of step s + 4, named by this step's header
-                bfirst = int(u32(o + 8));
-                nbuf = int(u32(o + 12)) - bfirst;
-                load_turn(bfirst, nbuf);
-            }
-            o = wrap(o + u32(o));
-            // LDS settled, global loads left in flight
-            __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        }
-        return;
-    }
-
-    // The consumer's registers for a step: the step's ring offset, size and waves; this wave's
-    // block, entry bound and lanes; per lane the row's input, destination, slot | count << 16,
-    // diagonal, the pre table (lane q holds pre[q]), the first SW_EREG values (pairs) and their
-    // slots (octets).  Producers and consumers run loops of their own, one barrier per step in
-    // each, so that the registers one role carries across steps are not held by the other.
-    uint32_t o = 0, size = 0, nw = 0, blk = 0, e = 0, L = 0, sc = 0, pv = 0;
+    // this wave's block of its team's coming step: where it is, its bound and lanes, the record of
+    // the block after it, and per lane the row's fields and first entries
+    uint32_t base = 0, e = 0, L = 0, sc = 0;
     int dst = -1;
     double vin = 0.0, d = 1.0;
     u32x4 vq[SW_EREG / 2], sq[SW_OREG];
-    // a step's header, then this wave's rows, then its first entries: three rounds of LDS reads
-    auto read_header = [&](uint32_t at, uint32_t &sz, uint32_t &nwv, u32x4 &rec) {
-        sz = u32(at);
-        nwv = u32(at + 4);
-        rec = v4(at + 16 + 16 * wv);
-    };
-    auto read_rows = [&](uint32_t at, const u32x4 &rec) {
-        blk = wrap(at + rec.x);
-        e = rec.y;
-        L = rec.z;
-        const uint32_t lq = uint32_t(lane) < L ? uint32_t(lane) : 0u;
-        vin = f64(blk + 8 * lq);
-        dst = int(u32(blk + uint32_t(sw_off_dest(int(L))) + 4 * lq));
-        sc = u32(blk + uint32_t(sw_off_sc(int(L))) + 4 * lq);
-        if (sweep_has_diag(KIND)) d = f64(blk + uint32_t(sw_off_diag(int(L))) + 8 * lq);
-        pv = u32(blk + uint32_t(sw_off_pre(KIND, int(L))) + 4 * (uint32_t(lane) < rec.w ? uint32_t(lane) : 0u));
-    };
-    auto pair_at = [&](int q) { return blk + 16 * (uint32_t(__builtin_amdgcn_readlane(int(pv), q)) + uint32_t(lane)); };
-    auto oct_at = [&](int r) {
-        const int np2 = int(e + 1) >> 1;
-        return blk + 16 * (uint32_t(__builtin_amdgcn_readlane(int(pv), np2 + r)) + uint32_t(lane));
-    };
-    // the first SW_EREG entries of this wave's rows (every read issued: past a row's end, or past
-    // the wave's block, a lane reads bytes it never uses, all of them in the ring)
-    auto read_entries = [&]() {
+    const uint32_t l4 = 4 * uint32_t(lane), l8 = 8 * uint32_t(lane), l16 = 16 * uint32_t(lane);
+    const uint8_t *zero = reinterpret_cast<const uint8_t *>(stream + zunit);
+    // Every lane loads its places as if it had a row (past the wave's rows it reads bytes it never
+    // uses, all inside the stream), pairs and octets past the block's bound from the zeros at the
+    // stream's end: each load is a scalar address plus the lane's offset, with no branch and no
+    // per-lane arithmetic.  The record is read into scalars before the loads replace it.
+    auto load_block = [&]() {
+        base = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.x)));
+        e = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.y)));
+        L = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.z)));
+        const uint8_t *b = reinterpret_cast<const uint8_t *>(stream + base);
+        // the record through a vector load (an offset the compiler cannot see is zero): a scalar
+        // load would be waited for at the barrier, a full memory latency every step
+        uint32_t zv;
+        __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zv));
+        rec = *reinterpret_cast<const u32x4 *>(b + zv);
+        vin = *reinterpret_cast<const double *>(b + sw_b_vin() + l8);
+        dst = *reinterpret_cast<const int *>(b + sw_b_dest(int(L)) + l4);
+        sc = *reinterpret_cast<const uint32_t *>(b + sw_b_sc(int(L)) + l4);
+        if (sweep_has_diag(KIND)) d = *reinterpret_cast<const double *>(b + sw_b_diag(int(L)) + l8);
+        const uint32_t np = (e + 1) >> 1, no = (e + 7) >> 3;
+        const uint8_t *pairs = b + 16 * sw_u_pairs(KIND, int(L)), *octs = pairs + 16 * np * L;
 #pragma unroll
-        for (int q = 0; q < SW_EREG / 2; ++q) vq[q] = v4(pair_at(q));
-#pragma unroll
-        for (int r = 0; r < SW_OREG; ++r) sq[r] = v4(oct_at(r));
-    };
-    {
-        u32x4 rec;
-        read_header(0, size, nw, rec);
-        if (wv < int(nw))
+        for (int q = 0; q < SW_EREG / 2; ++q)
         {
-            read_rows(0, rec);
-            read_entries();
+            const uint8_t *at = uint32_t(q) < np ? pairs + 16 * uint32_t(q) * L : zero;
+            vq[q] = *reinterpret_cast<const u32x4 *>(at + l16);
         }
-    }
-    for (int s = 0; s < nsteps; ++s)
-    {
-        // the next step's header now (it is in the ring); its rows once this step's are done
-        const uint32_t o2 = wrap(o + size);
-        uint32_t size2, nw2;
-        u32x4 rec2;
-        read_header(o2, size2, nw2, rec2);
-        if (wv < int(nw))
+#pragma unroll
+        for (int r = 0; r < SW_OREG; ++r)
         {
-            const int cnt = int(sc >> 16);
-            double acc = vin;
-            double zz[SW_EREG];
-            // every gather of the registered entries at once (past its row's end a lane reads slot
-            // SW_ZERO, whose 0.0 it never uses)
+            const uint8_t *at = uint32_t(r) < no ? octs + 16 * uint32_t(r) * L : zero;
+            sq[r] = *reinterpret_cast<const u32x4 *>(at + l16);
+        }
+    };
+    // The same instructions for a wave without rows (its registers hold zeros then, its lanes
+    // store nothing): a branch around them would leave the compiler unsure whether the loads are
+    // done, and it would then wait for the next block's loads in the middle of issuing them.
+    auto compute = [&]() {
+        // the loaded registers are first touched here: without this the compiler moves work on
+        // them (sc & 0xffff, say) up into the load step, where it waits for the loads it issued
+        __asm__ volatile("" : "+v"(sc), "+v"(dst), "+v"(vin), "+v"(d));
 #pragma unroll
-            for (int k = 0; k < SW_EREG; ++k)
-            {
-                const uint32_t raw = (sq[k >> 3][(k >> 1) & 3] >> (16 * (k & 1))) & (SW_SLOTS - 1);
-                zz[k] = zs[k < cnt ? raw : uint32_t(SW_ZERO)];
-            }
-            // keep the gathers together, ahead of the products: one round trip for all of them
-            __builtin_amdgcn_sched_barrier(0);
-            // the products in stored order: the reference's s -= v[p] * z[ix[p]], one rounding for
-            // the product and one for the difference; lanes past their row's end keep acc
+        for (int r = 0; r < SW_OREG; ++r) __asm__ volatile("" : "+v"(sq[r]));
 #pragma unroll
-            for (int k = 0; k < SW_EREG; ++k)
+        for (int q = 0; q < SW_EREG / 2; ++q) __asm__ volatile("" : "+v"(vq[q]));
+        double acc = vin;
+        double zz[SW_EREG];
+        // every gather of the registered entries at once: the octets hold the z values' LDS byte
+        // offsets (a padded entry's is slot SW_ZERO's, +0.0)
+#pragma unroll
+        for (int k = 0; k < SW_EREG; ++k)
+        {
+            const uint32_t word = sq[k >> 3][(k >> 1) & 3];
+            const uint32_t off = (k & 1) ? word >> 16 : word & 0xffffu;
+            zz[k] = *reinterpret_cast<const double *>(zsb + off);
+        }
+        // keep the gathers together, ahead of the products: one round trip for all of them
+        __builtin_amdgcn_sched_barrier(0);
+        // the products in stored order: the reference's s -= v[p] * z[ix[p]], one rounding for the
+        // product and one for the difference; a padded entry subtracts +0.0, which leaves acc as it is
+#pragma unroll
+        for (int k = 0; k < SW_EREG; ++k)
+        {
+            const u32x4 pq = vq[k >> 1];
+            const uint32_t lo = (k & 1) ? pq.z : pq.x, hi = (k & 1) ? pq.w : pq.y;
+            const double prod = __hiloint2double(int(hi), int(lo)) * zz[k];
+            acc = acc - prod;
+        }
+        // entries past SW_EREG (rare): loaded now, padded like the first ones up to the block's bound
+        if (e > uint32_t(SW_EREG))
+        {
+            const uint32_t np = (e + 1) >> 1;
+            const uint8_t *pairs = reinterpret_cast<const uint8_t *>(stream + base) + 16 * sw_u_pairs(KIND, int(L)),
+                          *octs = pairs + 16 * np * L;
+            for (int k0 = SW_EREG; k0 < int(e); k0 += 8)
             {
-                const u32x4 pq = vq[k >> 1];
-                const uint32_t lo = (k & 1) ? pq.z : pq.x, hi = (k & 1) ? pq.w : pq.y;
-                const double prod = __hiloint2double(int(hi), int(lo)) * zz[k];
-                acc = k < cnt ? acc - prod : acc;
-            }
-            // entries past SW_EREG (rare): straight from the ring, this step's bytes
-            if (int(e) > SW_EREG)
-            {
-                for (int k0 = SW_EREG; k0 < int(e); k0 += 8)
+                const u32x4 so = *reinterpret_cast<const u32x4 *>(octs + 16 * uint32_t(k0 >> 3) * L + l16);
+                double zk[8], vk[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
                 {
-                    const u32x4 so = v4(oct_at(k0 >> 3));
-                    double zk[8], vk[8];
+                    const uint32_t off = (u & 1) ? so[u >> 1] >> 16 : so[u >> 1] & 0xffffu;
+                    zk[u] = *reinterpret_cast<const double *>(zsb + off);
+                    const uint32_t q = min(uint32_t(k0 + u) >> 1, np - 1);
+                    vk[u] = *reinterpret_cast<const double *>(pairs + 16 * q * L + l16 + 8 * (u & 1));
+                }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                    {
-                        const uint32_t raw = (so[u >> 1] >> (16 * (u & 1))) & (SW_SLOTS - 1);
-                        zk[u] = zs[k0 + u < cnt ? raw : uint32_t(SW_ZERO)];
-                        vk[u] = f64(pair_at((k0 + u) >> 1) + 8 * (u & 1));
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                    {
-                        const double prod = vk[u] * zk[u];
-                        acc = k0 + u < cnt ? acc - prod : acc;
-                    }
+                for (int u = 0; u < 8; ++u)
+                {
+                    const double prod = vk[u] * zk[u];
+                    acc = k0 + u < int(e) ? acc - prod : acc;
                 }
             }
-            if (uint32_t(lane) < L)
-            {
-                double z = acc, res = acc;
-                if (KIND == SW_ILU_BWD) z = res = acc / d;
-                else if (KIND == SW_SSOR_FWD || KIND == SW_SSOR_BWD)
-                {
-                    z = acc * w / d;
-                    res = KIND == SW_SSOR_FWD ? z * (c * d / w) : z;
-                }
-                zs[sc & 0xffffu] = z;
-                if (dst >= 0) out[dst] = res;
-            }
         }
-        // the next step's rows and first entries, into the registers this step is done with
-        o = o2;
-        size = size2;
-        nw = nw2;
-        if (wv < int(nw))
+        if (uint32_t(lane) < L)
         {
-            read_rows(o, rec2);
-            read_entries();
+            double z = acc, res = acc;
+            if (KIND == SW_ILU_BWD) z = res = acc / d;
+            else if (KIND == SW_SSOR_FWD || KIND == SW_SSOR_BWD)
+            {
+                z = acc * w / d;
+                res = KIND == SW_SSOR_FWD ? z * (c * d / w) : z;
+            }
+            zs[sc & 0xffffu] = z;
+            if (dst >= 0) out[dst] = res;
         }
-        // LDS settled, global stores left in flight
-        __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+    // LDS settled, global loads and stores left in flight
+    auto barrier = []() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+
+    load_block();
+    for (int s = 0; s < team && s < nsteps; ++s) barrier();
+    int s = team;
+    for (; s + SW_TEAMS <= nsteps; s += SW_TEAMS)
+    {
+        compute();
+        barrier();
+        load_block();
+#pragma unroll
+        for (int k = 1; k < SW_TEAMS; ++k) barrier();
     }
+    if (s < nsteps)
+    {
+        compute();
+        barrier();
+        for (++s; s < nsteps; ++s) barrier();
+    }
+}
+
+// first: the teams' first blocks (a record per wave); zunit: the stream's zeros
+template <int KIND>
+__global__ __launch_bounds__(SW_THREADS) void k_sweep(int nsteps, const u32x4 *__restrict__ stream,
+                                                      const u32x4 *__restrict__ first, uint32_t zunit, double w, double c,
+                                                      double *out, const int *stop)
+{
+    __shared__ double zs[SW_SLOTS];
+    if (stop && *stop) return;
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+    if (t == 0) zs[SW_ZERO] = 0.0;
+    const u32x4 rec = first[wv];
+    __syncthreads();
+    // the team as a scalar: the steps' loop runs on scalar counters, not per lane
+    sweep_wave<KIND>(nsteps, __builtin_amdgcn_readfirstlane(wv / SW_TWAVES), stream, zunit, rec, w, c, out,
+                     reinterpret_cast<const uint8_t *>(zs), zs, lane);
 }
 
 // stream[in_pos[k]] = src[perm ? perm[k] : k]: the rows' right-hand side into the forward stream
@@ -246,19 +210,18 @@ __global__ void k_sweep_pack(int64_t nnz, const int *__restrict__ fmap, const in
 
 } // namespace
 
-void launch_sweep(int kind, int nsteps, const void *stream, const int *wb, int zbytes, int rbytes, double omega, double *out,
+void launch_sweep(int kind, int nsteps, const void *stream, const void *first, uint32_t zunit, double omega, double *out,
                   const int *stop, hipStream_t s)
 {
     if (nsteps <= 0) return;
     const double c = (2.0 - omega) / omega;
-    const u32x4 *st = static_cast<const u32x4 *>(stream);
-    const uint32_t zb = uint32_t(zbytes), rb = uint32_t(rbytes);
+    const u32x4 *st = static_cast<const u32x4 *>(stream), *fr = static_cast<const u32x4 *>(first);
     switch (kind)
     {
-    case SW_ILU_FWD: k_sweep<SW_ILU_FWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, wb, zb, rb, omega, c, out, stop); break;
-    case SW_ILU_BWD: k_sweep<SW_ILU_BWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, wb, zb, rb, omega, c, out, stop); break;
-    case SW_SSOR_FWD: k_sweep<SW_SSOR_FWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, wb, zb, rb, omega, c, out, stop); break;
-    default: k_sweep<SW_SSOR_BWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, wb, zb, rb, omega, c, out, stop); break;
+    case SW_ILU_FWD: k_sweep<SW_ILU_FWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
+    case SW_ILU_BWD: k_sweep<SW_ILU_BWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
+    case SW_SSOR_FWD: k_sweep<SW_SSOR_FWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
+    default: k_sweep<SW_SSOR_BWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
     }
 }
 

@@ -9,39 +9,32 @@
 // the schedule.
 //
 // The z values live in LDS slots: a row's slot is taken at its step and given back after the last
-// step that reads it (on C1's RCM order at most ~5.7k values are live at once); the LDS the slots
-// leave is the ring.  Everything a step
-// reads besides those slots -- its header, the rows' records and their entries (value, slot of the
-// column) -- is one contiguous piece of a static byte stream in HBM, which producer waves copy into
-// an LDS ring ahead of use while the consumer waves compute.  A consumer wave reads the next
-// step's piece into registers during the current step, so that after a barrier it only gathers the
-// z values its rows need and sums:
+// step that reads it (on C1's RCM order at most ~5.7k values are live at once).  Everything else a
+// step needs -- its rows' inputs, destinations, slots, counts, diagonals, entry values and the
+// slots of their columns -- comes straight from HBM into registers: the workgroup's waves form
+// SW_TEAMS teams of SW_TWAVES, team s % SW_TEAMS runs step s, and each wave loads the rows of its
+// team's next step during the step after its own, SW_TEAMS - 1 steps ahead of use.  A step then
+// costs one round of LDS gathers of z values, the products subtracted in stored order, and the
+// stores of its results.
 //
-//   step:  header (SW_HDR bytes): u32 size (bytes of the step), u32 waves, u32 wb[s + 4],
-//          u32 wb[s + 5] (the blocks the producer of step s loads for step s + 4), then per wave w a
-//          record of four u32 (woff, emax, L, npre); then one block per consumer wave with rows, at
-//          the step's offset + woff, for its L lanes (its rows, longest first, rounded up to a
-//          multiple of 8), every section a multiple of 16 bytes:
-//            f64 vin[L]    the row's input: r (forward, filled per application), y (backward,
-//                          written by the forward sweep)
-//            i32 dest[L]   where the result goes: forward -- the backward stream's vin field of the
-//                          row (index in doubles), backward -- the output vector's index; -1: none
-//            u32 sc[L]     the row's own LDS slot (SW_TRASH when no later row reads it) | its
-//                          entry count << 16
-//            f64 diag[L]   the row's diagonal (kinds that divide by it)
-//            u32 pre[npre] padded to a multiple of 4: where (in 16-byte units from the block's start)
-//                          pair q of the entries starts (q < npairs = ceil(emax / 2)), then where
-//                          slot octet r starts (r < ceil(emax / 8))
-//            pairs         entries 2q and 2q + 1 of lane j as two f64 at pre[q] + j (only the
-//                          rows with more than 2q entries: a jagged-diagonal layout by pairs)
-//            octets        the LDS slots of entries 8r .. 8r + 7 of lane j as eight u16 at
-//                          pre[npairs + r] + j (rows with more than 8r entries)
-//
-// Producer wave p handles the steps s with s % 4 == p: at step s it stores the blocks it loaded
-// four steps earlier -- blocks [wb[s], wb[s+1]) of SW_BLOCK bytes -- into the ring and loads those
-// of step s + 4, whose range step s's header holds (the first four ranges come from wb in HBM).
-// The plan places wb so that, while step s runs, the ring holds step s's bytes and all of step
-// s + 1's, and no store overwrites bytes the step may still read.
+// The stream holds one block per step and wave of the step's team (steps in order, waves in
+// order), 16-byte units, for the wave's L lanes (its rows, longest first, L a multiple of 8, 0 for
+// a wave with no rows):
+//   u32 next[4]     the block of the same wave in its team's next step: unit offset, emax, L, 0
+//   f64 vin[L]      the row's input: r (forward, filled per application), y (backward, written by
+//                   the forward sweep)
+//   i32 dest[L]     where the result goes: forward -- the backward stream's vin field of the row
+//                   (index in doubles), backward -- the output vector's index; -1: none
+//   u32 sc[L]       the row's own LDS slot (SW_TRASH when no later row reads it) | entries << 16
+//   f64 diag[L]     the row's diagonal (kinds that divide by it)
+//   pairs           entries 2q and 2q + 1 of lane j as two f64 at unit q L + j, q < ceil(emax / 2)
+//                   (rows shorter than that padded with +0.0)
+//   octets          the LDS byte offsets (8 x slot) of the z values of entries 8r .. 8r + 7 of lane
+//                   j as eight u16 at unit r L + j, r < ceil(emax / 8) (padding: 0, slot SW_ZERO)
+// The first blocks of the teams are listed in a table of SW_TEAMS x SW_TWAVES records; a block past
+// the last step is the empty block at the stream's end, whose zeros also serve every lane's pairs
+// and octets past a block's bound.  A padded entry multiplies +0.0 by slot SW_ZERO's +0.0, and
+// acc - (+0.0) is acc bit for bit, so the registered entries past a row's end need no select.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -49,71 +42,61 @@
 
 namespace femd_amd {
 
-constexpr int SW_CWAVES = 8;                   // consumer waves
-constexpr int SW_PWAVES = 4;                   // producer waves
-constexpr int SW_THREADS = 64 * (SW_CWAVES + SW_PWAVES);
-constexpr int SW_ROWS = 64 * SW_CWAVES;        // rows per step at most
-constexpr int SW_BLOCK = 1024;                 // bytes per producer block (64 lanes x 16 B)
-constexpr int SW_LDS = 152 * 1024;             // LDS bytes: the z slots, then the ring
-constexpr int SW_SLOTS = 8192;                 // LDS z slots at most
+constexpr int SW_TEAMS = 4;                    // teams of waves, one step each in turn
+constexpr int SW_TWAVES = 3;                   // waves per team
+constexpr int SW_WAVES = SW_TEAMS * SW_TWAVES;
+constexpr int SW_THREADS = 64 * SW_WAVES;
+constexpr int SW_ROWS = 64 * SW_TWAVES;        // rows per step at most
+constexpr int SW_SLOTS = 8192;                 // LDS z slots at most (64 KiB: byte offsets fit 16 bits)
 constexpr int SW_ZERO = 0, SW_TRASH = 1;       // slot 0 holds 0.0; slot 1 takes values nobody reads
-constexpr int SW_PMAX = 20;                    // blocks a producer wave carries per turn
-constexpr int SW_HDR = 144;
-constexpr int SW_AHEAD = SW_PWAVES;            // steps between a block's load and its store
+constexpr int SW_EREG = 24;                    // entries per row a wave holds in registers
+constexpr int SW_OREG = SW_EREG / 8;           // slot octets in registers
+constexpr int SW_TAIL = 128;                   // units of zeros past the last block
 
 enum { SW_ILU_FWD = 0, SW_ILU_BWD = 1, SW_SSOR_FWD = 2, SW_SSOR_BWD = 3 };
-
-constexpr int SW_EREG = 24;                    // entries per row a consumer keeps in registers
-constexpr int SW_OREG = SW_EREG / 8;           // slot octets in registers
 
 constexpr bool sweep_has_diag(int kind) { return kind != SW_ILU_FWD; }
 constexpr int sw_npairs(int emax) { return (emax + 1) / 2; }
 constexpr int sw_noct(int emax) { return (emax + 7) / 8; }
-constexpr int sw_npre(int emax) { return sw_npairs(emax) + sw_noct(emax); }
-// byte offsets inside a wave block of L lanes
-constexpr int64_t sw_off_dest(int L) { return 8 * int64_t(L); }
-constexpr int64_t sw_off_sc(int L) { return 12 * int64_t(L); }
-constexpr int64_t sw_off_diag(int L) { return 16 * int64_t(L); }
-constexpr int64_t sw_off_pre(int kind, int L) { return (sweep_has_diag(kind) ? 24 : 16) * int64_t(L); }
-// the first pair (16-byte units from the block's start)
-constexpr int64_t sw_unit_pairs(int kind, int L, int emax) { return (sw_off_pre(kind, L) + 4 * ((sw_npre(emax) + 3) / 4 * 4)) / 16; }
-// a wave block with T2 pair units and T8 octet units
-constexpr int64_t sw_wave_bytes(int kind, int L, int emax, int64_t T2, int64_t T8)
-{
-    return 16 * (sw_unit_pairs(kind, L, emax) + T2 + T8);
-}
+// unit offsets inside a block of L lanes
+constexpr int SW_HEAD = 1;                     // units of a block's header (next)
+constexpr int64_t sw_u_pairs(int kind, int L) { return SW_HEAD + (sweep_has_diag(kind) ? 24 : 16) * int64_t(L) / 16; }
+constexpr int64_t sw_u_octets(int kind, int L, int emax) { return sw_u_pairs(kind, L) + int64_t(sw_npairs(emax)) * L; }
+constexpr int64_t sw_block_units(int kind, int L, int emax) { return sw_u_octets(kind, L, emax) + int64_t(sw_noct(emax)) * L; }
+// byte offsets of the row fields from the block's start
+constexpr int64_t sw_b_vin() { return 16 * SW_HEAD; }
+constexpr int64_t sw_b_dest(int L) { return 16 * SW_HEAD + 8 * int64_t(L); }
+constexpr int64_t sw_b_sc(int L) { return 16 * SW_HEAD + 12 * int64_t(L); }
+constexpr int64_t sw_b_diag(int L) { return 16 * SW_HEAD + 16 * int64_t(L); }
 
 struct SweepPlan {
     int kind = 0;
     int n = 0;
-    int nsteps = 0;                  // steps the kernel runs (idle ones included)
-    int nreal = 0;                   // steps with rows
+    int nsteps = 0;
     int nlevels = 0;
     int nslots = 0;                  // slots used (incl. SW_ZERO, SW_TRASH)
     int max_emax = 0;
-    int zbytes = 0;                  // LDS bytes of the z slots (whole blocks)
-    int rbytes = 0;                  // LDS bytes of the ring (whole blocks): the rest
-    std::vector<uint8_t> stream;     // the static bytes (values, vin: zero), padded to whole blocks,
-                                     // and one spare block
-    std::vector<int> wb;             // nsteps + 2 * SW_AHEAD + 2 entries: blocks in the ring before step s
+    std::vector<uint8_t> stream;     // the blocks (values, vin: zero) and SW_TAIL units of zeros
+    std::vector<uint32_t> first;     // SW_TEAMS x SW_TWAVES records (u32 x 4 each): the teams' first blocks
+    uint32_t zunit = 0;              // the empty block's first unit of zeros
     std::vector<int64_t> vmap;       // per stored entry p of the pattern: its value's index (doubles) in
                                      // the stream, or -1 (its diagonal too, for kinds with diag)
     std::vector<int64_t> in_pos;     // per row: index (doubles) of its vin field
     int64_t entries = 0;             // stored entries the sweep reads (without padding)
-    int64_t nblocks() const { return static_cast<int64_t>(stream.size()) / SW_BLOCK; }
+    int64_t padded = 0;              // entry places in the pairs (with padding)
 };
 
 // Plan the sweep of `kind` over the square pattern (ip, ix) with diagonal positions diag: rows in
 // ascending order (forward, the strictly lower entries) or descending (backward, the strictly
 // upper ones).  dest[i]: the dest field of row i (see above).  Returns false, with the reason in
-// why, when the schedule does not fit the LDS (more live values than slots, or a step larger than
-// the ring allows).
+// why, when the schedule does not fit (more live values than slots, a stream beyond 16-byte
+// offsets of 32 bits).
 bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, const std::vector<int> &dest,
                 int max_rows, SweepPlan &out, std::string &why);
 
-// The plan run on the host exactly as the GPU runs it (ring, slots, producer schedule), for tests:
-// stream holds the packed values (vmap applied) and the vin fields; out receives the dest writes.
-// Returns false if the run reads a ring byte that no producer stored first (a planning error).
+// The plan run on the host exactly as the GPU runs it (teams, blocks, slots), for tests: stream
+// holds the packed values (vmap applied) and the vin fields; out receives the dest writes.  Returns
+// false if a block's record does not name its wave's next block, or a read leaves the stream.
 bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &stream, double omega, double *out, int64_t nout,
                    std::string &why);
 

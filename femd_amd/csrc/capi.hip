@@ -337,10 +337,14 @@ void precond_apply(femd_precond_s *M, const double *r, double *z, const int *sto
         // the planned sweeps: r (permuted) into the forward stream, the forward sweep writes y into
         // the backward stream, the backward sweep writes z (scattered back through the order)
         const int ilu = M->kind == FEMD_PC_ILU0;
+        double *stage = M->swork.as<double>();
         launch_sweep_fill(n, M->inposF.as<int>(), perm ? M->perm.as<int>() : nullptr, r, M->streamF.as<double>(), stop, s);
-        launch_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, M->nstepsF, M->streamF.p, M->firstF.p, M->zunitF, M->omega,
-                     M->streamB.as<double>(), stop, s);
-        launch_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, M->nstepsB, M->streamB.p, M->firstB.p, M->zunitB, M->omega, z, stop, s);
+        launch_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, M->eregF, M->nstepsF, M->streamF.p, M->firstF.p, M->zunitF, M->omega,
+                     stage, stop, s);
+        launch_sweep_scatter(n, M->scatF.as<int>(), stage, M->streamB.as<double>(), stop, s);
+        launch_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, M->eregB, M->nstepsB, M->streamB.p, M->firstB.p, M->zunitB, M->omega,
+                     stage, stop, s);
+        launch_sweep_scatter(n, M->scatB.as<int>(), stage, z, stop, s);
         return;
     }
     const double *rin = r;
@@ -531,27 +535,21 @@ void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vect
         for (int p = pip[i]; p < pip[i + 1]; ++p)
             if (pix[p] == i) dg[i] = p;
     const bool ilu = M->kind == FEMD_PC_ILU0;
-    std::vector<int> dest(n);
-    for (int i = 0; i < n; ++i) dest[i] = perm.empty() ? i : perm[i];
     SweepPlan B, F;
     std::string why;
-    int rows = SW_ROWS;
-    bool ok = false;
-    for (; rows >= 64 && !ok; rows /= 2)
-    {
-        ok = plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, pip.data(), pix.data(), dg.data(), dest, rows, B, why);
-        if (!ok) continue;
-        std::vector<int> fdest(n);
-        for (int i = 0; i < n; ++i)
-        {
-            if (B.in_pos[i] > INT32_MAX) { ok = false; why = "stream too large"; break; }
-            fdest[i] = int(B.in_pos[i]);
-        }
-        ok = ok && plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, pip.data(), pix.data(), dg.data(), fdest, rows, F, why);
-    }
+    bool ok = plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, pip.data(), pix.data(), dg.data(), SW_ROWS, B, why) &&
+              plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, pip.data(), pix.data(), dg.data(), SW_ROWS, F, why);
     if (ok && (F.stream.size() >= (size_t(1) << 31) || B.stream.size() >= (size_t(1) << 31)))
         ok = false, why = "a stream of 2 GiB or more";
     if (!ok) { M->sweep_note = why; return; }
+    // where the sweeps' results go: the forward sweep's into the backward stream's vin fields, the
+    // backward sweep's into the output vector (through the ordering)
+    std::vector<int> scatF(n), scatB(n);
+    for (int k = 0; k < n; ++k)
+    {
+        scatF[k] = int(B.in_pos[F.order[k]]);
+        scatB[k] = perm.empty() ? B.order[k] : perm[B.order[k]];
+    }
     const int64_t nnz = pip[n];
     std::vector<int> fmap(nnz), bmap(nnz), inpos(n);
     for (int64_t p = 0; p < nnz; ++p)
@@ -565,6 +563,9 @@ void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vect
     M->firstF = upload(F.first, s);
     M->firstB = upload(B.first, s);
     M->inposF = upload(inpos, s);
+    M->scatF = upload(scatF, s);
+    M->scatB = upload(scatB, s);
+    M->swork = DevBuf(8 * size_t(n));
     {
         DevBuf df = upload(fmap, s), db = upload(bmap, s);
         launch_sweep_pack(nnz, df.as<int>(), db.as<int>(), M->lu.as<double>(), M->streamF.as<double>(), M->streamB.as<double>(), s);
@@ -574,6 +575,8 @@ void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vect
     M->nstepsB = B.nsteps;
     M->zunitF = F.zunit;
     M->zunitB = B.zunit;
+    M->eregF = F.ereg;
+    M->eregB = B.ereg;
     M->nlevF = F.nlevels;
     M->nlevB = B.nlevels;
     M->bytesF = int64_t(F.stream.size());
@@ -1458,15 +1461,11 @@ int femd_sweep_host_apply(int n, const int *indptr, const int *indices, const do
         for (int i = 0; i < n; ++i)
             if (dg[i] < 0) fail(FEMD_EZEROPIVOT, "sweep_host_apply: no diagonal entry in row " + std::to_string(i));
         const bool ilu = kind == FEMD_PC_ILU0;
-        std::vector<int> dest(n);
-        for (int i = 0; i < n; ++i) dest[i] = i;
         SweepPlan B, F;
         std::string why;
-        if (!plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, indptr, indices, dg.data(), dest, SW_ROWS, B, why))
+        if (!plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, indptr, indices, dg.data(), SW_ROWS, B, why))
             fail(FEMD_EINVAL, "sweep_host_apply: " + why);
-        std::vector<int> fdest(n);
-        for (int i = 0; i < n; ++i) fdest[i] = int(B.in_pos[i]);
-        if (!plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, indptr, indices, dg.data(), fdest, SW_ROWS, F, why))
+        if (!plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, indptr, indices, dg.data(), SW_ROWS, F, why))
             fail(FEMD_EINVAL, "sweep_host_apply: " + why);
         std::vector<uint8_t> fs = F.stream, bs = B.stream;
         double *f64 = reinterpret_cast<double *>(fs.data()), *b64 = reinterpret_cast<double *>(bs.data());
@@ -1476,8 +1475,11 @@ int femd_sweep_host_apply(int n, const int *indptr, const int *indices, const do
             if (B.vmap[p] >= 0) b64[B.vmap[p]] = values[p];
         }
         for (int i = 0; i < n; ++i) f64[F.in_pos[i]] = r[i];
-        if (!emulate_sweep(F, fs, omega, b64, int64_t(bs.size() / 8), why)) fail(FEMD_EINVAL, "sweep_host_apply: " + why);
-        if (!emulate_sweep(B, bs, omega, z, n, why)) fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        std::vector<double> stage(n);
+        if (!emulate_sweep(F, fs, omega, stage.data(), why)) fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        for (int k = 0; k < n; ++k) b64[B.in_pos[F.order[k]]] = stage[k];
+        if (!emulate_sweep(B, bs, omega, stage.data(), why)) fail(FEMD_EINVAL, "sweep_host_apply: " + why);
+        for (int k = 0; k < n; ++k) z[B.order[k]] = stage[k];
         if (stats)
         {
             stats[0] = F.nsteps; stats[1] = B.nsteps; stats[2] = F.nlevels; stats[3] = B.nlevels;

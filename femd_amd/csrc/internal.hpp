@@ -254,8 +254,10 @@ struct femd_precond_s {
     bool sweep = false;
     int nstepsF = 0, nstepsB = 0, nlevF = 0, nlevB = 0;
     uint32_t zunitF = 0, zunitB = 0;     // the streams' zeros
+    int eregF = 24, eregB = 24;          // entries per row in registers
     int64_t bytesF = 0, bytesB = 0, entriesLU = 0;
     femd_amd::DevBuf streamF, streamB, firstF, firstB, inposF;
+    femd_amd::DevBuf scatF, scatB, swork;    // where the sweeps' results go; their staging vector
     std::string sweep_note;                             // why the sweep was not planned, when it was not
 };
 

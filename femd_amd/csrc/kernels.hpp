@@ -89,10 +89,11 @@ void launch_ssor_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, in
                        const double *r, double *z, const int *stop, hipStream_t s);
 
 // ---- the planned one-workgroup sweeps (sweep.hip, sweep.hpp) --------------------------------------
-void launch_sweep(int kind, int nsteps, const void *stream, const void *first, uint32_t zunit, double omega, double *out,
-                  const int *stop, hipStream_t s);
+void launch_sweep(int kind, int ereg, int nsteps, const void *stream, const void *first, uint32_t zunit, double omega,
+                  double *out, const int *stop, hipStream_t s);
 void launch_sweep_fill(int n, const int *in_pos, const int *perm, const double *src, double *stream, const int *stop,
                        hipStream_t s);
+void launch_sweep_scatter(int n, const int *to, const double *src, double *dst, const int *stop, hipStream_t s);
 void launch_sweep_pack(int64_t nnz, const int *fmap, const int *bmap, const double *v, double *fwd, double *bwd, hipStream_t s);
 
 // ---- fast (non-ordered) reductions and FP32 CG for the mixed-precision path ----------------------

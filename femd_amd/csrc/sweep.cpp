@@ -14,8 +14,8 @@ template <class T> void put(std::vector<uint8_t> &s, int64_t at, T v) { std::mem
 
 } // namespace
 
-bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, const std::vector<int> &dest,
-                int max_rows, SweepPlan &out, std::string &why)
+bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, int max_rows, SweepPlan &out,
+                std::string &why)
 {
     const bool fwd = kind == SW_ILU_FWD || kind == SW_SSOR_FWD;
     max_rows = std::max(1, std::min(max_rows, SW_ROWS));
@@ -93,7 +93,9 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
     };
     const int64_t nblocks = int64_t(nsteps) * SW_TWAVES;
     std::vector<int64_t> at(nblocks + 1, 0);        // unit offsets
-    std::vector<int> bL(nblocks, 0), be(nblocks, 0);
+    std::vector<int> bL(nblocks, 0), be(nblocks, 0), bn(nblocks, 0), bpos(nblocks, 0);
+    P.order.assign(n, 0);
+    int placed = 0;
     for (int s = 0; s < nsteps; ++s)
         for (int w = 0; w < SW_TWAVES; ++w)
         {
@@ -101,12 +103,34 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
             wave_rows(s, w, a0, a1);
             int e = 0;
             for (int a = a0; a < a1; ++a) e = std::max(e, count(rows[a]));
+            if (e > 0xffff)
+            {
+                why = "a row with more than 65535 entries in the triangle";
+                return false;
+            }
             const int64_t b = int64_t(s) * SW_TWAVES + w;
             bL[b] = (a1 - a0 + 7) / 8 * 8;
             be[b] = e;
+            bn[b] = a1 - a0;
+            bpos[b] = placed;
+            for (int a = a0; a < a1; ++a) P.order[placed++] = rows[a];
             P.max_emax = std::max(P.max_emax, e);
             at[b + 1] = at[b] + sw_block_units(kind, bL[b], e);
         }
+    {
+        // entries in registers: the fewest of 8, 16, 24 that hold all but 0.5 % of the waves'
+        // longest rows (the others load the rest when they run)
+        int64_t nonempty = 0, over8 = 0, over16 = 0;
+        for (int64_t b = 0; b < nblocks; ++b)
+            if (bL[b])
+            {
+                ++nonempty;
+                over8 += be[b] > 8;
+                over16 += be[b] > 16;
+            }
+        const int64_t allow = nonempty / 200;
+        P.ereg = over8 <= allow ? 8 : over16 <= allow ? 16 : 24;
+    }
     const int64_t total = at[nblocks] + SW_TAIL;
     if (total >= (int64_t(1) << 32))
     {
@@ -118,8 +142,9 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
         for (int k = 0; k < 4; ++k) r[k] = 0;
         if (b >= nblocks) { r[0] = uint32_t(at[nblocks]); return; }
         r[0] = uint32_t(at[b]);
-        r[1] = uint32_t(be[b]);
+        r[1] = uint32_t(be[b]) | uint32_t(bn[b]) << 16;
         r[2] = uint32_t(bL[b]);
+        r[3] = uint32_t(bpos[b]);
     };
     P.first.assign(4 * SW_WAVES, 0);
     for (int tm = 0; tm < SW_TEAMS; ++tm)
@@ -148,10 +173,9 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
                 {
                     const int i = rows[a];
                     const int c = count(i);
-                    put<int32_t>(S, o + sw_b_dest(L) + 4 * j, dest.empty() ? -1 : dest[i]);
-                    put<uint32_t>(S, o + sw_b_sc(L) + 4 * j, uint32_t(slot[i]) | uint32_t(c) << 16);
-                    P.in_pos[i] = (o + sw_b_vin() + 8 * j) / 8;
-                    if (sweep_has_diag(kind)) P.vmap[diag[i]] = (o + sw_b_diag(L) + 8 * j) / 8;
+                    put<uint32_t>(S, o + sw_b_sc(j), uint32_t(slot[i]) | uint32_t(c) << 16);
+                    P.in_pos[i] = (o + sw_b_vin(j)) / 8;
+                    if (sweep_has_diag(kind)) P.vmap[diag[i]] = (o + sw_b_diag(L, j)) / 8;
                     int k = 0;
                     for (int p = lo(i); p < hi(i); ++p, ++k)
                     {
@@ -161,9 +185,8 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
                 }
                 else
                 {
-                    put<int32_t>(S, o + sw_b_dest(L) + 4 * j, -1);
-                    put<uint32_t>(S, o + sw_b_sc(L) + 4 * j, uint32_t(SW_TRASH));
-                    if (sweep_has_diag(kind)) put<double>(S, o + sw_b_diag(L) + 8 * j, 1.0);
+                    put<uint32_t>(S, o + sw_b_sc(j), uint32_t(SW_TRASH));
+                    if (sweep_has_diag(kind)) put<double>(S, o + sw_b_diag(L, j), 1.0);
                 }
             }
             P.padded += int64_t(2 * sw_npairs(e)) * (a1 - a0);
@@ -173,15 +196,15 @@ bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, 
     return true;
 }
 
-bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, double *out, int64_t nout, std::string &why)
+bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, double *out, std::string &why)
 {
     const int kind = P.kind;
     const int64_t units = int64_t(S.size()) / 16;
     auto u32 = [&](int64_t x) { uint32_t v; std::memcpy(&v, S.data() + x, 4); return v; };
-    auto i32 = [&](int64_t x) { int32_t v; std::memcpy(&v, S.data() + x, 4); return v; };
     auto u16 = [&](int64_t x) { uint16_t v; std::memcpy(&v, S.data() + x, 2); return v; };
     auto f64 = [&](int64_t x) { double v; std::memcpy(&v, S.data() + x, 8); return v; };
-    if (int(P.first.size()) != 4 * SW_WAVES || P.nslots > SW_SLOTS || int64_t(P.zunit) + 64 > units)
+    if (int(P.first.size()) != 4 * SW_WAVES || P.nslots > SW_SLOTS || int64_t(P.zunit) + 64 > units ||
+        (P.ereg != 8 && P.ereg != 16 && P.ereg != 24))
     {
         why = "malformed plan";
         return false;
@@ -198,7 +221,13 @@ bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, do
         {
             uint32_t *r = &rec[4 * (tm * SW_TWAVES + wv)];
             const int64_t o = 16 * int64_t(r[0]);
-            const int e = int(r[1]), L = int(r[2]);
+            const int e = int(r[1] & 0xffffu), nr = int(r[1] >> 16), L = int(r[2]);
+            const int64_t pos = r[3];
+            if (nr > L || pos + nr > P.n)
+            {
+                why = "a block's rows out of range";
+                return false;
+            }
             // the loads the GPU issues for this block (every lane reads its place as if it had a
             // row: up to 64 units past a section's start) stay inside the stream
             if (int64_t(r[0]) + std::max<int64_t>(sw_block_units(kind, L, e), sw_u_pairs(kind, L)) + 64 > units)
@@ -209,8 +238,8 @@ bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, do
             const int64_t up = sw_u_pairs(kind, L), uo = sw_u_octets(kind, L, e);
             for (int j = 0; j < L; ++j)
             {
-                double acc = f64(o + sw_b_vin() + 8 * j);
-                const uint32_t sc = u32(o + sw_b_sc(L) + 4 * j);
+                double acc = f64(o + sw_b_vin(j));
+                const uint32_t sc = u32(o + sw_b_sc(j));
                 const int cnt = int(sc >> 16);
                 if (cnt > e)
                 {
@@ -218,7 +247,7 @@ bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, do
                     return false;
                 }
                 // every registered place: entries past the row's end are +0.0 times slot SW_ZERO
-                const int upto = std::max(cnt, std::min(e, SW_EREG));
+                const int upto = std::max(cnt, std::min(e, P.ereg));
                 for (int k = 0; k < upto; ++k)
                 {
                     const double v = f64(o + 16 * (up + int64_t(k / 2) * L + j) + 8 * (k & 1));
@@ -233,20 +262,15 @@ bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &S, double omega, do
                     acc = acc - prod;
                 }
                 double zv = acc, yv = 0.0;
-                if (kind == SW_ILU_BWD) zv = acc / f64(o + sw_b_diag(L) + 8 * j);
+                if (kind == SW_ILU_BWD) zv = acc / f64(o + sw_b_diag(L, j));
                 else if (kind == SW_SSOR_FWD || kind == SW_SSOR_BWD)
                 {
-                    const double d = f64(o + sw_b_diag(L) + 8 * j);
+                    const double d = f64(o + sw_b_diag(L, j));
                     zv = acc * w / d;
                     if (kind == SW_SSOR_FWD) yv = zv * (c * d / w);
                 }
                 written.emplace_back(int(sc & 0xffffu), zv);
-                const int32_t dst = i32(o + sw_b_dest(L) + 4 * j);
-                if (dst >= 0)
-                {
-                    if (dst >= nout) { why = "destination out of range"; return false; }
-                    out[dst] = kind == SW_SSOR_FWD ? yv : zv;
-                }
+                if (j < nr) out[pos + j] = kind == SW_SSOR_FWD ? yv : zv;
             }
             // the block names the wave's next one
             for (int k = 0; k < 4; ++k) r[k] = u32(o + 4 * k);

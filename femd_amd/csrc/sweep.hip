@@ -32,17 +32,17 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // the row's output scaled by (c d) / w for the backward sweep), SW_SSOR_BWD (z = (y - sum U z) w / d).
 // out: where the results go (the backward stream as doubles for a forward sweep, the output vector for a
 // backward one).
-template <int KIND>
+template <int KIND, int EREG>
 __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__restrict__ stream, uint32_t zunit, u32x4 rec,
                                            double w, double c, double *out, const uint8_t *zsb, double *zs, int lane)
 {
+    constexpr int OREG = EREG / 8;
     // this wave's block of its team's coming step: where it is, its bound and lanes, the record of
     // the block after it, and per lane the row's fields and first entries
-    uint32_t base = 0, e = 0, L = 0, sc = 0;
-    int dst = -1;
+    uint32_t base = 0, e = 0, nr = 0, L = 0, pos = 0, sc = 0;
     double vin = 0.0, d = 1.0;
-    u32x4 vq[SW_EREG / 2], sq[SW_OREG];
-    const uint32_t l4 = 4 * uint32_t(lane), l8 = 8 * uint32_t(lane), l16 = 16 * uint32_t(lane);
+    u32x4 vq[EREG / 2], sq[OREG];
+    const uint32_t l16 = 16 * uint32_t(lane);
     const uint8_t *zero = reinterpret_cast<const uint8_t *>(stream + zunit);
     // Every lane loads its places as if it had a row (past the wave's rows it reads bytes it never
     // uses, all inside the stream), pairs and octets past the block's bound from the zeros at the
@@ -50,28 +50,32 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
     // per-lane arithmetic.  The record is read into scalars before the loads replace it.
     auto load_block = [&]() {
         base = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.x)));
-        e = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.y)));
+        const uint32_t en = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.y)));
+        e = en & 0xffffu;
+        nr = en >> 16;
         L = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.z)));
+        pos = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.w)));
         const uint8_t *b = reinterpret_cast<const uint8_t *>(stream + base);
         // the record through a vector load (an offset the compiler cannot see is zero): a scalar
         // load would be waited for at the barrier, a full memory latency every step
         uint32_t zv;
         __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zv));
         rec = *reinterpret_cast<const u32x4 *>(b + zv);
-        vin = *reinterpret_cast<const double *>(b + sw_b_vin() + l8);
-        dst = *reinterpret_cast<const int *>(b + sw_b_dest(int(L)) + l4);
-        sc = *reinterpret_cast<const uint32_t *>(b + sw_b_sc(int(L)) + l4);
-        if (sweep_has_diag(KIND)) d = *reinterpret_cast<const double *>(b + sw_b_diag(int(L)) + l8);
+        // the row's input and slot | count: one unit per lane
+        const u32x4 rw = *reinterpret_cast<const u32x4 *>(b + sw_b_vin(0) + l16);
+        vin = __hiloint2double(int(rw.y), int(rw.x));
+        sc = rw.w;
+        if (sweep_has_diag(KIND)) d = *reinterpret_cast<const double *>(b + sw_b_diag(int(L), 0) + l16);
         const uint32_t np = (e + 1) >> 1, no = (e + 7) >> 3;
         const uint8_t *pairs = b + 16 * sw_u_pairs(KIND, int(L)), *octs = pairs + 16 * np * L;
 #pragma unroll
-        for (int q = 0; q < SW_EREG / 2; ++q)
+        for (int q = 0; q < EREG / 2; ++q)
         {
             const uint8_t *at = uint32_t(q) < np ? pairs + 16 * uint32_t(q) * L : zero;
             vq[q] = *reinterpret_cast<const u32x4 *>(at + l16);
         }
 #pragma unroll
-        for (int r = 0; r < SW_OREG; ++r)
+        for (int r = 0; r < OREG; ++r)
         {
             const uint8_t *at = uint32_t(r) < no ? octs + 16 * uint32_t(r) * L : zero;
             sq[r] = *reinterpret_cast<const u32x4 *>(at + l16);
@@ -83,17 +87,17 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
     auto compute = [&]() {
         // the loaded registers are first touched here: without this the compiler moves work on
         // them (sc & 0xffff, say) up into the load step, where it waits for the loads it issued
-        __asm__ volatile("" : "+v"(sc), "+v"(dst), "+v"(vin), "+v"(d));
+        __asm__ volatile("" : "+v"(sc), "+v"(vin), "+v"(d));
 #pragma unroll
-        for (int r = 0; r < SW_OREG; ++r) __asm__ volatile("" : "+v"(sq[r]));
+        for (int r = 0; r < OREG; ++r) __asm__ volatile("" : "+v"(sq[r]));
 #pragma unroll
-        for (int q = 0; q < SW_EREG / 2; ++q) __asm__ volatile("" : "+v"(vq[q]));
+        for (int q = 0; q < EREG / 2; ++q) __asm__ volatile("" : "+v"(vq[q]));
         double acc = vin;
-        double zz[SW_EREG];
+        double zz[EREG];
         // every gather of the registered entries at once: the octets hold the z values' LDS byte
         // offsets (a padded entry's is slot SW_ZERO's, +0.0)
 #pragma unroll
-        for (int k = 0; k < SW_EREG; ++k)
+        for (int k = 0; k < EREG; ++k)
         {
             const uint32_t word = sq[k >> 3][(k >> 1) & 3];
             const uint32_t off = (k & 1) ? word >> 16 : word & 0xffffu;
@@ -104,20 +108,20 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
         // the products in stored order: the reference's s -= v[p] * z[ix[p]], one rounding for the
         // product and one for the difference; a padded entry subtracts +0.0, which leaves acc as it is
 #pragma unroll
-        for (int k = 0; k < SW_EREG; ++k)
+        for (int k = 0; k < EREG; ++k)
         {
             const u32x4 pq = vq[k >> 1];
             const uint32_t lo = (k & 1) ? pq.z : pq.x, hi = (k & 1) ? pq.w : pq.y;
             const double prod = __hiloint2double(int(hi), int(lo)) * zz[k];
             acc = acc - prod;
         }
-        // entries past SW_EREG (rare): loaded now, padded like the first ones up to the block's bound
-        if (e > uint32_t(SW_EREG))
+        // entries past EREG (rare): loaded now, padded like the first ones up to the block's bound
+        if (e > uint32_t(EREG))
         {
             const uint32_t np = (e + 1) >> 1;
             const uint8_t *pairs = reinterpret_cast<const uint8_t *>(stream + base) + 16 * sw_u_pairs(KIND, int(L)),
                           *octs = pairs + 16 * np * L;
-            for (int k0 = SW_EREG; k0 < int(e); k0 += 8)
+            for (int k0 = EREG; k0 < int(e); k0 += 8)
             {
                 const u32x4 so = *reinterpret_cast<const u32x4 *>(octs + 16 * uint32_t(k0 >> 3) * L + l16);
                 double zk[8], vk[8];
@@ -147,7 +151,8 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
                 res = KIND == SW_SSOR_FWD ? z * (c * d / w) : z;
             }
             zs[sc & 0xffffu] = z;
-            if (dst >= 0) out[dst] = res;
+            // the results in the sweep's order: 64 consecutive doubles per wave
+            if (uint32_t(lane) < nr) out[pos + uint32_t(lane)] = res;
         }
     };
     // LDS settled, global loads and stores left in flight
@@ -172,8 +177,9 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
     }
 }
 
-// first: the teams' first blocks (a record per wave); zunit: the stream's zeros
-template <int KIND>
+// first: the teams' first blocks (a record per wave); zunit: the stream's zeros; EREG: entries per
+// row in registers
+template <int KIND, int EREG>
 __global__ __launch_bounds__(SW_THREADS) void k_sweep(int nsteps, const u32x4 *__restrict__ stream,
                                                       const u32x4 *__restrict__ first, uint32_t zunit, double w, double c,
                                                       double *out, const int *stop)
@@ -185,8 +191,17 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep(int nsteps, const u32x4 *_
     const u32x4 rec = first[wv];
     __syncthreads();
     // the team as a scalar: the steps' loop runs on scalar counters, not per lane
-    sweep_wave<KIND>(nsteps, __builtin_amdgcn_readfirstlane(wv / SW_TWAVES), stream, zunit, rec, w, c, out,
+    sweep_wave<KIND, EREG>(nsteps, __builtin_amdgcn_readfirstlane(wv / SW_TWAVES), stream, zunit, rec, w, c, out,
                      reinterpret_cast<const uint8_t *>(zs), zs, lane);
+}
+
+// dst[to[k]] = src[k]: a sweep's results (in its order) to where they go -- the backward stream's
+// vin fields, or the output vector through the ordering
+__global__ void k_sweep_scatter(int n, const int *__restrict__ to, const double *__restrict__ src, double *dst, const int *stop)
+{
+    if (stop && *stop) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[to[k]] = src[k];
 }
 
 // stream[in_pos[k]] = src[perm ? perm[k] : k]: the rows' right-hand side into the forward stream
@@ -210,18 +225,27 @@ __global__ void k_sweep_pack(int64_t nnz, const int *__restrict__ fmap, const in
 
 } // namespace
 
-void launch_sweep(int kind, int nsteps, const void *stream, const void *first, uint32_t zunit, double omega, double *out,
-                  const int *stop, hipStream_t s)
+template <int KIND>
+void launch_kind(int ereg, int nsteps, const u32x4 *st, const u32x4 *fr, uint32_t zunit, double omega, double c, double *out,
+                 const int *stop, hipStream_t s)
+{
+    if (ereg <= 8) k_sweep<KIND, 8><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop);
+    else if (ereg <= 16) k_sweep<KIND, 16><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop);
+    else k_sweep<KIND, 24><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop);
+}
+
+void launch_sweep(int kind, int ereg, int nsteps, const void *stream, const void *first, uint32_t zunit, double omega,
+                  double *out, const int *stop, hipStream_t s)
 {
     if (nsteps <= 0) return;
     const double c = (2.0 - omega) / omega;
     const u32x4 *st = static_cast<const u32x4 *>(stream), *fr = static_cast<const u32x4 *>(first);
     switch (kind)
     {
-    case SW_ILU_FWD: k_sweep<SW_ILU_FWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
-    case SW_ILU_BWD: k_sweep<SW_ILU_BWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
-    case SW_SSOR_FWD: k_sweep<SW_SSOR_FWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
-    default: k_sweep<SW_SSOR_BWD><<<1, SW_THREADS, 0, s>>>(nsteps, st, fr, zunit, omega, c, out, stop); break;
+    case SW_ILU_FWD: launch_kind<SW_ILU_FWD>(ereg, nsteps, st, fr, zunit, omega, c, out, stop, s); break;
+    case SW_ILU_BWD: launch_kind<SW_ILU_BWD>(ereg, nsteps, st, fr, zunit, omega, c, out, stop, s); break;
+    case SW_SSOR_FWD: launch_kind<SW_SSOR_FWD>(ereg, nsteps, st, fr, zunit, omega, c, out, stop, s); break;
+    default: launch_kind<SW_SSOR_BWD>(ereg, nsteps, st, fr, zunit, omega, c, out, stop, s); break;
     }
 }
 
@@ -229,6 +253,11 @@ void launch_sweep_fill(int n, const int *in_pos, const int *perm, const double *
                        hipStream_t s)
 {
     if (n > 0) k_sweep_fill<<<(n + 255) / 256, 256, 0, s>>>(n, in_pos, perm, src, stream, stop);
+}
+
+void launch_sweep_scatter(int n, const int *to, const double *src, double *dst, const int *stop, hipStream_t s)
+{
+    if (n > 0) k_sweep_scatter<<<(n + 255) / 256, 256, 0, s>>>(n, to, src, dst, stop);
 }
 
 void launch_sweep_pack(int64_t nnz, const int *fmap, const int *bmap, const double *v, double *fwd, double *bwd, hipStream_t s)

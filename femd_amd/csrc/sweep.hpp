@@ -20,17 +20,21 @@
 // The stream holds one block per step and wave of the step's team (steps in order, waves in
 // order), 16-byte units, for the wave's L lanes (its rows, longest first, L a multiple of 8, 0 for
 // a wave with no rows):
-//   u32 next[4]     the block of the same wave in its team's next step: unit offset, emax, L, 0
-//   f64 vin[L]      the row's input: r (forward, filled per application), y (backward, written by
-//                   the forward sweep)
-//   i32 dest[L]     where the result goes: forward -- the backward stream's vin field of the row
-//                   (index in doubles), backward -- the output vector's index; -1: none
-//   u32 sc[L]       the row's own LDS slot (SW_TRASH when no later row reads it) | entries << 16
-//   f64 diag[L]     the row's diagonal (kinds that divide by it)
+//   u32 next[4]     the block of the same wave in its team's next step: unit offset, emax | rows
+//                   << 16 (rows: the lanes that hold one), L, and where its results go: its first
+//                   row's position in the sweep's order
+//   rows[L]         per lane one unit: f64 vin, the row's input (r forward, filled per
+//                   application; y backward, filled from the forward sweep's results); u32 0;
+//                   u32 sc, the row's own LDS slot (SW_TRASH when no later row reads it) | entries
+//                   << 16
+//   diag[L]         per lane one unit: f64 the row's diagonal, 8 bytes of padding (kinds that
+//                   divide by it)
 //   pairs           entries 2q and 2q + 1 of lane j as two f64 at unit q L + j, q < ceil(emax / 2)
 //                   (rows shorter than that padded with +0.0)
 //   octets          the LDS byte offsets (8 x slot) of the z values of entries 8r .. 8r + 7 of lane
 //                   j as eight u16 at unit r L + j, r < ceil(emax / 8) (padding: 0, slot SW_ZERO)
+// A sweep writes its results in its own order of the rows (steps, then waves, then lanes), 64
+// consecutive doubles per wave; the caller moves them where they go with one parallel scatter.
 // The first blocks of the teams are listed in a table of SW_TEAMS x SW_TWAVES records; a block past
 // the last step is the empty block at the stream's end, whose zeros also serve every lane's pairs
 // and octets past a block's bound.  A padded entry multiplies +0.0 by slot SW_ZERO's +0.0, and
@@ -49,8 +53,8 @@ constexpr int SW_THREADS = 64 * SW_WAVES;
 constexpr int SW_ROWS = 64 * SW_TWAVES;        // rows per step at most
 constexpr int SW_SLOTS = 8192;                 // LDS z slots at most (64 KiB: byte offsets fit 16 bits)
 constexpr int SW_ZERO = 0, SW_TRASH = 1;       // slot 0 holds 0.0; slot 1 takes values nobody reads
-constexpr int SW_EREG = 24;                    // entries per row a wave holds in registers
-constexpr int SW_OREG = SW_EREG / 8;           // slot octets in registers
+constexpr int SW_EREG = 24;                    // entries per row a wave holds in registers, at most
+                                               // (the plan picks 8, 16 or 24 for its sweep)
 constexpr int SW_TAIL = 128;                   // units of zeros past the last block
 
 enum { SW_ILU_FWD = 0, SW_ILU_BWD = 1, SW_SSOR_FWD = 2, SW_SSOR_BWD = 3 };
@@ -60,14 +64,13 @@ constexpr int sw_npairs(int emax) { return (emax + 1) / 2; }
 constexpr int sw_noct(int emax) { return (emax + 7) / 8; }
 // unit offsets inside a block of L lanes
 constexpr int SW_HEAD = 1;                     // units of a block's header (next)
-constexpr int64_t sw_u_pairs(int kind, int L) { return SW_HEAD + (sweep_has_diag(kind) ? 24 : 16) * int64_t(L) / 16; }
+constexpr int64_t sw_u_pairs(int kind, int L) { return SW_HEAD + (sweep_has_diag(kind) ? 2 : 1) * int64_t(L); }
 constexpr int64_t sw_u_octets(int kind, int L, int emax) { return sw_u_pairs(kind, L) + int64_t(sw_npairs(emax)) * L; }
 constexpr int64_t sw_block_units(int kind, int L, int emax) { return sw_u_octets(kind, L, emax) + int64_t(sw_noct(emax)) * L; }
-// byte offsets of the row fields from the block's start
-constexpr int64_t sw_b_vin() { return 16 * SW_HEAD; }
-constexpr int64_t sw_b_dest(int L) { return 16 * SW_HEAD + 8 * int64_t(L); }
-constexpr int64_t sw_b_sc(int L) { return 16 * SW_HEAD + 12 * int64_t(L); }
-constexpr int64_t sw_b_diag(int L) { return 16 * SW_HEAD + 16 * int64_t(L); }
+// byte offsets of lane j's row fields from the block's start
+constexpr int64_t sw_b_vin(int j) { return 16 * (SW_HEAD + int64_t(j)); }
+constexpr int64_t sw_b_sc(int j) { return sw_b_vin(j) + 12; }
+constexpr int64_t sw_b_diag(int L, int j) { return 16 * (SW_HEAD + int64_t(L) + j); }
 
 struct SweepPlan {
     int kind = 0;
@@ -79,25 +82,26 @@ struct SweepPlan {
     std::vector<uint8_t> stream;     // the blocks (values, vin: zero) and SW_TAIL units of zeros
     std::vector<uint32_t> first;     // SW_TEAMS x SW_TWAVES records (u32 x 4 each): the teams' first blocks
     uint32_t zunit = 0;              // the empty block's first unit of zeros
+    int ereg = SW_EREG;              // entries per row in registers (8, 16 or 24)
     std::vector<int64_t> vmap;       // per stored entry p of the pattern: its value's index (doubles) in
                                      // the stream, or -1 (its diagonal too, for kinds with diag)
     std::vector<int64_t> in_pos;     // per row: index (doubles) of its vin field
+    std::vector<int> order;          // the rows in the sweep's order (where its results are written)
     int64_t entries = 0;             // stored entries the sweep reads (without padding)
     int64_t padded = 0;              // entry places in the pairs (with padding)
 };
 
 // Plan the sweep of `kind` over the square pattern (ip, ix) with diagonal positions diag: rows in
 // ascending order (forward, the strictly lower entries) or descending (backward, the strictly
-// upper ones).  dest[i]: the dest field of row i (see above).  Returns false, with the reason in
-// why, when the schedule does not fit (more live values than slots, a stream beyond 16-byte
-// offsets of 32 bits).
-bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, const std::vector<int> &dest,
-                int max_rows, SweepPlan &out, std::string &why);
+// upper ones).  Returns false, with the reason in why, when the schedule does not fit (more live
+// values than slots, a stream beyond 16-byte offsets of 32 bits).
+bool plan_sweep(int kind, int n, const int *ip, const int *ix, const int *diag, int max_rows, SweepPlan &out,
+                std::string &why);
 
 // The plan run on the host exactly as the GPU runs it (teams, blocks, slots), for tests: stream
-// holds the packed values (vmap applied) and the vin fields; out receives the dest writes.  Returns
-// false if a block's record does not name its wave's next block, or a read leaves the stream.
-bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &stream, double omega, double *out, int64_t nout,
-                   std::string &why);
+// holds the packed values (vmap applied) and the vin fields; out (n values) receives the results in
+// the sweep's order.  Returns false if a block's record does not name its wave's next block, or a
+// read leaves the stream.
+bool emulate_sweep(const SweepPlan &P, std::vector<uint8_t> &stream, double omega, double *out, std::string &why);
 
 } // namespace femd_amd

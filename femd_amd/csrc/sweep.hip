@@ -43,11 +43,10 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
     double vin = 0.0, d = 1.0;
     u32x4 vq[EREG / 2], sq[OREG];
     const uint32_t l16 = 16 * uint32_t(lane);
-    const uint8_t *zero = reinterpret_cast<const uint8_t *>(stream + zunit);
-    // Every lane loads its places as if it had a row (past the wave's rows it reads bytes it never
-    // uses, all inside the stream), pairs and octets past the block's bound from the zeros at the
-    // stream's end: each load is a scalar address plus the lane's offset, with no branch and no
-    // per-lane arithmetic.  The record is read into scalars before the loads replace it.
+    // Only the lanes with a row load (the others keep what they had and store nothing), and only
+    // the pairs and octets inside the block's bound; the registered places past it are set to
+    // zeros.  Each load is a scalar address plus the lane's offset.  The record is read into
+    // scalars before the loads replace it.
     auto load_block = [&]() {
         base = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.x)));
         const uint32_t en = uint32_t(__builtin_amdgcn_readfirstlane(int(rec.y)));
@@ -61,24 +60,27 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
         uint32_t zv;
         __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zv));
         rec = *reinterpret_cast<const u32x4 *>(b + zv);
-        // the row's input and slot | count: one unit per lane
-        const u32x4 rw = *reinterpret_cast<const u32x4 *>(b + sw_b_vin(0) + l16);
-        vin = __hiloint2double(int(rw.y), int(rw.x));
-        sc = rw.w;
-        if (sweep_has_diag(KIND)) d = *reinterpret_cast<const double *>(b + sw_b_diag(int(L), 0) + l16);
         const uint32_t np = (e + 1) >> 1, no = (e + 7) >> 3;
         const uint8_t *pairs = b + 16 * sw_u_pairs(KIND, int(L)), *octs = pairs + 16 * np * L;
-#pragma unroll
-        for (int q = 0; q < EREG / 2; ++q)
+        if (uint32_t(lane) < nr)
         {
-            const uint8_t *at = uint32_t(q) < np ? pairs + 16 * uint32_t(q) * L : zero;
-            vq[q] = *reinterpret_cast<const u32x4 *>(at + l16);
-        }
+            // the row's input and slot | count: one unit per lane
+            const u32x4 rw = *reinterpret_cast<const u32x4 *>(b + sw_b_vin(0) + l16);
+            vin = __hiloint2double(int(rw.y), int(rw.x));
+            sc = rw.w;
+            if (sweep_has_diag(KIND)) d = *reinterpret_cast<const double *>(b + sw_b_diag(int(L), 0) + 8 * uint32_t(lane));
 #pragma unroll
-        for (int r = 0; r < OREG; ++r)
-        {
-            const uint8_t *at = uint32_t(r) < no ? octs + 16 * uint32_t(r) * L : zero;
-            sq[r] = *reinterpret_cast<const u32x4 *>(at + l16);
+            for (int q = 0; q < EREG / 2; ++q)
+            {
+                if (uint32_t(q) < np) vq[q] = *reinterpret_cast<const u32x4 *>(pairs + 16 * uint32_t(q) * L + l16);
+                else vq[q] = u32x4{0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+            for (int r = 0; r < OREG; ++r)
+            {
+                if (uint32_t(r) < no) sq[r] = *reinterpret_cast<const u32x4 *>(octs + 16 * uint32_t(r) * L + l16);
+                else sq[r] = u32x4{0u, 0u, 0u, 0u};
+            }
         }
     };
     // The same instructions for a wave without rows (its registers hold zeros then, its lanes
@@ -141,7 +143,7 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
                 }
             }
         }
-        if (uint32_t(lane) < L)
+        if (uint32_t(lane) < nr)
         {
             double z = acc, res = acc;
             if (KIND == SW_ILU_BWD) z = res = acc / d;
@@ -152,7 +154,7 @@ __device__ __forceinline__ void sweep_wave(int nsteps, int team, const u32x4 *__
             }
             zs[sc & 0xffffu] = z;
             // the results in the sweep's order: 64 consecutive doubles per wave
-            if (uint32_t(lane) < nr) out[pos + uint32_t(lane)] = res;
+            out[pos + uint32_t(lane)] = res;
         }
     };
     // LDS settled, global loads and stores left in flight

@@ -27,8 +27,7 @@
 //                   application; y backward, filled from the forward sweep's results); u32 0;
 //                   u32 sc, the row's own LDS slot (SW_TRASH when no later row reads it) | entries
 //                   << 16
-//   diag[L]         per lane one unit: f64 the row's diagonal, 8 bytes of padding (kinds that
-//                   divide by it)
+//   f64 diag[L]     the row's diagonal (kinds that divide by it; L / 2 units)
 //   pairs           entries 2q and 2q + 1 of lane j as two f64 at unit q L + j, q < ceil(emax / 2)
 //                   (rows shorter than that padded with +0.0)
 //   octets          the LDS byte offsets (8 x slot) of the z values of entries 8r .. 8r + 7 of lane
@@ -64,13 +63,13 @@ constexpr int sw_npairs(int emax) { return (emax + 1) / 2; }
 constexpr int sw_noct(int emax) { return (emax + 7) / 8; }
 // unit offsets inside a block of L lanes
 constexpr int SW_HEAD = 1;                     // units of a block's header (next)
-constexpr int64_t sw_u_pairs(int kind, int L) { return SW_HEAD + (sweep_has_diag(kind) ? 2 : 1) * int64_t(L); }
+constexpr int64_t sw_u_pairs(int kind, int L) { return SW_HEAD + int64_t(L) + (sweep_has_diag(kind) ? int64_t(L) / 2 : 0); }
 constexpr int64_t sw_u_octets(int kind, int L, int emax) { return sw_u_pairs(kind, L) + int64_t(sw_npairs(emax)) * L; }
 constexpr int64_t sw_block_units(int kind, int L, int emax) { return sw_u_octets(kind, L, emax) + int64_t(sw_noct(emax)) * L; }
 // byte offsets of lane j's row fields from the block's start
 constexpr int64_t sw_b_vin(int j) { return 16 * (SW_HEAD + int64_t(j)); }
 constexpr int64_t sw_b_sc(int j) { return sw_b_vin(j) + 12; }
-constexpr int64_t sw_b_diag(int L, int j) { return 16 * (SW_HEAD + int64_t(L) + j); }
+constexpr int64_t sw_b_diag(int L, int j) { return 16 * (SW_HEAD + int64_t(L)) + 8 * int64_t(j); }
 
 struct SweepPlan {
     int kind = 0;

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <limits>
+#include <thread>
 
 using namespace femd_amd;
 
@@ -536,9 +537,14 @@ void plan_sweeps(femd_precond_s *M, const std::vector<int> &pip, const std::vect
             if (pix[p] == i) dg[i] = p;
     const bool ilu = M->kind == FEMD_PC_ILU0;
     SweepPlan B, F;
-    std::string why;
-    bool ok = plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, pip.data(), pix.data(), dg.data(), SW_ROWS, B, why) &&
-              plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, pip.data(), pix.data(), dg.data(), SW_ROWS, F, why);
+    std::string why, whyF;
+    // the two sweeps' plans are independent: the backward one on a thread of its own
+    bool okB = false;
+    std::thread tb([&] { okB = plan_sweep(ilu ? SW_ILU_BWD : SW_SSOR_BWD, n, pip.data(), pix.data(), dg.data(), SW_ROWS, B, why); });
+    const bool okF = plan_sweep(ilu ? SW_ILU_FWD : SW_SSOR_FWD, n, pip.data(), pix.data(), dg.data(), SW_ROWS, F, whyF);
+    tb.join();
+    bool ok = okB && okF;
+    if (okB && !okF) why = whyF;
     if (ok && (F.stream.size() >= (size_t(1) << 31) || B.stream.size() >= (size_t(1) << 31)))
         ok = false, why = "a stream of 2 GiB or more";
     if (!ok) { M->sweep_note = why; return; }
@@ -643,19 +649,25 @@ void build_precond(femd_precond_s *M, femd_csr_s *A, hipStream_t s)
     if (M->kind == FEMD_PC_ILU0)
     {
         reset_bad();
-        launch_ilu0_factor(M->nlevL, M->lptrL.as<int>(), M->lrowsL.as<int>(), M->ip.as<int>(), M->ix.as<int>(),
-                           M->diag.as<int>(), M->lu.as<double>(), bad.as<int>(), s);
+        std::vector<int> hdg(n, -1);
+        for (int i = 0; i < n; ++i)
+            for (int p = pip[i]; p < pip[i + 1]; ++p)
+                if (pix[p] == i) hdg[i] = p;
+        if (!std::getenv("FEMD_ILU_ONE_GROUP") && ilu0_levels_fit(n, pip.data(), pix.data(), hdg.data()))
+            launch_ilu0_levels(M->nlevL, L.ptr.data(), M->lrowsL.as<int>(), M->ip.as<int>(), M->ix.as<int>(),
+                               M->diag.as<int>(), M->lu.as<double>(), bad.as<int>(), s);
+        else
+            launch_ilu0_factor(M->nlevL, M->lptrL.as<int>(), M->lrowsL.as<int>(), M->ip.as<int>(), M->ix.as<int>(),
+                               M->diag.as<int>(), M->lu.as<double>(), bad.as<int>(), s);
         const int b1 = read_bad();
         if (b1 != big) fail(FEMD_EZEROPIVOT, "ILU(0): zero pivot in row " + std::to_string(b1));
         // the smallest pivot (a minimum: any order gives the same value)
-        std::vector<double> lu(P.nnz);
-        std::vector<int> dg(n);
-        HIPC(hipMemcpyAsync(lu.data(), M->lu.p, 8 * size_t(P.nnz), hipMemcpyDeviceToHost, s));
-        HIPC(hipMemcpyAsync(dg.data(), M->diag.p, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+        DevBuf mp(sizeof(double));
+        launch_min_pivot(n, M->diag.as<int>(), M->lu.as<double>(), mp.as<double>(), s);
+        double h = 0.0;
+        HIPC(hipMemcpyAsync(&h, mp.p, sizeof(double), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
-        double mp = std::numeric_limits<double>::infinity();
-        for (int i = 0; i < n; ++i) mp = std::min(mp, lu[dg[i]]);
-        M->min_pivot = mp;
+        M->min_pivot = n ? h : std::numeric_limits<double>::infinity();
     }
     plan_sweeps(M, pip, pix, perm, s);
     HIPC(hipStreamSynchronize(s));

@@ -727,6 +727,132 @@ __global__ __launch_bounds__(WG_SWEEP) void k_ilu0_factor(int nlev, const int *_
     }
 }
 
+// IKJ ILU(0) (precond.hpp:158-182) for the rows of one dependency level, one wave per row, every
+// wave of the GPU at work: the row's entries, the pivots and upper parts of the rows it eliminates
+// with, and where each of their columns sits in the row, are staged in LDS first (all loads in
+// flight together); then the eliminations run in the reference's order -- for each lower entry k
+// in stored order, l = a_ik / u_kk, then a_ij = a_ij - l u_kj for every j of row k's upper part in
+// row i -- so every value is the reference's bit for bit.  Rows of at most ILU_ROW entries whose
+// eliminations touch at most ILU_UPD upper entries in all (the host checks; else k_ilu0_factor).
+constexpr int ILU_ROW = 64, ILU_UPD = 1024, ILU_WAVES = 4;
+
+__global__ __launch_bounds__(64 * ILU_WAVES) void k_ilu0_level(int nrows, const int *__restrict__ rows,
+                                                               const int *__restrict__ ip, const int *__restrict__ ix,
+                                                               const int *__restrict__ diag, double *v, int *bad_row)
+{
+    struct Stage {
+        int cols[ILU_ROW];
+        int uoff[ILU_ROW + 1];
+        int udiag[ILU_ROW];
+        double vals[ILU_ROW];
+        double ukk[ILU_ROW];
+        int ucol[ILU_UPD];
+        double uval[ILU_UPD];
+        signed char at[ILU_UPD];
+    };
+    __shared__ Stage st_all[ILU_WAVES];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * ILU_WAVES + wv;
+    if (r >= nrows) return;
+    Stage &S = st_all[wv];
+    const int i = rows[r], p0 = ip[i], len = ip[i + 1] - p0, nlow = diag[i] - p0;
+    auto wave_sync = [] { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); __builtin_amdgcn_wave_barrier(); };
+    // the row, and per lower entry k its pivot and where its upper part starts and ends
+    int cnt = 0;
+    if (lane < len)
+    {
+        S.cols[lane] = ix[p0 + lane];
+        S.vals[lane] = v[p0 + lane];
+    }
+    if (lane < nlow)
+    {
+        const int k = ix[p0 + lane], dk = diag[k];
+        S.udiag[lane] = dk;
+        S.ukk[lane] = v[dk];
+        cnt = ip[k + 1] - dk - 1;
+    }
+    // exclusive prefix of the upper parts' sizes over the lower entries
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane < nlow) S.uoff[lane] = incl - cnt;
+    if (lane == nlow - 1) S.uoff[nlow] = incl;
+    wave_sync();
+    const int total = nlow > 0 ? S.uoff[nlow] : 0;
+    // the upper parts, and each entry's place in row i past the lower entry it comes from (-1: none)
+    for (int q = lane; q < total; q += 64)
+    {
+        int lo = 0, hi = nlow - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.uoff[mid] <= q) lo = mid;
+            else hi = mid - 1;
+        }
+        const int at = S.udiag[lo] + 1 + (q - S.uoff[lo]);
+        S.ucol[q] = ix[at];
+        S.uval[q] = v[at];
+    }
+    wave_sync();
+    for (int q = lane; q < total; q += 64)
+    {
+        int lo = 0, hi = nlow - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.uoff[mid] <= q) lo = mid;
+            else hi = mid - 1;
+        }
+        const int c = S.ucol[q];
+        int a = lo + 1, b = len;
+        while (a < b)
+        {
+            const int mid = (a + b) >> 1;
+            if (S.cols[mid] < c) a = mid + 1;
+            else b = mid;
+        }
+        S.at[q] = static_cast<signed char>(a < len && S.cols[a] == c ? a : -1);
+    }
+    wave_sync();
+    // the eliminations, lower entry by lower entry
+    for (int t = 0; t < nlow; ++t)
+    {
+        const double l = S.vals[t] / S.ukk[t];
+        wave_sync();
+        if (lane == 0) S.vals[t] = l;
+        const int q = S.uoff[t] + lane;
+        if (q < S.uoff[t + 1])
+        {
+            const int j = S.at[q];
+            if (j >= 0) S.vals[j] = S.vals[j] - l * S.uval[q];
+        }
+        wave_sync();
+    }
+    if (lane < len) v[p0 + lane] = S.vals[lane];
+    if (lane == 0 && S.vals[nlow] == 0.0) atomicMin(bad_row, i);
+}
+
+// the smallest pivot u_ii (a minimum: any order of the comparisons gives the same value)
+__global__ __launch_bounds__(1024) void k_min_pivot(int n, const int *__restrict__ diag, const double *__restrict__ v,
+                                                    double *out)
+{
+    __shared__ double part[1024];
+    double m = __builtin_inf();
+    for (int i = threadIdx.x; i < n; i += 1024) m = fmin(m, v[diag[i]]);
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1)
+    {
+        if (int(threadIdx.x) < w) part[threadIdx.x] = fmin(part[threadIdx.x], part[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
 // z = U^{-1} L^{-1} r (precond.hpp:189-205).
 __global__ __launch_bounds__(WG_SWEEP) void k_ilu0_apply(int nlevL, const int *__restrict__ lptrL,
                                                          const int *__restrict__ lrowsL, int nlevU,
@@ -1103,6 +1229,30 @@ void launch_ilu0_factor(int nlev, const int *lptr, const int *lrows, const int *
                         double *v, int *bad_row, hipStream_t s)
 {
     FEMD_LAUNCH(k_ilu0_factor, 1, WG_SWEEP, 0, s, nlev, lptr, lrows, ip, ix, diag, v, bad_row);
+}
+bool ilu0_levels_fit(int n, const int *ip, const int *ix, const int *diag)
+{
+    for (int i = 0; i < n; ++i)
+    {
+        if (ip[i + 1] - ip[i] > ILU_ROW) return false;
+        int64_t upd = 0;
+        for (int p = ip[i]; p < diag[i]; ++p) upd += ip[ix[p] + 1] - diag[ix[p]] - 1;
+        if (upd > ILU_UPD) return false;
+    }
+    return true;
+}
+void launch_ilu0_levels(int nlev, const int *lptr_host, const int *lrows, const int *ip, const int *ix, const int *diag,
+                        double *v, int *bad_row, hipStream_t s)
+{
+    for (int l = 0; l < nlev; ++l)
+    {
+        const int m = lptr_host[l + 1] - lptr_host[l];
+        if (m > 0) FEMD_LAUNCH(k_ilu0_level, (m + ILU_WAVES - 1) / ILU_WAVES, 64 * ILU_WAVES, 0, s, m, lrows + lptr_host[l], ip, ix, diag, v, bad_row);
+    }
+}
+void launch_min_pivot(int n, const int *diag, const double *v, double *out, hipStream_t s)
+{
+    FEMD_LAUNCH(k_min_pivot, 1, 1024, 0, s, n, diag, v, out);
 }
 void launch_ilu0_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, int nlevU, const int *lptrU,
                        const int *lrowsU, const int *ip, const int *ix, const int *diag, const double *v,

@@ -81,6 +81,12 @@ void launch_cg_pupdate(int64_t n, double *p, const double *r, const double *inv,
 // ---- triangular sweeps and factorization (sparse/precond.hpp), one workgroup, level by level ----
 void launch_ilu0_factor(int nlev, const int *lptr, const int *lrows, const int *ip, const int *ix, const int *diag,
                         double *v, int *bad_row, hipStream_t s);
+// the level-parallel factor (one launch per level, one wave per row) and whether a pattern fits it
+// (host arrays); the smallest pivot into out (one double on the device)
+bool ilu0_levels_fit(int n, const int *ip, const int *ix, const int *diag);
+void launch_ilu0_levels(int nlev, const int *lptr_host, const int *lrows, const int *ip, const int *ix, const int *diag,
+                        double *v, int *bad_row, hipStream_t s);
+void launch_min_pivot(int n, const int *diag, const double *v, double *out, hipStream_t s);
 void launch_ilu0_apply(int n, int nlevL, const int *lptrL, const int *lrowsL, int nlevU, const int *lptrU,
                        const int *lrowsU, const int *ip, const int *ix, const int *diag, const double *v,
                        const double *r, double *z, const int *stop, hipStream_t s);

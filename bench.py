@@ -47,6 +47,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "solves/sec at 1e-8 rel err vs FP64; SpMV HBM GB/s; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6          # MI355X dense FP64, matrix cores and vector FMA alike
 
 
 def log(*a):
@@ -70,6 +71,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--original-order", action="store_true",
                     help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
+    ap.add_argument("--exact-assembly", action="store_true",
+                    help="assemble the timed solves' matrix with the exact kernels instead of the FP64 matrix cores")
     ap.add_argument("--no-graph", action="store_true",
                     help="every solve with plain launches (for profilers that cannot follow graph replays)")
     args = ap.parse_args()
@@ -106,7 +109,9 @@ def main():
     zeros = np.zeros(n)
 
     def step(rtol=args.rtol):
+        prev = fd.assembly_mode("exact" if args.exact_assembly else "mfma")
         A = a_form.assemble(reuse=False)
+        fd.assembly_mode(prev)
         L.check(lib.femd_assemble_vector_dev(V._h, L_form.quad_degree, len(L_form.terms), aL, coeffs_L, b_dev, None))
         L.check(lib.femd_memcpy_htod(x_dev, zeros.ctypes.data, 8 * n))
         info = L.SolveInfo()
@@ -114,8 +119,10 @@ def main():
         L.check(lib.femd_stream_sync(None))
         return A, info
 
-    # the FP64 solution of the reference algorithm (Jacobi-CG, bit-identical to the CPU reference)
-    A, _ = step()
+    # the FP64 solution of the reference algorithm (exact assembly, Jacobi-CG: bit-identical to the
+    # CPU reference)
+    step()
+    A = a_form.assemble(reuse=False)
     b = L_form.assemble()
     with warnings.catch_warnings():
         # the recurrence reaches 1e-12 before the true residual does (FP64 attainable accuracy on
@@ -176,7 +183,30 @@ def main():
         asm()
     L.check(lib.femd_stream_sync(None))
     t_asm = (time.perf_counter() - ta) / 5
-    del keep_a
+    # the element-matrix kernel alone: exact (the reference's operation order, FP64 VALU) and on the
+    # FP64 matrix cores (one GEMM over the cells); the latter's matrix against the exact one
+    nt_a = len(a_form.terms)
+    t_el, f_el = np.zeros(1), np.zeros(1)
+    L.check(lib.femd_assembly_kernel_time(V._h, a_form.quad_degree, nt_a, aa, ab, coeffs_a, 0, 5, t_el, f_el))
+    t_exact = float(t_el[0])
+    L.check(lib.femd_assembly_kernel_time(V._h, a_form.quad_degree, nt_a, aa, ab, coeffs_a, 1, 10, t_el, f_el))
+    t_mf, fl_mf = float(t_el[0]), float(f_el[0])
+    prev_mode = fd.assembly_mode("mfma")
+    asm()
+    L.check(lib.femd_stream_sync(None))
+    ta = time.perf_counter()
+    for _ in range(5):
+        asm()
+    L.check(lib.femd_stream_sync(None))
+    t_asm_mf = (time.perf_counter() - ta) / 5
+    A_mf = a_form.assemble(reuse=False)
+    fd.assembly_mode(prev_mode)
+    asm()
+    A_ex = A64.data
+    dif = np.abs(A_mf.data - A_ex)
+    rows = np.maximum.reduceat(np.abs(A_ex), ip[:-1])
+    mf_err = float((np.maximum.reduceat(dif, ip[:-1]) / rows).max())
+    del A_mf, keep_a
     nloc = (V.degree + 1) * (V.degree + 2) // 2
     # the least an assembly moves: the values it writes (8 nnz), each cell's local-to-global map
     # (4 nloc per cell), the vertices (16 per point) and the cells' vertex indices (12 per cell)
@@ -184,6 +214,30 @@ def main():
     assembly = {"ms": round(1e3 * t_asm, 3), "algorithmic_bytes": B_asm, "gbs": round(B_asm / t_asm / 1e9, 1),
                 "frac": round(B_asm / t_asm / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"matrix assembly: {1e3 * t_asm:.1f} ms, {assembly['gbs']:.0f} GB/s of {B_asm} algorithmic bytes")
+    nq_c = C.c_int()
+    xi_b, eta_b, w_b = np.zeros(256), np.zeros(256), np.zeros(256)
+    L.check(lib.femd_quadrature(a_form.quad_degree, C.byref(nq_c), xi_b, eta_b, w_b))
+    nq_a = nq_c.value
+    codes = [np.asarray(pr.arrays()[0]).reshape(-1, 4) for pr in a_form.programs]
+    nvar = sum(1 for c in codes if not (c.shape[0] == 1 and c[0, 0] == 0))
+    nc = int(T.shape[0])
+    # the direct contraction's flops (every term, point, row and column: a multiply and an add),
+    # and the least the kernel moves: the element matrices it writes, the coefficient values at the
+    # points of each non-constant term and the cell geometry (6 doubles)
+    useful = 2.0 * nloc * nloc * nq_a * nt_a * nc
+    B_el = 8 * nloc * nloc * nc + 8 * nq_a * nc * nvar + 48 * nc
+    element = {"exact_us": round(1e3 * t_exact, 2), "mfma_us": round(1e3 * t_mf, 2),
+               "speedup": round(t_exact / t_mf, 2),
+               "mfma_issued_tflops": round(fl_mf / (t_mf * 1e-3) / 1e12, 2),
+               "mfma_frac_of_fp64_peak": round(fl_mf / (t_mf * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+               "direct_flops": useful, "mfma_issued_flops": fl_mf,
+               "exact_tflops": round(useful / (t_exact * 1e-3) / 1e12, 2),
+               "algorithmic_bytes": B_el, "mfma_gbs": round(B_el / (t_mf * 1e-3) / 1e9, 1),
+               "mfma_hbm_frac": round(B_el / (t_mf * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "assembly_ms_mfma": round(1e3 * t_asm_mf, 3), "row_rel_err_vs_exact": mf_err}
+    log(f"element kernel: exact {1e3 * t_exact:.1f} us ({element['exact_tflops']:.1f} TFLOP/s direct), matrix cores "
+        f"{1e3 * t_mf:.1f} us ({element['mfma_issued_tflops']:.1f} TFLOP/s issued, {element['mfma_gbs']:.0f} GB/s); "
+        f"assembly {1e3 * t_asm_mf:.2f} ms in mfma mode, values within {mf_err:.1e} of the exact ones (row-relative)")
 
     for _ in range(args.warmup):
         step()
@@ -280,9 +334,12 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64 assembly + f32 CG + f64 refinement", "data": "synthetic",
-        "config": {"workload": "P3 -div(kappa grad u) = f, seeded Delaunay unit square, FP64 assembly + FP32 "
-                               "Jacobi-CG with FP64 iterative refinement",
+        "vs_baseline": None,
+        "dtype": ("f64 assembly" if args.exact_assembly else "f64 MFMA assembly") + " + f32 CG + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": "P3 -div(kappa grad u) = f, seeded Delaunay unit square, FP64 "
+                               + ("" if args.exact_assembly else "matrix-core ") + "assembly + FP32 Jacobi-CG with "
+                               "FP64 iterative refinement",
                    "dofs": n, "nnz": nnz, "cells": int(T.shape[0]), "mesh_h": args.h, "min_angle_deg": round(min_ang, 2),
                    "rtol": args.rtol, "inner_tol": args.inner_tol, "parallelism": "single GPU"},
         "rel_err_vs_fp64_max": max(errs), "latency_ms_p50": round(1e3 * float(np.percentile(times, 50)), 3),
@@ -294,6 +351,7 @@ def main():
                            "solves_per_s": round(1.0 / t_fp64, 4), "speedup_of_value": round(value * t_fp64, 2)},
         "fp64_ilu0_rcm_gpu": ilu,
         "assembly": assembly,
+        "element_kernel": element,
         "spmv": {"fp64": {"us": round(t64 * 1e3, 2), "gbs": round(gbs(B64, t64), 1), "shape": variants["fp64"],
                           "flushed_us": round(c64 * 1e3, 2), "flushed_gbs": round(gbs(B64, c64), 1),
                           "flushed_frac": round(gbs(B64, c64) / HBM_PEAK_GBS, 4)},

@@ -126,6 +126,8 @@ _SIGS = {
     "femd_inner_timer_read": (_i, [C.POINTER(_i64), C.POINTER(_d)]),
     "femd_inner_timer_shape": (_i, [C.c_char_p, _i]),
     "femd_inner_order": (_i, [_i]),
+    "femd_assembly_mode": (_i, [_i]),
+    "femd_assembly_kernel_time": (_i, [_p, _i, _i, _ip, _ip, C.POINTER(Coefficient), _i, _i, _dp, _dp]),
     "femd_inner_order_pattern": (_i, [_i, _ip, _ip, _i, C.c_void_p, _ip, _ip, _ip, _ip]),
     "femd_value_dictionary": (_i, [_i]),
     "femd_batch_dictionary": (_i, [_p, _i, C.POINTER(_i64), C.POINTER(_i)]),

@@ -3,6 +3,7 @@
 #include "cg_engine.hpp"
 #include "sweep.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <limits>
@@ -270,6 +271,17 @@ void eval_program(femd_space_s *V, Quad &Q, const femd_coefficient &c, double *o
     HIPC(hipStreamSynchronize(s));           // code and consts are released on return
 }
 
+// Two coefficients with the same host values array, or the same program (instructions and
+// constants), have the same values at every point.
+bool same_coefficient(const femd_coefficient &x, const femd_coefficient &y)
+{
+    if (x.values || y.values) return x.values == y.values;
+    if (x.ninstr != y.ninstr || x.nconst != y.nconst) return false;
+    return std::equal(x.code, x.code + 4 * x.ninstr, y.code) && std::equal(x.consts, x.consts + x.nconst, y.consts);
+}
+
+// The terms' coefficients on the device: a constant stays a value; a program is evaluated at the
+// rule's points (once for terms whose coefficients are the same, which then share the array).
 Coeffs make_coeffs(femd_space_s *V, Quad &Q, int nt, const int *a, const int *b, const femd_coefficient *c, hipStream_t s)
 {
     if (nt < 1 || nt > MAX_TERMS) fail(FEMD_EINVAL, "assembly: 1.." + std::to_string(MAX_TERMS) + " terms per call");
@@ -288,6 +300,14 @@ Coeffs make_coeffs(femd_space_s *V, Quad &Q, int nt, const int *a, const int *b,
         C.T.c[t] = nullptr;
         C.T.cval[t] = 0.0;
         const femd_coefficient &k = c[t];
+        int twin = -1;
+        for (int u = 0; u < t && twin < 0; ++u)
+            if (C.T.c[u] && same_coefficient(k, c[u])) twin = u;
+        if (twin >= 0)
+        {
+            C.T.c[t] = C.T.c[twin];
+            continue;
+        }
         if (k.values)
         {
             C.bufs.push_back(upload(k.values, np, s));
@@ -1032,6 +1052,101 @@ void femd_csr_free(femd_csr A)
 // ---- assembly ----------------------------------------------------------------------------
 extern "C++" {
 namespace {
+std::atomic<int> g_assembly_mode{0};       // 0: exact VALU element kernels, 1: FP64 matrix cores
+
+// The plan of the matrix-core element kernel for one dx group (see mfma.hip): the reference-row
+// pairs (alpha, beta) the terms reach -- the value row for code 0, the two derivative rows for
+// codes 1 and 2 -- merged with their transposes when the terms are closed under swapping their
+// codes with the same coefficient; R[c, q][n] from the reference tables at the rule's points.
+// nullptr when the group is outside the kernel's sizes (the exact kernel runs instead).
+const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms, const int *a, const int *b,
+                          const femd_coefficient *coeffs, hipStream_t s)
+{
+    bool sym = true;
+    for (int t = 0; t < nterms && sym; ++t)
+    {
+        if (a[t] == b[t]) continue;
+        bool found = false;
+        for (int u = 0; u < nterms && !found; ++u)
+            found = a[u] == b[t] && b[u] == a[t] && same_coefficient(coeffs[t], coeffs[u]);
+        sym = found;
+    }
+    std::string key = std::to_string(degree) + (sym ? "s" : "n");
+    for (int t = 0; t < nterms; ++t) key += ":" + std::to_string(a[t]) + std::to_string(b[t]);
+    auto it = V->mfma_plans.find(key);
+    if (it != V->mfma_plans.end()) return it->second->rt.p ? it->second.get() : nullptr;
+
+    const int N = V->V.nloc, nq = Q.nq;
+    bool reach[3][3] = {};
+    for (int t = 0; t < nterms; ++t)
+        for (int al = 0; al < 3; ++al)
+            for (int be = 0; be < 3; ++be)
+                if ((a[t] == 0) == (al == 0) && (b[t] == 0) == (be == 0)) reach[al][be] = true;
+    std::vector<std::pair<int, int>> combos;
+    for (int al = 0; al < 3; ++al)
+        for (int be = 0; be < 3; ++be)
+            if (reach[al][be] && (!sym || al <= be)) combos.push_back({al, be});
+    auto P = std::make_unique<MfmaPlan>();
+    P->ncombo = static_cast<int>(combos.size());
+    P->spc = (nq + 3) / 4;                     // each combo's points padded to whole steps of four
+    P->ks = P->ncombo * P->spc;
+    P->nout = sym ? N * (N + 1) / 2 : N * N;
+    const int tiles = (P->nout + 15) / 16;
+    P->nt = std::min(tiles, 4);
+    P->ntile = (tiles + P->nt - 1) / P->nt * P->nt;
+    for (int c = 0; c < P->ncombo; ++c)
+    {
+        P->alpha |= unsigned(combos[c].first) << (2 * c);
+        P->beta |= unsigned(combos[c].second) << (2 * c);
+    }
+    const MfmaPlan *out = nullptr;
+    if (P->spc <= MFMA_SPCMAX)
+    {
+        // the output columns: (i, j) for i <= j (symmetric) or every (i, j), row-major
+        std::vector<int> oc(2 * size_t(16) * P->ntile, -1), ii, jj;
+        for (int i = 0; i < N; ++i)
+            for (int j = sym ? i : 0; j < N; ++j) { ii.push_back(i); jj.push_back(j); }
+        for (int n = 0; n < P->nout; ++n)
+        {
+            oc[2 * n] = ii[n] * N + jj[n];
+            oc[2 * n + 1] = sym && ii[n] != jj[n] ? jj[n] * N + ii[n] : -1;
+        }
+        const int npad = 16 * P->ntile;
+        std::vector<double> rt(size_t(4) * P->ks * npad, 0.0), tab(size_t(3) * N);
+        for (int q = 0; q < nq; ++q)
+        {
+            if (V->V.nv == 4) V->refq->tabulate(Q.xi[q], Q.eta[q], tab.data());
+            else V->ref->tabulate(Q.xi[q], Q.eta[q], tab.data());
+            for (int c = 0; c < P->ncombo; ++c)
+            {
+                const int al = combos[c].first, be = combos[c].second;
+                double *row = &rt[size_t(c * 4 * P->spc + q) * npad];
+                for (int n = 0; n < P->nout; ++n)
+                {
+                    const int i = ii[n], j = jj[n];
+                    double r = tab[al * N + i] * tab[be * N + j];
+                    if (sym && al != be) r += tab[be * N + i] * tab[al * N + j];
+                    row[n] = r;
+                }
+            }
+        }
+        P->rlds = mfma_table_bytes(P->ks, P->ntile) <= MFMA_LDS_MAX;
+        P->rt = upload(rt, s);
+        P->ocol = upload(oc, s);
+        HIPC(hipStreamSynchronize(s));
+        out = P.get();
+    }
+    V->mfma_plans[key] = std::move(P);
+    return out;
+}
+
+MfmaJob mfma_job(femd_space_s *V, Quad &Q, const MfmaPlan &P, double *E)
+{
+    return MfmaJob{V->V.ncells, Q.nq, V->V.nloc, P.ncombo, P.spc, P.ks, P.ntile, P.nout, P.alpha, P.beta,
+                   V->geom.as<double>(), Q.qw.as<double>(), Q.qxi.as<double>(), Q.qeta.as<double>(),
+                   P.rt.as<double>(), P.ocol.as<int2>(), E};
+}
+
 void assemble_matrix_dx(femd_space V, int quad_degree, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
                         femd_csr K, int mode)
 {
@@ -1045,7 +1160,14 @@ void assemble_matrix_dx(femd_space V, int quad_degree, int nterms, const int *a,
         Coeffs C = make_coeffs(Vs, Q, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
-        if (!launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
+        const MfmaPlan *mp = g_assembly_mode.load() == 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s) : nullptr;
+        bool done = false;
+        if (mp)
+        {
+            done = launch_elem_matrix_mfma(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mp->rlds, s);
+        }
+        if (!done &&
+            !launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
                                 Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s))
             fail(FEMD_EINVAL, "assemble_matrix: quadrature degree " + std::to_string(quad_degree) + " is too high for the element kernel");
         launch_gather_mode(Ks->pat->nnz, nullptr, Vs->mptr.as<int>(), Vs->mlist.as<int>(), E.as<double>(), Ks->v.as<double>(),
@@ -1072,6 +1194,13 @@ void assemble_vector_dx(femd_space V, int quad_degree, int nterms, const int *a,
 }
 } // namespace
 } // extern "C++"
+
+int femd_assembly_mode(int mode)
+{
+    const int prev = g_assembly_mode.load();
+    if (mode == 0 || mode == 1) g_assembly_mode.store(mode);
+    return prev;
+}
 
 int femd_assemble_matrix(femd_space V, int quad_degree, int nterms, const int *a, const int *b, const femd_coefficient *coeffs,
                          femd_csr K)
@@ -1585,6 +1714,49 @@ int femd_solve_refined(femd_csr A, const double *b, double *x, double tol, doubl
 }
 
 // ---- timing --------------------------------------------------------------------------------
+int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const int *a, const int *b,
+                              const femd_coefficient *coeffs, int mode, int reps, double *ms, double *mfma_flops)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        if (reps < 1 || (mode != 0 && mode != 1) || !ms) fail(FEMD_EINVAL, "assembly_kernel_time: bad arguments");
+        hipStream_t s = S(nullptr);
+        ensure_pattern(Vs, s);
+        Quad &Q = quad_of(Vs, quad_degree, s);
+        Coeffs C = make_coeffs(Vs, Q, nterms, a, b, coeffs, s);
+        const int n = Vs->V.nloc, nc = Vs->V.ncells;
+        DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
+        const MfmaPlan *mp = mode == 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s) : nullptr;
+        if (mode == 1 && !mp) fail(FEMD_EINVAL, "assembly_kernel_time: the group is outside the matrix-core kernel's sizes");
+        auto run = [&] {
+            bool ok;
+            if (mp)
+            {
+                ok = launch_elem_matrix_mfma(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mp->rlds, s);
+            }
+            else
+                ok = launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
+                                        Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s);
+            if (!ok) fail(FEMD_EINVAL, "assembly_kernel_time: no element kernel for this group");
+        };
+        run();
+        hipEvent_t e0, e1;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        HIPC(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) run();
+        HIPC(hipEventRecord(e1, s));
+        HIPC(hipEventSynchronize(e1));
+        float f = 0.0f;
+        HIPC(hipEventElapsedTime(&f, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = f / reps;
+        // the matrix-core work issued: 16 cells x 16 entries x 4 k (2048 flops) per instruction
+        if (mfma_flops) *mfma_flops = mp ? 2048.0 * double((nc + 15) / 16) * mp->ks * mp->ntile : 0.0;
+    });
+}
+
 int femd_event_timer_start(femd_stream st)
 {
     return api([&] {

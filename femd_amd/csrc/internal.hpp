@@ -198,6 +198,15 @@ struct Quad {
     DevBuf tab, qw, qxi, qeta;
 };
 
+// The matrix-core form of one dx group's element matrices (mfma.hip): its combos of reference
+// rows, the table R and the output columns, built once per (rule, codes, symmetry).
+struct MfmaPlan {
+    int ncombo = 0, spc = 0, ks = 0, ntile = 0, nout = 0, nt = 0;
+    unsigned alpha = 0, beta = 0;
+    bool rlds = false;
+    DevBuf rt, ocol;
+};
+
 } // namespace femd_amd
 
 // ------------------------------------------------------------------------------ the handles
@@ -214,6 +223,7 @@ struct femd_space_s {
     std::shared_ptr<femd_amd::DevPattern> pat;
     femd_amd::DevBuf mptr, mlist, vptr, vlist;
     std::map<int, std::unique_ptr<femd_amd::Quad>> quads;
+    std::map<std::string, std::unique_ptr<femd_amd::MfmaPlan>> mfma_plans;
     std::unique_ptr<femd_amd::RefElement> ref;     // triangles
     std::unique_ptr<femd_amd::RefQuad> refq;       // quadrilaterals
 };

@@ -25,6 +25,34 @@ bool launch_elem_matrix(int nv, int nloc, int nc, int nq, const double *geom, co
                         const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s);
 bool launch_elem_vector(int nv, int nloc, int nc, int nq, const double *geom, const double *tab, const double *qw,
                         const double *qxi, const double *qeta, const TermSet &T, double *out, hipStream_t s);
+// The matrix-core element kernel (mfma.hip): Ke[ij] = sum_k D_e[k] R[k][ij].  Step s = c spc + j of
+// the K loop holds combo c's points q = 4 j + 0..3 (points past nq padded with zero rows of R):
+// combo c pairs reference row alpha_c (bits 2c of alpha) on the test side with beta_c on the trial
+// side.  rt is R, (4 ks) x (16 ntile) doubles, ks = ncombo spc; ocol[n] the entries (i N + j)
+// computed column n goes to (.y = -1: none).
+constexpr int MFMA_SPCMAX = 8;               // steps per combo: rules of at most 32 points
+constexpr int MFMA_UMAX = 4;                 // distinct coefficient arrays per group
+constexpr size_t MFMA_LDS_MAX = 65536;       // R is staged in LDS up to this size
+struct MfmaJob {
+    int nc, nq, N, ncombo, spc, ks, ntile, nout;
+    unsigned alpha, beta;
+    const double *geom, *qw, *qxi, *qeta, *rt;
+    const int2 *ocol;
+    double *E;
+};
+// The terms as the kernel reads them: codes, the constant of a constant term (slot -1), or the
+// slot of its coefficient array among the nu distinct ones.
+struct MfmaTerms {
+    int nt, nu;
+    int a[MAX_TERMS], b[MAX_TERMS], slot[MAX_TERMS];
+    double cval[MAX_TERMS];
+    const double *cu[MFMA_UMAX];
+};
+size_t mfma_table_bytes(int ks, int ntile);
+// false: the rule, the coefficient count, the tile count or the LDS size is outside what the kernel
+// is built for (the exact kernel runs instead)
+bool launch_elem_matrix_mfma(int nv, const MfmaJob &J, const TermSet &T, int nt, bool rlds, hipStream_t s);
+
 void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s);
 // How a gather slot combines its element-order sum with the destination (k_gather).
 enum { GATHER_SET = 0, GATHER_SET_PZ = 1, GATHER_ADD = 2, GATHER_CONTINUE = 3 };

@@ -427,6 +427,21 @@ int femd_value_dictionary(int mode);
  * returns the previous mode.  Results differ between the two only by the rounding of the inner
  * iterations.  No counterpart in the reference. */
 int femd_inner_order(int mode);
+/* How femd_assemble_matrix / femd_assemble_matrix_add form a dx group's element matrices: mode 0
+ * (the default) with the exact kernels, in the reference's operation order
+ * (forms/assembly_2d.hpp:102-126), bit-identical to it; mode 1 on the FP64 matrix cores, as one
+ * GEMM over the cells, K_e[ij] = sum_k D_e[k] R[k][ij] with k running over (pair of reference
+ * rows, quadrature point) -- the same sums in another order, within a few units of the last
+ * place of the exact values (groups the kernel is not built for fall back to the exact kernels);
+ * -1 only queries.  Returns the previous mode.  Boundary (ds) groups and vectors always use the
+ * exact kernels.  No counterpart in the reference. */
+int femd_assembly_mode(int mode);
+/* Measurement: the average time (ms, HIP events on the library stream, `reps` launches after one
+ * untimed) of the element-matrix kernel alone for a dx group -- mode 0 the exact kernel, 1 the
+ * matrix-core one -- and (mfma_flops, may be NULL) the matrix-core flops one launch issues.  No
+ * counterpart in the reference. */
+int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const int *a, const int *b,
+                              const femd_coefficient *coeffs, int mode, int reps, double *ms, double *mfma_flops);
 /* The order femd_inner_order uses, computed on the host (no device needed): perm[k] (n) the
  * unknown at position k -- within every segment (row ranges segments[0..nseg], nseg 0: all rows),
  * the order a row-by-row scan first meets the unknowns as columns, unmet ones last -- and the

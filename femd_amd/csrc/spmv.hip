@@ -84,7 +84,7 @@ SpmvPlan make_plan(const DevPattern &P, int si, hipStream_t s)
     std::vector<uint16_t> lidx;
     spmv_tile_columns(P.h_ip.data(), P.h_ix.data(), rows, S.cap, uptr, ucol, lidx);
     p.uptr = upload(uptr, s);
-    if (ucol.empty()) ucol.push_back(0);
+    ucol.push_back(0);                                 // a valid index for a tile's clamped loads
     p.ucol = upload(ucol, s);
     lidx.resize(lidx.size() + 8, 0);                   // room for the last group's vector load
     p.lidx = upload(lidx, s);
@@ -380,7 +380,7 @@ void build_dict(const SpmvPlan &p, const void *v, int precision, int64_t nnz, Va
     h[size_t(nt)] = run;
     const size_t w = precision == FEMD_FP64 ? 8 : 4;
     d.doff = upload(h, s);
-    d.dval = DevBuf(w * size_t(std::max(run, 1)));
+    d.dval = DevBuf(w * (size_t(run) + 1));            // + 1: a valid index for a tile's clamped load
     d.vidx = DevBuf(size_t(nnz) + 8);
     if (precision == FEMD_FP64)
         hipLaunchKernelGGL(k_dict_write<double>, dim3(nt), dim3(DICT_THREADS), 0, s, blk, static_cast<const double *>(v),

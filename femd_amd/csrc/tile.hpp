@@ -97,14 +97,14 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
     }
     // the tile reads the value array: no dictionary, or (where the format allows it) too many values
     const bool raw = !VS::on || (VS::raw_tiles && nd == 0);
-    int rb = 0, re = 0;
-    typename Op::Own own{};
-    if (r < r1)
-    {
-        rb = ip[r] - p0;
-        re = ip[r + 1] - p0;
-        own = op.own(r);
-    }
+    // The loads below run on indices clamped into the arrays (a tile has at least one row; the
+    // column list and the dictionary carry one padding element) instead of under conditions: a
+    // load under a condition compiles to its own branch that waits for every load in flight (per
+    // lane or per wave alike), which turned the tile's loads into a chain of memory latencies.
+    const int rl = r < r1 ? r : r0;
+    int rb = ip[rl] - p0, re = ip[rl + 1] - p0;
+    typename Op::Own own = op.own(rl);
+    if (r >= r1) rb = re = 0;
     if (len > CAP)
     {
         // one row longer than the buffer: one lane, from global memory, in order
@@ -127,65 +127,103 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
     }
     // 1. distinct columns (and the tile's value dictionary) and entry loads
     const int u0 = td.u0, nu = td.u1 - td.u0;
+    // The first NUF slots of columns (and NGF groups of entries below) cover what a tile of finite-
+    // element rows holds (P3: about 480 distinct columns per 2048 entries), so they load without a
+    // condition; the rest only in the tile-wide branch of a tile that needs them.
+    constexpr int NUF = (NU + 2) / 3;
     int uc[NU];
 #pragma unroll
-    for (int i = 0; i < NU; ++i)
+    for (int i = 0; i < NUF; ++i)
     {
         const int u = static_cast<int>(threadIdx.x) + i * THREADS;
-        uc[i] = u < nu ? __builtin_nontemporal_load(ucol + u0 + u) : 0;
+        uc[i] = __builtin_nontemporal_load(ucol + u0 + (u < nu ? u : 0));
+    }
+    if (nu > NUF * THREADS)
+    {
+#pragma unroll
+        for (int i = NUF; i < NU; ++i)
+        {
+            const int u = static_cast<int>(threadIdx.x) + i * THREADS;
+            uc[i] = __builtin_nontemporal_load(ucol + u0 + (u < nu ? u : 0));
+        }
     }
     T dvv = 0;
-    if constexpr (VS::on)
-        if (static_cast<int>(threadIdx.x) < nd) dvv = vs.dval[d0 + threadIdx.x];
+    if constexpr (VS::on) dvv = vs.dval[d0 + (static_cast<int>(threadIdx.x) < nd ? static_cast<int>(threadIdx.x) : 0)];
     const int a0 = p0 & ~3, e = p0 + len;
     Vals4<T> vv[NGRP];
     u32 di[NGRP];
     u16x4 cc[NGRP];
-#pragma unroll
-    for (int g = 0; g < NGRP; ++g)
+    // whole groups of four: every group up to the tile's end lies inside the matrix's last whole
+    // group (groups past the end load the tile's first group again and are not used); only the tile
+    // holding the matrix's final partial group, or one without entries, loads entry by entry
+    const bool whole = len > 0 && int64_t(e) <= (nnz & ~int64_t(3));
+    if (whole)
     {
-        const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
-        if (a < e)
+        constexpr int NGF = CAP / (4 * THREADS) > 0 ? CAP / (4 * THREADS) : 1;
+        auto group = [&](int g) {
+            const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
+            const int ac = a < e ? a : a0;
+            if (raw) vv[g].load(v, ac);
+            else if constexpr (VS::on) di[g] = __builtin_nontemporal_load(reinterpret_cast<const u32 *>(vs.vidx + ac));
+            cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + ac));
+        };
+#pragma unroll
+        for (int g = 0; g < (NGF < NGRP ? NGF : NGRP); ++g) group(g);
+        if (e > a0 + 4 * NGF * THREADS)
         {
-            if (int64_t(a) + 4 <= nnz)
+#pragma unroll
+            for (int g = NGF; g < NGRP; ++g) group(g);
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int g = 0; g < NGRP; ++g)
+        {
+            const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
+            if (a < e)
             {
-                if (raw) vv[g].load(v, a);
-                else if constexpr (VS::on) di[g] = __builtin_nontemporal_load(reinterpret_cast<const u32 *>(vs.vidx + a));
-                cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
+                if (int64_t(a) + 4 <= nnz)
+                {
+                    if (raw) vv[g].load(v, a);
+                    else if constexpr (VS::on) di[g] = __builtin_nontemporal_load(reinterpret_cast<const u32 *>(vs.vidx + a));
+                    cc[g] = __builtin_nontemporal_load(reinterpret_cast<const u16x4 *>(lidx + a));
+                }
+                else
+                {
+                    if constexpr (VS::on) di[g] = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                    {
+                        const bool in = int64_t(a) + j < nnz;
+                        if (raw) vv[g].set(j, in ? v[a + j] : T(0));
+                        else if constexpr (VS::on) di[g] |= (in ? u32(vs.vidx[a + j]) : 0u) << (8 * j);
+                        cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
+                    }
+                }
             }
             else
             {
+                cc[g] = u16x4{0, 0, 0, 0};
                 if constexpr (VS::on) di[g] = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                {
-                    const bool in = int64_t(a) + j < nnz;
-                    if (raw) vv[g].set(j, in ? v[a + j] : T(0));
-                    else if constexpr (VS::on) di[g] |= (in ? u32(vs.vidx[a + j]) : 0u) << (8 * j);
-                    cc[g][j] = in ? lidx[a + j] : static_cast<unsigned short>(0);
-                }
             }
-        }
-        else
-        {
-            cc[g] = u16x4{0, 0, 0, 0};
-            if constexpr (VS::on) di[g] = 0;
         }
     }
     // every load above is in flight; the op may still decide that the tile has nothing to do
     if (!op.ready()) return typename Op::Part{};
     T xg[NU];
 #pragma unroll
-    for (int i = 0; i < NU; ++i)
+    for (int i = 0; i < NUF; ++i) xg[i] = op.gather(uc[i]);     // past nu: a valid column, not stored
+    if (nu > NUF * THREADS)
     {
-        const int u = static_cast<int>(threadIdx.x) + i * THREADS;
-        xg[i] = u < nu ? op.gather(uc[i]) : T(0);
+#pragma unroll
+        for (int i = NUF; i < NU; ++i) xg[i] = op.gather(uc[i]);
     }
 #pragma unroll
     for (int i = 0; i < NU; ++i)
     {
         const int u = static_cast<int>(threadIdx.x) + i * THREADS;
-        if (u < nu) xs[u] = xg[i];
+        if (u < nu) xs[u] = xg[i];       // slots past NUF hold values only when nu reaches them
     }
     if constexpr (VS::on)
         if (static_cast<int>(threadIdx.x) < nd) vs.dv[threadIdx.x] = dvv;

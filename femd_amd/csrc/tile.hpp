@@ -228,31 +228,47 @@ __device__ inline typename Op::Part tile_work(const TileDesc &td, int b, const i
     if constexpr (VS::on)
         if (static_cast<int>(threadIdx.x) < nd) vs.dv[threadIdx.x] = dvv;
     __syncthreads();
-    // 2. products
+    // 2. products: every gather from LDS first (positions masked into the buffer: an entry outside
+    //    the tile reads some slot and is not stored), then the stores of the tile's own entries
+    static_assert((CAP & (CAP - 1)) == 0, "CAP is a power of two");
+    T pv[NGRP][4];
+#pragma unroll
+    for (int g = 0; g < NGRP; ++g)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            T val;
+            if (raw) val = vv[g].get(j);
+            else if constexpr (VS::on) val = vs.dv[(di[g] >> (8 * j)) & 0xffu];
+            pv[g][j] = val * xs[cc[g][j] & (CAP - 1)];
+        }
 #pragma unroll
     for (int g = 0; g < NGRP; ++g)
     {
         const int a = a0 + 4 * (static_cast<int>(threadIdx.x) + g * THREADS);
-        if (a < e)
-        {
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-            {
-                const int k = a + j - p0;
-                T val;
-                if (raw) val = vv[g].get(j);
-                else if constexpr (VS::on) val = vs.dv[(di[g] >> (8 * j)) & 0xffu];
-                if (k >= 0 && k < len) prod[k] = val * xs[cc[g][j]];
-            }
+        for (int j = 0; j < 4; ++j)
+        {
+            const int k = a + j - p0;
+            if (k >= 0 && k < len) prod[k] = pv[g][j];
         }
     }
     __syncthreads();
-    // 3. row sums, left to right
+    // 3. row sums, left to right, four loads in flight at a time
     typename Op::Part part{};
     if (r < r1)
     {
         T s = 0;
-        for (int k = rb; k < re; ++k) s += prod[k];
+        int k = rb;
+        for (; k + 3 < re; k += 4)
+        {
+            const T t0 = prod[k], t1 = prod[k + 1], t2 = prod[k + 2], t3 = prod[k + 3];
+            s += t0;
+            s += t1;
+            s += t2;
+            s += t3;
+        }
+        for (; k < re; ++k) s += prod[k];
         part = op.store(r, s, own);
     }
     return part;

@@ -18,6 +18,9 @@ constexpr int BCHUNK = 1024;
 constexpr int RT = 256;            // threads of the per-system reductions
 constexpr int GRAPH_ITERS = 16;
 constexpr int64_t FUSE_ALPHA_MAX = int64_t(1) << 20;   // tiles x chunks of the largest system (8 MiB of extra reads)
+constexpr int UPDATE_WGS = 512;                          // one large system: update workgroups reading the tile partials
+                                                         // (C1: 408 ms per solve; 256: 425, 1024: 410)
+constexpr int64_t FUSE_ALPHA_RUN_MAX = int64_t(1) << 23; // ... and their reads at most (64 MiB)
 
 inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + BT - 1) / BT)); }
 
@@ -221,84 +224,97 @@ __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ cl
                                                   const T *P0, const T *P1, const T *__restrict__ q,
                                                   const float *__restrict__ dg, int odd, double *cparts,
                                                   const int *__restrict__ seg_tiles, const int *__restrict__ cptr,
-                                                  const double *tparts, CGSys *st)
+                                                  const double *tparts, CGSys *st, int nch, int per)
 {
     __shared__ double red[2 * (BT / 64)];
-    const int c = blockIdx.x, sys = cpid[c];
-    CGSys &S = st[sys];
-    if (!S.active) return;
-    double alpha = FUSE ? 0.0 : S.alpha;
-    if constexpr (FUSE)
-    {
-        double o[1];
-        const int t0 = seg_tiles[sys];
-        wg_fixed_sums<BT, 1>(tparts + t0, seg_tiles[sys + 1] - t0, red, o);
-        const bool lead = c == cptr[sys] && threadIdx.x == 0;
-        if (!(o[0] > 0.0))
-        {
-            if (lead) S.broke = 1;
-            return;
-        }
-        alpha = S.rr / o[0];
-        if (lead) S.alpha = alpha;
-    }
     const T *pnew = odd ? P0 : P1;
-    const T a = static_cast<T>(alpha);
-    double s = 0.0, t = 0.0;
-    const int64_t lo = clo[c], hi = clo[c + 1];
-    if ((lo & 3) == 0 && hi - lo == BCHUNK)
+    // chunks [c0, c1) of this workgroup (one, or with FUSE for one large system a short run, so that
+    // fewer workgroups read the tile partials)
+    const int c0 = blockIdx.x * per, c1 = min(c0 + per, nch);
+    int alpha_sys = -1;
+    double alpha = 0.0;
+    for (int c = c0; c < c1; ++c)
     {
-        // a whole aligned chunk: four consecutive rows per lane, vector loads and stores
-        const int64_t i = lo + 4 * static_cast<int64_t>(threadIdx.x);
-        Vec4<T> yv, pv, rv, qv;
-        yv.load(y + i); pv.load(pnew + i); rv.load(r + i); qv.load(q + i);
-        const float4 dv = *reinterpret_cast<const float4 *>(dg + i);
-        const float dd[4] = {dv.x, dv.y, dv.z, dv.w};
+        const int sys = cpid[c];
+        CGSys &S = st[sys];
+        if (!S.active) continue;
+        if constexpr (FUSE)
+        {
+            if (sys != alpha_sys)
+            {
+                double o[1];
+                const int t0 = seg_tiles[sys];
+                wg_fixed_sums<BT, 1>(tparts + t0, seg_tiles[sys + 1] - t0, red, o);
+                alpha_sys = sys;
+                if (!(o[0] > 0.0))
+                {
+                    if (c == cptr[sys] && threadIdx.x == 0) S.broke = 1;
+                    return;
+                }
+                alpha = S.rr / o[0];
+            }
+            if (c == cptr[sys] && threadIdx.x == 0) S.alpha = alpha;
+        }
+        else
+            alpha = S.alpha;
+        const T a = static_cast<T>(alpha);
+        double s = 0.0, t = 0.0;
+        const int64_t lo = clo[c], hi = clo[c + 1];
+        if ((lo & 3) == 0 && hi - lo == BCHUNK)
+        {
+            // a whole aligned chunk: four consecutive rows per lane, vector loads and stores
+            const int64_t i = lo + 4 * static_cast<int64_t>(threadIdx.x);
+            Vec4<T> yv, pv, rv, qv;
+            yv.load(y + i); pv.load(pnew + i); rv.load(r + i); qv.load(q + i);
+            const float4 dv = *reinterpret_cast<const float4 *>(dg + i);
+            const float dd[4] = {dv.x, dv.y, dv.z, dv.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-        {
-            yv.v[j] = fma(a, pv.v[j], yv.v[j]);
-            const T ri = fma(-a, qv.v[j], rv.v[j]);
-            rv.v[j] = ri;
-            const double e = static_cast<double>(ri) * static_cast<double>(ri);
-            s += e;
-            t += e * static_cast<double>(dd[j]);
+            for (int j = 0; j < 4; ++j)
+            {
+                yv.v[j] = fma(a, pv.v[j], yv.v[j]);
+                const T ri = fma(-a, qv.v[j], rv.v[j]);
+                rv.v[j] = ri;
+                const double e = static_cast<double>(ri) * static_cast<double>(ri);
+                s += e;
+                t += e * static_cast<double>(dd[j]);
+            }
+            yv.store(y + i);
+            rv.store(r + i);
         }
-        yv.store(y + i);
-        rv.store(r + i);
-    }
-    else
-        for (int64_t i = lo + threadIdx.x; i < hi; i += BT)
-        {
-            y[i] = fma(a, pnew[i], y[i]);
-            const T ri = fma(-a, q[i], r[i]);
-            r[i] = ri;
-            const double e = static_cast<double>(ri) * static_cast<double>(ri);
-            s += e;
-            t += e * static_cast<double>(dg[i]);
-        }
+        else
+            for (int64_t i = lo + threadIdx.x; i < hi; i += BT)
+            {
+                y[i] = fma(a, pnew[i], y[i]);
+                const T ri = fma(-a, q[i], r[i]);
+                r[i] = ri;
+                const double e = static_cast<double>(ri) * static_cast<double>(ri);
+                s += e;
+                t += e * static_cast<double>(dg[i]);
+            }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        s += __shfl_xor(s, off, 64);
-        t += __shfl_xor(t, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0)
-    {
-        red[2 * (threadIdx.x >> 6)] = s;
-        red[2 * (threadIdx.x >> 6) + 1] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        double a0 = 0.0, a1 = 0.0;
-        for (int w = 0; w < BT / 64; ++w)
+        for (int off = 32; off > 0; off >>= 1)
         {
-            a0 += red[2 * w];
-            a1 += red[2 * w + 1];
+            s += __shfl_xor(s, off, 64);
+            t += __shfl_xor(t, off, 64);
         }
-        cparts[2 * c] = a0;
-        cparts[2 * c + 1] = a1;
+        if ((threadIdx.x & 63) == 0)
+        {
+            red[2 * (threadIdx.x >> 6)] = s;
+            red[2 * (threadIdx.x >> 6) + 1] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            double a0 = 0.0, a1 = 0.0;
+            for (int w = 0; w < BT / 64; ++w)
+            {
+                a0 += red[2 * w];
+                a1 += red[2 * w + 1];
+            }
+            cparts[2 * c] = a0;
+            cparts[2 * c + 1] = a1;
+        }
+        __syncthreads();                      // red is reused by the next chunk
     }
 }
 
@@ -743,6 +759,19 @@ void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
         most = std::max(most, int64_t(plan->seg_tiles[size_t(p) + 1] - plan->seg_tiles[size_t(p)]) *
                                   int64_t(cptr_h[size_t(p) + 1] - cptr_h[size_t(p)]));
     fuse_alpha = most <= FUSE_ALPHA_MAX;
+    update_per = 1;
+    if (!fuse_alpha && np == 1)
+    {
+        // one large system: runs of chunks per workgroup, so that at most UPDATE_WGS workgroups
+        // read the tile partials
+        const int tiles = plan->seg_tiles[1] - plan->seg_tiles[0];
+        const int per = (nch + UPDATE_WGS - 1) / UPDATE_WGS;
+        if (int64_t(tiles) * ((nch + per - 1) / per) <= FUSE_ALPHA_RUN_MAX)
+        {
+            fuse_alpha = true;
+            update_per = per;
+        }
+    }
     const size_t N = size_t(std::max<int64_t>(n, 1)), w = prec == FEMD_FP64 ? 8 : 4;
     st = DevBuf(sizeof(CGSys) * size_t(std::max(np, 1)));
     tparts = DevBuf(8 * size_t(std::max(plan->ntiles, 1)));
@@ -838,15 +867,20 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
         else if (E.use_dict) launch_product<T, 1>(E, odd, s);
         else launch_product<T, 0>(E, odd, s);
         if (E.fuse_alpha)
-            hipLaunchKernelGGL((k_cg_update<T, true>), dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(),
-                               E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
-                               E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(), E.tparts.as<double>(), st);
+        {
+            const int per = E.update_per;
+            hipLaunchKernelGGL((k_cg_update<T, true>), dim3((E.nch + per - 1) / per), dim3(BT), 0, s, E.clo.as<int64_t>(),
+                               E.cpid.as<int>(), E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(),
+                               E.A->diagf.as<float>(), odd, E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(),
+                               E.tparts.as<double>(), st, E.nch, per);
+        }
         else
         {
             hipLaunchKernelGGL(k_cg_alpha, dim3(E.np), dim3(BT), 0, s, E.seg_tiles.as<int>(), E.tparts.as<double>(), st);
             hipLaunchKernelGGL((k_cg_update<T, false>), dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(),
                                E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
-                               E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(), E.tparts.as<double>(), st);
+                               E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(), E.tparts.as<double>(), st,
+                               E.nch, 1);
         }
         hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), odd, st);
         E.parity ^= 1;

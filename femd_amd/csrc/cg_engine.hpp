@@ -76,6 +76,7 @@ struct CGEngine {
     bool use_dict = false;              // the product reads the scaled values through their dictionary
     bool dict_raw = false;              // ... and some tiles, with too many values, read the values
     bool fuse_alpha = false;            // small systems: every update workgroup forms alpha itself
+    int update_per = 1;                 // chunks per update workgroup (> 1: one large system, alpha fused)
     int parity = 0;                     // the next iteration reads its direction from P1 (1) or P0 (0)
     DevBuf list, aoff;                  // the product's tile list (two int4 per tile) and its offsets
     int list_count = -1;

@@ -18,7 +18,7 @@ _lib.load()        # no CPU fallback: the HIP library must be built
 
 from .mesh import Mesh2D, QuadMesh, rectangle_mesh, mesh_from_arrays, rectangle_quad_mesh, quad_mesh_from_arrays
 from .spaces import LagrangeSpace, LagrangeSpace2D
-from .forms import (Const, Constant, Dx, FacetNormal, TestFunction, TrialFunction, abs_, as_vector, assembly_mode, cos, cosh, div, dot,
+from .forms import (Const, Constant, Dx, FacetNormal, Function, TestFunction, TrialFunction, abs_, as_vector, assembly_mode, cos, cosh, div, dot,
                     ds, dS, dx, exp, form, grad, inner, log, quadrature_degree, sech, set_quadrature_degree, sign, sin,
                     sinh, sqrt, tanh, x, y)
 

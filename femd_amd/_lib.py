@@ -31,10 +31,15 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
+class Field(C.Structure):
+    """femd_field: a known field an INPUT instruction reads (space, host coefficients, derivative code)."""
+    _fields_ = [("space", C.c_void_p), ("coefficients", C.POINTER(_d)), ("code", _i)]
+
+
 class Coefficient(C.Structure):
     """femd_coefficient: a register program (forms/integrand.hpp) or host values."""
     _fields_ = [("ninstr", _i), ("code", C.POINTER(_i)), ("nreg", _i), ("nconst", _i),
-                ("consts", C.POINTER(_d)), ("values", C.POINTER(_d))]
+                ("consts", C.POINTER(_d)), ("values", C.POINTER(_d)), ("ninputs", _i), ("inputs", C.POINTER(Field))]
 
 
 class SolveInfo(C.Structure):
@@ -187,8 +192,9 @@ def f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
-def coefficient(prog=None, values=None):
-    """A Coefficient struct and the arrays that keep its pointers alive."""
+def coefficient(prog=None, values=None, inputs=None):
+    """A Coefficient struct and the arrays that keep its pointers alive.  inputs: the known fields
+    the program reads, (space handle, coefficient vector, derivative code) per INPUT slot."""
     c = Coefficient()
     keep = []
     if values is not None:
@@ -205,15 +211,25 @@ def coefficient(prog=None, values=None):
     c.nreg = int(nreg)
     c.nconst = int(len(prog[2]))
     c.consts = consts.ctypes.data_as(C.POINTER(_d))
+    if inputs:
+        fields = (Field * len(inputs))()
+        for k, (space, vec, dcode) in enumerate(inputs):
+            v = f64(vec).reshape(-1)
+            keep.append(v)
+            fields[k] = Field(space, v.ctypes.data_as(C.POINTER(_d)), int(dcode))
+        keep.append(fields)
+        c.ninputs = len(inputs)
+        c.inputs = C.cast(fields, C.POINTER(Field))
     return c, keep
 
 
 def coefficient_array(items):
-    """items: list of (prog, values); returns the ctypes array and the keep-alive list."""
+    """items: list of (prog, values) or (prog, values, inputs); returns the ctypes array and the
+    keep-alive list."""
     arr = (Coefficient * len(items))()
     keep = []
-    for k, (prog, values) in enumerate(items):
-        c, kk = coefficient(prog, values)
+    for k, item in enumerate(items):
+        c, kk = coefficient(*item)
         arr[k] = c
         keep += kk
     return arr, keep

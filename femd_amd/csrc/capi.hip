@@ -197,10 +197,13 @@ void check_program(const femd_coefficient &c, bool normals = false)
         const int op = c.code[4 * k], o = c.code[4 * k + 1], a = c.code[4 * k + 2], b = c.code[4 * k + 3];
         if (o < 0 || o >= c.nreg) fail(FEMD_EINVAL, "coefficient: register out of range");
         const bool unary = op >= 9 && op <= 19;
-        if (((op == 3 || op == 4) && !normals) || op == 5 || op < 0 || op > 19)
+        if (((op == 3 || op == 4) && !normals) || (op == 5 && normals) || op < 0 || op > 19)
             fail(FEMD_EINVAL, "coefficient: instruction " + std::to_string(op) +
-                                  (op == 5 ? " (a known field) is not supported on the GPU"
+                                  (op == 5 ? " (a known field) is supported in dx terms only"
                                            : " (a facet normal) needs a ds term"));
+        if (op == 5 && (a < 0 || a >= c.ninputs || !c.inputs))
+            fail(FEMD_EINVAL, "coefficient: input " + std::to_string(a) + " out of range (" + std::to_string(c.ninputs) +
+                                  " known fields given)");
         if (op == 0 && (a < 0 || a >= c.nconst)) fail(FEMD_EINVAL, "coefficient: constant out of range");
         if ((op == 6 || op == 7) && (a < 0 || a >= c.nreg || b < 0 || b >= c.nreg)) fail(FEMD_EINVAL, "coefficient: register out of range");
         if (op == 8 && (a < 0 || a >= c.nreg || b < 0 || b >= c.nconst)) fail(FEMD_EINVAL, "coefficient: pow operands out of range");
@@ -258,17 +261,48 @@ std::vector<int> compact_registers(const femd_coefficient &c, int &nreg_out)
     return code;
 }
 
-void eval_program(femd_space_s *V, Quad &Q, const femd_coefficient &c, double *out, hipStream_t s)
+// The known fields a program reads, at the rule's points of V's cells (one device array each).
+std::vector<DevBuf> eval_inputs(femd_space_s *V, int degree, const femd_coefficient &c, InputSet &in, hipStream_t s)
+{
+    if (c.ninputs > MAX_INPUTS) fail(FEMD_EINVAL, "coefficient: at most " + std::to_string(MAX_INPUTS) + " known fields");
+    std::vector<DevBuf> out;
+    in = InputSet{};
+    for (int i = 0; i < c.ninputs; ++i)
+    {
+        const femd_field &f = c.inputs[i];
+        femd_space_s *W = f.space;
+        if (!W || !f.coefficients) fail(FEMD_EINVAL, "known field: null space or coefficients");
+        if (W->V.ncells != V->V.ncells || W->V.nv != V->V.nv)
+            fail(FEMD_EINVAL, "known field: its space is not on the form's cells");
+        if (f.code < 0 || f.code > 2) fail(FEMD_EINVAL, "derivative code is 0 (value), 1 (d/dx) or 2 (d/dy)");
+        ensure_geom(W, s);
+        if (!W->cell_dof_dev.p) W->cell_dof_dev = upload(W->V.cell_dof, s);
+        Quad &QW = quad_of(W, degree, s);
+        DevBuf u = upload(f.coefficients, size_t(std::max(W->V.dim, 1)), s);
+        out.emplace_back(8 * std::max<size_t>(size_t(V->V.ncells) * QW.nq, 1));
+        if (!launch_eval_field(W->V.nv, W->V.nloc, W->V.ncells, QW.nq, W->cell_dof_dev.as<int>(), u.as<double>(),
+                               QW.tab.as<double>(), W->geom.as<double>(), QW.qxi.as<double>(), QW.qeta.as<double>(), f.code,
+                               out.back().as<double>(), s))
+            fail(FEMD_EINVAL, "known field: no kernel for its local size");
+        in.p[i] = out.back().as<double>();
+        HIPC(hipStreamSynchronize(s));       // u is released on return
+    }
+    return out;
+}
+
+void eval_program(femd_space_s *V, Quad &Q, int degree, const femd_coefficient &c, double *out, hipStream_t s)
 {
     check_program(c);
     int nreg = 0;
     std::vector<int> compact = compact_registers(c, nreg);
     if (nreg > MAX_REGS) fail(FEMD_EINVAL, "coefficient: the program needs more than " + std::to_string(MAX_REGS) + " live registers");
+    InputSet in;
+    std::vector<DevBuf> fields = eval_inputs(V, degree, c, in, s);
     DevBuf code = upload(compact, s);
     DevBuf consts = upload(c.consts, static_cast<size_t>(std::max(c.nconst, 0)), s);
     launch_coefficient(V->V.ncells, Q.nq, V->V.nv, V->geom.as<double>(), Q.qxi.as<double>(), Q.qeta.as<double>(), c.ninstr,
-                       code.as<int>(), nreg, consts.as<double>(), out, s);
-    HIPC(hipStreamSynchronize(s));           // code and consts are released on return
+                       code.as<int>(), nreg, consts.as<double>(), in, out, s);
+    HIPC(hipStreamSynchronize(s));           // code, consts and the field values are released on return
 }
 
 // Two coefficients with the same host values array, or the same program (instructions and
@@ -276,13 +310,18 @@ void eval_program(femd_space_s *V, Quad &Q, const femd_coefficient &c, double *o
 bool same_coefficient(const femd_coefficient &x, const femd_coefficient &y)
 {
     if (x.values || y.values) return x.values == y.values;
-    if (x.ninstr != y.ninstr || x.nconst != y.nconst) return false;
+    if (x.ninstr != y.ninstr || x.nconst != y.nconst || x.ninputs != y.ninputs) return false;
+    for (int i = 0; i < x.ninputs; ++i)
+        if (x.inputs[i].space != y.inputs[i].space || x.inputs[i].coefficients != y.inputs[i].coefficients ||
+            x.inputs[i].code != y.inputs[i].code)
+            return false;
     return std::equal(x.code, x.code + 4 * x.ninstr, y.code) && std::equal(x.consts, x.consts + x.nconst, y.consts);
 }
 
 // The terms' coefficients on the device: a constant stays a value; a program is evaluated at the
 // rule's points (once for terms whose coefficients are the same, which then share the array).
-Coeffs make_coeffs(femd_space_s *V, Quad &Q, int nt, const int *a, const int *b, const femd_coefficient *c, hipStream_t s)
+Coeffs make_coeffs(femd_space_s *V, Quad &Q, int degree, int nt, const int *a, const int *b, const femd_coefficient *c,
+                   hipStream_t s)
 {
     if (nt < 1 || nt > MAX_TERMS) fail(FEMD_EINVAL, "assembly: 1.." + std::to_string(MAX_TERMS) + " terms per call");
     // the element kernels stage the basis table and the weights in 64 KiB of LDS
@@ -321,7 +360,7 @@ Coeffs make_coeffs(femd_space_s *V, Quad &Q, int nt, const int *a, const int *b,
         else
         {
             C.bufs.emplace_back(np * sizeof(double));
-            eval_program(V, Q, k, C.bufs.back().as<double>(), s);
+            eval_program(V, Q, degree, k, C.bufs.back().as<double>(), s);
             C.T.c[t] = C.bufs.back().as<double>();
         }
     }
@@ -1157,7 +1196,7 @@ void assemble_matrix_dx(femd_space V, int quad_degree, int nterms, const int *a,
         ensure_pattern(Vs, s);
         if (Ks->pat != Vs->pat) fail(FEMD_EPATTERN, "assemble_matrix: the matrix is not on the space's pattern");
         Quad &Q = quad_of(Vs, quad_degree, s);
-        Coeffs C = make_coeffs(Vs, Q, nterms, a, b, coeffs, s);
+        Coeffs C = make_coeffs(Vs, Q, quad_degree, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
         const MfmaPlan *mp = g_assembly_mode.load() == 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s) : nullptr;
@@ -1183,7 +1222,7 @@ void assemble_vector_dx(femd_space V, int quad_degree, int nterms, const int *a,
     femd_space_s *Vs = check_space(V);
     ensure_pattern(Vs, s);
     Quad &Q = quad_of(Vs, quad_degree, s);
-    Coeffs C = make_coeffs(Vs, Q, nterms, a, nullptr, coeffs, s);
+    Coeffs C = make_coeffs(Vs, Q, quad_degree, nterms, a, nullptr, coeffs, s);
     const int n = Vs->V.nloc, nc = Vs->V.ncells;
     DevBuf F(8 * std::max<size_t>(size_t(nc) * n, 1));
     if (!launch_elem_vector(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
@@ -1257,7 +1296,7 @@ int femd_eval_coefficient(femd_space V, int quad_degree, const femd_coefficient 
         Quad &Q = quad_of(Vs, quad_degree, s);
         const size_t np = size_t(Vs->V.ncells) * Q.nq;
         DevBuf d(8 * std::max<size_t>(np, 1));
-        eval_program(Vs, Q, *c, d.as<double>(), s);
+        eval_program(Vs, Q, quad_degree, *c, d.as<double>(), s);
         if (np) HIPC(hipMemcpyAsync(out, d.p, 8 * np, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
     });
@@ -1723,7 +1762,7 @@ int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const i
         hipStream_t s = S(nullptr);
         ensure_pattern(Vs, s);
         Quad &Q = quad_of(Vs, quad_degree, s);
-        Coeffs C = make_coeffs(Vs, Q, nterms, a, b, coeffs, s);
+        Coeffs C = make_coeffs(Vs, Q, quad_degree, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
         const MfmaPlan *mp = mode == 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s) : nullptr;

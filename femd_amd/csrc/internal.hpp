@@ -218,6 +218,7 @@ struct femd_space_s {
     femd_amd::MeshData M;                          // the mesh it was built on (its boundary facets)
     femd_amd::SpaceData V;
     femd_amd::DevBuf geom;
+    femd_amd::DevBuf cell_dof_dev;                 // cell_dof on the device (known fields on this space)
     femd_amd::PatternData P;
     bool have_host_pattern = false, have_pattern = false;
     std::shared_ptr<femd_amd::DevPattern> pat;

@@ -334,7 +334,7 @@ template <> struct RegFile<true> {
 
 template <class RF>
 __device__ inline double run_program(double x, double y, double nx, double ny, int ninstr, const int *__restrict__ code,
-                                     const double *__restrict__ consts, RF &reg)
+                                     const double *__restrict__ consts, RF &reg, const InputSet *in = nullptr, int64_t pt = 0)
 {
     int res = 0;
     for (int s = 0; s < ninstr; ++s)
@@ -348,6 +348,7 @@ __device__ inline double run_program(double x, double y, double nx, double ny, i
         case 2: v = y; break;
         case 3: v = (a < 0 ? -1.0 : 1.0) * nx; break;
         case 4: v = (a < 0 ? -1.0 : 1.0) * ny; break;
+        case 5: v = in->p[a][pt]; break;
         case 6: v = reg[a] + reg[b]; break;
         case 7: v = reg[a] * reg[b]; break;
         case 8:
@@ -393,7 +394,7 @@ template <bool LDS>
 __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, const double *__restrict__ geom,
                                                      const double *__restrict__ qxi, const double *__restrict__ qeta,
                                                      int ninstr, const int *__restrict__ code, int nreg,
-                                                     const double *__restrict__ consts, double *__restrict__ out)
+                                                     const double *__restrict__ consts, InputSet in, double *__restrict__ out)
 {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= static_cast<int64_t>(nc) * nq) return;
@@ -418,13 +419,51 @@ __global__ __launch_bounds__(TPB) void k_coefficient(int nc, int nq, int nv, con
     {
         extern __shared__ double sh[];
         RegFile<true> rf{sh + threadIdx.x};
-        out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts, rf);
+        out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts, rf, &in, k);
     }
     else
     {
         RegFile<false> rf;
-        out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts, rf);
+        out[k] = run_program(x, y, 0.0, 0.0, ninstr, code, consts, rf, &in, k);
     }
+}
+
+// One thread per (cell e, point q): a known field's value or x / y derivative there, the sums of
+// Cache2D::eval_multi (fe/cache_2d.hpp:245-288) in its order: the cell's coefficients gathered
+// (eliminated nodes read 0), then over the local functions l, value R[l] loc[l], or the row
+// a R_xi[l] + b R_eta[l] (J^{-1}'s column for x or y, as the cache forms it) times loc[l].
+template <int N, int NV>
+__global__ __launch_bounds__(TPB) void k_eval_field(int nc, int nq, const int *__restrict__ cell_dof,
+                                                    const double *__restrict__ u, const double *__restrict__ tab,
+                                                    const double *__restrict__ geom, const double *__restrict__ qxi,
+                                                    const double *__restrict__ qeta, int code, double *__restrict__ out)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<int64_t>(nc) * nq) return;
+    const int e = static_cast<int>(k / nq), q = static_cast<int>(k % nq);
+    const int *d = cell_dof + static_cast<int64_t>(e) * N;
+    double loc[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) loc[l] = d[l] >= 0 ? u[d[l]] : 0.0;
+    const double *R = tab + static_cast<int64_t>(q) * 3 * N;
+    double s = 0.0;
+    if (code == 0)
+    {
+#pragma unroll
+        for (int l = 0; l < N; ++l) s += R[l] * loc[l];
+    }
+    else
+    {
+        const CellJ cj = cell_jacobian<NV>(geom + (NV == 4 ? 8 : 6) * static_cast<int64_t>(e), qxi[q], qeta[q]);
+        const double a = code == 1 ? cj.i0 : cj.i1, b = code == 1 ? cj.i2 : cj.i3;
+#pragma unroll
+        for (int l = 0; l < N; ++l)
+        {
+            const double r = a * R[N + l] + b * R[2 * N + l];
+            s += r * loc[l];
+        }
+    }
+    out[k] = s;
 }
 
 // ============================================================================ SpMV
@@ -1123,7 +1162,7 @@ void launch_coefficient_pts(int64_t np, const double *x, const double *y, const 
 }
 
 void launch_coefficient(int nc, int nq, int nv, const double *geom, const double *qxi, const double *qeta, int ninstr,
-                        const int *code, int nreg, const double *consts, double *out, hipStream_t s)
+                        const int *code, int nreg, const double *consts, const InputSet &in, double *out, hipStream_t s)
 {
     const int64_t m = static_cast<int64_t>(nc) * nq;
     if (m <= 0) return;
@@ -1131,10 +1170,35 @@ void launch_coefficient(int nc, int nq, int nv, const double *geom, const double
     // a larger one in scratch memory; the arithmetic is the same
     if (nreg >= 1 && nreg <= 16)
         FEMD_LAUNCH(k_coefficient<true>, grid_for(m), TPB, size_t(nreg) * TPB * sizeof(double), s, nc, nq, nv, geom, qxi,
-                    qeta, ninstr, code, nreg, consts, out);
+                    qeta, ninstr, code, nreg, consts, in, out);
     else
-        FEMD_LAUNCH(k_coefficient<false>, grid_for(m), TPB, 0, s, nc, nq, nv, geom, qxi, qeta, ninstr, code, nreg, consts, out);
+        FEMD_LAUNCH(k_coefficient<false>, grid_for(m), TPB, 0, s, nc, nq, nv, geom, qxi, qeta, ninstr, code, nreg, consts, in,
+                    out);
 }
+
+#define FIELD_CASE(N, NV)                                                                                                  \
+    case N:                                                                                                                \
+        FEMD_LAUNCH((k_eval_field<N, NV>), grid_for(m), TPB, 0, s, nc, nq, cell_dof, u, tab, geom, qxi, qeta, code, out); \
+        return true;
+
+bool launch_eval_field(int nv, int nloc, int nc, int nq, const int *cell_dof, const double *u, const double *tab,
+                       const double *geom, const double *qxi, const double *qeta, int code, double *out, hipStream_t s)
+{
+    const int64_t m = static_cast<int64_t>(nc) * nq;
+    if (m <= 0) return true;
+    if (nv == 4)
+        switch (nloc)
+        {
+            FIELD_CASE(4, 4) FIELD_CASE(9, 4) FIELD_CASE(16, 4) FIELD_CASE(25, 4)
+        default: return false;
+        }
+    switch (nloc)
+    {
+        FIELD_CASE(3, 3) FIELD_CASE(6, 3) FIELD_CASE(10, 3) FIELD_CASE(15, 3) FIELD_CASE(21, 3)
+    default: return false;
+    }
+}
+#undef FIELD_CASE
 
 void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,
                    double *y, const int *stop, hipStream_t s)

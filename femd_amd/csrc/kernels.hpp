@@ -11,6 +11,13 @@ constexpr int CHUNK = 4096;                // the reference's reduction chunk (u
 constexpr int MAX_TERMS = 8;
 constexpr int MAX_REGS = 64;
 
+constexpr int MAX_INPUTS = 8;
+
+// The known fields a coefficient program reads (INPUT a: p[a][cell * nq + q]).
+struct InputSet {
+    const double *p[MAX_INPUTS];
+};
+
 // Coefficients of up to MAX_TERMS terms: a device array [cell * nq + q] or one constant.
 struct TermSet {
     int nt;
@@ -67,7 +74,13 @@ bool launch_facet_vector(int nloc, int ne, int nq, const int *var, const double 
 void launch_coefficient_pts(int64_t np, const double *x, const double *y, const double *nx, const double *ny, int ninstr,
                             const int *code, const double *consts, double *out, hipStream_t s);
 void launch_coefficient(int nc, int nq, int nv, const double *geom, const double *qxi, const double *qeta, int ninstr,
-                        const int *code, int nreg, const double *consts, double *out, hipStream_t s);
+                        const int *code, int nreg, const double *consts, const InputSet &in, double *out, hipStream_t s);
+// A known field (coefficients u, the cell_dof of its space, -1 eliminated) at the nq points of every
+// cell: its value (code 0) or x / y derivative (1 / 2), Cache2D::eval_multi's sums
+// (fe/cache_2d.hpp:245-288).  tab: the field space's reference table at the points.  false: no
+// kernel for this local size.
+bool launch_eval_field(int nv, int nloc, int nc, int nq, const int *cell_dof, const double *u, const double *tab,
+                       const double *geom, const double *qxi, const double *qeta, int code, double *out, hipStream_t s);
 
 // ---- SpMV -----------------------------------------------------------------------------------
 void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,

@@ -10,8 +10,8 @@ degree its rules infer (forms.py:2443-2500); each coefficient becomes a register
 Integrand instruction set (forms/integrand.hpp:33-46) in the reference's emission order
 (forms.py:1822-1909), which the GPU evaluates at the quadrature points.  Because the expression
 trees have the reference's shapes, the assembled values are the reference's bit for bit.  Known
-fields inside coefficients, dS measures and 1D meshes are outside this path and raise
-NotImplementedError.
+fields (Function) enter coefficients of dx terms through INPUT instructions.  dS measures, known
+fields on ds terms and 1D meshes are outside this path and raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -161,6 +161,96 @@ def TestFunction(V):
 
 def TrialFunction(V):
     return Argument("trial", V)
+
+
+def _deriv_code(deriv, what):
+    dx_, dy_ = deriv
+    if dx_ + dy_ > 1:
+        raise ValueError(f"{what}: derivatives of order {dx_ + dy_} are not available on a C^0 Lagrange space")
+    return 0 if dx_ + dy_ == 0 else (1 if dx_ else 2)
+
+
+class Field(Expr):
+    """A known field, or one of its first derivatives, inside a coefficient: the Function whose
+    coefficient vector is read and a partial-derivative multi-index.  The program reads it through
+    an INPUT instruction (forms/integrand.hpp:39); the GPU evaluates it at the quadrature points as
+    Cache2D::eval_multi does (fe/cache_2d.hpp:245-288)."""
+    def __init__(self, owner, deriv=(0, 0)):
+        self._owner, self.deriv = owner, tuple(deriv)
+
+    @property
+    def owner(self):
+        return self._owner
+
+    @property
+    def space(self):
+        return self.owner.space
+
+    def slot(self):
+        """What an INPUT slot stands for: the vector read and the derivative."""
+        return (id(self.owner), self.deriv)
+
+    def code(self):
+        return _deriv_code(self.deriv, repr(self))
+
+    def __repr__(self):
+        dx_, dy_ = self.deriv
+        name = self.owner.name
+        return name if dx_ + dy_ == 0 else f"d{name}/{'dx' * dx_}{'dy' * dy_}"
+
+
+class Function(Field):
+    """A known field in a Lagrange space that carries its coefficient vector (V.dim values;
+    python/femd/forms.py:806-866).  Inside a form it is read when the form is assembled, so a form
+    with a Function is assembled anew each time.
+
+        u = fd.Function(V, "u")           # zero
+        u.vector[:] = ...  /  u.interpolate(f)  /  u.assign(w)
+        form(u * dot(grad(w), grad(v)) * dx)"""
+    def __init__(self, V, name=None, vector=None):
+        if isinstance(name, (np.ndarray, list, tuple)) and vector is None:
+            name, vector = None, name
+        self._space = V
+        self.name = "w" if name is None else str(name)
+        self._vector = np.zeros(V.dim) if vector is None else self._checked(vector)
+        super().__init__(None, (0, 0))
+
+    @property
+    def owner(self):
+        # a Function reads its own vector (held as a property: an attribute would be a reference cycle)
+        return self
+
+    @property
+    def space(self):
+        return self._space
+
+    def _checked(self, a):
+        v = np.array(a, dtype=np.float64).reshape(-1)
+        if v.shape != (self._space.dim,):
+            raise ValueError(f"Function: vector length {v.size} != V.dim = {self._space.dim}")
+        return v
+
+    @property
+    def vector(self):
+        return self._vector
+
+    @vector.setter
+    def vector(self, a):
+        self._vector = self._checked(a)
+
+    def __array__(self, dtype=None, copy=None):
+        return self._vector if dtype is None else self._vector.astype(dtype)
+
+    def assign(self, other):
+        """Copy another Function's (or an array's) coefficients into this one."""
+        self.vector = other.vector if isinstance(other, Function) else other
+        return self
+
+    def interpolate(self, f):
+        """Nodal interpolation of f (a callable of x, y, or an expression in fd.x, fd.y) onto the
+        space's free nodes."""
+        self.vector = self._space.interpolate(f)
+        return self
 
 
 def _as_expr(o):
@@ -332,8 +422,16 @@ def _d_argument(e, i):
     return Argument(e.kind, e.space, tuple(d))
 
 
+def _d_field(e, i):
+    d = list(e.deriv)
+    d[i] += 1
+    return Field(e.owner, tuple(d))
+
+
 _DERIVATIVE = {
     Argument: _d_argument,
+    Field: _d_field,
+    Function: _d_field,
     Sum: lambda e, i: Sum([Dx(t, i) for t in e.terms]),
     Prod: _d_product,
     Const: lambda e, i: Const(0.0),
@@ -567,6 +665,8 @@ _POLY_DEGREE = {
     Normal: lambda e: 0,
     Spatial: lambda e: 1,
     Argument: _arg_degree,
+    Field: _arg_degree,
+    Function: _arg_degree,
     Sum: lambda e: _combine([poly_degree(t) for t in e.terms], max),
     Prod: lambda e: _combine([poly_degree(f) for f in e.factors], sum),
     Pow: lambda e: _pow_degree(e.n, poly_degree(e.base)),
@@ -591,9 +691,9 @@ def quad_degree_estimate(e, kmax):
     d = poly_degree(e)
     if d is not None:
         return d
-    if not _has_arguments(e):
+    if not _mentions(e, (Argument, Field)):          # data alone
         return kmax
-    if isinstance(e, Argument):
+    if isinstance(e, (Argument, Field)):
         return _arg_degree(e)
     if isinstance(e, Sum):
         return max((quad_degree_estimate(t, kmax) for t in e.terms), default=0)
@@ -646,13 +746,15 @@ class Program:
     Emission order (the reference's, python/femd/forms.py:1822-1909): a node's operands before
     the node; a sum (product) emits its first term, then each further term followed at once by an
     ADD (MUL) of the running value and that term; a power emits its base, then POW with the
-    exponent in the constant table; a node met again (the same object) reuses its register.
-    Every instruction writes a new register, so register k holds instruction k's result."""
+    exponent in the constant table; a node met again (the same object) reuses its register; a
+    known field emits INPUT with the slot of its (vector, derivative), slots numbered in order of
+    first use.  Every instruction writes a new register, so register k holds instruction k's result."""
     CONST, X, Y, NX, NY, INPUT, ADD, MUL, POW = range(9)
     _FUNCS = dict(_OPCODE)
 
     def __init__(self, expr):
         self.code, self.consts, self.memo = [], [], {}
+        self.inputs, self.input_slots = [], {}       # the fields read, one per (vector, derivative)
         self._compile(expr)
 
     @property
@@ -677,8 +779,12 @@ class Program:
             return self._put(self.X if e.index == 0 else self.Y)
         if isinstance(e, Normal):
             return self._put(self.NX if e.index == 0 else self.NY, 1)
-        if isinstance(e, Argument):
-            raise NotImplementedError("known fields inside coefficients are outside this path")
+        if isinstance(e, Field):
+            key = e.slot()
+            if key not in self.input_slots:
+                self.input_slots[key] = len(self.inputs)
+                self.inputs.append(e)
+            return self._put(self.INPUT, self.input_slots[key])
         raise NotImplementedError(f"cannot compile {e!r}")
 
     def _compile(self, root):
@@ -731,6 +837,10 @@ class Program:
             frame[2] = out
         else:
             frame[2] = self._put(self._FUNCS[node.name], reg)
+
+    def fields(self):
+        """The known fields the INPUT slots read: (space handle, coefficient vector, derivative code)."""
+        return [(f.space._h, f.owner.vector, f.code()) for f in self.inputs]
 
     def arrays(self):
         """(code int32 (n, 4), registers, constants float64 with the Constants' current values)."""
@@ -847,14 +957,24 @@ class Form:
             key = ("dx",) if t.measure.kind == "dx" else ("ds", t.measure.where)
             self.groups.setdefault(key, []).append(i)
         self.programs = [Program(t.coeff) for t in self.terms]
-        self.constant = not any(isinstance(c, Constant) for p in self.programs for c in p.consts)
+        for p in self.programs:
+            for f in p.inputs:
+                f.code()                                      # first derivatives at most
+                if f.space.mesh is not self.test_space.mesh:
+                    raise ValueError(f"{f!r}: a known field must live on the form's mesh")
+        for t, p in zip(self.terms, self.programs):
+            if p.inputs and t.measure.kind != "dx":
+                raise NotImplementedError("known fields in ds terms are outside this path")
+        # a form reading Constants or known fields is assembled anew each time
+        self.constant = not any(isinstance(c, Constant) for p in self.programs for c in p.consts) and \
+            not any(p.inputs for p in self.programs)
         self._result = None
 
     def _coefficients(self, idx=None):
         items = []
         for i in (range(len(self.terms)) if idx is None else idx):
             code, nreg, consts = self.programs[i].arrays()
-            items.append(((code, nreg, consts), None))
+            items.append(((code, nreg, consts), None, self.programs[i].fields()))
         return L.coefficient_array(items)
 
     def assemble(self, reuse: bool = True):

@@ -124,9 +124,11 @@ int femd_pattern(femd_space V, int64_t *nnz, int *indptr, int *indices);
 /* ---------------------------------------------------------------- coefficients
  * The coefficient of one term, at every quadrature point: either a register program in the
  * instruction set of forms/integrand.hpp:33-46 (4 ints per instruction: op, out, a, b; ops
- * CONST X Y ADD MUL POW SIN COS EXP LOG TANH SQRT SINH COSH SECH ABS SIGN; no INPUT, no
- * normals), evaluated on the GPU at the quadrature points, or, when `values` is not NULL,
- * ncells x nq host values laid out [cell * nq + q] (Cache2D order, fe/cache_2d.hpp:98-118). */
+ * CONST X Y INPUT ADD MUL POW SIN COS EXP LOG TANH SQRT SINH COSH SECH ABS SIGN; normals on ds
+ * terms only; INPUT a reads the known field inputs[a]), evaluated on the GPU at the quadrature
+ * points, or, when `values` is not NULL, ncells x nq host values laid out [cell * nq + q]
+ * (Cache2D order, fe/cache_2d.hpp:98-118). */
+typedef struct femd_field_s femd_field;
 typedef struct {
     int ninstr;
     const int *code;
@@ -134,7 +136,19 @@ typedef struct {
     int nconst;
     const double *consts;
     const double *values;
+    int ninputs;                 /* known fields the program reads (INPUT a reads inputs[a]) */
+    const femd_field *inputs;
 } femd_coefficient;
+/* A known field read by a program's INPUT instruction (forms/integrand.hpp:39; the reference's
+ * FormAssembler2D::evaluate_fields, forms/assembler_2d.hpp:336-366): a Lagrange function given by
+ * its coefficients (dim of its space, host) on a space over the same cells as the form's, read at
+ * the quadrature points as its value (code 0) or its x / y derivative (code 1 / 2), summed in the
+ * order of Cache2D::eval_multi (fe/cache_2d.hpp:245-288). */
+struct femd_field_s {
+    femd_space space;
+    const double *coefficients;
+    int code;
+};
 
 /* The program with its registers renamed by liveness, as the GPU runs it (host only; no
  * counterpart in the reference, whose Integrand keeps one register per node): code_out has

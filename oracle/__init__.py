@@ -218,6 +218,10 @@ class OracleProblem:
     def assemble_vector(self, degree, a, coeffs):
         return self.o._assemble_vector(self, degree, a, coeffs)
 
+    def eval_field(self, degree, c, codes):
+        """A known field with coefficients c at the cell points: (len(codes), ncells * nq)."""
+        return self.o._eval_field(self, degree, c, codes)
+
 
 class Oracle(_Lib):
     prefix = "orc_"
@@ -250,6 +254,8 @@ class Oracle(_Lib):
         self._amg = self._fn("assemble_matrix_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
         self._avg = self._fn("assemble_vector_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
         self._ien = self._fn("integrand_eval_n", _I, _I, _ip, _I, _dp, _L, _dp, _dp, _dp, _dp, _dp)
+        self._iei = self._fn("integrand_eval_in", _I, _I, _ip, _I, _dp, _L, _dp, _dp, _I, _dp, _dp)
+        self._evf = self._fn("eval_field", _I, _P, _I, _dp, _I, _ip, _dp)
 
     def quad_rectangle(self, nx, ny, k, dirichlet=(), all_exterior=True, x0=0.0, y0=0.0, x1=1.0, y1=1.0, equispaced=True):
         d = _i32(list(dirichlet)) if len(dirichlet) else np.zeros(1, dtype=np.int32)
@@ -378,6 +384,23 @@ class Oracle(_Lib):
                               nxs, nys, out))
         return out
 
+    def integrand_eval_in(self, code, nreg, consts, xs, ys, inputs):
+        """The program with inputs: inputs is (ninput, npts)."""
+        code = _i32(code).reshape(-1)
+        xs, ys = _f64(xs), _f64(ys)
+        inp = _f64(inputs).reshape(-1, xs.size) if len(inputs) else np.zeros((0, xs.size))
+        out = np.empty(xs.size)
+        self._check(self._iei(code.size // 4, code, int(nreg), _f64(consts) if len(consts) else np.zeros(1), xs.size, xs, ys,
+                              inp.shape[0], np.ascontiguousarray(inp).reshape(-1) if inp.size else np.zeros(1), out))
+        return out
+
+    def _eval_field(self, P, degree, c, codes):
+        npts = P.cache_points(degree)[0].size
+        codes = _i32(codes)
+        out = np.empty((codes.size, npts))
+        self._check(self._evf(P.h, int(degree), _f64(c), codes.size, codes, out.reshape(-1)))
+        return out
+
     def ilu0_factor(self, ip, ix, v, ordering="rcm"):
         ip, ix, v = _i32(ip), _i32(ix), _f64(v)
         n = ip.size - 1
@@ -421,6 +444,8 @@ class Ref(_Lib):
         self._amg = self._fn("assemble_matrix_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
         self._avg = self._fn("assemble_vector_groups", _I, _P, _I, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp)
         self._ien = self._fn("integrand_eval_n", _I, _I, _ip, _I, _I, _dp, _L, _dp, _dp, _dp, _dp, _dp)
+        self._iei = self._fn("integrand_eval_in", _I, _I, _ip, _I, _I, _dp, _L, _dp, _dp, _I, _dp, _dp)
+        self._evf = self._fn("eval_field", _I, _P, _I, _dp, _I, _ip, _dp)
 
     def quad_from_arrays(self, points, quads, segments=None, markers=None, k=1, dirichlet=(), all_exterior=True,
                          equispaced=True):
@@ -536,6 +561,18 @@ class Ref(_Lib):
         self._check(self._ien(code.size // 4, code, int(nreg), len(consts), _f64(consts) if len(consts) else np.zeros(1),
                               xs.size, xs, ys, nxs, nys, out))
         return out
+
+    def integrand_eval_in(self, code, nreg, consts, xs, ys, inputs):
+        code = _i32(code).reshape(-1)
+        xs, ys = _f64(xs), _f64(ys)
+        inp = _f64(inputs).reshape(-1, xs.size) if len(inputs) else np.zeros((0, xs.size))
+        out = np.empty(xs.size)
+        self._check(self._iei(code.size // 4, code, int(nreg), len(consts), _f64(consts) if len(consts) else np.zeros(1),
+                              xs.size, xs, ys, inp.shape[0], np.ascontiguousarray(inp).reshape(-1) if inp.size else np.zeros(1),
+                              out))
+        return out
+
+    _eval_field = Oracle._eval_field
 
     def asymmetry(self, ip, ix, v):
         ip, ix, v = _i32(ip), _i32(ix), _f64(v)

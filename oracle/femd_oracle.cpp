@@ -823,7 +823,7 @@ void pattern(const Problem &P, std::vector<int> &ip, std::vector<int> &ix)
 // ------------------------------------------------------------------------------- integrand
 // forms/integrand.hpp:33-46 (op codes), 300-342 (run), 159-163 (sech)
 double run_program(int ninstr, const int *code, int nreg, const double *consts, double x, double y, std::vector<double> &reg,
-                   double nx = 0.0, double ny = 0.0)
+                   double nx = 0.0, double ny = 0.0, const double *const *inputs = nullptr, std::size_t pt = 0)
 {
     reg.assign(static_cast<std::size_t>(nreg), 0.0);
     int res = -1;
@@ -838,6 +838,10 @@ double run_program(int ninstr, const int *code, int nreg, const double *consts, 
         case 2: v = y; break;
         case 3: v = (a < 0 ? -1.0 : 1.0) * nx; break;     // integrand.hpp:314-315: the normal, a the side's sign
         case 4: v = (a < 0 ? -1.0 : 1.0) * ny; break;
+        case 5:                                            // integrand.hpp:316: input a at the point
+            if (!inputs) throw std::invalid_argument("integrand: the program reads an input");
+            v = inputs[a][pt];
+            break;
         case 6: v = reg[a] + reg[b]; break;
         case 7: v = reg[a] * reg[b]; break;
         case 8:
@@ -934,6 +938,57 @@ void phys_row(const Problem &P, const Quad &Q, int c, int q, int code, double *o
     }
     const double a = code == 1 ? Ji[0] : Ji[1], b = code == 1 ? Ji[2] : Ji[3];
     for (int l = 0; l < n; ++l) out[l] = a * R[n + l] + b * R[2 * n + l];
+}
+
+// fe/cache_2d.hpp:245-288 (Cache2D::eval_multi): a known field with coefficients c (dim of them)
+// at every point, out[i * np + e * nq + q] for codes[i] (0 value, 1 d/dx, 2 d/dy).  The cell's
+// coefficients are gathered once (eliminated nodes read 0); the value alone is summed over the
+// local functions, and with any derivative asked for the three sums run together, each local
+// function's x and y rows formed from J^{-1} as the cache does.
+void eval_field(const Problem &P, const Quad &Q, const double *c, int ncodes, const int *codes, double *out)
+{
+    bool deriv = false;
+    for (int i = 0; i < ncodes; ++i)
+    {
+        if (codes[i] < 0 || codes[i] > 2) throw std::invalid_argument("derivative code is 0 (value), 1 (d/dx) or 2 (d/dy)");
+        deriv = deriv || codes[i] > 0;
+    }
+    const int n = P.nloc, nc = P.mesh.nt();
+    const std::size_t np = static_cast<std::size_t>(nc) * Q.nq;
+    std::vector<double> loc(static_cast<std::size_t>(n));
+    for (int e = 0; e < nc; ++e)
+    {
+        const int *d = &P.cell_dof[static_cast<std::size_t>(e) * n];
+        for (int l = 0; l < n; ++l) loc[l] = d[l] >= 0 ? c[d[l]] : 0.0;
+        for (int q = 0; q < Q.nq; ++q)
+        {
+            const double *R = &Q.tab[static_cast<std::size_t>(q) * 3 * n];
+            const std::size_t k = static_cast<std::size_t>(e) * Q.nq + q;
+            double s[3] = {0.0, 0.0, 0.0};
+            if (!deriv)
+                for (int l = 0; l < n; ++l) s[0] += R[l] * loc[l];
+            else
+            {
+                double Ji[4];
+                if (P.quad()) std::copy_n(&Q.Ji[4 * k], 4, Ji);
+                else
+                {
+                    const double *J = &P.J[4 * e];
+                    const double dt = J[0] * J[3] - J[1] * J[2];
+                    Ji[0] = J[3] / dt; Ji[1] = -J[1] / dt; Ji[2] = -J[2] / dt; Ji[3] = J[0] / dt;
+                }
+                for (int l = 0; l < n; ++l)
+                {
+                    s[0] += R[l] * loc[l];
+                    const double rx = Ji[0] * R[n + l] + Ji[2] * R[2 * n + l];
+                    s[1] += rx * loc[l];
+                    const double ry = Ji[1] * R[n + l] + Ji[3] * R[2 * n + l];
+                    s[2] += ry * loc[l];
+                }
+            }
+            for (int i = 0; i < ncodes; ++i) out[static_cast<std::size_t>(i) * np + k] = s[codes[i]];
+        }
+    }
 }
 
 // forms/assembly_2d.hpp:102-126 (element matrix: terms, points, rows, columns, zero products
@@ -1947,6 +2002,29 @@ int orc_integrand_eval(int ninstr, const int *code, int nreg, const double *cons
     return guard([&] {
         std::vector<double> reg;
         for (long i = 0; i < npts; ++i) out[i] = run_program(ninstr, code, nreg, consts, xs ? xs[i] : 0.0, ys ? ys[i] : 0.0, reg);
+    });
+}
+
+/// The integrand with inputs: ninput arrays of npts values each, laid out one after another.
+int orc_integrand_eval_in(int ninstr, const int *code, int nreg, const double *consts, long npts, const double *xs,
+                          const double *ys, int ninput, const double *inputs, double *out)
+{
+    return guard([&] {
+        std::vector<const double *> in(static_cast<std::size_t>(std::max(ninput, 0)));
+        for (int i = 0; i < ninput; ++i) in[static_cast<std::size_t>(i)] = inputs + static_cast<std::size_t>(i) * npts;
+        std::vector<double> reg;
+        for (long i = 0; i < npts; ++i)
+            out[i] = run_program(ninstr, code, nreg, consts, xs ? xs[i] : 0.0, ys ? ys[i] : 0.0, reg, 0.0, 0.0, in.data(),
+                                 static_cast<std::size_t>(i));
+    });
+}
+
+/// A known field at the cell points of the rule of a degree (Cache2D::eval_multi).
+int orc_eval_field(void *h, int degree, const double *c, int ncodes, const int *codes, double *out)
+{
+    return guard([&] {
+        const Problem &P = *static_cast<Problem *>(h);
+        eval_field(P, make_quad(P, degree), c, ncodes, codes, out);
     });
 }
 

@@ -18,7 +18,8 @@
 //   ref_pattern             forms/assembly_2d.hpp:36-41, sparse/csr.hpp:84-125
 //   ref_assemble_matrix     fe/cache_2d.hpp:67-121, forms/assembly_2d.hpp:136-164,219-305
 //   ref_assemble_vector     forms/assembly_2d.hpp:173-211
-//   ref_integrand_eval      forms/integrand.hpp:85-109,300-342
+//   ref_integrand_eval      forms/integrand.hpp:85-109,300-342 (ref_integrand_eval_in: with inputs)
+//   ref_eval_field          fe/cache_2d.hpp:245-288 (Cache2D::eval_multi)
 //   ref_spmv / ref_ddot     sparse/csr.hpp:203-215, util/omp.hpp:176-192
 //   ref_cg                  krylov/cg.hpp:245-350 as python/src/femd_bindings.cpp:799-833 calls it
 //   ref_krylov              krylov/adapters.hpp:116-147 (csnewton gmres / lgmres)
@@ -402,6 +403,28 @@ int ref_integrand_eval(int ninstr, const int *code, int nreg, int nconst, const 
     return guard([&] {
         Integrand P(std::vector<int>(code, code + 4 * ninstr), nreg, nconst, 0);
         P.evaluate(static_cast<std::size_t>(npts), xs, ys, nullptr, nullptr, {}, std::vector<double>(consts, consts + nconst), out);
+    });
+}
+
+/// Integrand::evaluate with inputs: ninput arrays of npts values, one after another.
+int ref_integrand_eval_in(int ninstr, const int *code, int nreg, int nconst, const double *consts, long npts, const double *xs,
+                          const double *ys, int ninput, const double *inputs, double *out)
+{
+    return guard([&] {
+        Integrand P(std::vector<int>(code, code + 4 * ninstr), nreg, nconst, ninput);
+        std::vector<const double *> in(static_cast<std::size_t>(ninput));
+        for (int i = 0; i < ninput; ++i) in[static_cast<std::size_t>(i)] = inputs + static_cast<std::size_t>(i) * npts;
+        P.evaluate(static_cast<std::size_t>(npts), xs, ys, nullptr, nullptr, in, std::vector<double>(consts, consts + nconst), out);
+    });
+}
+
+/// Cache2D::eval_multi: a known field (coefficients c, dim of them) at the cell points.
+int ref_eval_field(void *sh, int degree, const double *c, int ncodes, const int *codes, double *out)
+{
+    return guard([&] {
+        const Space2D &V = *static_cast<Space *>(sh)->V;
+        Cache2D Q(V, degree);
+        Q.eval_multi(c, false, codes, ncodes, out);
     });
 }
 

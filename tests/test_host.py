@@ -241,6 +241,33 @@ def test_program_matches_reference_instruction_order():
     assert nreg == 6 and consts.tolist() == [2.0, 3.0]
 
 
+def test_program_inputs_for_known_fields():
+    """A known field emits INPUT with the slot of its (vector, derivative), slots in order of first
+    use; the same node met again reuses its register, another node of the same slot gets its own
+    INPUT of that slot (forms.py:1870-1880)."""
+    from femd_amd.forms import Program
+    m = fd.rectangle_mesh(0, 0, 1, 1, 3, 3)
+    V = fd.LagrangeSpace(m, 2)
+    w, z = fd.Function(V, "w"), fd.Function(V, "z")
+    dw = fd.Dx(w, 0)
+    e = w * w + dw * fd.Dx(w, 0) + z
+    P = Program(e)
+    code, nreg, consts = P.arrays()
+    assert [c[0] for c in code.tolist()] == [5, 7, 5, 5, 7, 6, 5, 6]
+    assert [c[2] for c in code.tolist() if c[0] == 5] == [0, 1, 1, 2]
+    assert [(f.owner.name, f.deriv) for f in P.inputs] == [("w", (0, 0)), ("w", (1, 0)), ("z", (0, 0))]
+    assert [f[2] for f in P.fields()] == [0, 1, 0]
+    # a form with a known field is assembled anew each time; its degree counts the field's
+    u, v = fd.TrialFunction(V), fd.TestFunction(V)
+    a = fd.form(w * fd.dot(fd.grad(u), fd.grad(v)) * fd.dx)
+    assert not a.constant and a.quad_degree == 4
+    assert fd.form(fd.sin(w) * u * v * fd.dx).quad_degree == 4 + 3
+    with pytest.raises(ValueError):
+        fd.Function(V, "bad", np.zeros(V.dim + 1))
+    with pytest.raises(NotImplementedError):
+        fd.form(w * u * v * fd.ds)
+
+
 def _interpret(code, consts, x, y):
     """A numpy interpreter of the instruction set (forms/integrand.hpp:300-342)."""
     reg = {}

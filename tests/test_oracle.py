@@ -77,6 +77,40 @@ def test_integrand_vs_reference(orc, ref):
         assert np.array_equal(orc.integrand_eval(code, nreg, consts, xs, ys), ref.integrand_eval(code, nreg, consts, xs, ys))
 
 
+@pytest.mark.parametrize("k,diag", [(1, "right"), (2, "alternate"), (3, "crossed"), (4, "left")])
+def test_field_eval_vs_reference(orc, ref, k, diag):
+    """Cache2D::eval_multi (fe/cache_2d.hpp:245-288): a field's values and derivatives at the cell
+    points, bit for bit, on a structured and an unstructured mesh, for any order of codes."""
+    from femd_amd import problems
+    Pts, T, _ = problems.delaunay_square(0.2, seed=k)
+    cases = [(orc.rectangle(5, 4, k, diag), ref.rectangle(5, 4, k, diag)),
+             (orc.from_arrays(Pts, T, k=k), ref.from_arrays(Pts, T, k=k))]
+    for po, pr in cases:
+        c = np.random.default_rng(k).standard_normal(po.dim)
+        for deg in (2, 2 * k, 2 * k + 2):
+            for codes in ([0], [1], [2], [2, 0, 1]):
+                assert np.array_equal(po.eval_field(deg, c, codes), pr.eval_field(deg, c, codes))
+
+
+def test_integrand_inputs_vs_reference(orc, ref):
+    """INPUT instructions (forms/integrand.hpp:39, 316) read the field values at each point."""
+    from femd_amd import forms
+    rng = np.random.default_rng(1)
+    xs, ys = rng.random(3000), rng.random(3000)
+    m = forms.Function.__new__(forms.Function)           # two known fields by identity only
+    m2 = forms.Function.__new__(forms.Function)
+    w, z = forms.Field(m), forms.Field(m2, (1, 0))
+    e = (1.0 + w * w) * forms.exp(z) + forms.sin(forms.x * w) - z / (2.0 + w)
+    P = forms.Program(e)
+    code, nreg, consts = P.arrays()
+    assert len(P.inputs) == 2
+    inp = rng.standard_normal((2, xs.size))
+    got = orc.integrand_eval_in(code, nreg, consts, xs, ys, inp)
+    assert np.array_equal(got, ref.integrand_eval_in(code, nreg, consts, xs, ys, inp))
+    a, b = inp
+    assert np.allclose(got, (1.0 + a * a) * np.exp(b) + np.sin(xs * a) - b / (2.0 + a), rtol=1e-14, atol=1e-14)
+
+
 def _c0(lib, n=16, k=2):
     P = lib.rectangle(n, n, k)
     ip, ix = P.pattern()
@@ -247,6 +281,29 @@ def test_patch_test(orc, k):
     rhs = b[free] - D[np.ix_(free, bd)] @ ue[bd]
     sol = np.linalg.solve(D[np.ix_(free, free)], rhs)
     assert np.abs(sol - ue[free]).max() < 1e-11
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_field_eval_reproduces_polynomials(orc, k):
+    """A field interpolating a polynomial of degree k is that polynomial: its values and x / y
+    derivatives at the points are the polynomial's (a transposed J^{-1} or a swapped code breaks
+    the derivatives on the skewed cells of an unstructured mesh)."""
+    from femd_amd import problems
+    Pts, T, _ = problems.delaunay_square(0.15, seed=2)
+    P = orc.from_arrays(Pts, T, k=k, all_exterior=False, dirichlet=())
+    num = P.numbering()
+    xy = num["raw_xy"][num["ad2raw"]]                     # each free unknown's node
+    X, Y = xy[:, 0], xy[:, 1]
+    f = lambda x, y: 0.3 + 1.7 * x ** k - 0.9 * y ** k + 0.5 * x * y ** (k - 1)
+    fx = lambda x, y: 1.7 * k * x ** (k - 1) + 0.5 * y ** (k - 1)
+    fy = lambda x, y: -0.9 * k * y ** (k - 1) + 0.5 * (k - 1) * x * y ** max(k - 2, 0)
+    c = f(X, Y)
+    deg = 2 * k
+    x, y, _ = P.cache_points(deg)
+    v, dx, dy = P.eval_field(deg, c, [0, 1, 2])
+    assert np.allclose(v, f(x, y), atol=1e-12)
+    assert np.allclose(dx, fx(x, y), atol=1e-10)
+    assert np.allclose(dy, fy(x, y), atol=1e-10)
 
 
 def test_ilu0_tridiagonal_is_exact_lu(orc):

@@ -73,6 +73,8 @@ def main():
                     help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
     ap.add_argument("--exact-assembly", action="store_true",
                     help="assemble the timed solves' matrix with the exact kernels instead of the FP64 matrix cores")
+    ap.add_argument("--no-live-traffic", action="store_true",
+                    help="take roofline.traffic from the committed profile instead of two rocprofv3 --pmc passes")
     ap.add_argument("--no-graph", action="store_true",
                     help="every solve with plain launches (for profilers that cannot follow graph replays)")
     args = ap.parse_args()
@@ -313,7 +315,10 @@ def main():
     del Ar
     log(f"SpMV in the inner CG's order: FP64 {o64 * 1e3:.1f} us {gbs(B64, o64):.0f} GB/s (flushed {gbs(B64, q64):.0f}); "
         f"FP32 {o32 * 1e3:.1f} us {gbs(B32, o32):.0f} GB/s (flushed {gbs(B32, q32):.0f})")
-    traffic = measured_traffic("k_cg_product")
+    traffic, traffic_source = (None, None) if args.no_live_traffic else live_traffic(inner_shape, log)
+    if traffic is None:
+        traffic = measured_traffic("k_cg_product")
+        traffic_source = "profiles/hbm_traffic_p3_kappa_1.5M.md (committed)" if traffic is not None else None
 
     # ---- CPU baseline: the reference's own code, bounded sample ---------------------------
     cpu = None
@@ -366,11 +371,59 @@ def main():
         "roofline": {"kernel": f"k_cg_product (product of the FP32 inner CG with the direction update, {inner_shape})",
                      "bound": "hbm",
                      "achieved": round(gbs_fused, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs_fused / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(gbs_fused / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_source,
                      "algorithmic_bytes": B_fused, "avg_us": round(1e3 * t_fused, 2), "timed_launches": launches.value},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def live_traffic(shape, log, timeout=240):
+    """HBM bytes per launch of the inner CG product, measured in this run: two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch) over a short mixed-precision solve of the same
+    workload with the product's launch shape pinned, each scaled by what the counter reads for a
+    64 MiB device copy (FETCH_SIZE counts about half of the bytes a copy is known to read).  Child
+    processes after the timed region; (None, None) when the profiler is missing or fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None, None
+    per = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", counter, "--", sys.executable,
+                   "-m", "femd_amd.roofline", "--product-only", "--shape", shape, "--cg-tol", "1e-3"]
+            try:
+                subprocess.run(cmd, cwd=ROOT, env=env, timeout=timeout, capture_output=True)
+            except (subprocess.TimeoutExpired, OSError) as e:
+                log(f"live traffic: {counter} pass failed ({e})")
+                return None, None
+            files = glob.glob(os.path.join(d, "**", f"{counter}_counter_collection.csv"), recursive=True)
+            if not files:
+                log(f"live traffic: no {counter} counters written")
+                return None, None
+            prod, copy = [], []
+            with open(files[0]) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    name, v = r.get("Kernel_Name", ""), float(r["Counter_Value"])
+                    if "k_cg_product" in name:
+                        prod.append(v)
+                    elif "k_copy" in name and int(float(r.get("Grid_Size", 0))) == (64 << 20) // 8:
+                        copy.append(v)
+            if not prod or not copy:
+                log(f"live traffic: {counter}: {len(prod)} product and {len(copy)} copy dispatches")
+                return None, None
+            scale = (64 << 20) / (sum(copy) / len(copy) * 1024.0)
+            per[counter] = sum(prod) / len(prod) * 1024.0 * scale
+            log(f"live traffic: {counter} {per[counter] / 1e6:.1f} MB per product launch ({len(prod)} dispatches, "
+                f"copy calibration x{scale:.4f})")
+    return int(per["FETCH_SIZE"] + per["WRITE_SIZE"]), "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, measured in this run"
 
 
 def measured_traffic(kernel):

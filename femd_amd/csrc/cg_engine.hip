@@ -117,20 +117,24 @@ __global__ __launch_bounds__(BT) void k_scale_of_diag(int n, const int *__restri
 // of A, in row perm[k] and column perm[ix[p]], so out[p] = v[emap[p]] s_i s_j -- the same product,
 // in the same order, as for the entry in place (one lane per row: a set-up pass).  perm and emap
 // NULL: the original order.
+// Sixteen consecutive lanes per row, so that a row's entries are read and written together.
+constexpr int SV_LANES = 16;
 template <typename T>
 __global__ __launch_bounds__(BT) void k_scaled_values(int n, const int *ip, const int *ix, const int *__restrict__ emap,
                                                       const int *__restrict__ perm, const double *v, const double *sc, T *out)
 {
-    const int k = blockIdx.x * BT + threadIdx.x;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * BT + threadIdx.x;
+    const int k = static_cast<int>(t / SV_LANES), j = static_cast<int>(t % SV_LANES);
     if (k >= n) return;
+    const int p0 = ip[k], p1 = ip[k + 1];
     if (!perm)
     {
         const double si = sc[k];
-        for (int p = ip[k]; p < ip[k + 1]; ++p) out[p] = static_cast<T>(v[p] * si * sc[ix[p]]);
+        for (int p = p0 + j; p < p1; p += SV_LANES) out[p] = static_cast<T>(v[p] * si * sc[ix[p]]);
         return;
     }
     const double si = sc[perm[k]];
-    for (int p = ip[k]; p < ip[k + 1]; ++p) out[p] = static_cast<T>(v[emap[p]] * si * sc[perm[ix[p]]]);
+    for (int p = p0 + j; p < p1; p += SV_LANES) out[p] = static_cast<T>(v[emap[p]] * si * sc[perm[ix[p]]]);
 }
 
 // ---- iteration kernels -----------------------------------------------------------------------
@@ -653,7 +657,7 @@ void scaled_values(femd_csr_s *A, hipStream_t s, bool reorder)
     {
         hipLaunchKernelGGL(k_scale_of_diag, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, perm, diag, A->v.as<double>(),
                            A->scale.as<double>(), A->diagf.as<float>(), bad.as<int>());
-        hipLaunchKernelGGL(k_scaled_values<float>, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, Q.ip.as<int>(), Q.ix.as<int>(),
+        hipLaunchKernelGGL(k_scaled_values<float>, dim3((int64_t(n) * SV_LANES + BT - 1) / BT), dim3(BT), 0, s, n, Q.ip.as<int>(), Q.ix.as<int>(),
                            emap, perm, A->v.as<double>(), A->scale.as<double>(), A->v32s.as<float>());
     }
     int badrow = big;
@@ -674,7 +678,7 @@ void scaled_values64(femd_csr_s *A, hipStream_t s, bool reorder)
     const DevPattern &Q = L ? *L->pat : *A->pat;
     A->v64s.ensure(sizeof(double) * std::max<int64_t>(Q.nnz, 1));
     if (Q.n > 0)
-        hipLaunchKernelGGL(k_scaled_values<double>, dim3((Q.n + BT - 1) / BT), dim3(BT), 0, s, Q.n, Q.ip.as<int>(),
+        hipLaunchKernelGGL(k_scaled_values<double>, dim3((int64_t(Q.n) * SV_LANES + BT - 1) / BT), dim3(BT), 0, s, Q.n, Q.ip.as<int>(),
                            Q.ix.as<int>(), L ? L->emap.as<int>() : nullptr, L ? L->perm.as<int>() : nullptr,
                            A->v.as<double>(), A->scale.as<double>(), A->v64s.as<double>());
     A->v64s_valid = true;

@@ -585,16 +585,20 @@ __global__ void k_inv_diag(int n, const int *diag, const double *v, double *inv,
 // max |a_ij| and max |a_ij - a_ji| (sparse/csr.hpp:631-644); maxima are order independent.
 // The position of every entry's transpose (j, i) in row j, or -1 (a binary search in the sorted
 // row; once per pattern).
+// sixteen consecutive lanes per row i: entry p = (i, j) looks for (j, i) by a binary search of row j
+constexpr int TP_LANES = 16;
 __global__ __launch_bounds__(TPB) void k_transpose_positions(int n, const int *ip, const int *ix, int *tpos)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        for (int p = ip[i]; p < ip[i + 1]; ++p)
-        {
-            const int j = ix[p];
-            int lo = ip[j], hi = ip[j + 1];
-            while (lo < hi) { const int m = (lo + hi) >> 1; if (ix[m] < i) lo = m + 1; else hi = m; }
-            tpos[p] = (lo < ip[j + 1] && ix[lo] == i) ? lo : -1;
-        }
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = static_cast<int>(t / TP_LANES);
+    if (i >= n) return;
+    for (int p = ip[i] + static_cast<int>(t % TP_LANES); p < ip[i + 1]; p += TP_LANES)
+    {
+        const int j = ix[p];
+        int lo = ip[j], hi = ip[j + 1];
+        while (lo < hi) { const int m = (lo + hi) >> 1; if (ix[m] < i) lo = m + 1; else hi = m; }
+        tpos[p] = (lo < ip[j + 1] && ix[lo] == i) ? lo : -1;
+    }
 }
 
 // max |a_ij| and max |a_ij - a_ji| (a missing transpose counts as 0), one pass over the entries
@@ -610,7 +614,10 @@ __global__ __launch_bounds__(TPB) void k_asymmetry(int64_t nnz, const int *__res
         const double a = v[p];
         const int q = tpos[p];
         big = fmax(big, fabs(a));
-        dif = fmax(dif, fabs(a - (q >= 0 ? v[q] : 0.0)));
+        // |a_ij - a_ji| once per pair, from the entry that comes first (the gather of the other
+        // half is skipped); an entry without a transpose against 0
+        if (q < 0) dif = fmax(dif, fabs(a));
+        else if (q > p) dif = fmax(dif, fabs(a - v[q]));
     }
     sb[threadIdx.x] = big; sd[threadIdx.x] = dif;
     __syncthreads();
@@ -1266,7 +1273,7 @@ void launch_inv_diag(int n, const int *diag, const double *v, double *inv, int *
 }
 void launch_transpose_positions(int n, const int *ip, const int *ix, int *tpos, hipStream_t s)
 {
-    if (n > 0) FEMD_LAUNCH(k_transpose_positions, std::min(grid_for(n), 2048), TPB, 0, s, n, ip, ix, tpos);
+    if (n > 0) FEMD_LAUNCH(k_transpose_positions, grid_for(int64_t(n) * TP_LANES), TPB, 0, s, n, ip, ix, tpos);
 }
 void launch_asymmetry(int64_t nnz, const int *tpos, const double *v, double *out2, hipStream_t s)
 {

@@ -1098,8 +1098,9 @@ std::atomic<int> g_assembly_mode{0};       // 0: exact VALU element kernels, 1: 
 // codes 1 and 2 -- merged with their transposes when the terms are closed under swapping their
 // codes with the same coefficient; R[c, q][n] from the reference tables at the rule's points.
 // nullptr when the group is outside the kernel's sizes (the exact kernel runs instead).
+// prec: 0 FP64 (steps of four k), 1 f16, 2 bf16 (steps of sixteen k, R rounded to 16 bits)
 const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms, const int *a, const int *b,
-                          const femd_coefficient *coeffs, hipStream_t s)
+                          const femd_coefficient *coeffs, hipStream_t s, int prec = 0)
 {
     bool sym = true;
     for (int t = 0; t < nterms && sym; ++t)
@@ -1110,7 +1111,7 @@ const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms
             found = a[u] == b[t] && b[u] == a[t] && same_coefficient(coeffs[t], coeffs[u]);
         sym = found;
     }
-    std::string key = std::to_string(degree) + (sym ? "s" : "n");
+    std::string key = std::to_string(prec) + ":" + std::to_string(degree) + (sym ? "s" : "n");
     for (int t = 0; t < nterms; ++t) key += ":" + std::to_string(a[t]) + std::to_string(b[t]);
     auto it = V->mfma_plans.find(key);
     if (it != V->mfma_plans.end()) return it->second->rt.p ? it->second.get() : nullptr;
@@ -1127,11 +1128,13 @@ const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms
             if (reach[al][be] && (!sym || al <= be)) combos.push_back({al, be});
     auto P = std::make_unique<MfmaPlan>();
     P->ncombo = static_cast<int>(combos.size());
-    P->spc = (nq + 3) / 4;                     // each combo's points padded to whole steps of four
+    const int kstep = prec ? 16 : 4;
+    P->spc = (nq + kstep - 1) / kstep;         // each combo's points padded to whole steps
     P->ks = P->ncombo * P->spc;
     P->nout = sym ? N * (N + 1) / 2 : N * N;
     const int tiles = (P->nout + 15) / 16;
     P->nt = std::min(tiles, 4);
+    if (prec && P->nt == 3) P->nt = 4;
     P->ntile = (tiles + P->nt - 1) / P->nt * P->nt;
     for (int c = 0; c < P->ncombo; ++c)
     {
@@ -1139,7 +1142,7 @@ const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms
         P->beta |= unsigned(combos[c].second) << (2 * c);
     }
     const MfmaPlan *out = nullptr;
-    if (P->spc <= MFMA_SPCMAX)
+    if (P->spc <= (prec ? 2 : MFMA_SPCMAX))
     {
         // the output columns: (i, j) for i <= j (symmetric) or every (i, j), row-major
         std::vector<int> oc(2 * size_t(16) * P->ntile, -1), ii, jj;
@@ -1151,7 +1154,7 @@ const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms
             oc[2 * n + 1] = sym && ii[n] != jj[n] ? jj[n] * N + ii[n] : -1;
         }
         const int npad = 16 * P->ntile;
-        std::vector<double> rt(size_t(4) * P->ks * npad, 0.0), tab(size_t(3) * N);
+        std::vector<double> rt(size_t(kstep) * P->ks * npad, 0.0), tab(size_t(3) * N);
         for (int q = 0; q < nq; ++q)
         {
             if (V->V.nv == 4) V->refq->tabulate(Q.xi[q], Q.eta[q], tab.data());
@@ -1159,7 +1162,7 @@ const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms
             for (int c = 0; c < P->ncombo; ++c)
             {
                 const int al = combos[c].first, be = combos[c].second;
-                double *row = &rt[size_t(c * 4 * P->spc + q) * npad];
+                double *row = &rt[size_t(c * kstep * P->spc + q) * npad];
                 for (int n = 0; n < P->nout; ++n)
                 {
                     const int i = ii[n], j = jj[n];
@@ -1169,8 +1172,37 @@ const MfmaPlan *mfma_plan(femd_space_s *V, int degree, const Quad &Q, int nterms
                 }
             }
         }
-        P->rlds = mfma_table_bytes(P->ks, P->ntile) <= MFMA_LDS_MAX;
-        P->rt = upload(rt, s);
+        if (prec)
+        {
+            // 16-bit R laid out [step][entry][16 k], rounded to nearest even
+            std::vector<uint16_t> r16(size_t(16) * P->ks * npad, 0);
+            for (int st = 0; st < P->ks; ++st)
+                for (int n = 0; n < npad; ++n)
+                    for (int k = 0; k < 16; ++k)
+                    {
+                        const float f = static_cast<float>(rt[size_t(st * 16 + k) * npad + n]);
+                        uint16_t h;
+                        if (prec == 1)
+                        {
+                            const _Float16 x = static_cast<_Float16>(f);
+                            std::memcpy(&h, &x, 2);
+                        }
+                        else
+                        {
+                            uint32_t u;
+                            std::memcpy(&u, &f, 4);
+                            h = static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+                        }
+                        r16[(size_t(st) * npad + n) * 16 + k] = h;
+                    }
+            P->rlds = false;
+            P->rt = upload(r16, s);
+        }
+        else
+        {
+            P->rlds = mfma_table_bytes(P->ks, P->ntile) <= MFMA_LDS_MAX;
+            P->rt = upload(rt, s);
+        }
         P->ocol = upload(oc, s);
         HIPC(hipStreamSynchronize(s));
         out = P.get();
@@ -1199,12 +1231,13 @@ void assemble_matrix_dx(femd_space V, int quad_degree, int nterms, const int *a,
         Coeffs C = make_coeffs(Vs, Q, quad_degree, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
-        const MfmaPlan *mp = g_assembly_mode.load() == 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s) : nullptr;
+        const int mode = g_assembly_mode.load();
+        const MfmaPlan *mp = mode >= 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s, mode - 1) : nullptr;
         bool done = false;
-        if (mp)
-        {
+        if (mp && mode == 1)
             done = launch_elem_matrix_mfma(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mp->rlds, s);
-        }
+        else if (mp)
+            done = launch_elem_matrix_mfma16(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mode == 3, s);
         if (!done &&
             !launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
                                 Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s))
@@ -1237,7 +1270,7 @@ void assemble_vector_dx(femd_space V, int quad_degree, int nterms, const int *a,
 int femd_assembly_mode(int mode)
 {
     const int prev = g_assembly_mode.load();
-    if (mode == 0 || mode == 1) g_assembly_mode.store(mode);
+    if (mode >= 0 && mode <= 3) g_assembly_mode.store(mode);
     return prev;
 }
 
@@ -1758,21 +1791,21 @@ int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const i
 {
     return api([&] {
         femd_space_s *Vs = check_space(V);
-        if (reps < 1 || (mode != 0 && mode != 1) || !ms) fail(FEMD_EINVAL, "assembly_kernel_time: bad arguments");
+        if (reps < 1 || mode < 0 || mode > 3 || !ms) fail(FEMD_EINVAL, "assembly_kernel_time: bad arguments");
         hipStream_t s = S(nullptr);
         ensure_pattern(Vs, s);
         Quad &Q = quad_of(Vs, quad_degree, s);
         Coeffs C = make_coeffs(Vs, Q, quad_degree, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
-        const MfmaPlan *mp = mode == 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s) : nullptr;
-        if (mode == 1 && !mp) fail(FEMD_EINVAL, "assembly_kernel_time: the group is outside the matrix-core kernel's sizes");
+        const MfmaPlan *mp = mode >= 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s, mode - 1) : nullptr;
+        if (mode >= 1 && !mp) fail(FEMD_EINVAL, "assembly_kernel_time: the group is outside the matrix-core kernel's sizes");
         auto run = [&] {
             bool ok;
-            if (mp)
-            {
+            if (mp && mode == 1)
                 ok = launch_elem_matrix_mfma(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mp->rlds, s);
-            }
+            else if (mp)
+                ok = launch_elem_matrix_mfma16(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mode == 3, s);
             else
                 ok = launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
                                         Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s);
@@ -1792,7 +1825,7 @@ int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const i
         (void)hipEventDestroy(e1);
         *ms = f / reps;
         // the matrix-core work issued: 16 cells x 16 entries x 4 k (2048 flops) per instruction
-        if (mfma_flops) *mfma_flops = mp ? 2048.0 * double((nc + 15) / 16) * mp->ks * mp->ntile : 0.0;
+        if (mfma_flops) *mfma_flops = mp ? (mode == 1 ? 2048.0 : 8192.0) * double((nc + 15) / 16) * mp->ks * mp->ntile : 0.0;
     });
 }
 

@@ -59,6 +59,10 @@ size_t mfma_table_bytes(int ks, int ntile);
 // false: the rule, the coefficient count, the tile count or the LDS size is outside what the kernel
 // is built for (the exact kernel runs instead)
 bool launch_elem_matrix_mfma(int nv, const MfmaJob &J, const TermSet &T, int nt, bool rlds, hipStream_t s);
+// The same with D and R rounded to 16 bits (f16, or bf16 with bf16 set) and FP32 accumulation
+// (v_mfma_f32_16x16x16*): J.rt holds R in 16 bits laid out [step of 16 k][entry][16 k], J.spc the
+// steps of 16 per combo (at most two).  For the precision experiment.
+bool launch_elem_matrix_mfma16(int nv, const MfmaJob &J, const TermSet &T, int nt, bool bf16, hipStream_t s);
 
 void launch_gather(int64_t n, const int *ptr, const int *list, const double *src, double *dst, bool plus_zero, hipStream_t s);
 // How a gather slot combines its element-order sum with the destination (k_gather).

@@ -208,6 +208,133 @@ __global__ __launch_bounds__(MFMA_TPB) __attribute__((amdgpu_waves_per_eu(NV == 
     }
 }
 
+// ---- 16-bit inputs, FP32 accumulation (the precision experiment) ----------------------------
+// The same contraction on v_mfma_f32_16x16x16{f16,bf16}: D and R rounded to 16 bits, products
+// summed in FP32, the element matrix widened to FP64.  One instruction takes 16 k: lane l holds
+// A[cell l%16][k 4(l/16) .. 4(l/16)+3] and B[those k][entry l%16]; its result's element v is row
+// (cell) 4 (l/16) + v, column (entry) l%16.  A combo's points are padded to steps of 16; R is laid
+// out [step][entry][16 k] so that a lane reads its four k with one 8-byte load.
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round to nearest even, to the upper 16 bits of the FP32 value
+__device__ inline short to_bf16(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return static_cast<short>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+constexpr int MFMA16_SPCMAX = 2;             // steps of 16 per combo: rules of at most 32 points
+
+template <int NV, int NT, int UMAX, bool BF>
+__global__ __launch_bounds__(MFMA_TPB) void k_elem_matrix_mfma16(MfmaJob J, MfmaTerms T)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ci = lane & 15, kk = lane >> 4;
+    const int ngroups = (J.nc + 15) >> 4;
+    const int64_t N2 = int64_t(J.N) * J.N;
+    const int npad = 16 * J.ntile;
+    const unsigned short *R16 = reinterpret_cast<const unsigned short *>(J.rt);
+    constexpr int NG = NV == 4 ? 8 : 6;
+    for (int g = blockIdx.x * (MFMA_TPB / 64) + wave; g < ngroups; g += gridDim.x * (MFMA_TPB / 64))
+    {
+        const int e = g * 16 + ci;
+        const bool valid = e < J.nc;
+        double gl[NG];
+        const double *gm = J.geom + NG * int64_t(valid ? e : 0);
+#pragma unroll
+        for (int u = 0; u < NG; ++u) gl[u] = gm[u];
+        // this lane's points: q = 16 j + 4 kk + r
+        double wq[MFMA16_SPCMAX][4], cv[UMAX][MFMA16_SPCMAX][4];
+#pragma unroll
+        for (int j = 0; j < MFMA16_SPCMAX; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+                const int q = 16 * j + 4 * kk + r;
+                const bool in = valid && j < J.spc && q < J.nq;
+                wq[j][r] = in ? J.qw[q] : 0.0;
+#pragma unroll
+                for (int u = 0; u < UMAX; ++u) cv[u][j][r] = in && u < T.nu ? T.cu[u][int64_t(e) * J.nq + q] : 0.0;
+            }
+        const Jinv jc0 = jinv<NV>(gl, 0.0, 0.0);
+        for (int t0 = 0; t0 < J.ntile; t0 += NT)
+        {
+            f32x4 acc[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < J.ncombo; ++c)
+            {
+                const int al = (J.alpha >> (2 * c)) & 3, be = (J.beta >> (2 * c)) & 3;
+                double W[UMAX], W0 = 0.0;
+                if constexpr (NV == 3) W0 = combo_weights<UMAX>(T, al, be, jc0, W);
+#pragma unroll
+                for (int j = 0; j < MFMA16_SPCMAX; ++j)
+                {
+                    if (j >= J.spc) break;
+                    const int step = c * J.spc + j;
+                    f16x4 ah;
+                    bf16x4 ab;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                    {
+                        double ad = jc0.ad;
+                        if constexpr (NV == 4)
+                        {
+                            const int q = min(16 * j + 4 * kk + r, J.nq - 1);
+                            const Jinv jq = jinv<NV>(gl, J.qxi[q], J.qeta[q]);
+                            W0 = combo_weights<UMAX>(T, al, be, jq, W);
+                            ad = jq.ad;
+                        }
+                        double d = W0;
+#pragma unroll
+                        for (int u = 0; u < UMAX; ++u) d += cv[u][j][r] * W[u];
+                        d *= wq[j][r] * ad;
+                        if constexpr (BF) ab[r] = to_bf16(static_cast<float>(d));
+                        else ah[r] = static_cast<_Float16>(static_cast<float>(d));
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                    {
+                        const uint2 raw = *reinterpret_cast<const uint2 *>(
+                            R16 + ((int64_t(step) * npad + (t0 + t) * 16 + ci) * 16 + 4 * kk));
+                        if constexpr (BF)
+                        {
+                            bf16x4 b;
+                            b[0] = short(raw.x & 0xffff); b[1] = short(raw.x >> 16);
+                            b[2] = short(raw.y & 0xffff); b[3] = short(raw.y >> 16);
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, b, acc[t], 0, 0, 0);
+                        }
+                        else
+                        {
+                            const f16x4 b = __builtin_bit_cast(f16x4, raw);
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, b, acc[t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            // element v of this lane: cell 4 kk + v of the group, entry (t0 + t) * 16 + ci
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+            {
+                const int n = (t0 + t) * 16 + ci;
+                if (n >= J.nout) continue;
+                const int2 oc = J.ocol[n];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int cell = g * 16 + 4 * kk + v;
+                    if (cell >= J.nc) continue;
+                    double *Ke = J.E + cell * N2;
+                    Ke[oc.x] = static_cast<double>(acc[t][v]);
+                    if (oc.y >= 0) Ke[oc.y] = static_cast<double>(acc[t][v]);
+                }
+            }
+        }
+    }
+}
+
 template <int NV, int NT, int UMAX>
 void launch_u(const MfmaJob &J, const MfmaTerms &T, bool rlds, size_t lds, int grid, hipStream_t s)
 {
@@ -237,17 +364,12 @@ bool launch_nv(const MfmaJob &J, const MfmaTerms &T, int nt, bool rlds, size_t l
     default: return false;
     }
 }
-} // namespace
 
-size_t mfma_table_bytes(int ks, int ntile) { return sizeof(double) * size_t(4) * ks * 16 * ntile; }
-
-bool launch_elem_matrix_mfma(int nv, const MfmaJob &J, const TermSet &TS, int nt, bool rlds, hipStream_t s)
+// the terms by coefficient: constants folded into W0, the distinct arrays numbered; false past
+// MFMA_UMAX arrays
+bool mfma_terms(const TermSet &TS, MfmaTerms &T)
 {
-    if (J.nc == 0) return true;
-    const size_t lds = rlds ? mfma_table_bytes(J.ks, J.ntile) : 0;
-    if (lds > MFMA_LDS_MAX || J.spc > MFMA_SPCMAX || J.ntile % nt != 0) return false;
-    // the terms by coefficient: constants folded into W0, the distinct arrays numbered
-    MfmaTerms T{};
+    T = MfmaTerms{};
     T.nt = TS.nt;
     for (int t = 0; t < TS.nt; ++t)
     {
@@ -265,6 +387,50 @@ bool launch_elem_matrix_mfma(int nv, const MfmaJob &J, const TermSet &TS, int nt
             T.slot[t] = T.nu++;
         }
     }
+    return true;
+}
+
+template <int NV, int NT, bool BF>
+void launch16_nt(const MfmaJob &J, const MfmaTerms &T, int grid, hipStream_t s)
+{
+    if (T.nu <= 1) hipLaunchKernelGGL((k_elem_matrix_mfma16<NV, NT, 1, BF>), dim3(grid), dim3(MFMA_TPB), 0, s, J, T);
+    else if (T.nu == 2) hipLaunchKernelGGL((k_elem_matrix_mfma16<NV, NT, 2, BF>), dim3(grid), dim3(MFMA_TPB), 0, s, J, T);
+    else hipLaunchKernelGGL((k_elem_matrix_mfma16<NV, NT, MFMA_UMAX, BF>), dim3(grid), dim3(MFMA_TPB), 0, s, J, T);
+}
+
+template <int NV, bool BF>
+bool launch16_nv(const MfmaJob &J, const MfmaTerms &T, int nt, int grid, hipStream_t s)
+{
+    switch (nt)
+    {
+    case 1: launch16_nt<NV, 1, BF>(J, T, grid, s); return true;
+    case 2: launch16_nt<NV, 2, BF>(J, T, grid, s); return true;
+    case 4: launch16_nt<NV, 4, BF>(J, T, grid, s); return true;
+    default: return false;
+    }
+}
+} // namespace
+
+size_t mfma_table_bytes(int ks, int ntile) { return sizeof(double) * size_t(4) * ks * 16 * ntile; }
+
+bool launch_elem_matrix_mfma16(int nv, const MfmaJob &J, const TermSet &TS, int nt, bool bf16, hipStream_t s)
+{
+    if (J.nc == 0) return true;
+    if (J.spc > MFMA16_SPCMAX || J.ntile % nt != 0) return false;
+    MfmaTerms T;
+    if (!mfma_terms(TS, T)) return false;
+    const int grid = std::min(((J.nc + 15) / 16 + 3) / 4, 1024);
+    if (nv == 4) return bf16 ? launch16_nv<4, true>(J, T, nt, grid, s) : launch16_nv<4, false>(J, T, nt, grid, s);
+    return bf16 ? launch16_nv<3, true>(J, T, nt, grid, s) : launch16_nv<3, false>(J, T, nt, grid, s);
+}
+
+bool launch_elem_matrix_mfma(int nv, const MfmaJob &J, const TermSet &TS, int nt, bool rlds, hipStream_t s)
+{
+    if (J.nc == 0) return true;
+    const size_t lds = rlds ? mfma_table_bytes(J.ks, J.ntile) : 0;
+    if (lds > MFMA_LDS_MAX || J.spc > MFMA_SPCMAX || J.ntile % nt != 0) return false;
+    MfmaTerms T;
+    if (!mfma_terms(TS, T)) return false;
     const int groups = (J.nc + 15) / 16;
     const int grid = std::min((groups + 3) / 4, 1024);
     if (nv == 4) return launch_nv<4>(J, T, nt, rlds, lds, grid, s);

@@ -724,14 +724,15 @@ def quadrature_degree():
     return _default_quad_degree
 
 
-_ASSEMBLY_MODES = ("exact", "mfma")
+_ASSEMBLY_MODES = ("exact", "mfma", "mfma_f16", "mfma_bf16")
 
 
 def assembly_mode(mode=None) -> str:
     """How matrices form their dx element matrices: "exact" (the default) in the reference's
     operation order, bit-identical to it, or "mfma" on the FP64 matrix cores as one GEMM over the
-    cells (the same sums in another order, within a few units of the last place).  With `mode`
-    set it first switches; returns the previous mode."""
+    cells (the same sums in another order, within a few units of the last place); "mfma_f16" and
+    "mfma_bf16" round the GEMM's operands to 16 bits and accumulate in FP32 (low-precision local
+    matrices, for experiments).  With `mode` set it first switches; returns the previous mode."""
     if mode is not None and mode not in _ASSEMBLY_MODES:
         raise ValueError(f"assembly_mode: mode must be one of {_ASSEMBLY_MODES}, got {mode!r}")
     prev = L.load().femd_assembly_mode(-1 if mode is None else _ASSEMBLY_MODES.index(mode))

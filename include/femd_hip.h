@@ -447,12 +447,15 @@ int femd_inner_order(int mode);
  * GEMM over the cells, K_e[ij] = sum_k D_e[k] R[k][ij] with k running over (pair of reference
  * rows, quadrature point) -- the same sums in another order, within a few units of the last
  * place of the exact values (groups the kernel is not built for fall back to the exact kernels);
- * -1 only queries.  Returns the previous mode.  Boundary (ds) groups and vectors always use the
+ * modes 2 and 3 the same GEMM with D and R rounded to f16 (2) or bf16 (3) and FP32 accumulation
+ * (v_mfma_f32_16x16x16*), values within a few units of the 16-bit rounding -- for experiments with
+ * low-precision local matrices; -1 only queries.  Returns the previous mode.  Boundary (ds) groups and vectors always use the
  * exact kernels.  No counterpart in the reference. */
 int femd_assembly_mode(int mode);
 /* Measurement: the average time (ms, HIP events on the library stream, `reps` launches after one
  * untimed) of the element-matrix kernel alone for a dx group -- mode 0 the exact kernel, 1 the
- * matrix-core one -- and (mfma_flops, may be NULL) the matrix-core flops one launch issues.  No
+ * matrix-core one, 2 / 3 its f16 / bf16 variant -- and (mfma_flops, may be NULL) the matrix-core
+ * flops one launch issues.  No
  * counterpart in the reference. */
 int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const int *a, const int *b,
                               const femd_coefficient *coeffs, int mode, int reps, double *ms, double *mfma_flops);

@@ -168,3 +168,54 @@ def test_mfma_fallback_and_mode_switch(orc):
     assert np.array_equal(got, want)
     with pytest.raises(ValueError):
         fd.assembly_mode("bf16")
+
+
+def test_mfma_16bit_local_matrices(orc):
+    """Modes mfma_f16 / mfma_bf16 round the GEMM's operands to 16 bits and sum in FP32: the
+    entries carry the 16-bit rounding (f16 2^-11, bf16 2^-8 relative per operand) and no more;
+    refinement with the f16 matrix as the correction operator converges on a well-conditioned
+    system (P2 -Laplace u + u on 32 x 32)."""
+    _name("mfma f16 / bf16 local matrices")
+    Pts, T, _ = problems.delaunay_square(0.06, seed=5)
+    V = fd.LagrangeSpace(fd.mesh_from_arrays(Pts, T), 3, bc="dirichlet")
+    Po = orc.from_arrays(Pts, T, k=3)
+    kap, _ = problems.kappa_field(1)
+    x, y, _ = Po.cache_points(8)
+    ck = orc.integrand_eval(*fd.forms.Program(kap).arrays(), x, y)
+    ip, ix = Po.pattern()
+    Ao = Po.assemble_matrix(8, [1, 2], [1, 2], np.stack([ck, ck]))
+    bars = {"mfma_f16": (1e-6, 5e-3), "mfma_bf16": (1e-5, 5e-2)}
+    for mode, (lo, hi) in bars.items():
+        prev = fd.assembly_mode(mode)
+        try:
+            K = _assemble(V, 8, [1, 2], [1, 2], [ck, ck])
+        finally:
+            fd.assembly_mode(prev)
+        err = _row_err(ip, K.data, Ao)
+        print(f"  {mode}: row-relative error {err:.2e}")
+        assert lo <= err <= hi, (mode, err)
+        assert K.asymmetry() == 0.0                # i <= j computed, written to both places
+    from femd_amd import precision_bench
+    m = fd.rectangle_mesh(0, 0, 1, 1, 32, 32)
+    V2 = fd.LagrangeSpace(m, 2, bc="dirichlet")
+    u, v = fd.TrialFunction(V2), fd.TestFunction(V2)
+    a = fd.form((fd.dot(fd.grad(u), fd.grad(v)) + u * v) * fd.dx)
+    A = a.assemble()
+    b = fd.form(problems.fourier_source(0)[0] * v * fd.dx).assemble()
+    prev = fd.assembly_mode("mfma_f16")
+    try:
+        Alo = a.assemble(reuse=False)
+    finally:
+        fd.assembly_mode(prev)
+    xr = np.zeros(A.shape[0])
+    hist = []
+    for _ in range(12):
+        r = b - A.matvec(xr)
+        hist.append(np.linalg.norm(r) / np.linalg.norm(b))
+        if hist[-1] <= 1e-10:
+            break
+        d, _info = fd.cg(Alo, r, M=Alo.preconditioner("jacobi"), tol=1e-13)
+        xr = xr + d
+    print(f"  f16 refinement residuals {[f'{h:.1e}' for h in hist]}")
+    assert hist[-1] <= 1e-10 and len(hist) <= 8
+    assert precision_bench.row_err(A.indptr, Alo.data, A.data) < 5e-3

@@ -73,6 +73,8 @@ def main():
                     help="the inner CG keeps the original order of the unknowns instead of its own (first-met columns)")
     ap.add_argument("--exact-assembly", action="store_true",
                     help="assemble the timed solves' matrix with the exact kernels instead of the FP64 matrix cores")
+    ap.add_argument("--product-shape", default=None,
+                    help="pin the launch shape of the inner CG's product (e.g. tile512x4096) instead of the tuned one")
     ap.add_argument("--no-live-traffic", action="store_true",
                     help="take roofline.traffic from the committed profile instead of two rocprofv3 --pmc passes")
     ap.add_argument("--no-graph", action="store_true",
@@ -93,6 +95,8 @@ def main():
         L.check(lib.femd_inner_timer(1))       # the sampled kernel timer runs solves without graphs
 
     V, a_form, L_form, pinfo = problems.p3_kappa_problem(args.h)
+    if args.product_shape:
+        L.check(lib.femd_spmv_select(a_form.assemble(reuse=False)._h, L.FP32, args.product_shape.encode()))
     P, T, min_ang = pinfo["points"], pinfo["triangles"], pinfo["min_angle"]
     ip, ix = V.pattern()
     n, nnz = V.dim, int(ix.size)

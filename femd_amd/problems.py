@@ -158,14 +158,8 @@ def delaunay_square(h: float, seed: int = 0, jitter: float = 0.12, min_angle_deg
     tri = tri[order]
     # renumber vertices in first-use order (locality of the vertex numbering)
     first = np.full(P.shape[0], -1, dtype=np.int64)
-    flat = tri.reshape(-1)
-    seen = np.zeros(P.shape[0], dtype=bool)
-    newid = []
-    for v in flat:
-        if not seen[v]:
-            seen[v] = True
-            newid.append(v)
-    newid = np.asarray(newid)
+    used, at = np.unique(tri.reshape(-1), return_index=True)
+    newid = used[np.argsort(at, kind="stable")]
     first[newid] = np.arange(newid.size)
     P = P[newid]
     tri = first[tri]

@@ -1231,13 +1231,13 @@ void assemble_matrix_dx(femd_space V, int quad_degree, int nterms, const int *a,
         Coeffs C = make_coeffs(Vs, Q, quad_degree, nterms, a, b, coeffs, s);
         const int n = Vs->V.nloc, nc = Vs->V.ncells;
         DevBuf E(8 * std::max<size_t>(size_t(nc) * n * n, 1));
-        const int mode = g_assembly_mode.load();
-        const MfmaPlan *mp = mode >= 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s, mode - 1) : nullptr;
+        const int amode = g_assembly_mode.load();
+        const MfmaPlan *mp = amode >= 1 ? mfma_plan(Vs, quad_degree, Q, nterms, a, b, coeffs, s, amode - 1) : nullptr;
         bool done = false;
-        if (mp && mode == 1)
+        if (mp && amode == 1)
             done = launch_elem_matrix_mfma(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mp->rlds, s);
         else if (mp)
-            done = launch_elem_matrix_mfma16(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, mode == 3, s);
+            done = launch_elem_matrix_mfma16(Vs->V.nv, mfma_job(Vs, Q, *mp, E.as<double>()), C.T, mp->nt, amode == 3, s);
         if (!done &&
             !launch_elem_matrix(Vs->V.nv, n, nc, Q.nq, Vs->geom.as<double>(), Q.tab.as<double>(), Q.qw.as<double>(),
                                 Q.qxi.as<double>(), Q.qeta.as<double>(), C.T, E.as<double>(), s))

@@ -26,7 +26,7 @@ n = FacetNormal()
 from .sparse import Preconditioner, SparseMatrix, SparseSolver
 from .krylov import KrylovInfo, cg, gmres, lgmres, solve_refined
 from .linalg import solve
-from .batch import Batch, inner_order, value_dictionary
+from .batch import Batch, device_setup, inner_order, value_dictionary
 from .timestep import BackwardEuler
 from .serve import Server
 

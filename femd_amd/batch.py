@@ -86,6 +86,14 @@ def inner_order(enable=None) -> bool:
     return bool(L.load().femd_inner_order(-1 if enable is None else int(bool(enable))))
 
 
+def device_setup(enable=None) -> bool:
+    """Whether the inner CG's set-up (its order of the unknowns, the reordered pattern and the SpMV
+    tiles' column lists) is built on the GPU rather than on the host; both give the same arrays,
+    so results are bit-identical either way.  With `enable` set it first switches; returns the
+    previous setting."""
+    return bool(L.load().femd_device_setup(-1 if enable is None else int(bool(enable))))
+
+
 def value_dictionary(enable=None) -> bool:
     """Whether the inner CG may read matrices through lossless per-tile value dictionaries (results
     are bit-identical either way); with `enable` set it first switches them on or off.  Returns the

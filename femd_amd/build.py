@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", f"-I{CSRC}", f"-I{ROOT / 'include'}"]
 
-SOURCES = [("host.cpp", "cxx"), ("sweep.cpp", "cxx"), ("sweep.hip", "hip"), ("kernels.hip", "hip"), ("capi.hip", "hip"), ("krylov.hip", "hip"), ("batch.hip", "hip"), ("spmv.hip", "hip"), ("mfma.hip", "hip"), ("cg_engine.hip", "hip"), ("timestep.hip", "hip"), ("serve.hip", "hip")]
+SOURCES = [("host.cpp", "cxx"), ("sweep.cpp", "cxx"), ("sweep.hip", "hip"), ("kernels.hip", "hip"), ("capi.hip", "hip"), ("krylov.hip", "hip"), ("batch.hip", "hip"), ("spmv.hip", "hip"), ("mfma.hip", "hip"), ("cg_engine.hip", "hip"), ("setup_dev.hip", "hip"), ("timestep.hip", "hip"), ("serve.hip", "hip")]
 
 
 def _run(cmd):

@@ -603,6 +603,14 @@ const LocalOrder &local_order(DevPattern &P, hipStream_t s)
     if (P.local) return *P.local;
     if (P.n != P.m) fail(FEMD_EINVAL, "inner CG: the matrix must be square");
     const int n = P.n;
+    if (device_setup_enabled())
+    {
+        auto L = std::make_unique<LocalOrder>();
+        if (!device_first_met_order(n, P.ip.as<int>(), P.ix.as<int>(), P.nnz, P.segments, *L, s))
+            fail(FEMD_EINVAL, "inner CG: a system's rows have columns outside it");
+        P.local = std::move(L);
+        return *P.local;
+    }
     std::vector<int> perm, ip2, ix2, emap;
     if (!first_met_order(n, P.h_ip.data(), P.h_ix.data(), P.segments, perm, ip2, ix2, emap))
         fail(FEMD_EINVAL, "inner CG: a system's rows have columns outside it");

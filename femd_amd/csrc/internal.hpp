@@ -174,6 +174,16 @@ struct DevPattern {
 // The tuned SpMV of a pattern (values give the tuning runs realistic data).  Every shape sums each
 // row left to right in the reference's order.
 const SpmvPlan &spmv_plan(DevPattern &P, int precision, const void *values, hipStream_t s);
+
+// The inner CG's set-up on the device (setup_dev.hip), on unless femd_device_setup(0).  The first-met
+// order of a square pattern's unknowns into L (its pattern's host row pointers included; false
+// when a row meets a column outside its segment), and the column lists of tiles
+// (blocks: first row, first entry of every tile boundary) -- both equal to host.cpp's.
+bool device_setup_enabled();
+bool device_first_met_order(int n, const int *ip, const int *ix, int64_t nnz, const std::vector<int64_t> &segments,
+                            LocalOrder &L, hipStream_t s);
+void device_tile_columns(const int *ix, int64_t nnz, int ntiles, const int2 *blocks, int cap, DevBuf &uptr, DevBuf &ucol,
+                         DevBuf &lidx, hipStream_t s);
 void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, const int *stop,
                 hipStream_t s);
 void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, const int *stop,

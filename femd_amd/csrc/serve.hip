@@ -104,10 +104,11 @@ void admit(femd_server_s *S)
     G->left = static_cast<int>(take.size());
     std::vector<femd_csr> mats;
     for (const Request &r : take) mats.push_back(r.A);
-    // the group's engine keeps the original order: building the reordered pattern on the host for
-    // every group sits on the admission path, and measured it cost the stream more than the faster
-    // iterations gave back (200 requests: 15.3 against 16.0 solves/s, p50 0.89 against 0.50 s)
-    G->B = batch_build(static_cast<int>(mats.size()), mats.data(), G->stream, false);
+    // the group's engine works in the inner CG's order when that order and the tile lists are
+    // built on the device; built on the host for every group they sat on the admission path and
+    // cost the stream more than the faster iterations gave back (200 requests: 15.3 against 16.0
+    // solves/s, p50 0.89 against 0.50 s)
+    G->B = batch_build(static_cast<int>(mats.size()), mats.data(), G->stream, inner_order_enabled() && device_setup_enabled());
     const int64_t nr = G->B->nrows;
     G->bcat = DevBuf(8 * size_t(std::max<int64_t>(nr, 1)));
     G->xcat = DevBuf(8 * size_t(std::max<int64_t>(nr, 1)));

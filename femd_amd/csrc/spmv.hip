@@ -80,6 +80,11 @@ SpmvPlan make_plan(const DevPattern &P, int si, hipStream_t s)
         b2[2 * k + 1] = P.h_ip[static_cast<size_t>(rows[k])];
     }
     p.blocks = upload(b2, s);
+    if (device_setup_enabled())
+    {
+        device_tile_columns(P.ix.as<int>(), P.nnz, p.ntiles, p.blocks.as<int2>(), S.cap, p.uptr, p.ucol, p.lidx, s);
+        return p;
+    }
     std::vector<int> uptr, ucol;
     std::vector<uint16_t> lidx;
     spmv_tile_columns(P.h_ip.data(), P.h_ix.data(), rows, S.cap, uptr, ucol, lidx);

@@ -467,6 +467,17 @@ int femd_assembly_kernel_time(femd_space V, int quad_degree, int nterms, const i
  * columns outside it.  No counterpart in the reference. */
 int femd_inner_order_pattern(int n, const int *indptr, const int *indices, int nseg, const int64_t *segments, int *perm,
                              int *indptr2, int *indices2, int *emap);
+/* The same order and reordered pattern computed on the GPU (host arrays in and out; the inner CG
+ * builds it this way while femd_device_setup is on): first positions by an atomic minimum over the
+ * entries, two radix sorts.  Equal to femd_inner_order_pattern's, entry for entry.  No counterpart
+ * in the reference. */
+int femd_inner_order_pattern_gpu(int n, const int *indptr, const int *indices, int nseg, const int64_t *segments, int *perm,
+                                 int *indptr2, int *indices2, int *emap);
+/* Where the inner CG's set-up is built: mode 1 (the default) on the GPU -- the first-met order and
+ * reordered pattern, and the column lists of the SpMV tiles (one workgroup per tile sorting its
+ * columns in LDS) -- 0 on the host (host.cpp's builders, which give the same arrays), -1 only
+ * queries; returns the previous mode.  No counterpart in the reference. */
+int femd_device_setup(int mode);
 int femd_event_timer_start(femd_stream s);
 int femd_event_timer_stop(femd_stream s, double *ms);
 

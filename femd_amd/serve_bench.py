@@ -47,7 +47,8 @@ def _delaunay_dofs(h, k):
 
 def draw_requests(count, rate, dof_min, dof_max, seed, unstructured=0.5):
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, count)) if rate > 0 else np.zeros(count)
+    gaps = rng.exponential(1.0, count)           # drawn at every rate, so that the requests do not depend on it
+    arrivals = np.cumsum(gaps) / rate if rate > 0 else np.zeros(count)
     out = []
     for i in range(count):
         k = int(rng.integers(1, 4))

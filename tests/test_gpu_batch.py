@@ -108,3 +108,18 @@ def test_inner_order_switch(orc):
             assert info.converged and np.linalg.norm(x - refs[0]) / np.linalg.norm(refs[0]) <= 1e-8, mode
     finally:
         fd.inner_order(prev)
+
+
+def test_large_system_alone_and_in_a_mix():
+    """A system large enough for the single-system update path (alpha formed in the update from
+    the tile partials, each workgroup taking a run of chunks): alone, its iterates are those it has
+    next to a small system (where alpha is formed by its own kernel), bit for bit."""
+    print("\n[case] 1.2M-unknown P1 system alone and with a small one", flush=True)
+    V, A, b = _system(1100, 1, 11)
+    _, As, bs = _system(9, 2, 12)
+    alone, info1 = fd.Batch([A]).solve([b], tol=1e-9, inner_tol=0.3)
+    mix, info2 = fd.Batch([A, As]).solve([b, bs], tol=1e-9, inner_tol=0.3)
+    assert info1["converged"] and info2["converged"]
+    assert np.array_equal(alone[0], mix[0])
+    x, info = fd.solve_refined(A, b, tol=1e-9, inner_tol=0.3)
+    assert info.converged and np.array_equal(x, alone[0])

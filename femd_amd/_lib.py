@@ -136,6 +136,7 @@ _SIGS = {
     "femd_inner_order_pattern": (_i, [_i, _ip, _ip, _i, C.c_void_p, _ip, _ip, _ip, _ip]),
     "femd_inner_order_pattern_gpu": (_i, [_i, _ip, _ip, _i, C.c_void_p, _ip, _ip, _ip, _ip]),
     "femd_device_setup": (_i, [_i]),
+    "femd_pattern_lists": (_i, [C.c_void_p, _i] + [C.c_void_p] * 8),
     "femd_value_dictionary": (_i, [_i]),
     "femd_batch_dictionary": (_i, [_p, _i, C.POINTER(_i64), C.POINTER(_i)]),
     "femd_server_create": (_i, [_i64, _i, _i, _d, _d, _i, C.POINTER(_p)]),

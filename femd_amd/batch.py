@@ -87,9 +87,9 @@ def inner_order(enable=None) -> bool:
 
 
 def device_setup(enable=None) -> bool:
-    """Whether the inner CG's set-up (its order of the unknowns, the reordered pattern and the SpMV
-    tiles' column lists) is built on the GPU rather than on the host; both give the same arrays,
-    so results are bit-identical either way.  With `enable` set it first switches; returns the
+    """Whether the set-up (a space's sparsity pattern and gather lists, the inner CG's order of the
+    unknowns with its reordered pattern, and the SpMV tiles' column lists) is built on the GPU
+    rather than on the host; both give the same arrays, so results are bit-identical either way.  With `enable` set it first switches; returns the
     previous setting."""
     return bool(L.load().femd_device_setup(-1 if enable is None else int(bool(enable))))
 

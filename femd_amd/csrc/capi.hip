@@ -171,6 +171,33 @@ void ensure_geom(femd_space_s *V, hipStream_t s)
 void ensure_pattern(femd_space_s *V, hipStream_t s)
 {
     if (V->have_pattern) return;
+    if (!V->have_host_pattern && device_setup_enabled())
+    {
+        // built on the device from cell_dof; the host keeps the row pointers and columns
+        ensure_geom(V, s);
+        if (!V->cell_dof_dev.p) V->cell_dof_dev = upload(V->V.cell_dof, s);
+        DevPatternLists D;
+        device_build_pattern(V->V.dim, V->V.ncells, V->V.nloc, V->cell_dof_dev.as<int>(), D, s);
+        V->P = PatternData{};
+        V->P.n = V->V.dim;
+        V->P.indptr = D.h_ip;
+        V->P.indices = D.h_ix;
+        V->have_host_pattern = true;
+        auto P = std::make_shared<DevPattern>();
+        P->n = P->m = V->V.dim;
+        P->nnz = D.nnz;
+        P->h_ip = std::move(D.h_ip);
+        P->h_ix = std::move(D.h_ix);
+        P->ip = std::move(D.ip);
+        P->ix = std::move(D.ix);
+        V->pat = std::move(P);
+        V->mptr = std::move(D.mptr);
+        V->mlist = std::move(D.mlist);
+        V->vptr = std::move(D.vptr);
+        V->vlist = std::move(D.vlist);
+        V->have_pattern = true;
+        return;
+    }
     ensure_host_pattern(V);
     ensure_geom(V, s);
     V->pat = make_pattern(V->V.dim, V->V.dim, V->P.indptr, V->P.indices, s);
@@ -921,6 +948,48 @@ int femd_pattern(femd_space V, int64_t *nnz, int *indptr, int *indices)
         if (nnz) *nnz = Vs->P.nnz();
         if (indptr) std::copy(Vs->P.indptr.begin(), Vs->P.indptr.end(), indptr);
         if (indices) std::copy(Vs->P.indices.begin(), Vs->P.indices.end(), indices);
+    });
+}
+
+int femd_pattern_lists(femd_space V, int where, int64_t *nnz, int64_t *ncon, int *indptr, int *indices, int *mptr,
+                       int *mlist, int *vptr, int *vlist)
+{
+    return api([&] {
+        femd_space_s *Vs = check_space(V);
+        if (where != 0 && where != 1) fail(FEMD_EINVAL, "pattern_lists: where is 0 (host) or 1 (GPU)");
+        const int n = Vs->V.dim;
+        auto put = [](const int *src, size_t k, int *dst) { if (dst && k) std::copy(src, src + k, dst); };
+        if (where == 0)
+        {
+            const PatternData P = build_pattern(Vs->V);
+            if (nnz) *nnz = P.nnz();
+            if (ncon) *ncon = static_cast<int64_t>(P.mlist.size());
+            put(P.indptr.data(), P.indptr.size(), indptr);
+            put(P.indices.data(), P.indices.size(), indices);
+            put(P.mptr.data(), P.mptr.size(), mptr);
+            put(P.mlist.data(), P.mlist.size(), mlist);
+            put(P.vptr.data(), P.vptr.size(), vptr);
+            put(P.vlist.data(), P.vlist.size(), vlist);
+            return;
+        }
+        hipStream_t s = S(nullptr);
+        DevBuf cd = upload(Vs->V.cell_dof, s);
+        DevPatternLists D;
+        device_build_pattern(n, Vs->V.ncells, Vs->V.nloc, cd.as<int>(), D, s);
+        if (nnz) *nnz = D.nnz;
+        if (ncon) *ncon = D.ncon;
+        put(D.h_ip.data(), D.h_ip.size(), indptr);
+        put(D.h_ix.data(), D.h_ix.size(), indices);
+        auto get = [&](const DevBuf &b, size_t k, int *dst) {
+            if (dst && k) HIPC(hipMemcpyAsync(dst, b.p, 4 * k, hipMemcpyDeviceToHost, s));
+        };
+        int64_t nv = 0;
+        for (int x : Vs->V.cell_dof) nv += x >= 0;
+        get(D.mptr, size_t(D.nnz) + 1, mptr);
+        get(D.mlist, size_t(D.ncon), mlist);
+        get(D.vptr, size_t(n) + 1, vptr);
+        get(D.vlist, size_t(nv), vlist);
+        HIPC(hipStreamSynchronize(s));
     });
 }
 

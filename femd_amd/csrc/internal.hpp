@@ -184,6 +184,15 @@ bool device_first_met_order(int n, const int *ip, const int *ix, int64_t nnz, co
                             LocalOrder &L, hipStream_t s);
 void device_tile_columns(const int *ix, int64_t nnz, int ntiles, const int2 *blocks, int cap, DevBuf &uptr, DevBuf &ucol,
                          DevBuf &lidx, hipStream_t s);
+// A space's pattern with its gather lists (PatternData's arrays, equal to build_pattern's) from its
+// cell_dof on the device; the row pointers and columns are also copied to the host.
+struct DevPatternLists {
+    int n = 0;
+    int64_t nnz = 0, ncon = 0;
+    DevBuf ip, ix, mptr, mlist, vptr, vlist;
+    std::vector<int> h_ip, h_ix;
+};
+void device_build_pattern(int n, int ncells, int nl, const int *cell_dof_dev, DevPatternLists &out, hipStream_t s);
 void spmv_run64(const SpmvPlan &p, const DevPattern &P, const double *v, const double *x, double *y, const int *stop,
                 hipStream_t s);
 void spmv_run32(const SpmvPlan &p, const DevPattern &P, const float *v, const float *x, float *y, const int *stop,

@@ -207,6 +207,59 @@ __global__ void k_tile_compact(int ntiles, const int2 *blk, const int *uptr, con
     for (int q = threadIdx.x; q < nu; q += blockDim.x) ucol[u0 + q] = tmp[p0 + q];
 }
 
+// ---- the pattern of a space and its gather lists ----------------------------------------------
+// every element-matrix entry e = (c nl + a) nl + b keyed by (row d[a], column d[b]), or by (n, 0)
+// (after every stored entry) when either dof is constrained
+__global__ void k_pair_keys(int64_t E, int nl, int n, const int *__restrict__ cd, uint64_t *key, int *val, int *count)
+{
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int nn = nl * nl;
+    const int c = static_cast<int>(e / nn), ab = static_cast<int>(e % nn), a = ab / nl, b = ab % nl;
+    const int ra = cd[int64_t(c) * nl + a], rb = cd[int64_t(c) * nl + b];
+    const bool ok = ra >= 0 && rb >= 0;
+    key[e] = ok ? (uint64_t(uint32_t(ra)) << 32) | uint32_t(rb) : uint64_t(uint32_t(n)) << 32;
+    val[e] = static_cast<int>(e);
+    if (ok) atomicAdd(count, 1);
+}
+
+// first[t] = 1 where sorted entry t starts a stored entry (t < M), 0 past M
+__global__ void k_first_flags(int64_t M, const uint64_t *__restrict__ key, int *flag)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t > M) return;
+    flag[t] = t < M && (t == 0 || key[t] != key[t - 1]) ? 1 : 0;
+}
+
+// the stored entries: columns, the first element entry of each, the rows' counts
+__global__ void k_emit_entries(int64_t M, const uint64_t *__restrict__ key, const int *__restrict__ flag,
+                               const int *__restrict__ pos, int *ix, int *mptr, int *rowcnt)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    if (t == M - 1) mptr[pos[M]] = static_cast<int>(M);
+    if (!flag[t]) return;
+    const int p = pos[t];
+    ix[p] = static_cast<int>(uint32_t(key[t]));
+    mptr[p] = static_cast<int>(t);
+    atomicAdd(&rowcnt[key[t] >> 32], 1);
+}
+
+// every (cell, local dof) keyed by its dof, n when constrained
+__global__ void k_dof_keys(int64_t L, int n, const int *__restrict__ cd, uint32_t *key, int *val, int *count, int *dofcnt)
+{
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= L) return;
+    const int d = cd[e];
+    key[e] = d >= 0 ? uint32_t(d) : uint32_t(n);
+    val[e] = static_cast<int>(e);
+    if (d >= 0)
+    {
+        atomicAdd(count, 1);
+        atomicAdd(&dofcnt[d], 1);
+    }
+}
+
 template <class K, class V>
 void sort_pairs(const K *kin, K *kout, const V *vin, V *vout, int64_t n, int end_bit, hipStream_t s)
 {
@@ -320,6 +373,71 @@ void device_tile_columns(const int *ix, int64_t nnz, int ntiles, const int2 *blo
     hipLaunchKernelGGL(k_tile_compact, dim3(ntiles), dim3(256), 0, s, ntiles, blocks, uptr.as<int>(), tmp.as<int>(),
                        ucol.as<int>());
     HIPC(hipStreamSynchronize(s));                     // cnt, tmp return to the pool
+}
+
+void device_build_pattern(int n, int ncells, int nl, const int *cell_dof_dev, DevPatternLists &out, hipStream_t s)
+{
+    const int64_t E = int64_t(ncells) * nl * nl, L = int64_t(ncells) * nl;
+    if (E >= (int64_t(1) << 31)) fail(FEMD_EINVAL, "pattern: element matrices exceed 2^31 entries");
+    out.n = n;
+    DevBuf counter(2 * sizeof(int));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, int64_t(2), 0, counter.as<int>());
+    // the stored entries and, for each, its element entries in ascending order
+    int M = 0;
+    {
+        DevBuf k0(8 * size_t(std::max<int64_t>(E, 1))), k1(8 * size_t(std::max<int64_t>(E, 1)));
+        DevBuf v0(4 * size_t(std::max<int64_t>(E, 1)));
+        out.mlist = DevBuf(4 * size_t(std::max<int64_t>(E, 1)));
+        if (E > 0)
+        {
+            hipLaunchKernelGGL(k_pair_keys, dim3(grid(E, 256)), dim3(256), 0, s, E, nl, n, cell_dof_dev, k0.as<uint64_t>(),
+                               v0.as<int>(), counter.as<int>());
+            sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int>(), out.mlist.as<int>(), E,
+                       std::min(64, 32 + bit_length(uint64_t(n))), s);
+        }
+        HIPC(hipMemcpyAsync(&M, counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        out.ncon = M;
+        DevBuf flag(4 * (size_t(M) + 1)), pos(4 * (size_t(M) + 1)), rowcnt(4 * (size_t(n) + 1));
+        hipLaunchKernelGGL(k_fill_int, dim3(grid(int64_t(n) + 1, 256)), dim3(256), 0, s, int64_t(n) + 1, 0, rowcnt.as<int>());
+        hipLaunchKernelGGL(k_first_flags, dim3(grid(int64_t(M) + 1, 256)), dim3(256), 0, s, int64_t(M), k1.as<uint64_t>(),
+                           flag.as<int>());
+        exclusive_sum(flag.as<int>(), pos.as<int>(), int64_t(M) + 1, s);
+        int nnz = 0;
+        HIPC(hipMemcpyAsync(&nnz, pos.as<int>() + M, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        out.nnz = nnz;
+        out.ix = DevBuf(4 * size_t(std::max(nnz, 1)));
+        out.mptr = DevBuf(4 * (size_t(nnz) + 1));
+        if (M == 0) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, int64_t(1), 0, out.mptr.as<int>());
+        else
+            hipLaunchKernelGGL(k_emit_entries, dim3(grid(M, 256)), dim3(256), 0, s, int64_t(M), k1.as<uint64_t>(),
+                               flag.as<int>(), pos.as<int>(), out.ix.as<int>(), out.mptr.as<int>(), rowcnt.as<int>());
+        out.ip = DevBuf(4 * (size_t(n) + 1));
+        exclusive_sum(rowcnt.as<int>(), out.ip.as<int>(), int64_t(n) + 1, s);
+    }
+    // every dof's (cell, local dof) entries in ascending order
+    {
+        DevBuf k0(4 * size_t(std::max<int64_t>(L, 1))), k1(4 * size_t(std::max<int64_t>(L, 1)));
+        DevBuf v0(4 * size_t(std::max<int64_t>(L, 1))), dofcnt(4 * (size_t(n) + 1));
+        out.vlist = DevBuf(4 * size_t(std::max<int64_t>(L, 1)));
+        hipLaunchKernelGGL(k_fill_int, dim3(grid(int64_t(n) + 1, 256)), dim3(256), 0, s, int64_t(n) + 1, 0, dofcnt.as<int>());
+        if (L > 0)
+        {
+            hipLaunchKernelGGL(k_dof_keys, dim3(grid(L, 256)), dim3(256), 0, s, L, n, cell_dof_dev, k0.as<uint32_t>(),
+                               v0.as<int>(), counter.as<int>() + 1, dofcnt.as<int>());
+            sort_pairs(k0.as<uint32_t>(), k1.as<uint32_t>(), v0.as<int>(), out.vlist.as<int>(), L,
+                       std::max(1, bit_length(uint64_t(n))), s);
+        }
+        out.vptr = DevBuf(4 * (size_t(n) + 1));
+        exclusive_sum(dofcnt.as<int>(), out.vptr.as<int>(), int64_t(n) + 1, s);
+        HIPC(hipStreamSynchronize(s));
+    }
+    out.h_ip.assign(size_t(n) + 1, 0);
+    out.h_ix.assign(size_t(out.nnz), 0);
+    HIPC(hipMemcpyAsync(out.h_ip.data(), out.ip.p, 4 * (size_t(n) + 1), hipMemcpyDeviceToHost, s));
+    if (out.nnz) HIPC(hipMemcpyAsync(out.h_ix.data(), out.ix.p, 4 * size_t(out.nnz), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
 }
 
 } // namespace femd_amd

@@ -473,11 +473,21 @@ int femd_inner_order_pattern(int n, const int *indptr, const int *indices, int n
  * in the reference. */
 int femd_inner_order_pattern_gpu(int n, const int *indptr, const int *indices, int nseg, const int64_t *segments, int *perm,
                                  int *indptr2, int *indices2, int *emap);
-/* Where the inner CG's set-up is built: mode 1 (the default) on the GPU -- the first-met order and
- * reordered pattern, and the column lists of the SpMV tiles (one workgroup per tile sorting its
- * columns in LDS) -- 0 on the host (host.cpp's builders, which give the same arrays), -1 only
- * queries; returns the previous mode.  No counterpart in the reference. */
+/* Where the set-up is built: mode 1 (the default) on the GPU -- a space's pattern and gather lists
+ * (radix sorts of the element entries and cell dofs), the inner CG's first-met order and reordered
+ * pattern, and the column lists of the SpMV tiles (one workgroup per tile sorting its columns in
+ * LDS) -- 0 on the host (host.cpp's builders, which give the same arrays), -1 only queries;
+ * returns the previous mode.  No counterpart in the reference. */
 int femd_device_setup(int mode);
+/* A space's pattern and its gather lists built afresh on the host (where 0, the builder femd_pattern
+ * uses) or on the GPU (where 1, the one assembly uses while femd_device_setup is on): nnz, ncon (the
+ * element entries landing in the pattern), indptr (dim + 1), indices (nnz), mptr (nnz + 1) and
+ * mlist (ncon) -- stored entry p receives element entries mlist[mptr[p] .. mptr[p + 1]), element
+ * entry (c nloc + a) nloc + b, ascending -- vptr (dim + 1) and vlist (the unconstrained
+ * (cell, local dof) pairs, c nloc + a, ascending per dof).  Any array may be NULL (sizes only).  For
+ * checking the two builders against each other.  No counterpart in the reference. */
+int femd_pattern_lists(femd_space V, int where, int64_t *nnz, int64_t *ncon, int *indptr, int *indices, int *mptr,
+                       int *mlist, int *vptr, int *vlist);
 int femd_event_timer_start(femd_stream s);
 int femd_event_timer_stop(femd_stream s, double *ms);
 

@@ -141,3 +141,74 @@ def test_solves_set_up_on_gpu_are_bit_identical():
     finally:
         fd.device_setup(prev)
         fd.inner_order(prev_order)
+
+
+def _lists(V, where):
+    import ctypes as C
+    lib = L.load()
+    nnz, ncon = C.c_int64(), C.c_int64()
+    L.check(lib.femd_pattern_lists(V._h, where, C.byref(nnz), C.byref(ncon), *([None] * 6)))
+    nloc = V.nloc if hasattr(V, "nloc") else None
+    nv = int((np.asarray(V.cell_dofs()) >= 0).sum()) if hasattr(V, "cell_dofs") else None
+    arrs = [np.full(V.dim + 1, -7, np.int32), np.full(max(nnz.value, 1), -7, np.int32),
+            np.full(nnz.value + 1, -7, np.int32), np.full(max(ncon.value, 1), -7, np.int32),
+            np.full(V.dim + 1, -7, np.int32), np.full(max(nv if nv is not None else 1, 1), -7, np.int32)]
+    L.check(lib.femd_pattern_lists(V._h, where, None, None, *[a.ctypes.data for a in arrs]))
+    return nnz.value, ncon.value, arrs
+
+
+@pytest.mark.parametrize("case", ["p1", "p2_dirichlet", "p3_delaunay", "q2_markers", "p2_alternate"])
+def test_pattern_and_gather_lists_equal_host(case):
+    """The pattern and its gather lists built on the GPU (radix sorts of the element entries by
+    (row, column) and of the cell dofs by dof) equal the host builder's, array for array; assembly
+    on a space whose pattern the GPU built is bit-identical to assembly on a host-built one."""
+    _name(f"pattern and gather lists on the GPU: {case}")
+    if case == "p1":
+        V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 23, 17), 1)
+    elif case == "p2_dirichlet":
+        V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 30, 30), 2, bc="dirichlet")
+    elif case == "p3_delaunay":
+        Pts, T, _ = problems.delaunay_square(0.03, seed=6)
+        V = fd.LagrangeSpace(fd.mesh_from_arrays(Pts, T), 3, bc="dirichlet")
+    elif case == "q2_markers":
+        V = fd.LagrangeSpace(fd.quad_mesh_from_arrays(*_quads(9, 7)), 2, bc="dirichlet")
+    else:
+        V = fd.LagrangeSpace(fd.rectangle_mesh(0, 0, 1, 1, 12, 9, diagonal="alternate"), 2, dirichlet=[2])
+    nnz_h, ncon_h, host = _lists(V, 0)
+    nnz_d, ncon_d, dev = _lists(V, 1)
+    assert nnz_h == nnz_d and ncon_h == ncon_d
+    for k, (h, d) in enumerate(zip(host, dev)):
+        assert np.array_equal(h, d), k
+    ip, ix = V.pattern()
+    assert np.array_equal(ip, host[0]) and np.array_equal(ix, host[1][:nnz_h])
+
+
+def _quads(nx, ny):
+    rng = np.random.default_rng(8)
+    pts = np.array([(i / nx + (0 if i in (0, nx) else 0.2 * (rng.random() - 0.5) / nx),
+                     j / ny + (0 if j in (0, ny) else 0.2 * (rng.random() - 0.5) / ny))
+                    for j in range(ny + 1) for i in range(nx + 1)])
+    q = np.array([[j * (nx + 1) + i, j * (nx + 1) + i + 1, (j + 1) * (nx + 1) + i + 1, (j + 1) * (nx + 1) + i]
+                  for j in range(ny) for i in range(nx)], dtype=np.int32)
+    return pts, q
+
+
+def test_assembly_on_gpu_built_pattern_is_bit_identical():
+    _name("assembly on GPU-built and host-built patterns")
+    prev = fd.device_setup()
+    Pts, T, _ = problems.delaunay_square(0.04, seed=3)
+    try:
+        out = {}
+        for mode in (False, True):
+            fd.device_setup(mode)
+            V = fd.LagrangeSpace(fd.mesh_from_arrays(Pts, T), 2, bc="dirichlet")
+            f, _ = problems.fourier_source(1)
+            u, v = fd.TrialFunction(V), fd.TestFunction(V)
+            a = fd.form(problems.kappa_field(1)[0] * fd.dot(fd.grad(u), fd.grad(v)) * fd.dx + u * v * fd.ds)
+            A = a.assemble()
+            b = fd.form(f * v * fd.dx).assemble()
+            out[mode] = (A.indptr.copy(), A.indices.copy(), A.data.copy(), b)
+        for h, d in zip(out[False], out[True]):
+            assert np.array_equal(h, d)
+    finally:
+        fd.device_setup(prev)

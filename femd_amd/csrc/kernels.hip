@@ -510,7 +510,22 @@ __global__ __launch_bounds__(TPB) void k_dot_parts(int64_t n, const double *__re
     __shared__ double lane[8];
     const int64_t lo = static_cast<int64_t>(blockIdx.x) * CHUNK;
     const int len = static_cast<int>(n - lo < CHUNK ? n - lo : CHUNK);
-    for (int k = threadIdx.x; k < len; k += blockDim.x) prod[k] = x[lo + k] * y[lo + k];
+    // every load issued at once, on an index inside the chunk (a load under a condition, or one
+    // per trip of a loop, waits for the loads before it)
+    double a[CHUNK / TPB], b[CHUNK / TPB];
+#pragma unroll
+    for (int j = 0; j < CHUNK / TPB; ++j)
+    {
+        const int k = j * TPB + static_cast<int>(threadIdx.x), kc = k < len ? k : 0;
+        a[j] = x[lo + kc];
+        b[j] = y[lo + kc];
+    }
+#pragma unroll
+    for (int j = 0; j < CHUNK / TPB; ++j)
+    {
+        const int k = j * TPB + static_cast<int>(threadIdx.x);
+        if (k < len) prod[k] = a[j] * b[j];
+    }
     __syncthreads();
     const double s = chunk_sum(prod, len, lane);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
@@ -660,18 +675,31 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, double *__restrict
     const double alpha = st->alpha;
     const int64_t lo = static_cast<int64_t>(blockIdx.x) * CHUNK;
     const int len = static_cast<int>(n - lo < CHUNK ? n - lo : CHUNK);
+    // every load issued at once, on an index inside the chunk (a load under a condition waits for
+    // the loads before it); the stores keep to the chunk
+    double rv[CHUNK / TPB], av[CHUNK / TPB], xv[CHUNK / TPB], pv[CHUNK / TPB], dv[CHUNK / TPB];
+#pragma unroll
+    for (int j = 0; j < CHUNK / TPB; ++j)
+    {
+        const int k = j * TPB + static_cast<int>(threadIdx.x), kc = k < len ? k : 0;
+        rv[j] = r[lo + kc];
+        av[j] = Ap[lo + kc];
+        xv[j] = x[lo + kc];
+        pv[j] = p[lo + kc];
+        dv[j] = inv ? inv[lo + kc] : 0.0;
+    }
     double e2[CHUNK / TPB];
 #pragma unroll
     for (int j = 0; j < CHUNK / TPB; ++j)
     {
-        const int k = j * TPB + threadIdx.x;
+        const int k = j * TPB + static_cast<int>(threadIdx.x);
+        const double ri = rv[j] - alpha * av[j];
+        e2[j] = ri * ri;
         if (k < len)
         {
             const int64_t i = lo + k;
-            const double ri = r[i] - alpha * Ap[i];
             r[i] = ri;
-            x[i] = x[i] + alpha * p[i];
-            e2[j] = ri * ri;
+            x[i] = xv[j] + alpha * pv[j];
             prod[k] = e2[j];
         }
     }
@@ -682,8 +710,8 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, double *__restrict
 #pragma unroll
     for (int j = 0; j < CHUNK / TPB; ++j)
     {
-        const int k = j * TPB + threadIdx.x;
-        if (k < len) prod[k] = e2[j] * inv[lo + k];
+        const int k = j * TPB + static_cast<int>(threadIdx.x);
+        if (k < len) prod[k] = e2[j] * dv[j];
     }
     __syncthreads();
     const double srz = chunk_sum(prod, len, lane);

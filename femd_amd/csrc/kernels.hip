@@ -467,40 +467,6 @@ __global__ __launch_bounds__(TPB) void k_eval_field(int nc, int nq, const int *_
 }
 
 // ============================================================================ SpMV
-// CSR-stream: block b owns rows [blocks[b], blocks[b+1]).  The block's products v[p] * x[ix[p]]
-// are formed with coalesced loads of values and columns and staged in LDS; then one thread per
-// row adds its products left to right, the order of CSRMatrix::apply.  A row longer than the
-// staging buffer has a block of its own and is summed from global memory by one thread.
-template <typename T, int CAP>
-__global__ __launch_bounds__(SPMV_THREADS) void k_spmv(const int *__restrict__ blocks, const int *__restrict__ ip,
-                                                       const int *__restrict__ ix, const T *__restrict__ v,
-                                                       const T *__restrict__ x, T *__restrict__ y, const int *stop)
-{
-    if (stop && *stop) return;
-    __shared__ T prod[CAP];
-    const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
-    const int p0 = ip[r0], len = ip[r1] - p0;
-    if (len > CAP)
-    {
-        if (threadIdx.x == 0)
-        {
-            T s = 0;
-            for (int p = p0; p < p0 + len; ++p) s += v[p] * x[ix[p]];
-            y[r0] = s;
-        }
-        return;
-    }
-    for (int k = threadIdx.x; k < len; k += blockDim.x) prod[k] = v[p0 + k] * x[ix[p0 + k]];
-    __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r < r1)
-    {
-        T s = 0;
-        for (int k = ip[r] - p0, e = ip[r + 1] - p0; k < e; ++k) s += prod[k];
-        y[r] = s;
-    }
-}
-
 // ============================================================================ ordered dot
 __global__ __launch_bounds__(TPB) void k_dot_parts(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
                                                    double *__restrict__ parts, const int *stop)
@@ -1234,18 +1200,6 @@ bool launch_eval_field(int nv, int nloc, int nc, int nq, const int *cell_dof, co
     }
 }
 #undef FIELD_CASE
-
-void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,
-                   double *y, const int *stop, hipStream_t s)
-{
-    if (nblocks > 0) FEMD_LAUNCH((k_spmv<double, SPMV_CAP64>), nblocks, SPMV_THREADS, 0, s, blocks, ip, ix, v, x, y, stop);
-}
-
-void launch_spmv32(int nblocks, const int *blocks, const int *ip, const int *ix, const float *v, const float *x,
-                   float *y, const int *stop, hipStream_t s)
-{
-    if (nblocks > 0) FEMD_LAUNCH((k_spmv<float, SPMV_CAP64>), nblocks, SPMV_THREADS, 0, s, blocks, ip, ix, v, x, y, stop);
-}
 
 int nchunks(int64_t n) { return static_cast<int>((n + CHUNK - 1) / CHUNK); }
 

@@ -5,8 +5,6 @@
 
 namespace femd_amd {
 
-constexpr int SPMV_THREADS = 256;          // rows per CSR-stream block at most
-constexpr int SPMV_CAP64 = 4096;           // staged products per block, FP64 (32 KiB of LDS)
 constexpr int CHUNK = 4096;                // the reference's reduction chunk (util/omp.hpp:91)
 constexpr int MAX_TERMS = 8;
 constexpr int MAX_REGS = 64;
@@ -87,10 +85,6 @@ bool launch_eval_field(int nv, int nloc, int nc, int nq, const int *cell_dof, co
                        const double *geom, const double *qxi, const double *qeta, int code, double *out, hipStream_t s);
 
 // ---- SpMV -----------------------------------------------------------------------------------
-void launch_spmv64(int nblocks, const int *blocks, const int *ip, const int *ix, const double *v, const double *x,
-                   double *y, const int *stop, hipStream_t s);
-void launch_spmv32(int nblocks, const int *blocks, const int *ip, const int *ix, const float *v, const float *x,
-                   float *y, const int *stop, hipStream_t s);
 
 // ---- ordered reductions (util/omp.hpp:135-192) -----------------------------------------------
 int nchunks(int64_t n);

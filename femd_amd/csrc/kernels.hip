@@ -66,6 +66,41 @@ __device__ inline double chunk_sum(const double *prod, int len, double *lane)
     return out;
 }
 
+// chunk_sum of two arrays at once: lanes 0-7 of the first wave take a's eight chains, lanes 8-15
+// b's, in the same instructions; thread 0 gets a's sum in *sa, thread 8 b's in *sb (the same
+// bits as two calls of chunk_sum).
+__device__ inline void chunk_sum2(const double *a, const double *b, int len, double *lane, double *sa, double *sb)
+{
+    const int tid = threadIdx.x, full = len & ~7;
+    if (tid < 16)
+    {
+        const double *src = tid < 8 ? a : b;
+        double s = 0.0;
+        int i = tid & 7;
+        for (; i + 8 * 15 < full; i += 8 * 16)
+        {
+            double t[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) t[u] = src[i + 8 * u];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += t[u];
+        }
+        for (; i < full; i += 8) s += src[i];
+        lane[tid] = s;
+    }
+    __syncthreads();
+    if (tid == 0 || tid == 8)
+    {
+        const double *src = tid == 0 ? a : b;
+        const double *l0 = lane + tid;
+        double s0 = l0[0];
+        for (int i = full; i < len; ++i) s0 += src[i];
+        double l[8] = {s0, l0[1], l0[2], l0[3], l0[4], l0[5], l0[6], l0[7]};
+        *(tid == 0 ? sa : sb) = lane_combine(l);
+    }
+    __syncthreads();
+}
+
 // The chunk partial sums added left to right from zero (util/omp.hpp:186-191); one chunk is
 // returned as it is.
 // Computed by a whole workgroup (every thread must call it; thread 0 gets the result): the
@@ -637,7 +672,8 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, double *__restrict
 {
     if (st->stop) return;
     __shared__ double prod[CHUNK];
-    __shared__ double lane[8];
+    __shared__ double prod2[CHUNK];
+    __shared__ double lane[16];
     const double alpha = st->alpha;
     const int64_t lo = static_cast<int64_t>(blockIdx.x) * CHUNK;
     const int len = static_cast<int>(n - lo < CHUNK ? n - lo : CHUNK);
@@ -669,19 +705,25 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, double *__restrict
             prod[k] = e2[j];
         }
     }
-    __syncthreads();
-    const double srr = chunk_sum(prod, len, lane);
-    if (threadIdx.x == 0) parts_rr[blockIdx.x] = srr;
-    if (!inv) return;
+    if (!inv)
+    {
+        __syncthreads();
+        const double srr = chunk_sum(prod, len, lane);
+        if (threadIdx.x == 0) parts_rr[blockIdx.x] = srr;
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < CHUNK / TPB; ++j)
     {
         const int k = j * TPB + static_cast<int>(threadIdx.x);
-        if (k < len) prod[k] = e2[j] * dv[j];
+        if (k < len) prod2[k] = e2[j] * dv[j];
     }
     __syncthreads();
-    const double srz = chunk_sum(prod, len, lane);
-    if (threadIdx.x == 0) parts_rz[blockIdx.x] = srz;
+    // r.r and r.D^-1 r of the chunk, their chains side by side
+    double srr = 0.0, srz = 0.0;
+    chunk_sum2(prod, prod2, len, lane, &srr, &srz);
+    if (threadIdx.x == 0) parts_rr[blockIdx.x] = srr;
+    if (threadIdx.x == 8) parts_rz[blockIdx.x] = srz;
 }
 
 // After the update: ||r||, the stopping test, beta and the loop-top tests (cg.hpp:330-349).

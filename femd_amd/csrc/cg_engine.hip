@@ -28,8 +28,9 @@ inline int grid1(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + 
 // thread of the workgroup: lane l adds its elements i = l + k THREADS in k order, loading K of
 // them per round all at once (one memory latency per round, not per element), then the wave
 // (shuffles) and the waves in order.  The order depends only on n and THREADS.  red holds
-// W THREADS / 64 doubles.
-template <int THREADS, int W, int K = (W == 1 ? 16 : 8)>
+// W THREADS / 64 doubles.  COH: a is read with device-coherent loads (values other workgroups of
+// the running kernel stored with device-scope stores; see arrive_last).
+template <int THREADS, int W, int K = (W == 1 ? 16 : 8), bool COH = false>
 __device__ inline void wg_fixed_sums(const double *a, int n, double *red, double (&out)[W])
 {
     double s[W];
@@ -43,7 +44,12 @@ __device__ inline void wg_fixed_sums(const double *a, int n, double *red, double
         {
             const int i = base + k * THREADS + static_cast<int>(threadIdx.x);
 #pragma unroll
-            for (int w = 0; w < W; ++w) v[k][w] = i < n ? a[static_cast<int64_t>(i) * W + w] : 0.0;
+            for (int w = 0; w < W; ++w)
+            {
+                const double *e = a + static_cast<int64_t>(i) * W + w;
+                if constexpr (COH) v[k][w] = i < n ? __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+                else v[k][w] = i < n ? *e : 0.0;
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -217,31 +223,86 @@ __global__ __launch_bounds__(BT) void k_cg_alpha(const int *__restrict__ seg_til
     S.alpha = S.rr / pq;
 }
 
+// The end of an iteration for one system (thread 0): the new r.r and r.D r, the pause tests, beta.
+__device__ inline void beta_verdict(CGSys &S, double rr, double rt, int odd)
+{
+    S.it += 1;
+    S.total += 1;
+    S.dir = odd ^ 1;                        // the new direction's buffer
+    int reason = 0;
+    if (!(rr == rr) || !(rt == rt)) reason = 2;
+    else if (rr <= S.thr || rt <= S.tgt) reason = 1;
+    else if (S.it >= S.max_iter) reason = 3;
+    if (reason)
+    {
+        S.active = 0;
+        S.reason = reason;
+        S.rr = rr;
+        return;
+    }
+    S.beta = rr / S.rr;
+    S.rr = rr;
+}
+
+// Arrival counters: the update workgroups of a system count themselves in, and the last one forms
+// the system's beta, so that no kernel runs for it.  Same-address device-scope atomics serialise
+// (about 10 ns each on MI355X: 1444 arrivals on one counter cost 15 us), so a system with more
+// than 64 workgroups counts in two levels: workgroup w adds to first-level counter w % 64, the last
+// arrival of each adds to the system's counter.  Counters lie 64 ints (256 bytes) apart.  Each
+// chunk partial is written with a device-scope store (it writes through the XCD's L2) and the
+// wave waits for those stores before its first atomic, so the last workgroup, reading with
+// device-scope loads, sees every partial; no device-scope fence is needed (one writes back the
+// whole L2 of the XCD, which costs more than the kernel it would save).  The last arrival of each
+// counter clears it for the next iteration.  Valid in thread 0.
+__device__ inline bool arrive_last(int *cnt, int nw, int w)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (nw <= 64)
+    {
+        if (__hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != nw - 1) return false;
+        __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    const int g = w & 63, size = nw / 64 + (g < nw % 64 ? 1 : 0);
+    int *c1 = cnt + 64 * (1 + g);
+    if (__hip_atomic_fetch_add(c1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != size - 1) return false;
+    __hip_atomic_store(c1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 63) return false;
+    __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
 // y += alpha p', r -= alpha q over one chunk of one system, and the chunk's partials of r.r and
-// r.D r (D = diag(A): rho^2 r.D r is the squared norm of the unscaled residual r stands for)
+// r.D r (D = diag(A): rho^2 r.D r is the squared norm of the unscaled residual r stands for).
 // With FUSE (small systems) every workgroup forms alpha = r.r / p'.q itself from the system's tile
-// partials, in one fixed order (the same bits in every workgroup), and k_cg_alpha is not launched:
-// the system's first chunk records alpha, or a breakdown (p'.q <= 0) in S.broke, which no update
-// workgroup reads (k_cg_beta pauses the system).
+// partials, in one fixed order (the same bits in every workgroup), and k_cg_alpha is not launched;
+// the system's first chunk records alpha.  The system's last workgroup (arrive_last: counters at
+// acnt + abase[sys], anw[sys] workgroups) forms beta from the chunk partials (BT lanes, the order
+// of the other per-system sums), or pauses the system after a breakdown (p'.q <= 0, seen alike
+// by every workgroup).  A workgroup holds the chunks of one system.
 template <typename T, bool FUSE>
 __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ clo, const int *__restrict__ cpid, T *y, T *r,
                                                   const T *P0, const T *P1, const T *__restrict__ q,
                                                   const float *__restrict__ dg, int odd, double *cparts,
                                                   const int *__restrict__ seg_tiles, const int *__restrict__ cptr,
-                                                  const double *tparts, CGSys *st, int nch, int per)
+                                                  const double *tparts, CGSys *st, int nch, int per, int *acnt,
+                                                  const int *__restrict__ abase, const int *__restrict__ anw)
 {
     __shared__ double red[2 * (BT / 64)];
+    __shared__ int last;
     const T *pnew = odd ? P0 : P1;
     // chunks [c0, c1) of this workgroup (one, or with FUSE for one large system a short run, so that
     // fewer workgroups read the tile partials)
     const int c0 = blockIdx.x * per, c1 = min(c0 + per, nch);
-    int alpha_sys = -1;
+    int alpha_sys = -1, run_sys = -1;
+    bool broken = false;
     double alpha = 0.0;
     for (int c = c0; c < c1; ++c)
     {
         const int sys = cpid[c];
         CGSys &S = st[sys];
         if (!S.active) continue;
+        run_sys = sys;
         if constexpr (FUSE)
         {
             if (sys != alpha_sys)
@@ -252,8 +313,8 @@ __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ cl
                 alpha_sys = sys;
                 if (!(o[0] > 0.0))
                 {
-                    if (c == cptr[sys] && threadIdx.x == 0) S.broke = 1;
-                    return;
+                    broken = true;
+                    break;
                 }
                 alpha = S.rr / o[0];
             }
@@ -315,51 +376,32 @@ __global__ __launch_bounds__(BT) void k_cg_update(const int64_t *__restrict__ cl
                 a0 += red[2 * w];
                 a1 += red[2 * w + 1];
             }
-            cparts[2 * c] = a0;
-            cparts[2 * c + 1] = a1;
+            __hip_atomic_store(cparts + 2 * c, a0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(cparts + 2 * c + 1, a1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();                      // red is reused by the next chunk
     }
-}
-
-// one workgroup per system: the new r.r and r.D r over its chunks, the pause tests, beta
-__global__ __launch_bounds__(RT) void k_cg_beta(const int *__restrict__ cptr, const double *__restrict__ cparts, int odd,
-                                                CGSys *st)
-{
-    CGSys &S = st[blockIdx.x];
-    if (!S.active) return;
-    if (S.broke)
+    // the system's last workgroup: beta, or the pause after a breakdown
+    if (run_sys < 0) return;                  // the system is not running: nothing to count
+    const int sys = run_sys;
+    if (threadIdx.x == 0)
+        last = arrive_last(acnt + abase[sys], anw[sys], per > 1 ? static_cast<int>(blockIdx.x) : c0 - cptr[sys]);
+    __syncthreads();
+    if (!last) return;
+    CGSys &S = st[sys];
+    if (broken)
     {
         if (threadIdx.x == 0)
         {
             S.active = 0;
             S.reason = 2;
-            S.broke = 0;
         }
         return;
     }
-    __shared__ double red[2 * (RT / 64)];
-    const int c0 = cptr[blockIdx.x], c1 = cptr[blockIdx.x + 1];
+    const int k0 = cptr[sys], k1 = cptr[sys + 1];
     double o[2];
-    wg_fixed_sums<RT, 2>(cparts + 2 * c0, c1 - c0, red, o);
-    const double rr = o[0], rt = o[1];
-    if (threadIdx.x != 0) return;
-    S.it += 1;
-    S.total += 1;
-    S.dir = odd ^ 1;                        // the new direction's buffer
-    int reason = 0;
-    if (!(rr == rr) || !(rt == rt)) reason = 2;
-    else if (rr <= S.thr || rt <= S.tgt) reason = 1;
-    else if (S.it >= S.max_iter) reason = 3;
-    if (reason)
-    {
-        S.active = 0;
-        S.reason = reason;
-        S.rr = rr;
-        return;
-    }
-    S.beta = rr / S.rr;
-    S.rr = rr;
+    wg_fixed_sums<BT, 2, 8, true>(cparts + 2 * k0, k1 - k0, red, o);
+    if (threadIdx.x == 0) beta_verdict(S, o[0], o[1], odd);
 }
 
 // ---- maintenance kernels (on the systems with sel[sys] != 0) ---------------------------------
@@ -532,7 +574,6 @@ __global__ __launch_bounds__(RT) void k_cg_fresh_state(const int *__restrict__ c
     S.thr = tol2 * rr;
     S.rho = rn;
     S.reason = 0;
-    S.broke = 0;
     S.active = (rn > 0.0 && rr > 0.0) ? 1 : 0;
 }
 
@@ -784,6 +825,23 @@ void CGEngine::bind(femd_csr_s *M, int prec, hipStream_t s)
             update_per = per;
         }
     }
+    // arrival counters of the update workgroups (k_cg_update, arrive_last): per system one counter,
+    // or one and 64 first-level ones when more than 64 workgroups hold its chunks; 64 ints apart
+    {
+        std::vector<int> base(size_t(std::max(np, 1)), 0), nwg(size_t(std::max(np, 1)), 0);
+        int64_t at = 0;
+        for (int p = 0; p < np; ++p)
+        {
+            const int nw = update_per > 1 ? (nch + update_per - 1) / update_per : cptr_h[size_t(p) + 1] - cptr_h[size_t(p)];
+            nwg[size_t(p)] = nw;
+            base[size_t(p)] = static_cast<int>(at);
+            at += 64 * (nw > 64 ? 65 : 1);
+        }
+        acnt = DevBuf(4 * size_t(std::max<int64_t>(at, 64)));
+        HIPC(hipMemsetAsync(acnt.p, 0, 4 * size_t(std::max<int64_t>(at, 64)), s));
+        abase = upload(base, s);
+        anw = upload(nwg, s);
+    }
     const size_t N = size_t(std::max<int64_t>(n, 1)), w = prec == FEMD_FP64 ? 8 : 4;
     st = DevBuf(sizeof(CGSys) * size_t(std::max(np, 1)));
     tparts = DevBuf(8 * size_t(std::max(plan->ntiles, 1)));
@@ -878,13 +936,15 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
         if (E.use_dict && E.dict_raw) launch_product<T, 2>(E, odd, s);
         else if (E.use_dict) launch_product<T, 1>(E, odd, s);
         else launch_product<T, 0>(E, odd, s);
+        int *ac = E.acnt.as<int>();
+        const int *ab = E.abase.as<int>(), *an = E.anw.as<int>();
         if (E.fuse_alpha)
         {
             const int per = E.update_per;
             hipLaunchKernelGGL((k_cg_update<T, true>), dim3((E.nch + per - 1) / per), dim3(BT), 0, s, E.clo.as<int64_t>(),
                                E.cpid.as<int>(), E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(),
                                E.A->diagf.as<float>(), odd, E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(),
-                               E.tparts.as<double>(), st, E.nch, per);
+                               E.tparts.as<double>(), st, E.nch, per, ac, ab, an);
         }
         else
         {
@@ -892,9 +952,8 @@ void iterate_t(CGEngine &E, int count, hipStream_t s)
             hipLaunchKernelGGL((k_cg_update<T, false>), dim3(E.nch), dim3(BT), 0, s, E.clo.as<int64_t>(), E.cpid.as<int>(),
                                E.y.as<T>(), E.r.as<T>(), E.P0.as<T>(), E.P1.as<T>(), E.q.as<T>(), E.A->diagf.as<float>(), odd,
                                E.cparts.as<double>(), E.seg_tiles.as<int>(), E.cptr.as<int>(), E.tparts.as<double>(), st,
-                               E.nch, 1);
+                               E.nch, 1, ac, ab, an);
         }
-        hipLaunchKernelGGL(k_cg_beta, dim3(E.np), dim3(RT), 0, s, E.cptr.as<int>(), E.cparts.as<double>(), odd, st);
         E.parity ^= 1;
     }
 }

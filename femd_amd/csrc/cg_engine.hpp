@@ -5,15 +5,17 @@
 // for many independent systems at once (the systems of a block-diagonal matrix whose pattern has
 // one segment per system; every system keeps its own scalars and state).
 //
-// Iteration of every running system (four kernels, no atomics, reductions in a fixed order):
+// Iteration of every running system (two or three kernels, reductions in a fixed order):
 //   product  p' = r + beta p at the gathered columns and the own rows (into the other direction
 //            buffer), q = (S A S) p', partials of p'.q per tile
-//   alpha    per system: p'.q from its tiles' partials, alpha = r.r / p'.q
-//   update   y += alpha p', r -= alpha q, partials of r.r and of r.D r per chunk (D = diag(A))
-//   beta     per system: the new r.r; pause when it has dropped by inner_tol^2 since the segment
-//            began, or when rho^2 r.D r -- the squared norm of the unscaled residual b - A x the
-//            recurrence stands for -- has reached the solve's target (tol ||b||)^2, or at the
-//            iteration cap, or on breakdown; beta = r.r_new / r.r
+//   alpha    per system: p'.q from its tiles' partials, alpha = r.r / p'.q (for small systems,
+//            and one large one, every update workgroup forms it instead)
+//   update   y += alpha p', r -= alpha q, partials of r.r and of r.D r per chunk (D = diag(A));
+//            the last of a system's workgroups to finish (counted in by device-scope atomics)
+//            sums them: pause when r.r has dropped by inner_tol^2 since the segment began, or
+//            when rho^2 r.D r -- the squared norm of the unscaled residual b - A x the recurrence
+//            stands for -- has reached the solve's target (tol ||b||)^2, or at the iteration cap,
+//            or on breakdown; beta = r.r_new / r.r
 // A paused system's kernels return at once, so a fixed launch sequence (and a captured graph of
 // it) serves any mix of running and finished systems.
 //
@@ -43,8 +45,7 @@ struct CGSys {
     double rho;          // the FP64 residual norm the system is scaled by
     int it, total, max_iter, active, reason;   // it: since the last restart; total: all
     int dir;             // the direction buffer (P0 / P1) holding the current direction
-    int broke;           // p'.q <= 0 seen by the update (alpha folded into it): beta pauses the system
-    int pad;
+    int pad[2];
 };
 
 // S = diag(A)^(-1/2) (FP64) and S A S (FP32 / FP64) on the matrix, in the pattern's LocalOrder when
@@ -79,6 +80,7 @@ struct CGEngine {
     int update_per = 1;                 // chunks per update workgroup (> 1: one large system, alpha fused)
     int parity = 0;                     // the next iteration reads its direction from P1 (1) or P0 (0)
     DevBuf list, aoff;                  // the product's tile list (two int4 per tile) and its offsets
+    DevBuf acnt, abase, anw;            // the update's arrival counters, each system's first and its workgroups
     int list_count = -1;
     int dict_gen = 0;
     hipGraph_t graph = nullptr;

@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--strict-rtol", type=float, default=1e-10,
                     help="two extra steps at this rtol, reported beside the headline (0: none)")
-    ap.add_argument("--inner-tol", type=float, default=0.3)
+    ap.add_argument("--inner-tol", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full-limit", type=float, default=150.0,
                     help="run a CPU reference solve in full when a short sample predicts it within this many "

@@ -123,3 +123,27 @@ def test_large_system_alone_and_in_a_mix():
     assert np.array_equal(alone[0], mix[0])
     x, info = fd.solve_refined(A, b, tol=1e-9, inner_tol=0.3)
     assert info.converged and np.array_equal(x, alone[0])
+
+
+@pytest.mark.parametrize("blocks", [5, 35000])
+def test_inner_cg_breakdown_ends_the_solve(blocks):
+    """An indefinite matrix with a positive diagonal (2x2 blocks [[1, 2], [2, 1]]) and b = (1, -1)
+    per block: b.Ab = -2 per block, so the first inner iteration breaks down (p.q <= 0).  Every
+    update workgroup of the system sees it; the last to finish pauses the system (one arrival
+    counter for 5 blocks, two levels for 35000: 69 update workgroups), the host restarts it, and
+    the solve ends unconverged after max_outer rounds instead of looping or returning NaN."""
+    print(f"\n[case] inner CG breakdown, {blocks} blocks of [[1, 2], [2, 1]]", flush=True)
+    n = 2 * blocks
+    ip = np.arange(0, 2 * n + 1, 2, dtype=np.int32)
+    ix = np.empty(2 * n, dtype=np.int32)
+    ix[0::4], ix[1::4] = np.arange(0, n, 2), np.arange(1, n, 2)
+    ix[2::4], ix[3::4] = np.arange(0, n, 2), np.arange(1, n, 2)
+    v = np.tile([1.0, 2.0, 2.0, 1.0], blocks)
+    A = fd.SparseMatrix.from_arrays(n, n, ip, ix, v)
+    b = np.tile([1.0, -1.0], blocks)
+    x, info = fd.solve_refined(A, b, tol=1e-8, inner_tol=0.3, max_outer=4)
+    assert not info.converged
+    assert np.all(np.isfinite(x))
+    # the same system again: the arrival counters were left cleared
+    x2, info2 = fd.solve_refined(A, b, tol=1e-8, inner_tol=0.3, max_outer=4)
+    assert not info2.converged and np.array_equal(x, x2) and info2.iterations == info.iterations
